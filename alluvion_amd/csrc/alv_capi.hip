@@ -1,0 +1,1265 @@
+// alv_capi.hip -- implementation of the C ABI (include/alv/alv.h): device
+// memory, boundary (Pile) management, volume-map construction and the
+// per-step orchestration of the DFSPH / IISPH solvers.
+//
+// Orchestration differs from the reference's Runner (runner.hpp:4817-4902),
+// which synchronises after every launch: a step here is a stream of
+// asynchronous launches on one HIP stream.  The Jacobi loops are enqueued in
+// chunks; iteration kernels check a device flag and return at once after
+// convergence, so the host waits once per chunk rather than once per
+// iteration, and once at the end of the step to read the report.
+
+#include <float.h>
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/alv/alv.h"
+#include "alv_device.hpp"
+#include "alv_kernels.hpp"
+
+using namespace alv;
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define ALV_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(ALV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  ALV_HIP(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
+  ALV_HIP(hipMemset(*p, 0, count * sizeof(T)));
+  return ALV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Peirce product rule (M = 14): cos(phi), sin(phi), b from the 30-point
+// Gauss-Legendre rule; r = sqrt((x+1)/2), c = w/(4 sqrt 2) from the 15-point
+// Gauss-Jacobi(0, 1/2) rule.  Nodes and weights come from the Golub-Welsch
+// eigenproblem of the Jacobi matrix (implicit-shift QL, long double).
+// ---------------------------------------------------------------------------
+void gauss_rule(int n, const std::vector<long double>& a, const std::vector<long double>& b,
+                long double mu0, std::vector<long double>& x, std::vector<long double>& w) {
+  std::vector<long double> d(a), e(n, 0.0L), z(n, 0.0L);
+  for (int i = 0; i + 1 < n; ++i) e[i] = b[i];
+  z[0] = 1.0L;
+  for (int l = 0; l < n; ++l) {
+    for (int guard = 0; guard < 100; ++guard) {
+      int m = l;
+      for (; m < n - 1; ++m)
+        if (fabsl(e[m]) <= 1e-30L * (fabsl(d[m]) + fabsl(d[m + 1])) + 1e-300L) break;
+      if (m == l) break;
+      long double g = (d[l + 1] - d[l]) / (2.0L * e[l]);
+      long double r = hypotl(g, 1.0L);
+      g = d[m] - d[l] + e[l] / (g + copysignl(r, g));
+      long double s = 1.0L, c = 1.0L, p = 0.0L;
+      bool deflated = false;
+      for (int i = m - 1; i >= l; --i) {
+        long double f = s * e[i], bb = c * e[i];
+        r = hypotl(f, g);
+        e[i + 1] = r;
+        if (r == 0.0L) {
+          d[i + 1] -= p;
+          e[m] = 0.0L;
+          deflated = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * s + 2.0L * c * bb;
+        p = s * r;
+        d[i + 1] = g + p;
+        g = c * r - bb;
+        long double t = z[i + 1];
+        z[i + 1] = s * z[i] + c * t;
+        z[i] = c * z[i] - s * t;
+      }
+      if (deflated) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0L;
+    }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int i, int j) { return d[i] < d[j]; });
+  x.resize(n);
+  w.resize(n);
+  for (int i = 0; i < n; ++i) {
+    x[i] = d[idx[i]];
+    w[i] = mu0 * z[idx[i]] * z[idx[i]];
+  }
+}
+
+void peirce_table(float out[120]) {
+  std::vector<long double> x, w;
+  {
+    const int n = 30;
+    std::vector<long double> a(n, 0.0L), b(n - 1);
+    for (int k = 1; k < n; ++k) b[k - 1] = sqrtl((long double)k * k / (4.0L * k * k - 1.0L));
+    gauss_rule(n, a, b, 2.0L, x, w);
+    for (int j = 0; j < n; ++j) {
+      out[j] = float(x[j]);
+      out[30 + j] = float(sqrtl(1.0L - x[j] * x[j]));
+      out[60 + j] = float(w[j]);
+    }
+  }
+  {
+    const int n = 15;
+    const long double be = 0.5L;
+    std::vector<long double> a(n), b(n - 1);
+    for (int k = 0; k < n; ++k) {
+      long double s = 2.0L * k + be;
+      a[k] = (be * be) / (s * (s + 2.0L));
+    }
+    for (int k = 1; k < n; ++k) {
+      long double s = 2.0L * k + be;
+      b[k - 1] = sqrtl(4.0L * k * k * (k + be) * (k + be) / (s * s * (s + 1.0L) * (s - 1.0L)));
+    }
+    gauss_rule(n, a, b, powl(2.0L, 2.5L) / 3.0L, x, w);
+    for (int k = 0; k < n; ++k) {
+      out[90 + k] = float(sqrtl((x[k] + 1.0L) / 2.0L));
+      out[105 + k] = float(w[k] / (4.0L * sqrtl(2.0L)));
+    }
+  }
+}
+
+SimParams make_params(const alv_const& cn, const alv_consti& cni, unsigned num_boundaries,
+                      unsigned n) {
+  SimParams p;
+  memset(&p, 0, sizeof(p));
+  p.h = cn.kernel_radius;
+  p.h2 = cn.kernel_radius_sqr;
+  p.k = cn.cubic_kernel_k;
+  p.l = cn.cubic_kernel_l;
+  p.w0 = cn.cubic_kernel_zero;
+  p.radius = cn.particle_radius;
+  p.vol = cn.particle_vol;
+  p.mass = cn.particle_mass;
+  p.rho0 = cn.density0;
+  p.visc = cn.viscosity;
+  p.bvisc = cn.boundary_viscosity;
+  p.eps = cn.dfsph_factor_epsilon;
+  for (int a = 0; a < 3; ++a) {
+    p.g[a] = cn.gravity[a];
+    float L = cn.wrap_length[a];
+    p.wrap_len[a] = L;
+    p.wrap_min[a] = L * -0.5f;
+    p.wrap_max[a] = L * 0.5f;
+    if (L > 0.0f) p.wrap_mask |= 1 << a;
+    p.res[a] = int(cni.grid_res[a]);
+    p.off[a] = cni.grid_offset[a];
+  }
+  p.kmax = cni.max_num_neighbors_per_particle;
+  p.cell_cap = cni.max_num_particles_per_cell;
+  p.num_cells = cni.grid_res[0] * cni.grid_res[1] * cni.grid_res[2];
+  p.num_boundaries = num_boundaries;
+  p.n = n;
+  return p;
+}
+
+// Per-kernel timing: HIP events recorded around every launch on the launch
+// stream; elapsed times are collected after the step's final synchronisation.
+class EventProfiler : public KernelProfiler {
+ public:
+  struct Stat {
+    std::string name;
+    double total_ms = 0.0;
+    unsigned long long launches = 0;
+  };
+  ~EventProfiler() override {
+    for (auto& p : pending_) {
+      (void)hipEventDestroy(p.b);
+      (void)hipEventDestroy(p.e);
+    }
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+  }
+  void begin(const char* name, hipStream_t s) override {
+    Pending p;
+    p.name = name;
+    p.b = get();
+    p.e = get();
+    (void)hipEventRecord(p.b, s);
+    pending_.push_back(p);
+  }
+  void end(const char*, hipStream_t s) override {
+    if (!pending_.empty()) (void)hipEventRecord(pending_.back().e, s);
+  }
+  void collect() {
+    for (auto& p : pending_) {
+      float ms = 0.f;
+      if (hipEventSynchronize(p.e) == hipSuccess && hipEventElapsedTime(&ms, p.b, p.e) == hipSuccess) {
+        Stat& st = find(p.name);
+        st.total_ms += ms;
+        st.launches += 1;
+      }
+      pool_.push_back(p.b);
+      pool_.push_back(p.e);
+    }
+    pending_.clear();
+  }
+  void reset() {
+    collect();
+    stats_.clear();
+  }
+  std::vector<Stat>& stats() { return stats_; }
+
+ private:
+  struct Pending {
+    const char* name;
+    hipEvent_t b, e;
+  };
+  hipEvent_t get() {
+    if (!pool_.empty()) {
+      hipEvent_t e = pool_.back();
+      pool_.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  Stat& find(const char* name) {
+    for (auto& s : stats_)
+      if (s.name == name) return s;
+    stats_.push_back(Stat());
+    stats_.back().name = name;
+    return stats_.back();
+  }
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> pool_;
+  std::vector<Stat> stats_;
+};
+EventProfiler* g_event_profiler = nullptr;
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct alv_pile {
+  std::vector<BoundaryDev> bnd;  // kinematics live in bnd[b].x/v/q/omega
+  std::vector<float*> volumes;
+  std::vector<float> restitution, friction;
+  std::vector<std::array<float, 3>> inertia, force, torque, accel;
+  float gravity[3] = {0.f, 0.f, 0.f};
+  unsigned max_num_contacts = 0;
+  float* peirce = nullptr;  // device, 120 floats
+  ~alv_pile() {
+    for (float* v : volumes) (void)hipFree(v);
+    if (peirce) (void)hipFree(peirce);
+  }
+};
+
+struct alv_solver {
+  int kind = ALV_SOLVER_DF;
+  alv_pile* pile = nullptr;
+  unsigned cap = 0;
+  alv_const cn;
+  alv_consti cni;
+  alv_solver_settings set;
+  alv_solver_report rep;
+  DevArrays A;
+  hipStream_t stream = nullptr;
+  StepCtl* ctl_host = nullptr;  // pinned
+  BoundaryDev* bnd_host = nullptr;  // pinned upload buffer
+  BoundaryDev* bnd_dev = nullptr;
+  double* body_host = nullptr;  // pinned [B*6]
+  unsigned bnd_capacity = 0;
+  unsigned num_cells = 0;
+  unsigned dirty = 0;
+  unsigned last_div = 2, last_dens = 3;
+  unsigned* export_ids = nullptr;
+  unsigned* export_counts = nullptr;
+  std::vector<void*> allocations;
+
+  ~alv_solver() {
+    if (stream) hipStreamSynchronize(stream);
+    for (void* p : allocations) hipFree(p);
+    if (export_ids) hipFree(export_ids);
+    if (export_counts) hipFree(export_counts);
+    if (bnd_dev) hipFree(bnd_dev);
+    if (ctl_host) hipHostFree(ctl_host);
+    if (bnd_host) hipHostFree(bnd_host);
+    if (body_host) hipHostFree(body_host);
+    if (stream) hipStreamDestroy(stream);
+  }
+  template <class T>
+  int alloc(T** p, size_t count) {
+    int rc = dev_alloc(p, count);
+    if (rc == ALV_OK) allocations.push_back(*p);
+    return rc;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// device and memory
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* alv_last_error(void) { return g_last_error.c_str(); }
+const char* alv_version(void) { return "alluvion-amd 0.1 (gfx950)"; }
+
+int alv_set_device(int device) {
+  ALV_HIP(hipSetDevice(device));
+  return ALV_OK;
+}
+int alv_get_device_count(int* count) {
+  if (!count) return fail(ALV_ERR_ARG, "count is null");
+  ALV_HIP(hipGetDeviceCount(count));
+  return ALV_OK;
+}
+int alv_device_synchronize(void) {
+  ALV_HIP(hipDeviceSynchronize());
+  return ALV_OK;
+}
+int alv_malloc(void** dptr, size_t bytes) {
+  if (!dptr) return fail(ALV_ERR_ARG, "dptr is null");
+  *dptr = nullptr;
+  if (bytes == 0) return ALV_OK;
+  ALV_HIP(hipMalloc(dptr, bytes));
+  ALV_HIP(hipMemset(*dptr, 0, bytes));
+  return ALV_OK;
+}
+int alv_free(void* dptr) {
+  if (dptr) ALV_HIP(hipFree(dptr));
+  return ALV_OK;
+}
+int alv_malloc_pinned(void** hptr, size_t bytes) {
+  if (!hptr) return fail(ALV_ERR_ARG, "hptr is null");
+  *hptr = nullptr;
+  if (bytes == 0) return ALV_OK;
+  ALV_HIP(hipHostMalloc(hptr, bytes, hipHostMallocDefault));
+  memset(*hptr, 0, bytes);
+  return ALV_OK;
+}
+int alv_free_pinned(void* hptr) {
+  if (hptr) ALV_HIP(hipHostFree(hptr));
+  return ALV_OK;
+}
+int alv_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  if (bytes) ALV_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return ALV_OK;
+}
+int alv_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  if (bytes) ALV_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return ALV_OK;
+}
+int alv_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  if (bytes) ALV_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return ALV_OK;
+}
+int alv_memset(void* dptr, int value, size_t bytes) {
+  if (bytes) ALV_HIP(hipMemset(dptr, value, bytes));
+  return ALV_OK;
+}
+
+}  // extern "C"
+
+// reductions for Runner::sum / Runner::max
+__global__ __launch_bounds__(256) void k_sum_f32(const float* d, size_t n, double* part) {
+  __shared__ double lds[4];
+  double acc = 0.0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    acc += double(d[i]);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63u) == 0) lds[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+__global__ __launch_bounds__(256) void k_max_f32(const float* d, size_t n, float* part) {
+  __shared__ float lds[4];
+  float acc = -FLT_MAX;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    acc = fmaxf(acc, d[i]);
+  for (int o = 32; o > 0; o >>= 1) acc = fmaxf(acc, __shfl_down(acc, o, 64));
+  if ((threadIdx.x & 63u) == 0) lds[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
+}
+
+extern "C" {
+
+int alv_sum_f32(const float* d, size_t n, double* out) {
+  if (!out) return fail(ALV_ERR_ARG, "out is null");
+  const unsigned nb = 1024;
+  double* part = nullptr;
+  ALV_HIP(hipMalloc(&part, nb * sizeof(double)));
+  hipLaunchKernelGGL(k_sum_f32, dim3(nb), dim3(256), 0, 0, d, n, part);
+  std::vector<double> h(nb);
+  hipError_t e = hipMemcpy(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(part);
+  if (e != hipSuccess) return fail(ALV_ERR_HIP, hipGetErrorString(e));
+  double s = 0.0;
+  for (double v : h) s += v;
+  *out = s;
+  return ALV_OK;
+}
+int alv_max_f32(const float* d, size_t n, float* out) {
+  if (!out) return fail(ALV_ERR_ARG, "out is null");
+  const unsigned nb = 1024;
+  float* part = nullptr;
+  ALV_HIP(hipMalloc(&part, nb * sizeof(float)));
+  hipLaunchKernelGGL(k_max_f32, dim3(nb), dim3(256), 0, 0, d, n, part);
+  std::vector<float> h(nb);
+  hipError_t e = hipMemcpy(h.data(), part, nb * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(part);
+  if (e != hipSuccess) return fail(ALV_ERR_HIP, hipGetErrorString(e));
+  float m = -FLT_MAX;
+  for (float v : h) m = std::max(m, v);
+  *out = m;
+  return ALV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// constants
+// ---------------------------------------------------------------------------
+void alv_const_set_kernel_radius(alv_const* cn, float h) {
+  const float pi = 3.14159265358979323846f;
+  float h2 = h * h;
+  float h3 = h2 * h;
+  float h4 = h2 * h2;
+  cn->kernel_radius = h;
+  cn->kernel_radius_sqr = h2;
+  cn->cubic_kernel_k = 8.0f / h3 / pi;
+  cn->cubic_kernel_l = 48.0f / h4 / pi;
+  cn->cubic_kernel_zero = cn->cubic_kernel_k;
+}
+void alv_const_set_particle_attr(alv_const* cn, float radius, float mass, float density) {
+  float vol = mass / density;
+  cn->particle_radius = radius;
+  cn->particle_mass = mass;
+  cn->density0 = density;
+  cn->particle_vol = vol;
+}
+
+// ---------------------------------------------------------------------------
+// Pile
+// ---------------------------------------------------------------------------
+int alv_pile_create(alv_pile** out, unsigned max_num_contacts, int volume_method) {
+  if (!out) return fail(ALV_ERR_ARG, "out is null");
+  if (volume_method != ALV_VOLUME_MAP)
+    return fail(ALV_ERR_UNSUPPORTED, "only VolumeMethod.volume_map is supported");
+  alv_pile* p = new (std::nothrow) alv_pile();
+  if (!p) return fail(ALV_ERR_STATE, "out of host memory");
+  p->max_num_contacts = max_num_contacts;
+  float table[120];
+  peirce_table(table);
+  hipError_t e = hipMalloc(&p->peirce, sizeof(table));
+  if (e == hipSuccess) e = hipMemcpy(p->peirce, table, sizeof(table), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    delete p;
+    return fail(ALV_ERR_HIP, hipGetErrorString(e));
+  }
+  *out = p;
+  return ALV_OK;
+}
+
+int alv_pile_destroy(alv_pile* pile) {
+  delete pile;
+  return ALV_OK;
+}
+
+int alv_pile_add(alv_pile* pile, const alv_const* cn, const alv_distance* dist,
+                 const unsigned resolution[3], float cell_width, float margin, float sign,
+                 float mass, float restitution, float friction, const float inertia[3],
+                 const float x[3], const float q[4], unsigned* boundary_id) {
+  if (!pile || !cn || !dist) return fail(ALV_ERR_ARG, "null argument");
+  if (pile->bnd.size() >= 32) return fail(ALV_ERR_UNSUPPORTED, "at most 32 boundaries");
+  BoundaryDev B;
+  memset(&B, 0, sizeof(B));
+  B.type = dist->type;
+  B.sign = sign;
+  float aabb_min[3], aabb_max[3];
+  const float* pr = dist->params;
+  if (dist->type == ALV_DISTANCE_BOX) {
+    // BoxDistance (dg/box_distance.hpp:11-17)
+    B.outset = pr[3];
+    float hx2 = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      B.half[a] = pr[a] * 0.5f;
+      aabb_min[a] = pr[a] * -0.5f - B.outset;
+      aabb_max[a] = B.half[a] + B.outset;
+    }
+    hx2 = B.half[0] * B.half[0] + B.half[1] * B.half[1] + B.half[2] * B.half[2];
+    B.max_distance = sqrtf(hx2) + B.outset;
+  } else if (dist->type == ALV_DISTANCE_BOX_SHELL) {
+    // BoxShellDistance (dg/box_shell_distance.hpp:11-24)
+    float th = pr[3];
+    B.outset = pr[4];
+    for (int a = 0; a < 3; ++a) {
+      B.half[a] = pr[a] * 0.5f;
+      B.half_outer[a] = B.half[a] + th;
+      aabb_min[a] = pr[a] * -0.5f - th - B.outset;
+      aabb_max[a] = B.half[a] + th + B.outset;
+    }
+    float o2 = B.half_outer[0] * B.half_outer[0] + B.half_outer[1] * B.half_outer[1] +
+               B.half_outer[2] * B.half_outer[2];
+    B.max_distance = sqrtf(o2) + B.outset;
+  } else if (dist->type == ALV_DISTANCE_SPHERE) {
+    // SphereDistance (dg/sphere_distance.hpp:10-16)
+    B.radius = pr[0];
+    for (int a = 0; a < 3; ++a) {
+      aabb_min[a] = -B.radius;
+      aabb_max[a] = B.radius;
+    }
+    B.max_distance = B.radius;
+  } else {
+    return fail(ALV_ERR_UNSUPPORTED, "distance type not supported");
+  }
+  // calculate_domain_and_resolution (pile.hpp:40-65)
+  if (std::isinf(margin)) margin = sign < 0 ? cn->kernel_radius : cn->kernel_radius * 2;
+  unsigned res[3] = {resolution[0], resolution[1], resolution[2]};
+  for (int a = 0; a < 3; ++a) {
+    B.geom.dmin[a] = aabb_min[a] - margin;
+    B.geom.dmax[a] = aabb_max[a] + margin;
+  }
+  if (res[0] == 0 || res[1] == 0 || res[2] == 0) {
+    if (cell_width == 0) cell_width = cn->particle_radius;
+    for (int a = 0; a < 3; ++a)
+      if (res[a] == 0) res[a] = unsigned((B.geom.dmax[a] - B.geom.dmin[a]) / cell_width);
+  }
+  // calculate_grid_attributes (pile.hpp:66-78)
+  for (int a = 0; a < 3; ++a) {
+    if (res[a] == 0) return fail(ALV_ERR_ARG, "zero grid resolution");
+    B.geom.res[a] = res[a];
+    B.geom.cs[a] = (B.geom.dmax[a] - B.geom.dmin[a]) / float(res[a]);
+  }
+  size_t nv = size_t(res[0] + 1) * (res[1] + 1) * (res[2] + 1);
+  size_t ne = size_t(res[0]) * (res[1] + 1) * (res[2] + 1) +
+              size_t(res[0] + 1) * res[1] * (res[2] + 1) +
+              size_t(res[0] + 1) * (res[1] + 1) * res[2];
+  size_t nodes = nv + 2 * ne;
+  if (nodes >= (1ull << 32)) return fail(ALV_ERR_UNSUPPORTED, "volume map too large");
+  B.geom.num_nodes = unsigned(nodes);
+  float* vol = nullptr;
+  int rc = dev_alloc(&vol, nodes);
+  if (rc != ALV_OK) return rc;
+  B.volume = vol;
+  for (int a = 0; a < 3; ++a) {
+    B.x[a] = x ? x[a] : 0.f;
+    B.v[a] = 0.f;
+    B.omega[a] = 0.f;
+  }
+  for (int a = 0; a < 4; ++a) B.q[a] = q ? q[a] : (a == 3 ? 1.f : 0.f);
+  B.mass = mass;
+  B.dynamic = mass != 0.f ? 1 : 0;
+  // build the volume map (update_volume_field, analytic variant)
+  alv_consti cni;
+  memset(&cni, 0, sizeof(cni));
+  SimParams p = make_params(*cn, cni, 0, 0);
+  const unsigned chunk = 1u << 20;
+  for (unsigned b = 0; b < B.geom.num_nodes; b += chunk) {
+    unsigned e = std::min(B.geom.num_nodes, b + chunk);
+    launch_volume_field_analytic(p, B, vol, pile->peirce, b, e, 0);
+    ALV_HIP(hipGetLastError());
+  }
+  ALV_HIP(hipDeviceSynchronize());
+  pile->bnd.push_back(B);
+  pile->volumes.push_back(vol);
+  pile->restitution.push_back(restitution);
+  pile->friction.push_back(friction);
+  std::array<float, 3> I = {inertia ? inertia[0] : 1.f, inertia ? inertia[1] : 1.f,
+                            inertia ? inertia[2] : 1.f};
+  pile->inertia.push_back(I);
+  pile->force.push_back({0.f, 0.f, 0.f});
+  pile->torque.push_back({0.f, 0.f, 0.f});
+  pile->accel.push_back({0.f, 0.f, 0.f});
+  if (boundary_id) *boundary_id = unsigned(pile->bnd.size() - 1);
+  return ALV_OK;
+}
+
+int alv_pile_get_size(const alv_pile* pile, unsigned* size) {
+  if (!pile || !size) return fail(ALV_ERR_ARG, "null argument");
+  *size = unsigned(pile->bnd.size());
+  return ALV_OK;
+}
+
+int alv_pile_set_kinematics(alv_pile* pile, unsigned b, const float x[3], const float v[3],
+                            const float q[4], const float omega[3]) {
+  if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
+  BoundaryDev& B = pile->bnd[b];
+  for (int a = 0; a < 3; ++a) {
+    if (x) B.x[a] = x[a];
+    if (v) B.v[a] = v[a];
+    if (omega) B.omega[a] = omega[a];
+  }
+  if (q)
+    for (int a = 0; a < 4; ++a) B.q[a] = q[a];
+  return ALV_OK;
+}
+
+int alv_pile_get_kinematics(const alv_pile* pile, unsigned b, float x[3], float v[3],
+                            float q[4], float omega[3], float force[3], float torque[3]) {
+  if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
+  const BoundaryDev& B = pile->bnd[b];
+  for (int a = 0; a < 3; ++a) {
+    if (x) x[a] = B.x[a];
+    if (v) v[a] = B.v[a];
+    if (omega) omega[a] = B.omega[a];
+    if (force) force[a] = pile->force[b][a];
+    if (torque) torque[a] = pile->torque[b][a];
+  }
+  if (q)
+    for (int a = 0; a < 4; ++a) q[a] = B.q[a];
+  return ALV_OK;
+}
+
+int alv_pile_set_gravity(alv_pile* pile, const float g[3]) {
+  if (!pile || !g) return fail(ALV_ERR_ARG, "null argument");
+  for (int a = 0; a < 3; ++a) pile->gravity[a] = g[a];
+  return ALV_OK;
+}
+
+// Pile::integrate_kinematics (pile.hpp:794-815) with calculate_dq and
+// calculate_angular_acceleration (runner.hpp:202-252).
+int alv_pile_integrate_kinematics(alv_pile* pile, float dt) {
+  if (!pile) return fail(ALV_ERR_ARG, "null pile");
+  for (size_t b = 0; b < pile->bnd.size(); ++b) {
+    BoundaryDev& B = pile->bnd[b];
+    float4 q = make_float4(B.q[0], B.q[1], B.q[2], B.q[3]);
+    auto dq_of = [](vec3 w, float4 qq) {
+      return make_float4(0.5f * (w.x * qq.w + w.y * qq.z - w.z * qq.y),
+                         0.5f * (-w.x * qq.z + w.y * qq.w + w.z * qq.x),
+                         0.5f * (w.x * qq.y - w.y * qq.x + w.z * qq.w),
+                         0.5f * (-w.x * qq.x - w.y * qq.y - w.z * qq.z));
+    };
+    auto qnorm = [](float4 qq) {
+      float s = 1.0f / sqrtf(qq.x * qq.x + qq.y * qq.y + qq.z * qq.z + qq.w * qq.w);
+      return make_float4(qq.x * s, qq.y * s, qq.z * s, qq.w * s);
+    };
+    vec3 xx = mk(B.x[0], B.x[1], B.x[2]);
+    vec3 vv = mk(B.v[0], B.v[1], B.v[2]);
+    vec3 ww = mk(B.omega[0], B.omega[1], B.omega[2]);
+    if (B.mass == 0.f) {
+      xx += vv * dt;
+      float4 dq = dq_of(ww, q);
+      q = make_float4(q.x + dt * dq.x, q.y + dt * dq.y, q.z + dt * dq.z, q.w + dt * dq.w);
+      q = qnorm(q);
+    } else {
+      vec3 F = mk(pile->force[b][0], pile->force[b][1], pile->force[b][2]);
+      vec3 T = mk(pile->torque[b][0], pile->torque[b][1], pile->torque[b][2]);
+      vv += ((1.0f / B.mass) * F) * dt;
+      vec3 local = rotate(T, conjugate(q));
+      const auto& I = pile->inertia[b];
+      vec3 scaled = mk(local.x / I[0], local.y / I[1], local.z / I[2]);
+      ww += rotate(scaled, q) * dt;
+      vec3 acc = mk(pile->gravity[0], pile->gravity[1], pile->gravity[2]);
+      pile->accel[b] = {acc.x, acc.y, acc.z};
+      float4 dq = dq_of(ww, q);
+      q = make_float4(q.x + dt * dq.x, q.y + dt * dq.y, q.z + dt * dq.z, q.w + dt * dq.w);
+      q = qnorm(q);
+      vec3 dx = (acc * dt + vv) * dt;
+      xx += dx;
+      vv = (1.0f / dt) * dx;
+    }
+    B.x[0] = xx.x;
+    B.x[1] = xx.y;
+    B.x[2] = xx.z;
+    B.v[0] = vv.x;
+    B.v[1] = vv.y;
+    B.v[2] = vv.z;
+    B.omega[0] = ww.x;
+    B.omega[1] = ww.y;
+    B.omega[2] = ww.z;
+    B.q[0] = q.x;
+    B.q[1] = q.y;
+    B.q[2] = q.z;
+    B.q[3] = q.w;
+  }
+  return ALV_OK;
+}
+
+int alv_pile_get_volume_grid(const alv_pile* pile, unsigned b, float** nodes,
+                             unsigned* num_nodes) {
+  if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
+  if (nodes) *nodes = pile->volumes[b];
+  if (num_nodes) *num_nodes = pile->bnd[b].geom.num_nodes;
+  return ALV_OK;
+}
+
+int alv_pile_get_grid_info(const alv_pile* pile, unsigned b, float dmin[3], float dmax[3],
+                           unsigned res[3], float cs[3], float* sign, float* mass) {
+  if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
+  const BoundaryDev& B = pile->bnd[b];
+  for (int a = 0; a < 3; ++a) {
+    if (dmin) dmin[a] = B.geom.dmin[a];
+    if (dmax) dmax[a] = B.geom.dmax[a];
+    if (res) res[a] = B.geom.res[a];
+    if (cs) cs[a] = B.geom.cs[a];
+  }
+  if (sign) *sign = B.sign;
+  if (mass) *mass = B.mass;
+  return ALV_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Solver
+// ---------------------------------------------------------------------------
+static int solver_alloc(alv_solver* S) {
+  const unsigned cap = S->cap;
+  const unsigned C = S->num_cells;
+  const unsigned K = S->cni.max_num_neighbors_per_particle;
+  DevArrays& A = S->A;
+  memset(&A, 0, sizeof(A));
+  A.cap = cap;
+  int rc = ALV_OK;
+#define ALV_ALLOC(ptr, cnt) \
+  if ((rc = S->alloc(&(ptr), (cnt))) != ALV_OK) return rc;
+  ALV_ALLOC(A.ux, size_t(cap) * 3);
+  ALV_ALLOC(A.uv, size_t(cap) * 3);
+  ALV_ALLOC(A.ua, size_t(cap) * 3);
+  ALV_ALLOC(A.udensity, cap);
+  ALV_ALLOC(A.ualpha, cap);
+  ALV_ALLOC(A.ukappa, cap);
+  ALV_ALLOC(A.ukappa_v, cap);
+  ALV_ALLOC(A.urho_adv, cap);
+  ALV_ALLOC(A.unnb, cap);
+  ALV_ALLOC(A.upressure, cap);
+  ALV_ALLOC(A.key, cap);
+  ALV_ALLOC(A.slot, cap);
+  ALV_ALLOC(A.sorted_id, cap);
+  ALV_ALLOC(A.cell_count, size_t(C) + 1);
+  ALV_ALLOC(A.cell_start, size_t(C) + 2);
+  ALV_ALLOC(A.scan_tmp, (size_t(C) + 1 + 4095) / 4096 + 1);
+  ALV_ALLOC(A.anomaly, 4);
+  ALV_ALLOC(A.x, cap);
+  ALV_ALLOC(A.v, cap);
+  ALV_ALLOC(A.a, cap);
+  ALV_ALLOC(A.density, cap);
+  ALV_ALLOC(A.alpha, cap);
+  ALV_ALLOC(A.kappa, cap);
+  ALV_ALLOC(A.kappa_v, cap);
+  ALV_ALLOC(A.rho_adv, cap);
+  ALV_ALLOC(A.kfac, cap);
+  ALV_ALLOC(A.nbr, size_t(K) * cap);
+  ALV_ALLOC(A.nnb, cap);
+  ALV_ALLOC(A.bmask, cap);
+  const size_t BN = size_t(std::max<size_t>(1, S->pile->bnd.size())) * cap;
+  ALV_ALLOC(A.brec, BN);
+  ALV_ALLOC(A.bker, BN);
+  ALV_ALLOC(A.bforce, BN);
+  ALV_ALLOC(A.btorque, BN);
+  if (S->kind == ALV_SOLVER_II) {
+    ALV_ALLOC(A.pressure, cap);
+    ALV_ALLOC(A.last_pressure, cap);
+    ALV_ALLOC(A.aii, cap);
+    ALV_ALLOC(A.dii, cap);
+    ALV_ALLOC(A.dij_pj, cap);
+    ALV_ALLOC(A.adv_density, cap);
+    ALV_ALLOC(A.density_err, cap);
+    ALV_ALLOC(A.pacc, cap);
+  }
+  ALV_ALLOC(A.partials, grid_for(cap) + 1);
+  ALV_ALLOC(A.fpartials, grid_for(cap) + 1);
+  ALV_ALLOC(A.body_sums, std::max<size_t>(1, S->pile->bnd.size()) * 6);
+  ALV_ALLOC(A.ctl, 1);
+#undef ALV_ALLOC
+  S->bnd_capacity = unsigned(std::max<size_t>(1, S->pile->bnd.size()));
+  ALV_HIP(hipMalloc(&S->bnd_dev, S->bnd_capacity * sizeof(BoundaryDev)));
+  ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->bnd_host),
+                        S->bnd_capacity * sizeof(BoundaryDev), hipHostMallocDefault));
+  ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->ctl_host), sizeof(StepCtl),
+                        hipHostMallocDefault));
+  ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->body_host),
+                        S->bnd_capacity * 6 * sizeof(double), hipHostMallocDefault));
+  memset(S->ctl_host, 0, sizeof(StepCtl));
+  A.bnd = S->bnd_dev;
+  return ALV_OK;
+}
+
+extern "C" {
+
+int alv_solver_create(alv_solver** out, int kind, alv_pile* pile, unsigned max_num_particles,
+                      const alv_const* cn, const alv_consti* cni) {
+  if (!out || !pile || !cn || !cni) return fail(ALV_ERR_ARG, "null argument");
+  if (kind != ALV_SOLVER_DF && kind != ALV_SOLVER_II)
+    return fail(ALV_ERR_ARG, "unknown solver kind");
+  if (max_num_particles == 0) return fail(ALV_ERR_ARG, "max_num_particles is zero");
+  if (cni->max_num_neighbors_per_particle == 0)
+    return fail(ALV_ERR_ARG, "max_num_neighbors_per_particle is zero");
+  size_t C = size_t(cni->grid_res[0]) * cni->grid_res[1] * cni->grid_res[2];
+  if (C == 0 || C > (1ull << 31)) return fail(ALV_ERR_ARG, "grid_res out of range");
+  if ((C + 1 + 4095) / 4096 > 4096) return fail(ALV_ERR_UNSUPPORTED, "grid has too many cells");
+  alv_solver* S = new (std::nothrow) alv_solver();
+  if (!S) return fail(ALV_ERR_STATE, "out of host memory");
+  S->kind = kind;
+  S->pile = pile;
+  S->cap = max_num_particles;
+  S->cn = *cn;
+  S->cni = *cni;
+  S->num_cells = unsigned(C);
+  memset(&S->set, 0, sizeof(S->set));
+  S->set.num_particles = max_num_particles;
+  S->set.density_change_tolerance = 1e-3f;
+  S->set.density_error_tolerance = 1e-3f;
+  S->set.min_density_solve = 2;
+  S->set.max_density_solve = 100;
+  S->set.min_divergence_solve = 1;
+  S->set.max_divergence_solve = 100;
+  S->set.enable_divergence_solve = 1;
+  S->set.enable_density_solve = 1;
+  memset(&S->rep, 0, sizeof(S->rep));
+  hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete S;
+    return fail(ALV_ERR_HIP, hipGetErrorString(e));
+  }
+  int rc = solver_alloc(S);
+  if (rc != ALV_OK) {
+    delete S;
+    return rc;
+  }
+  *out = S;
+  return ALV_OK;
+}
+
+int alv_solver_destroy(alv_solver* solver) {
+  delete solver;
+  return ALV_OK;
+}
+
+int alv_solver_set_const(alv_solver* S, const alv_const* cn, const alv_consti* cni) {
+  if (!S || !cn) return fail(ALV_ERR_ARG, "null argument");
+  S->cn = *cn;
+  if (cni) {
+    if (cni->grid_res[0] != S->cni.grid_res[0] || cni->grid_res[1] != S->cni.grid_res[1] ||
+        cni->grid_res[2] != S->cni.grid_res[2] ||
+        cni->max_num_neighbors_per_particle != S->cni.max_num_neighbors_per_particle)
+      return fail(ALV_ERR_STATE, "grid_res and max_num_neighbors_per_particle are fixed at "
+                                 "solver construction");
+    S->cni = *cni;
+  }
+  return ALV_OK;
+}
+
+int alv_solver_set_settings(alv_solver* S, const alv_solver_settings* st) {
+  if (!S || !st) return fail(ALV_ERR_ARG, "null argument");
+  if (st->num_particles > S->cap)
+    return fail(ALV_ERR_ARG, "num_particles exceeds max_num_particles");
+  S->set = *st;
+  return ALV_OK;
+}
+
+int alv_solver_get_settings(const alv_solver* S, alv_solver_settings* st) {
+  if (!S || !st) return fail(ALV_ERR_ARG, "null argument");
+  *st = S->set;
+  return ALV_OK;
+}
+
+int alv_solver_get_report(const alv_solver* S, alv_solver_report* r) {
+  if (!S || !r) return fail(ALV_ERR_ARG, "null argument");
+  *r = S->rep;
+  r->t = S->set.t;
+  r->dt = S->set.dt;
+  return ALV_OK;
+}
+
+}  // extern "C"
+
+namespace {
+SimParams solver_params(const alv_solver* S) {
+  return make_params(S->cn, S->cni, unsigned(S->pile->bnd.size()), S->set.num_particles);
+}
+
+int upload_step_inputs(alv_solver* S) {
+  const size_t nb = S->pile->bnd.size();
+  if (nb > S->bnd_capacity)
+    return fail(ALV_ERR_STATE, "boundaries were added after the solver was created");
+  if (nb) {
+    memcpy(S->bnd_host, S->pile->bnd.data(), nb * sizeof(BoundaryDev));
+    ALV_HIP(hipMemcpyAsync(S->bnd_dev, S->bnd_host, nb * sizeof(BoundaryDev),
+                           hipMemcpyHostToDevice, S->stream));
+  }
+  StepCtl* c = S->ctl_host;
+  memset(c, 0, sizeof(StepCtl));
+  c->dt = S->set.dt;
+  c->cfl = S->set.cfl;
+  c->min_dt = S->set.min_dt;
+  c->max_dt = S->set.max_dt;
+  c->tol_div = S->set.density_change_tolerance;
+  c->tol_dens = S->set.density_error_tolerance;
+  c->min_div = S->set.min_divergence_solve;
+  c->max_div = S->set.max_divergence_solve;
+  c->min_dens = S->set.min_density_solve;
+  c->max_dens = S->set.max_density_solve;
+  c->length_scale = S->cn.particle_radius * 2.0f;
+  ALV_HIP(hipMemcpyAsync(S->A.ctl, c, sizeof(StepCtl), hipMemcpyHostToDevice, S->stream));
+  return ALV_OK;
+}
+
+int read_ctl(alv_solver* S) {
+  ALV_HIP(hipMemcpyAsync(S->ctl_host, S->A.ctl, sizeof(StepCtl), hipMemcpyDeviceToHost,
+                         S->stream));
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  return ALV_OK;
+}
+
+// grid + gather + boundary records + neighbour list + density (+ factor)
+int neighbor_phase(alv_solver* S, const SimParams& p, int with_factor) {
+  DevArrays& A = S->A;
+  ALV_HIP(hipMemsetAsync(A.cell_count, 0, (size_t(p.num_cells) + 1) * sizeof(unsigned),
+                         S->stream));
+  ALV_HIP(hipMemsetAsync(A.anomaly, 0, 3 * sizeof(unsigned), S->stream));
+  if (p.n == 0) return ALV_OK;
+  launch_cell_count(p, A, S->stream);
+  launch_scan(p.num_cells + 1, A.cell_count, A.cell_start, A.scan_tmp, S->stream);
+  launch_scatter(p, A, S->stream);
+  launch_cell_sort(p, A, S->stream);
+  launch_gather_boundary(p, A, S->kind, S->stream);
+  launch_neighbors(p, A, p.wrap_mask != 0, with_factor, S->stream);
+  ALV_HIP(hipGetLastError());
+  return ALV_OK;
+}
+
+// Enqueues Jacobi iterations in chunks until the device flag says done.
+template <class Launch>
+int run_loop(alv_solver* S, bool divergence, unsigned max_iter, unsigned* last,
+             Launch launch) {
+  unsigned issued = 0;
+  unsigned chunk = std::max(1u, *last);
+  while (issued < max_iter) {
+    unsigned todo = std::min(chunk, max_iter - issued);
+    for (unsigned k = 0; k < todo; ++k) launch();
+    ALV_HIP(hipGetLastError());
+    issued += todo;
+    int rc = read_ctl(S);
+    if (rc != ALV_OK) return rc;
+    if (divergence ? S->ctl_host->div_done : S->ctl_host->dens_done) break;
+    chunk = std::max(1u, chunk / 2);
+  }
+  return ALV_OK;
+}
+
+int finish_step(alv_solver* S, const SimParams& p) {
+  int rc = read_ctl(S);
+  if (rc != ALV_OK) return rc;
+  if (g_event_profiler) g_event_profiler->collect();
+  const StepCtl& c = *S->ctl_host;
+  S->set.dt = c.dt;
+  S->rep.num_divergence_solve = c.div_iter;
+  S->rep.num_density_solve = c.dens_iter;
+  S->rep.mean_density_change = c.mean_density_change;
+  S->rep.mean_density_error = c.mean_density_error;
+  S->rep.particle_max_v2 = c.particle_max_v2;
+  S->rep.pile_max_v2 = c.pile_max_v2;
+  S->rep.max_v2 = c.max_v2;
+  S->rep.cfl_dt = c.cfl_dt;
+  S->rep.utilized_cfl = c.utilized_cfl;
+  if (c.div_iter) S->last_div = c.div_iter;
+  if (c.dens_iter) S->last_dens = c.dens_iter;
+  // rigid force sums (solver_df.hpp:347-353)
+  alv_pile* pile = S->pile;
+  bool any_dynamic = false;
+  for (const auto& B : pile->bnd) any_dynamic |= B.dynamic != 0;
+  if (any_dynamic && p.n) {
+    launch_body_sums(p, S->A, S->stream);
+    ALV_HIP(hipGetLastError());
+    ALV_HIP(hipMemcpyAsync(S->body_host, S->A.body_sums, pile->bnd.size() * 6 * sizeof(double),
+                           hipMemcpyDeviceToHost, S->stream));
+    ALV_HIP(hipStreamSynchronize(S->stream));
+    for (size_t b = 0; b < pile->bnd.size(); ++b) {
+      if (!pile->bnd[b].dynamic) continue;
+      for (int a = 0; a < 3; ++a) {
+        pile->force[b][a] = float(S->body_host[b * 6 + a]);
+        pile->torque[b][a] = float(S->body_host[b * 6 + 3 + a]);
+      }
+    }
+  }
+  alv_pile_integrate_kinematics(pile, S->set.dt);
+  S->set.t += S->set.dt;
+  S->dirty = ALV_F_DENSITY | ALV_F_ALPHA | ALV_F_RHO_ADV | ALV_F_A | ALV_F_NNB;
+  return ALV_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int alv_solver_step(alv_solver* S) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  const SimParams p = solver_params(S);
+  int rc = upload_step_inputs(S);
+  if (rc != ALV_OK) return rc;
+  DevArrays& A = S->A;
+  hipStream_t st = S->stream;
+  if (S->kind == ALV_SOLVER_DF) {
+    // SolverDf::step (solver_df.hpp:74-359)
+    const int div = S->set.enable_divergence_solve != 0;
+    const int dens = S->set.enable_density_solve != 0;
+    rc = neighbor_phase(S, p, div);
+    if (rc != ALV_OK) return rc;
+    launch_ctl_reset(A, st);
+    if (p.n) {
+      if (div) {
+        launch_df_warm_start_div(p, A, st);
+        rc = run_loop(S, true, S->set.max_divergence_solve, &S->last_div,
+                      [&] { launch_df_div_iteration(p, A, st); });
+        if (rc != ALV_OK) return rc;
+      }
+      launch_df_viscosity(p, A, div, st);
+    }
+    launch_update_dt(p, A, st);
+    if (p.n) {
+      launch_df_integrate_np(p, A, st);
+      if (dens) {
+        launch_df_warm_start_dens(p, A, st);
+        rc = run_loop(S, false, S->set.max_density_solve, &S->last_dens,
+                      [&] { launch_df_dens_iteration(p, A, st); });
+        if (rc != ALV_OK) return rc;
+      }
+      launch_df_finish(p, A, dens, st);
+    }
+  } else {
+    // SolverIi::step (solver_ii.hpp:76-243)
+    rc = neighbor_phase(S, p, 0);
+    if (rc != ALV_OK) return rc;
+    launch_ctl_reset(A, st);
+    if (p.n) launch_ii_viscosity_predict(p, A, st);
+    launch_update_dt(p, A, st);
+    if (p.n) {
+      launch_ii_predict(p, A, st);
+      rc = run_loop(S, false, S->set.max_density_solve, &S->last_dens,
+                    [&] { launch_ii_iteration(p, A, st); });
+      if (rc != ALV_OK) return rc;
+      launch_ii_finish(p, A, st);
+    }
+  }
+  ALV_HIP(hipGetLastError());
+  return finish_step(S, p);
+}
+
+int alv_solver_update_particle_neighbors(alv_solver* S) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  const SimParams p = solver_params(S);
+  int rc = upload_step_inputs(S);
+  if (rc != ALV_OK) return rc;
+  rc = neighbor_phase(S, p, S->kind == ALV_SOLVER_DF ? 1 : 0);
+  if (rc != ALV_OK) return rc;
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  S->dirty |= ALV_F_DENSITY | ALV_F_NNB | ALV_F_ALPHA;
+  return ALV_OK;
+}
+
+int alv_solver_reset_solving_var(alv_solver* S) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  if (S->kind == ALV_SOLVER_DF) {
+    ALV_HIP(hipMemset(S->A.ukappa, 0, size_t(S->cap) * sizeof(float)));
+    ALV_HIP(hipMemset(S->A.ukappa_v, 0, size_t(S->cap) * sizeof(float)));
+  } else {
+    ALV_HIP(hipMemset(S->A.upressure, 0, size_t(S->cap) * sizeof(float)));
+  }
+  return ALV_OK;
+}
+
+int alv_solver_reset_t(alv_solver* S) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  S->set.t = 0.f;
+  S->set.dt = S->set.initial_dt;
+  return ALV_OK;
+}
+
+int alv_solver_get_field(alv_solver* S, int field, void** dptr, size_t* n_elements,
+                         unsigned* n_components) {
+  if (!S || !dptr) return fail(ALV_ERR_ARG, "null argument");
+  DevArrays& A = S->A;
+  const size_t N = S->cap;
+  unsigned need = 0;
+  void* ptr = nullptr;
+  size_t n = N;
+  unsigned comp = 1;
+  switch (field) {
+    case ALV_FIELD_X: ptr = A.ux; comp = 3; break;
+    case ALV_FIELD_V: ptr = A.uv; comp = 3; break;
+    case ALV_FIELD_A: ptr = A.ua; comp = 3; need = ALV_F_A; break;
+    case ALV_FIELD_DENSITY: ptr = A.udensity; need = ALV_F_DENSITY; break;
+    case ALV_FIELD_DFSPH_FACTOR: ptr = A.ualpha; need = ALV_F_ALPHA; break;
+    case ALV_FIELD_KAPPA: ptr = A.ukappa; break;
+    case ALV_FIELD_KAPPA_V: ptr = A.ukappa_v; break;
+    case ALV_FIELD_DENSITY_ADV: ptr = A.urho_adv; need = ALV_F_RHO_ADV; break;
+    case ALV_FIELD_NUM_NEIGHBORS: ptr = A.unnb; need = ALV_F_NNB; break;
+    case ALV_FIELD_PRESSURE: ptr = A.upressure; break;
+    case ALV_FIELD_PID_LENGTH: ptr = A.cell_count; n = S->num_cells; break;
+    case ALV_FIELD_GRID_ANOMALY: ptr = A.anomaly; n = 3; break;
+    case ALV_FIELD_CELL_START: ptr = A.cell_start; n = size_t(S->num_cells) + 1; break;
+    default: return fail(ALV_ERR_ARG, "unknown field");
+  }
+  if (need & S->dirty) {
+    const SimParams p = solver_params(S);
+    if (p.n) {
+      launch_unsort_fields(p, A, need, S->stream);
+      ALV_HIP(hipGetLastError());
+    }
+    ALV_HIP(hipStreamSynchronize(S->stream));
+    S->dirty &= ~need;
+  }
+  *dptr = ptr;
+  if (n_elements) *n_elements = n;
+  if (n_components) *n_components = comp;
+  return ALV_OK;
+}
+
+int alv_solver_get_neighbors(alv_solver* S, unsigned* ids, unsigned* counts) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  const SimParams p = solver_params(S);
+  const size_t K = S->cni.max_num_neighbors_per_particle;
+  if (!S->export_ids) {
+    ALV_HIP(hipMalloc(&S->export_ids, K * S->cap * sizeof(unsigned)));
+    ALV_HIP(hipMalloc(&S->export_counts, size_t(S->cap) * sizeof(unsigned)));
+  }
+  ALV_HIP(hipMemsetAsync(S->export_ids, 0xff, K * S->cap * sizeof(unsigned), S->stream));
+  ALV_HIP(hipMemsetAsync(S->export_counts, 0, size_t(S->cap) * sizeof(unsigned), S->stream));
+  if (p.n) launch_export_neighbors(p, S->A, S->export_ids, S->export_counts, S->stream);
+  ALV_HIP(hipGetLastError());
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  if (ids) ALV_HIP(hipMemcpy(ids, S->export_ids, K * p.n * sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (counts)
+    ALV_HIP(hipMemcpy(counts, S->export_counts, size_t(p.n) * sizeof(unsigned),
+                      hipMemcpyDeviceToHost));
+  return ALV_OK;
+}
+
+int alv_solver_get_grid(alv_solver* S, unsigned* cell_count, unsigned* cell_start,
+                        unsigned* anomaly) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  const size_t C = S->num_cells;
+  if (cell_count)
+    ALV_HIP(hipMemcpy(cell_count, S->A.cell_count, C * sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (cell_start)
+    ALV_HIP(hipMemcpy(cell_start, S->A.cell_start, (C + 1) * sizeof(unsigned),
+                      hipMemcpyDeviceToHost));
+  if (anomaly) ALV_HIP(hipMemcpy(anomaly, S->A.anomaly, 3 * sizeof(unsigned), hipMemcpyDeviceToHost));
+  return ALV_OK;
+}
+
+int alv_node_positions(const float dmin[3], const unsigned res[3], const float cs[3],
+                       float* out, unsigned* num_nodes) {
+  if (!dmin || !res || !cs) return fail(ALV_ERR_ARG, "null argument");
+  GridGeomDev g;
+  memset(&g, 0, sizeof(g));
+  for (int a = 0; a < 3; ++a) {
+    g.dmin[a] = dmin[a];
+    g.cs[a] = cs[a];
+    g.res[a] = res[a];
+  }
+  size_t nv = size_t(res[0] + 1) * (res[1] + 1) * (res[2] + 1);
+  size_t ne = size_t(res[0]) * (res[1] + 1) * (res[2] + 1) +
+              size_t(res[0] + 1) * res[1] * (res[2] + 1) +
+              size_t(res[0] + 1) * (res[1] + 1) * res[2];
+  g.num_nodes = unsigned(nv + 2 * ne);
+  if (num_nodes) *num_nodes = g.num_nodes;
+  if (!out) return ALV_OK;
+  float* d = nullptr;
+  ALV_HIP(hipMalloc(&d, size_t(g.num_nodes) * 3 * sizeof(float)));
+  launch_node_positions(g, d, 0);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpy(out, d, size_t(g.num_nodes) * 3 * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(ALV_ERR_HIP, hipGetErrorString(e));
+  return ALV_OK;
+}
+
+int alv_volume_from_distance_map(const alv_const* cn, const float dmin[3], const float dmax[3],
+                                 const unsigned res[3], const float cs[3], float sign,
+                                 const float* distance, float* volume, unsigned num_nodes) {
+  if (!cn || !distance || !volume) return fail(ALV_ERR_ARG, "null argument");
+  GridGeomDev g;
+  memset(&g, 0, sizeof(g));
+  for (int a = 0; a < 3; ++a) {
+    g.dmin[a] = dmin[a];
+    g.dmax[a] = dmax[a];
+    g.cs[a] = cs[a];
+    g.res[a] = res[a];
+  }
+  g.num_nodes = num_nodes;
+  float table[120];
+  peirce_table(table);
+  float *dd = nullptr, *dv = nullptr, *dp = nullptr;
+  ALV_HIP(hipMalloc(&dd, size_t(num_nodes) * sizeof(float)));
+  ALV_HIP(hipMalloc(&dv, size_t(num_nodes) * sizeof(float)));
+  ALV_HIP(hipMalloc(&dp, sizeof(table)));
+  hipError_t e = hipMemcpy(dd, distance, size_t(num_nodes) * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dp, table, sizeof(table), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    alv_consti cni;
+    memset(&cni, 0, sizeof(cni));
+    SimParams p = make_params(*cn, cni, 0, 0);
+    launch_volume_field_grid(p, g, dd, dv, sign, dp, 0, num_nodes, 0);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(volume, dv, size_t(num_nodes) * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(dd);
+  (void)hipFree(dv);
+  (void)hipFree(dp);
+  if (e != hipSuccess) return fail(ALV_ERR_HIP, hipGetErrorString(e));
+  return ALV_OK;
+}
+
+int alv_profiling_enable(int enable) {
+  if (enable && !g_event_profiler) {
+    g_event_profiler = new EventProfiler();
+    g_profiler = g_event_profiler;
+  } else if (!enable && g_event_profiler) {
+    ALV_HIP(hipDeviceSynchronize());
+    g_profiler = nullptr;
+    g_event_profiler->collect();
+  }
+  return ALV_OK;
+}
+
+int alv_profiling_reset(void) {
+  if (g_event_profiler) {
+    ALV_HIP(hipDeviceSynchronize());
+    g_event_profiler->reset();
+  }
+  return ALV_OK;
+}
+
+int alv_profiling_num_kernels(unsigned* n) {
+  if (!n) return fail(ALV_ERR_ARG, "null argument");
+  if (g_event_profiler) {
+    ALV_HIP(hipDeviceSynchronize());
+    g_event_profiler->collect();
+  }
+  *n = g_event_profiler ? unsigned(g_event_profiler->stats().size()) : 0u;
+  return ALV_OK;
+}
+
+int alv_profiling_kernel(unsigned index, const char** name, double* total_ms,
+                         unsigned long long* launches) {
+  if (!g_event_profiler || index >= g_event_profiler->stats().size())
+    return fail(ALV_ERR_ARG, "kernel index out of range");
+  const auto& st = g_event_profiler->stats()[index];
+  if (name) *name = st.name.c_str();
+  if (total_ms) *total_ms = st.total_ms;
+  if (launches) *launches = st.launches;
+  return ALV_OK;
+}
+
+}  // extern "C"

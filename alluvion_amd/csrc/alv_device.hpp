@@ -1,0 +1,340 @@
+// alv_device.hpp -- device-side maths for the SPH solver core (gfx950).
+//
+// Own vector type and inline functions used by every kernel.  Expression
+// order follows the reference kernels (kennychufk/alluvion runner.hpp) so that
+// the FP32 results are reproducible; all code is compiled with
+// -ffp-contract=off (no fused multiply-add), and FP32 division and sqrt are
+// the correctly rounded HIP defaults.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "alv_types.hpp"
+
+namespace alv {
+
+struct vec3 {
+  float x, y, z;
+};
+__host__ __device__ inline vec3 mk(float x, float y, float z) {
+  vec3 r;
+  r.x = x;
+  r.y = y;
+  r.z = z;
+  return r;
+}
+__host__ __device__ inline vec3 operator+(vec3 a, vec3 b) {
+  return mk(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+__host__ __device__ inline vec3 operator-(vec3 a, vec3 b) {
+  return mk(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+__host__ __device__ inline vec3 operator-(vec3 a) { return mk(-a.x, -a.y, -a.z); }
+__host__ __device__ inline vec3 operator*(float s, vec3 a) {
+  return mk(s * a.x, s * a.y, s * a.z);
+}
+__host__ __device__ inline vec3 operator*(vec3 a, float s) {
+  return mk(a.x * s, a.y * s, a.z * s);
+}
+__host__ __device__ inline vec3 operator*(vec3 a, vec3 b) {
+  return mk(a.x * b.x, a.y * b.y, a.z * b.z);
+}
+__host__ __device__ inline vec3& operator+=(vec3& a, vec3 b) {
+  a = a + b;
+  return a;
+}
+__host__ __device__ inline vec3& operator-=(vec3& a, vec3 b) {
+  a = a - b;
+  return a;
+}
+__host__ __device__ inline float dot(vec3 a, vec3 b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__host__ __device__ inline vec3 cross(vec3 a, vec3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ inline float length_sqr(vec3 a) { return dot(a, a); }
+
+__device__ inline vec3 ld3(const float4& v) { return mk(v.x, v.y, v.z); }
+__device__ inline vec3 ld3(const float* p, unsigned i) {
+  return mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+__device__ inline void st3(float* p, unsigned i, vec3 v) {
+  p[3 * i] = v.x;
+  p[3 * i + 1] = v.y;
+  p[3 * i + 2] = v.z;
+}
+__device__ inline float4 f4(vec3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+// --- cubic spline kernel (runner.hpp:596-638) ------------------------------
+__device__ inline float W_dist(float r, const SimParams& p) {
+  float q = fabsf(r) / p.h;
+  float conj = 1.0f - q;
+  float result = 0.0f;
+  if (q <= 0.5f)
+    result = (6.0f * q - 6.0f) * q * q + 1.0f;
+  else if (q <= 1.0f)
+    result = 2.0f * conj * conj * conj;
+  return result * p.k;
+}
+__device__ inline float W_d2(float r2, const SimParams& p) {
+  float q2 = r2 / p.h2;
+  float q = sqrtf(r2) / p.h;
+  float conj = 1.0f - q;
+  float result = 0.0f;
+  if (q <= 0.5f)
+    result = (6.0f * q - 6.0f) * q2 + 1.0f;
+  else if (q <= 1.0f)
+    result = 2.0f * conj * conj * conj;
+  return result * p.k;
+}
+__device__ inline vec3 gradW(vec3 d, const SimParams& p) {
+  float rl2 = length_sqr(d);
+  float rl = sqrtf(rl2);
+  float q = rl / p.h;
+  float scale = (q <= 0.5f ? (q * 3.0f - 2.0f) / p.h
+                           : (q <= 1.0f ? (1.0f - q) * (q - 1.0f) / rl : 0.0f));
+  return (p.l * scale) * d;
+}
+
+// --- quaternion rotation (q = x, y, z, w) -----------------------------------
+__host__ __device__ inline vec3 rotate(vec3 v, float4 q) {
+  float xx = q.x * q.x, yy = q.y * q.y, zz = q.z * q.z;
+  float xy = q.x * q.y, xz = q.x * q.z, yz = q.y * q.z;
+  float wx = q.w * q.x, wy = q.w * q.y, wz = q.w * q.z;
+  vec3 r;
+  r.x = (1.0f - 2.0f * (yy + zz)) * v.x + 2.0f * (xy - wz) * v.y +
+        2.0f * (xz + wy) * v.z;
+  r.y = 2.0f * (xy + wz) * v.x + (1.0f - 2.0f * (xx + zz)) * v.y +
+        2.0f * (yz - wx) * v.z;
+  r.z = 2.0f * (xz - wy) * v.x + 2.0f * (yz + wx) * v.y +
+        (1.0f - 2.0f * (xx + yy)) * v.z;
+  return r;
+}
+__host__ __device__ inline float4 conjugate(float4 q) {
+  return make_float4(-q.x, -q.y, -q.z, q.w);
+}
+
+// --- analytic SDFs (dg/box_distance.hpp, dg/box_shell_distance.hpp,
+//     dg/sphere_distance.hpp; device variants) --------------------------------
+__device__ inline float box_dist(vec3 x, vec3 half) {
+  vec3 diff = mk(fabsf(x.x) - half.x, fabsf(x.y) - half.y, fabsf(x.z) - half.z);
+  vec3 cl = mk(fmaxf(diff.x, 0.0f), fmaxf(diff.y, 0.0f), fmaxf(diff.z, 0.0f));
+  return sqrtf(length_sqr(cl)) + fminf(fmaxf(diff.x, fmaxf(diff.y, diff.z)), 0.0f);
+}
+__device__ inline float sdf_eval(const BoundaryDev& b, vec3 x) {
+  if (b.type == ALV_DIST_BOX) return box_dist(x, mk(b.half[0], b.half[1], b.half[2])) - b.outset;
+  if (b.type == ALV_DIST_BOX_SHELL) {
+    float inner = box_dist(x, mk(b.half[0], b.half[1], b.half[2])) - b.outset;
+    float outer =
+        box_dist(x, mk(b.half_outer[0], b.half_outer[1], b.half_outer[2])) - b.outset;
+    return fmaxf(outer, -inner);
+  }
+  return sqrtf(length_sqr(x)) - b.radius;
+}
+__device__ inline vec3 sdf_gradient(const BoundaryDev& b, vec3 x, float scale) {
+  if (b.type == ALV_DIST_SPHERE) return x;
+  float step = scale * 0.00390625f;
+  return mk(sdf_eval(b, x + mk(step, 0.0f, 0.0f)) - sdf_eval(b, x - mk(step, 0.0f, 0.0f)),
+            sdf_eval(b, x + mk(0.0f, step, 0.0f)) - sdf_eval(b, x - mk(0.0f, step, 0.0f)),
+            sdf_eval(b, x + mk(0.0f, 0.0f, step)) - sdf_eval(b, x - mk(0.0f, 0.0f, step)));
+}
+
+// --- 32-node cubic Lagrange volume map (runner.hpp:1131-1594) ---------------
+// Node numbering: (rx+1)(ry+1)(rz+1) corner nodes with x fastest, then two
+// nodes on every x-, y- and z-directed cell edge.
+__host__ __device__ inline vec3 node_position(const GridGeomDev& g, unsigned l) {
+  const unsigned rx = g.res[0], ry = g.res[1], rz = g.res[2];
+  unsigned nv = (rx + 1) * (ry + 1) * (rz + 1);
+  unsigned ne_x = rx * (ry + 1) * (rz + 1);
+  unsigned ne_y = (rx + 1) * ry * (rz + 1);
+  vec3 dmin = mk(g.dmin[0], g.dmin[1], g.dmin[2]);
+  vec3 cs = mk(g.cs[0], g.cs[1], g.cs[2]);
+  vec3 x;
+  if (l < nv) {
+    unsigned temp = l % ((ry + 1) * (rx + 1));
+    x = dmin + cs * mk(float(temp % (rx + 1)), float(temp / (rx + 1)),
+                       float(l / ((ry + 1) * (rx + 1))));
+  } else if (l < nv + 2 * ne_x) {
+    l -= nv;
+    unsigned e = l / 2;
+    unsigned temp = e % ((ry + 1) * rx);
+    x = dmin + cs * mk(float(temp % rx), float(temp / rx), float(e / ((ry + 1) * rx)));
+    x.x += (1.0f + float(l % 2)) / 3.0f * cs.x;
+  } else if (l < nv + 2 * (ne_x + ne_y)) {
+    l -= (nv + 2 * ne_x);
+    unsigned e = l / 2;
+    unsigned temp = e % ((rz + 1) * ry);
+    x = dmin + cs * mk(float(e / ((rz + 1) * ry)), float(temp % ry), float(temp / ry));
+    x.y += (1.0f + float(l % 2)) / 3.0f * cs.y;
+  } else {
+    l -= (nv + 2 * (ne_x + ne_y));
+    unsigned e = l / 2;
+    unsigned temp = e % ((rx + 1) * rz);
+    x = dmin + cs * mk(float(temp / rz), float(e / ((rx + 1) * rz)), float(temp % rz));
+    x.z += (1.0f + float(l % 2)) / 3.0f * cs.z;
+  }
+  return x;
+}
+
+// Locates x in the grid; returns false outside the (closed) domain.
+__device__ inline bool grid_resolve(const GridGeomDev& g, vec3 x, int ip[3], vec3& inner) {
+  vec3 inv = mk(1.0f / g.cs[0], 1.0f / g.cs[1], 1.0f / g.cs[2]);
+  if (x.x >= g.dmin[0] && x.y >= g.dmin[1] && x.z >= g.dmin[2] && g.dmax[0] >= x.x &&
+      g.dmax[1] >= x.y && g.dmax[2] >= x.z) {
+    vec3 f = (x - mk(g.dmin[0], g.dmin[1], g.dmin[2])) * inv;
+    ip[0] = min(int(f.x), int(g.res[0]) - 1);
+    ip[1] = min(int(f.y), int(g.res[1]) - 1);
+    ip[2] = min(int(f.z), int(g.res[2]) - 1);
+    vec3 sd_min = mk(g.dmin[0], g.dmin[1], g.dmin[2]) +
+                  mk(g.cs[0], g.cs[1], g.cs[2]) * mk(float(ip[0]), float(ip[1]), float(ip[2]));
+    vec3 t = (x - sd_min) * inv;
+    inner = mk(2.0f * t.x - 1.0f, 2.0f * t.y - 1.0f, 2.0f * t.z - 1.0f);
+    return true;
+  }
+  return false;
+}
+
+__host__ __device__ inline void grid_cell_nodes(const unsigned res[3], const int ip[3],
+                                                unsigned* cell) {
+  const unsigned rx = res[0], ry = res[1], rz = res[2];
+  const unsigned i = ip[0], j = ip[1], k = ip[2];
+  const unsigned sx = rx + 1, sxy = (rx + 1) * (ry + 1);
+#pragma unroll
+  for (unsigned c = 0; c < 8; ++c) {
+    unsigned di = c & 1u, dj = (c >> 1) & 1u, dk = (c >> 2) & 1u;
+    cell[c] = sxy * (k + dk) + sx * (j + dj) + (i + di);
+  }
+  unsigned base = (rx + 1) * (ry + 1) * (rz + 1);
+#pragma unroll
+  for (unsigned e = 0; e < 4; ++e) {
+    unsigned dj = e >> 1, dk = e & 1u;
+    unsigned n = base + 2 * (rx * (ry + 1) * (k + dk) + rx * (j + dj) + i);
+    cell[8 + 2 * e] = n;
+    cell[9 + 2 * e] = n + 1;
+  }
+  base += 2 * (rx * (ry + 1) * (rz + 1));
+#pragma unroll
+  for (unsigned e = 0; e < 4; ++e) {
+    unsigned di = e & 1u, dk = e >> 1;
+    unsigned n = base + 2 * (ry * (rz + 1) * (i + di) + ry * (k + dk) + j);
+    cell[16 + 2 * e] = n;
+    cell[17 + 2 * e] = n + 1;
+  }
+  base += 2 * ((rx + 1) * ry * (rz + 1));
+#pragma unroll
+  for (unsigned e = 0; e < 4; ++e) {
+    unsigned di = e >> 1, dj = e & 1u;
+    unsigned n = base + 2 * (rz * (rx + 1) * (j + dj) + rz * (i + di) + k);
+    cell[24 + 2 * e] = n;
+    cell[25 + 2 * e] = n + 1;
+  }
+}
+
+// Value of the 32-node element at local coordinate xi, interpolating the
+// nodal values `nodes` of the cell.  Corner node (a,b,c) in {-1,1}^3:
+//   N = (1/64)(1+a x)(1+b y)(1+c z)(9(x^2+y^2+z^2) - 19);
+// edge node at u = s/3 along an edge with transverse signs (p, q):
+//   N = (9/64)(1-u^2)(1+3 s u)(1+p v1)(1+q v2).
+__device__ inline float grid_interpolate(const float* __restrict__ nodes,
+                                         const unsigned* cell, vec3 xi) {
+  const float x = xi.x, y = xi.y, z = xi.z;
+  const float k64 = 1.0f / 64.0f;
+  const float k964 = 9.0f / 64.0f;
+  const float F = k64 * (9.0f * (x * x + y * y + z * z) - 19.0f);
+  float phi = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    float a = (c & 1) ? 1.0f : -1.0f;
+    float b = (c & 2) ? 1.0f : -1.0f;
+    float cc = (c & 4) ? 1.0f : -1.0f;
+    float A = 1.0f + a * x, B = 1.0f + b * y, C = 1.0f + cc * z;
+    float N = F * (A * B) * C;
+    phi += nodes[cell[c]] * N;
+  }
+#pragma unroll
+  for (int axis = 0; axis < 3; ++axis) {
+    float u = axis == 0 ? x : (axis == 1 ? y : z);
+    float v1 = axis == 0 ? y : x;
+    float v2 = axis == 2 ? y : z;
+    float one_m_u2 = 1.0f - u * u;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float p, q;
+      if (axis == 0) {
+        p = (e >> 1) ? 1.0f : -1.0f;
+        q = (e & 1) ? 1.0f : -1.0f;
+      } else if (axis == 1) {
+        p = (e & 1) ? 1.0f : -1.0f;
+        q = (e >> 1) ? 1.0f : -1.0f;
+      } else {
+        p = (e >> 1) ? 1.0f : -1.0f;
+        q = (e & 1) ? 1.0f : -1.0f;
+      }
+      float P = 1.0f + p * v1, Q = 1.0f + q * v2;
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2) {
+        float s = s2 ? 1.0f : -1.0f;
+        float L = 1.0f + 3.0f * s * u;
+        int n = 8 + axis * 8 + 2 * e + s2;
+        float G = k964 * (one_m_u2 * L);
+        float N = G * (P * Q);
+        phi += nodes[cell[n]] * N;
+      }
+    }
+  }
+  return phi;
+}
+
+// Interpolates the 32-node map at x; FLT_MAX outside (used by the volume
+// field builder for distance maps, runner.hpp:1596-1614).
+__device__ inline float grid_sample(const GridGeomDev& g, const float* __restrict__ nodes,
+                                    vec3 x, float outside) {
+  int ip[3];
+  vec3 inner;
+  if (!grid_resolve(g, x, ip, inner)) return outside;
+  unsigned cell[32];
+  grid_cell_nodes(g.res, ip, cell);
+  return grid_interpolate(nodes, cell, inner);
+}
+
+// --- neighbour grid helpers (runner.hpp:1827-1838, 689-707) -----------------
+__device__ inline void cell_ipos(vec3 x, const SimParams& p, int ip[3]) {
+  ip[0] = int(floorf(x.x / p.h)) - p.off[0];
+  ip[1] = int(floorf(x.y / p.h)) - p.off[1];
+  ip[2] = int(floorf(x.z / p.h)) - p.off[2];
+}
+__device__ inline bool within_grid(const int ip[3], const SimParams& p) {
+  return 0 <= ip[0] && ip[0] < p.res[0] && 0 <= ip[1] && ip[1] < p.res[1] && 0 <= ip[2] &&
+         ip[2] < p.res[2];
+}
+__device__ inline unsigned cell_linear(const int ip[3], const SimParams& p) {
+  return (unsigned(ip[0]) * unsigned(p.res[1]) + unsigned(ip[1])) * unsigned(p.res[2]) +
+         unsigned(ip[2]);
+}
+__device__ inline float wrap1(float d, float len, float lo, float hi) {
+  if (d >= hi)
+    d -= len;
+  else if (d < lo)
+    d += len;
+  return d;
+}
+__device__ inline vec3 wrap_disp(vec3 d, const SimParams& p) {
+  if (p.wrap_mask & 1) d.x = wrap1(d.x, p.wrap_len[0], p.wrap_min[0], p.wrap_max[0]);
+  if (p.wrap_mask & 2) d.y = wrap1(d.y, p.wrap_len[1], p.wrap_min[1], p.wrap_max[1]);
+  if (p.wrap_mask & 4) d.z = wrap1(d.z, p.wrap_len[2], p.wrap_min[2], p.wrap_max[2]);
+  return d;
+}
+
+// Block index remapped so that each of the 8 XCDs (which receive blocks
+// round-robin) works on one contiguous range of particles: neighbouring
+// particles then share an XCD's L2.
+__device__ inline unsigned xcd_block() {
+  const unsigned b = blockIdx.x, G = gridDim.x;
+  const unsigned q = G / 8u, r = G % 8u;
+  const unsigned x = b % 8u, k = b / 8u;
+  return x * q + min(x, r) + k;
+}
+
+}  // namespace alv

@@ -1,0 +1,147 @@
+// alv_kernels.hpp -- host-side launchers for the HIP kernels of the solver
+// core.  All launches are asynchronous on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "alv_types.hpp"
+
+namespace alv {
+
+// Device pointer bundle of one solver (passed to kernels by value).
+// "u*" arrays are in user (API) order and hold AoS float3 vectors like the
+// reference's Variable<1,float3>; all other per-particle arrays are in
+// cell-sorted order, SoA, float4-padded vectors.
+struct DevArrays {
+  unsigned cap;  // max_num_particles: row stride of [B, cap] and [K, cap]
+  // user order
+  float* ux;
+  float* uv;
+  float* ua;
+  float* udensity;
+  float* ualpha;
+  float* ukappa;
+  float* ukappa_v;
+  float* urho_adv;
+  unsigned* unnb;
+  float* upressure;
+  // counting sort
+  unsigned* key;
+  unsigned* slot;
+  unsigned* sorted_id;  // sorted position -> user index
+  unsigned* cell_count;  // [num_cells + 1]; last bin = outside the grid
+  unsigned* cell_start;  // [num_cells + 2]
+  unsigned* scan_tmp;
+  unsigned* anomaly;  // [3]
+  // sorted order
+  float4* x;
+  float4* v;
+  float4* a;
+  float* density;
+  float* alpha;
+  float* kappa;
+  float* kappa_v;
+  float* rho_adv;
+  float* kfac;  // stiffness k_j of the running Jacobi iteration
+  unsigned* nbr;  // [kmax, cap], column-major
+  unsigned* nnb;
+  float4* brec;  // [B, cap] (x_b, Psi)
+  float4* bker;  // [B, cap] (Psi gradW, Psi W)
+  unsigned* bmask;  // bit b set when Psi_b != 0
+  float4* bforce;  // [B, cap]
+  float4* btorque;  // [B, cap]
+  // IISPH (sorted order)
+  float* pressure;
+  float* last_pressure;
+  float* aii;
+  float4* dii;
+  float4* dij_pj;
+  float* adv_density;
+  float* density_err;
+  float4* pacc;
+  // reductions and control
+  double* partials;  // per-block FP64 partial sums
+  float* fpartials;  // per-block float maxima
+  double* body_sums;  // [B, 6]
+  StepCtl* ctl;
+  const BoundaryDev* bnd;
+};
+
+constexpr unsigned kBlock = 256;
+
+// Optional per-kernel timing hook (HIP events around each launch); null
+// when profiling is off.
+struct KernelProfiler {
+  virtual ~KernelProfiler() = default;
+  virtual void begin(const char* name, hipStream_t s) = 0;
+  virtual void end(const char* name, hipStream_t s) = 0;
+};
+extern KernelProfiler* g_profiler;
+inline unsigned grid_for(unsigned n) { return (n + kBlock - 1) / kBlock; }
+
+// neighbour search (counting sort)
+void launch_cell_count(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_scan(unsigned count, const unsigned* in, unsigned* out, unsigned* tmp,
+                 hipStream_t s);
+void launch_scatter(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_cell_sort(const SimParams& p, const DevArrays& A, hipStream_t s);
+// gather user -> sorted and compute boundary records
+void launch_gather_boundary(const SimParams& p, const DevArrays& A, int solver_kind,
+                            hipStream_t s);
+// neighbour list + density (+ DFSPH factor when with_factor)
+void launch_neighbors(const SimParams& p, const DevArrays& A, int wrap_any, int with_factor,
+                      hipStream_t s);
+
+// DFSPH
+void launch_df_warm_start_div(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_df_div_loop_prep(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_df_div_iteration(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_df_viscosity(const SimParams& p, const DevArrays& A, int div_finish,
+                         hipStream_t s);
+void launch_update_dt(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_df_integrate_np(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_df_warm_start_dens(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_df_dens_iteration(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_df_finish(const SimParams& p, const DevArrays& A, int dens_finish, hipStream_t s);
+
+// IISPH
+void launch_ii_viscosity_predict(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_ii_predict(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_ii_iteration(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_ii_finish(const SimParams& p, const DevArrays& A, hipStream_t s);
+
+// rigid bodies: per-body FP64 force / torque sums into A.body_sums
+void launch_body_sums(const SimParams& p, const DevArrays& A, hipStream_t s);
+// sorted -> user order for the lazily synchronised API arrays
+void launch_unsort_fields(const SimParams& p, const DevArrays& A, unsigned field_mask,
+                          hipStream_t s);
+// reference-layout neighbour ids (user order) for inspection: ids[i*kmax+t]
+void launch_export_neighbors(const SimParams& p, const DevArrays& A, unsigned* ids,
+                             unsigned* counts, hipStream_t s);
+// reset the control block for a fresh solve
+void launch_ctl_reset(const DevArrays& A, hipStream_t s);
+
+// volume maps (setup)
+void launch_volume_field_analytic(const SimParams& p, const BoundaryDev& b, float* volume,
+                                  const float* peirce, unsigned begin, unsigned end,
+                                  hipStream_t s);
+void launch_volume_field_grid(const SimParams& p, const GridGeomDev& g,
+                              const float* distance, float* volume, float sign,
+                              const float* peirce, unsigned begin, unsigned end,
+                              hipStream_t s);
+void launch_node_positions(const GridGeomDev& g, float* out, hipStream_t s);
+
+// field sync masks for launch_unsort_fields
+enum : unsigned {
+  ALV_F_DENSITY = 1u << 0,
+  ALV_F_ALPHA = 1u << 1,
+  ALV_F_RHO_ADV = 1u << 2,
+  ALV_F_A = 1u << 3,
+  ALV_F_NNB = 1u << 4,
+  ALV_F_PRESSURE = 1u << 5,
+};
+
+// field codes written by launch_unsort_fields / gathered in gather_boundary
+enum { ALV_KIND_DF = 0, ALV_KIND_II = 1 };
+
+}  // namespace alv

@@ -1,0 +1,1170 @@
+// alv_step.hip -- HIP kernels of the per-step SPH solver core for gfx950.
+//
+// Layout: particles are counting-sorted by grid cell every step (cell size h,
+// z-fastest cell numbering as the reference's pid_length,
+// variable.hpp:286-290; ascending particle id inside a cell).  All per-step
+// passes run in that sorted order on SoA arrays, so a wavefront's 64 lanes
+// touch neighbouring particles and the neighbour gathers stay in L2.  The
+// neighbour list stores u32 sorted indices column-major ([slot][particle]) so
+// that slot t of 64 lanes is one 256-byte coalesced load; displacements are
+// recomputed from positions (bit-identical to the reference's stored x_ij,
+// because positions do not change inside a step).
+//
+// Convergence: each residual pass writes one FP64 partial sum per block; a
+// one-block finaliser reduces them in a fixed order and updates the device
+// control block (iteration count, done flag) with the reference's
+// min/max-iteration semantics (solver_df.hpp:131-133, 296-298).  Iteration
+// kernels return at once when the flag is set, so the host can enqueue
+// iterations in chunks without a round trip per iteration.
+
+#include <float.h>
+
+#include "alv_device.hpp"
+#include "alv_kernels.hpp"
+
+namespace alv {
+
+#define ALV_TID() (xcd_block() * kBlock + threadIdx.x)
+
+__device__ inline float std_max(float a, float b) { return (a < b) ? b : a; }
+__device__ inline float std_min(float a, float b) { return (b < a) ? b : a; }
+
+// ---------------------------------------------------------------------------
+// wave / block reductions (wave64)
+// ---------------------------------------------------------------------------
+__device__ inline double block_sum_d(double v, double* lds) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (unsigned i = 0; i < blockDim.x / 64u; ++i) r += lds[i];
+  return r;
+}
+__device__ inline float block_max_f(float v, float* lds) {
+  for (int o = 32; o > 0; o >>= 1) v = std_max(v, __shfl_down(v, o, 64));
+  const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  float r = -FLT_MAX;
+  if (threadIdx.x == 0)
+    for (unsigned i = 0; i < blockDim.x / 64u; ++i) r = std_max(r, lds[i]);
+  return r;
+}
+// exclusive scan of one u32 per thread over a block of 256 threads
+__device__ inline unsigned block_excl_scan_u32(unsigned v, unsigned* lds, unsigned* total) {
+  const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  unsigned inc = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned t = __shfl_up(inc, o, 64);
+    if (lane >= unsigned(o)) inc += t;
+  }
+  if (lane == 63) lds[w] = inc;
+  __syncthreads();
+  unsigned off = 0, tot = 0;
+  for (unsigned i = 0; i < blockDim.x / 64u; ++i) {
+    unsigned t = lds[i];
+    if (i < w) off += t;
+    tot += t;
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return off + inc - v;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour search: counting sort of particles by cell
+// (replaces update_particle_grid, runner.hpp:1840-1867)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cell_count(SimParams p, DevArrays A) {
+  const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= p.n) return;
+  vec3 x = ld3(A.ux, i);
+  int ip[3];
+  cell_ipos(x, p, ip);
+  unsigned k;
+  if (within_grid(ip, p)) {
+    k = cell_linear(ip, p);
+  } else {
+    k = p.num_cells;  // outside the grid: own bin after the last cell
+    A.anomaly[0] = 1u;
+  }
+  A.key[i] = k;
+  A.slot[i] = atomicAdd(&A.cell_count[k], 1u);
+}
+
+constexpr unsigned kScanPerThread = 16, kScanPerBlock = kScanPerThread * kBlock;
+
+__global__ __launch_bounds__(256) void k_scan_reduce(unsigned count, const unsigned* in,
+                                                     unsigned* block_sums) {
+  __shared__ double lds[4];
+  const unsigned base = blockIdx.x * kScanPerBlock;
+  unsigned s = 0;
+  for (unsigned k = 0; k < kScanPerThread; ++k) {
+    unsigned idx = base + k * kBlock + threadIdx.x;
+    if (idx < count) s += in[idx];
+  }
+  double t = block_sum_d(double(s), lds);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = unsigned(t);
+}
+
+// single block: exclusive scan of the block sums, in place (<= 4096 blocks)
+__global__ __launch_bounds__(256) void k_scan_top(unsigned nblocks, unsigned* sums) {
+  __shared__ unsigned lds[4];
+  const unsigned per = (nblocks + kBlock - 1) / kBlock;
+  const unsigned base = threadIdx.x * per;
+  unsigned local = 0;
+  for (unsigned k = 0; k < per; ++k)
+    if (base + k < nblocks) local += sums[base + k];
+  unsigned off = block_excl_scan_u32(local, lds, nullptr);
+  for (unsigned k = 0; k < per; ++k) {
+    if (base + k < nblocks) {
+      unsigned v = sums[base + k];
+      sums[base + k] = off;
+      off += v;
+    }
+  }
+}
+
+// out[0..count] = exclusive scan of in[0..count); out[count] = total
+__global__ __launch_bounds__(256) void k_scan_apply(unsigned count, const unsigned* in,
+                                                    unsigned* out, const unsigned* offsets) {
+  __shared__ unsigned lds[4];
+  const unsigned base = blockIdx.x * kScanPerBlock + threadIdx.x * kScanPerThread;
+  unsigned vals[kScanPerThread];
+  unsigned local = 0;
+#pragma unroll
+  for (unsigned k = 0; k < kScanPerThread; ++k) {
+    vals[k] = (base + k < count) ? in[base + k] : 0u;
+    local += vals[k];
+  }
+  unsigned run = block_excl_scan_u32(local, lds, nullptr) + offsets[blockIdx.x];
+#pragma unroll
+  for (unsigned k = 0; k < kScanPerThread; ++k) {
+    if (base + k < count) out[base + k] = run;
+    run += vals[k];
+    if (base + k + 1 == count) out[count] = run;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(SimParams p, DevArrays A) {
+  const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= p.n) return;
+  A.sorted_id[A.cell_start[A.key[i]] + A.slot[i]] = i;
+}
+
+// Makes the order inside each cell canonical (ascending particle id); the
+// atomic slots of k_cell_count are arrival-ordered.
+__global__ __launch_bounds__(256) void k_cell_sort(SimParams p, DevArrays A) {
+  const unsigned c = blockIdx.x * kBlock + threadIdx.x;
+  if (c > p.num_cells) return;
+  const unsigned b = A.cell_start[c], e = A.cell_start[c + 1];
+  if (c < p.num_cells && e - b > p.cell_cap) A.anomaly[1] = 1u;
+  if (e - b < 2) return;
+  unsigned* ids = A.sorted_id;
+  if (e - b <= 32) {
+    unsigned loc[32];
+    const unsigned m = e - b;
+    for (unsigned t = 0; t < m; ++t) loc[t] = ids[b + t];
+    for (unsigned t = 1; t < m; ++t) {
+      unsigned v = loc[t];
+      unsigned u = t;
+      while (u > 0 && loc[u - 1] > v) {
+        loc[u] = loc[u - 1];
+        --u;
+      }
+      loc[u] = v;
+    }
+    for (unsigned t = 0; t < m; ++t) ids[b + t] = loc[t];
+  } else {
+    for (unsigned t = b + 1; t < e; ++t) {
+      unsigned v = ids[t];
+      unsigned u = t;
+      while (u > b && ids[u - 1] > v) {
+        ids[u] = ids[u - 1];
+        --u;
+      }
+      ids[u] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gather user -> sorted order and compute the boundary records
+// (compute_particle_boundary_analytic<0>, runner.hpp:1982-2002, with
+// compute_volume_and_boundary_x_analytic, runner.hpp:1738-1771)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gather_boundary(SimParams p, DevArrays A, int kind) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const unsigned i = A.sorted_id[s];
+  vec3 x = ld3(A.ux, i);
+  vec3 v = ld3(A.uv, i);
+  A.x[s] = f4(x, 0.0f);
+  A.v[s] = f4(v, 0.0f);
+  if (kind == ALV_KIND_DF) {
+    A.kappa[s] = A.ukappa[i];
+    A.kappa_v[s] = A.ukappa_v[i];
+  } else {
+    A.pressure[s] = A.upressure[i];
+  }
+  unsigned mask = 0;
+  for (unsigned b = 0; b < p.num_boundaries; ++b) {
+    const BoundaryDev& B = A.bnd[b];
+    const float4 q = make_float4(B.q[0], B.q[1], B.q[2], B.q[3]);
+    vec3 local = rotate(x - mk(B.x[0], B.x[1], B.x[2]), conjugate(q));
+    float d = sdf_eval(B, local) * B.sign;
+    vec3 normal = rotate(sdf_gradient(B, local, p.h) * B.sign, q);
+    float nl2 = length_sqr(normal);
+    normal = normal * (nl2 > 0.0f ? 1.0f / sqrtf(nl2) : 0.0f);
+    vec3 xi_bxj = d * normal;
+    vec3 bx = x - xi_bxj;
+    float vol = 0.0f;
+    int ip[3];
+    vec3 inner;
+    if (grid_resolve(B.geom, local, ip, inner)) {
+      unsigned cell[32];
+      grid_cell_nodes(B.geom.res, ip, cell);
+      vol = grid_interpolate(B.volume, cell, inner);
+    }
+    vec3 kg = gradW(xi_bxj, p);
+    const size_t o = size_t(b) * A.cap + s;
+    A.brec[o] = f4(bx, vol);
+    A.bker[o] = make_float4(vol * kg.x, vol * kg.y, vol * kg.z, vol * W_dist(d, p));
+    if (vol != 0.0f) mask |= 1u << b;
+    if (B.dynamic) {
+      A.bforce[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+      A.btorque[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+  A.bmask[s] = mask;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour list (make_neighbor_list<wrap>, runner.hpp:1869-1911) fused with
+// compute_density (runner.hpp:2052-2072) and, for DFSPH,
+// compute_dfsph_factor (runner.hpp:3124-3154).  Cells are visited in the
+// reference order (dx outer, dz inner); inside a cell in ascending particle
+// id, which is ascending sorted index.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_neighbors(SimParams p, DevArrays A, int with_factor) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const unsigned K = p.kmax, cap = A.cap;
+  const vec3 xi = ld3(A.x[s]);
+  int ip[3];
+  cell_ipos(xi, p, ip);
+  unsigned cnt = 0, last_j = 0;
+  float dens = p.vol * p.w0;
+  float sgk = 0.0f;
+  vec3 gpi = mk(0.f, 0.f, 0.f);
+  for (int t = 0; t < 27; ++t) {
+    int np[3] = {ip[0] + t / 9 - 1, ip[1] + (t / 3) % 3 - 1, ip[2] + t % 3 - 1};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      if (p.wrap_mask & (1 << ax)) {
+        if (np[ax] >= p.res[ax])
+          np[ax] -= p.res[ax];
+        else if (np[ax] < 0)
+          np[ax] += p.res[ax];
+      }
+    }
+    if (!within_grid(np, p)) continue;
+    const unsigned c = cell_linear(np, p);
+    const unsigned jb = A.cell_start[c], je = A.cell_start[c + 1];
+    for (unsigned j = jb; j < je; ++j) {
+      if (j == s) continue;
+      vec3 d = wrap_disp(xi - ld3(A.x[j]), p);
+      float r2 = length_sqr(d);
+      if (r2 < p.h2) {
+        A.nbr[size_t(min(cnt, K - 1)) * cap + s] = j;
+        if (cnt + 1 < K) {
+          dens += p.vol * W_d2(r2, p);
+          if (with_factor) {
+            vec3 gp = (-p.vol) * gradW(d, p);
+            sgk += length_sqr(gp);
+            gpi -= gp;
+          }
+        }
+        last_j = j;
+        ++cnt;
+      }
+    }
+  }
+  if (cnt >= K) {
+    // slot K-1 holds the last neighbour found (the reference overwrites it)
+    vec3 d = wrap_disp(xi - ld3(A.x[last_j]), p);
+    dens += p.vol * W_d2(length_sqr(d), p);
+    if (with_factor) {
+      vec3 gp = (-p.vol) * gradW(d, p);
+      sgk += length_sqr(gp);
+      gpi -= gp;
+    }
+    if (cnt > K) A.anomaly[2] = 1u;
+  }
+  unsigned mask = A.bmask[s];
+  while (mask) {
+    const unsigned b = __builtin_ctz(mask);
+    mask &= mask - 1u;
+    float4 bk = A.bker[size_t(b) * cap + s];
+    dens += bk.w;
+    gpi += mk(bk.x, bk.y, bk.z);
+  }
+  A.nnb[s] = min(cnt, K);
+  A.density[s] = dens * p.rho0;
+  if (with_factor) {
+    sgk += length_sqr(gpi);
+    A.alpha[s] = sgk > p.eps ? -1.0f / sgk : 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DFSPH building blocks
+// ---------------------------------------------------------------------------
+__device__ inline vec3 rigid_velocity(const BoundaryDev& B, vec3 bx) {
+  return cross(mk(B.omega[0], B.omega[1], B.omega[2]), bx - mk(B.x[0], B.x[1], B.x[2])) +
+         mk(B.v[0], B.v[1], B.v[2]);
+}
+
+// sum_j V (v_i - v_j) . gradW_ij + sum_b (v_i - v_b) . Psi gradW_b
+__device__ inline float velocity_divergence(const SimParams& p, const DevArrays& A,
+                                            unsigned s, vec3 xi, vec3 vi, unsigned nn) {
+  float acc = 0.0f;
+  for (unsigned t = 0; t < nn; ++t) {
+    const unsigned j = A.nbr[size_t(t) * A.cap + s];
+    vec3 d = wrap_disp(xi - ld3(A.x[j]), p);
+    acc += p.vol * dot(vi - ld3(A.v[j]), gradW(d, p));
+  }
+  unsigned mask = A.bmask[s];
+  while (mask) {
+    const unsigned b = __builtin_ctz(mask);
+    mask &= mask - 1u;
+    const size_t o = size_t(b) * A.cap + s;
+    float4 r = A.brec[o], k = A.bker[o];
+    vec3 velj = rigid_velocity(A.bnd[b], ld3(r));
+    acc += dot(vi - velj, mk(k.x, k.y, k.z));
+  }
+  return acc;
+}
+
+// compute_density_change (runner.hpp:3192-3228)
+__device__ inline float density_change(const SimParams& p, const DevArrays& A, unsigned s) {
+  const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
+  const unsigned nn = A.nnb[s];
+  float da = velocity_divergence(p, A, s, xi, vi, nn);
+  da = std_max(da, 0.0f);
+  if (nn < 20u) da = 0.0f;
+  return da;
+}
+
+// compute_density_adv (runner.hpp:3493-3523)
+__device__ inline float density_adv(const SimParams& p, const DevArrays& A, unsigned s,
+                                    float dt) {
+  const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
+  const unsigned nn = A.nnb[s];
+  float delta = velocity_divergence(p, A, s, xi, vi, nn);
+  return std_max(A.density[s] / p.rho0 + dt * delta, 1.0f);
+}
+
+// Velocity change of one stiffness pass (warm_start_divergence_solve_1,
+// divergence_solve_iteration, warm_start_pressure_solve1,
+// pressure_solve_iteration: runner.hpp:3252-3295, 3316-3363, 3452-3491,
+// 3543-3588).  kj[] holds every particle's stiffness of this pass.
+template <bool kDivergenceForm>
+__device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsigned s,
+                                    float k_i, const float* __restrict__ kj, float dt) {
+  const vec3 xi = ld3(A.x[s]);
+  const unsigned nn = A.nnb[s];
+  vec3 dv = mk(0.f, 0.f, 0.f);
+  for (unsigned t = 0; t < nn; ++t) {
+    const unsigned j = A.nbr[size_t(t) * A.cap + s];
+    const float k_sum = k_i + kj[j];
+    if (fabsf(k_sum) > p.eps) {
+      vec3 d = wrap_disp(xi - ld3(A.x[j]), p);
+      if (kDivergenceForm) {
+        vec3 grad_p_j = (-p.vol) * gradW(d, p);
+        dv -= (dt * k_sum) * grad_p_j;
+      } else {
+        dv += ((dt * k_sum) * p.vol) * gradW(d, p);
+      }
+    }
+  }
+  if (fabsf(k_i) > p.eps) {
+    unsigned mask = A.bmask[s];
+    while (mask) {
+      const unsigned b = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      const size_t o = size_t(b) * A.cap + s;
+      float4 k = A.bker[o];
+      vec3 acc = k_i * (-mk(k.x, k.y, k.z));
+      dv -= dt * acc;
+      const BoundaryDev& B = A.bnd[b];
+      if (B.dynamic) {
+        vec3 f = p.mass * acc;
+        vec3 tq = cross(ld3(A.brec[o]) - mk(B.x[0], B.x[1], B.x[2]), f);
+        float4 F = A.bforce[o], T = A.btorque[o];
+        A.bforce[o] = f4(ld3(F) + f, 0.0f);
+        A.btorque[o] = f4(ld3(T) + tq, 0.0f);
+      }
+    }
+  }
+  return dv;
+}
+
+// warm_start_divergence_solve_0 (runner.hpp:3230-3250)
+__global__ __launch_bounds__(256) void k_ws_div0(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  float dadv = density_change(p, A, s);
+  A.kappa_v[s] = (dadv > 0.0f) ? 0.5f * std_max(A.kappa_v[s], -0.5f) / dt : 0.0f;
+}
+// warm_start_divergence_solve_1 (runner.hpp:3252-3295)
+__global__ __launch_bounds__(256) void k_ws_div1(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  vec3 dv = stiffness_dv<true>(p, A, s, A.kappa_v[s], A.kappa_v, dt);
+  A.v[s] = f4(ld3(A.v[s]) + dv, 0.0f);
+}
+// compute_velocity_of_density_change (runner.hpp:3297-3314); also stores the
+// first iteration's stiffness k = rho_adv * alpha
+__global__ __launch_bounds__(256) void k_vel_density_change(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  float dadv = density_change(p, A, s);
+  A.rho_adv[s] = dadv;
+  float inv_dt = 1.0f / dt;
+  float alpha = A.alpha[s] * inv_dt;
+  A.alpha[s] = alpha;
+  A.kfac[s] = dadv * alpha;
+}
+// divergence_solve_iteration (runner.hpp:3316-3363)
+__global__ __launch_bounds__(256) void k_div_iter(SimParams p, DevArrays A) {
+  if (A.ctl->div_done) return;
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  const float k_i = A.kfac[s];
+  A.kappa_v[s] += k_i;
+  vec3 dv = stiffness_dv<true>(p, A, s, k_i, A.kfac, dt);
+  A.v[s] = f4(ld3(A.v[s]) + dv, 0.0f);
+}
+// compute_divergence_solve_density_error (runner.hpp:3396-3410) + FP64
+// partial sums of the residual
+__global__ __launch_bounds__(256) void k_div_err(SimParams p, DevArrays A) {
+  __shared__ double lds[4];
+  if (A.ctl->div_done) return;
+  const unsigned s = ALV_TID();
+  double part = 0.0;
+  if (s < p.n) {
+    float dadv = density_change(p, A, s);
+    A.rho_adv[s] = dadv;
+    A.kfac[s] = dadv * A.alpha[s];
+    part = double(dadv);
+  }
+  double t = block_sum_d(part, lds);
+  if (threadIdx.x == 0) A.partials[xcd_block()] = t;
+}
+__global__ __launch_bounds__(256) void k_div_finalize(SimParams p, DevArrays A,
+                                                      unsigned nblocks) {
+  __shared__ double lds[4];
+  StepCtl* ctl = A.ctl;
+  if (ctl->div_done) return;
+  double part = 0.0;
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) part += A.partials[b];
+  double t = block_sum_d(part, lds);
+  if (threadIdx.x == 0) {
+    float mean = float(t / double(p.n));
+    unsigned it = ctl->div_iter + 1u;
+    ctl->div_iter = it;
+    ctl->mean_density_change = mean;
+    bool again = (mean > ctl->tol_div / ctl->dt || it < ctl->min_div) && it < ctl->max_div;
+    ctl->div_done = again ? 0u : 1u;
+  }
+}
+
+// Control block reset before the solves of a step.
+__global__ void k_ctl_reset(DevArrays A) {
+  StepCtl* ctl = A.ctl;
+  ctl->div_iter = 0u;
+  ctl->dens_iter = 0u;
+  const float mx = FLT_MAX;
+  bool div_go = (mx > ctl->tol_div / ctl->dt || 0u < ctl->min_div) && 0u < ctl->max_div;
+  ctl->div_done = div_go ? 0u : 1u;
+  ctl->mean_density_change = mx;
+  ctl->mean_density_error = mx;
+  ctl->dens_done = 0u;  // decided after update_dt (tolerance does not use dt)
+  bool dens_go = (mx > ctl->tol_dens || 0u < ctl->min_dens) && 0u < ctl->max_dens;
+  ctl->dens_done = dens_go ? 0u : 1u;
+}
+
+// ---------------------------------------------------------------------------
+// Viscosity (compute_viscosity, runner.hpp:2150-2249) with clear_acceleration
+// (a = g, runner.hpp:1974-1980), optional divergence_solve_finish
+// (runner.hpp:3412-3420) and the per-block maxima of calculate_cfl_v2
+// (runner.hpp:2533-2541).
+// ---------------------------------------------------------------------------
+__device__ inline void orthogonal_vectors(vec3 n, vec3& t1, vec3& t2) {
+  vec3 v = mk(1.f, 0.f, 0.f);
+  if (fabsf(dot(v, n)) > 0.999f) {
+    v.x = 0.f;
+    v.y = 1.f;
+  }
+  t1 = cross(n, v);
+  t2 = cross(n, t1);
+  t1 = t1 * (1.0f / sqrtf(length_sqr(t1)));
+  t2 = t2 * (1.0f / sqrtf(length_sqr(t2)));
+}
+
+__global__ __launch_bounds__(256) void k_viscosity(SimParams p, DevArrays A, int div_finish) {
+  __shared__ float lds[4];
+  const unsigned s = ALV_TID();
+  const float dt = A.ctl->dt;
+  float cfl = -FLT_MAX;
+  if (s < p.n) {
+    const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
+    const unsigned nn = A.nnb[s];
+    vec3 da = mk(0.f, 0.f, 0.f);
+    for (unsigned t = 0; t < nn; ++t) {
+      const unsigned j = A.nbr[size_t(t) * A.cap + s];
+      vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
+      da += (p.visc * p.mass / A.density[j] * dot(xixj, gradW(xixj, p)) /
+             (length_sqr(xixj) + 0.01f * p.h2)) *
+            (vi - ld3(A.v[j]));
+    }
+    unsigned mask = A.bmask[s];
+    while (mask) {
+      const unsigned b = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      const size_t o = size_t(b) * A.cap + s;
+      const BoundaryDev& B = A.bnd[b];
+      float4 rec = A.brec[o];
+      vec3 bx = ld3(rec);
+      vec3 r_x = mk(B.x[0], B.x[1], B.x[2]);
+      vec3 normal = bx - xi;
+      float nl2 = length_sqr(normal);
+      if (nl2 > 0.0f) {
+        normal = normal * (1.0f / sqrtf(nl2));
+        vec3 t1, t2;
+        orthogonal_vectors(normal, t1, t2);
+        float dist = p.h - sqrtf(nl2);
+        vec3 xs[4] = {bx - t1 * dist, bx + t1 * dist, bx - t2 * dist, bx + t2 * dist};
+        float vol = 0.25f * rec.w;
+        vec3 as[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          vec3 xix = xi - xs[q];
+          vec3 gw = gradW(xix, p);
+          vec3 vq = rigid_velocity(B, xs[q]);
+          as[q] = (p.bvisc * vol * dot(xix, gw) / (length_sqr(xix) + 0.01f * p.h2)) * (vi - vq);
+        }
+        da += as[0] + as[1] + as[2] + as[3];
+        if (B.dynamic) {
+          vec3 f[4], tq[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            f[q] = (-p.mass) * as[q];
+            tq[q] = cross(xs[q] - r_x, f[q]);
+          }
+          float4 F = A.bforce[o], T = A.btorque[o];
+          A.bforce[o] = f4(ld3(F) + (f[0] + f[1] + f[2] + f[3]), 0.f);
+          A.btorque[o] = f4(ld3(T) + (tq[0] + tq[1] + tq[2] + tq[3]), 0.f);
+        }
+      }
+    }
+    vec3 a = mk(p.g[0], p.g[1], p.g[2]) + da;
+    A.a[s] = f4(a, 0.f);
+    cfl = length_sqr(vi + a * dt);
+    if (div_finish) {
+      A.alpha[s] *= dt;
+      A.kappa_v[s] *= dt;
+    }
+  }
+  float m = block_max_f(cfl, lds);
+  if (threadIdx.x == 0) A.fpartials[xcd_block()] = m;
+}
+
+// Solver::update_dt (solver.hpp:153-169) on the device.
+__global__ __launch_bounds__(256) void k_update_dt(SimParams p, DevArrays A, unsigned nblocks) {
+  __shared__ float lds[4];
+  float m = -FLT_MAX;
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) m = std_max(m, A.fpartials[b]);
+  float pm = block_max_f(m, lds);
+  if (threadIdx.x == 0) {
+    StepCtl* ctl = A.ctl;
+    float pile_max = -FLT_MAX;
+    for (unsigned b = 0; b < p.num_boundaries; ++b) {
+      const BoundaryDev& B = A.bnd[b];
+      float v2 = length_sqr(cross(mk(B.omega[0], B.omega[1], B.omega[2]),
+                                  mk(B.max_distance, 0.f, 0.f)) +
+                            mk(B.v[0], B.v[1], B.v[2]));
+      if (v2 > pile_max) pile_max = v2;
+    }
+    float max_v2 = std_max(pm, pile_max);
+    float max_v = sqrtf(max_v2);
+    float ls = ctl->length_scale;
+    float cfl_dt = ctl->cfl * ls / max_v;
+    float dt = std_max(std_min(cfl_dt, ctl->max_dt), ctl->min_dt);
+    ctl->particle_max_v2 = pm;
+    ctl->pile_max_v2 = pile_max;
+    ctl->max_v2 = max_v2;
+    ctl->cfl_dt = cfl_dt;
+    ctl->dt = dt;
+    ctl->utilized_cfl = dt * max_v / ls;
+  }
+}
+
+// integrate_non_pressure_acceleration (runner.hpp:3422-3428)
+__global__ __launch_bounds__(256) void k_integrate_np(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  A.v[s] = f4(ld3(A.v[s]) + dt * ld3(A.a[s]), 0.f);
+}
+
+// warm_start_pressure_solve0 (runner.hpp:3430-3450)
+__global__ __launch_bounds__(256) void k_ws_p0(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  float dadv = density_adv(p, A, s, dt);
+  A.kappa[s] = (dadv > 1.0f) ? (0.5f * std_max(A.kappa[s], -0.00025f) / (dt * dt)) : 0.0f;
+}
+// warm_start_pressure_solve1 (runner.hpp:3452-3491)
+__global__ __launch_bounds__(256) void k_ws_p1(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  vec3 dv = stiffness_dv<false>(p, A, s, A.kappa[s], A.kappa, dt);
+  A.v[s] = f4(ld3(A.v[s]) + dv, 0.0f);
+}
+// compute_rho_adv (runner.hpp:3525-3541); stores k = (rho_adv - 1) * alpha
+__global__ __launch_bounds__(256) void k_rho_adv(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  float dadv = density_adv(p, A, s, dt);
+  A.rho_adv[s] = dadv;
+  float alpha = A.alpha[s] * (1.0f / (dt * dt));
+  A.alpha[s] = alpha;
+  A.kfac[s] = (dadv - 1.0f) * alpha;
+}
+// pressure_solve_iteration (runner.hpp:3543-3588)
+__global__ __launch_bounds__(256) void k_p_iter(SimParams p, DevArrays A) {
+  if (A.ctl->dens_done) return;
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  const float k_i = A.kfac[s];
+  A.kappa[s] += k_i;
+  vec3 dv = stiffness_dv<false>(p, A, s, k_i, A.kfac, dt);
+  A.v[s] = f4(ld3(A.v[s]) + dv, 0.0f);
+}
+// compute_pressure_solve_density_error (runner.hpp:3590-3604) + FP64
+// partial sums of (rho_adv - 1)
+__global__ __launch_bounds__(256) void k_p_err(SimParams p, DevArrays A) {
+  __shared__ double lds[4];
+  if (A.ctl->dens_done) return;
+  const unsigned s = ALV_TID();
+  const float dt = A.ctl->dt;
+  double part = 0.0;
+  if (s < p.n) {
+    float dadv = density_adv(p, A, s, dt);
+    A.rho_adv[s] = dadv;
+    A.kfac[s] = (dadv - 1.0f) * A.alpha[s];
+    part = double(dadv - 1.0f);
+  }
+  double t = block_sum_d(part, lds);
+  if (threadIdx.x == 0) A.partials[xcd_block()] = t;
+}
+__global__ __launch_bounds__(256) void k_p_finalize(SimParams p, DevArrays A, unsigned nblocks) {
+  __shared__ double lds[4];
+  StepCtl* ctl = A.ctl;
+  if (ctl->dens_done) return;
+  double part = 0.0;
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) part += A.partials[b];
+  double t = block_sum_d(part, lds);
+  if (threadIdx.x == 0) {
+    float mean = float(t / double(p.n));
+    unsigned it = ctl->dens_iter + 1u;
+    ctl->dens_iter = it;
+    ctl->mean_density_error = mean;
+    bool again = (mean > ctl->tol_dens || it < ctl->min_dens) && it < ctl->max_dens;
+    ctl->dens_done = again ? 0u : 1u;
+  }
+}
+
+// pressure_solve_finish + integrate_velocity<wrap> (runner.hpp:3606-3623),
+// then the state goes back to user order.
+__global__ __launch_bounds__(256) void k_df_finish(SimParams p, DevArrays A, int dens_finish) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  const unsigned i = A.sorted_id[s];
+  vec3 v = ld3(A.v[s]);
+  vec3 x = wrap_disp(ld3(A.x[s]) + dt * v, p);
+  float kappa = A.kappa[s];
+  if (dens_finish) kappa *= dt * dt;
+  st3(A.ux, i, x);
+  st3(A.uv, i, v);
+  A.ukappa[i] = kappa;
+  A.ukappa_v[i] = A.kappa_v[s];
+}
+
+// ---------------------------------------------------------------------------
+// IISPH (runner.hpp:2609-2887)
+// ---------------------------------------------------------------------------
+// predict_advection0 (runner.hpp:2609-2639)
+__global__ __launch_bounds__(256) void k_ii_predict0(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  A.v[s] = f4(ld3(A.v[s]) + dt * ld3(A.a[s]), 0.f);
+  A.last_pressure[s] = A.pressure[s] * 0.5f;
+  const vec3 xi = ld3(A.x[s]);
+  const float dens = A.density[s];
+  const float dens2 = dens * dens;
+  vec3 d = mk(0.f, 0.f, 0.f);
+  const unsigned nn = A.nnb[s];
+  for (unsigned t = 0; t < nn; ++t) {
+    const unsigned j = A.nbr[size_t(t) * A.cap + s];
+    vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
+    d -= (p.vol / dens2) * gradW(xixj, p);
+  }
+  unsigned mask = A.bmask[s];
+  while (mask) {
+    const unsigned b = __builtin_ctz(mask);
+    mask &= mask - 1u;
+    float4 k = A.bker[size_t(b) * A.cap + s];
+    d -= mk(k.x / dens2, k.y / dens2, k.z / dens2);
+  }
+  A.dii[s] = f4(d * p.rho0, 0.f);
+}
+// predict_advection1 (runner.hpp:2640-2690)
+__global__ __launch_bounds__(256) void k_ii_predict1(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]), d_ii = ld3(A.dii[s]);
+  const float dens = A.density[s];
+  const float dens2 = dens * dens;
+  const float inv_dens2 = 1.0f / dens2;
+  const float dpi = p.vol * inv_dens2;
+  float dadv = dens / p.rho0;
+  float a_ii = 0.f;
+  const unsigned nn = A.nnb[s];
+  for (unsigned t = 0; t < nn; ++t) {
+    const unsigned j = A.nbr[size_t(t) * A.cap + s];
+    vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
+    vec3 gw = gradW(xixj, p);
+    vec3 dji = (dpi * p.rho0) * gw;
+    dadv += dt * p.vol * dot(vi - ld3(A.v[j]), gw);
+    a_ii += p.vol * dot(d_ii - dji, gw);
+  }
+  unsigned mask = A.bmask[s];
+  while (mask) {
+    const unsigned b = __builtin_ctz(mask);
+    mask &= mask - 1u;
+    const size_t o = size_t(b) * A.cap + s;
+    float4 r = A.brec[o], k = A.bker[o];
+    vec3 g = mk(k.x, k.y, k.z);
+    vec3 velj = rigid_velocity(A.bnd[b], ld3(r));
+    vec3 dji = (inv_dens2 * p.rho0) * g;
+    dadv += dt * dot(vi - velj, g);
+    a_ii += dot(d_ii - dji, g);
+  }
+  A.adv_density[s] = dadv;
+  A.aii[s] = a_ii * p.rho0;
+}
+// pressure_solve_iteration0 (runner.hpp:2691-2715).  The previous
+// iteration's copy last_pressure <- pressure (solver_ii.hpp:204) is folded in:
+// from the second iteration on, neighbour values are read from `pressure`
+// and each particle refreshes its own last_pressure.
+__global__ __launch_bounds__(256) void k_ii_iter0(SimParams p, DevArrays A) {
+  if (A.ctl->dens_done) return;
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float* lp = (A.ctl->dens_iter == 0u) ? A.last_pressure : A.pressure;
+  const vec3 xi = ld3(A.x[s]);
+  vec3 acc = mk(0.f, 0.f, 0.f);
+  const unsigned nn = A.nnb[s];
+  for (unsigned t = 0; t < nn; ++t) {
+    const unsigned j = A.nbr[size_t(t) * A.cap + s];
+    vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
+    float dj = A.density[j];
+    float dj2 = dj * dj;
+    acc -= (p.vol / dj2 * lp[j]) * gradW(xixj, p);
+  }
+  A.dij_pj[s] = f4(acc * p.rho0, 0.f);
+  A.last_pressure[s] = lp[s];
+}
+// pressure_solve_iteration1 (runner.hpp:2716-2772) + FP64 partial sums
+__global__ __launch_bounds__(256) void k_ii_iter1(SimParams p, DevArrays A) {
+  __shared__ double lds[4];
+  if (A.ctl->dens_done) return;
+  const unsigned s = ALV_TID();
+  const float dt = A.ctl->dt;
+  double part = 0.0;
+  if (s < p.n) {
+    const vec3 xi = ld3(A.x[s]);
+    const float lp = A.last_pressure[s];
+    const vec3 dpj = ld3(A.dij_pj[s]);
+    const float dens = A.density[s];
+    const float dens2 = dens * dens;
+    const float dpi = p.vol / dens2;
+    const float bb = 1.0f - A.adv_density[s];
+    const float dt2 = dt * dt;
+    const float a_ii = A.aii[s];
+    const float denom = a_ii * dt2;
+    const float w = 2.0f / 3.0f;
+    float pr = 0.f;
+    float sum_tmp = 0.f;
+    const unsigned nn = A.nnb[s];
+    for (unsigned t = 0; t < nn; ++t) {
+      const unsigned j = A.nbr[size_t(t) * A.cap + s];
+      vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
+      vec3 djk_pk = ld3(A.dij_pj[j]);
+      vec3 gw = gradW(xixj, p);
+      vec3 dji = (dpi * p.rho0) * gw;
+      vec3 dji_pi = dji * lp;
+      sum_tmp += p.vol * dot(dpj - ld3(A.dii[j]) * A.last_pressure[j] - (djk_pk - dji_pi), gw);
+    }
+    unsigned mask = A.bmask[s];
+    while (mask) {
+      const unsigned b = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      float4 k = A.bker[size_t(b) * A.cap + s];
+      sum_tmp += dot(dpj, mk(k.x, k.y, k.z));
+    }
+    sum_tmp *= p.rho0;
+    if (fabsf(denom) > 1.0e-9f)
+      pr = std_max((1.0f - w) * lp + w / denom * (bb - dt2 * sum_tmp), 0.0f);
+    float err = pr == 0.f ? 0.f : (denom * pr + sum_tmp * dt2 - bb);
+    A.density_err[s] = err;
+    A.pressure[s] = pr;
+    part = double(err);
+  }
+  double t = block_sum_d(part, lds);
+  if (threadIdx.x == 0) A.partials[xcd_block()] = t;
+}
+__global__ __launch_bounds__(256) void k_ii_finalize(SimParams p, DevArrays A, unsigned nblocks) {
+  __shared__ double lds[4];
+  StepCtl* ctl = A.ctl;
+  if (ctl->dens_done) return;
+  double part = 0.0;
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) part += A.partials[b];
+  double t = block_sum_d(part, lds);
+  if (threadIdx.x == 0) {
+    float mean = float(t / double(p.n));
+    unsigned it = ctl->dens_iter + 1u;
+    ctl->dens_iter = it;
+    ctl->mean_density_error = mean;
+    bool again = (mean > ctl->tol_dens || it < ctl->min_dens) && it < ctl->max_dens;
+    ctl->dens_done = again ? 0u : 1u;
+  }
+}
+// compute_pressure_accels (runner.hpp:2774-2817)
+__global__ __launch_bounds__(256) void k_ii_pressure_accel(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const vec3 xi = ld3(A.x[s]);
+  const float dens = A.density[s];
+  const float dens2 = dens * dens;
+  const float dpi = A.pressure[s] / dens2;
+  vec3 ai = mk(0.f, 0.f, 0.f);
+  const unsigned nn = A.nnb[s];
+  for (unsigned t = 0; t < nn; ++t) {
+    const unsigned j = A.nbr[size_t(t) * A.cap + s];
+    vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
+    float dj = A.density[j];
+    float dj2 = dj * dj;
+    float dpj = A.pressure[j] / dj2;
+    ai -= (p.mass * (dpi + dpj)) * gradW(xixj, p);
+  }
+  unsigned mask = A.bmask[s];
+  while (mask) {
+    const unsigned b = __builtin_ctz(mask);
+    mask &= mask - 1u;
+    const size_t o = size_t(b) * A.cap + s;
+    float4 k = A.bker[o];
+    vec3 acc = (p.rho0 * dpi) * mk(k.x, k.y, k.z);
+    ai -= acc;
+    const BoundaryDev& B = A.bnd[b];
+    if (B.dynamic) {
+      vec3 f = p.mass * acc;
+      vec3 tq = cross(ld3(A.brec[o]) - mk(B.x[0], B.x[1], B.x[2]), f);
+      float4 F = A.bforce[o], T = A.btorque[o];
+      A.bforce[o] = f4(ld3(F) + f, 0.f);
+      A.btorque[o] = f4(ld3(T) + tq, 0.f);
+    }
+  }
+  A.pacc[s] = f4(p.rho0 * ai, 0.f);
+}
+// kinematic_integration<wrap> (runner.hpp:2873-2887) + back to user order
+__global__ __launch_bounds__(256) void k_ii_finish(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const float dt = A.ctl->dt;
+  const unsigned i = A.sorted_id[s];
+  vec3 v = ld3(A.v[s]);
+  v += ld3(A.pacc[s]) * dt;
+  vec3 x = wrap_disp(ld3(A.x[s]) + v * dt, p);
+  st3(A.ux, i, x);
+  st3(A.uv, i, v);
+  A.upressure[i] = A.pressure[s];
+}
+
+// ---------------------------------------------------------------------------
+// Rigid-body sums (solver_df.hpp:347-353): FP64, one block per boundary.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_body_sums(SimParams p, DevArrays A) {
+  __shared__ double lds[4];
+  const unsigned b = blockIdx.x;
+  if (!A.bnd[b].dynamic) return;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (unsigned s = threadIdx.x; s < p.n; s += blockDim.x) {
+    const size_t o = size_t(b) * A.cap + s;
+    float4 F = A.bforce[o], T = A.btorque[o];
+    acc[0] += F.x;
+    acc[1] += F.y;
+    acc[2] += F.z;
+    acc[3] += T.x;
+    acc[4] += T.y;
+    acc[5] += T.z;
+  }
+  for (int c = 0; c < 6; ++c) {
+    double t = block_sum_d(acc[c], lds);
+    if (threadIdx.x == 0) A.body_sums[b * 6 + c] = t;
+    __syncthreads();
+  }
+}
+
+// sorted -> user order for the API arrays that are synchronised lazily
+__global__ __launch_bounds__(256) void k_unsort(SimParams p, DevArrays A, unsigned mask) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const unsigned i = A.sorted_id[s];
+  if (mask & ALV_F_DENSITY) A.udensity[i] = A.density[s];
+  if (mask & ALV_F_ALPHA) A.ualpha[i] = A.alpha[s];
+  if (mask & ALV_F_RHO_ADV) A.urho_adv[i] = A.rho_adv[s];
+  if (mask & ALV_F_A) st3(A.ua, i, ld3(A.a[s]));
+  if (mask & ALV_F_NNB) A.unnb[i] = A.nnb[s];
+}
+
+__global__ __launch_bounds__(256) void k_export_neighbors(SimParams p, DevArrays A,
+                                                          unsigned* ids, unsigned* counts) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  const unsigned i = A.sorted_id[s];
+  const unsigned nn = A.nnb[s];
+  counts[i] = nn;
+  for (unsigned t = 0; t < nn; ++t)
+    ids[size_t(i) * p.kmax + t] = A.sorted_id[A.nbr[size_t(t) * A.cap + s]];
+}
+
+// ---------------------------------------------------------------------------
+// Volume maps (update_volume_field, runner.hpp:1616-1694).  Peirce constants
+// are packed as cosphi[30], sinphi[30], b[30], r[15], c[15].
+// ---------------------------------------------------------------------------
+constexpr float kPiF = 3.14159265358979323846f;
+
+__global__ __launch_bounds__(256) void k_volume_analytic(SimParams p, BoundaryDev B,
+                                                         float* volume, const float* pc,
+                                                         unsigned begin, unsigned end) {
+  const unsigned l = begin + blockIdx.x * kBlock + threadIdx.x;
+  if (l >= end) return;
+  const float* cosphi = pc;
+  const float* sinphi = pc + 30;
+  const float* kb = pc + 60;
+  const float* kr = pc + 90;
+  const float* kc = pc + 105;
+  vec3 x = node_position(B.geom, l);
+  float sum = 0.0f;
+  for (unsigned i = 0; i < 60; ++i) {
+    float theta = 2.0f * kPiF * float(i + 1) / 60.0f;
+    float ct = cosf(theta), st = sinf(theta);
+    for (unsigned j = 0; j < 30; ++j) {
+      for (unsigned k = 0; k < 15; ++k) {
+        float wijk = 2.0f * kPiF * kb[j] * kc[k] / 60.0f;
+        vec3 u = mk(sinphi[j] * ct, sinphi[j] * st, cosphi[j]);
+        vec3 q = (p.h * kr[k]) * u;
+        float dist = sdf_eval(B, x + q);
+        float dm = dist * B.sign;
+        sum += wijk * (dm <= 0.0f ? 1.0f : W_dist(dm, p) / p.w0);
+      }
+    }
+  }
+  volume[l] = sum * p.h2 * p.h;
+}
+
+__global__ __launch_bounds__(256) void k_volume_grid(SimParams p, GridGeomDev g,
+                                                     const float* __restrict__ dist,
+                                                     float* volume, float sign,
+                                                     const float* pc, unsigned begin,
+                                                     unsigned end) {
+  const unsigned l = begin + blockIdx.x * kBlock + threadIdx.x;
+  if (l >= end) return;
+  const float* cosphi = pc;
+  const float* sinphi = pc + 30;
+  const float* kb = pc + 60;
+  const float* kr = pc + 90;
+  const float* kc = pc + 105;
+  vec3 x = node_position(g, l);
+  float sum = 0.0f;
+  const float wi = 2.0f * kPiF / 60.0f;
+  for (unsigned i = 0; i < 60; ++i) {
+    float theta = 2.0f * kPiF * float(i + 1) / 60.0f;
+    float ct = cosf(theta), st = sinf(theta);
+    for (unsigned j = 0; j < 30; ++j) {
+      vec3 u = mk(sinphi[j] * ct, sinphi[j] * st, cosphi[j]);
+      float wij = wi * kb[j];
+      for (unsigned k = 0; k < 15; ++k) {
+        float wijk = wij * kc[k];
+        vec3 q = (p.h * kr[k]) * u;
+        float d = grid_sample(g, dist, x + q, FLT_MAX);
+        float dm = d * sign;
+        sum += wijk * (dm <= 0.0f ? 1.0f : W_dist(dm, p) / p.w0);
+      }
+    }
+  }
+  volume[l] = sum * p.h2 * p.h;
+}
+
+__global__ __launch_bounds__(256) void k_node_positions(GridGeomDev g, float* out) {
+  const unsigned l = blockIdx.x * kBlock + threadIdx.x;
+  if (l >= g.num_nodes) return;
+  vec3 x = node_position(g, l);
+  out[3 * l] = x.x;
+  out[3 * l + 1] = x.y;
+  out[3 * l + 2] = x.z;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+KernelProfiler* g_profiler = nullptr;
+
+#define ALV_LAUNCH_GRID(kernel, grid, s, ...)               \
+  do {                                                      \
+    if (g_profiler) g_profiler->begin(#kernel, s);          \
+    hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, s, __VA_ARGS__); \
+    if (g_profiler) g_profiler->end(#kernel, s);            \
+  } while (0)
+#define ALV_LAUNCH(kernel, n, s, ...) ALV_LAUNCH_GRID(kernel, dim3(grid_for(n)), s, __VA_ARGS__)
+#define ALV_LAUNCH1(kernel, s, ...) ALV_LAUNCH_GRID(kernel, dim3(1), s, __VA_ARGS__)
+
+void launch_cell_count(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_cell_count, p.n, s, p, A);
+}
+void launch_scan(unsigned count, const unsigned* in, unsigned* out, unsigned* tmp,
+                 hipStream_t s) {
+  const unsigned nb = (count + kScanPerBlock - 1) / kScanPerBlock;
+  ALV_LAUNCH_GRID(k_scan_reduce, dim3(nb), s, count, in, tmp);
+  ALV_LAUNCH1(k_scan_top, s, nb, tmp);
+  ALV_LAUNCH_GRID(k_scan_apply, dim3(nb), s, count, in, out, (const unsigned*)tmp);
+}
+void launch_scatter(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_scatter, p.n, s, p, A);
+}
+void launch_cell_sort(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_cell_sort, p.num_cells + 1, s, p, A);
+}
+void launch_gather_boundary(const SimParams& p, const DevArrays& A, int kind, hipStream_t s) {
+  ALV_LAUNCH(k_gather_boundary, p.n, s, p, A, kind);
+}
+void launch_neighbors(const SimParams& p, const DevArrays& A, int wrap_any, int with_factor,
+                      hipStream_t s) {
+  (void)wrap_any;
+  ALV_LAUNCH(k_neighbors, p.n, s, p, A, with_factor);
+}
+void launch_df_warm_start_div(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_ws_div0, p.n, s, p, A);
+  ALV_LAUNCH(k_ws_div1, p.n, s, p, A);
+  ALV_LAUNCH(k_vel_density_change, p.n, s, p, A);
+}
+void launch_df_div_loop_prep(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  (void)p;
+  (void)A;
+  (void)s;
+}
+void launch_df_div_iteration(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_div_iter, p.n, s, p, A);
+  ALV_LAUNCH(k_div_err, p.n, s, p, A);
+  ALV_LAUNCH1(k_div_finalize, s, p, A, grid_for(p.n));
+}
+void launch_df_viscosity(const SimParams& p, const DevArrays& A, int div_finish,
+                         hipStream_t s) {
+  ALV_LAUNCH(k_viscosity, p.n, s, p, A, div_finish);
+}
+void launch_update_dt(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH1(k_update_dt, s, p, A, grid_for(p.n));
+}
+void launch_df_integrate_np(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_integrate_np, p.n, s, p, A);
+}
+void launch_df_warm_start_dens(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_ws_p0, p.n, s, p, A);
+  ALV_LAUNCH(k_ws_p1, p.n, s, p, A);
+  ALV_LAUNCH(k_rho_adv, p.n, s, p, A);
+}
+void launch_df_dens_iteration(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_p_iter, p.n, s, p, A);
+  ALV_LAUNCH(k_p_err, p.n, s, p, A);
+  ALV_LAUNCH1(k_p_finalize, s, p, A, grid_for(p.n));
+}
+void launch_df_finish(const SimParams& p, const DevArrays& A, int dens_finish, hipStream_t s) {
+  ALV_LAUNCH(k_df_finish, p.n, s, p, A, dens_finish);
+}
+void launch_ii_viscosity_predict(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_viscosity, p.n, s, p, A, 0);
+}
+void launch_ii_predict(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_ii_predict0, p.n, s, p, A);
+  ALV_LAUNCH(k_ii_predict1, p.n, s, p, A);
+}
+void launch_ii_iteration(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_ii_iter0, p.n, s, p, A);
+  ALV_LAUNCH(k_ii_iter1, p.n, s, p, A);
+  ALV_LAUNCH1(k_ii_finalize, s, p, A, grid_for(p.n));
+}
+void launch_ii_finish(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_ii_pressure_accel, p.n, s, p, A);
+  ALV_LAUNCH(k_ii_finish, p.n, s, p, A);
+}
+void launch_body_sums(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  if (p.num_boundaries == 0) return;
+  ALV_LAUNCH_GRID(k_body_sums, dim3(p.num_boundaries), s, p, A);
+}
+void launch_unsort_fields(const SimParams& p, const DevArrays& A, unsigned mask,
+                          hipStream_t s) {
+  ALV_LAUNCH(k_unsort, p.n, s, p, A, mask);
+}
+void launch_export_neighbors(const SimParams& p, const DevArrays& A, unsigned* ids,
+                             unsigned* counts, hipStream_t s) {
+  ALV_LAUNCH(k_export_neighbors, p.n, s, p, A, ids, counts);
+}
+void launch_ctl_reset(const DevArrays& A, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, A);
+}
+void launch_volume_field_analytic(const SimParams& p, const BoundaryDev& b, float* volume,
+                                  const float* peirce, unsigned begin, unsigned end,
+                                  hipStream_t s) {
+  if (end <= begin) return;
+  hipLaunchKernelGGL(k_volume_analytic, dim3(grid_for(end - begin)), dim3(kBlock), 0, s, p, b,
+                     volume, peirce, begin, end);
+}
+void launch_volume_field_grid(const SimParams& p, const GridGeomDev& g, const float* distance,
+                              float* volume, float sign, const float* peirce, unsigned begin,
+                              unsigned end, hipStream_t s) {
+  if (end <= begin) return;
+  hipLaunchKernelGGL(k_volume_grid, dim3(grid_for(end - begin)), dim3(kBlock), 0, s, p, g,
+                     distance, volume, sign, peirce, begin, end);
+}
+void launch_node_positions(const GridGeomDev& g, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_node_positions, dim3(grid_for(g.num_nodes)), dim3(kBlock), 0, s, g, out);
+}
+
+}  // namespace alv

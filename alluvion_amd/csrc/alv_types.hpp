@@ -1,0 +1,69 @@
+// alv_types.hpp -- plain-old-data blocks shared by the host runtime and the
+// HIP kernels.  SimParams replaces the reference's process-global
+// __constant__ Const<TF>/Consti (constants.hpp:40-143): it is built per
+// solver on the host and passed to every kernel by value, so several solvers
+// (or scenes) with different constants can coexist in one process.
+#pragma once
+
+#include <stdint.h>
+
+namespace alv {
+
+enum { ALV_DIST_BOX = 0, ALV_DIST_BOX_SHELL = 1, ALV_DIST_SPHERE = 2 };
+
+struct SimParams {
+  // kernel constants (set_kernel_radius, constants.hpp:100-120)
+  float h, h2, k, l, w0;
+  // particle attributes (set_particle_attr, constants.hpp:122-128)
+  float radius, vol, mass, rho0;
+  float visc, bvisc, eps;
+  float g[3];
+  // periodic wrap (set_wrap_length, constants.hpp:130-136); bit a = axis a
+  int wrap_mask;
+  float wrap_len[3], wrap_min[3], wrap_max[3];
+  // neighbour grid (Consti, constants.hpp:40-49)
+  int res[3], off[3];
+  unsigned kmax, cell_cap, num_cells;
+  unsigned num_boundaries;
+  unsigned n;  // particles in this step
+};
+
+// Geometry of one 32-node cubic Lagrange map (Pile lists, pile.hpp:106-112).
+struct GridGeomDev {
+  float dmin[3], dmax[3], cs[3];
+  unsigned res[3];
+  unsigned num_nodes;
+};
+
+// One boundary (container or rigid body) as the kernels see it.
+struct BoundaryDev {
+  int type;
+  float sign;
+  float half[3], half_outer[3];
+  float outset, radius, max_distance;
+  GridGeomDev geom;
+  const float* volume;  // device pointer to the volume map nodes
+  float x[3], v[3], omega[3];
+  float q[4];  // (x, y, z, w)
+  float mass;
+  int dynamic;  // mass != 0: forces are accumulated for it
+};
+
+// Device-resident solver control block: time step, convergence bookkeeping
+// and report scalars.  Kernels read dt from here, so a step needs no host
+// round trip to pick its time step.
+struct StepCtl {
+  float dt;
+  float cfl, min_dt, max_dt;
+  float tol_div;   // density_change_tolerance
+  float tol_dens;  // density_error_tolerance
+  unsigned min_div, max_div, min_dens, max_dens;
+  unsigned div_iter, dens_iter;
+  unsigned div_done, dens_done;
+  float mean_density_change, mean_density_error;
+  float particle_max_v2, pile_max_v2, max_v2, cfl_dt, utilized_cfl;
+  float length_scale;
+  unsigned pad[2];
+};
+
+}  // namespace alv

@@ -1,0 +1,406 @@
+// pyalluvion.cpp -- pybind11 module alluvion_amd._alluvion.
+//
+// Exposes the reference's Python names for the solver path
+// (lib/pyalluvion.cu:1279-1486): Store with create{D}D{type}{n} /
+// remove{...}, Variable{D}D{type}{n}, Runnerfloat, Constfloat, ConstiN,
+// Pilefloat, SolverDffloat, SolverIifloat, dg.{Box,BoxShell,Sphere}Distancefloat,
+// the vector types and the NumericType / VolumeMethod enums.  Everything is
+// a thin layer over alv/alluvion.hpp, which itself calls the C ABI (alv.h).
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <tuple>
+
+#include "alv/alluvion.hpp"
+
+namespace py = pybind11;
+using namespace alluvion;
+
+namespace {
+
+template <typename V>
+void declare_vec(py::module& m, const char* name) {}
+
+template <U D, typename M>
+void declare_variable(py::module& m, py::class_<Store>& store, const std::string& name) {
+  using Var = Variable<D, M>;
+  store.def(("create" + name).c_str(),
+            [](Store& s, std::vector<U> shape) {
+              if (shape.size() != D) throw std::invalid_argument("shape rank mismatch");
+              std::array<U, D> sh;
+              for (U i = 0; i < D; ++i) sh[i] = shape[i];
+              return s.create<D, M>(sh);
+            },
+            py::return_value_policy::take_ownership);
+  store.def(("remove" + name).c_str(), [](Store& s, Var& v) { s.remove<D, M>(v); });
+  py::class_<Var>(m, ("Variable" + name).c_str())
+      .def(py::init<const Var&>())
+      .def_property_readonly("ptr", [](Var const& v) { return reinterpret_cast<uintptr_t>(v.ptr_); })
+      .def("get_bytes",
+           [](Var& v, py::array_t<unsigned char> bytes, U offset) {
+             v.get_bytes(bytes.mutable_data(), U(bytes.size()), offset);
+           },
+           py::arg("bytes"), py::arg("offset") = 0)
+      .def("set_bytes",
+           [](Var& v, py::array_t<unsigned char> bytes, U offset) {
+             v.set_bytes(bytes.data(), U(bytes.size()), offset);
+           },
+           py::arg("bytes"), py::arg("offset") = 0)
+      .def("set_from", py::overload_cast<Var const&, U, U>(&Var::set_from), py::arg("src"),
+           py::arg("num_elements"), py::arg("offset") = 0)
+      .def("set_from", py::overload_cast<Var const&>(&Var::set_from), py::arg("src"))
+      .def("set_zero", &Var::set_zero)
+      .def("set_same", py::overload_cast<int, U, U>(&Var::set_same), py::arg("value"),
+           py::arg("num_elements"), py::arg("offset") = 0)
+      .def("set_same", py::overload_cast<int>(&Var::set_same), py::arg("value"))
+      .def("get_type", &Var::get_type)
+      .def("get_num_primitives_per_element", &Var::get_num_primitives_per_element)
+      .def("get_linear_shape", &Var::get_linear_shape)
+      .def("get_num_primitives", &Var::get_num_primitives)
+      .def("get_num_bytes", &Var::get_num_bytes)
+      .def("get_shape", &Var::get_shape);
+}
+
+template <typename T>
+py::class_<T> vec3_class(py::module& m, const char* name) {
+  return py::class_<T>(m, name)
+      .def(py::init([](decltype(T::x) x, decltype(T::x) y, decltype(T::x) z) { return T{x, y, z}; }),
+           py::arg("x") = 0, py::arg("y") = 0, py::arg("z") = 0)
+      .def_readwrite("x", &T::x)
+      .def_readwrite("y", &T::y)
+      .def_readwrite("z", &T::z)
+      .def("__iter__",
+           [](T const& v) { return py::iter(py::make_tuple(v.x, v.y, v.z)); })
+      .def("__repr__", [name](T const& v) {
+        return std::string(name) + "(" + std::to_string(v.x) + ", " + std::to_string(v.y) + ", " +
+               std::to_string(v.z) + ")";
+      });
+}
+
+// Indexable view of one kinematic quantity of a pile (PinnedVariable x_, v_,
+// q_, omega_ in the reference, pile.hpp:103-110).
+template <typename T>
+struct KinView {
+  Pile<float>* pile;
+  int which;  // 0 x, 1 v, 2 q, 3 omega
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_alluvion, m) {
+  m.doc() = "MI355X (gfx950) SPH solver core with alluvion's Python names.";
+
+  py::enum_<NumericType>(m, "NumericType")
+      .value("f32", NumericType::f32)
+      .value("f64", NumericType::f64)
+      .value("i32", NumericType::i32)
+      .value("u32", NumericType::u32)
+      .value("undefined", NumericType::undefined);
+  py::enum_<VolumeMethod>(m, "VolumeMethod")
+      .value("volume_map", VolumeMethod::volume_map)
+      .value("pellets", VolumeMethod::pellets);
+
+  py::class_<float2>(m, "float2")
+      .def(py::init([](float x, float y) { return float2{x, y}; }), py::arg("x") = 0,
+           py::arg("y") = 0)
+      .def_readwrite("x", &float2::x)
+      .def_readwrite("y", &float2::y);
+  vec3_class<float3>(m, "float3");
+  vec3_class<uint3>(m, "uint3");
+  vec3_class<int3>(m, "int3");
+  py::class_<float4>(m, "float4")
+      .def(py::init([](float x, float y, float z, float w) { return float4{x, y, z, w}; }),
+           py::arg("x") = 0, py::arg("y") = 0, py::arg("z") = 0, py::arg("w") = 0)
+      .def_readwrite("x", &float4::x)
+      .def_readwrite("y", &float4::y)
+      .def_readwrite("z", &float4::z)
+      .def_readwrite("w", &float4::w)
+      .def("__iter__",
+           [](float4 const& v) { return py::iter(py::make_tuple(v.x, v.y, v.z, v.w)); });
+
+  py::class_<Const<float>>(m, "Constfloat")
+      .def(py::init<>())
+      .def("set_kernel_radius", &Const<float>::set_kernel_radius)
+      .def("set_particle_attr", &Const<float>::set_particle_attr)
+      .def("set_wrap_length", &Const<float>::set_wrap_length)
+      .def_readonly("kernel_radius", &Const<float>::kernel_radius)
+      .def_readonly("kernel_radius_sqr", &Const<float>::kernel_radius_sqr)
+      .def_readonly("cubic_kernel_k", &Const<float>::cubic_kernel_k)
+      .def_readonly("cubic_kernel_l", &Const<float>::cubic_kernel_l)
+      .def_readonly("cubic_kernel_zero", &Const<float>::cubic_kernel_zero)
+      .def_readonly("particle_radius", &Const<float>::particle_radius)
+      .def_readonly("particle_vol", &Const<float>::particle_vol)
+      .def_readonly("particle_mass", &Const<float>::particle_mass)
+      .def_readonly("density0", &Const<float>::density0)
+      .def_readonly("wrap_length", &Const<float>::wrap_length)
+      .def_readonly("wrap_min", &Const<float>::wrap_min)
+      .def_readonly("wrap_max", &Const<float>::wrap_max)
+      .def_readwrite("wrap_length_x", &Const<float>::wrap_length_x)
+      .def_readwrite("wrap_length_z", &Const<float>::wrap_length_z)
+      .def_readwrite("viscosity", &Const<float>::viscosity)
+      .def_readwrite("boundary_viscosity", &Const<float>::boundary_viscosity)
+      .def_readwrite("vorticity_coeff", &Const<float>::vorticity_coeff)
+      .def_readwrite("boundary_vorticity_coeff", &Const<float>::boundary_vorticity_coeff)
+      .def_readwrite("inertia_inverse", &Const<float>::inertia_inverse)
+      .def_readwrite("viscosity_omega", &Const<float>::viscosity_omega)
+      .def_readwrite("surface_tension_coeff", &Const<float>::surface_tension_coeff)
+      .def_readwrite("surface_tension_boundary_coeff",
+                     &Const<float>::surface_tension_boundary_coeff)
+      .def_readwrite("gravity", &Const<float>::gravity)
+      .def_readwrite("boundary_epsilon", &Const<float>::boundary_epsilon)
+      .def_readwrite("dfsph_factor_epsilon", &Const<float>::dfsph_factor_epsilon)
+      .def_readwrite("contact_tolerance", &Const<float>::contact_tolerance);
+
+  py::class_<ConstiN>(m, "ConstiN")
+      .def(py::init<>())
+      .def_readonly("max_num_contacts", &ConstiN::max_num_contacts)
+      .def_readwrite("num_boundaries", &ConstiN::num_boundaries)
+      .def_readwrite("max_num_particles_per_cell", &ConstiN::max_num_particles_per_cell)
+      .def_readwrite("grid_res", &ConstiN::grid_res)
+      .def_readwrite("grid_offset", &ConstiN::grid_offset)
+      .def_readwrite("max_num_neighbors_per_particle", &ConstiN::max_num_neighbors_per_particle);
+
+  py::class_<Store> store =
+      py::class_<Store>(m, "Store")
+          .def(py::init<>())
+          .def_static("set_device", &Store::set_device)
+          .def("has_display", [](Store const&) { return false; })
+          .def_property_readonly("cnfloat", &Store::get_cn, py::return_value_policy::reference_internal)
+          .def_property_readonly("cni", &Store::get_cni, py::return_value_policy::reference_internal)
+          .def("create_cnfloat", &Store::create_cn)
+          .def("copy_cn_externalfloat", &Store::copy_cn_external)
+          .def("copy_cnfloat", &Store::copy_cn);
+
+  declare_variable<1, float>(m, store, "1Dfloat");
+  declare_variable<1, float2>(m, store, "1Dfloat2");
+  declare_variable<1, float3>(m, store, "1Dfloat3");
+  declare_variable<1, float4>(m, store, "1Dfloat4");
+  declare_variable<2, float>(m, store, "2Dfloat");
+  declare_variable<2, float3>(m, store, "2Dfloat3");
+  declare_variable<2, float4>(m, store, "2Dfloat4");
+  declare_variable<3, float>(m, store, "3Dfloat");
+  declare_variable<4, float>(m, store, "4Dfloat");
+  declare_variable<1, U>(m, store, "1Duint");
+  declare_variable<2, U>(m, store, "2Duint");
+  declare_variable<3, U>(m, store, "3Duint");
+  declare_variable<1, I>(m, store, "1Dint");
+
+  py::class_<Runner<float>>(m, "Runnerfloat")
+      .def(py::init<>())
+      .def_static("sum", [](Variable<1, float> const& v, U n, U off) { return Runner<float>::sum(v, n, off); },
+                  py::arg("var"), py::arg("n"), py::arg("offset") = 0)
+      .def_static("sum", [](Variable<1, float3> const& v, U n, U off) { return Runner<float>::sum(v, n, off); },
+                  py::arg("var"), py::arg("n"), py::arg("offset") = 0)
+      .def_static("max", [](Variable<1, float> const& v, U n, U off) { return Runner<float>::max(v, n, off); },
+                  py::arg("var"), py::arg("n"), py::arg("offset") = 0);
+
+  // Distances are created with the reference policy: Pile::add takes
+  // ownership (pile.hpp:113,303; pyalluvion.cu:612-614).
+  py::module dg = m.def_submodule("dg", "analytic distances");
+  using Dist = dg::Distance<float3, float>;
+  py::class_<Dist>(dg, "Distancefloat")
+      .def_property_readonly("aabb_min", &Dist::get_aabb_min)
+      .def_property_readonly("aabb_max", &Dist::get_aabb_max);
+  py::class_<dg::BoxDistance<float3, float>, Dist>(dg, "BoxDistancefloat")
+      .def_static("create", &dg::BoxDistance<float3, float>::create, py::arg("widths"),
+                  py::arg("outset") = 0, py::return_value_policy::reference);
+  py::class_<dg::BoxShellDistance<float3, float>, Dist>(dg, "BoxShellDistancefloat")
+      .def_static("create", &dg::BoxShellDistance<float3, float>::create,
+                  py::arg("inner_widths"), py::arg("thickness"), py::arg("outset") = 0,
+                  py::return_value_policy::reference);
+  py::class_<dg::SphereDistance<float3, float>, Dist>(dg, "SphereDistancefloat")
+      .def_static("create", &dg::SphereDistance<float3, float>::create, py::arg("radius"),
+                  py::return_value_policy::reference);
+
+  using P = Pile<float>;
+  py::class_<P>(m, "Pilefloat")
+      .def(py::init<Store&, Runner<float>&, U, VolumeMethod, U, Const<float>*, ConstiN*>(),
+           py::arg("store"), py::arg("runner"), py::arg("max_num_contacts"),
+           py::arg("volume_method") = VolumeMethod::volume_map,
+           py::arg("max_num_pellets") = 10000, py::arg("cn") = nullptr, py::arg("cni") = nullptr,
+           py::keep_alive<1, 2>())
+      .def("add",
+           [](P& p, Dist* distance, U3 resolution, float cell_width, float margin, float sign,
+              py::object collision_mesh, float mass, float restitution, float friction,
+              float3 inertia_tensor, float3 x, float4 q, py::object display_mesh) {
+             (void)collision_mesh;
+             (void)display_mesh;
+             return p.add(distance, resolution, cell_width, margin, sign, mass, restitution,
+                          friction, inertia_tensor, x, q);
+           },
+           py::arg("distance"), py::arg("resolution") = U3{0, 0, 0}, py::arg("cell_width") = 0,
+           py::arg("margin") = std::numeric_limits<float>::infinity(), py::arg("sign") = 1,
+           py::arg("collision_mesh") = py::none(), py::arg("mass") = 0,
+           py::arg("restitution") = 1, py::arg("friction") = 0,
+           py::arg("inertia_tensor") = float3{1, 1, 1}, py::arg("x") = float3{0, 0, 0},
+           py::arg("q") = float4{0, 0, 0, 1}, py::arg("display_mesh") = py::none(),
+           py::keep_alive<1, 2>())
+      .def("get_size", &P::get_size)
+      .def_readwrite("mass", &P::mass_)
+      .def_readwrite("restitution", &P::restitution_)
+      .def_readwrite("friction", &P::friction_)
+      .def_readwrite("inertia_tensor", &P::inertia_tensor_)
+      .def("get_x", &P::get_x)
+      .def("get_v", &P::get_v)
+      .def("get_q", &P::get_q)
+      .def("get_omega", &P::get_omega)
+      .def("get_force", &P::get_force)
+      .def("get_torque", &P::get_torque)
+      .def("set_x", &P::set_x)
+      .def("set_v", &P::set_v)
+      .def("set_q", &P::set_q)
+      .def("set_omega", &P::set_omega)
+      .def("set_gravity", &P::set_gravity)
+      .def("integrate_kinematics", &P::integrate_kinematics)
+      .def("copy_kinematics_to_device", &P::copy_kinematics_to_device)
+      .def_property_readonly("volume_grids",
+                             [](P const& p) {
+                               std::vector<Variable<1, float>> out;
+                               for (U i = 0; i < p.get_size(); ++i) out.push_back(p.get_volume_grid(i));
+                               return out;
+                             })
+      .def_property_readonly("domain_min_list",
+                             [](P const& p) {
+                               std::vector<float3> out;
+                               for (U i = 0; i < p.get_size(); ++i) out.push_back(p.get_grid_info(i).domain_min);
+                               return out;
+                             })
+      .def_property_readonly("domain_max_list",
+                             [](P const& p) {
+                               std::vector<float3> out;
+                               for (U i = 0; i < p.get_size(); ++i) out.push_back(p.get_grid_info(i).domain_max);
+                               return out;
+                             })
+      .def_property_readonly("resolution_list",
+                             [](P const& p) {
+                               std::vector<U3> out;
+                               for (U i = 0; i < p.get_size(); ++i) out.push_back(p.get_grid_info(i).resolution);
+                               return out;
+                             })
+      .def_property_readonly("cell_size_list",
+                             [](P const& p) {
+                               std::vector<float3> out;
+                               for (U i = 0; i < p.get_size(); ++i) out.push_back(p.get_grid_info(i).cell_size);
+                               return out;
+                             })
+      .def_property_readonly("sign_list",
+                             [](P const& p) {
+                               std::vector<float> out;
+                               for (U i = 0; i < p.get_size(); ++i) out.push_back(p.get_grid_info(i).sign);
+                               return out;
+                             })
+      .def_property_readonly("grid_size_list", [](P const& p) {
+        std::vector<U> out;
+        for (U i = 0; i < p.get_size(); ++i) out.push_back(p.get_volume_grid(i).get_linear_shape());
+        return out;
+      });
+
+  // Solver classes
+  auto solver_common = [](auto& cls) {
+    using S = typename std::remove_reference<decltype(cls)>::type::type;
+    cls.def_readonly("max_num_particles", &S::max_num_particles)
+        .def_property("num_particles", &S::get_num_particles, &S::set_num_particles)
+        .def_property("t", &S::get_t, &S::set_t)
+        .def_property("dt", &S::get_dt, &S::set_dt)
+        .def_property("initial_dt", &S::get_initial_dt, &S::set_initial_dt)
+        .def_property("max_dt", &S::get_max_dt, &S::set_max_dt)
+        .def_property("min_dt", &S::get_min_dt, &S::set_min_dt)
+        .def_property("cfl", &S::get_cfl, &S::set_cfl)
+        .def_readonly("particle_radius", &S::particle_radius)
+        .def_property_readonly("particle_max_v2", [](S const& s) { return s.report().particle_max_v2; })
+        .def_property_readonly("pile_max_v2", [](S const& s) { return s.report().pile_max_v2; })
+        .def_property_readonly("max_v2", [](S const& s) { return s.report().max_v2; })
+        .def_property_readonly("cfl_dt", [](S const& s) { return s.report().cfl_dt; })
+        .def_property_readonly("utilized_cfl", [](S const& s) { return s.report().utilized_cfl; })
+        .def_property_readonly("particle_x", py::cpp_function(&S::particle_x, py::keep_alive<0, 1>()))
+        .def_property_readonly("particle_v", py::cpp_function(&S::particle_v, py::keep_alive<0, 1>()))
+        .def_property_readonly("particle_a", py::cpp_function(&S::particle_a, py::keep_alive<0, 1>()))
+        .def_property_readonly("particle_density", py::cpp_function(&S::particle_density, py::keep_alive<0, 1>()))
+        .def_property_readonly("particle_num_neighbors", py::cpp_function(&S::particle_num_neighbors, py::keep_alive<0, 1>()))
+        .def_property_readonly("grid_anomaly", py::cpp_function(&S::grid_anomaly, py::keep_alive<0, 1>()))
+        .def_property_readonly("pid_length", py::cpp_function(&S::pid_length, py::keep_alive<0, 1>()))
+        .def("reset_solving_var", &S::reset_solving_var)
+        .def("reset_t", &S::reset_t)
+        .def("compute_all_boundaries", &S::compute_all_boundaries)
+        .def("update_particle_neighbors", &S::template update_particle_neighbors<0>)
+        .def("update_particle_neighbors_wrap1", &S::template update_particle_neighbors<1>)
+        .def("step", &S::template step<0>)
+        .def("step_wrap1", &S::template step<1>)
+        .def("get_neighbors",
+             [](S& s) {
+               std::vector<U> ids, counts;
+               s.get_neighbors(ids, counts);
+               U K = U(ids.size() / std::max<size_t>(1, counts.size()));
+               py::array_t<U> a({size_t(counts.size()), size_t(K)});
+               std::memcpy(a.mutable_data(), ids.data(), ids.size() * sizeof(U));
+               py::array_t<U> c(counts.size());
+               std::memcpy(c.mutable_data(), counts.data(), counts.size() * sizeof(U));
+               return py::make_tuple(a, c);
+             })
+        .def("get_grid", [](S& s) {
+          std::vector<U> cnt, st, an;
+          s.get_grid(cnt, st, an);
+          return py::make_tuple(py::array_t<U>(cnt.size(), cnt.data()),
+                                py::array_t<U>(st.size(), st.data()),
+                                py::array_t<U>(an.size(), an.data()));
+        });
+  };
+
+  // Solver base class (solver.hpp:8-306) carries the common fields.
+  py::class_<SolverBase<float>> base(m, "Solverfloat");
+  solver_common(base);
+
+  using DF = SolverDf<float>;
+  py::class_<DF, SolverBase<float>> df(m, "SolverDffloat");
+  df.def(py::init<Runner<float>&, Pile<float>&, Store&, U, U, bool, bool, Const<float> const*,
+                  ConstiN const*, bool>(),
+         py::arg("runner"), py::arg("pile"), py::arg("store"), py::arg("max_num_particles"),
+         py::arg("num_ushers") = 0, py::arg("enable_surface_tension") = false,
+         py::arg("enable_vorticity") = false, py::arg("cn") = nullptr, py::arg("cni") = nullptr,
+         py::arg("graphical") = false, py::keep_alive<1, 3>())
+      .def_property_readonly("num_divergence_solve",
+                             [](DF const& s) { return s.report().num_divergence_solve; })
+      .def_property_readonly("num_density_solve", [](DF const& s) { return s.report().num_density_solve; })
+      .def_property_readonly("mean_density_change",
+                             [](DF const& s) { return s.report().mean_density_change; })
+      .def_property_readonly("mean_density_error",
+                             [](DF const& s) { return s.report().mean_density_error; })
+      .def_property("enable_divergence_solve", &DF::get_enable_divergence_solve,
+                    &DF::set_enable_divergence_solve)
+      .def_property("enable_density_solve", &DF::get_enable_density_solve,
+                    &DF::set_enable_density_solve)
+      .def_property("density_change_tolerance", &DF::get_density_change_tolerance,
+                    &DF::set_density_change_tolerance)
+      .def_property("density_error_tolerance", &DF::get_density_error_tolerance,
+                    &DF::set_density_error_tolerance)
+      .def_property("min_density_solve", &DF::get_min_density_solve, &DF::set_min_density_solve)
+      .def_property("max_density_solve", &DF::get_max_density_solve, &DF::set_max_density_solve)
+      .def_property("min_divergence_solve", &DF::get_min_divergence_solve,
+                    &DF::set_min_divergence_solve)
+      .def_property("max_divergence_solve", &DF::get_max_divergence_solve,
+                    &DF::set_max_divergence_solve)
+      .def_property_readonly("particle_dfsph_factor", py::cpp_function(&DF::particle_dfsph_factor, py::keep_alive<0, 1>()))
+      .def_property_readonly("particle_kappa", py::cpp_function(&DF::particle_kappa, py::keep_alive<0, 1>()))
+      .def_property_readonly("particle_kappa_v", py::cpp_function(&DF::particle_kappa_v, py::keep_alive<0, 1>()))
+      .def_property_readonly("particle_density_adv", py::cpp_function(&DF::particle_density_adv, py::keep_alive<0, 1>()));
+
+  using II = SolverIi<float>;
+  py::class_<II, SolverBase<float>> ii(m, "SolverIifloat");
+  ii.def(py::init<Runner<float>&, Pile<float>&, Store&, U, U, bool, bool, Const<float> const*,
+                  ConstiN const*, bool>(),
+         py::arg("runner"), py::arg("pile"), py::arg("store"), py::arg("max_num_particles"),
+         py::arg("num_ushers") = 0, py::arg("enable_surface_tension") = false,
+         py::arg("enable_vorticity") = false, py::arg("cn") = nullptr, py::arg("cni") = nullptr,
+         py::arg("graphical") = false, py::keep_alive<1, 3>())
+      .def_property_readonly("num_density_solve", [](II const& s) { return s.report().num_density_solve; })
+      .def_property_readonly("mean_density_error",
+                             [](II const& s) { return s.report().mean_density_error; })
+      .def_property("density_error_tolerance", &II::get_density_error_tolerance,
+                    &II::set_density_error_tolerance)
+      .def_property("min_density_solve", &II::get_min_density_solve, &II::set_min_density_solve)
+      .def_property("max_density_solve", &II::get_max_density_solve, &II::set_max_density_solve)
+      .def_property_readonly("particle_pressure", py::cpp_function(&II::particle_pressure, py::keep_alive<0, 1>()));
+
+  m.def("version", []() { return std::string(alv_version()); });
+}

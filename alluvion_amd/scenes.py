@@ -1,0 +1,221 @@
+"""Deterministic synthetic scenes for the solver path.
+
+Random numbers come from SplitMix64 (Steele, Lea, Flood 2014) mapped to
+uniform doubles in [0, 1) by the top 53 bits, so every scene is reproducible
+from its seed on any machine.  Lattice index order follows the reference's
+create_fluid_block mode 0 (runner.hpp:970-995): z fastest, then x, then y.
+
+Scenes:
+* ``dam_break_small``: 16,384 particles (32 x 32 x 16 lattice, r = 2.5 mm,
+  spacing 2r, jitter +-0.01 r) in a 0.4 m cube container with a 64^3 volume
+  map; DFSPH at dt = 2e-4 s.
+* ``dam_break_rigid``: 1,048,576 particles (128 x 128 x 64) in a 1.0 m cube
+  with four dynamic rigid bodies (two spheres r = 0.05 m, two cubes of edge
+  0.08 m, 128^3 volume maps) seeded above the fluid.
+* ``periodic_random``: N particles uniform in the periodic unit cube
+  [-0.5, 0.5)^3 with uniform random velocities (+-0.5 m/s).
+"""
+from dataclasses import dataclass, field
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+_MASK64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n successive SplitMix64 outputs (uint64) from the given seed."""
+    with np.errstate(over="ignore"):
+        state = np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64) * np.uint64(
+            0x9E3779B97F4A7C15)
+        z = state
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed: int, n: int) -> np.ndarray:
+    """n uniform doubles in [0, 1)."""
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+@dataclass
+class BoundarySpec:
+    kind: str  # "box", "box_shell", "sphere"
+    params: Tuple[float, ...]
+    sign: float
+    resolution: Tuple[int, int, int]
+    mass: float = 0.0
+    inertia: Tuple[float, float, float] = (1.0, 1.0, 1.0)
+    x: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    q: Tuple[float, float, float, float] = (0.0, 0.0, 0.0, 1.0)
+
+
+@dataclass
+class Scene:
+    name: str
+    x: np.ndarray  # (N, 3) float32
+    v: np.ndarray  # (N, 3) float32
+    particle_radius: float = 0.0025
+    kernel_radius: float = 0.01
+    density0: float = 1000.0
+    particle_mass: float = 1.25e-4
+    viscosity: float = 0.01
+    boundary_viscosity: float = 0.01
+    dfsph_factor_epsilon: float = 1e-6
+    gravity: Tuple[float, float, float] = (0.0, -9.81, 0.0)
+    wrap_length: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    grid_offset: Tuple[int, int, int] = (-21, -21, -21)
+    grid_res: Tuple[int, int, int] = (42, 42, 42)
+    max_num_neighbors: int = 64
+    max_num_particles_per_cell: int = 64
+    dt: float = 2e-4
+    cfl: float = 0.4
+    density_change_tolerance: float = 1e-3
+    density_error_tolerance: float = 1e-3
+    min_divergence_solve: int = 1
+    max_divergence_solve: int = 100
+    min_density_solve: int = 2
+    max_density_solve: int = 100
+    steps: int = 1000
+    boundaries: List[BoundarySpec] = field(default_factory=list)
+    pile_gravity: Tuple[float, float, float] = (0.0, -9.81, 0.0)
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+
+def lattice_block(nx: int, ny: int, nz: int, radius: float, box_min: float,
+                  seed: int, jitter: float) -> np.ndarray:
+    """Jittered lattice in the reference's create_fluid_block mode-0 order."""
+    n = nx * ny * nz
+    p = np.arange(n, dtype=np.int64)
+    j = (p % (nx * nz)) // nz  # x
+    k = p // (nx * nz)  # y
+    l = p % nz  # z
+    d = 2.0 * radius
+    start = box_min + d
+    pos = np.stack([start + d * j, start + d * k, start + d * l], axis=1)
+    u = uniform(seed, 3 * n).reshape(n, 3)
+    pos = pos + (2.0 * u - 1.0) * jitter
+    return pos.astype(np.float32)
+
+
+def dam_break_small(seed: int = 1) -> Scene:
+    r = 0.0025
+    x = lattice_block(32, 32, 16, r, -0.2, seed, 0.01 * r)
+    return Scene(
+        name="dam_break_16k",
+        x=x,
+        v=np.zeros_like(x),
+        grid_offset=(-21, -21, -21),
+        grid_res=(42, 42, 42),
+        steps=1000,
+        boundaries=[BoundarySpec("box", (0.4, 0.4, 0.4, 0.0), -1.0, (64, 64, 64))],
+    )
+
+
+def dam_break_rigid(seed: int = 2, body_seed: int = 3, nx: int = 128, ny: int = 128,
+                    nz: int = 64, body_resolution: int = 128) -> Scene:
+    r = 0.0025
+    x = lattice_block(nx, ny, nz, r, -0.5, seed, 0.01 * r)
+    u = uniform(body_seed, 12).reshape(4, 3)
+    bodies = []
+    sphere_r, box_a = 0.05, 0.08
+    rho_sphere, rho_box = 500.0, 800.0
+    for b in range(4):
+        pos = (-0.35 + 0.7 * u[b, 0], 0.2 + 0.2 * u[b, 1], -0.35 + 0.7 * u[b, 2])
+        res = (body_resolution,) * 3
+        if b < 2:
+            m = rho_sphere * 4.0 / 3.0 * np.pi * sphere_r ** 3
+            inertia = (0.4 * m * sphere_r ** 2,) * 3
+            bodies.append(BoundarySpec("sphere", (sphere_r,), 1.0, res, m, inertia, pos))
+        else:
+            m = rho_box * box_a ** 3
+            inertia = (m * box_a ** 2 / 6.0,) * 3
+            bodies.append(BoundarySpec("box", (box_a, box_a, box_a, 0.0), 1.0, res, m, inertia,
+                                       pos))
+    container = BoundarySpec("box", (1.0, 1.0, 1.0, 0.0), -1.0, (64, 64, 64))
+    return Scene(
+        name="dam_break_1m_rigid",
+        x=x,
+        v=np.zeros_like(x),
+        grid_offset=(-51, -51, -51),
+        grid_res=(102, 102, 102),
+        steps=500,
+        boundaries=[container] + bodies,
+    )
+
+
+def periodic_random(n: int, seed: int = 4, vseed: int = 5) -> Scene:
+    x = (uniform(seed, 3 * n).reshape(n, 3) - 0.5).astype(np.float32)
+    v = ((uniform(vseed, 3 * n).reshape(n, 3) - 0.5)).astype(np.float32)
+    return Scene(
+        name="periodic_random_%d" % n,
+        x=x,
+        v=v,
+        gravity=(0.0, 0.0, 0.0),
+        wrap_length=(1.0, 1.0, 1.0),
+        grid_offset=(-50, -50, -50),
+        grid_res=(100, 100, 100),
+        steps=100,
+        boundaries=[],
+    )
+
+
+def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optional[int] = None):
+    """Creates Depot, Pile and solver for a scene on the GPU path.
+
+    Returns (dp, pile, solver, cn, cni).  The particle state is uploaded.
+    """
+    from . import Depot, _alluvion as _al
+
+    if dp is None:
+        dp = Depot(np.float32)
+    cn, cni = dp.create_cn()
+    cn.set_kernel_radius(scene.kernel_radius)
+    cn.set_particle_attr(scene.particle_radius, scene.particle_mass, scene.density0)
+    cn.gravity = dp.f3(*scene.gravity)
+    cn.viscosity = scene.viscosity
+    cn.boundary_viscosity = scene.boundary_viscosity
+    cn.dfsph_factor_epsilon = scene.dfsph_factor_epsilon
+    if scene.wrap_length[1] > 0:
+        cn.set_wrap_length(scene.wrap_length[1])
+    cn.wrap_length_x = scene.wrap_length[0]
+    cn.wrap_length_z = scene.wrap_length[2]
+    cni.grid_res = _al.uint3(*scene.grid_res)
+    cni.grid_offset = _al.int3(*scene.grid_offset)
+    cni.max_num_neighbors_per_particle = scene.max_num_neighbors
+    cni.max_num_particles_per_cell = scene.max_num_particles_per_cell
+    runner = dp.Runner()
+    pile = dp.Pile(dp, runner, 0, cn=cn, cni=cni)
+    for b in scene.boundaries:
+        if b.kind == "box":
+            dist = dp.BoxDistance.create(dp.f3(*b.params[:3]), b.params[3])
+        elif b.kind == "box_shell":
+            dist = dp.BoxShellDistance.create(dp.f3(*b.params[:3]), b.params[3], b.params[4])
+        else:
+            dist = dp.SphereDistance.create(b.params[0])
+        pile.add(dist, _al.uint3(*b.resolution), sign=b.sign, mass=b.mass,
+                 inertia_tensor=dp.f3(*b.inertia), x=dp.f3(*b.x), q=dp.f4(*b.q))
+    pile.set_gravity(dp.f3(*scene.pile_gravity))
+    n_max = scene.n if max_particles is None else max_particles
+    cls = dp.SolverDf if kind == "df" else dp.SolverIi
+    solver = cls(runner, pile, dp, n_max, cn=cn, cni=cni)
+    solver.num_particles = scene.n
+    solver.dt = scene.dt
+    solver.max_dt = scene.dt
+    solver.min_dt = scene.dt
+    solver.cfl = scene.cfl
+    solver.density_error_tolerance = scene.density_error_tolerance
+    solver.min_density_solve = scene.min_density_solve
+    solver.max_density_solve = scene.max_density_solve
+    if kind == "df":
+        solver.density_change_tolerance = scene.density_change_tolerance
+        solver.min_divergence_solve = scene.min_divergence_solve
+        solver.max_divergence_solve = scene.max_divergence_solve
+    dp.coat(solver.particle_x).set(scene.x)
+    dp.coat(solver.particle_v).set(scene.v)
+    return dp, pile, solver, cn, cni

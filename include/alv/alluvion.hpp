@@ -1,0 +1,659 @@
+// alluvion.hpp -- header-only C++ mirror of the reference's host API for the
+// solver path (kennychufk/alluvion: store.hpp, variable.hpp, constants.hpp,
+// pile.hpp, solver.hpp, solver_df.hpp, solver_ii.hpp, dg/*_distance.hpp),
+// implemented over the C ABI in alv.h.  Class, method and field names follow
+// the reference so that host code written against it compiles with few
+// changes.  Only TF = float is provided: the device path computes in FP32.
+// Errors throw std::runtime_error like the reference (allocator.cu:35-41).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "alv.h"
+
+namespace alluvion {
+
+using U = unsigned int;
+using I = int;
+
+struct float2 {
+  float x, y;
+};
+struct float3 {
+  float x, y, z;
+};
+struct float4 {
+  float x, y, z, w;
+};
+struct uint3 {
+  U x, y, z;
+};
+struct int3 {
+  I x, y, z;
+};
+using U3 = uint3;
+using I3 = int3;
+
+inline void check(int rc) {
+  if (rc != ALV_OK) throw std::runtime_error(alv_last_error());
+}
+
+// NumericType (data_type.hpp)
+enum class NumericType { f32, f64, i32, u32, undefined };
+template <typename M>
+struct TypeInfo;
+template <>
+struct TypeInfo<float> {
+  static constexpr NumericType type = NumericType::f32;
+  static constexpr U n = 1;
+};
+template <>
+struct TypeInfo<float2> {
+  static constexpr NumericType type = NumericType::f32;
+  static constexpr U n = 2;
+};
+template <>
+struct TypeInfo<float3> {
+  static constexpr NumericType type = NumericType::f32;
+  static constexpr U n = 3;
+};
+template <>
+struct TypeInfo<float4> {
+  static constexpr NumericType type = NumericType::f32;
+  static constexpr U n = 4;
+};
+template <>
+struct TypeInfo<U> {
+  static constexpr NumericType type = NumericType::u32;
+  static constexpr U n = 1;
+};
+template <>
+struct TypeInfo<I> {
+  static constexpr NumericType type = NumericType::i32;
+  static constexpr U n = 1;
+};
+template <>
+struct TypeInfo<uint3> {
+  static constexpr NumericType type = NumericType::u32;
+  static constexpr U n = 3;
+};
+
+// Variable<D, M> (variable.hpp:30-316): non-owning {shape, device pointer}.
+template <U D, typename M>
+class Variable {
+ public:
+  Variable() : ptr_(nullptr) { shape_.fill(0); }
+  Variable(std::array<U, D> const& shape, void* ptr) : shape_(shape), ptr_(ptr) {}
+  U get_linear_shape() const {
+    U n = 1;
+    for (U s : shape_) n *= s;
+    return n;
+  }
+  U get_num_primitives_per_element() const { return TypeInfo<M>::n; }
+  U get_num_primitives() const { return get_linear_shape() * TypeInfo<M>::n; }
+  U get_num_bytes() const { return get_linear_shape() * sizeof(M); }
+  NumericType get_type() const { return TypeInfo<M>::type; }
+  std::vector<U> get_shape() const { return std::vector<U>(shape_.begin(), shape_.end()); }
+  void get_bytes(void* dst, U num_bytes, U offset = 0) const {
+    if (num_bytes == 0) return;
+    if (offset + num_bytes > get_num_bytes())
+      throw std::out_of_range("Retrieving more bytes than allocated");
+    check(alv_memcpy_d2h(dst, static_cast<char*>(ptr_) + offset, num_bytes));
+  }
+  void get_bytes(void* dst) const { get_bytes(dst, get_num_bytes()); }
+  void set_bytes(void const* src, U num_bytes, U offset = 0) {
+    if (num_bytes == 0) return;
+    if (offset + num_bytes > get_num_bytes())
+      throw std::out_of_range("Setting more bytes than allocated");
+    check(alv_memcpy_h2d(static_cast<char*>(ptr_) + offset, src, num_bytes));
+  }
+  void set_bytes(void const* src) { set_bytes(src, get_num_bytes()); }
+  void set_zero() { check(alv_memset(ptr_, 0, get_num_bytes())); }
+  void set_same(int value, U num_elements, U offset = 0) {
+    check(alv_memset(static_cast<char*>(ptr_) + offset * sizeof(M), value,
+                     num_elements * sizeof(M)));
+  }
+  void set_same(int value) { check(alv_memset(ptr_, value, get_num_bytes())); }
+  void set_from(Variable const& src, U num_elements, U offset = 0) {
+    check(alv_memcpy_d2d(static_cast<char*>(ptr_) + offset * sizeof(M), src.ptr_,
+                         num_elements * sizeof(M)));
+  }
+  void set_from(Variable const& src) {
+    check(alv_memcpy_d2d(ptr_, src.ptr_, std::min(get_num_bytes(), src.get_num_bytes())));
+  }
+  std::array<U, D> shape_;
+  void* ptr_;
+};
+
+// Const<float> (constants.hpp:51-137)
+template <typename TF>
+struct Const;
+template <>
+struct Const<float> {
+  float kernel_radius = 0, kernel_radius_sqr = 0, cubic_kernel_k = 0, cubic_kernel_l = 0,
+        cubic_kernel_zero = 0;
+  float particle_radius = 0, particle_vol = 0, particle_mass = 0, density0 = 0;
+  float viscosity = 0, boundary_viscosity = 0, vorticity_coeff = 0,
+        boundary_vorticity_coeff = 0, inertia_inverse = 0, viscosity_omega = 0,
+        surface_tension_coeff = 0, surface_tension_boundary_coeff = 0;
+  float3 gravity{0, 0, 0};
+  float boundary_epsilon = 0, dfsph_factor_epsilon = 0, contact_tolerance = 0;
+  float wrap_length = 0, wrap_min = 0, wrap_max = 0;
+  // extension: periodic x and z (0 = off)
+  float wrap_length_x = 0, wrap_length_z = 0;
+
+  void set_kernel_radius(float h) {
+    alv_const c{};
+    alv_const_set_kernel_radius(&c, h);
+    kernel_radius = c.kernel_radius;
+    kernel_radius_sqr = c.kernel_radius_sqr;
+    cubic_kernel_k = c.cubic_kernel_k;
+    cubic_kernel_l = c.cubic_kernel_l;
+    cubic_kernel_zero = c.cubic_kernel_zero;
+  }
+  void set_particle_attr(float radius, float mass, float density) {
+    alv_const c{};
+    alv_const_set_particle_attr(&c, radius, mass, density);
+    particle_radius = c.particle_radius;
+    particle_mass = c.particle_mass;
+    density0 = c.density0;
+    particle_vol = c.particle_vol;
+  }
+  void set_wrap_length(float l) {
+    wrap_length = l;
+    wrap_min = l * -0.5f;
+    wrap_max = l * 0.5f;
+  }
+  void set_cubic_discretization_constants() {}
+  alv_const to_c() const {
+    alv_const c{};
+    c.kernel_radius = kernel_radius;
+    c.kernel_radius_sqr = kernel_radius_sqr;
+    c.cubic_kernel_k = cubic_kernel_k;
+    c.cubic_kernel_l = cubic_kernel_l;
+    c.cubic_kernel_zero = cubic_kernel_zero;
+    c.particle_radius = particle_radius;
+    c.particle_vol = particle_vol;
+    c.particle_mass = particle_mass;
+    c.density0 = density0;
+    c.viscosity = viscosity;
+    c.boundary_viscosity = boundary_viscosity;
+    c.gravity[0] = gravity.x;
+    c.gravity[1] = gravity.y;
+    c.gravity[2] = gravity.z;
+    c.boundary_epsilon = boundary_epsilon;
+    c.dfsph_factor_epsilon = dfsph_factor_epsilon;
+    c.contact_tolerance = contact_tolerance;
+    c.wrap_length[0] = wrap_length_x;
+    c.wrap_length[1] = wrap_length;
+    c.wrap_length[2] = wrap_length_z;
+    return c;
+  }
+};
+using Constf = Const<float>;
+
+// Consti (constants.hpp:40-49)
+struct ConstiN {
+  U max_num_particles_per_cell = 64;
+  U3 grid_res{0, 0, 0};
+  I3 grid_offset{0, 0, 0};
+  U max_num_neighbors_per_particle = 64;
+  U num_boundaries = 0;
+  U max_num_contacts = 0;
+  alv_consti to_c() const {
+    alv_consti c{};
+    c.max_num_particles_per_cell = max_num_particles_per_cell;
+    c.grid_res[0] = grid_res.x;
+    c.grid_res[1] = grid_res.y;
+    c.grid_res[2] = grid_res.z;
+    c.grid_offset[0] = grid_offset.x;
+    c.grid_offset[1] = grid_offset.y;
+    c.grid_offset[2] = grid_offset.z;
+    c.max_num_neighbors_per_particle = max_num_neighbors_per_particle;
+    c.num_boundaries = num_boundaries;
+    c.max_num_contacts = max_num_contacts;
+    return c;
+  }
+};
+
+// Store (store.hpp:18-170): device buffers, constants.
+class Store {
+ public:
+  Store() = default;
+  ~Store() {
+    for (void* p : owned_) alv_free(p);
+  }
+  static void set_device(int device) { check(alv_set_device(device)); }
+  template <U D, typename M>
+  Variable<D, M>* create(std::array<U, D> const& shape) {
+    size_t n = 1;
+    for (U s : shape) n *= s;
+    void* p = nullptr;
+    check(alv_malloc(&p, n * sizeof(M)));
+    if (p) owned_.push_back(p);
+    return new Variable<D, M>(shape, p);
+  }
+  template <U D, typename M>
+  void remove(Variable<D, M>& var) {
+    for (auto it = owned_.begin(); it != owned_.end(); ++it) {
+      if (*it == var.ptr_) {
+        alv_free(var.ptr_);
+        owned_.erase(it);
+        break;
+      }
+    }
+    var.ptr_ = nullptr;
+  }
+  Const<float>& get_cn() { return cn_; }
+  ConstiN& get_cni() { return cni_; }
+  std::tuple<Const<float>, ConstiN> create_cn() { return {Const<float>(), ConstiN()}; }
+  // copy_cn / copy_cn_external (store.hpp:142-164): the constants that the
+  // next solver step will use.
+  void copy_cn() {
+    cn_dev_ = cn_;
+    cni_dev_ = cni_;
+    has_dev_ = true;
+  }
+  void copy_cn_external(Const<float> const& cn, ConstiN const& cni) {
+    cn_dev_ = cn;
+    cni_dev_ = cni;
+    has_dev_ = true;
+  }
+  bool has_device_constants() const { return has_dev_; }
+  Const<float> const& device_cn() const { return cn_dev_; }
+  ConstiN const& device_cni() const { return cni_dev_; }
+
+ private:
+  std::vector<void*> owned_;
+  Const<float> cn_, cn_dev_;
+  ConstiN cni_, cni_dev_;
+  bool has_dev_ = false;
+};
+
+// Runner<float> (runner.hpp:4558-6312): reductions.
+template <typename TF>
+class Runner {
+ public:
+  template <U D, typename M>
+  static TF sum(Variable<D, M> const& var, U n, U offset = 0) {
+    double s = 0;
+    check(alv_sum_f32(reinterpret_cast<const float*>(static_cast<const char*>(var.ptr_) +
+                                                     size_t(offset) * sizeof(M)),
+                      size_t(n) * TypeInfo<M>::n, &s));
+    return TF(s);
+  }
+  template <U D, typename M>
+  static TF max(Variable<D, M> const& var, U n, U offset = 0) {
+    float m = 0;
+    check(alv_max_f32(reinterpret_cast<const float*>(static_cast<const char*>(var.ptr_) +
+                                                     size_t(offset) * sizeof(M)),
+                      size_t(n) * TypeInfo<M>::n, &m));
+    return TF(m);
+  }
+};
+
+namespace dg {
+// Distance hierarchy (dg/distance.hpp, box_distance.hpp,
+// box_shell_distance.hpp, sphere_distance.hpp).
+template <typename TF3, typename TF>
+class Distance {
+ public:
+  virtual ~Distance() = default;
+  virtual alv_distance to_c() const = 0;
+  TF3 aabb_min_{0, 0, 0}, aabb_max_{0, 0, 0};
+  TF max_distance_ = 0;
+  TF3 get_aabb_min() const { return aabb_min_; }
+  TF3 get_aabb_max() const { return aabb_max_; }
+  TF get_max_distance() const { return max_distance_; }
+};
+template <typename TF3, typename TF>
+class BoxDistance : public Distance<TF3, TF> {
+ public:
+  BoxDistance(TF3 widths, TF outset = 0) : widths_(widths), outset_(outset) {
+    this->aabb_min_ = {widths.x * -0.5f - outset, widths.y * -0.5f - outset,
+                       widths.z * -0.5f - outset};
+    this->aabb_max_ = {widths.x * 0.5f + outset, widths.y * 0.5f + outset,
+                       widths.z * 0.5f + outset};
+  }
+  static BoxDistance* create(TF3 widths, TF outset = 0) { return new BoxDistance(widths, outset); }
+  alv_distance to_c() const override {
+    alv_distance d{};
+    d.type = ALV_DISTANCE_BOX;
+    d.params[0] = widths_.x;
+    d.params[1] = widths_.y;
+    d.params[2] = widths_.z;
+    d.params[3] = outset_;
+    return d;
+  }
+  TF3 widths_;
+  TF outset_;
+};
+template <typename TF3, typename TF>
+class BoxShellDistance : public Distance<TF3, TF> {
+ public:
+  BoxShellDistance(TF3 inner_widths, TF thickness, TF outset = 0)
+      : inner_(inner_widths), thickness_(thickness), outset_(outset) {}
+  static BoxShellDistance* create(TF3 inner_widths, TF thickness, TF outset = 0) {
+    return new BoxShellDistance(inner_widths, thickness, outset);
+  }
+  alv_distance to_c() const override {
+    alv_distance d{};
+    d.type = ALV_DISTANCE_BOX_SHELL;
+    d.params[0] = inner_.x;
+    d.params[1] = inner_.y;
+    d.params[2] = inner_.z;
+    d.params[3] = thickness_;
+    d.params[4] = outset_;
+    return d;
+  }
+  TF3 inner_;
+  TF thickness_, outset_;
+};
+template <typename TF3, typename TF>
+class SphereDistance : public Distance<TF3, TF> {
+ public:
+  explicit SphereDistance(TF radius) : radius_(radius) {}
+  static SphereDistance* create(TF radius) { return new SphereDistance(radius); }
+  alv_distance to_c() const override {
+    alv_distance d{};
+    d.type = ALV_DISTANCE_SPHERE;
+    d.params[0] = radius_;
+    return d;
+  }
+  TF radius_;
+};
+}  // namespace dg
+
+enum class VolumeMethod { volume_map = 0, pellets = 1 };
+
+// Pile<float> (pile.hpp:20-1066): boundaries with volume maps and rigid
+// kinematics.  Host kinematics are exposed through get/set helpers.
+template <typename TF>
+class Pile {
+ public:
+  Pile(Store& store, Runner<TF>& runner, U max_num_contacts,
+       VolumeMethod volume_method = VolumeMethod::volume_map, U max_num_pellets = 10000,
+       Const<TF>* cn = nullptr, ConstiN* cni = nullptr)
+      : store_(store), cn_(cn ? cn : &store.get_cn()), cni_(cni ? cni : &store.get_cni()) {
+    (void)runner;
+    (void)max_num_pellets;
+    check(alv_pile_create(&pile_, max_num_contacts, static_cast<int>(volume_method)));
+    cni_->max_num_contacts = max_num_contacts;
+    if (cn)
+      store.copy_cn_external(*cn_, *cni_);
+    else
+      store.copy_cn();
+  }
+  ~Pile() { alv_pile_destroy(pile_); }
+  Pile(Pile const&) = delete;
+  Pile& operator=(Pile const&) = delete;
+
+  U add(dg::Distance<float3, TF>* distance, U3 resolution = U3{0, 0, 0}, TF cell_width = 0,
+        TF margin = std::numeric_limits<TF>::infinity(), TF sign = 1, TF mass = 0,
+        TF restitution = 1, TF friction = 0, float3 inertia_tensor = float3{1, 1, 1},
+        float3 x = float3{0, 0, 0}, float4 q = float4{0, 0, 0, 1}) {
+    distances_.emplace_back(distance);
+    alv_distance d = distance->to_c();
+    alv_const cn = cn_->to_c();
+    unsigned res[3] = {resolution.x, resolution.y, resolution.z};
+    float inertia[3] = {inertia_tensor.x, inertia_tensor.y, inertia_tensor.z};
+    float xx[3] = {x.x, x.y, x.z};
+    float qq[4] = {q.x, q.y, q.z, q.w};
+    unsigned id = 0;
+    check(alv_pile_add(pile_, &cn, &d, res, cell_width, margin, sign, mass, restitution,
+                       friction, inertia, xx, qq, &id));
+    mass_.push_back(mass);
+    restitution_.push_back(restitution);
+    friction_.push_back(friction);
+    inertia_tensor_.push_back(inertia_tensor);
+    cni_->num_boundaries = get_size();
+    return id;
+  }
+  U get_size() const {
+    unsigned n = 0;
+    check(alv_pile_get_size(pile_, &n));
+    return n;
+  }
+  float3 get_x(U i) const { return kin(i).x; }
+  float3 get_v(U i) const { return kin(i).v; }
+  float4 get_q(U i) const { return kin(i).q; }
+  float3 get_omega(U i) const { return kin(i).omega; }
+  float3 get_force(U i) const { return kin(i).force; }
+  float3 get_torque(U i) const { return kin(i).torque; }
+  void set_x(U i, float3 x) { check(alv_pile_set_kinematics(pile_, i, &x.x, nullptr, nullptr, nullptr)); }
+  void set_v(U i, float3 v) { check(alv_pile_set_kinematics(pile_, i, nullptr, &v.x, nullptr, nullptr)); }
+  void set_q(U i, float4 q) { check(alv_pile_set_kinematics(pile_, i, nullptr, nullptr, &q.x, nullptr)); }
+  void set_omega(U i, float3 w) {
+    check(alv_pile_set_kinematics(pile_, i, nullptr, nullptr, nullptr, &w.x));
+  }
+  void set_gravity(float3 g) { check(alv_pile_set_gravity(pile_, &g.x)); }
+  void integrate_kinematics(TF dt) { check(alv_pile_integrate_kinematics(pile_, dt)); }
+  void copy_kinematics_to_device() {}
+  Variable<1, float> get_volume_grid(U i) const {
+    float* p = nullptr;
+    unsigned n = 0;
+    check(alv_pile_get_volume_grid(pile_, i, &p, &n));
+    return Variable<1, float>({n}, p);
+  }
+  struct GridInfo {
+    float3 domain_min, domain_max, cell_size;
+    U3 resolution;
+    float sign, mass;
+  };
+  GridInfo get_grid_info(U i) const {
+    GridInfo g;
+    check(alv_pile_get_grid_info(pile_, i, &g.domain_min.x, &g.domain_max.x, &g.resolution.x,
+                                 &g.cell_size.x, &g.sign, &g.mass));
+    return g;
+  }
+  alv_pile* handle() { return pile_; }
+
+  std::vector<TF> mass_, restitution_, friction_;
+  std::vector<float3> inertia_tensor_;
+
+ private:
+  struct Kin {
+    float3 x, v, omega, force, torque;
+    float4 q;
+  };
+  Kin kin(U i) const {
+    Kin k;
+    check(alv_pile_get_kinematics(pile_, i, &k.x.x, &k.v.x, &k.q.x, &k.omega.x, &k.force.x,
+                                  &k.torque.x));
+    return k;
+  }
+  Store& store_;
+  Const<TF>* cn_;
+  ConstiN* cni_;
+  alv_pile* pile_ = nullptr;
+  std::vector<std::unique_ptr<dg::Distance<float3, TF>>> distances_;
+};
+
+// Solver base (solver.hpp:8-306) over alv_solver.
+template <typename TF>
+class SolverBase {
+ public:
+  SolverBase(int kind, Runner<TF>& runner, Pile<TF>& pile, Store& store, U max_num_particles,
+             U num_ushers, bool enable_surface_tension, bool enable_vorticity,
+             Const<TF> const* cn, ConstiN const* cni, bool graphical)
+      : max_num_particles(max_num_particles), store_(store), pile_(pile) {
+    (void)runner;
+    if (num_ushers) throw std::runtime_error("ushers are not supported on this path");
+    if (enable_surface_tension || enable_vorticity)
+      throw std::runtime_error("surface tension and vorticity are not supported on this path");
+    if (graphical) throw std::runtime_error("graphical variables are not supported");
+    if (cn)
+      store.copy_cn_external(*cn, *cni);
+    else
+      store.copy_cn();
+    alv_const c = store.device_cn().to_c();
+    alv_consti ci = store.device_cni().to_c();
+    check(alv_solver_create(&solver_, kind, pile.handle(), max_num_particles, &c, &ci));
+    check(alv_solver_get_settings(solver_, &set_));
+    particle_radius = store.device_cn().particle_radius;
+  }
+  virtual ~SolverBase() { alv_solver_destroy(solver_); }
+  SolverBase(SolverBase const&) = delete;
+  SolverBase& operator=(SolverBase const&) = delete;
+
+  // settings (read-write fields of the reference)
+  U get_num_particles() const { return set_.num_particles; }
+  void set_num_particles(U n) { set_.num_particles = n; push(); }
+#define ALV_RW(name)                                 \
+  TF get_##name() const { return set_.name; }        \
+  void set_##name(TF value) {                        \
+    set_.name = value;                               \
+    push();                                          \
+  }
+  ALV_RW(t)
+  ALV_RW(dt)
+  ALV_RW(initial_dt)
+  ALV_RW(max_dt)
+  ALV_RW(min_dt)
+  ALV_RW(cfl)
+#undef ALV_RW
+  alv_solver_report report() const {
+    alv_solver_report r;
+    check(alv_solver_get_report(solver_, &r));
+    return r;
+  }
+  void reset_solving_var() { check(alv_solver_reset_solving_var(solver_)); }
+  void reset_t() {
+    check(alv_solver_reset_t(solver_));
+    pull();
+  }
+  template <U wrap = 0>
+  void update_particle_neighbors() {
+    sync_const();
+    check(alv_solver_update_particle_neighbors(solver_));
+  }
+  void compute_all_boundaries() {}
+  template <U wrap = 0>
+  void step() {
+    sync_const();
+    push();
+    check(alv_solver_step(solver_));
+    pull();
+  }
+  template <typename M>
+  Variable<1, M> field1(int f) {
+    void* p = nullptr;
+    size_t n = 0;
+    unsigned c = 0;
+    check(alv_solver_get_field(solver_, f, &p, &n, &c));
+    return Variable<1, M>({U(n)}, p);
+  }
+  Variable<1, float3> particle_x() { return field1<float3>(ALV_FIELD_X); }
+  Variable<1, float3> particle_v() { return field1<float3>(ALV_FIELD_V); }
+  Variable<1, float3> particle_a() { return field1<float3>(ALV_FIELD_A); }
+  Variable<1, float> particle_density() { return field1<float>(ALV_FIELD_DENSITY); }
+  Variable<1, U> particle_num_neighbors() { return field1<U>(ALV_FIELD_NUM_NEIGHBORS); }
+  Variable<1, U> grid_anomaly() { return field1<U>(ALV_FIELD_GRID_ANOMALY); }
+  Variable<3, U> pid_length() {
+    void* p = nullptr;
+    check(alv_solver_get_field(solver_, ALV_FIELD_PID_LENGTH, &p, nullptr, nullptr));
+    ConstiN const& ci = store_.device_cni();
+    return Variable<3, U>({ci.grid_res.x, ci.grid_res.y, ci.grid_res.z}, p);
+  }
+  // neighbour ids in the reference layout [N, Kmax] (host vectors)
+  void get_neighbors(std::vector<U>& ids, std::vector<U>& counts) {
+    ids.assign(size_t(max_num_particles) * store_.device_cni().max_num_neighbors_per_particle,
+               0xffffffffu);
+    counts.assign(max_num_particles, 0);
+    check(alv_solver_get_neighbors(solver_, ids.data(), counts.data()));
+  }
+  void get_grid(std::vector<U>& count, std::vector<U>& start, std::vector<U>& anomaly) {
+    ConstiN const& ci = store_.device_cni();
+    size_t C = size_t(ci.grid_res.x) * ci.grid_res.y * ci.grid_res.z;
+    count.assign(C, 0);
+    start.assign(C + 1, 0);
+    anomaly.assign(3, 0);
+    check(alv_solver_get_grid(solver_, count.data(), start.data(), anomaly.data()));
+  }
+  alv_solver* handle() { return solver_; }
+
+  const U max_num_particles;
+  TF particle_radius = 0;
+
+ protected:
+  void push() { check(alv_solver_set_settings(solver_, &set_)); }
+  void pull() { check(alv_solver_get_settings(solver_, &set_)); }
+  void sync_const() {
+    alv_const c = store_.device_cn().to_c();
+    alv_consti ci = store_.device_cni().to_c();
+    check(alv_solver_set_const(solver_, &c, &ci));
+  }
+  Store& store_;
+  Pile<TF>& pile_;
+  alv_solver* solver_ = nullptr;
+  alv_solver_settings set_{};
+};
+
+// SolverDf<float> (solver_df.hpp:10-386)
+template <typename TF>
+class SolverDf : public SolverBase<TF> {
+ public:
+  SolverDf(Runner<TF>& runner, Pile<TF>& pile, Store& store, U max_num_particles,
+           U num_ushers = 0, bool enable_surface_tension = false, bool enable_vorticity = false,
+           Const<TF> const* cn = nullptr, ConstiN const* cni = nullptr, bool graphical = false)
+      : SolverBase<TF>(ALV_SOLVER_DF, runner, pile, store, max_num_particles, num_ushers,
+                       enable_surface_tension, enable_vorticity, cn, cni, graphical) {}
+#define ALV_RWS(type, name)                            \
+  type get_##name() const { return type(this->set_.name); } \
+  void set_##name(type value) {                        \
+    this->set_.name = value;                           \
+    this->push();                                      \
+  }
+  ALV_RWS(bool, enable_divergence_solve)
+  ALV_RWS(bool, enable_density_solve)
+  ALV_RWS(TF, density_change_tolerance)
+  ALV_RWS(TF, density_error_tolerance)
+  ALV_RWS(U, min_density_solve)
+  ALV_RWS(U, max_density_solve)
+  ALV_RWS(U, min_divergence_solve)
+  ALV_RWS(U, max_divergence_solve)
+#undef ALV_RWS
+  Variable<1, float> particle_dfsph_factor() { return this->template field1<float>(ALV_FIELD_DFSPH_FACTOR); }
+  Variable<1, float> particle_kappa() { return this->template field1<float>(ALV_FIELD_KAPPA); }
+  Variable<1, float> particle_kappa_v() { return this->template field1<float>(ALV_FIELD_KAPPA_V); }
+  Variable<1, float> particle_density_adv() {
+    return this->template field1<float>(ALV_FIELD_DENSITY_ADV);
+  }
+};
+
+// SolverIi<float> (solver_ii.hpp:8-266)
+template <typename TF>
+class SolverIi : public SolverBase<TF> {
+ public:
+  SolverIi(Runner<TF>& runner, Pile<TF>& pile, Store& store, U max_num_particles,
+           U num_ushers = 0, bool enable_surface_tension = false, bool enable_vorticity = false,
+           Const<TF> const* cn = nullptr, ConstiN const* cni = nullptr, bool graphical = false)
+      : SolverBase<TF>(ALV_SOLVER_II, runner, pile, store, max_num_particles, num_ushers,
+                       enable_surface_tension, enable_vorticity, cn, cni, graphical) {}
+  TF get_density_error_tolerance() const { return this->set_.density_error_tolerance; }
+  void set_density_error_tolerance(TF v) {
+    this->set_.density_error_tolerance = v;
+    this->push();
+  }
+  U get_min_density_solve() const { return this->set_.min_density_solve; }
+  void set_min_density_solve(U v) {
+    this->set_.min_density_solve = v;
+    this->push();
+  }
+  U get_max_density_solve() const { return this->set_.max_density_solve; }
+  void set_max_density_solve(U v) {
+    this->set_.max_density_solve = v;
+    this->push();
+  }
+  Variable<1, float> particle_pressure() { return this->template field1<float>(ALV_FIELD_PRESSURE); }
+};
+
+}  // namespace alluvion

@@ -1,0 +1,274 @@
+/*
+ * alv.h -- C ABI of the MI355X (gfx950) SPH solver core.
+ *
+ * This is the drop-in boundary for the per-step solver path of
+ * kennychufk/alluvion.  The reference exposes that path as C++ templates
+ * (Store, Variable, Runner<TF>, Pile<TF>, SolverDf<TF>, SolverIi<TF>) bound to
+ * Python by pybind11 (lib/pyalluvion.cu:1279-1486).  Here the same operations
+ * are plain extern "C" functions over opaque handles, raw device pointers and
+ * sizes; the header-only C++ mirror alv/alluvion.hpp and the pybind11 module
+ * alluvion_amd._alluvion are thin layers on top.  Every function returns
+ * ALV_OK or an error code; alv_last_error() gives the message (the reference
+ * throws std::runtime_error, allocator.cu:35-41).
+ *
+ * Precision: FP32 state and arithmetic; convergence and rigid-force sums are
+ * accumulated in FP64.  Not thread-safe per handle; one HIP stream per solver.
+ */
+#ifndef ALV_ALV_H
+#define ALV_ALV_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALV_OK 0
+#define ALV_ERR_HIP 1
+#define ALV_ERR_ARG 2
+#define ALV_ERR_UNSUPPORTED 3
+#define ALV_ERR_STATE 4
+
+/* Message of the last failing call on this thread. */
+const char* alv_last_error(void);
+/* Library version string. */
+const char* alv_version(void);
+
+/* ---- device and memory: Store / Variable / Allocator -------------------- */
+/* Store::set_device (lib/store.cu:18-20) */
+int alv_set_device(int device);
+int alv_get_device_count(int* count);
+int alv_device_synchronize(void);
+/* Store::create / Store::remove device buffers (store.hpp:45-84,
+ * allocator.hpp:13-24).  Memory is zero-filled. */
+int alv_malloc(void** dptr, size_t bytes);
+int alv_free(void* dptr);
+/* Store::create_pinned (store.hpp:56-66) */
+int alv_malloc_pinned(void** hptr, size_t bytes);
+int alv_free_pinned(void* hptr);
+/* Variable::set_bytes / get_bytes / set_from / set_zero
+ * (variable.hpp:60-131) */
+int alv_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int alv_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int alv_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int alv_memset(void* dptr, int value, size_t bytes);
+/* Runner::sum / Runner::max over n FP32 primitives (runner.hpp:4646-4675);
+ * the sum accumulates in FP64. */
+int alv_sum_f32(const float* d, size_t n, double* out);
+int alv_max_f32(const float* d, size_t n, float* out);
+
+/* ---- constants: Const<float> and Consti (constants.hpp:40-137) ----------- */
+typedef struct alv_const {
+  float kernel_radius;
+  float kernel_radius_sqr;
+  float cubic_kernel_k;
+  float cubic_kernel_l;
+  float cubic_kernel_zero;
+  float particle_radius;
+  float particle_vol;
+  float particle_mass;
+  float density0;
+  float viscosity;
+  float boundary_viscosity;
+  float gravity[3];
+  float boundary_epsilon;
+  float dfsph_factor_epsilon;
+  float contact_tolerance;
+  /* periodic wrap; [1] is the reference's y-wrap (set_wrap_length), [0] and
+   * [2] extend it to x and z; length 0 disables an axis */
+  float wrap_length[3];
+} alv_const;
+
+typedef struct alv_consti {
+  unsigned max_num_particles_per_cell;
+  unsigned grid_res[3];
+  int grid_offset[3];
+  unsigned max_num_neighbors_per_particle;
+  unsigned num_boundaries;
+  unsigned max_num_contacts;
+} alv_consti;
+
+/* Const::set_kernel_radius / set_particle_attr / set_wrap_length
+ * (constants.hpp:100-136), computed in FP32 like the reference. */
+void alv_const_set_kernel_radius(alv_const* cn, float h);
+void alv_const_set_particle_attr(alv_const* cn, float radius, float mass,
+                                 float density);
+
+/* ---- boundaries: Pile<float> (pile.hpp:20-1066) --------------------------- */
+typedef struct alv_pile alv_pile;
+
+/* Analytic distances (dg/box_distance.hpp, dg/box_shell_distance.hpp,
+ * dg/sphere_distance.hpp). */
+#define ALV_DISTANCE_BOX 0
+#define ALV_DISTANCE_BOX_SHELL 1
+#define ALV_DISTANCE_SPHERE 2
+typedef struct alv_distance {
+  int type;
+  /* box: widths[3], outset; box shell: inner widths[3], thickness, outset;
+   * sphere: radius */
+  float params[5];
+} alv_distance;
+
+/* VolumeMethod::volume_map is the only method on this path. */
+#define ALV_VOLUME_MAP 0
+
+/* Pile::Pile (pile.hpp:151-192) */
+int alv_pile_create(alv_pile** out, unsigned max_num_contacts,
+                    int volume_method);
+int alv_pile_destroy(alv_pile* pile);
+/* Pile::add (pile.hpp:285-352): computes the map domain and resolution
+ * (calculate_domain_and_resolution, pile.hpp:40-65) and builds the volume
+ * map on the device (update_volume_field, runner.hpp:1660-1694). */
+int alv_pile_add(alv_pile* pile, const alv_const* cn,
+                 const alv_distance* distance, const unsigned resolution[3],
+                 float cell_width, float margin, float sign, float mass,
+                 float restitution, float friction, const float inertia[3],
+                 const float x[3], const float q[4], unsigned* boundary_id);
+int alv_pile_get_size(const alv_pile* pile, unsigned* size);
+/* Host kinematics x_, v_, q_, omega_ (pile.hpp:103-110) */
+int alv_pile_set_kinematics(alv_pile* pile, unsigned b, const float x[3],
+                            const float v[3], const float q[4],
+                            const float omega[3]);
+int alv_pile_get_kinematics(const alv_pile* pile, unsigned b, float x[3],
+                            float v[3], float q[4], float omega[3],
+                            float force[3], float torque[3]);
+int alv_pile_set_gravity(alv_pile* pile, const float g[3]);
+/* Pile::integrate_kinematics (pile.hpp:794-815) */
+int alv_pile_integrate_kinematics(alv_pile* pile, float dt);
+/* Volume map of boundary b (device pointer) and its grid geometry. */
+int alv_pile_get_volume_grid(const alv_pile* pile, unsigned b, float** nodes,
+                             unsigned* num_nodes);
+int alv_pile_get_grid_info(const alv_pile* pile, unsigned b,
+                           float domain_min[3], float domain_max[3],
+                           unsigned resolution[3], float cell_size[3],
+                           float* sign, float* mass);
+
+/* ---- solvers: SolverDf<float> / SolverIi<float> --------------------------- */
+typedef struct alv_solver alv_solver;
+#define ALV_SOLVER_DF 0
+#define ALV_SOLVER_II 1
+
+/* Public fields of Solver / SolverDf / SolverIi (solver.hpp:261-272,
+ * solver_df.hpp:372-379, solver_ii.hpp:259-262). */
+typedef struct alv_solver_settings {
+  unsigned num_particles;
+  float t;
+  float dt;
+  float initial_dt;
+  float max_dt;
+  float min_dt;
+  float cfl;
+  float density_change_tolerance;
+  float density_error_tolerance;
+  unsigned min_density_solve;
+  unsigned max_density_solve;
+  unsigned min_divergence_solve;
+  unsigned max_divergence_solve;
+  int enable_divergence_solve;
+  int enable_density_solve;
+} alv_solver_settings;
+
+/* Report fields (solver.hpp:274-279, solver_df.hpp:381-385). */
+typedef struct alv_solver_report {
+  unsigned num_divergence_solve;
+  unsigned num_density_solve;
+  float mean_density_change;
+  float mean_density_error;
+  float particle_max_v2;
+  float pile_max_v2;
+  float max_v2;
+  float cfl_dt;
+  float utilized_cfl;
+  float t;
+  float dt;
+} alv_solver_report;
+
+/* SolverDf / SolverIi constructors (solver.hpp:19-94, solver_df.hpp:48-67,
+ * solver_ii.hpp:40-65).  The pile must outlive the solver. */
+int alv_solver_create(alv_solver** out, int kind, alv_pile* pile,
+                      unsigned max_num_particles, const alv_const* cn,
+                      const alv_consti* cni);
+int alv_solver_destroy(alv_solver* solver);
+/* Store::copy_cn_external (store.hpp:142-164): constants used from the next
+ * step on. */
+int alv_solver_set_const(alv_solver* solver, const alv_const* cn,
+                         const alv_consti* cni);
+int alv_solver_set_settings(alv_solver* solver,
+                            const alv_solver_settings* settings);
+int alv_solver_get_settings(const alv_solver* solver,
+                            alv_solver_settings* settings);
+int alv_solver_get_report(const alv_solver* solver,
+                          alv_solver_report* report);
+
+/* Particle arrays (solver.hpp:286-305, solver_df.hpp:367-370,
+ * solver_ii.hpp:250-257) in user (particle id) order.  *dptr is a device
+ * pointer valid for the solver's lifetime; n_elements and n_components give
+ * its shape ([B, N] arrays report B*N elements).  Arrays that the step
+ * computes in cell-sorted order are brought to user order on request. */
+#define ALV_FIELD_X 0
+#define ALV_FIELD_V 1
+#define ALV_FIELD_A 2
+#define ALV_FIELD_DENSITY 3
+#define ALV_FIELD_DFSPH_FACTOR 4
+#define ALV_FIELD_KAPPA 5
+#define ALV_FIELD_KAPPA_V 6
+#define ALV_FIELD_DENSITY_ADV 7
+#define ALV_FIELD_NUM_NEIGHBORS 8
+#define ALV_FIELD_PRESSURE 9
+#define ALV_FIELD_PID_LENGTH 10
+#define ALV_FIELD_GRID_ANOMALY 11
+#define ALV_FIELD_CELL_START 12
+int alv_solver_get_field(alv_solver* solver, int field, void** dptr,
+                         size_t* n_elements, unsigned* n_components);
+
+/* SolverDf::step<wrap> (solver_df.hpp:74-359) / SolverIi::step<wrap>
+ * (solver_ii.hpp:76-243).  Periodicity follows alv_const.wrap_length; the
+ * `wrap` argument of the reference (y only) is implied by it. */
+int alv_solver_step(alv_solver* solver);
+/* Solver::compute_all_boundaries (solver.hpp:170-185) followed by
+ * Solver::update_particle_neighbors<wrap> (solver.hpp:225-260). */
+int alv_solver_update_particle_neighbors(alv_solver* solver);
+/* Solver::reset_solving_var (solver_df.hpp:361-365, solver_ii.hpp:245-248)
+ * and Solver::reset_t (solver.hpp:148-152). */
+int alv_solver_reset_solving_var(alv_solver* solver);
+int alv_solver_reset_t(alv_solver* solver);
+/* Neighbour list in the reference layout (particle_neighbors /
+ * particle_num_neighbors, solver.hpp:78-85): host buffers ids[N*Kmax]
+ * (neighbour particle ids, canonical order) and counts[N]. */
+int alv_solver_get_neighbors(alv_solver* solver, unsigned* ids,
+                             unsigned* counts);
+/* Cell counts (pid_length) [C], cell offsets [C+1] and grid_anomaly[3] into
+ * host buffers. */
+int alv_solver_get_grid(alv_solver* solver, unsigned* cell_count,
+                        unsigned* cell_start, unsigned* anomaly);
+
+/* ---- per-kernel timing (Runner::launch statistics, runner.hpp:4835-4859) -- */
+/* HIP events around every kernel launch of this process; off by default.
+ * Times are gathered at the end of each step, so enabling it adds no host
+ * synchronisation inside a step. */
+int alv_profiling_enable(int enable);
+int alv_profiling_reset(void);
+int alv_profiling_num_kernels(unsigned* n);
+int alv_profiling_kernel(unsigned index, const char** name, double* total_ms,
+                         unsigned long long* launches);
+
+/* ---- stand-alone volume-map kernels (tests/field_op.cu, field_gen.cu) ----- */
+/* index_to_node_position over all nodes of a grid (runner.hpp:1131-1178):
+ * out[3*num_nodes] host buffer. */
+int alv_node_positions(const float domain_min[3], const unsigned resolution[3],
+                       const float cell_size[3], float* out,
+                       unsigned* num_nodes);
+/* update_volume_field, distance-map variant (runner.hpp:1616-1658), on host
+ * buffers: distance[num_nodes] in, volume[num_nodes] out. */
+int alv_volume_from_distance_map(const alv_const* cn,
+                                 const float domain_min[3],
+                                 const float domain_max[3],
+                                 const unsigned resolution[3],
+                                 const float cell_size[3], float sign,
+                                 const float* distance, float* volume,
+                                 unsigned num_nodes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

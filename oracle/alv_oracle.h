@@ -1,0 +1,170 @@
+/*
+ * alv_oracle.h -- C interface of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain C++17/OpenMP restatement
+ * of the reference's SPH solver-core algorithm (kennychufk/alluvion,
+ * include/alluvion/runner.hpp, solver.hpp, solver_df.hpp, solver_ii.hpp,
+ * pile.hpp).  It exists to check the HIP product path and to time the CPU
+ * baseline.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it; the product library never links or calls it.
+ *
+ * Every per-particle loop is independent, so results do not depend on the
+ * OpenMP thread count.  Reductions are serial FP64 sums in particle order.
+ */
+#ifndef ALV_ORACLE_H
+#define ALV_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Physical and grid constants (mirror of Const<TF> + Consti,
+ * reference constants.hpp:40-137).  Derived constants (h^2, cubic k, l, W(0),
+ * particle volume, wrap bounds) are computed inside the oracle in the working
+ * precision, exactly as the reference setters do. */
+typedef struct {
+  double kernel_radius;
+  double particle_radius;
+  double particle_mass;
+  double density0;
+  double viscosity;
+  double boundary_viscosity;
+  double dfsph_factor_epsilon;
+  double gravity[3];
+  /* Periodic wrap per axis (0 = off).  The reference wraps y only
+   * (runner.hpp:689-707); x and z are the extension used by the
+   * uniform-random periodic stress scene. */
+  double wrap_length[3];
+  int grid_res[3];
+  int grid_offset[3];
+  unsigned max_num_neighbors_per_particle;
+  unsigned max_num_particles_per_cell;
+} orc_params;
+
+/* Solver controls (mirror of SolverDf / SolverIi / Solver public fields,
+ * solver_df.hpp:62-67, solver.hpp:261-272). */
+typedef struct {
+  double dt;
+  double max_dt;
+  double min_dt;
+  double cfl;
+  double density_change_tolerance;
+  double density_error_tolerance;
+  unsigned min_density_solve;
+  unsigned max_density_solve;
+  unsigned min_divergence_solve;
+  unsigned max_divergence_solve;
+  int enable_divergence_solve;
+  int enable_density_solve;
+} orc_settings;
+
+typedef struct {
+  unsigned num_divergence_solve;
+  unsigned num_density_solve;
+  double mean_density_change;
+  double mean_density_error;
+  double particle_max_v2;
+  double pile_max_v2;
+  double max_v2;
+  double cfl_dt;
+  double utilized_cfl;
+  double t;
+  double dt;
+} orc_report;
+
+/* Analytic distance types (reference dg/box_distance.hpp,
+ * dg/box_shell_distance.hpp, dg/sphere_distance.hpp). */
+enum { ORC_BOX = 0, ORC_BOX_SHELL = 1, ORC_SPHERE = 2 };
+
+typedef struct {
+  int type;
+  /* box: widths[3], outset; shell: inner widths[3], thickness, outset;
+   * sphere: radius */
+  double p[5];
+  double sign;
+  unsigned resolution[3];
+  double margin; /* negative = reference default (h if sign<0 else 2h) */
+  double mass;
+  double inertia[3];
+  double x[3];
+  double q[4]; /* (x, y, z, w) */
+} orc_boundary_desc;
+
+/* kind: 0 = DFSPH (SolverDf), 1 = IISPH (SolverIi).  precision: 4 or 8. */
+void* orc_create(int precision, int kind, unsigned max_num_particles,
+                 const orc_params* params);
+void orc_destroy(void* h);
+int orc_set_settings(void* h, const orc_settings* s);
+int orc_set_num_particles(void* h, unsigned n);
+
+/* Adds an analytic boundary; computes domain/resolution/cell size like
+ * Pile::add (pile.hpp:40-78,285-352).  The volume map starts empty. */
+int orc_add_boundary(void* h, const orc_boundary_desc* d);
+unsigned orc_num_nodes(void* h, unsigned b);
+int orc_get_grid_info(void* h, unsigned b, double* domain_min,
+                      double* domain_max, unsigned* resolution,
+                      double* cell_size);
+/* Volume map: build with the Peirce rule (runner.hpp:1660-1694) over nodes
+ * [begin, end), or set / get the node values (working precision). */
+int orc_build_volume_map(void* h, unsigned b, unsigned begin, unsigned end);
+int orc_set_volume_map(void* h, unsigned b, const void* nodes);
+int orc_get_volume_map(void* h, unsigned b, void* out);
+int orc_set_rigid(void* h, unsigned b, const double* x, const double* v,
+                  const double* q, const double* omega);
+int orc_get_rigid(void* h, unsigned b, double* x, double* v, double* q,
+                  double* omega, double* force, double* torque);
+int orc_set_gravity_pile(void* h, const double* g);
+
+/* Named per-particle arrays in working precision (vectors are packed xyz):
+ * x v a density dfsph_factor kappa kappa_v density_adv pressure
+ * last_pressure aii dii dij_pj adv_density pressure_accel density_err
+ * num_neighbors(u32) boundary(4/elt, [B,N]) boundary_kernel(4/elt, [B,N])
+ * force torque ([B,N] vec). */
+int orc_set_array(void* h, const char* name, const void* data);
+int orc_get_array(void* h, const char* name, void* out);
+
+/* Solver::update_particle_neighbors<wrap> (solver.hpp:225-260): grid,
+ * neighbour list and density.  Requires boundaries computed first. */
+int orc_compute_all_boundaries(void* h);
+int orc_update_particle_neighbors(void* h);
+/* Neighbour list in the canonical order (27 cells in the reference order,
+ * ascending particle id inside a cell): ids [N, Kmax], displacements
+ * [N, Kmax, 3] in working precision. */
+int orc_get_neighbors(void* h, unsigned* ids, void* xij);
+int orc_get_grid(void* h, unsigned* cell_count, unsigned* cell_start,
+                 unsigned* anomaly);
+int orc_step(void* h);
+int orc_get_report(void* h, orc_report* r);
+
+/* Stand-alone helpers used by the tests. */
+int orc_peirce_constants(double* cosphi, double* sinphi, double* b,
+                         double* r, double* c);
+/* Cubic spline W and grad W (runner.hpp:596-638) in precision `precision`:
+ * out[i*4 + 0..2] = gradW(d_i), out[i*4 + 3] = W(|d_i|) via d2 form. */
+int orc_cubic_kernel(int precision, double h, size_t n, const double* d,
+                     double* out);
+/* 32-node cubic Lagrange shape functions and gradients
+ * (runner.hpp:1259-1584) at local coordinate xi (double). */
+int orc_shape_functions(const double* xi, double* N, double* dN);
+/* Node position of node index l (runner.hpp:1131-1178). */
+int orc_node_position(const double* domain_min, const unsigned* res,
+                      const double* cell_size, unsigned l, double* out);
+/* Indices of the 32 nodes of cell ipos (runner.hpp:1197-1257). */
+int orc_cell_nodes(const unsigned* res, const int* ipos, unsigned* out);
+/* Volume map from a sampled distance map (the grid variant,
+ * runner.hpp:1616-1658), precision 4 or 8, nodes [begin, end). */
+int orc_volume_from_distance_map(int precision, double kernel_radius,
+                                 const double* domain_min,
+                                 const double* domain_max,
+                                 const unsigned* res, const double* cell_size,
+                                 double sign, const double* distance_nodes,
+                                 unsigned begin, unsigned end, double* out);
+/* Serial FP64 sum (Runner::sum with FP64 accumulation). */
+double orc_sum_f32(const float* v, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

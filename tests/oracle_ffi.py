@@ -1,0 +1,376 @@
+"""ctypes bindings of the CPU oracle (oracle/alv_oracle.h).
+
+Test infrastructure: used only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("kernel_radius", ctypes.c_double),
+        ("particle_radius", ctypes.c_double),
+        ("particle_mass", ctypes.c_double),
+        ("density0", ctypes.c_double),
+        ("viscosity", ctypes.c_double),
+        ("boundary_viscosity", ctypes.c_double),
+        ("dfsph_factor_epsilon", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
+        ("wrap_length", ctypes.c_double * 3),
+        ("grid_res", ctypes.c_int * 3),
+        ("grid_offset", ctypes.c_int * 3),
+        ("max_num_neighbors_per_particle", ctypes.c_uint),
+        ("max_num_particles_per_cell", ctypes.c_uint),
+    ]
+
+
+class OrcSettings(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("max_dt", ctypes.c_double),
+        ("min_dt", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("density_change_tolerance", ctypes.c_double),
+        ("density_error_tolerance", ctypes.c_double),
+        ("min_density_solve", ctypes.c_uint),
+        ("max_density_solve", ctypes.c_uint),
+        ("min_divergence_solve", ctypes.c_uint),
+        ("max_divergence_solve", ctypes.c_uint),
+        ("enable_divergence_solve", ctypes.c_int),
+        ("enable_density_solve", ctypes.c_int),
+    ]
+
+
+class OrcReport(ctypes.Structure):
+    _fields_ = [
+        ("num_divergence_solve", ctypes.c_uint),
+        ("num_density_solve", ctypes.c_uint),
+        ("mean_density_change", ctypes.c_double),
+        ("mean_density_error", ctypes.c_double),
+        ("particle_max_v2", ctypes.c_double),
+        ("pile_max_v2", ctypes.c_double),
+        ("max_v2", ctypes.c_double),
+        ("cfl_dt", ctypes.c_double),
+        ("utilized_cfl", ctypes.c_double),
+        ("t", ctypes.c_double),
+        ("dt", ctypes.c_double),
+    ]
+
+
+class OrcBoundaryDesc(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int),
+        ("p", ctypes.c_double * 5),
+        ("sign", ctypes.c_double),
+        ("resolution", ctypes.c_uint * 3),
+        ("margin", ctypes.c_double),
+        ("mass", ctypes.c_double),
+        ("inertia", ctypes.c_double * 3),
+        ("x", ctypes.c_double * 3),
+        ("q", ctypes.c_double * 4),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, cp = ctypes.c_void_p, ctypes.c_char_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint, ctypes.POINTER(OrcParams)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_set_settings.argtypes = [vp, ctypes.POINTER(OrcSettings)]
+        L.orc_set_num_particles.argtypes = [vp, ctypes.c_uint]
+        L.orc_add_boundary.argtypes = [vp, ctypes.POINTER(OrcBoundaryDesc)]
+        L.orc_num_nodes.restype = ctypes.c_uint
+        L.orc_num_nodes.argtypes = [vp, ctypes.c_uint]
+        L.orc_get_grid_info.argtypes = [vp, ctypes.c_uint, dp, dp, ctypes.POINTER(ctypes.c_uint), dp]
+        L.orc_build_volume_map.argtypes = [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint]
+        L.orc_set_volume_map.argtypes = [vp, ctypes.c_uint, vp]
+        L.orc_get_volume_map.argtypes = [vp, ctypes.c_uint, vp]
+        L.orc_set_rigid.argtypes = [vp, ctypes.c_uint, dp, dp, dp, dp]
+        L.orc_get_rigid.argtypes = [vp, ctypes.c_uint, dp, dp, dp, dp, dp, dp]
+        L.orc_set_gravity_pile.argtypes = [vp, dp]
+        L.orc_set_array.argtypes = [vp, cp, vp]
+        L.orc_get_array.argtypes = [vp, cp, vp]
+        L.orc_compute_all_boundaries.argtypes = [vp]
+        L.orc_update_particle_neighbors.argtypes = [vp]
+        L.orc_get_neighbors.argtypes = [vp, vp, vp]
+        L.orc_get_grid.argtypes = [vp, vp, vp, vp]
+        L.orc_step.argtypes = [vp]
+        L.orc_get_report.argtypes = [vp, ctypes.POINTER(OrcReport)]
+        L.orc_peirce_constants.argtypes = [dp, dp, dp, dp, dp]
+        L.orc_cubic_kernel.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_size_t, dp, dp]
+        L.orc_shape_functions.argtypes = [dp, dp, dp]
+        L.orc_node_position.argtypes = [dp, ctypes.POINTER(ctypes.c_uint), dp, ctypes.c_uint, dp]
+        L.orc_cell_nodes.argtypes = [ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_uint)]
+        L.orc_volume_from_distance_map.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp,
+                                                   ctypes.POINTER(ctypes.c_uint), dp,
+                                                   ctypes.c_double, dp, ctypes.c_uint,
+                                                   ctypes.c_uint, dp]
+        L.orc_sum_f32.restype = ctypes.c_double
+        L.orc_sum_f32.argtypes = [vp, ctypes.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+_KIND = {"box": 0, "box_shell": 1, "sphere": 2}
+_VEC_ARRAYS = {"x", "v", "a", "dii", "dij_pj", "pressure_accel"}
+
+
+class Oracle:
+    """One oracle solver (DFSPH or IISPH) in FP32 or FP64."""
+
+    def __init__(self, scene, precision=4, kind="df", max_particles=None, margin=-1.0):
+        self.L = lib()
+        self.precision = precision
+        self.dtype = np.float32 if precision == 4 else np.float64
+        self.scene = scene
+        self.kmax = scene.max_num_neighbors
+        p = OrcParams()
+        p.kernel_radius = scene.kernel_radius
+        p.particle_radius = scene.particle_radius
+        p.particle_mass = scene.particle_mass
+        p.density0 = scene.density0
+        p.viscosity = scene.viscosity
+        p.boundary_viscosity = scene.boundary_viscosity
+        p.dfsph_factor_epsilon = scene.dfsph_factor_epsilon
+        for a in range(3):
+            p.gravity[a] = scene.gravity[a]
+            p.wrap_length[a] = scene.wrap_length[a]
+            p.grid_res[a] = scene.grid_res[a]
+            p.grid_offset[a] = scene.grid_offset[a]
+        p.max_num_neighbors_per_particle = scene.max_num_neighbors
+        p.max_num_particles_per_cell = scene.max_num_particles_per_cell
+        self.max_n = scene.n if max_particles is None else max_particles
+        self.h = self.L.orc_create(precision, 0 if kind == "df" else 1, self.max_n, ctypes.byref(p))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+        self.L.orc_set_num_particles(self.h, scene.n)
+        self.n = scene.n
+        s = OrcSettings()
+        s.dt = s.max_dt = s.min_dt = scene.dt
+        s.cfl = scene.cfl
+        s.density_change_tolerance = scene.density_change_tolerance
+        s.density_error_tolerance = scene.density_error_tolerance
+        s.min_density_solve = scene.min_density_solve
+        s.max_density_solve = scene.max_density_solve
+        s.min_divergence_solve = scene.min_divergence_solve
+        s.max_divergence_solve = scene.max_divergence_solve
+        s.enable_divergence_solve = 1
+        s.enable_density_solve = 1
+        self.settings = s
+        self.L.orc_set_settings(self.h, ctypes.byref(s))
+        for b in scene.boundaries:
+            d = OrcBoundaryDesc()
+            d.type = _KIND[b.kind]
+            for i, v in enumerate(b.params):
+                d.p[i] = v
+            d.sign = b.sign
+            for a in range(3):
+                d.resolution[a] = b.resolution[a]
+                d.inertia[a] = b.inertia[a]
+                d.x[a] = b.x[a]
+            for a in range(4):
+                d.q[a] = b.q[a]
+            d.margin = margin
+            d.mass = b.mass
+            self.L.orc_add_boundary(self.h, ctypes.byref(d))
+        g = np.array(scene.pile_gravity, dtype=np.float64)
+        self.L.orc_set_gravity_pile(self.h, _dptr(g))
+        full = np.zeros((self.max_n, 3), self.dtype)
+        full[: scene.n] = scene.x
+        self.set("x", full)
+        full = np.zeros((self.max_n, 3), self.dtype)
+        full[: scene.n] = scene.v
+        self.set("v", full)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def update_settings(self, **kw):
+        for k, v in kw.items():
+            setattr(self.settings, k, v)
+        self.L.orc_set_settings(self.h, ctypes.byref(self.settings))
+
+    # arrays
+    def _shape(self, name):
+        nb = len(self.scene.boundaries)
+        if name in _VEC_ARRAYS:
+            return (self.max_n, 3), self.dtype
+        if name in ("boundary", "boundary_kernel"):
+            return (nb, self.max_n, 4), self.dtype
+        if name in ("force", "torque"):
+            return (nb, self.max_n, 3), self.dtype
+        if name == "num_neighbors":
+            return (self.max_n,), np.uint32
+        return (self.max_n,), self.dtype
+
+    def get(self, name):
+        shape, dt = self._shape(name)
+        out = np.zeros(shape, dt)
+        rc = self.L.orc_get_array(self.h, name.encode(), out.ctypes.data)
+        if rc != 0:
+            raise KeyError(name)
+        return out
+
+    def set(self, name, arr):
+        shape, dt = self._shape(name)
+        a = np.ascontiguousarray(arr, dtype=dt).reshape(shape)
+        rc = self.L.orc_set_array(self.h, name.encode(), a.ctypes.data)
+        if rc != 0:
+            raise KeyError(name)
+
+    def num_nodes(self, b):
+        return int(self.L.orc_num_nodes(self.h, b))
+
+    def grid_info(self, b):
+        dmin = np.zeros(3)
+        dmax = np.zeros(3)
+        cs = np.zeros(3)
+        res = (ctypes.c_uint * 3)()
+        self.L.orc_get_grid_info(self.h, b, _dptr(dmin), _dptr(dmax), res, _dptr(cs))
+        return dmin, dmax, np.array(list(res)), cs
+
+    def build_volume_map(self, b, begin=0, end=None):
+        if end is None:
+            end = self.num_nodes(b)
+        self.L.orc_build_volume_map(self.h, b, begin, end)
+
+    def volume_map(self, b):
+        out = np.zeros(self.num_nodes(b), self.dtype)
+        self.L.orc_get_volume_map(self.h, b, out.ctypes.data)
+        return out
+
+    def set_volume_map(self, b, nodes):
+        a = np.ascontiguousarray(nodes, dtype=self.dtype)
+        assert a.size == self.num_nodes(b)
+        self.L.orc_set_volume_map(self.h, b, a.ctypes.data)
+
+    def rigid(self, b):
+        out = [np.zeros(k) for k in (3, 3, 4, 3, 3, 3)]
+        self.L.orc_get_rigid(self.h, b, *[_dptr(o) for o in out])
+        return dict(zip(["x", "v", "q", "omega", "force", "torque"], out))
+
+    def set_rigid(self, b, x=None, v=None, q=None, omega=None):
+        args = []
+        for val, k in ((x, 3), (v, 3), (q, 4), (omega, 3)):
+            if val is None:
+                args.append(ctypes.POINTER(ctypes.c_double)())
+            else:
+                arr = np.ascontiguousarray(val, dtype=np.float64)
+                args.append(_dptr(arr))
+                setattr(self, "_keep_%d" % len(args), arr)
+        self.L.orc_set_rigid(self.h, b, *args)
+
+    def compute_all_boundaries(self):
+        self.L.orc_compute_all_boundaries(self.h)
+
+    def update_particle_neighbors(self):
+        self.L.orc_update_particle_neighbors(self.h)
+
+    def neighbors(self):
+        ids = np.zeros((self.max_n, self.kmax), np.uint32)
+        self.L.orc_get_neighbors(self.h, ids.ctypes.data, None)
+        return ids, self.get("num_neighbors")
+
+    def grid(self):
+        C = int(np.prod(self.scene.grid_res))
+        cnt = np.zeros(C, np.uint32)
+        start = np.zeros(C + 1, np.uint32)
+        an = np.zeros(3, np.uint32)
+        self.L.orc_get_grid(self.h, cnt.ctypes.data, start.ctypes.data, an.ctypes.data)
+        return cnt, start, an
+
+    def step(self):
+        self.L.orc_step(self.h)
+
+    def report(self):
+        r = OrcReport()
+        self.L.orc_get_report(self.h, ctypes.byref(r))
+        return {f: getattr(r, f) for f, _ in OrcReport._fields_}
+
+
+def peirce_constants():
+    L = lib()
+    arrs = [np.zeros(30), np.zeros(30), np.zeros(30), np.zeros(15), np.zeros(15)]
+    L.orc_peirce_constants(*[_dptr(a) for a in arrs])
+    return dict(zip(["cosphi", "sinphi", "b", "r", "c"], arrs))
+
+
+def cubic_kernel(d, h, precision=8):
+    d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((d.shape[0], 4))
+    lib().orc_cubic_kernel(precision, h, d.shape[0], _dptr(d), _dptr(out))
+    return out[:, :3], out[:, 3]
+
+
+def shape_functions(xi):
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    N = np.zeros(32)
+    dN = np.zeros((32, 3))
+    lib().orc_shape_functions(_dptr(xi), _dptr(N), _dptr(dN))
+    return N, dN
+
+
+def node_position(dmin, res, cs, l):
+    dmin = np.ascontiguousarray(dmin, dtype=np.float64)
+    cs = np.ascontiguousarray(cs, dtype=np.float64)
+    r = (ctypes.c_uint * 3)(*[int(v) for v in res])
+    out = np.zeros(3)
+    lib().orc_node_position(_dptr(dmin), r, _dptr(cs), l, _dptr(out))
+    return out
+
+
+def cell_nodes(res, ipos):
+    r = (ctypes.c_uint * 3)(*[int(v) for v in res])
+    ip = (ctypes.c_int * 3)(*[int(v) for v in ipos])
+    out = (ctypes.c_uint * 32)()
+    lib().orc_cell_nodes(r, ip, out)
+    return np.array(list(out), dtype=np.int64)
+
+
+def num_nodes(res):
+    rx, ry, rz = [int(v) for v in res]
+    nv = (rx + 1) * (ry + 1) * (rz + 1)
+    ne = rx * (ry + 1) * (rz + 1) + (rx + 1) * ry * (rz + 1) + (rx + 1) * (ry + 1) * rz
+    return nv + 2 * ne
+
+
+def volume_from_distance_map(precision, h, dmin, dmax, res, cs, sign, dist, begin=0, end=None):
+    if end is None:
+        end = len(dist)
+    dmin = np.ascontiguousarray(dmin, dtype=np.float64)
+    dmax = np.ascontiguousarray(dmax, dtype=np.float64)
+    cs = np.ascontiguousarray(cs, dtype=np.float64)
+    dist = np.ascontiguousarray(dist, dtype=np.float64)
+    r = (ctypes.c_uint * 3)(*[int(v) for v in res])
+    out = np.zeros(end - begin)
+    lib().orc_volume_from_distance_map(precision, h, _dptr(dmin), _dptr(dmax), r, _dptr(cs),
+                                       sign, _dptr(dist), begin, end, _dptr(out))
+    return out

@@ -1,0 +1,206 @@
+"""GPU (HIP, through the C ABI) vs the CPU oracle on the same seeded inputs.
+
+Integer data (cell counts, offsets, anomaly flags, neighbour lists in
+canonical order) must be bit-exact.  Floating-point state after one step from
+an identical state must agree to 1e-5 relative to the field's scale
+(max-norm), the per-step tolerance of the FP32 solver; iteration counts must
+be identical.
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+from alluvion_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = max(np.abs(b).max(), 1e-30)
+    return np.abs(a - b).max() / scale
+
+
+def gpu_scene(scene, kind="df"):
+    dp, pile, solver, cn, cni = scenes.build_solver(scene, kind)
+    return dp, pile, solver
+
+
+def oracle_like(scene, dp, pile, kind="df", precision=4):
+    orc = of.Oracle(scene, precision=precision, kind=kind)
+    for b, var in enumerate(pile.volume_grids):
+        orc.set_volume_map(b, dp.coat(var).get())
+    return orc
+
+
+def sync_state(dp, solver, orc, kind="df"):
+    n = solver.num_particles
+    orc.set("x", dp.coat(solver.particle_x).get()[:n])
+    orc.set("v", dp.coat(solver.particle_v).get()[:n])
+    if kind == "df":
+        orc.set("kappa", dp.coat(solver.particle_kappa).get()[:n])
+        orc.set("kappa_v", dp.coat(solver.particle_kappa_v).get()[:n])
+    else:
+        orc.set("pressure", dp.coat(solver.particle_pressure).get()[:n])
+
+
+def test_volume_map_matches_oracle():
+    sc = scenes.dam_break_small()
+    sc.boundaries[0].resolution = (8, 8, 8)
+    dp, pile, solver = gpu_scene(sc)
+    gpu = dp.coat(pile.volume_grids[0]).get()
+    orc = of.Oracle(sc, precision=4)
+    orc.build_volume_map(0)
+    ref = orc.volume_map(0)
+    scale = 4 * np.pi / 3 * sc.kernel_radius ** 3
+    assert gpu.shape == ref.shape
+    err = np.abs(gpu.astype(np.float64) - ref).max() / scale
+    print("volume map max error / ball volume: %.3g" % err)
+    assert err <= 1e-5, err
+
+
+def test_field_op_and_field_gen_kats():
+    import alluvion_amd._alluvion as al  # noqa: F401
+    import ctypes
+    from alluvion_amd import LIB_PATH
+    L = ctypes.CDLL(LIB_PATH)
+    fp = ctypes.POINTER(ctypes.c_float)
+    # field_op: node positions on the reference's 11^3 grid
+    dmin = np.array([-23.1, 12.23, -31.44], np.float32)
+    dmax = np.array([45.1, 103.2, 12.4], np.float32)
+    res = np.array([11, 11, 11], np.uint32)
+    cs = ((dmax - dmin) / res.astype(np.float32)).astype(np.float32)
+    nn = of.num_nodes(res)
+    out = np.zeros((nn, 3), np.float32)
+    n_out = ctypes.c_uint(0)
+    rc = L.alv_node_positions(dmin.ctypes.data_as(fp), res.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+                              cs.ctypes.data_as(fp), out.ctypes.data_as(fp), ctypes.byref(n_out))
+    assert rc == 0 and n_out.value == nn
+    want = np.array([of.node_position(dmin.astype(np.float64), res, cs.astype(np.float64), l)
+                     for l in range(nn)])
+    np.testing.assert_allclose(out, want, rtol=1e-5, atol=1e-4)
+    # field_gen: ellipsoid volume field, r = 1.25, sign 1
+    r = 1.25
+    margin = 4 * r
+    dmin = np.array([-30 - margin, -20 - margin, -40 - margin], np.float32)
+    dmax = np.array([30 + margin, 20 + margin, 40 + margin], np.float32)
+    cs = ((dmax - dmin) / res.astype(np.float32)).astype(np.float32)
+    pos = np.array([of.node_position(dmin.astype(np.float64), res, cs.astype(np.float64), l)
+                    for l in range(nn)])
+    dist = (pos[:, 0] ** 2 / 625 + pos[:, 1] ** 2 / 225 + pos[:, 2] ** 2 / 1225 - 1.0).astype(np.float32)
+
+    class AlvConst(ctypes.Structure):
+        _fields_ = [(k, ctypes.c_float) for k in (
+            "kernel_radius", "kernel_radius_sqr", "cubic_kernel_k", "cubic_kernel_l",
+            "cubic_kernel_zero", "particle_radius", "particle_vol", "particle_mass", "density0",
+            "viscosity", "boundary_viscosity")] + [("gravity", ctypes.c_float * 3)] + [
+            (k, ctypes.c_float) for k in ("boundary_epsilon", "dfsph_factor_epsilon",
+                                          "contact_tolerance")] + [
+            ("wrap_length", ctypes.c_float * 3)]
+
+    cn = AlvConst()
+    L.alv_const_set_kernel_radius(ctypes.byref(cn), ctypes.c_float(r))
+    L.alv_const_set_particle_attr(ctypes.byref(cn), ctypes.c_float(0.25), ctypes.c_float(0.2),
+                                  ctypes.c_float(1.0))
+    vol = np.zeros(nn, np.float32)
+    rc = L.alv_volume_from_distance_map(
+        ctypes.byref(cn), dmin.ctypes.data_as(fp), dmax.ctypes.data_as(fp),
+        res.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), cs.ctypes.data_as(fp),
+        ctypes.c_float(1.0), dist.ctypes.data_as(fp), vol.ctypes.data_as(fp), ctypes.c_uint(nn))
+    assert rc == 0
+    ref = of.volume_from_distance_map(4, r, dmin, dmax, res, cs, 1.0, dist.astype(np.float64))
+    scale = 4 * np.pi / 3 * r ** 3
+    # doctest::Approx default: relative 1.19e-5 of the magnitude (field_gen.cu:93-100)
+    assert np.all(np.abs(vol - ref) <= 1.2e-5 * np.maximum(np.abs(ref), scale * 1e-2))
+
+
+def neighbor_parity(scene):
+    dp, pile, solver = gpu_scene(scene)
+    solver.update_particle_neighbors()
+    ids, cnt = solver.get_neighbors()
+    gcount, gstart, ganom = solver.get_grid()
+    orc = oracle_like(scene, dp, pile)
+    # alv_solver_update_particle_neighbors computes the boundary records first
+    # (Solver::compute_all_boundaries, then update_particle_neighbors)
+    orc.compute_all_boundaries()
+    orc.update_particle_neighbors()
+    oids, ocnt = orc.neighbors()
+    ocount, ostart, oanom = orc.grid()
+    n = scene.n
+    np.testing.assert_array_equal(gcount, ocount)
+    np.testing.assert_array_equal(gstart, ostart)
+    np.testing.assert_array_equal(ganom, oanom)
+    np.testing.assert_array_equal(cnt[:n], ocnt[:n])
+    K = ids.shape[1]
+    mask = np.arange(K)[None, :] < ocnt[:n, None]
+    np.testing.assert_array_equal(np.where(mask, ids[:n], 0), np.where(mask, oids[:n], 0))
+    return dp, solver, orc
+
+
+def test_neighbor_search_dam_break_bit_exact():
+    sc = scenes.dam_break_small()
+    dp, solver, orc = neighbor_parity(sc)
+    dens = dp.coat(solver.particle_density).get()[: sc.n]
+    assert rel_err(dens, orc.get("density")[: sc.n]) < REL
+
+
+def test_neighbor_search_periodic_bit_exact():
+    n = int(round(8_388_608 * 0.2 ** 3))
+    sc = scenes.periodic_random(n)
+    sc.x = (sc.x * np.float32(0.2)).astype(np.float32)
+    sc.wrap_length = (0.2, 0.2, 0.2)
+    sc.grid_offset = (-10, -10, -10)
+    sc.grid_res = (20, 20, 20)
+    neighbor_parity(sc)
+
+
+@pytest.mark.parametrize("warm", [0, 40])
+def test_dfsph_step_parity_dam_break(warm):
+    sc = scenes.dam_break_small()
+    dp, pile, solver = gpu_scene(sc)
+    for _ in range(warm):
+        solver.step()
+    orc = oracle_like(sc, dp, pile)
+    sync_state(dp, solver, orc)
+    solver.step()
+    orc.step()
+    n = sc.n
+    rep = orc.report()
+    assert solver.num_divergence_solve == rep["num_divergence_solve"]
+    assert solver.num_density_solve == rep["num_density_solve"]
+    for name, gvar, oname in (("x", solver.particle_x, "x"), ("v", solver.particle_v, "v"),
+                              ("density", solver.particle_density, "density"),
+                              ("kappa", solver.particle_kappa, "kappa"),
+                              ("kappa_v", solver.particle_kappa_v, "kappa_v")):
+        g = dp.coat(gvar).get()[:n]
+        o = orc.get(oname)[:n]
+        if name == "x":
+            # positions: relative to the displacement scale of one step
+            err = np.abs(g - o).max() / max(np.abs(dp.coat(solver.particle_v).get()[:n]).max() * sc.dt, 1e-12)
+            assert err < 1e-3, name
+            assert np.abs(g - o).max() < 1e-6
+        else:
+            assert rel_err(g, o) < REL, (name, rel_err(g, o))
+    assert abs(solver.mean_density_error - rep["mean_density_error"]) <= 1e-6
+
+
+def test_iisph_step_parity_dam_break():
+    sc = scenes.dam_break_small()
+    dp, pile, solver = gpu_scene(sc, "ii")
+    for _ in range(20):
+        solver.step()
+    orc = oracle_like(sc, dp, pile, kind="ii")
+    sync_state(dp, solver, orc, kind="ii")
+    solver.step()
+    orc.step()
+    n = sc.n
+    rep = orc.report()
+    assert solver.num_density_solve == rep["num_density_solve"]
+    for name, gvar in (("v", solver.particle_v), ("density", solver.particle_density),
+                       ("pressure", solver.particle_pressure)):
+        g = dp.coat(gvar).get()[:n]
+        o = orc.get(name)[:n]
+        assert rel_err(g, o) < REL, (name, rel_err(g, o))
