@@ -776,6 +776,7 @@ static int solver_alloc(alv_solver* S) {
   ALV_ALLOC(A.partials, grid_for(cap) + 1);
   ALV_ALLOC(A.fpartials, grid_for(cap) + 1);
   ALV_ALLOC(A.body_sums, std::max<size_t>(1, S->pile->bnd.size()) * 6);
+  ALV_ALLOC(A.body_partials, std::max<size_t>(1, S->pile->bnd.size()) * 256 * 6);
   ALV_ALLOC(A.ctl, 1);
 #undef ALV_ALLOC
   S->bnd_capacity = unsigned(std::max<size_t>(1, S->pile->bnd.size()));
@@ -1105,6 +1106,7 @@ int alv_solver_get_field(alv_solver* S, int field, void** dptr, size_t* n_elemen
     case ALV_FIELD_PID_LENGTH: ptr = A.cell_count; n = S->num_cells; break;
     case ALV_FIELD_GRID_ANOMALY: ptr = A.anomaly; n = 3; break;
     case ALV_FIELD_CELL_START: ptr = A.cell_start; n = size_t(S->num_cells) + 1; break;
+    case ALV_FIELD_BOUNDARY_MASK: ptr = A.bmask; n = S->set.num_particles; break;
     default: return fail(ALV_ERR_ARG, "unknown field");
   }
   if (need & S->dirty) {

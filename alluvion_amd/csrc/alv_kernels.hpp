@@ -62,6 +62,7 @@ struct DevArrays {
   // reductions and control
   double* partials;  // per-block FP64 partial sums
   float* fpartials;  // per-block float maxima
+  double* body_partials;  // [B, 256, 6]
   double* body_sums;  // [B, 6]
   StepCtl* ctl;
   const BoundaryDev* bnd;

@@ -19,6 +19,8 @@
 
 #include <float.h>
 
+#include <algorithm>
+
 #include "alv_device.hpp"
 #include "alv_kernels.hpp"
 
@@ -27,6 +29,13 @@ namespace alv {
 #define ALV_TID() (xcd_block() * kBlock + threadIdx.x)
 
 __device__ inline float std_max(float a, float b) { return (a < b) ? b : a; }
+
+// The w lane of the padded velocity carries the density, so the viscosity
+// pass gathers v_j and rho_j with one load; velocity updates keep it.
+__device__ inline void add_v(DevArrays& A, unsigned s, vec3 dv) {
+  float4 v = A.v[s];
+  A.v[s] = make_float4(v.x + dv.x, v.y + dv.y, v.z + dv.z, v.w);
+}
 __device__ inline float std_min(float a, float b) { return (b < a) ? b : a; }
 
 // ---------------------------------------------------------------------------
@@ -155,17 +164,19 @@ __global__ __launch_bounds__(256) void k_scatter(SimParams p, DevArrays A) {
 }
 
 // Makes the order inside each cell canonical (ascending particle id); the
-// atomic slots of k_cell_count are arrival-ordered.
+// atomic slots of k_cell_count are arrival-ordered.  The bin of particles
+// outside the grid is left unordered: nothing depends on its order (those
+// particles are in no cell and their own neighbour scans are independent).
 __global__ __launch_bounds__(256) void k_cell_sort(SimParams p, DevArrays A) {
   const unsigned c = blockIdx.x * kBlock + threadIdx.x;
-  if (c > p.num_cells) return;
+  if (c >= p.num_cells) return;
   const unsigned b = A.cell_start[c], e = A.cell_start[c + 1];
-  if (c < p.num_cells && e - b > p.cell_cap) A.anomaly[1] = 1u;
+  if (e - b > p.cell_cap) A.anomaly[1] = 1u;
   if (e - b < 2) return;
   unsigned* ids = A.sorted_id;
-  if (e - b <= 32) {
+  const unsigned m = e - b;
+  if (m <= 32) {
     unsigned loc[32];
-    const unsigned m = e - b;
     for (unsigned t = 0; t < m; ++t) loc[t] = ids[b + t];
     for (unsigned t = 1; t < m; ++t) {
       unsigned v = loc[t];
@@ -178,14 +189,19 @@ __global__ __launch_bounds__(256) void k_cell_sort(SimParams p, DevArrays A) {
     }
     for (unsigned t = 0; t < m; ++t) ids[b + t] = loc[t];
   } else {
-    for (unsigned t = b + 1; t < e; ++t) {
-      unsigned v = ids[t];
-      unsigned u = t;
-      while (u > b && ids[u - 1] > v) {
-        ids[u] = ids[u - 1];
-        --u;
+    // over-full cell (an anomaly): shell sort in place, gaps 3k+1
+    unsigned gap = 1;
+    while (gap < m / 3) gap = 3 * gap + 1;
+    for (; gap > 0; gap /= 3) {
+      for (unsigned t = gap; t < m; ++t) {
+        unsigned v = ids[b + t];
+        unsigned u = t;
+        while (u >= gap && ids[b + u - gap] > v) {
+          ids[b + u] = ids[b + u - gap];
+          u -= gap;
+        }
+        ids[b + u] = v;
       }
-      ids[u] = v;
     }
   }
 }
@@ -228,14 +244,18 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SimParams p, DevArrays 
       grid_cell_nodes(B.geom.res, ip, cell);
       vol = grid_interpolate(B.volume, cell, inner);
     }
-    vec3 kg = gradW(xi_bxj, p);
-    const size_t o = size_t(b) * A.cap + s;
-    A.brec[o] = f4(bx, vol);
-    A.bker[o] = make_float4(vol * kg.x, vol * kg.y, vol * kg.z, vol * W_dist(d, p));
-    if (vol != 0.0f) mask |= 1u << b;
-    if (B.dynamic) {
-      A.bforce[o] = make_float4(0.f, 0.f, 0.f, 0.f);
-      A.btorque[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // Records are stored only where Psi != 0; every contribution of a
+    // boundary with Psi = 0 is an exact zero, so the passes skip it.
+    if (vol != 0.0f) {
+      vec3 kg = gradW(xi_bxj, p);
+      const size_t o = size_t(b) * A.cap + s;
+      A.brec[o] = f4(bx, vol);
+      A.bker[o] = make_float4(vol * kg.x, vol * kg.y, vol * kg.z, vol * W_dist(d, p));
+      mask |= 1u << b;
+      if (B.dynamic) {
+        A.bforce[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+        A.btorque[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
     }
   }
   A.bmask[s] = mask;
@@ -259,21 +279,33 @@ __global__ __launch_bounds__(256) void k_neighbors(SimParams p, DevArrays A, int
   float dens = p.vol * p.w0;
   float sgk = 0.0f;
   vec3 gpi = mk(0.f, 0.f, 0.f);
-  for (int t = 0; t < 27; ++t) {
-    int np[3] = {ip[0] + t / 9 - 1, ip[1] + (t / 3) % 3 - 1, ip[2] + t % 3 - 1};
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      if (p.wrap_mask & (1 << ax)) {
-        if (np[ax] >= p.res[ax])
-          np[ax] -= p.res[ax];
-        else if (np[ax] < 0)
-          np[ax] += p.res[ax];
+  // The 27 cells in the reference order (dx outer, dz inner): for each
+  // (dx, dy) row the up-to-3 cells along z are contiguous in the sorted
+  // order unless a periodic wrap splits them, so each row is 1-3 spans.
+  for (int r = 0; r < 9; ++r) {
+    int nx = ip[0] + r / 3 - 1, ny = ip[1] + r % 3 - 1;
+    if (p.wrap_mask & 1) nx = nx >= p.res[0] ? nx - p.res[0] : (nx < 0 ? nx + p.res[0] : nx);
+    if (p.wrap_mask & 2) ny = ny >= p.res[1] ? ny - p.res[1] : (ny < 0 ? ny + p.res[1] : ny);
+    if (nx < 0 || nx >= p.res[0] || ny < 0 || ny >= p.res[1]) continue;
+    const unsigned row = (unsigned(nx) * unsigned(p.res[1]) + unsigned(ny)) * unsigned(p.res[2]);
+    unsigned span_b[3], span_e[3], nspan = 0;
+    int prev = -2;
+    for (int dz = -1; dz <= 1; ++dz) {
+      int nz = ip[2] + dz;
+      if (p.wrap_mask & 4) nz = nz >= p.res[2] ? nz - p.res[2] : (nz < 0 ? nz + p.res[2] : nz);
+      if (nz < 0 || nz >= p.res[2]) continue;
+      const unsigned c = row + unsigned(nz);
+      if (nz == prev + 1) {
+        span_e[nspan - 1] = A.cell_start[c + 1];
+      } else {
+        span_b[nspan] = A.cell_start[c];
+        span_e[nspan] = A.cell_start[c + 1];
+        ++nspan;
       }
+      prev = nz;
     }
-    if (!within_grid(np, p)) continue;
-    const unsigned c = cell_linear(np, p);
-    const unsigned jb = A.cell_start[c], je = A.cell_start[c + 1];
-    for (unsigned j = jb; j < je; ++j) {
+    for (unsigned q = 0; q < nspan; ++q)
+    for (unsigned j = span_b[q], je = span_e[q]; j < je; ++j) {
       if (j == s) continue;
       vec3 d = wrap_disp(xi - ld3(A.x[j]), p);
       float r2 = length_sqr(d);
@@ -312,7 +344,9 @@ __global__ __launch_bounds__(256) void k_neighbors(SimParams p, DevArrays A, int
     gpi += mk(bk.x, bk.y, bk.z);
   }
   A.nnb[s] = min(cnt, K);
-  A.density[s] = dens * p.rho0;
+  const float rho = dens * p.rho0;
+  A.density[s] = rho;
+  reinterpret_cast<float*>(A.v + s)[3] = rho;
   if (with_factor) {
     sgk += length_sqr(gpi);
     A.alpha[s] = sgk > p.eps ? -1.0f / sgk : 0.0f;
@@ -380,7 +414,7 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
   for (unsigned t = 0; t < nn; ++t) {
     const unsigned j = A.nbr[size_t(t) * A.cap + s];
     const float k_sum = k_i + kj[j];
-    if (fabsf(k_sum) > p.eps) {
+    if (fabsf(k_sum) > p.eps) {  // the position is gathered only when needed
       vec3 d = wrap_disp(xi - ld3(A.x[j]), p);
       if (kDivergenceForm) {
         vec3 grad_p_j = (-p.vol) * gradW(d, p);
@@ -418,7 +452,8 @@ __global__ __launch_bounds__(256) void k_ws_div0(SimParams p, DevArrays A) {
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
   float dadv = density_change(p, A, s);
-  A.kappa_v[s] = (dadv > 0.0f) ? 0.5f * std_max(A.kappa_v[s], -0.5f) / dt : 0.0f;
+  const float kv = (dadv > 0.0f) ? 0.5f * std_max(A.kappa_v[s], -0.5f) / dt : 0.0f;
+  A.kappa_v[s] = kv;
 }
 // warm_start_divergence_solve_1 (runner.hpp:3252-3295)
 __global__ __launch_bounds__(256) void k_ws_div1(SimParams p, DevArrays A) {
@@ -426,7 +461,7 @@ __global__ __launch_bounds__(256) void k_ws_div1(SimParams p, DevArrays A) {
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
   vec3 dv = stiffness_dv<true>(p, A, s, A.kappa_v[s], A.kappa_v, dt);
-  A.v[s] = f4(ld3(A.v[s]) + dv, 0.0f);
+  add_v(A, s, dv);
 }
 // compute_velocity_of_density_change (runner.hpp:3297-3314); also stores the
 // first iteration's stiffness k = rho_adv * alpha
@@ -439,7 +474,8 @@ __global__ __launch_bounds__(256) void k_vel_density_change(SimParams p, DevArra
   float inv_dt = 1.0f / dt;
   float alpha = A.alpha[s] * inv_dt;
   A.alpha[s] = alpha;
-  A.kfac[s] = dadv * alpha;
+  const float k = dadv * alpha;
+  A.kfac[s] = k;
 }
 // divergence_solve_iteration (runner.hpp:3316-3363)
 __global__ __launch_bounds__(256) void k_div_iter(SimParams p, DevArrays A) {
@@ -450,7 +486,7 @@ __global__ __launch_bounds__(256) void k_div_iter(SimParams p, DevArrays A) {
   const float k_i = A.kfac[s];
   A.kappa_v[s] += k_i;
   vec3 dv = stiffness_dv<true>(p, A, s, k_i, A.kfac, dt);
-  A.v[s] = f4(ld3(A.v[s]) + dv, 0.0f);
+  add_v(A, s, dv);
 }
 // compute_divergence_solve_density_error (runner.hpp:3396-3410) + FP64
 // partial sums of the residual
@@ -462,7 +498,8 @@ __global__ __launch_bounds__(256) void k_div_err(SimParams p, DevArrays A) {
   if (s < p.n) {
     float dadv = density_change(p, A, s);
     A.rho_adv[s] = dadv;
-    A.kfac[s] = dadv * A.alpha[s];
+    const float k = dadv * A.alpha[s];
+    A.kfac[s] = k;
     part = double(dadv);
   }
   double t = block_sum_d(part, lds);
@@ -531,9 +568,10 @@ __global__ __launch_bounds__(256) void k_viscosity(SimParams p, DevArrays A, int
     for (unsigned t = 0; t < nn; ++t) {
       const unsigned j = A.nbr[size_t(t) * A.cap + s];
       vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
-      da += (p.visc * p.mass / A.density[j] * dot(xixj, gradW(xixj, p)) /
+      const float4 vj = A.v[j];  // w = density of j
+      da += (p.visc * p.mass / vj.w * dot(xixj, gradW(xixj, p)) /
              (length_sqr(xixj) + 0.01f * p.h2)) *
-            (vi - ld3(A.v[j]));
+            (vi - ld3(vj));
     }
     unsigned mask = A.bmask[s];
     while (mask) {
@@ -622,7 +660,7 @@ __global__ __launch_bounds__(256) void k_integrate_np(SimParams p, DevArrays A) 
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
-  A.v[s] = f4(ld3(A.v[s]) + dt * ld3(A.a[s]), 0.f);
+  add_v(A, s, dt * ld3(A.a[s]));
 }
 
 // warm_start_pressure_solve0 (runner.hpp:3430-3450)
@@ -631,7 +669,8 @@ __global__ __launch_bounds__(256) void k_ws_p0(SimParams p, DevArrays A) {
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
   float dadv = density_adv(p, A, s, dt);
-  A.kappa[s] = (dadv > 1.0f) ? (0.5f * std_max(A.kappa[s], -0.00025f) / (dt * dt)) : 0.0f;
+  const float k = (dadv > 1.0f) ? (0.5f * std_max(A.kappa[s], -0.00025f) / (dt * dt)) : 0.0f;
+  A.kappa[s] = k;
 }
 // warm_start_pressure_solve1 (runner.hpp:3452-3491)
 __global__ __launch_bounds__(256) void k_ws_p1(SimParams p, DevArrays A) {
@@ -639,7 +678,7 @@ __global__ __launch_bounds__(256) void k_ws_p1(SimParams p, DevArrays A) {
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
   vec3 dv = stiffness_dv<false>(p, A, s, A.kappa[s], A.kappa, dt);
-  A.v[s] = f4(ld3(A.v[s]) + dv, 0.0f);
+  add_v(A, s, dv);
 }
 // compute_rho_adv (runner.hpp:3525-3541); stores k = (rho_adv - 1) * alpha
 __global__ __launch_bounds__(256) void k_rho_adv(SimParams p, DevArrays A) {
@@ -650,7 +689,8 @@ __global__ __launch_bounds__(256) void k_rho_adv(SimParams p, DevArrays A) {
   A.rho_adv[s] = dadv;
   float alpha = A.alpha[s] * (1.0f / (dt * dt));
   A.alpha[s] = alpha;
-  A.kfac[s] = (dadv - 1.0f) * alpha;
+  const float k = (dadv - 1.0f) * alpha;
+  A.kfac[s] = k;
 }
 // pressure_solve_iteration (runner.hpp:3543-3588)
 __global__ __launch_bounds__(256) void k_p_iter(SimParams p, DevArrays A) {
@@ -661,7 +701,7 @@ __global__ __launch_bounds__(256) void k_p_iter(SimParams p, DevArrays A) {
   const float k_i = A.kfac[s];
   A.kappa[s] += k_i;
   vec3 dv = stiffness_dv<false>(p, A, s, k_i, A.kfac, dt);
-  A.v[s] = f4(ld3(A.v[s]) + dv, 0.0f);
+  add_v(A, s, dv);
 }
 // compute_pressure_solve_density_error (runner.hpp:3590-3604) + FP64
 // partial sums of (rho_adv - 1)
@@ -674,7 +714,8 @@ __global__ __launch_bounds__(256) void k_p_err(SimParams p, DevArrays A) {
   if (s < p.n) {
     float dadv = density_adv(p, A, s, dt);
     A.rho_adv[s] = dadv;
-    A.kfac[s] = (dadv - 1.0f) * A.alpha[s];
+    const float k = (dadv - 1.0f) * A.alpha[s];
+    A.kfac[s] = k;
     part = double(dadv - 1.0f);
   }
   double t = block_sum_d(part, lds);
@@ -722,7 +763,7 @@ __global__ __launch_bounds__(256) void k_ii_predict0(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
-  A.v[s] = f4(ld3(A.v[s]) + dt * ld3(A.a[s]), 0.f);
+  add_v(A, s, dt * ld3(A.a[s]));
   A.last_pressure[s] = A.pressure[s] * 0.5f;
   const vec3 xi = ld3(A.x[s]);
   const float dens = A.density[s];
@@ -918,25 +959,46 @@ __global__ __launch_bounds__(256) void k_ii_finish(SimParams p, DevArrays A) {
 }
 
 // ---------------------------------------------------------------------------
-// Rigid-body sums (solver_df.hpp:347-353): FP64, one block per boundary.
+// Rigid-body sums (solver_df.hpp:347-353): FP64, deterministic two-level
+// reduction over the (particle, body) pairs with Psi != 0.  Grid (G, B):
+// block (g, b) writes the partial sums of body b over its particle range.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_body_sums(SimParams p, DevArrays A) {
+constexpr unsigned kBodyBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_body_partials(SimParams p, DevArrays A) {
   __shared__ double lds[4];
-  const unsigned b = blockIdx.x;
+  const unsigned b = blockIdx.y;
   if (!A.bnd[b].dynamic) return;
+  const unsigned bit = 1u << b;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (unsigned s = threadIdx.x; s < p.n; s += blockDim.x) {
-    const size_t o = size_t(b) * A.cap + s;
-    float4 F = A.bforce[o], T = A.btorque[o];
-    acc[0] += F.x;
-    acc[1] += F.y;
-    acc[2] += F.z;
-    acc[3] += T.x;
-    acc[4] += T.y;
-    acc[5] += T.z;
+  for (unsigned s = blockIdx.x * kBlock + threadIdx.x; s < p.n; s += gridDim.x * kBlock) {
+    if (A.bmask[s] & bit) {
+      const size_t o = size_t(b) * A.cap + s;
+      float4 F = A.bforce[o], T = A.btorque[o];
+      acc[0] += F.x;
+      acc[1] += F.y;
+      acc[2] += F.z;
+      acc[3] += T.x;
+      acc[4] += T.y;
+      acc[5] += T.z;
+    }
   }
   for (int c = 0; c < 6; ++c) {
     double t = block_sum_d(acc[c], lds);
+    if (threadIdx.x == 0) A.body_partials[(size_t(b) * gridDim.x + blockIdx.x) * 6 + c] = t;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_body_sums(SimParams p, DevArrays A, unsigned nblk) {
+  __shared__ double lds[4];
+  const unsigned b = blockIdx.x;
+  if (!A.bnd[b].dynamic) return;
+  for (int c = 0; c < 6; ++c) {
+    double acc = 0.0;
+    for (unsigned k = threadIdx.x; k < nblk; k += blockDim.x)
+      acc += A.body_partials[(size_t(b) * nblk + k) * 6 + c];
+    double t = block_sum_d(acc, lds);
     if (threadIdx.x == 0) A.body_sums[b * 6 + c] = t;
     __syncthreads();
   }
@@ -1070,7 +1132,7 @@ void launch_scatter(const SimParams& p, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH(k_scatter, p.n, s, p, A);
 }
 void launch_cell_sort(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_cell_sort, p.num_cells + 1, s, p, A);
+  ALV_LAUNCH(k_cell_sort, p.num_cells, s, p, A);
 }
 void launch_gather_boundary(const SimParams& p, const DevArrays& A, int kind, hipStream_t s) {
   ALV_LAUNCH(k_gather_boundary, p.n, s, p, A, kind);
@@ -1136,7 +1198,9 @@ void launch_ii_finish(const SimParams& p, const DevArrays& A, hipStream_t s) {
 }
 void launch_body_sums(const SimParams& p, const DevArrays& A, hipStream_t s) {
   if (p.num_boundaries == 0) return;
-  ALV_LAUNCH_GRID(k_body_sums, dim3(p.num_boundaries), s, p, A);
+  const unsigned nblk = std::min(kBodyBlocks, std::max(1u, grid_for(p.n)));
+  ALV_LAUNCH_GRID(k_body_partials, dim3(nblk, p.num_boundaries), s, p, A);
+  ALV_LAUNCH_GRID(k_body_sums, dim3(p.num_boundaries), s, p, A, nblk);
 }
 void launch_unsort_fields(const SimParams& p, const DevArrays& A, unsigned mask,
                           hipStream_t s) {

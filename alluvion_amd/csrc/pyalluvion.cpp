@@ -319,6 +319,7 @@ PYBIND11_MODULE(_alluvion, m) {
         .def_property_readonly("particle_density", py::cpp_function(&S::particle_density, py::keep_alive<0, 1>()))
         .def_property_readonly("particle_num_neighbors", py::cpp_function(&S::particle_num_neighbors, py::keep_alive<0, 1>()))
         .def_property_readonly("grid_anomaly", py::cpp_function(&S::grid_anomaly, py::keep_alive<0, 1>()))
+        .def_property_readonly("boundary_mask", py::cpp_function(&S::boundary_mask, py::keep_alive<0, 1>()))
         .def_property_readonly("pid_length", py::cpp_function(&S::pid_length, py::keep_alive<0, 1>()))
         .def("reset_solving_var", &S::reset_solving_var)
         .def("reset_t", &S::reset_t)
@@ -403,4 +404,21 @@ PYBIND11_MODULE(_alluvion, m) {
       .def_property_readonly("particle_pressure", py::cpp_function(&II::particle_pressure, py::keep_alive<0, 1>()));
 
   m.def("version", []() { return std::string(alv_version()); });
+  // Per-kernel HIP-event timing (Runner::launch statistics, runner.hpp:4835-4859)
+  m.def("profiling_enable", [](bool on) { check(alv_profiling_enable(on ? 1 : 0)); });
+  m.def("profiling_reset", []() { check(alv_profiling_reset()); });
+  m.def("profiling_stats", []() {
+    unsigned n = 0;
+    check(alv_profiling_num_kernels(&n));
+    py::dict out;
+    for (unsigned i = 0; i < n; ++i) {
+      const char* name = nullptr;
+      double ms = 0;
+      unsigned long long launches = 0;
+      check(alv_profiling_kernel(i, &name, &ms, &launches));
+      out[py::str(name)] = py::make_tuple(ms, launches);
+    }
+    return out;
+  });
+  m.def("device_synchronize", []() { check(alv_device_synchronize()); });
 }

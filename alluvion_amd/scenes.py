@@ -5,6 +5,13 @@ uniform doubles in [0, 1) by the top 53 bits, so every scene is reproducible
 from its seed on any machine.  Lattice index order follows the reference's
 create_fluid_block mode 0 (runner.hpp:970-995): z fastest, then x, then y.
 
+Fluid blocks start one kernel radius h inside the container walls (first
+lattice layer at distance h), the placement that the reference's scene
+helpers produce by dropping every particle closer than h to a boundary
+(compute_fluid_block_internal, runner.hpp:1040-1061).  Closer placement puts
+wall particles far above rest density under the volume-map boundary term
+(a corner particle at 2r starts at 2.6 rho0) and the first step explodes.
+
 Scenes:
 * ``dam_break_small``: 16,384 particles (32 x 32 x 16 lattice, r = 2.5 mm,
   spacing 2r, jitter +-0.01 r) in a 0.4 m cube container with a 64^3 volume
@@ -104,8 +111,8 @@ def lattice_block(nx: int, ny: int, nz: int, radius: float, box_min: float,
 
 
 def dam_break_small(seed: int = 1) -> Scene:
-    r = 0.0025
-    x = lattice_block(32, 32, 16, r, -0.2, seed, 0.01 * r)
+    r, h = 0.0025, 0.01
+    x = lattice_block(32, 32, 16, r, -0.2 + h - 2 * r, seed, 0.01 * r)
     return Scene(
         name="dam_break_16k",
         x=x,
@@ -119,8 +126,8 @@ def dam_break_small(seed: int = 1) -> Scene:
 
 def dam_break_rigid(seed: int = 2, body_seed: int = 3, nx: int = 128, ny: int = 128,
                     nz: int = 64, body_resolution: int = 128) -> Scene:
-    r = 0.0025
-    x = lattice_block(nx, ny, nz, r, -0.5, seed, 0.01 * r)
+    r, h = 0.0025, 0.01
+    x = lattice_block(nx, ny, nz, r, -0.5 + h - 2 * r, seed, 0.01 * r)
     u = uniform(body_seed, 12).reshape(4, 3)
     bodies = []
     sphere_r, box_a = 0.05, 0.08
@@ -165,7 +172,8 @@ def periodic_random(n: int, seed: int = 4, vseed: int = 5) -> Scene:
     )
 
 
-def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optional[int] = None):
+def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optional[int] = None,
+                 log=None):
     """Creates Depot, Pile and solver for a scene on the GPU path.
 
     Returns (dp, pile, solver, cn, cni).  The particle state is uploaded.
@@ -200,6 +208,8 @@ def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optiona
             dist = dp.SphereDistance.create(b.params[0])
         pile.add(dist, _al.uint3(*b.resolution), sign=b.sign, mass=b.mass,
                  inertia_tensor=dp.f3(*b.inertia), x=dp.f3(*b.x), q=dp.f4(*b.q))
+        if log:
+            log("boundary %s %s built" % (b.kind, b.resolution))
     pile.set_gravity(dp.f3(*scene.pile_gravity))
     n_max = scene.n if max_particles is None else max_particles
     cls = dp.SolverDf if kind == "df" else dp.SolverIi

@@ -557,6 +557,7 @@ class SolverBase {
   Variable<1, float> particle_density() { return field1<float>(ALV_FIELD_DENSITY); }
   Variable<1, U> particle_num_neighbors() { return field1<U>(ALV_FIELD_NUM_NEIGHBORS); }
   Variable<1, U> grid_anomaly() { return field1<U>(ALV_FIELD_GRID_ANOMALY); }
+  Variable<1, U> boundary_mask() { return field1<U>(ALV_FIELD_BOUNDARY_MASK); }
   Variable<3, U> pid_length() {
     void* p = nullptr;
     check(alv_solver_get_field(solver_, ALV_FIELD_PID_LENGTH, &p, nullptr, nullptr));

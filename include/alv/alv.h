@@ -218,6 +218,9 @@ int alv_solver_get_report(const alv_solver* solver,
 #define ALV_FIELD_PID_LENGTH 10
 #define ALV_FIELD_GRID_ANOMALY 11
 #define ALV_FIELD_CELL_START 12
+/* Per-particle u32 bit mask of the boundaries with non-zero volume
+ * (bit b = boundary b), in cell-sorted order; for statistics. */
+#define ALV_FIELD_BOUNDARY_MASK 13
 int alv_solver_get_field(alv_solver* solver, int field, void** dptr,
                          size_t* n_elements, unsigned* n_components);
 
