@@ -78,23 +78,24 @@ __device__ inline float W_dist(float r, const SimParams& p) {
     result = 2.0f * conj * conj * conj;
   return result * p.k;
 }
+// Both branch values are computed and one is selected, so the neighbour
+// loops stay branch-free (the selected value is the same FP32 result).
 __device__ inline float W_d2(float r2, const SimParams& p) {
   float q2 = r2 / p.h2;
   float q = sqrtf(r2) / p.h;
   float conj = 1.0f - q;
-  float result = 0.0f;
-  if (q <= 0.5f)
-    result = (6.0f * q - 6.0f) * q2 + 1.0f;
-  else if (q <= 1.0f)
-    result = 2.0f * conj * conj * conj;
+  float inner = (6.0f * q - 6.0f) * q2 + 1.0f;
+  float outer = 2.0f * conj * conj * conj;
+  float result = q <= 0.5f ? inner : (q <= 1.0f ? outer : 0.0f);
   return result * p.k;
 }
 __device__ inline vec3 gradW(vec3 d, const SimParams& p) {
   float rl2 = length_sqr(d);
   float rl = sqrtf(rl2);
   float q = rl / p.h;
-  float scale = (q <= 0.5f ? (q * 3.0f - 2.0f) / p.h
-                           : (q <= 1.0f ? (1.0f - q) * (q - 1.0f) / rl : 0.0f));
+  float s_in = (q * 3.0f - 2.0f) / p.h;
+  float s_out = (1.0f - q) * (q - 1.0f) / rl;
+  float scale = q <= 0.5f ? s_in : (q <= 1.0f ? s_out : 0.0f);
   return (p.l * scale) * d;
 }
 

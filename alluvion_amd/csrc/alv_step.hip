@@ -30,6 +30,27 @@ namespace alv {
 
 __device__ inline float std_max(float a, float b) { return (a < b) ? b : a; }
 
+// Periodic wrap of a displacement, resolved at compile time (the reference's
+// template <U wrap>, runner.hpp:1869) so that non-periodic neighbour loops
+// carry no wrap branches.
+template <bool W>
+__device__ inline vec3 wrapd(vec3 d, const SimParams& p) {
+  if constexpr (W) return wrap_disp(d, p);
+  return d;
+}
+
+// Neighbour loops run in batches: the kNB indices, then all gathers of the
+// batch, are issued before any arithmetic, so a wave waits for two memory
+// latencies per batch instead of two per neighbour.  Empty slots point at the
+// particle itself (a valid address whose values are ignored); contributions
+// are still added in list order.
+constexpr int kNB = 4;
+__device__ inline void batch_ids(const DevArrays& A, unsigned s, unsigned nn, unsigned t0,
+                                 unsigned (&j)[kNB]) {
+#pragma unroll
+  for (int u = 0; u < kNB; ++u) j[u] = (t0 + u < nn) ? A.nbr[size_t(t0 + u) * A.cap + s] : s;
+}
+
 // The w lane of the padded velocity carries the density, so the viscosity
 // pass gathers v_j and rho_j with one load; velocity updates keep it.
 __device__ inline void add_v(DevArrays& A, unsigned s, vec3 dv) {
@@ -268,6 +289,7 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SimParams p, DevArrays 
 // reference order (dx outer, dz inner); inside a cell in ascending particle
 // id, which is ascending sorted index.
 // ---------------------------------------------------------------------------
+template <bool W>
 __global__ __launch_bounds__(256) void k_neighbors(SimParams p, DevArrays A, int with_factor) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
@@ -307,7 +329,7 @@ __global__ __launch_bounds__(256) void k_neighbors(SimParams p, DevArrays A, int
     for (unsigned q = 0; q < nspan; ++q)
     for (unsigned j = span_b[q], je = span_e[q]; j < je; ++j) {
       if (j == s) continue;
-      vec3 d = wrap_disp(xi - ld3(A.x[j]), p);
+      vec3 d = wrapd<W>(xi - ld3(A.x[j]), p);
       float r2 = length_sqr(d);
       if (r2 < p.h2) {
         A.nbr[size_t(min(cnt, K - 1)) * cap + s] = j;
@@ -326,7 +348,7 @@ __global__ __launch_bounds__(256) void k_neighbors(SimParams p, DevArrays A, int
   }
   if (cnt >= K) {
     // slot K-1 holds the last neighbour found (the reference overwrites it)
-    vec3 d = wrap_disp(xi - ld3(A.x[last_j]), p);
+    vec3 d = wrapd<W>(xi - ld3(A.x[last_j]), p);
     dens += p.vol * W_d2(length_sqr(d), p);
     if (with_factor) {
       vec3 gp = (-p.vol) * gradW(d, p);
@@ -362,13 +384,26 @@ __device__ inline vec3 rigid_velocity(const BoundaryDev& B, vec3 bx) {
 }
 
 // sum_j V (v_i - v_j) . gradW_ij + sum_b (v_i - v_b) . Psi gradW_b
+template <bool W>
 __device__ inline float velocity_divergence(const SimParams& p, const DevArrays& A,
                                             unsigned s, vec3 xi, vec3 vi, unsigned nn) {
   float acc = 0.0f;
-  for (unsigned t = 0; t < nn; ++t) {
-    const unsigned j = A.nbr[size_t(t) * A.cap + s];
-    vec3 d = wrap_disp(xi - ld3(A.x[j]), p);
-    acc += p.vol * dot(vi - ld3(A.v[j]), gradW(d, p));
+  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+    unsigned j[kNB];
+    batch_ids(A, s, nn, t0, j);
+    float4 xj[kNB], vj[kNB];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      xj[u] = A.x[j[u]];
+      vj[u] = A.v[j[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      if (t0 + u < nn) {
+        vec3 d = wrapd<W>(xi - ld3(xj[u]), p);
+        acc += p.vol * dot(vi - ld3(vj[u]), gradW(d, p));
+      }
+    }
   }
   unsigned mask = A.bmask[s];
   while (mask) {
@@ -383,21 +418,23 @@ __device__ inline float velocity_divergence(const SimParams& p, const DevArrays&
 }
 
 // compute_density_change (runner.hpp:3192-3228)
+template <bool W>
 __device__ inline float density_change(const SimParams& p, const DevArrays& A, unsigned s) {
   const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
   const unsigned nn = A.nnb[s];
-  float da = velocity_divergence(p, A, s, xi, vi, nn);
+  float da = velocity_divergence<W>(p, A, s, xi, vi, nn);
   da = std_max(da, 0.0f);
   if (nn < 20u) da = 0.0f;
   return da;
 }
 
 // compute_density_adv (runner.hpp:3493-3523)
+template <bool W>
 __device__ inline float density_adv(const SimParams& p, const DevArrays& A, unsigned s,
                                     float dt) {
   const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
   const unsigned nn = A.nnb[s];
-  float delta = velocity_divergence(p, A, s, xi, vi, nn);
+  float delta = velocity_divergence<W>(p, A, s, xi, vi, nn);
   return std_max(A.density[s] / p.rho0 + dt * delta, 1.0f);
 }
 
@@ -405,22 +442,34 @@ __device__ inline float density_adv(const SimParams& p, const DevArrays& A, unsi
 // divergence_solve_iteration, warm_start_pressure_solve1,
 // pressure_solve_iteration: runner.hpp:3252-3295, 3316-3363, 3452-3491,
 // 3543-3588).  kj[] holds every particle's stiffness of this pass.
-template <bool kDivergenceForm>
+template <bool W, bool kDivergenceForm>
 __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsigned s,
-                                    float k_i, const float* __restrict__ kj, float dt) {
+                                    float k_i, const float* __restrict__ kjs, float dt) {
   const vec3 xi = ld3(A.x[s]);
   const unsigned nn = A.nnb[s];
   vec3 dv = mk(0.f, 0.f, 0.f);
-  for (unsigned t = 0; t < nn; ++t) {
-    const unsigned j = A.nbr[size_t(t) * A.cap + s];
-    const float k_sum = k_i + kj[j];
-    if (fabsf(k_sum) > p.eps) {  // the position is gathered only when needed
-      vec3 d = wrap_disp(xi - ld3(A.x[j]), p);
-      if (kDivergenceForm) {
-        vec3 grad_p_j = (-p.vol) * gradW(d, p);
-        dv -= (dt * k_sum) * grad_p_j;
-      } else {
-        dv += ((dt * k_sum) * p.vol) * gradW(d, p);
+  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+    unsigned j[kNB];
+    batch_ids(A, s, nn, t0, j);
+    float kj[kNB];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) kj[u] = kjs[j[u]];
+    // positions are gathered only for pairs that contribute
+    float4 xj[kNB];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u)
+      xj[u] = (fabsf(k_i + kj[u]) > p.eps) ? A.x[j[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      const float k_sum = k_i + kj[u];
+      if (t0 + u < nn && fabsf(k_sum) > p.eps) {
+        vec3 d = wrapd<W>(xi - ld3(xj[u]), p);
+        if (kDivergenceForm) {
+          vec3 grad_p_j = (-p.vol) * gradW(d, p);
+          dv -= (dt * k_sum) * grad_p_j;
+        } else {
+          dv += ((dt * k_sum) * p.vol) * gradW(d, p);
+        }
       }
     }
   }
@@ -447,29 +496,32 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
 }
 
 // warm_start_divergence_solve_0 (runner.hpp:3230-3250)
+template <bool W>
 __global__ __launch_bounds__(256) void k_ws_div0(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = density_change(p, A, s);
+  float dadv = density_change<W>(p, A, s);
   const float kv = (dadv > 0.0f) ? 0.5f * std_max(A.kappa_v[s], -0.5f) / dt : 0.0f;
   A.kappa_v[s] = kv;
 }
 // warm_start_divergence_solve_1 (runner.hpp:3252-3295)
+template <bool W>
 __global__ __launch_bounds__(256) void k_ws_div1(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
-  vec3 dv = stiffness_dv<true>(p, A, s, A.kappa_v[s], A.kappa_v, dt);
+  vec3 dv = stiffness_dv<W, true>(p, A, s, A.kappa_v[s], A.kappa_v, dt);
   add_v(A, s, dv);
 }
 // compute_velocity_of_density_change (runner.hpp:3297-3314); also stores the
 // first iteration's stiffness k = rho_adv * alpha
+template <bool W>
 __global__ __launch_bounds__(256) void k_vel_density_change(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = density_change(p, A, s);
+  float dadv = density_change<W>(p, A, s);
   A.rho_adv[s] = dadv;
   float inv_dt = 1.0f / dt;
   float alpha = A.alpha[s] * inv_dt;
@@ -478,6 +530,7 @@ __global__ __launch_bounds__(256) void k_vel_density_change(SimParams p, DevArra
   A.kfac[s] = k;
 }
 // divergence_solve_iteration (runner.hpp:3316-3363)
+template <bool W>
 __global__ __launch_bounds__(256) void k_div_iter(SimParams p, DevArrays A) {
   if (A.ctl->div_done) return;
   const unsigned s = ALV_TID();
@@ -485,18 +538,19 @@ __global__ __launch_bounds__(256) void k_div_iter(SimParams p, DevArrays A) {
   const float dt = A.ctl->dt;
   const float k_i = A.kfac[s];
   A.kappa_v[s] += k_i;
-  vec3 dv = stiffness_dv<true>(p, A, s, k_i, A.kfac, dt);
+  vec3 dv = stiffness_dv<W, true>(p, A, s, k_i, A.kfac, dt);
   add_v(A, s, dv);
 }
 // compute_divergence_solve_density_error (runner.hpp:3396-3410) + FP64
 // partial sums of the residual
+template <bool W>
 __global__ __launch_bounds__(256) void k_div_err(SimParams p, DevArrays A) {
   __shared__ double lds[4];
   if (A.ctl->div_done) return;
   const unsigned s = ALV_TID();
   double part = 0.0;
   if (s < p.n) {
-    float dadv = density_change(p, A, s);
+    float dadv = density_change<W>(p, A, s);
     A.rho_adv[s] = dadv;
     const float k = dadv * A.alpha[s];
     A.kfac[s] = k;
@@ -556,6 +610,7 @@ __device__ inline void orthogonal_vectors(vec3 n, vec3& t1, vec3& t2) {
   t2 = t2 * (1.0f / sqrtf(length_sqr(t2)));
 }
 
+template <bool W>
 __global__ __launch_bounds__(256) void k_viscosity(SimParams p, DevArrays A, int div_finish) {
   __shared__ float lds[4];
   const unsigned s = ALV_TID();
@@ -565,13 +620,24 @@ __global__ __launch_bounds__(256) void k_viscosity(SimParams p, DevArrays A, int
     const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
     const unsigned nn = A.nnb[s];
     vec3 da = mk(0.f, 0.f, 0.f);
-    for (unsigned t = 0; t < nn; ++t) {
-      const unsigned j = A.nbr[size_t(t) * A.cap + s];
-      vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
-      const float4 vj = A.v[j];  // w = density of j
-      da += (p.visc * p.mass / vj.w * dot(xixj, gradW(xixj, p)) /
-             (length_sqr(xixj) + 0.01f * p.h2)) *
-            (vi - ld3(vj));
+    for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+      unsigned j[kNB];
+      batch_ids(A, s, nn, t0, j);
+      float4 xj[kNB], vj[kNB];  // vj.w = density of j
+#pragma unroll
+      for (int u = 0; u < kNB; ++u) {
+        xj[u] = A.x[j[u]];
+        vj[u] = A.v[j[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kNB; ++u) {
+        if (t0 + u < nn) {
+          vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+          da += (p.visc * p.mass / vj[u].w * dot(xixj, gradW(xixj, p)) /
+                 (length_sqr(xixj) + 0.01f * p.h2)) *
+                (vi - ld3(vj[u]));
+        }
+      }
     }
     unsigned mask = A.bmask[s];
     while (mask) {
@@ -664,28 +730,31 @@ __global__ __launch_bounds__(256) void k_integrate_np(SimParams p, DevArrays A) 
 }
 
 // warm_start_pressure_solve0 (runner.hpp:3430-3450)
+template <bool W>
 __global__ __launch_bounds__(256) void k_ws_p0(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = density_adv(p, A, s, dt);
+  float dadv = density_adv<W>(p, A, s, dt);
   const float k = (dadv > 1.0f) ? (0.5f * std_max(A.kappa[s], -0.00025f) / (dt * dt)) : 0.0f;
   A.kappa[s] = k;
 }
 // warm_start_pressure_solve1 (runner.hpp:3452-3491)
+template <bool W>
 __global__ __launch_bounds__(256) void k_ws_p1(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
-  vec3 dv = stiffness_dv<false>(p, A, s, A.kappa[s], A.kappa, dt);
+  vec3 dv = stiffness_dv<W, false>(p, A, s, A.kappa[s], A.kappa, dt);
   add_v(A, s, dv);
 }
 // compute_rho_adv (runner.hpp:3525-3541); stores k = (rho_adv - 1) * alpha
+template <bool W>
 __global__ __launch_bounds__(256) void k_rho_adv(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = density_adv(p, A, s, dt);
+  float dadv = density_adv<W>(p, A, s, dt);
   A.rho_adv[s] = dadv;
   float alpha = A.alpha[s] * (1.0f / (dt * dt));
   A.alpha[s] = alpha;
@@ -693,6 +762,7 @@ __global__ __launch_bounds__(256) void k_rho_adv(SimParams p, DevArrays A) {
   A.kfac[s] = k;
 }
 // pressure_solve_iteration (runner.hpp:3543-3588)
+template <bool W>
 __global__ __launch_bounds__(256) void k_p_iter(SimParams p, DevArrays A) {
   if (A.ctl->dens_done) return;
   const unsigned s = ALV_TID();
@@ -700,11 +770,12 @@ __global__ __launch_bounds__(256) void k_p_iter(SimParams p, DevArrays A) {
   const float dt = A.ctl->dt;
   const float k_i = A.kfac[s];
   A.kappa[s] += k_i;
-  vec3 dv = stiffness_dv<false>(p, A, s, k_i, A.kfac, dt);
+  vec3 dv = stiffness_dv<W, false>(p, A, s, k_i, A.kfac, dt);
   add_v(A, s, dv);
 }
 // compute_pressure_solve_density_error (runner.hpp:3590-3604) + FP64
 // partial sums of (rho_adv - 1)
+template <bool W>
 __global__ __launch_bounds__(256) void k_p_err(SimParams p, DevArrays A) {
   __shared__ double lds[4];
   if (A.ctl->dens_done) return;
@@ -712,7 +783,7 @@ __global__ __launch_bounds__(256) void k_p_err(SimParams p, DevArrays A) {
   const float dt = A.ctl->dt;
   double part = 0.0;
   if (s < p.n) {
-    float dadv = density_adv(p, A, s, dt);
+    float dadv = density_adv<W>(p, A, s, dt);
     A.rho_adv[s] = dadv;
     const float k = (dadv - 1.0f) * A.alpha[s];
     A.kfac[s] = k;
@@ -740,13 +811,14 @@ __global__ __launch_bounds__(256) void k_p_finalize(SimParams p, DevArrays A, un
 
 // pressure_solve_finish + integrate_velocity<wrap> (runner.hpp:3606-3623),
 // then the state goes back to user order.
+template <bool W>
 __global__ __launch_bounds__(256) void k_df_finish(SimParams p, DevArrays A, int dens_finish) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
   const unsigned i = A.sorted_id[s];
   vec3 v = ld3(A.v[s]);
-  vec3 x = wrap_disp(ld3(A.x[s]) + dt * v, p);
+  vec3 x = wrapd<W>(ld3(A.x[s]) + dt * v, p);
   float kappa = A.kappa[s];
   if (dens_finish) kappa *= dt * dt;
   st3(A.ux, i, x);
@@ -759,6 +831,7 @@ __global__ __launch_bounds__(256) void k_df_finish(SimParams p, DevArrays A, int
 // IISPH (runner.hpp:2609-2887)
 // ---------------------------------------------------------------------------
 // predict_advection0 (runner.hpp:2609-2639)
+template <bool W>
 __global__ __launch_bounds__(256) void k_ii_predict0(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
@@ -770,10 +843,19 @@ __global__ __launch_bounds__(256) void k_ii_predict0(SimParams p, DevArrays A) {
   const float dens2 = dens * dens;
   vec3 d = mk(0.f, 0.f, 0.f);
   const unsigned nn = A.nnb[s];
-  for (unsigned t = 0; t < nn; ++t) {
-    const unsigned j = A.nbr[size_t(t) * A.cap + s];
-    vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
-    d -= (p.vol / dens2) * gradW(xixj, p);
+  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+    unsigned j[kNB];
+    batch_ids(A, s, nn, t0, j);
+    float4 xj[kNB];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) xj[u] = A.x[j[u]];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      if (t0 + u < nn) {
+        vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+        d -= (p.vol / dens2) * gradW(xixj, p);
+      }
+    }
   }
   unsigned mask = A.bmask[s];
   while (mask) {
@@ -785,6 +867,7 @@ __global__ __launch_bounds__(256) void k_ii_predict0(SimParams p, DevArrays A) {
   A.dii[s] = f4(d * p.rho0, 0.f);
 }
 // predict_advection1 (runner.hpp:2640-2690)
+template <bool W>
 __global__ __launch_bounds__(256) void k_ii_predict1(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
@@ -797,13 +880,25 @@ __global__ __launch_bounds__(256) void k_ii_predict1(SimParams p, DevArrays A) {
   float dadv = dens / p.rho0;
   float a_ii = 0.f;
   const unsigned nn = A.nnb[s];
-  for (unsigned t = 0; t < nn; ++t) {
-    const unsigned j = A.nbr[size_t(t) * A.cap + s];
-    vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
-    vec3 gw = gradW(xixj, p);
-    vec3 dji = (dpi * p.rho0) * gw;
-    dadv += dt * p.vol * dot(vi - ld3(A.v[j]), gw);
-    a_ii += p.vol * dot(d_ii - dji, gw);
+  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+    unsigned j[kNB];
+    batch_ids(A, s, nn, t0, j);
+    float4 xj[kNB], vj[kNB];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      xj[u] = A.x[j[u]];
+      vj[u] = A.v[j[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      if (t0 + u < nn) {
+        vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+        vec3 gw = gradW(xixj, p);
+        vec3 dji = (dpi * p.rho0) * gw;
+        dadv += dt * p.vol * dot(vi - ld3(vj[u]), gw);
+        a_ii += p.vol * dot(d_ii - dji, gw);
+      }
+    }
   }
   unsigned mask = A.bmask[s];
   while (mask) {
@@ -824,6 +919,7 @@ __global__ __launch_bounds__(256) void k_ii_predict1(SimParams p, DevArrays A) {
 // iteration's copy last_pressure <- pressure (solver_ii.hpp:204) is folded in:
 // from the second iteration on, neighbour values are read from `pressure`
 // and each particle refreshes its own last_pressure.
+template <bool W>
 __global__ __launch_bounds__(256) void k_ii_iter0(SimParams p, DevArrays A) {
   if (A.ctl->dens_done) return;
   const unsigned s = ALV_TID();
@@ -832,17 +928,32 @@ __global__ __launch_bounds__(256) void k_ii_iter0(SimParams p, DevArrays A) {
   const vec3 xi = ld3(A.x[s]);
   vec3 acc = mk(0.f, 0.f, 0.f);
   const unsigned nn = A.nnb[s];
-  for (unsigned t = 0; t < nn; ++t) {
-    const unsigned j = A.nbr[size_t(t) * A.cap + s];
-    vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
-    float dj = A.density[j];
-    float dj2 = dj * dj;
-    acc -= (p.vol / dj2 * lp[j]) * gradW(xixj, p);
+  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+    unsigned j[kNB];
+    batch_ids(A, s, nn, t0, j);
+    float4 xj[kNB];
+    float djs[kNB], lpj[kNB];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      xj[u] = A.x[j[u]];
+      djs[u] = A.density[j[u]];
+      lpj[u] = lp[j[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      if (t0 + u < nn) {
+        vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+        float dj = djs[u];
+        float dj2 = dj * dj;
+        acc -= (p.vol / dj2 * lpj[u]) * gradW(xixj, p);
+      }
+    }
   }
   A.dij_pj[s] = f4(acc * p.rho0, 0.f);
   A.last_pressure[s] = lp[s];
 }
 // pressure_solve_iteration1 (runner.hpp:2716-2772) + FP64 partial sums
+template <bool W>
 __global__ __launch_bounds__(256) void k_ii_iter1(SimParams p, DevArrays A) {
   __shared__ double lds[4];
   if (A.ctl->dens_done) return;
@@ -864,14 +975,29 @@ __global__ __launch_bounds__(256) void k_ii_iter1(SimParams p, DevArrays A) {
     float pr = 0.f;
     float sum_tmp = 0.f;
     const unsigned nn = A.nnb[s];
-    for (unsigned t = 0; t < nn; ++t) {
-      const unsigned j = A.nbr[size_t(t) * A.cap + s];
-      vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
-      vec3 djk_pk = ld3(A.dij_pj[j]);
-      vec3 gw = gradW(xixj, p);
-      vec3 dji = (dpi * p.rho0) * gw;
-      vec3 dji_pi = dji * lp;
-      sum_tmp += p.vol * dot(dpj - ld3(A.dii[j]) * A.last_pressure[j] - (djk_pk - dji_pi), gw);
+    for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+      unsigned j[kNB];
+      batch_ids(A, s, nn, t0, j);
+      float4 xj[kNB], dpk[kNB], diij[kNB];
+      float lpj[kNB];
+#pragma unroll
+      for (int u = 0; u < kNB; ++u) {
+        xj[u] = A.x[j[u]];
+        dpk[u] = A.dij_pj[j[u]];
+        diij[u] = A.dii[j[u]];
+        lpj[u] = A.last_pressure[j[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kNB; ++u) {
+        if (t0 + u < nn) {
+          vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+          vec3 djk_pk = ld3(dpk[u]);
+          vec3 gw = gradW(xixj, p);
+          vec3 dji = (dpi * p.rho0) * gw;
+          vec3 dji_pi = dji * lp;
+          sum_tmp += p.vol * dot(dpj - ld3(diij[u]) * lpj[u] - (djk_pk - dji_pi), gw);
+        }
+      }
     }
     unsigned mask = A.bmask[s];
     while (mask) {
@@ -908,6 +1034,7 @@ __global__ __launch_bounds__(256) void k_ii_finalize(SimParams p, DevArrays A, u
   }
 }
 // compute_pressure_accels (runner.hpp:2774-2817)
+template <bool W>
 __global__ __launch_bounds__(256) void k_ii_pressure_accel(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
@@ -917,13 +1044,27 @@ __global__ __launch_bounds__(256) void k_ii_pressure_accel(SimParams p, DevArray
   const float dpi = A.pressure[s] / dens2;
   vec3 ai = mk(0.f, 0.f, 0.f);
   const unsigned nn = A.nnb[s];
-  for (unsigned t = 0; t < nn; ++t) {
-    const unsigned j = A.nbr[size_t(t) * A.cap + s];
-    vec3 xixj = wrap_disp(xi - ld3(A.x[j]), p);
-    float dj = A.density[j];
-    float dj2 = dj * dj;
-    float dpj = A.pressure[j] / dj2;
-    ai -= (p.mass * (dpi + dpj)) * gradW(xixj, p);
+  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+    unsigned j[kNB];
+    batch_ids(A, s, nn, t0, j);
+    float4 xj[kNB];
+    float djs[kNB], pj[kNB];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      xj[u] = A.x[j[u]];
+      djs[u] = A.density[j[u]];
+      pj[u] = A.pressure[j[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      if (t0 + u < nn) {
+        vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+        float dj = djs[u];
+        float dj2 = dj * dj;
+        float dpj = pj[u] / dj2;
+        ai -= (p.mass * (dpi + dpj)) * gradW(xixj, p);
+      }
+    }
   }
   unsigned mask = A.bmask[s];
   while (mask) {
@@ -945,6 +1086,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure_accel(SimParams p, DevArray
   A.pacc[s] = f4(p.rho0 * ai, 0.f);
 }
 // kinematic_integration<wrap> (runner.hpp:2873-2887) + back to user order
+template <bool W>
 __global__ __launch_bounds__(256) void k_ii_finish(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
@@ -952,7 +1094,7 @@ __global__ __launch_bounds__(256) void k_ii_finish(SimParams p, DevArrays A) {
   const unsigned i = A.sorted_id[s];
   vec3 v = ld3(A.v[s]);
   v += ld3(A.pacc[s]) * dt;
-  vec3 x = wrap_disp(ld3(A.x[s]) + v * dt, p);
+  vec3 x = wrapd<W>(ld3(A.x[s]) + v * dt, p);
   st3(A.ux, i, x);
   st3(A.uv, i, v);
   A.upressure[i] = A.pressure[s];
@@ -1116,6 +1258,16 @@ KernelProfiler* g_profiler = nullptr;
     if (g_profiler) g_profiler->end(#kernel, s);            \
   } while (0)
 #define ALV_LAUNCH(kernel, n, s, ...) ALV_LAUNCH_GRID(kernel, dim3(grid_for(n)), s, __VA_ARGS__)
+// kernels templated on the periodic wrap: one name for both instances
+#define ALV_LAUNCH_W(kernel, n, s, p, ...)                                        \
+  do {                                                                            \
+    if (g_profiler) g_profiler->begin(#kernel, s);                                \
+    if ((p).wrap_mask)                                                            \
+      hipLaunchKernelGGL(kernel<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, p, __VA_ARGS__); \
+    else                                                                          \
+      hipLaunchKernelGGL(kernel<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, p, __VA_ARGS__); \
+    if (g_profiler) g_profiler->end(#kernel, s);                                  \
+  } while (0)
 #define ALV_LAUNCH1(kernel, s, ...) ALV_LAUNCH_GRID(kernel, dim3(1), s, __VA_ARGS__)
 
 void launch_cell_count(const SimParams& p, const DevArrays& A, hipStream_t s) {
@@ -1140,12 +1292,12 @@ void launch_gather_boundary(const SimParams& p, const DevArrays& A, int kind, hi
 void launch_neighbors(const SimParams& p, const DevArrays& A, int wrap_any, int with_factor,
                       hipStream_t s) {
   (void)wrap_any;
-  ALV_LAUNCH(k_neighbors, p.n, s, p, A, with_factor);
+  ALV_LAUNCH_W(k_neighbors, p.n, s, p, A, with_factor);
 }
 void launch_df_warm_start_div(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_ws_div0, p.n, s, p, A);
-  ALV_LAUNCH(k_ws_div1, p.n, s, p, A);
-  ALV_LAUNCH(k_vel_density_change, p.n, s, p, A);
+  ALV_LAUNCH_W(k_ws_div0, p.n, s, p, A);
+  ALV_LAUNCH_W(k_ws_div1, p.n, s, p, A);
+  ALV_LAUNCH_W(k_vel_density_change, p.n, s, p, A);
 }
 void launch_df_div_loop_prep(const SimParams& p, const DevArrays& A, hipStream_t s) {
   (void)p;
@@ -1153,13 +1305,13 @@ void launch_df_div_loop_prep(const SimParams& p, const DevArrays& A, hipStream_t
   (void)s;
 }
 void launch_df_div_iteration(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_div_iter, p.n, s, p, A);
-  ALV_LAUNCH(k_div_err, p.n, s, p, A);
+  ALV_LAUNCH_W(k_div_iter, p.n, s, p, A);
+  ALV_LAUNCH_W(k_div_err, p.n, s, p, A);
   ALV_LAUNCH1(k_div_finalize, s, p, A, grid_for(p.n));
 }
 void launch_df_viscosity(const SimParams& p, const DevArrays& A, int div_finish,
                          hipStream_t s) {
-  ALV_LAUNCH(k_viscosity, p.n, s, p, A, div_finish);
+  ALV_LAUNCH_W(k_viscosity, p.n, s, p, A, div_finish);
 }
 void launch_update_dt(const SimParams& p, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH1(k_update_dt, s, p, A, grid_for(p.n));
@@ -1168,33 +1320,33 @@ void launch_df_integrate_np(const SimParams& p, const DevArrays& A, hipStream_t 
   ALV_LAUNCH(k_integrate_np, p.n, s, p, A);
 }
 void launch_df_warm_start_dens(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_ws_p0, p.n, s, p, A);
-  ALV_LAUNCH(k_ws_p1, p.n, s, p, A);
-  ALV_LAUNCH(k_rho_adv, p.n, s, p, A);
+  ALV_LAUNCH_W(k_ws_p0, p.n, s, p, A);
+  ALV_LAUNCH_W(k_ws_p1, p.n, s, p, A);
+  ALV_LAUNCH_W(k_rho_adv, p.n, s, p, A);
 }
 void launch_df_dens_iteration(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_p_iter, p.n, s, p, A);
-  ALV_LAUNCH(k_p_err, p.n, s, p, A);
+  ALV_LAUNCH_W(k_p_iter, p.n, s, p, A);
+  ALV_LAUNCH_W(k_p_err, p.n, s, p, A);
   ALV_LAUNCH1(k_p_finalize, s, p, A, grid_for(p.n));
 }
 void launch_df_finish(const SimParams& p, const DevArrays& A, int dens_finish, hipStream_t s) {
-  ALV_LAUNCH(k_df_finish, p.n, s, p, A, dens_finish);
+  ALV_LAUNCH_W(k_df_finish, p.n, s, p, A, dens_finish);
 }
 void launch_ii_viscosity_predict(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_viscosity, p.n, s, p, A, 0);
+  ALV_LAUNCH_W(k_viscosity, p.n, s, p, A, 0);
 }
 void launch_ii_predict(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_ii_predict0, p.n, s, p, A);
-  ALV_LAUNCH(k_ii_predict1, p.n, s, p, A);
+  ALV_LAUNCH_W(k_ii_predict0, p.n, s, p, A);
+  ALV_LAUNCH_W(k_ii_predict1, p.n, s, p, A);
 }
 void launch_ii_iteration(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_ii_iter0, p.n, s, p, A);
-  ALV_LAUNCH(k_ii_iter1, p.n, s, p, A);
+  ALV_LAUNCH_W(k_ii_iter0, p.n, s, p, A);
+  ALV_LAUNCH_W(k_ii_iter1, p.n, s, p, A);
   ALV_LAUNCH1(k_ii_finalize, s, p, A, grid_for(p.n));
 }
 void launch_ii_finish(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_ii_pressure_accel, p.n, s, p, A);
-  ALV_LAUNCH(k_ii_finish, p.n, s, p, A);
+  ALV_LAUNCH_W(k_ii_pressure_accel, p.n, s, p, A);
+  ALV_LAUNCH_W(k_ii_finish, p.n, s, p, A);
 }
 void launch_body_sums(const SimParams& p, const DevArrays& A, hipStream_t s) {
   if (p.num_boundaries == 0) return;
