@@ -34,6 +34,14 @@ int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
+}  // namespace
+
+namespace alv {
+// error reporting shared with the other C ABI translation units
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace alv
+
+namespace {
 
 #define ALV_HIP(call)                                                              \
   do {                                                                             \
@@ -290,6 +298,10 @@ struct alv_solver {
   unsigned last_div = 2, last_dens = 3;
   unsigned* export_ids = nullptr;
   unsigned* export_counts = nullptr;
+  alv_mlp* net = nullptr;  // learned warm start (not owned)
+  int net_div = 0, net_dens = 0;  // which warm starts use it
+  int log_features = 0;
+  int features_valid = 0;  // the last step recorded warm-start features
   std::vector<void*> allocations;
 
   ~alv_solver() {
@@ -755,7 +767,10 @@ static int solver_alloc(alv_solver* S) {
   ALV_ALLOC(A.kappa_v, cap);
   ALV_ALLOC(A.rho_adv, cap);
   ALV_ALLOC(A.kfac, cap);
-  ALV_ALLOC(A.nbr, size_t(K) * cap);
+  ALV_ALLOC(A.feat, size_t(cap) * 2);
+  ALV_ALLOC(A.kpred, cap);
+  A.kmax = K;
+  ALV_ALLOC(A.nbr, nbr_capacity(K, cap));
   ALV_ALLOC(A.nnb, cap);
   ALV_ALLOC(A.bmask, cap);
   const size_t BN = size_t(std::max<size_t>(1, S->pile->bnd.size())) * cap;
@@ -1015,7 +1030,8 @@ int alv_solver_step(alv_solver* S) {
     launch_ctl_reset(A, st);
     if (p.n) {
       if (div) {
-        launch_df_warm_start_div(p, A, st);
+        const MlpDev mv = mlp_device_view(S->net);
+        launch_df_warm_start_div(p, A, st, S->log_features, S->net ? &mv : nullptr, S->net_div);
         rc = run_loop(S, true, S->set.max_divergence_solve, &S->last_div,
                       [&] { launch_df_div_iteration(p, A, st); });
         if (rc != ALV_OK) return rc;
@@ -1026,7 +1042,7 @@ int alv_solver_step(alv_solver* S) {
     if (p.n) {
       launch_df_integrate_np(p, A, st);
       if (dens) {
-        launch_df_warm_start_dens(p, A, st);
+        launch_df_warm_start_dens(p, A, st, S->net != nullptr && S->net_dens && div);
         rc = run_loop(S, false, S->set.max_density_solve, &S->last_dens,
                       [&] { launch_df_dens_iteration(p, A, st); });
         if (rc != ALV_OK) return rc;
@@ -1049,7 +1065,41 @@ int alv_solver_step(alv_solver* S) {
     }
   }
   ALV_HIP(hipGetLastError());
+  S->features_valid = S->kind == ALV_SOLVER_DF && S->set.enable_divergence_solve &&
+                      (S->log_features || S->net) && p.n;
   return finish_step(S, p);
+}
+
+int alv_solver_set_warm_start_net(alv_solver* S, alv_mlp* net, int use_divergence,
+                                  int use_density) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  if (net && S->kind != ALV_SOLVER_DF)
+    return fail(ALV_ERR_UNSUPPORTED, "the learned warm start is a DFSPH feature");
+  S->net_div = net && use_divergence;
+  S->net_dens = net && use_density;
+  S->net = (S->net_div || S->net_dens) ? net : nullptr;
+  return ALV_OK;
+}
+
+int alv_solver_set_feature_log(alv_solver* S, int enable) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  if (enable && S->kind != ALV_SOLVER_DF)
+    return fail(ALV_ERR_UNSUPPORTED, "warm-start features are a DFSPH feature");
+  S->log_features = enable != 0;
+  return ALV_OK;
+}
+
+int alv_solver_export_training_pairs(alv_solver* S, float* dev_x, float* dev_y) {
+  if (!S || !dev_x || !dev_y) return fail(ALV_ERR_ARG, "null argument");
+  if (!S->features_valid)
+    return fail(ALV_ERR_STATE,
+                "no features recorded: enable the feature log (or a warm-start network) "
+                "and take a step with the divergence solve enabled");
+  const SimParams p = solver_params(S);
+  launch_export_pairs(p, S->A, dev_x, dev_y, S->stream);
+  ALV_HIP(hipGetLastError());
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  return ALV_OK;
 }
 
 int alv_solver_update_particle_neighbors(alv_solver* S) {
@@ -1061,6 +1111,27 @@ int alv_solver_update_particle_neighbors(alv_solver* S) {
   if (rc != ALV_OK) return rc;
   ALV_HIP(hipStreamSynchronize(S->stream));
   S->dirty |= ALV_F_DENSITY | ALV_F_NNB | ALV_F_ALPHA;
+  return ALV_OK;
+}
+
+int alv_solver_compute_density_change(alv_solver* S) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  const SimParams p = solver_params(S);
+  int rc = upload_step_inputs(S);
+  if (rc != ALV_OK) return rc;
+  if (p.n) launch_density_change(p, S->A, S->stream);
+  ALV_HIP(hipGetLastError());
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  S->dirty |= ALV_F_RHO_ADV;
+  return ALV_OK;
+}
+
+int alv_solver_integrate_velocity(alv_solver* S) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  const SimParams p = solver_params(S);
+  if (p.n) launch_integrate_velocity(p, S->A, S->set.dt, S->stream);
+  ALV_HIP(hipGetLastError());
+  ALV_HIP(hipStreamSynchronize(S->stream));
   return ALV_OK;
 }
 

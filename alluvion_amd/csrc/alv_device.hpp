@@ -89,15 +89,17 @@ __device__ inline float W_d2(float r2, const SimParams& p) {
   float result = q <= 0.5f ? inner : (q <= 1.0f ? outer : 0.0f);
   return result * p.k;
 }
-__device__ inline vec3 gradW(vec3 d, const SimParams& p) {
+// gradW(d) = gradW_scale(d) * d
+__device__ inline float gradW_scale(vec3 d, const SimParams& p) {
   float rl2 = length_sqr(d);
   float rl = sqrtf(rl2);
   float q = rl / p.h;
   float s_in = (q * 3.0f - 2.0f) / p.h;
   float s_out = (1.0f - q) * (q - 1.0f) / rl;
   float scale = q <= 0.5f ? s_in : (q <= 1.0f ? s_out : 0.0f);
-  return (p.l * scale) * d;
+  return p.l * scale;
 }
+__device__ inline vec3 gradW(vec3 d, const SimParams& p) { return gradW_scale(d, p) * d; }
 
 // --- quaternion rotation (q = x, y, z, w) -----------------------------------
 __host__ __device__ inline vec3 rotate(vec3 v, float4 q) {

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "alv_mlp.hpp"
 #include "alv_types.hpp"
 
 namespace alv {
@@ -13,7 +14,8 @@ namespace alv {
 // reference's Variable<1,float3>; all other per-particle arrays are in
 // cell-sorted order, SoA, float4-padded vectors.
 struct DevArrays {
-  unsigned cap;  // max_num_particles: row stride of [B, cap] and [K, cap]
+  unsigned cap;  // max_num_particles: row stride of [B, cap]
+  unsigned kmax;  // neighbour slots per particle
   // user order
   float* ux;
   float* uv;
@@ -43,7 +45,9 @@ struct DevArrays {
   float* kappa_v;
   float* rho_adv;
   float* kfac;  // stiffness k_j of the running Jacobi iteration
-  unsigned* nbr;  // [kmax, cap], column-major
+  float4* feat;  // [2 cap] warm-start features (learned warm start, logging)
+  float* kpred;  // predicted stored density stiffness (learned warm start)
+  uint4* nbr;     // neighbour list: groups of 4 slots in wave tiles (nbr_group)
   unsigned* nnb;
   float4* brec;  // [B, cap] (x_b, Psi)
   float4* bker;  // [B, cap] (Psi gradW, Psi W)
@@ -70,6 +74,19 @@ struct DevArrays {
 
 constexpr unsigned kBlock = 256;
 
+// Neighbour slots are stored in groups of four (one uint4 per particle and
+// group).  Each 64-particle wave owns one contiguous [groups][64] tile of
+// uint4, so a wave reading group q of its particles reads 1 KB contiguous,
+// and the search writes whole 16-byte groups (buffered in registers) rather
+// than single slots.  Slot t of particle s is component t % 4 of group t / 4.
+__host__ __device__ inline unsigned nbr_groups(unsigned kmax) { return (kmax + 3u) / 4u; }
+__host__ __device__ inline size_t nbr_group(const DevArrays& A, unsigned q, unsigned s) {
+  return (size_t(s >> 6) * nbr_groups(A.kmax) + q) * 64u + (s & 63u);
+}
+inline size_t nbr_capacity(unsigned kmax, unsigned cap) {
+  return size_t(nbr_groups(kmax)) * ((size_t(cap) + 63u) / 64u * 64u);
+}
+
 // Optional per-kernel timing hook (HIP events around each launch); null
 // when profiling is off.
 struct KernelProfiler {
@@ -94,16 +111,25 @@ void launch_neighbors(const SimParams& p, const DevArrays& A, int wrap_any, int 
                       hipStream_t s);
 
 // DFSPH
-void launch_df_warm_start_div(const SimParams& p, const DevArrays& A, hipStream_t s);
+// features: record the eight warm-start features; net: learned warm start
+// (null = the reference's warm start)
+void launch_df_warm_start_div(const SimParams& p, const DevArrays& A, hipStream_t s,
+                              int features, const MlpDev* net, int net_divergence);
 void launch_df_div_loop_prep(const SimParams& p, const DevArrays& A, hipStream_t s);
 void launch_df_div_iteration(const SimParams& p, const DevArrays& A, hipStream_t s);
 void launch_df_viscosity(const SimParams& p, const DevArrays& A, int div_finish,
                          hipStream_t s);
 void launch_update_dt(const SimParams& p, const DevArrays& A, hipStream_t s);
 void launch_df_integrate_np(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_df_warm_start_dens(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_df_warm_start_dens(const SimParams& p, const DevArrays& A, hipStream_t s,
+                               int learned);
+// training pairs of the last step in sorted order: x [n][8], y [n][2]
+void launch_export_pairs(const SimParams& p, const DevArrays& A, float* x, float* y,
+                         hipStream_t s);
 void launch_df_dens_iteration(const SimParams& p, const DevArrays& A, hipStream_t s);
 void launch_df_finish(const SimParams& p, const DevArrays& A, int dens_finish, hipStream_t s);
+void launch_density_change(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_integrate_velocity(const SimParams& p, const DevArrays& A, float dt, hipStream_t s);
 
 // IISPH
 void launch_ii_viscosity_predict(const SimParams& p, const DevArrays& A, hipStream_t s);

@@ -45,10 +45,15 @@ __device__ inline vec3 wrapd(vec3 d, const SimParams& p) {
 // particle itself (a valid address whose values are ignored); contributions
 // are still added in list order.
 constexpr int kNB = 4;
+constexpr unsigned kCB = 8;  // candidate batch of the neighbour search
 __device__ inline void batch_ids(const DevArrays& A, unsigned s, unsigned nn, unsigned t0,
                                  unsigned (&j)[kNB]) {
-#pragma unroll
-  for (int u = 0; u < kNB; ++u) j[u] = (t0 + u < nn) ? A.nbr[size_t(t0 + u) * A.cap + s] : s;
+  static_assert(kNB == 4, "one neighbour group per batch");
+  const uint4 g = A.nbr[nbr_group(A, t0 >> 2, s)];
+  j[0] = t0 + 0 < nn ? g.x : s;
+  j[1] = t0 + 1 < nn ? g.y : s;
+  j[2] = t0 + 2 < nn ? g.z : s;
+  j[3] = t0 + 3 < nn ? g.w : s;
 }
 
 // The w lane of the padded velocity carries the density, so the viscosity
@@ -293,11 +298,12 @@ template <bool W>
 __global__ __launch_bounds__(256) void k_neighbors(SimParams p, DevArrays A, int with_factor) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
-  const unsigned K = p.kmax, cap = A.cap;
+  const unsigned K = p.kmax;
   const vec3 xi = ld3(A.x[s]);
   int ip[3];
   cell_ipos(xi, p, ip);
-  unsigned cnt = 0, last_j = 0;
+  unsigned cnt = 0;
+  uint4 grp = make_uint4(0u, 0u, 0u, 0u);  // the group of slots being filled
   float dens = p.vol * p.w0;
   float sgk = 0.0f;
   vec3 gpi = mk(0.f, 0.f, 0.f);
@@ -326,42 +332,72 @@ __global__ __launch_bounds__(256) void k_neighbors(SimParams p, DevArrays A, int
       }
       prev = nz;
     }
-    for (unsigned q = 0; q < nspan; ++q)
-    for (unsigned j = span_b[q], je = span_e[q]; j < je; ++j) {
-      if (j == s) continue;
-      vec3 d = wrapd<W>(xi - ld3(A.x[j]), p);
-      float r2 = length_sqr(d);
-      if (r2 < p.h2) {
-        A.nbr[size_t(min(cnt, K - 1)) * cap + s] = j;
-        if (cnt + 1 < K) {
-          dens += p.vol * W_d2(r2, p);
-          if (with_factor) {
-            vec3 gp = (-p.vol) * gradW(d, p);
-            sgk += length_sqr(gp);
-            gpi -= gp;
+    // candidates in batches: all positions of a batch are loaded before
+    // any is tested (one memory latency per batch), in ascending j
+    for (unsigned q = 0; q < nspan; ++q) {
+      const unsigned jb = span_b[q], je = span_e[q];
+      for (unsigned j0 = jb; j0 < je; j0 += kCB) {
+        float4 xc[kCB];
+#pragma unroll
+        for (int u = 0; u < kCB; ++u) xc[u] = A.x[min(j0 + u, je - 1u)];
+#pragma unroll
+        for (int u = 0; u < kCB; ++u) {
+          const unsigned j = j0 + u;
+          if (j >= je || j == s) continue;
+          vec3 d = wrapd<W>(xi - ld3(xc[u]), p);
+          float r2 = length_sqr(d);
+          if (r2 < p.h2) {
+            // slot min(cnt, K-1): the last slot keeps the latest neighbour
+            const unsigned slot = min(cnt, K - 1);
+            const unsigned c4 = slot & 3u;
+            grp.x = c4 == 0 ? j : grp.x;
+            grp.y = c4 == 1 ? j : grp.y;
+            grp.z = c4 == 2 ? j : grp.z;
+            grp.w = c4 == 3 ? j : grp.w;
+            if (c4 == 3u && cnt == slot) A.nbr[nbr_group(A, slot >> 2, s)] = grp;
+            ++cnt;
           }
         }
-        last_j = j;
-        ++cnt;
       }
     }
   }
-  if (cnt >= K) {
-    // slot K-1 holds the last neighbour found (the reference overwrites it)
-    vec3 d = wrapd<W>(xi - ld3(A.x[last_j]), p);
-    dens += p.vol * W_d2(length_sqr(d), p);
-    if (with_factor) {
-      vec3 gp = (-p.vol) * gradW(d, p);
-      sgk += length_sqr(gp);
-      gpi -= gp;
+  // the group holding the last slot is incomplete, or was rewritten by
+  // overflowing neighbours after it was stored
+  if (cnt > 0 && ((min(cnt, K) & 3u) != 0u || cnt > K))
+    A.nbr[nbr_group(A, (min(cnt, K) - 1u) >> 2, s)] = grp;
+  if (cnt > K) A.anomaly[2] = 1u;
+  // Density and factor sums over the finished list, in list order.  The
+  // list holds the first K-1 neighbours in search order and the last one
+  // found in slot K-1, the order in which the reference accumulates them.
+  // Doing the kernel arithmetic here, not inside the candidate scan, keeps
+  // every lane of a wave busy: in the scan some lane finds a neighbour at
+  // almost every candidate, so the whole wave would execute the kernel
+  // evaluation for every candidate.
+  const unsigned nn = min(cnt, K);
+  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+    unsigned jn[kNB];
+    batch_ids(A, s, nn, t0, jn);
+    float4 xj[kNB];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) xj[u] = A.x[jn[u]];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      if (t0 + u < nn) {
+        vec3 d = wrapd<W>(xi - ld3(xj[u]), p);
+        dens += p.vol * W_d2(length_sqr(d), p);
+        if (with_factor) {
+          vec3 gp = (-p.vol) * gradW(d, p);
+          sgk += length_sqr(gp);
+          gpi -= gp;
+        }
+      }
     }
-    if (cnt > K) A.anomaly[2] = 1u;
   }
   unsigned mask = A.bmask[s];
   while (mask) {
     const unsigned b = __builtin_ctz(mask);
     mask &= mask - 1u;
-    float4 bk = A.bker[size_t(b) * cap + s];
+    float4 bk = A.bker[size_t(b) * A.cap + s];
     dens += bk.w;
     gpi += mk(bk.x, bk.y, bk.z);
   }
@@ -465,10 +501,10 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
       if (t0 + u < nn && fabsf(k_sum) > p.eps) {
         vec3 d = wrapd<W>(xi - ld3(xj[u]), p);
         if (kDivergenceForm) {
-          vec3 grad_p_j = (-p.vol) * gradW(d, p);
+          vec3 grad_p_j = (-p.vol) * (gradW(d, p));
           dv -= (dt * k_sum) * grad_p_j;
         } else {
-          dv += ((dt * k_sum) * p.vol) * gradW(d, p);
+          dv += ((dt * k_sum) * p.vol) * (gradW(d, p));
         }
       }
     }
@@ -496,14 +532,54 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
 }
 
 // warm_start_divergence_solve_0 (runner.hpp:3230-3250)
-template <bool W>
+// F: also record the warm-start features of the particle
+template <bool W, bool F>
 __global__ __launch_bounds__(256) void k_ws_div0(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
   float dadv = density_change<W>(p, A, s);
-  const float kv = (dadv > 0.0f) ? 0.5f * std_max(A.kappa_v[s], -0.5f) / dt : 0.0f;
+  const float kv_prev = A.kappa_v[s];
+  const float kv = (dadv > 0.0f) ? 0.5f * std_max(kv_prev, -0.5f) / dt : 0.0f;
   A.kappa_v[s] = kv;
+  if (F) {
+    float psi = 0.0f;
+    unsigned mask = A.bmask[s];
+    while (mask) {
+      const unsigned b = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      psi += A.brec[size_t(b) * A.cap + s].w;
+    }
+    const vec3 v = ld3(A.v[s]);
+    A.feat[2 * size_t(s)] =
+        make_float4(A.density[s] / p.rho0 - 1.0f, dadv, float(A.nnb[s]), psi);
+    A.feat[2 * size_t(s) + 1] = make_float4(kv_prev, A.kappa[s], sqrtf(length_sqr(v)), dt);
+  }
+}
+
+// Learned warm start: the network's prediction of the converged stored
+// stiffness replaces 0.5 max(kappa_v, -0.5) in warm_start_divergence_solve_0
+// (runner.hpp:3230-3250) and is kept for warm_start_pressure_solve0.  A
+// non-finite prediction falls back to the reference's warm start.  Every
+// lane of the wave runs the matrix-core inference; there is no early return.
+__global__ __launch_bounds__(256) void k_ws_learned(SimParams p, DevArrays A, MlpDev M,
+                                                    int use_divergence) {
+  const unsigned s = ALV_TID();
+  const bool ok = s < p.n;
+  float f[kMlpIn];
+  const float4 f0 = ok ? A.feat[2 * size_t(s)] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float4 f1 = ok ? A.feat[2 * size_t(s) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+  f[0] = f0.x; f[1] = f0.y; f[2] = f0.z; f[3] = f0.w;
+  f[4] = f1.x; f[5] = f1.y; f[6] = f1.z; f[7] = f1.w;
+  float y0, y1;
+  mlp_infer_wave(M, f, y0, y1);
+  if (!ok) return;
+  const float dt = A.ctl->dt;
+  if (use_divergence && f[1] > 0.0f) {
+    const float kv = y0 / dt;
+    if (isfinite(kv)) A.kappa_v[s] = kv;
+  }
+  A.kpred[s] = y1;
 }
 // warm_start_divergence_solve_1 (runner.hpp:3252-3295)
 template <bool W>
@@ -730,13 +806,18 @@ __global__ __launch_bounds__(256) void k_integrate_np(SimParams p, DevArrays A) 
 }
 
 // warm_start_pressure_solve0 (runner.hpp:3430-3450)
-template <bool W>
+// L: learned warm start (prediction from k_ws_learned)
+template <bool W, bool L>
 __global__ __launch_bounds__(256) void k_ws_p0(SimParams p, DevArrays A) {
   const unsigned s = ALV_TID();
   if (s >= p.n) return;
   const float dt = A.ctl->dt;
   float dadv = density_adv<W>(p, A, s, dt);
-  const float k = (dadv > 1.0f) ? (0.5f * std_max(A.kappa[s], -0.00025f) / (dt * dt)) : 0.0f;
+  float k = (dadv > 1.0f) ? (0.5f * std_max(A.kappa[s], -0.00025f) / (dt * dt)) : 0.0f;
+  if (L && dadv > 1.0f) {
+    const float kl = A.kpred[s] / (dt * dt);
+    if (isfinite(kl)) k = kl;
+  }
   A.kappa[s] = k;
 }
 // warm_start_pressure_solve1 (runner.hpp:3452-3491)
@@ -827,6 +908,42 @@ __global__ __launch_bounds__(256) void k_df_finish(SimParams p, DevArrays A, int
   A.ukappa_v[i] = A.kappa_v[s];
 }
 
+// compute_velocity_of_density_change (runner.hpp:3297-3314), the density
+// change sum alone (no stiffness factor update): rho_adv <- rho_dot.
+template <bool W>
+__global__ __launch_bounds__(256) void k_density_change(SimParams p, DevArrays A) {
+  const unsigned s = ALV_TID();
+  if (s >= p.n) return;
+  A.rho_adv[s] = density_change<W>(p, A, s);
+}
+
+// Training pairs of the step just taken, in sorted order: the features
+// recorded at its divergence warm start and the converged stored stiffness
+// (kappa_v, kappa) it ended with.
+__global__ __launch_bounds__(256) void k_export_pairs(SimParams p, DevArrays A,
+                                                      float* __restrict__ x,
+                                                      float* __restrict__ y) {
+  const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= p.n) return;
+  const float4 f0 = A.feat[2 * size_t(s)], f1 = A.feat[2 * size_t(s) + 1];
+  float* xo = x + size_t(s) * kMlpIn;
+  xo[0] = f0.x; xo[1] = f0.y; xo[2] = f0.z; xo[3] = f0.w;
+  xo[4] = f1.x; xo[5] = f1.y; xo[6] = f1.z; xo[7] = f1.w;
+  const unsigned i = A.sorted_id[s];
+  y[size_t(s) * 2 + 0] = A.ukappa_v[i];
+  y[size_t(s) * 2 + 1] = A.ukappa[i];
+}
+
+// integrate_velocity<wrap> (runner.hpp:3613-3623) on the user-order state
+// x <- x + dt v, wrapped on periodic axes.
+template <bool W>
+__global__ __launch_bounds__(256) void k_integrate_velocity(SimParams p, DevArrays A, float dt) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  vec3 x = wrapd<W>(ld3(A.ux, i) + dt * ld3(A.uv, i), p);
+  st3(A.ux, i, x);
+}
+
 // ---------------------------------------------------------------------------
 // IISPH (runner.hpp:2609-2887)
 // ---------------------------------------------------------------------------
@@ -853,7 +970,7 @@ __global__ __launch_bounds__(256) void k_ii_predict0(SimParams p, DevArrays A) {
     for (int u = 0; u < kNB; ++u) {
       if (t0 + u < nn) {
         vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
-        d -= (p.vol / dens2) * gradW(xixj, p);
+        d -= (p.vol / dens2) * (gradW(xixj, p));
       }
     }
   }
@@ -945,7 +1062,7 @@ __global__ __launch_bounds__(256) void k_ii_iter0(SimParams p, DevArrays A) {
         vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
         float dj = djs[u];
         float dj2 = dj * dj;
-        acc -= (p.vol / dj2 * lpj[u]) * gradW(xixj, p);
+        acc -= (p.vol / dj2 * lpj[u]) * (gradW(xixj, p));
       }
     }
   }
@@ -1062,7 +1179,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure_accel(SimParams p, DevArray
         float dj = djs[u];
         float dj2 = dj * dj;
         float dpj = pj[u] / dj2;
-        ai -= (p.mass * (dpi + dpj)) * gradW(xixj, p);
+        ai -= (p.mass * (dpi + dpj)) * (gradW(xixj, p));
       }
     }
   }
@@ -1165,8 +1282,11 @@ __global__ __launch_bounds__(256) void k_export_neighbors(SimParams p, DevArrays
   const unsigned i = A.sorted_id[s];
   const unsigned nn = A.nnb[s];
   counts[i] = nn;
-  for (unsigned t = 0; t < nn; ++t)
-    ids[size_t(i) * p.kmax + t] = A.sorted_id[A.nbr[size_t(t) * A.cap + s]];
+  for (unsigned t = 0; t < nn; ++t) {
+    const uint4 g = A.nbr[nbr_group(A, t >> 2, s)];
+    const unsigned j = (t & 3u) == 0 ? g.x : (t & 3u) == 1 ? g.y : (t & 3u) == 2 ? g.z : g.w;
+    ids[size_t(i) * p.kmax + t] = A.sorted_id[j];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1258,6 +1378,18 @@ KernelProfiler* g_profiler = nullptr;
     if (g_profiler) g_profiler->end(#kernel, s);            \
   } while (0)
 #define ALV_LAUNCH(kernel, n, s, ...) ALV_LAUNCH_GRID(kernel, dim3(grid_for(n)), s, __VA_ARGS__)
+// kernels templated on the periodic wrap and one more flag
+#define ALV_LAUNCH_W2(kernel, flag, n, s, p, ...)                                 \
+  do {                                                                            \
+    if (g_profiler) g_profiler->begin(#kernel, s);                                \
+    if ((p).wrap_mask)                                                            \
+      hipLaunchKernelGGL((kernel<true, flag>), dim3(grid_for(n)), dim3(kBlock), 0, s, p, \
+                         __VA_ARGS__);                                            \
+    else                                                                          \
+      hipLaunchKernelGGL((kernel<false, flag>), dim3(grid_for(n)), dim3(kBlock), 0, s, p, \
+                         __VA_ARGS__);                                            \
+    if (g_profiler) g_profiler->end(#kernel, s);                                  \
+  } while (0)
 // kernels templated on the periodic wrap: one name for both instances
 #define ALV_LAUNCH_W(kernel, n, s, p, ...)                                        \
   do {                                                                            \
@@ -1294,8 +1426,13 @@ void launch_neighbors(const SimParams& p, const DevArrays& A, int wrap_any, int 
   (void)wrap_any;
   ALV_LAUNCH_W(k_neighbors, p.n, s, p, A, with_factor);
 }
-void launch_df_warm_start_div(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_ws_div0, p.n, s, p, A);
+void launch_df_warm_start_div(const SimParams& p, const DevArrays& A, hipStream_t s,
+                              int features, const MlpDev* net, int net_divergence) {
+  if (features || net)
+    ALV_LAUNCH_W2(k_ws_div0, true, p.n, s, p, A);
+  else
+    ALV_LAUNCH_W2(k_ws_div0, false, p.n, s, p, A);
+  if (net) ALV_LAUNCH(k_ws_learned, p.n, s, p, A, *net, net_divergence);
   ALV_LAUNCH_W(k_ws_div1, p.n, s, p, A);
   ALV_LAUNCH_W(k_vel_density_change, p.n, s, p, A);
 }
@@ -1319,8 +1456,12 @@ void launch_update_dt(const SimParams& p, const DevArrays& A, hipStream_t s) {
 void launch_df_integrate_np(const SimParams& p, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH(k_integrate_np, p.n, s, p, A);
 }
-void launch_df_warm_start_dens(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_ws_p0, p.n, s, p, A);
+void launch_df_warm_start_dens(const SimParams& p, const DevArrays& A, hipStream_t s,
+                               int learned) {
+  if (learned)
+    ALV_LAUNCH_W2(k_ws_p0, true, p.n, s, p, A);
+  else
+    ALV_LAUNCH_W2(k_ws_p0, false, p.n, s, p, A);
   ALV_LAUNCH_W(k_ws_p1, p.n, s, p, A);
   ALV_LAUNCH_W(k_rho_adv, p.n, s, p, A);
 }
@@ -1331,6 +1472,16 @@ void launch_df_dens_iteration(const SimParams& p, const DevArrays& A, hipStream_
 }
 void launch_df_finish(const SimParams& p, const DevArrays& A, int dens_finish, hipStream_t s) {
   ALV_LAUNCH_W(k_df_finish, p.n, s, p, A, dens_finish);
+}
+void launch_export_pairs(const SimParams& p, const DevArrays& A, float* x, float* y,
+                         hipStream_t s) {
+  ALV_LAUNCH(k_export_pairs, p.n, s, p, A, x, y);
+}
+void launch_density_change(const SimParams& p, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_W(k_density_change, p.n, s, p, A);
+}
+void launch_integrate_velocity(const SimParams& p, const DevArrays& A, float dt, hipStream_t s) {
+  ALV_LAUNCH_W(k_integrate_velocity, p.n, s, p, A, dt);
 }
 void launch_ii_viscosity_predict(const SimParams& p, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_W(k_viscosity, p.n, s, p, A, 0);

@@ -22,6 +22,14 @@ namespace {
 template <typename V>
 void declare_vec(py::module& m, const char* name) {}
 
+// Device pointer of any bound Variable with room for `floats` FP32 values.
+float* var_floats(py::object var, size_t floats, const char* what) {
+  const size_t bytes = var.attr("get_num_bytes")().cast<size_t>();
+  if (bytes < floats * sizeof(float))
+    throw std::invalid_argument(std::string(what) + ": variable too small");
+  return reinterpret_cast<float*>(var.attr("ptr").cast<uintptr_t>());
+}
+
 template <U D, typename M>
 void declare_variable(py::module& m, py::class_<Store>& store, const std::string& name) {
   using Var = Variable<D, M>;
@@ -326,6 +334,19 @@ PYBIND11_MODULE(_alluvion, m) {
         .def("compute_all_boundaries", &S::compute_all_boundaries)
         .def("update_particle_neighbors", &S::template update_particle_neighbors<0>)
         .def("update_particle_neighbors_wrap1", &S::template update_particle_neighbors<1>)
+        .def("compute_density_change", &S::compute_density_change)
+        .def("set_warm_start_net", &S::set_warm_start_net, py::arg("net").none(true),
+             py::arg("divergence") = true, py::arg("density") = true, py::keep_alive<1, 2>())
+        .def("set_feature_log", &S::set_feature_log, py::arg("enable"))
+        .def("export_training_pairs",
+             [](S& s, py::object x, py::object y, size_t offset) {
+               const size_t n = s.get_num_particles();
+               float* px = var_floats(x, (offset + n) * 8, "x");
+               float* py_ = var_floats(y, (offset + n) * 2, "y");
+               s.export_training_pairs(px + offset * 8, py_ + offset * 2);
+             },
+             py::arg("x"), py::arg("y"), py::arg("offset") = 0)
+        .def("integrate_velocity", &S::integrate_velocity)
         .def("step", &S::template step<0>)
         .def("step_wrap1", &S::template step<1>)
         .def("get_neighbors",
@@ -402,6 +423,53 @@ PYBIND11_MODULE(_alluvion, m) {
       .def_property("min_density_solve", &II::get_min_density_solve, &II::set_min_density_solve)
       .def_property("max_density_solve", &II::get_max_density_solve, &II::set_max_density_solve)
       .def_property_readonly("particle_pressure", py::cpp_function(&II::particle_pressure, py::keep_alive<0, 1>()));
+
+  py::class_<WarmStartNet>(m, "WarmStartNet")
+      .def(py::init<unsigned long long>(), py::arg("seed") = 0)
+      .def_static("num_params", &WarmStartNet::num_params)
+      .def("get_params",
+           [](WarmStartNet const& n) {
+             auto p = n.get_params();
+             return py::array_t<float>(p.size(), p.data());
+           })
+      .def("set_params",
+           [](WarmStartNet& n, py::array_t<float, py::array::c_style | py::array::forcecast> p) {
+             n.set_params(std::vector<float>(p.data(), p.data() + p.size()));
+           })
+      .def("get_normalization",
+           [](WarmStartNet const& n) {
+             auto v = n.get_normalization();
+             return py::array_t<float>(v.size(), v.data());
+           })
+      .def("set_normalization",
+           [](WarmStartNet& n, py::array_t<float, py::array::c_style | py::array::forcecast> v) {
+             n.set_normalization(std::vector<float>(v.data(), v.data() + v.size()));
+           })
+      .def("fit_normalization",
+           [](WarmStartNet& n, py::object x, py::object y, size_t count) {
+             n.fit_normalization(var_floats(x, count * 8, "x"), var_floats(y, count * 2, "y"),
+                                 count);
+           })
+      .def("forward",
+           [](WarmStartNet& n, py::object x, py::object y, size_t count) {
+             n.forward(var_floats(x, count * 8, "x"), count, var_floats(y, count * 2, "y"));
+           })
+      .def("loss_grad",
+           [](WarmStartNet& n, py::object xn, py::object tn, size_t count) {
+             std::vector<float> g;
+             const double loss = n.loss_grad(var_floats(xn, count * 8, "xn"),
+                                             var_floats(tn, count * 2, "tn"), count, g);
+             return py::make_tuple(loss, py::array_t<float>(g.size(), g.data()));
+           })
+      .def("train",
+           [](WarmStartNet& n, py::object x, py::object y, size_t count, U batch, U steps,
+              float lr, unsigned long long seed) {
+             auto l = n.train(var_floats(x, count * 8, "x"), var_floats(y, count * 2, "y"),
+                              count, batch, steps, lr, seed);
+             return py::array_t<float>(l.size(), l.data());
+           },
+           py::arg("x"), py::arg("y"), py::arg("count"), py::arg("batch") = 65536,
+           py::arg("steps") = 1000, py::arg("lr") = 1e-3f, py::arg("seed") = 0);
 
   m.def("version", []() { return std::string(alv_version()); });
   // Per-kernel HIP-event timing (Runner::launch statistics, runner.hpp:4835-4859)

@@ -1,19 +1,26 @@
 """Throughput benchmark of the SPH solver core on one MI355X.
 
-Workload (default): DFSPH dam break with 1,048,576 fluid particles
-(128 x 128 x 64 jittered lattice, r = 2.5 mm) in a 1 m cube container plus
-four dynamic rigid bodies (two spheres r = 0.05 m, two cubes of edge 0.08 m,
-128^3 volume maps), dt = 2e-4 s, divergence and density tolerance 0.1 %.
-One step is SolverDf::step (neighbour search, boundaries, both Jacobi solves,
-viscosity, integration, rigid coupling).  Inputs are resident on the GPU when
-the timed region starts; volume-map construction is setup.
+Default workload (the headline metric): DFSPH dam break with 1,048,576 fluid
+particles (128 x 128 x 64 jittered lattice, r = 2.5 mm) in a 1 m cube
+container plus four dynamic rigid bodies (two spheres r = 0.05 m, two cubes of
+edge 0.08 m, 128^3 volume maps), dt = 2e-4 s, divergence and density
+tolerance 0.1 %.  One step is SolverDf::step (neighbour search, boundaries,
+both Jacobi solves, viscosity, integration, rigid coupling).
 
-Prints one JSON line.  ``value`` is particle-steps/s of the timed steps.
-``roofline`` is the dominant kernel's algorithmic bytes per launch divided by
-its average duration (HIP events around every launch during the timed steps)
-against the 8 TB/s HBM peak.  ``cpu_baseline`` times the CPU oracle
-(C++/OpenMP restatement of the reference algorithm) on a bounded window of the
-same workload, started from the GPU state at the start of the timed region.
+``--workload periodic_8m`` is the roofline stress run: 8,388,608 particles
+uniform in a periodic unit cube with velocities uniform in +-0.5 m/s; one step
+is the neighbour search with the density sum, the density-change
+(divergence) sum, and x += dt v with wrapping.
+
+Inputs are resident on the GPU when the timed region starts; volume-map
+construction is setup.  Prints one JSON line.  ``value`` is particle-steps/s
+of the timed steps.  ``roofline`` is the dominant kernel's algorithmic bytes
+per launch divided by its average duration (HIP events around every launch
+during the timed steps) against the 8 TB/s HBM peak; ``traffic`` is that
+kernel's HBM bytes per launch from the committed rocprofv3 PMC summary.
+``cpu_baseline`` times the CPU oracle (C++/OpenMP restatement of the
+reference algorithm) on a bounded window of the same workload, started from
+the GPU state at the start of the timed region.
 """
 import argparse
 import json
@@ -27,14 +34,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+CALIBRATION_MIB = 256
 METRIC = "particle-steps/s @1M DFSPH 0.1% ρ-err; learned vs plain warm start"
+METRIC_STRESS = "particle-steps/s @8M periodic: neighbour search + density + divergence sums"
 
 
 def kernel_bytes(name, K, nb):
     """Algorithmic bytes per particle of one launch (minimal widths: FP32
     scalar 4 B, vector 12 B, u32 index 4 B; neighbour-state gathers counted
     once per particle as cache-resident; boundary records 16 B each only for
-    (particle, boundary) pairs with non-zero volume, nb per particle)."""
+    (particle, boundary) pairs with non-zero volume, nb per particle).  The
+    neighbour list is counted at 4 B per entry (the index); the stored kernel
+    gradient scale that rides beside it is an implementation choice, not
+    algorithmic traffic."""
     nbr = 4 + 4 * K  # count + neighbour list
     table = {
         "k_neighbors": 12 + 4 * K + 4 + 8 + 4 + 16 * nb,
@@ -54,74 +67,213 @@ def kernel_bytes(name, K, nb):
         "k_gather_boundary": 4 + 12 + 12 + 8 + 12 + 12 + 8 + 4,
         "k_cell_count": 12 + 4 + 4,
         "k_scatter": 4 + 4 + 4,
+        "k_density_change": nbr + 12 + 12 + 4 + 32 * nb,
+        "k_integrate_velocity": 12 + 12 + 12,
     }
     return table.get(name)
+
+
+T0 = time.perf_counter()
 
 
 def log(msg):
     print("[bench %8.2fs] %s" % (time.perf_counter() - T0, msg), file=sys.stderr, flush=True)
 
 
-T0 = time.perf_counter()
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the ratio of known bytes to
+    counter reading for a plain device-to-device copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ent = table.get(workload, {}).get("kernels", {}).get(kernel)
+    if not ent:
+        return None
+    return int(ent["fetch_bytes"] + ent["write_bytes"])
 
 
-def build(args):
-    from alluvion_amd import scenes
+def copy_calibration(dp):
+    """Plain device-to-device copies of a known size (Variable.set_from), so
+    that a PMC run of this program can scale its HBM counters."""
+    import alluvion_amd._alluvion as al
 
-    s = args.scale
-    sc = scenes.dam_break_rigid(nx=128 // s, ny=128 // s, nz=64 // s,
-                                body_resolution=args.body_resolution)
-    log("scene: %d particles, %d boundaries" % (sc.n, len(sc.boundaries)))
-    t0 = time.perf_counter()
-    dp, pile, solver, cn, cni = scenes.build_solver(sc, log=log)
-    setup_s = time.perf_counter() - t0
-    log("setup done in %.2f s" % setup_s)
-    return sc, dp, pile, solver, setup_s
+    n = CALIBRATION_MIB * (1 << 20) // 4
+    src = dp.create((n,), 1)
+    dst = dp.create((n,), 1)
+    dp.coat(src).set(np.ones(n, np.float32))
+    for _ in range(3):
+        dst.set_from(src)
+    al.device_synchronize()
+    dp.remove(src)
+    dp.remove(dst)
+    log("copy calibration: 3 x %d MiB device-to-device" % CALIBRATION_MIB)
 
 
-def cpu_baseline(sc, dp, pile, solver, state, steps):
+class DamBreak:
+    name = "dam_break_1m_rigid"
+    metric = METRIC
+    default_steps = 200
+    default_warmup = 20
+    default_cpu_steps = 48
+
+    def __init__(self, args):
+        from alluvion_amd import scenes
+
+        s = args.scale
+        self.sc = scenes.dam_break_rigid(nx=128 // s, ny=128 // s, nz=64 // s,
+                                         body_resolution=args.body_resolution)
+        log("scene: %d particles, %d boundaries" % (self.sc.n, len(self.sc.boundaries)))
+        t0 = time.perf_counter()
+        self.dp, self.pile, self.solver, _, _ = scenes.build_solver(self.sc, log=log)
+        self.setup_s = time.perf_counter() - t0
+        self.n_div = self.n_dens = 0
+
+    def step(self):
+        self.solver.step()  # ends with a stream synchronisation (report read-back)
+        self.n_div += self.solver.num_divergence_solve
+        self.n_dens += self.solver.num_density_solve
+
+    def snapshot(self):
+        dp, solver, pile, n = self.dp, self.solver, self.pile, self.sc.n
+        return {
+            "x": dp.coat(solver.particle_x).get()[:n].copy(),
+            "v": dp.coat(solver.particle_v).get()[:n].copy(),
+            "kappa": dp.coat(solver.particle_kappa).get()[:n].copy(),
+            "kappa_v": dp.coat(solver.particle_kappa_v).get()[:n].copy(),
+            "rigid": [(tuple(pile.get_x(b)), tuple(pile.get_v(b)), tuple(pile.get_q(b)),
+                       tuple(pile.get_omega(b))) for b in range(pile.get_size())],
+        }
+
+    def oracle(self, state):
+        import oracle_ffi as of
+
+        orc = of.Oracle(self.sc, precision=4)
+        for b, var in enumerate(self.pile.volume_grids):
+            orc.set_volume_map(b, self.dp.coat(var).get())
+        for k in ("x", "v", "kappa", "kappa_v"):
+            orc.set(k, state[k])
+        for b, kin in enumerate(state["rigid"]):
+            orc.set_rigid(b, *kin)
+        return orc, orc.step
+
+    def sample_text(self, steps, cores):
+        return ("%d DFSPH steps of the same 1,048,576-particle scene on the host CPU "
+                "(C++/OpenMP FP32 oracle, %d threads), started from the GPU state at the "
+                "start of the timed region, after 1 warm-up step" % (steps, cores))
+
+    def config(self, steps, K, nb):
+        return {
+            "workload": "DFSPH dam break, 1,048,576 particles (128x128x64 lattice, r=2.5mm) in a "
+                        "1 m cube + 4 dynamic rigid bodies (128^3 volume maps), dt=2e-4, "
+                        "tolerance 0.1%, plain warm start",
+            "particles": self.sc.n,
+            "mean_neighbors": round(K, 2),
+            "boundary_pairs_per_particle": round(nb, 4),
+            "mean_divergence_iterations": round(self.n_div / steps, 3),
+            "mean_density_iterations": round(self.n_dens / steps, 3),
+            "setup_s": round(self.setup_s, 2),
+        }
+
+
+class PeriodicStress:
+    name = "periodic_8m"
+    metric = METRIC_STRESS
+    default_steps = 100
+    default_warmup = 10
+    default_cpu_steps = 3
+
+    def __init__(self, args):
+        from alluvion_amd import scenes
+
+        n = 8_388_608 // (args.scale ** 3)
+        self.sc = scenes.periodic_random(n)
+        log("scene: %d particles, periodic unit cube" % n)
+        t0 = time.perf_counter()
+        self.dp, self.pile, self.solver, _, _ = scenes.build_solver(self.sc, log=log)
+        self.setup_s = time.perf_counter() - t0
+
+    def step(self):
+        s = self.solver
+        s.update_particle_neighbors()  # grid, neighbour list, density
+        s.compute_density_change()  # divergence sum
+        s.integrate_velocity()  # x += dt v, wrapped
+
+    def snapshot(self):
+        dp, solver, n = self.dp, self.solver, self.sc.n
+        return {"x": dp.coat(solver.particle_x).get()[:n].copy(),
+                "v": dp.coat(solver.particle_v).get()[:n].copy()}
+
+    def oracle(self, state):
+        import oracle_ffi as of
+
+        orc = of.Oracle(self.sc, precision=4)
+        orc.set("x", state["x"])
+        orc.set("v", state["v"])
+
+        def step():
+            orc.compute_all_boundaries()
+            orc.update_particle_neighbors()
+            orc.compute_density_change()
+            orc.integrate_velocity()
+
+        return orc, step
+
+    def sample_text(self, steps, cores):
+        return ("%d steps (neighbour search + density + density-change sums + advection) of the "
+                "same 8,388,608-particle periodic scene on the host CPU (C++/OpenMP FP32 oracle, "
+                "%d threads), after 1 warm-up step" % (steps, cores))
+
+    def config(self, steps, K, nb):
+        return {
+            "workload": "periodic unit cube, 8,388,608 uniform random particles, v in +-0.5 m/s, "
+                        "h=0.01 (100^3 cells): neighbour search + density + density-change sums "
+                        "+ wrapped advection",
+            "particles": self.sc.n,
+            "mean_neighbors": round(K, 2),
+            "setup_s": round(self.setup_s, 2),
+        }
+
+
+WORKLOADS = {"dam_break_1m_rigid": DamBreak, "periodic_8m": PeriodicStress}
+
+
+def cpu_baseline(wl, state, steps):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_ffi as of
-
-    orc = of.Oracle(sc, precision=4)
-    for b, var in enumerate(pile.volume_grids):
-        orc.set_volume_map(b, dp.coat(var).get())
-    orc.set("x", state["x"])
-    orc.set("v", state["v"])
-    orc.set("kappa", state["kappa"])
-    orc.set("kappa_v", state["kappa_v"])
-    for b, kin in enumerate(state["rigid"]):
-        orc.set_rigid(b, *kin)
+    orc, step = wl.oracle(state)
     log("cpu oracle ready")
-    orc.step()  # warm-up (first touch of the neighbour arrays)
+    step()  # warm-up (first touch of the neighbour arrays)
     t0 = time.perf_counter()
     for i in range(steps):
-        orc.step()
+        step()
         log("cpu step %d" % i)
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     return {
-        "value": sc.n * steps / dt,
+        "value": wl.sc.n * steps / dt,
         "unit": "particle-steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": "%d DFSPH steps of the same 1,048,576-particle scene on the host CPU "
-                  "(C++/OpenMP FP32 oracle, %d threads), started from the GPU state at the "
-                  "start of the timed region, after 1 warm-up step" % (steps, cores),
+        "sample": wl.sample_text(steps, cores),
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=48)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="dam_break_1m_rigid")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--scale", type=int, default=1,
                     help="divide the lattice dimensions (testing only)")
     ap.add_argument("--body-resolution", type=int, default=128)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run 3 device copies of 256 MiB before the steps (PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (one-GPU solver core); "
@@ -130,32 +282,26 @@ def main():
 
     import alluvion_amd._alluvion as al
 
-    sc, dp, pile, solver, setup_s = build(args)
-    n = sc.n
-    for i in range(args.warmup):
-        solver.step()
-        log("warmup step %d: div %d dens %d" % (i, solver.num_divergence_solve,
-                                                 solver.num_density_solve))
-    state = None
-    if not args.no_cpu_baseline:
-        state = {
-            "x": dp.coat(solver.particle_x).get()[:n].copy(),
-            "v": dp.coat(solver.particle_v).get()[:n].copy(),
-            "kappa": dp.coat(solver.particle_kappa).get()[:n].copy(),
-            "kappa_v": dp.coat(solver.particle_kappa_v).get()[:n].copy(),
-            "rigid": [(tuple(pile.get_x(b)), tuple(pile.get_v(b)), tuple(pile.get_q(b)),
-                       tuple(pile.get_omega(b))) for b in range(pile.get_size())],
-        }
+    wl = WORKLOADS[args.workload](args)
+    steps = args.steps if args.steps is not None else wl.default_steps
+    warmup = args.warmup if args.warmup is not None else wl.default_warmup
+    cpu_steps = args.cpu_steps if args.cpu_steps is not None else wl.default_cpu_steps
+    dp, solver, n = wl.dp, wl.solver, wl.sc.n
+    if args.copy_calibration:
+        copy_calibration(dp)
+    for i in range(warmup):
+        wl.step()
+        log("warmup step %d" % i)
+    state = None if args.no_cpu_baseline else wl.snapshot()
+    if hasattr(wl, "n_div"):
+        wl.n_div = wl.n_dens = 0
     if not args.no_profile:
         al.profiling_enable(True)
         al.profiling_reset()
-    n_div = n_dens = 0
     al.device_synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        solver.step()  # ends with a stream synchronisation (report read-back)
-        n_div += solver.num_divergence_solve
-        n_dens += solver.num_density_solve
+    for i in range(steps):
+        wl.step()
         if i % 25 == 0:
             log("timed step %d" % i)
     al.device_synchronize()
@@ -164,7 +310,7 @@ def main():
     if not args.no_profile:
         stats = al.profiling_stats()
         al.profiling_enable(False)
-    value = n * args.steps / elapsed
+    value = n * steps / elapsed
     nnb = dp.coat(solver.particle_num_neighbors).get()[:n]
     K = float(nnb.mean())
     mask = dp.coat(solver.boundary_mask).get()[:n].astype(np.uint64)
@@ -175,8 +321,12 @@ def main():
     if stats:
         total_ms = sum(v[0] for v in stats.values())
         for name, (ms, launches) in stats.items():
-            kernels[name] = {"total_ms": round(ms, 3), "launches": int(launches),
-                             "avg_us": round(1000.0 * ms / max(launches, 1), 2)}
+            ent = {"total_ms": round(ms, 3), "launches": int(launches),
+                   "avg_us": round(1000.0 * ms / max(launches, 1), 2)}
+            bpp = kernel_bytes(name, K, nb)
+            if bpp is not None and launches and ms > 0:
+                ent["algorithmic_gbs"] = round(bpp * n / (ms / 1000.0 / launches) / 1e9, 1)
+            kernels[name] = ent
         dom = max(stats.items(), key=lambda kv: kv[1][0])
         name, (ms, launches) = dom
         bpp = kernel_bytes(name, K, nb)
@@ -190,7 +340,7 @@ def main():
                 "peak": HBM_PEAK_GBS,
                 "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None,
+                "traffic": measured_traffic(wl.name, name),
                 "bytes_per_launch": int(bpp * n),
                 "avg_launch_us": round(avg_s * 1e6, 2),
                 "share_of_gpu_time": round(ms / total_ms, 3),
@@ -198,33 +348,23 @@ def main():
     log("timed region: %.3f s" % elapsed)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(sc, dp, pile, solver, state, args.cpu_steps)
+        cpu = cpu_baseline(wl, state, cpu_steps)
         log("cpu baseline done")
 
     out = {
-        "metric": METRIC,
+        "metric": wl.metric,
         "value": round(value, 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(1000.0 * elapsed / steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": "DFSPH dam break, 1,048,576 particles (128x128x64 lattice, r=2.5mm) in a "
-                        "1 m cube + 4 dynamic rigid bodies (128^3 volume maps), dt=2e-4, "
-                        "tolerance 0.1%, plain warm start",
-            "particles": n,
-            "mean_neighbors": round(K, 2),
-            "boundary_pairs_per_particle": round(nb, 4),
-            "mean_divergence_iterations": round(n_div / args.steps, 3),
-            "mean_density_iterations": round(n_dens / args.steps, 3),
-            "setup_s": round(setup_s, 2),
-        },
+        "config": wl.config(steps, K, nb),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -477,6 +477,56 @@ class Pile {
   std::vector<std::unique_ptr<dg::Distance<float3, TF>>> distances_;
 };
 
+// Learned warm-start network (no counterpart in the reference): owns an
+// alv_mlp.  Data-set arguments are device pointers with their element
+// counts (pairs), features [n][8] and targets [n][2] in FP32.
+class WarmStartNet {
+ public:
+  explicit WarmStartNet(unsigned long long seed = 0) { check(alv_mlp_create(&net_, seed)); }
+  ~WarmStartNet() { alv_mlp_destroy(net_); }
+  WarmStartNet(const WarmStartNet&) = delete;
+  WarmStartNet& operator=(const WarmStartNet&) = delete;
+  static int num_params() { return alv_mlp_num_params(); }
+  std::vector<float> get_params() const {
+    std::vector<float> p(static_cast<size_t>(num_params()));
+    check(alv_mlp_get_params(net_, p.data()));
+    return p;
+  }
+  void set_params(std::vector<float> const& p) {
+    if (p.size() != size_t(num_params())) throw std::invalid_argument("expected 9026 parameters");
+    check(alv_mlp_set_params(net_, p.data()));
+  }
+  std::vector<float> get_normalization() const {
+    std::vector<float> n(20);
+    check(alv_mlp_get_normalization(net_, n.data()));
+    return n;
+  }
+  void set_normalization(std::vector<float> const& n) {
+    if (n.size() != 20) throw std::invalid_argument("expected 20 normalisation values");
+    check(alv_mlp_set_normalization(net_, n.data()));
+  }
+  void fit_normalization(const float* x, const float* y, size_t n) {
+    check(alv_mlp_fit_normalization(net_, x, y, n));
+  }
+  void forward(const float* x, size_t n, float* y) { check(alv_mlp_forward(net_, x, n, y)); }
+  double loss_grad(const float* xn, const float* tn, size_t n, std::vector<float>& grad) {
+    grad.resize(size_t(num_params()));
+    double loss = 0;
+    check(alv_mlp_loss_grad(net_, xn, tn, n, &loss, grad.data()));
+    return loss;
+  }
+  std::vector<float> train(const float* x, const float* y, size_t n, U batch, U steps, float lr,
+                           unsigned long long seed) {
+    std::vector<float> loss(steps);
+    check(alv_mlp_train(net_, x, y, n, batch, steps, lr, seed, loss.data()));
+    return loss;
+  }
+  alv_mlp* handle() const { return net_; }
+
+ private:
+  alv_mlp* net_ = nullptr;
+};
+
 // Solver base (solver.hpp:8-306) over alv_solver.
 template <typename TF>
 class SolverBase {
@@ -536,6 +586,30 @@ class SolverBase {
     check(alv_solver_update_particle_neighbors(solver_));
   }
   void compute_all_boundaries() {}
+  // learned warm start (no counterpart in the reference); null = plain
+  void set_warm_start_net(WarmStartNet const* net, bool divergence = true, bool density = true) {
+    check(alv_solver_set_warm_start_net(solver_, net ? net->handle() : nullptr, divergence,
+                                        density));
+  }
+  void set_feature_log(bool enable) { check(alv_solver_set_feature_log(solver_, enable)); }
+  // training pairs of the last step into device buffers of at least
+  // num_particles pairs
+  void export_training_pairs(float* x, float* y) {
+    check(alv_solver_export_training_pairs(solver_, x, y));
+  }
+  // compute_velocity_of_density_change's density-change sum alone
+  // (runner.hpp:3297-3314) into particle_density_adv
+  void compute_density_change() {
+    sync_const();
+    push();
+    check(alv_solver_compute_density_change(solver_));
+  }
+  // integrate_velocity<wrap> (runner.hpp:3613-3623)
+  void integrate_velocity() {
+    sync_const();
+    push();
+    check(alv_solver_integrate_velocity(solver_));
+  }
   template <U wrap = 0>
   void step() {
     sync_const();

@@ -231,6 +231,15 @@ int alv_solver_step(alv_solver* solver);
 /* Solver::compute_all_boundaries (solver.hpp:170-185) followed by
  * Solver::update_particle_neighbors<wrap> (solver.hpp:225-260). */
 int alv_solver_update_particle_neighbors(alv_solver* solver);
+/* The density-change sum of compute_velocity_of_density_change
+ * (runner.hpp:3297-3314, via compute_density_change 3192-3228) for every
+ * particle, into particle_density_adv, without the dfsph_factor update.
+ * Uses the neighbour list and boundary records of the last
+ * update_particle_neighbors. */
+int alv_solver_compute_density_change(alv_solver* solver);
+/* integrate_velocity<wrap> (runner.hpp:3613-3623): x += dt v with the
+ * solver's dt, wrapped on every periodic axis. */
+int alv_solver_integrate_velocity(alv_solver* solver);
 /* Solver::reset_solving_var (solver_df.hpp:361-365, solver_ii.hpp:245-248)
  * and Solver::reset_t (solver.hpp:148-152). */
 int alv_solver_reset_solving_var(alv_solver* solver);
@@ -270,6 +279,61 @@ int alv_volume_from_distance_map(const alv_const* cn,
                                  const float cell_size[3], float sign,
                                  const float* distance, float* volume,
                                  unsigned num_nodes);
+
+/* ---- learned warm start --------------------------------------------------
+ * No counterpart in the reference.  A small network (8 -> 64 -> 64 -> 64 ->
+ * 2, SiLU; bfloat16 weights and activations, FP32 accumulation on matrix
+ * cores) predicts each particle's converged divergence and density stiffness
+ * (the stored forms kappa_v and kappa of runner.hpp:3417-3418,3606-3611) from
+ * eight local features: rho/rho0 - 1, the density change rho_dot of
+ * compute_density_change (runner.hpp:3192-3228), the neighbour count, the
+ * summed boundary volume sum_b Psi_b, the previous kappa_v and kappa, |v| and
+ * dt.  The solver uses the predictions in place of the warm starts of
+ * warm_start_divergence_solve_0 and warm_start_pressure_solve0
+ * (runner.hpp:3230-3250,3430-3450); the convergence tests are unchanged.
+ *
+ * Parameters are a flat FP32 vector W1[64][8] b1 W2[64][64] b2 W3[64][64] b3
+ * W4[2][64] b4 (9026 values).  Normalisation: 20 floats, in_shift[8],
+ * in_scale[8], out_shift[2], out_scale[2]; the network sees
+ * (f - in_shift) * in_scale and its outputs y map to y * out_scale +
+ * out_shift.  Device pointers are marked dev_. */
+typedef struct alv_mlp alv_mlp;
+int alv_mlp_num_params(void);
+/* Glorot-uniform weights from SplitMix64(seed), zero biases, identity
+ * normalisation. */
+int alv_mlp_create(alv_mlp** out, unsigned long long seed);
+int alv_mlp_destroy(alv_mlp* net);
+int alv_mlp_set_params(alv_mlp* net, const float* host_params);
+int alv_mlp_get_params(alv_mlp* net, float* host_params);
+int alv_mlp_set_normalization(alv_mlp* net, const float* norm20);
+int alv_mlp_get_normalization(alv_mlp* net, float* norm20);
+/* mean / standard deviation of the inputs and targets of a data set
+ * (dev_x [n][8], dev_y [n][2]) over a strided sample of at most 2^20 pairs */
+int alv_mlp_fit_normalization(alv_mlp* net, const float* dev_x, const float* dev_y, size_t n);
+/* raw features dev_x [n][8] -> de-normalised predictions dev_y [n][2] */
+int alv_mlp_forward(alv_mlp* net, const float* dev_x, size_t n, float* dev_y);
+/* mean squared error of normalised batch (dev_xn [n][8], dev_tn [n][2]) and
+ * its gradient (host_grad[9026]); n a positive multiple of 2048 */
+int alv_mlp_loss_grad(alv_mlp* net, const float* dev_xn, const float* dev_tn, size_t n,
+                      double* loss, float* host_grad);
+/* `steps` Adam steps (learning rate lr) on random mini-batches of `batch`
+ * pairs (a multiple of 2048) drawn from the data set; host_loss[steps]
+ * (optional) receives the batch losses */
+int alv_mlp_train(alv_mlp* net, const float* dev_x, const float* dev_y, size_t n,
+                  unsigned batch, unsigned steps, float lr, unsigned long long seed,
+                  float* host_loss);
+/* Use a network for the DFSPH warm starts of later steps: the divergence
+ * warm start (use_divergence), the density warm start (use_density), or
+ * both.  A null network or two zero flags restore the reference's warm
+ * starts.  The solver does not own the network. */
+int alv_solver_set_warm_start_net(alv_solver* solver, alv_mlp* net, int use_divergence,
+                                  int use_density);
+/* Record the eight warm-start features during later steps. */
+int alv_solver_set_feature_log(alv_solver* solver, int enable);
+/* Training pairs of the last step (num_particles of them, in an internal
+ * particle order): features dev_x [n][8] and the converged stored
+ * (kappa_v, kappa) dev_y [n][2]. */
+int alv_solver_export_training_pairs(alv_solver* solver, float* dev_x, float* dev_y);
 
 #ifdef __cplusplus
 }

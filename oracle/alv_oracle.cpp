@@ -1725,6 +1725,22 @@ int orc_update_particle_neighbors(void* h) {
   });
 }
 
+int orc_compute_density_change(void* h) {
+  return dispatch(h, [&](auto& s) {
+    // compute_velocity_of_density_change (runner.hpp:3297-3314), sum only
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < long(s.n); ++i) s.density_adv[i] = s.density_change(unsigned(i));
+    return 0;
+  });
+}
+
+int orc_integrate_velocity(void* h) {
+  return dispatch(h, [&](auto& s) {
+    s.integrate_velocity();
+    return 0;
+  });
+}
+
 int orc_get_neighbors(void* h, unsigned* ids, void* xij) {
   return dispatch(h, [&](auto& s) {
     size_t cnt = size_t(s.max_n) * s.c.kmax;

@@ -133,6 +133,11 @@ int orc_update_particle_neighbors(void* h);
  * ascending particle id inside a cell): ids [N, Kmax], displacements
  * [N, Kmax, 3] in working precision. */
 int orc_get_neighbors(void* h, unsigned* ids, void* xij);
+/* density-change sum of every particle into density_adv (needs the
+ * boundary records and neighbour list of the current state) */
+int orc_compute_density_change(void* h);
+/* x += dt v, wrapped on periodic axes */
+int orc_integrate_velocity(void* h);
 int orc_get_grid(void* h, unsigned* cell_count, unsigned* cell_start,
                  unsigned* anomaly);
 int orc_step(void* h);
@@ -163,6 +168,15 @@ int orc_volume_from_distance_map(int precision, double kernel_radius,
                                  unsigned begin, unsigned end, double* out);
 /* Serial FP64 sum (Runner::sum with FP64 accumulation). */
 double orc_sum_f32(const float* v, size_t n);
+
+/* Learned warm-start network (mlp_oracle.cpp): 8 -> 64 -> 64 -> 64 -> 2,
+ * SiLU, flat FP32 parameters W1 b1 W2 b2 W3 b3 W4 b4 (9026).  `quantize`
+ * selects the bfloat16 rounding points of the device kernels; 0 is exact
+ * FP64.  x: [n][8] normalised inputs, t: [n][2] normalised targets. */
+int orc_mlp_num_params(void);
+int orc_mlp_forward(const float* params, size_t n, const float* x, int quantize, double* y);
+int orc_mlp_loss_grad(const float* params, size_t n, const float* x, const float* t,
+                      int quantize, double* loss, double* grad);
 
 #ifdef __cplusplus
 }

@@ -129,6 +129,11 @@ def lib():
                                                    ctypes.c_uint, dp]
         L.orc_sum_f32.restype = ctypes.c_double
         L.orc_sum_f32.argtypes = [vp, ctypes.c_size_t]
+        L.orc_compute_density_change.argtypes = [vp]
+        L.orc_integrate_velocity.argtypes = [vp]
+        L.orc_mlp_num_params.restype = ctypes.c_int
+        L.orc_mlp_forward.argtypes = [vp, ctypes.c_size_t, vp, ctypes.c_int, vp]
+        L.orc_mlp_loss_grad.argtypes = [vp, ctypes.c_size_t, vp, vp, ctypes.c_int, dp, vp]
         _lib = L
     return _lib
 
@@ -294,6 +299,12 @@ class Oracle:
     def update_particle_neighbors(self):
         self.L.orc_update_particle_neighbors(self.h)
 
+    def compute_density_change(self):
+        self.L.orc_compute_density_change(self.h)
+
+    def integrate_velocity(self):
+        self.L.orc_integrate_velocity(self.h)
+
     def neighbors(self):
         ids = np.zeros((self.max_n, self.kmax), np.uint32)
         self.L.orc_get_neighbors(self.h, ids.ctypes.data, None)
@@ -374,3 +385,30 @@ def volume_from_distance_map(precision, h, dmin, dmax, res, cs, sign, dist, begi
     lib().orc_volume_from_distance_map(precision, h, _dptr(dmin), _dptr(dmax), r, _dptr(cs),
                                        sign, _dptr(dist), begin, end, _dptr(out))
     return out
+
+
+MLP_PARAMS = 9026
+
+
+def mlp_forward(params, x, quantize=True):
+    """Warm-start network forward pass (normalised inputs [n, 8] -> [n, 2])."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros((x.shape[0], 2))
+    lib().orc_mlp_forward(params.ctypes.data, x.shape[0], x.ctypes.data, int(quantize),
+                          y.ctypes.data)
+    return y
+
+
+def mlp_loss_grad(params, x, t, quantize=True):
+    """Mean-squared loss over the batch and its gradient w.r.t. the 9026
+    parameters (FP64)."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(MLP_PARAMS)
+    loss = ctypes.c_double(0)
+    rc = lib().orc_mlp_loss_grad(params.ctypes.data, x.shape[0], x.ctypes.data, t.ctypes.data,
+                                 int(quantize), ctypes.byref(loss), g.ctypes.data)
+    assert rc == 0
+    return loss.value, g

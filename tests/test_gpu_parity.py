@@ -241,3 +241,99 @@ def test_step_parity_periodic(kind):
     gx = dp.coat(solver.particle_x).get()[:n]
     ox = orc.get("x")[:n]
     assert np.abs(gx - ox).max() < 1e-6
+
+
+def scaled_periodic(frac=0.2):
+    """The 8M-particle periodic stress scene cut to a periodic cube of edge
+    `frac` at the same number density (same neighbour statistics)."""
+    n = int(round(8_388_608 * frac ** 3))
+    sc = scenes.periodic_random(n)
+    sc.x = (sc.x * np.float32(frac)).astype(np.float32)
+    sc.wrap_length = (frac, frac, frac)
+    c = int(round(frac / sc.kernel_radius))
+    sc.grid_offset = (-c // 2,) * 3
+    sc.grid_res = (c,) * 3
+    return sc
+
+
+def test_stress_path_parity_periodic():
+    """Neighbour search + density + density-change sums + wrapped advection,
+    three steps, against the oracle doing the same calls."""
+    sc = scaled_periodic()
+    dp, pile, solver = gpu_scene(sc)
+    orc = oracle_like(sc, dp, pile)
+    orc.set("x", sc.x)
+    orc.set("v", sc.v)
+    n = sc.n
+    for step in range(3):
+        solver.update_particle_neighbors()
+        solver.compute_density_change()
+        orc.compute_all_boundaries()
+        orc.update_particle_neighbors()
+        orc.compute_density_change()
+        ids, cnt = solver.get_neighbors()
+        oids, ocnt = orc.neighbors()
+        np.testing.assert_array_equal(cnt[:n], ocnt[:n])
+        mask = np.arange(ids.shape[1])[None, :] < ocnt[:n, None]
+        np.testing.assert_array_equal(np.where(mask, ids[:n], 0), np.where(mask, oids[:n], 0))
+        dens = dp.coat(solver.particle_density).get()[:n]
+        assert rel_err(dens, orc.get("density")[:n]) < REL, step
+        dadv = dp.coat(solver.particle_density_adv).get()[:n]
+        assert rel_err(dadv, orc.get("density_adv")[:n]) < REL, step
+        solver.integrate_velocity()
+        orc.integrate_velocity()
+        gx = dp.coat(solver.particle_x).get()[:n]
+        np.testing.assert_array_equal(gx, orc.get("x")[:n])
+
+
+def _f32_pair_r2(xi, xj, L):
+    """|x_i - x_j|^2 with the minimum image, in the device's FP32 operation
+    order: d = x_i - x_j; d -= L if d >= L/2; d += L if d < -L/2;
+    r2 = (dx*dx + dy*dy) + dz*dz."""
+    f = np.float32
+    d = (xi[None, :] - xj).astype(f)
+    lo, hi = f(L) * f(-0.5), f(L) * f(0.5)
+    d = np.where(d >= hi, d - f(L), np.where(d < lo, d + f(L), d)).astype(f)
+    return ((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]).astype(f)
+
+
+def test_stress_8m_sampled_neighbors_brute_force():
+    """Full-size periodic stress scene (8,388,608 particles): for 2,000 sampled
+    particles the neighbour set equals brute force with the same FP32
+    predicate (|x_ij|^2 < h^2, minimum image), and the density equals an FP64
+    sum over that set."""
+    from scipy.spatial import cKDTree
+
+    sc = scenes.periodic_random(8_388_608)
+    dp, pile, solver = gpu_scene(sc)
+    solver.update_particle_neighbors()
+    n = sc.n
+    ids, cnt = solver.get_neighbors()
+    K = ids.shape[1]
+    dens = dp.coat(solver.particle_density).get()[:n]
+    x = sc.x
+    tree = cKDTree((x.astype(np.float64) + 0.5) % 1.0, boxsize=1.0)
+    rng = np.random.default_rng(7)
+    sample = rng.choice(n, 2000, replace=False)
+    h = np.float32(sc.kernel_radius)
+    h2 = h * h
+    kc = 8.0 / (np.pi * float(h) ** 3)
+    vol = sc.particle_mass / sc.density0
+    checked = 0
+    for i in sample:
+        cand = np.array(tree.query_ball_point((x[i].astype(np.float64) + 0.5) % 1.0,
+                                              float(h) * 1.001), dtype=np.int64)
+        cand = cand[cand != i]
+        r2 = _f32_pair_r2(x[i], x[cand], 1.0)
+        true = np.sort(cand[r2 < h2])
+        if len(true) > K:
+            assert cnt[i] == K
+            continue
+        assert cnt[i] == len(true), i
+        np.testing.assert_array_equal(np.sort(ids[i, :cnt[i]]), true)
+        q = np.sqrt(r2[r2 < h2].astype(np.float64)) / float(h)
+        w = np.where(q <= 0.5, 6 * q ** 3 - 6 * q ** 2 + 1, 2 * (1 - q) ** 3) * kc
+        want = sc.density0 * vol * (kc + w.sum())
+        assert abs(dens[i] - want) <= 1e-5 * want, i
+        checked += 1
+    assert checked > 1900

@@ -1,0 +1,162 @@
+"""Learned warm start on the GPU: the matrix-core network against the CPU
+oracle (forward values and parameter gradients with the same bfloat16
+rounding points), training on the device, and the solver path with the
+network switched in."""
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+from alluvion_amd import Depot, scenes
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [64 * 8, 64, 64 * 64, 64, 64 * 64, 64, 2 * 64, 2]
+OFFS = np.concatenate([[0], np.cumsum(SIZES)])
+IDENTITY_NORM = np.array([0] * 8 + [1] * 8 + [0, 0, 1, 1], np.float32)
+
+
+def random_params(seed, scale=0.6):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(of.MLP_PARAMS, np.float32)
+    fan_in = [8, 1, 64, 1, 64, 1, 64, 1]
+    for b in range(8):
+        sc = scale * np.sqrt(3.0 / fan_in[b])
+        p[OFFS[b]:OFFS[b + 1]] = rng.uniform(-sc, sc, SIZES[b])
+    return p
+
+
+def device_array(dp, a):
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    v = dp.create((a.size,), 1)
+    dp.coat(v).set(a)
+    return v
+
+
+def test_forward_matches_oracle():
+    import alluvion_amd._alluvion as al
+    dp = Depot(np.float32)
+    rng = np.random.default_rng(11)
+    n = 5000  # not a multiple of the 16-sample tile
+    x = rng.standard_normal((n, 8)).astype(np.float32)
+    p = random_params(12)
+    net = al.WarmStartNet(0)
+    net.set_params(p)
+    net.set_normalization(IDENTITY_NORM)
+    X = device_array(dp, x)
+    Y = dp.create((n * 2,), 1)
+    net.forward(X, Y, n)
+    y = dp.coat(Y).get().reshape(n, 2)
+    ref = of.mlp_forward(p, x, quantize=True)
+    scale = np.abs(ref).max()
+    err = np.abs(y - ref)
+    # FP32 accumulation against FP64: differences stay tiny except where a
+    # bf16 activation rounds the other way, which moves an output by about
+    # one bf16 step of one hidden unit
+    assert np.median(err) <= 1e-4 * scale, np.median(err) / scale
+    assert err.max() <= 2e-2 * scale, err.max() / scale
+    # normalisation: shift/scale on the inputs and the outputs
+    norm = np.array(list(np.full(8, 0.5)) + list(np.full(8, 2.0)) + [1.0, -3.0, 10.0, 0.1],
+                    np.float32)
+    net.set_normalization(norm)
+    net.forward(X, Y, n)
+    y2 = dp.coat(Y).get().reshape(n, 2)
+    ref2 = of.mlp_forward(p, (x - 0.5) * 2.0, quantize=True) * np.array([10.0, 0.1]) + \
+        np.array([1.0, -3.0])
+    assert np.median(np.abs(y2 - ref2) / np.abs(ref2).max(0)) <= 1e-4
+
+
+def test_gradient_matches_oracle():
+    import alluvion_amd._alluvion as al
+    dp = Depot(np.float32)
+    rng = np.random.default_rng(21)
+    n = 4096
+    x = rng.standard_normal((n, 8)).astype(np.float32)
+    t = rng.standard_normal((n, 2)).astype(np.float32)
+    p = random_params(22)
+    net = al.WarmStartNet(0)
+    net.set_params(p)
+    loss, g = net.loss_grad(device_array(dp, x), device_array(dp, t), n)
+    oloss, og = of.mlp_loss_grad(p, x, t, quantize=True)
+    assert abs(loss - oloss) <= 1e-4 * oloss, (loss, oloss)
+    for b in range(8):
+        gb, ob = g[OFFS[b]:OFFS[b + 1]], og[OFFS[b]:OFFS[b + 1]]
+        rel = np.linalg.norm(gb - ob) / max(np.linalg.norm(ob), 1e-30)
+        assert rel <= 2e-3, (b, rel)
+    # the exact-arithmetic gradient differs by the bf16 rounding, much more
+    _, eg = of.mlp_loss_grad(p, x, t, quantize=False)
+    assert np.linalg.norm(g - eg) > np.linalg.norm(g - og)
+
+
+def test_training_fits_a_smooth_map():
+    import alluvion_amd._alluvion as al
+    dp = Depot(np.float32)
+    rng = np.random.default_rng(31)
+    n = 1 << 18
+    x = rng.standard_normal((n, 8)).astype(np.float32)
+    t = np.stack([np.sin(x[:, 0]) + 0.5 * x[:, 1], x[:, 2] * x[:, 3] - 0.3 * x[:, 7]], 1)
+    t = (3.0 * t + 5.0).astype(np.float32)  # the normalisation must undo this
+    X, T = device_array(dp, x), device_array(dp, t)
+    net = al.WarmStartNet(3)
+    net.fit_normalization(X, T, n)
+    loss = net.train(X, T, n, 8192, 400, 3e-3, 5)
+    assert np.all(np.isfinite(loss))
+    assert loss[-20:].mean() < 0.1 * loss[:5].mean(), (loss[:5], loss[-20:])
+    Y = dp.create((n * 2,), 1)
+    net.forward(X, Y, n)
+    y = dp.coat(Y).get().reshape(n, 2)
+    rmse = np.sqrt(((y - t) ** 2).mean(0))
+    assert np.all(rmse < 0.35 * t.std(0)), rmse
+
+
+def test_feature_log_and_learned_step():
+    """Features are recorded with their meaning; a step with a network
+    switched in meets the unchanged convergence test, and with an all-zero
+    network (every prediction 0) matches the oracle's step with zero warm
+    starts."""
+    import alluvion_amd._alluvion as al
+    sc = scenes.dam_break_small()
+    dp, pile, solver, cn, cni = scenes.build_solver(sc, "df")
+    n = sc.n
+    for _ in range(30):
+        solver.step()
+    solver.set_feature_log(True)
+    kv_prev = dp.coat(solver.particle_kappa_v).get()[:n].copy()
+    k_prev = dp.coat(solver.particle_kappa).get()[:n].copy()
+    v_prev = dp.coat(solver.particle_v).get()[:n].copy()
+    solver.step()
+    X = dp.create((n * 8,), 1)
+    Y = dp.create((n * 2,), 1)
+    solver.export_training_pairs(X, Y)
+    f = dp.coat(X).get().reshape(n, 8)
+    y = dp.coat(Y).get().reshape(n, 2)
+    # the pairs come in an internal order: match them by (kappa_v, kappa)
+    kv_now = dp.coat(solver.particle_kappa_v).get()[:n]
+    k_now = dp.coat(solver.particle_kappa).get()[:n]
+    assert sorted(map(tuple, y)) == sorted(zip(kv_now, k_now))
+    assert np.allclose(np.sort(f[:, 4]), np.sort(kv_prev))
+    assert np.allclose(np.sort(f[:, 5]), np.sort(k_prev))
+    assert np.allclose(np.sort(f[:, 6]), np.sort(np.linalg.norm(v_prev, axis=1)), rtol=1e-6)
+    assert np.all(f[:, 7] == np.float32(sc.dt))
+    assert np.all(f[:, 2] == np.floor(f[:, 2])) and f[:, 2].max() <= sc.max_num_neighbors
+    assert np.all(f[:, 1] >= 0)
+    # an all-zero network predicts 0 for both stiffnesses
+    net = al.WarmStartNet(0)
+    net.set_params(np.zeros(of.MLP_PARAMS, np.float32))
+    solver.set_warm_start_net(net)
+    orc = of.Oracle(sc, precision=4)
+    for b, var in enumerate(pile.volume_grids):
+        orc.set_volume_map(b, dp.coat(var).get())
+    orc.set("x", dp.coat(solver.particle_x).get()[:n])
+    orc.set("v", dp.coat(solver.particle_v).get()[:n])
+    orc.set("kappa", np.zeros(n, np.float32))
+    orc.set("kappa_v", np.zeros(n, np.float32))
+    solver.step()
+    orc.step()
+    rep = orc.report()
+    assert solver.num_divergence_solve == rep["num_divergence_solve"]
+    assert solver.num_density_solve == rep["num_density_solve"]
+    for name, var in (("v", solver.particle_v), ("density", solver.particle_density)):
+        g = dp.coat(var).get()[:n].astype(np.float64)
+        o = orc.get(name)[:n].astype(np.float64)
+        assert np.abs(g - o).max() <= 1e-5 * np.abs(o).max(), name
+    solver.set_warm_start_net(None)
