@@ -19,6 +19,7 @@
 #include <cmath>
 #include <new>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../include/alv/alv.h"
@@ -302,18 +303,53 @@ struct alv_solver {
   int net_div = 0, net_dens = 0;  // which warm starts use it
   int log_features = 0;
   int features_valid = 0;  // the last step recorded warm-start features
+  // Scenes held by this solver: one for a solver made by alv_solver_create,
+  // several for a batch.  Every scene owns a block-aligned range of particle
+  // slots, a cell range and the boundaries of its pile.
+  struct Scene {
+    alv_pile* pile;
+    unsigned n, first, span;
+    unsigned cell_base, num_cells;
+    int off[3];
+    unsigned res[3];
+    unsigned bnd_base;
+    float t, dt;  // batch: time and step of the scene
+    unsigned last_div, last_dens;
+    alv_solver_report rep;
+  };
+  std::vector<Scene> scenes;
+  bool batch = false;
+  unsigned cells_total = 0;  // cell slots of all scenes
+  SimParams* par_dev = nullptr;
+  SimParams* par_host = nullptr;  // pinned; the table as last uploaded
+  int par_valid = 0;
+  unsigned short* blk_scene = nullptr;
+  unsigned short* cblk_scene = nullptr;
+  // Step segments replayed as HIP graphs (captured on first use and again
+  // when what they contain changes).
+  struct Segment {
+    hipGraphExec_t exec = nullptr;
+    unsigned long long key = ~0ull;
+  };
+  Segment seg[3];
+  int use_graph = 0;
+  unsigned slack = 0;  // Jacobi iterations enqueued beyond the last step's count
+  unsigned chunk_div = 0, chunk_dens = 0, shrink_div = 0, shrink_dens = 0;
   std::vector<void*> allocations;
 
   ~alv_solver() {
-    if (stream) hipStreamSynchronize(stream);
-    for (void* p : allocations) hipFree(p);
-    if (export_ids) hipFree(export_ids);
-    if (export_counts) hipFree(export_counts);
-    if (bnd_dev) hipFree(bnd_dev);
-    if (ctl_host) hipHostFree(ctl_host);
-    if (bnd_host) hipHostFree(bnd_host);
-    if (body_host) hipHostFree(body_host);
-    if (stream) hipStreamDestroy(stream);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocations) (void)hipFree(p);
+    if (export_ids) (void)hipFree(export_ids);
+    if (export_counts) (void)hipFree(export_counts);
+    if (bnd_dev) (void)hipFree(bnd_dev);
+    if (ctl_host) (void)hipHostFree(ctl_host);
+    if (bnd_host) (void)hipHostFree(bnd_host);
+    if (body_host) (void)hipHostFree(body_host);
+    if (par_host) (void)hipHostFree(par_host);
+    for (auto& g : seg)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (stream) (void)hipStreamDestroy(stream);
   }
   template <class T>
   int alloc(T** p, size_t count) {
@@ -754,9 +790,14 @@ static int solver_alloc(alv_solver* S) {
   ALV_ALLOC(A.key, cap);
   ALV_ALLOC(A.slot, cap);
   ALV_ALLOC(A.sorted_id, cap);
-  ALV_ALLOC(A.cell_count, size_t(C) + 1);
-  ALV_ALLOC(A.cell_start, size_t(C) + 2);
-  ALV_ALLOC(A.scan_tmp, (size_t(C) + 1 + 4095) / 4096 + 1);
+  const size_t CT = S->cells_total;
+  ALV_ALLOC(A.cell_count, CT);
+  ALV_ALLOC(A.cell_start, CT + 1);
+  ALV_ALLOC(A.scan_tmp, (CT + 4095) / 4096 + 1);
+  const size_t NS = S->scenes.size();
+  ALV_ALLOC(S->par_dev, NS);
+  ALV_ALLOC(S->blk_scene, size_t(grid_for(cap)) + 1);
+  ALV_ALLOC(S->cblk_scene, CT / kBlock + 1);
   ALV_ALLOC(A.anomaly, 4);
   ALV_ALLOC(A.x, cap);
   ALV_ALLOC(A.v, cap);
@@ -773,7 +814,12 @@ static int solver_alloc(alv_solver* S) {
   ALV_ALLOC(A.nbr, nbr_capacity(K, cap));
   ALV_ALLOC(A.nnb, cap);
   ALV_ALLOC(A.bmask, cap);
-  const size_t BN = size_t(std::max<size_t>(1, S->pile->bnd.size())) * cap;
+  size_t max_bnd = 0, all_bnd = 0;
+  for (const auto& sc : S->scenes) {
+    max_bnd = std::max(max_bnd, sc.pile->bnd.size());
+    all_bnd += sc.pile->bnd.size();
+  }
+  const size_t BN = std::max<size_t>(1, max_bnd) * cap;
   ALV_ALLOC(A.brec, BN);
   ALV_ALLOC(A.bker, BN);
   ALV_ALLOC(A.bforce, BN);
@@ -790,19 +836,38 @@ static int solver_alloc(alv_solver* S) {
   }
   ALV_ALLOC(A.partials, grid_for(cap) + 1);
   ALV_ALLOC(A.fpartials, grid_for(cap) + 1);
-  ALV_ALLOC(A.body_sums, std::max<size_t>(1, S->pile->bnd.size()) * 6);
-  ALV_ALLOC(A.body_partials, std::max<size_t>(1, S->pile->bnd.size()) * 256 * 6);
-  ALV_ALLOC(A.ctl, 1);
+  ALV_ALLOC(A.body_sums, std::max<size_t>(1, max_bnd) * 6);
+  ALV_ALLOC(A.body_partials, std::max<size_t>(1, max_bnd) * 256 * 6);
+  ALV_ALLOC(A.ctl, NS);
 #undef ALV_ALLOC
-  S->bnd_capacity = unsigned(std::max<size_t>(1, S->pile->bnd.size()));
+  S->bnd_capacity = unsigned(std::max<size_t>(1, all_bnd));
+  // scene of every particle block and every cell block
+  {
+    std::vector<unsigned short> bs(size_t(grid_for(cap)) + 1, 0), cs(CT / kBlock + 1, 0);
+    for (size_t k = 0; k < NS; ++k) {
+      const auto& sc = S->scenes[k];
+      for (unsigned b = sc.first / kBlock; b < (sc.first + sc.span) / kBlock; ++b)
+        bs[b] = (unsigned short)k;
+      const unsigned cspan = (k + 1 < NS ? S->scenes[k + 1].cell_base : unsigned(CT)) - sc.cell_base;
+      for (unsigned b = sc.cell_base / kBlock; b < (sc.cell_base + cspan) / kBlock; ++b)
+        cs[b] = (unsigned short)k;
+    }
+    ALV_HIP(hipMemcpy(S->blk_scene, bs.data(), bs.size() * sizeof(unsigned short),
+                      hipMemcpyHostToDevice));
+    ALV_HIP(hipMemcpy(S->cblk_scene, cs.data(), cs.size() * sizeof(unsigned short),
+                      hipMemcpyHostToDevice));
+  }
+  ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->par_host), NS * sizeof(SimParams),
+                        hipHostMallocDefault));
+  memset(S->par_host, 0, NS * sizeof(SimParams));
   ALV_HIP(hipMalloc(&S->bnd_dev, S->bnd_capacity * sizeof(BoundaryDev)));
   ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->bnd_host),
                         S->bnd_capacity * sizeof(BoundaryDev), hipHostMallocDefault));
-  ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->ctl_host), sizeof(StepCtl),
+  ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->ctl_host), NS * sizeof(StepCtl),
                         hipHostMallocDefault));
   ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->body_host),
                         S->bnd_capacity * 6 * sizeof(double), hipHostMallocDefault));
-  memset(S->ctl_host, 0, sizeof(StepCtl));
+  memset(S->ctl_host, 0, NS * sizeof(StepCtl));
   A.bnd = S->bnd_dev;
   return ALV_OK;
 }
@@ -819,7 +884,6 @@ int alv_solver_create(alv_solver** out, int kind, alv_pile* pile, unsigned max_n
     return fail(ALV_ERR_ARG, "max_num_neighbors_per_particle is zero");
   size_t C = size_t(cni->grid_res[0]) * cni->grid_res[1] * cni->grid_res[2];
   if (C == 0 || C > (1ull << 31)) return fail(ALV_ERR_ARG, "grid_res out of range");
-  if ((C + 1 + 4095) / 4096 > 4096) return fail(ALV_ERR_UNSUPPORTED, "grid has too many cells");
   alv_solver* S = new (std::nothrow) alv_solver();
   if (!S) return fail(ALV_ERR_STATE, "out of host memory");
   S->kind = kind;
@@ -828,6 +892,22 @@ int alv_solver_create(alv_solver** out, int kind, alv_pile* pile, unsigned max_n
   S->cn = *cn;
   S->cni = *cni;
   S->num_cells = unsigned(C);
+  {
+    alv_solver::Scene sc;
+    memset(&sc, 0, sizeof(sc));
+    sc.pile = pile;
+    sc.n = max_num_particles;
+    sc.first = 0;
+    sc.span = grid_for(max_num_particles) * kBlock;
+    sc.cell_base = 0;
+    sc.num_cells = unsigned(C);
+    for (int a = 0; a < 3; ++a) {
+      sc.off[a] = cni->grid_offset[a];
+      sc.res[a] = cni->grid_res[a];
+    }
+    S->scenes.push_back(sc);
+    S->cells_total = grid_for(unsigned(C) + 1) * kBlock;
+  }
   memset(&S->set, 0, sizeof(S->set));
   S->set.num_particles = max_num_particles;
   S->set.density_change_tolerance = 1e-3f;
@@ -858,10 +938,113 @@ int alv_solver_destroy(alv_solver* solver) {
   return ALV_OK;
 }
 
+int alv_solver_create_batch(alv_solver** out, int kind, unsigned num_scenes,
+                            const alv_scene_desc* descs, const alv_const* cn,
+                            const alv_consti* cni) {
+  if (!out || !descs || !cn || !cni) return fail(ALV_ERR_ARG, "null argument");
+  if (kind != ALV_SOLVER_DF && kind != ALV_SOLVER_II)
+    return fail(ALV_ERR_ARG, "unknown solver kind");
+  if (num_scenes == 0 || num_scenes > 65535) return fail(ALV_ERR_ARG, "num_scenes out of range");
+  if (cni->max_num_neighbors_per_particle == 0)
+    return fail(ALV_ERR_ARG, "max_num_neighbors_per_particle is zero");
+  std::unique_ptr<alv_solver> S(new (std::nothrow) alv_solver());
+  if (!S) return fail(ALV_ERR_STATE, "out of host memory");
+  size_t first = 0, cells = 0, bnd = 0;
+  for (unsigned k = 0; k < num_scenes; ++k) {
+    const alv_scene_desc& d = descs[k];
+    if (!d.pile) return fail(ALV_ERR_ARG, "scene without a pile");
+    if (d.num_particles == 0) return fail(ALV_ERR_ARG, "scene without particles");
+    for (const auto& B : d.pile->bnd)
+      if (B.dynamic)
+        return fail(ALV_ERR_UNSUPPORTED, "batched scenes take static boundaries only");
+    const size_t C = size_t(d.grid_res[0]) * d.grid_res[1] * d.grid_res[2];
+    if (C == 0) return fail(ALV_ERR_ARG, "grid_res out of range");
+    alv_solver::Scene sc;
+    memset(&sc, 0, sizeof(sc));
+    sc.pile = d.pile;
+    sc.n = d.num_particles;
+    sc.first = unsigned(first);
+    sc.span = grid_for(d.num_particles) * kBlock;
+    sc.cell_base = unsigned(cells);
+    sc.num_cells = unsigned(C);
+    for (int a = 0; a < 3; ++a) {
+      sc.off[a] = d.grid_offset[a];
+      sc.res[a] = d.grid_res[a];
+    }
+    sc.bnd_base = unsigned(bnd);
+    sc.last_div = 2;
+    sc.last_dens = 3;
+    first += sc.span;
+    cells += size_t(grid_for(unsigned(C) + 1)) * kBlock;
+    bnd += d.pile->bnd.size();
+    if (first > (1ull << 31) || cells > (1ull << 31))
+      return fail(ALV_ERR_UNSUPPORTED, "batch too large");
+    S->scenes.push_back(sc);
+  }
+  S->batch = true;
+  S->kind = kind;
+  S->pile = descs[0].pile;
+  S->cap = unsigned(first);
+  S->cn = *cn;
+  S->cni = *cni;
+  for (int a = 0; a < 3; ++a) {
+    S->cni.grid_res[a] = descs[0].grid_res[a];
+    S->cni.grid_offset[a] = descs[0].grid_offset[a];
+  }
+  S->num_cells = S->scenes[0].num_cells;
+  S->cells_total = unsigned(cells);
+  memset(&S->set, 0, sizeof(S->set));
+  S->set.num_particles = S->cap;
+  S->set.density_change_tolerance = 1e-3f;
+  S->set.density_error_tolerance = 1e-3f;
+  S->set.min_density_solve = 2;
+  S->set.max_density_solve = 100;
+  S->set.min_divergence_solve = 1;
+  S->set.max_divergence_solve = 100;
+  S->set.enable_divergence_solve = 1;
+  S->set.enable_density_solve = 1;
+  memset(&S->rep, 0, sizeof(S->rep));
+  hipError_t e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return fail(ALV_ERR_HIP, hipGetErrorString(e));
+  int rc = solver_alloc(S.get());
+  if (rc != ALV_OK) return rc;
+  *out = S.release();
+  return ALV_OK;
+}
+
+int alv_solver_num_scenes(const alv_solver* S, unsigned* n) {
+  if (!S || !n) return fail(ALV_ERR_ARG, "null argument");
+  *n = unsigned(S->scenes.size());
+  return ALV_OK;
+}
+
+int alv_solver_scene_range(const alv_solver* S, unsigned scene, unsigned* first,
+                           unsigned* count) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  if (scene >= S->scenes.size()) return fail(ALV_ERR_ARG, "scene out of range");
+  const auto& sc = S->scenes[scene];
+  if (first) *first = sc.first;
+  if (count) *count = S->batch ? sc.n : S->set.num_particles;
+  return ALV_OK;
+}
+
+int alv_solver_scene_report(const alv_solver* S, unsigned scene, alv_solver_report* r) {
+  if (!S || !r) return fail(ALV_ERR_ARG, "null argument");
+  if (scene >= S->scenes.size()) return fail(ALV_ERR_ARG, "scene out of range");
+  if (!S->batch) return alv_solver_get_report(S, r);
+  *r = S->scenes[scene].rep;
+  return ALV_OK;
+}
+
 int alv_solver_set_const(alv_solver* S, const alv_const* cn, const alv_consti* cni) {
   if (!S || !cn) return fail(ALV_ERR_ARG, "null argument");
   S->cn = *cn;
-  if (cni) {
+  if (cni && S->batch) {
+    // every scene keeps the grid it was created with
+    if (cni->max_num_neighbors_per_particle != S->cni.max_num_neighbors_per_particle)
+      return fail(ALV_ERR_STATE, "max_num_neighbors_per_particle is fixed at solver construction");
+    S->cni.max_num_particles_per_cell = cni->max_num_particles_per_cell;
+  } else if (cni) {
     if (cni->grid_res[0] != S->cni.grid_res[0] || cni->grid_res[1] != S->cni.grid_res[1] ||
         cni->grid_res[2] != S->cni.grid_res[2] ||
         cni->max_num_neighbors_per_particle != S->cni.max_num_neighbors_per_particle)
@@ -876,6 +1059,17 @@ int alv_solver_set_settings(alv_solver* S, const alv_solver_settings* st) {
   if (!S || !st) return fail(ALV_ERR_ARG, "null argument");
   if (st->num_particles > S->cap)
     return fail(ALV_ERR_ARG, "num_particles exceeds max_num_particles");
+  if (S->batch) {
+    // a batch keeps every scene's particles; a new dt or t applies to all
+    // scenes
+    for (auto& sc : S->scenes) {
+      if (st->dt != S->set.dt || sc.dt == 0.0f) sc.dt = st->dt;
+      if (st->t != S->set.t) sc.t = st->t;
+    }
+    S->set = *st;
+    S->set.num_particles = S->cap;
+    return ALV_OK;
+  }
   S->set = *st;
   return ALV_OK;
 }
@@ -897,64 +1091,121 @@ int alv_solver_get_report(const alv_solver* S, alv_solver_report* r) {
 }  // extern "C"
 
 namespace {
-SimParams solver_params(const alv_solver* S) {
-  return make_params(S->cn, S->cni, unsigned(S->pile->bnd.size()), S->set.num_particles);
+// Scene parameters of every scene, uploaded when they changed, and the
+// launch description.  A single-scene solver follows num_particles.
+int solver_launch(alv_solver* S, Launch* L) {
+  if (!S->batch) {
+    auto& sc = S->scenes[0];
+    sc.n = S->set.num_particles;
+    sc.span = grid_for(sc.n) * kBlock;
+  }
+  const size_t ns = S->scenes.size();
+  bool changed = !S->par_valid;
+  unsigned pblocks = 0, active = 0;
+  for (size_t k = 0; k < ns; ++k) {
+    const auto& sc = S->scenes[k];
+    SimParams p = make_params(S->cn, S->cni, unsigned(sc.pile->bnd.size()), sc.n);
+    for (int a = 0; a < 3; ++a) {
+      p.res[a] = int(sc.res[a]);
+      p.off[a] = sc.off[a];
+    }
+    p.num_cells = sc.num_cells;
+    p.first = sc.first;
+    p.span = sc.span;
+    p.cell_base = sc.cell_base;
+    p.bnd_base = sc.bnd_base;
+    if (memcmp(&p, &S->par_host[k], sizeof(p)) != 0) {
+      S->par_host[k] = p;
+      changed = true;
+    }
+    pblocks = std::max(pblocks, (sc.first + sc.span) / kBlock);
+    active += sc.n;
+  }
+  if (changed) {
+    ALV_HIP(hipMemcpyAsync(S->par_dev, S->par_host, ns * sizeof(SimParams),
+                           hipMemcpyHostToDevice, S->stream));
+    S->par_valid = 1;
+  }
+  L->T = SceneTab{S->par_dev, S->blk_scene, S->cblk_scene};
+  L->pblocks = pblocks;
+  L->cells = S->cells_total;
+  L->cblocks = S->cells_total / kBlock;
+  L->scenes = unsigned(ns);
+  L->active = active;
+  L->p0 = S->par_host[0];
+  return ALV_OK;
 }
 
 int upload_step_inputs(alv_solver* S) {
-  const size_t nb = S->pile->bnd.size();
-  if (nb > S->bnd_capacity)
-    return fail(ALV_ERR_STATE, "boundaries were added after the solver was created");
-  if (nb) {
-    memcpy(S->bnd_host, S->pile->bnd.data(), nb * sizeof(BoundaryDev));
+  size_t nb = 0;
+  for (const auto& sc : S->scenes) {
+    const size_t b = sc.pile->bnd.size();
+    if (sc.bnd_base + b > S->bnd_capacity)
+      return fail(ALV_ERR_STATE, "boundaries were added after the solver was created");
+    if (b) memcpy(S->bnd_host + sc.bnd_base, sc.pile->bnd.data(), b * sizeof(BoundaryDev));
+    nb = std::max<size_t>(nb, sc.bnd_base + b);
+  }
+  if (nb)
     ALV_HIP(hipMemcpyAsync(S->bnd_dev, S->bnd_host, nb * sizeof(BoundaryDev),
                            hipMemcpyHostToDevice, S->stream));
+  for (size_t k = 0; k < S->scenes.size(); ++k) {
+    StepCtl* c = S->ctl_host + k;
+    memset(c, 0, sizeof(StepCtl));
+    c->dt = S->batch ? S->scenes[k].dt : S->set.dt;
+    c->cfl = S->set.cfl;
+    c->min_dt = S->set.min_dt;
+    c->max_dt = S->set.max_dt;
+    c->tol_div = S->set.density_change_tolerance;
+    c->tol_dens = S->set.density_error_tolerance;
+    c->min_div = S->set.min_divergence_solve;
+    c->max_div = S->set.max_divergence_solve;
+    c->min_dens = S->set.min_density_solve;
+    c->max_dens = S->set.max_density_solve;
+    c->length_scale = S->cn.particle_radius * 2.0f;
   }
-  StepCtl* c = S->ctl_host;
-  memset(c, 0, sizeof(StepCtl));
-  c->dt = S->set.dt;
-  c->cfl = S->set.cfl;
-  c->min_dt = S->set.min_dt;
-  c->max_dt = S->set.max_dt;
-  c->tol_div = S->set.density_change_tolerance;
-  c->tol_dens = S->set.density_error_tolerance;
-  c->min_div = S->set.min_divergence_solve;
-  c->max_div = S->set.max_divergence_solve;
-  c->min_dens = S->set.min_density_solve;
-  c->max_dens = S->set.max_density_solve;
-  c->length_scale = S->cn.particle_radius * 2.0f;
-  ALV_HIP(hipMemcpyAsync(S->A.ctl, c, sizeof(StepCtl), hipMemcpyHostToDevice, S->stream));
+  ALV_HIP(hipMemcpyAsync(S->A.ctl, S->ctl_host, S->scenes.size() * sizeof(StepCtl),
+                         hipMemcpyHostToDevice, S->stream));
   return ALV_OK;
 }
 
 int read_ctl(alv_solver* S) {
-  ALV_HIP(hipMemcpyAsync(S->ctl_host, S->A.ctl, sizeof(StepCtl), hipMemcpyDeviceToHost,
-                         S->stream));
+  ALV_HIP(hipMemcpyAsync(S->ctl_host, S->A.ctl, S->scenes.size() * sizeof(StepCtl),
+                         hipMemcpyDeviceToHost, S->stream));
   ALV_HIP(hipStreamSynchronize(S->stream));
   return ALV_OK;
 }
 
-// grid + gather + boundary records + neighbour list + density (+ factor)
-int neighbor_phase(alv_solver* S, const SimParams& p, int with_factor) {
+bool all_done(const alv_solver* S, bool divergence) {
+  for (size_t k = 0; k < S->scenes.size(); ++k) {
+    const StepCtl& c = S->ctl_host[k];
+    if (!(divergence ? c.div_done : c.dens_done)) return false;
+  }
+  return true;
+}
+
+// grid + gather + boundary records + neighbour list + density (+ factor).
+// Only kernels (no memset nodes), so the sequence can be captured in a graph.
+int neighbor_phase(alv_solver* S, const Launch& L, int with_factor) {
   DevArrays& A = S->A;
-  ALV_HIP(hipMemsetAsync(A.cell_count, 0, (size_t(p.num_cells) + 1) * sizeof(unsigned),
-                         S->stream));
-  ALV_HIP(hipMemsetAsync(A.anomaly, 0, 3 * sizeof(unsigned), S->stream));
-  if (p.n == 0) return ALV_OK;
-  launch_cell_count(p, A, S->stream);
-  launch_scan(p.num_cells + 1, A.cell_count, A.cell_start, A.scan_tmp, S->stream);
-  launch_scatter(p, A, S->stream);
-  launch_cell_sort(p, A, S->stream);
-  launch_gather_boundary(p, A, S->kind, S->stream);
-  launch_neighbors(p, A, p.wrap_mask != 0, with_factor, S->stream);
+  launch_fill_u32(A.cell_count, 0u, L.cells, S->stream);
+  launch_fill_u32(A.anomaly, 0u, 3, S->stream);
+  if (L.active == 0) return ALV_OK;
+  launch_cell_count(L, A, S->stream);
+  launch_reserve_padding(L, A, S->stream);
+  launch_scan(L.cells, A.cell_count, A.cell_start, A.scan_tmp, S->stream);
+  launch_scatter(L, A, S->stream);
+  launch_cell_sort(L, A, S->stream);
+  launch_gather_boundary(L, A, S->kind, S->stream);
+  launch_neighbors(L, A, with_factor, S->stream);
   ALV_HIP(hipGetLastError());
   return ALV_OK;
 }
 
-// Enqueues Jacobi iterations in chunks until the device flag says done.
-template <class Launch>
+// Enqueues Jacobi iterations in chunks until every scene's device flag says
+// done.
+template <class LaunchFn>
 int run_loop(alv_solver* S, bool divergence, unsigned max_iter, unsigned* last,
-             Launch launch) {
+             LaunchFn launch) {
   unsigned issued = 0;
   unsigned chunk = std::max(1u, *last);
   while (issued < max_iter) {
@@ -964,35 +1215,64 @@ int run_loop(alv_solver* S, bool divergence, unsigned max_iter, unsigned* last,
     issued += todo;
     int rc = read_ctl(S);
     if (rc != ALV_OK) return rc;
-    if (divergence ? S->ctl_host->div_done : S->ctl_host->dens_done) break;
+    if (all_done(S, divergence)) break;
     chunk = std::max(1u, chunk / 2);
   }
   return ALV_OK;
 }
 
-int finish_step(alv_solver* S, const SimParams& p) {
+void fill_report(alv_solver_report* r, const StepCtl& c) {
+  r->num_divergence_solve = c.div_iter;
+  r->num_density_solve = c.dens_iter;
+  r->mean_density_change = c.mean_density_change;
+  r->mean_density_error = c.mean_density_error;
+  r->particle_max_v2 = c.particle_max_v2;
+  r->pile_max_v2 = c.pile_max_v2;
+  r->max_v2 = c.max_v2;
+  r->cfl_dt = c.cfl_dt;
+  r->utilized_cfl = c.utilized_cfl;
+}
+
+int finish_step(alv_solver* S, const Launch& L) {
   int rc = read_ctl(S);
   if (rc != ALV_OK) return rc;
   if (g_event_profiler) g_event_profiler->collect();
+  S->dirty = ALV_F_DENSITY | ALV_F_ALPHA | ALV_F_RHO_ADV | ALV_F_A | ALV_F_NNB;
+  if (S->batch) {
+    // every scene keeps its own step, time and report; the solver's report
+    // is the maximum iteration counts over the scenes
+    unsigned mdiv = 0, mdens = 0;
+    for (size_t k = 0; k < S->scenes.size(); ++k) {
+      auto& sc = S->scenes[k];
+      const StepCtl& c = S->ctl_host[k];
+      fill_report(&sc.rep, c);
+      sc.rep.t = sc.t + c.dt;
+      sc.rep.dt = c.dt;
+      sc.t += c.dt;
+      sc.dt = c.dt;
+      mdiv = std::max(mdiv, c.div_iter);
+      mdens = std::max(mdens, c.dens_iter);
+    }
+    fill_report(&S->rep, S->ctl_host[0]);
+    S->rep.num_divergence_solve = mdiv;
+    S->rep.num_density_solve = mdens;
+    if (mdiv) S->last_div = mdiv;
+    if (mdens) S->last_dens = mdens;
+    S->set.dt = S->scenes[0].dt;
+    S->set.t = S->scenes[0].t;
+    return ALV_OK;
+  }
   const StepCtl& c = *S->ctl_host;
   S->set.dt = c.dt;
-  S->rep.num_divergence_solve = c.div_iter;
-  S->rep.num_density_solve = c.dens_iter;
-  S->rep.mean_density_change = c.mean_density_change;
-  S->rep.mean_density_error = c.mean_density_error;
-  S->rep.particle_max_v2 = c.particle_max_v2;
-  S->rep.pile_max_v2 = c.pile_max_v2;
-  S->rep.max_v2 = c.max_v2;
-  S->rep.cfl_dt = c.cfl_dt;
-  S->rep.utilized_cfl = c.utilized_cfl;
+  fill_report(&S->rep, c);
   if (c.div_iter) S->last_div = c.div_iter;
   if (c.dens_iter) S->last_dens = c.dens_iter;
   // rigid force sums (solver_df.hpp:347-353)
   alv_pile* pile = S->pile;
   bool any_dynamic = false;
   for (const auto& B : pile->bnd) any_dynamic |= B.dynamic != 0;
-  if (any_dynamic && p.n) {
-    launch_body_sums(p, S->A, S->stream);
+  if (any_dynamic && L.active) {
+    launch_body_sums(L, S->A, S->stream);
     ALV_HIP(hipGetLastError());
     ALV_HIP(hipMemcpyAsync(S->body_host, S->A.body_sums, pile->bnd.size() * 6 * sizeof(double),
                            hipMemcpyDeviceToHost, S->stream));
@@ -1007,7 +1287,79 @@ int finish_step(alv_solver* S, const SimParams& p) {
   }
   alv_pile_integrate_kinematics(pile, S->set.dt);
   S->set.t += S->set.dt;
-  S->dirty = ALV_F_DENSITY | ALV_F_ALPHA | ALV_F_RHO_ADV | ALV_F_A | ALV_F_NNB;
+  return ALV_OK;
+}
+}  // namespace
+
+extern "C" {
+
+}  // extern "C"
+
+namespace {
+// Runs one segment of a step: enqueued directly, or replayed from a graph
+// captured from the same enqueue sequence (profiling needs the direct path).
+template <class Enqueue>
+int run_segment(alv_solver* S, int which, unsigned long long key, Enqueue enqueue) {
+  if (!S->use_graph || g_profiler) {
+    enqueue();
+    ALV_HIP(hipGetLastError());
+    return ALV_OK;
+  }
+  alv_solver::Segment& G = S->seg[which];
+  if (!G.exec || G.key != key) {
+    if (G.exec) {
+      ALV_HIP(hipGraphExecDestroy(G.exec));
+      G.exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    ALV_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    hipError_t e = hipStreamEndCapture(S->stream, &graph);
+    if (e != hipSuccess) return fail(ALV_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      G.exec = nullptr;
+      return fail(ALV_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    G.key = key;
+  }
+  ALV_HIP(hipGraphLaunch(G.exec, S->stream));
+  return ALV_OK;
+}
+
+// Iterations enqueued before the first convergence check: the last step's
+// count plus the slack; grows at once, shrinks after 16 smaller steps (so a
+// captured graph is not rebuilt for every small change).
+unsigned next_chunk(unsigned last, unsigned slack, unsigned lo, unsigned hi, unsigned* chunk,
+                    unsigned* shrink) {
+  const unsigned want = std::min(hi, std::max(std::max(lo, 1u), last + slack));
+  if (want > *chunk || *chunk == 0 || *chunk > hi) {
+    *chunk = want;
+    *shrink = 0;
+  } else if (want < *chunk && ++*shrink >= 16) {
+    *chunk = want;
+    *shrink = 0;
+  }
+  return *chunk;
+}
+
+// Continues a Jacobi loop after its first chunk until every scene is done.
+template <class LaunchFn>
+int finish_loop(alv_solver* S, bool divergence, unsigned issued, unsigned max_iter,
+                LaunchFn launch) {
+  int rc = read_ctl(S);
+  if (rc != ALV_OK) return rc;
+  unsigned chunk = 1;
+  while (!all_done(S, divergence) && issued < max_iter) {
+    const unsigned todo = std::min(chunk, max_iter - issued);
+    for (unsigned k = 0; k < todo; ++k) launch();
+    ALV_HIP(hipGetLastError());
+    issued += todo;
+    rc = read_ctl(S);
+    if (rc != ALV_OK) return rc;
+    chunk = std::min(chunk * 2, 8u);
+  }
   return ALV_OK;
 }
 }  // namespace
@@ -1016,58 +1368,94 @@ extern "C" {
 
 int alv_solver_step(alv_solver* S) {
   if (!S) return fail(ALV_ERR_ARG, "null solver");
-  const SimParams p = solver_params(S);
-  int rc = upload_step_inputs(S);
+  Launch L;
+  int rc = solver_launch(S, &L);
+  if (rc != ALV_OK) return rc;
+  rc = upload_step_inputs(S);
   if (rc != ALV_OK) return rc;
   DevArrays& A = S->A;
   hipStream_t st = S->stream;
   if (S->kind == ALV_SOLVER_DF) {
-    // SolverDf::step (solver_df.hpp:74-359)
+    // SolverDf::step (solver_df.hpp:74-359) in three segments; the host
+    // checks convergence between them
     const int div = S->set.enable_divergence_solve != 0;
     const int dens = S->set.enable_density_solve != 0;
-    rc = neighbor_phase(S, p, div);
+    const int learned_dens = S->net != nullptr && S->net_dens && div;
+    const unsigned cdiv = next_chunk(S->last_div, S->slack, S->set.min_divergence_solve,
+                                     S->set.max_divergence_solve, &S->chunk_div, &S->shrink_div);
+    const unsigned cdens = next_chunk(S->last_dens, S->slack, S->set.min_density_solve,
+                                      S->set.max_density_solve, &S->chunk_dens, &S->shrink_dens);
+    const MlpDev mv = mlp_device_view(S->net);
+    const unsigned long long base = (unsigned long long)L.pblocks << 24 ^
+                                    (unsigned long long)(L.active != 0) << 23 ^
+                                    (unsigned long long)(L.p0.wrap_mask != 0) << 22;
+    const unsigned long long keyA =
+        base ^ ((unsigned long long)cdiv << 8) ^ (div << 1) ^ (S->log_features << 2) ^
+        ((S->net != nullptr) << 3) ^ (S->net_div << 4) ^
+        ((unsigned long long)(uintptr_t)S->net << 32);
+    rc = run_segment(S, 0, keyA, [&] {
+      if (neighbor_phase(S, L, div) != ALV_OK) return;
+      launch_ctl_reset(L, A, st);
+      if (L.active && div) {
+        launch_df_warm_start_div(L, A, st, S->log_features, S->net ? &mv : nullptr, S->net_div);
+        for (unsigned k = 0; k < cdiv; ++k) launch_df_div_iteration(L, A, st);
+      }
+    });
     if (rc != ALV_OK) return rc;
-    launch_ctl_reset(A, st);
-    if (p.n) {
-      if (div) {
-        const MlpDev mv = mlp_device_view(S->net);
-        launch_df_warm_start_div(p, A, st, S->log_features, S->net ? &mv : nullptr, S->net_div);
-        rc = run_loop(S, true, S->set.max_divergence_solve, &S->last_div,
-                      [&] { launch_df_div_iteration(p, A, st); });
-        if (rc != ALV_OK) return rc;
-      }
-      launch_df_viscosity(p, A, div, st);
+    if (L.active && div) {
+      rc = finish_loop(S, true, cdiv, S->set.max_divergence_solve,
+                       [&] { launch_df_div_iteration(L, A, st); });
+      if (rc != ALV_OK) return rc;
     }
-    launch_update_dt(p, A, st);
-    if (p.n) {
-      launch_df_integrate_np(p, A, st);
-      if (dens) {
-        launch_df_warm_start_dens(p, A, st, S->net != nullptr && S->net_dens && div);
-        rc = run_loop(S, false, S->set.max_density_solve, &S->last_dens,
-                      [&] { launch_df_dens_iteration(p, A, st); });
-        if (rc != ALV_OK) return rc;
+    const unsigned long long keyB = base ^ ((unsigned long long)cdens << 8) ^ (div << 1) ^
+                                    (dens << 2) ^ (learned_dens << 3);
+    rc = run_segment(S, 1, keyB, [&] {
+      if (L.active) launch_df_viscosity(L, A, div, st);
+      launch_update_dt(L, A, st);
+      if (L.active) {
+        launch_df_integrate_np(L, A, st);
+        if (dens) {
+          launch_df_warm_start_dens(L, A, st, learned_dens);
+          for (unsigned k = 0; k < cdens; ++k) launch_df_dens_iteration(L, A, st);
+        }
       }
-      launch_df_finish(p, A, dens, st);
+    });
+    if (rc != ALV_OK) return rc;
+    if (L.active && dens) {
+      rc = finish_loop(S, false, cdens, S->set.max_density_solve,
+                       [&] { launch_df_dens_iteration(L, A, st); });
+      if (rc != ALV_OK) return rc;
     }
+    rc = run_segment(S, 2, base ^ (dens << 1), [&] {
+      if (L.active) launch_df_finish(L, A, dens, st);
+    });
+    if (rc != ALV_OK) return rc;
   } else {
     // SolverIi::step (solver_ii.hpp:76-243)
-    rc = neighbor_phase(S, p, 0);
+    rc = neighbor_phase(S, L, 0);
     if (rc != ALV_OK) return rc;
-    launch_ctl_reset(A, st);
-    if (p.n) launch_ii_viscosity_predict(p, A, st);
-    launch_update_dt(p, A, st);
-    if (p.n) {
-      launch_ii_predict(p, A, st);
+    launch_ctl_reset(L, A, st);
+    if (L.active) launch_ii_viscosity_predict(L, A, st);
+    launch_update_dt(L, A, st);
+    if (L.active) {
+      launch_ii_predict(L, A, st);
       rc = run_loop(S, false, S->set.max_density_solve, &S->last_dens,
-                    [&] { launch_ii_iteration(p, A, st); });
+                    [&] { launch_ii_iteration(L, A, st); });
       if (rc != ALV_OK) return rc;
-      launch_ii_finish(p, A, st);
+      launch_ii_finish(L, A, st);
     }
   }
   ALV_HIP(hipGetLastError());
   S->features_valid = S->kind == ALV_SOLVER_DF && S->set.enable_divergence_solve &&
-                      (S->log_features || S->net) && p.n;
-  return finish_step(S, p);
+                      (S->log_features || S->net) && L.active;
+  return finish_step(S, L);
+}
+
+int alv_solver_set_graph(alv_solver* S, int enable, unsigned slack) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  S->use_graph = enable != 0;
+  S->slack = slack;
+  return ALV_OK;
 }
 
 int alv_solver_set_warm_start_net(alv_solver* S, alv_mlp* net, int use_divergence,
@@ -1095,8 +1483,10 @@ int alv_solver_export_training_pairs(alv_solver* S, float* dev_x, float* dev_y) 
     return fail(ALV_ERR_STATE,
                 "no features recorded: enable the feature log (or a warm-start network) "
                 "and take a step with the divergence solve enabled");
-  const SimParams p = solver_params(S);
-  launch_export_pairs(p, S->A, dev_x, dev_y, S->stream);
+  Launch L;
+  int rc = solver_launch(S, &L);
+  if (rc != ALV_OK) return rc;
+  launch_export_pairs(L, S->A, dev_x, dev_y, S->stream);
   ALV_HIP(hipGetLastError());
   ALV_HIP(hipStreamSynchronize(S->stream));
   return ALV_OK;
@@ -1104,10 +1494,12 @@ int alv_solver_export_training_pairs(alv_solver* S, float* dev_x, float* dev_y) 
 
 int alv_solver_update_particle_neighbors(alv_solver* S) {
   if (!S) return fail(ALV_ERR_ARG, "null solver");
-  const SimParams p = solver_params(S);
-  int rc = upload_step_inputs(S);
+  Launch L;
+  int rc = solver_launch(S, &L);
   if (rc != ALV_OK) return rc;
-  rc = neighbor_phase(S, p, S->kind == ALV_SOLVER_DF ? 1 : 0);
+  rc = upload_step_inputs(S);
+  if (rc != ALV_OK) return rc;
+  rc = neighbor_phase(S, L, S->kind == ALV_SOLVER_DF ? 1 : 0);
   if (rc != ALV_OK) return rc;
   ALV_HIP(hipStreamSynchronize(S->stream));
   S->dirty |= ALV_F_DENSITY | ALV_F_NNB | ALV_F_ALPHA;
@@ -1116,10 +1508,12 @@ int alv_solver_update_particle_neighbors(alv_solver* S) {
 
 int alv_solver_compute_density_change(alv_solver* S) {
   if (!S) return fail(ALV_ERR_ARG, "null solver");
-  const SimParams p = solver_params(S);
-  int rc = upload_step_inputs(S);
+  Launch L;
+  int rc = solver_launch(S, &L);
   if (rc != ALV_OK) return rc;
-  if (p.n) launch_density_change(p, S->A, S->stream);
+  rc = upload_step_inputs(S);
+  if (rc != ALV_OK) return rc;
+  if (L.active) launch_density_change(L, S->A, S->stream);
   ALV_HIP(hipGetLastError());
   ALV_HIP(hipStreamSynchronize(S->stream));
   S->dirty |= ALV_F_RHO_ADV;
@@ -1128,8 +1522,10 @@ int alv_solver_compute_density_change(alv_solver* S) {
 
 int alv_solver_integrate_velocity(alv_solver* S) {
   if (!S) return fail(ALV_ERR_ARG, "null solver");
-  const SimParams p = solver_params(S);
-  if (p.n) launch_integrate_velocity(p, S->A, S->set.dt, S->stream);
+  Launch L;
+  int rc = solver_launch(S, &L);
+  if (rc != ALV_OK) return rc;
+  if (L.active) launch_integrate_velocity(L, S->A, S->set.dt, S->stream);
   ALV_HIP(hipGetLastError());
   ALV_HIP(hipStreamSynchronize(S->stream));
   return ALV_OK;
@@ -1181,9 +1577,11 @@ int alv_solver_get_field(alv_solver* S, int field, void** dptr, size_t* n_elemen
     default: return fail(ALV_ERR_ARG, "unknown field");
   }
   if (need & S->dirty) {
-    const SimParams p = solver_params(S);
-    if (p.n) {
-      launch_unsort_fields(p, A, need, S->stream);
+    Launch L;
+    int rc = solver_launch(S, &L);
+    if (rc != ALV_OK) return rc;
+    if (L.active) {
+      launch_unsort_fields(L, A, need, S->stream);
       ALV_HIP(hipGetLastError());
     }
     ALV_HIP(hipStreamSynchronize(S->stream));
@@ -1197,20 +1595,23 @@ int alv_solver_get_field(alv_solver* S, int field, void** dptr, size_t* n_elemen
 
 int alv_solver_get_neighbors(alv_solver* S, unsigned* ids, unsigned* counts) {
   if (!S) return fail(ALV_ERR_ARG, "null solver");
-  const SimParams p = solver_params(S);
+  Launch L;
+  int rc = solver_launch(S, &L);
+  if (rc != ALV_OK) return rc;
   const size_t K = S->cni.max_num_neighbors_per_particle;
+  const size_t n = S->batch ? size_t(S->cap) : size_t(S->set.num_particles);
   if (!S->export_ids) {
     ALV_HIP(hipMalloc(&S->export_ids, K * S->cap * sizeof(unsigned)));
     ALV_HIP(hipMalloc(&S->export_counts, size_t(S->cap) * sizeof(unsigned)));
   }
   ALV_HIP(hipMemsetAsync(S->export_ids, 0xff, K * S->cap * sizeof(unsigned), S->stream));
   ALV_HIP(hipMemsetAsync(S->export_counts, 0, size_t(S->cap) * sizeof(unsigned), S->stream));
-  if (p.n) launch_export_neighbors(p, S->A, S->export_ids, S->export_counts, S->stream);
+  if (L.active) launch_export_neighbors(L, S->A, S->export_ids, S->export_counts, S->stream);
   ALV_HIP(hipGetLastError());
   ALV_HIP(hipStreamSynchronize(S->stream));
-  if (ids) ALV_HIP(hipMemcpy(ids, S->export_ids, K * p.n * sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (ids) ALV_HIP(hipMemcpy(ids, S->export_ids, K * n * sizeof(unsigned), hipMemcpyDeviceToHost));
   if (counts)
-    ALV_HIP(hipMemcpy(counts, S->export_counts, size_t(p.n) * sizeof(unsigned),
+    ALV_HIP(hipMemcpy(counts, S->export_counts, n * sizeof(unsigned),
                       hipMemcpyDeviceToHost));
   return ALV_OK;
 }

@@ -97,56 +97,96 @@ struct KernelProfiler {
 extern KernelProfiler* g_profiler;
 inline unsigned grid_for(unsigned n) { return (n + kBlock - 1) / kBlock; }
 
+// Scene table of a solver (device memory): the parameters of every scene and
+// the scene of every 256-particle block and every 256-cell block.  A solver
+// of one scene has a table of one entry.
+struct SceneTab {
+  const SimParams* par;
+  const unsigned short* blk_scene;
+  const unsigned short* cblk_scene;
+};
+
+// Host-side description of one launch set.
+struct Launch {
+  SceneTab T;
+  unsigned pblocks;  // particle blocks: sum of span / 256
+  unsigned cblocks;  // cell blocks
+  unsigned cells;    // cell slots: sum of the per-scene cell ranges
+  unsigned scenes;
+  unsigned active;   // particles in all scenes
+  SimParams p0;      // scene 0 (shared constants, wrap flags)
+};
+
+// Particle-kernel prologue: this block's particle index s (blocks remapped
+// per XCD, or linear), its scene's parameters p, and the scene's control
+// block and boundaries in A.
+#define ALV_SCENE_AT(T, A, s, blk)                       \
+  const unsigned s = (blk) * kBlock + threadIdx.x;       \
+  const unsigned scene_ = (T).blk_scene[(blk)];          \
+  const SimParams p = (T).par[scene_];                   \
+  (A).ctl += scene_;                                     \
+  (A).bnd += p.bnd_base
+#define ALV_SCENE(T, A, s) ALV_SCENE_AT(T, A, s, xcd_block())
+#define ALV_SCENE_LINEAR(T, A, s) ALV_SCENE_AT(T, A, s, blockIdx.x)
+// One block per scene (reductions over a scene's blocks).
+#define ALV_SCENE_BLOCK(T, A)                            \
+  const unsigned scene_ = blockIdx.x;                    \
+  const SimParams p = (T).par[scene_];                   \
+  (A).ctl += scene_;                                     \
+  (A).bnd += p.bnd_base;                                 \
+  const unsigned blk_begin = p.first / kBlock, blk_end = blk_begin + p.span / kBlock
+
 // neighbour search (counting sort)
-void launch_cell_count(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_fill_u32(unsigned* dst, unsigned value, size_t count, hipStream_t s);
+void launch_cell_count(const Launch& L, const DevArrays& A, hipStream_t s);
+// reserves each scene's unused slots at the end of its outside-the-grid bin
+void launch_reserve_padding(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_scan(unsigned count, const unsigned* in, unsigned* out, unsigned* tmp,
                  hipStream_t s);
-void launch_scatter(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_cell_sort(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_scatter(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_cell_sort(const Launch& L, const DevArrays& A, hipStream_t s);
 // gather user -> sorted and compute boundary records
-void launch_gather_boundary(const SimParams& p, const DevArrays& A, int solver_kind,
+void launch_gather_boundary(const Launch& L, const DevArrays& A, int solver_kind,
                             hipStream_t s);
 // neighbour list + density (+ DFSPH factor when with_factor)
-void launch_neighbors(const SimParams& p, const DevArrays& A, int wrap_any, int with_factor,
-                      hipStream_t s);
+void launch_neighbors(const Launch& L, const DevArrays& A, int with_factor, hipStream_t s);
 
 // DFSPH
 // features: record the eight warm-start features; net: learned warm start
 // (null = the reference's warm start)
-void launch_df_warm_start_div(const SimParams& p, const DevArrays& A, hipStream_t s,
+void launch_df_warm_start_div(const Launch& L, const DevArrays& A, hipStream_t s,
                               int features, const MlpDev* net, int net_divergence);
-void launch_df_div_loop_prep(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_df_div_iteration(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_df_viscosity(const SimParams& p, const DevArrays& A, int div_finish,
+void launch_df_div_iteration(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_df_viscosity(const Launch& L, const DevArrays& A, int div_finish,
                          hipStream_t s);
-void launch_update_dt(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_df_integrate_np(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_df_warm_start_dens(const SimParams& p, const DevArrays& A, hipStream_t s,
+void launch_update_dt(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_df_integrate_np(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_df_warm_start_dens(const Launch& L, const DevArrays& A, hipStream_t s,
                                int learned);
 // training pairs of the last step in sorted order: x [n][8], y [n][2]
-void launch_export_pairs(const SimParams& p, const DevArrays& A, float* x, float* y,
+void launch_export_pairs(const Launch& L, const DevArrays& A, float* x, float* y,
                          hipStream_t s);
-void launch_df_dens_iteration(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_df_finish(const SimParams& p, const DevArrays& A, int dens_finish, hipStream_t s);
-void launch_density_change(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_integrate_velocity(const SimParams& p, const DevArrays& A, float dt, hipStream_t s);
+void launch_df_dens_iteration(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_df_finish(const Launch& L, const DevArrays& A, int dens_finish, hipStream_t s);
+void launch_density_change(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_integrate_velocity(const Launch& L, const DevArrays& A, float dt, hipStream_t s);
 
 // IISPH
-void launch_ii_viscosity_predict(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_ii_predict(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_ii_iteration(const SimParams& p, const DevArrays& A, hipStream_t s);
-void launch_ii_finish(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_ii_viscosity_predict(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_ii_predict(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_ii_iteration(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_ii_finish(const Launch& L, const DevArrays& A, hipStream_t s);
 
 // rigid bodies: per-body FP64 force / torque sums into A.body_sums
-void launch_body_sums(const SimParams& p, const DevArrays& A, hipStream_t s);
+void launch_body_sums(const Launch& L, const DevArrays& A, hipStream_t s);
 // sorted -> user order for the lazily synchronised API arrays
-void launch_unsort_fields(const SimParams& p, const DevArrays& A, unsigned field_mask,
+void launch_unsort_fields(const Launch& L, const DevArrays& A, unsigned field_mask,
                           hipStream_t s);
 // reference-layout neighbour ids (user order) for inspection: ids[i*kmax+t]
-void launch_export_neighbors(const SimParams& p, const DevArrays& A, unsigned* ids,
+void launch_export_neighbors(const Launch& L, const DevArrays& A, unsigned* ids,
                              unsigned* counts, hipStream_t s);
 // reset the control block for a fresh solve
-void launch_ctl_reset(const DevArrays& A, hipStream_t s);
+void launch_ctl_reset(const Launch& L, const DevArrays& A, hipStream_t s);
 
 // volume maps (setup)
 void launch_volume_field_analytic(const SimParams& p, const BoundaryDev& b, float* volume,

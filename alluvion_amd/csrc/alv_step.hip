@@ -112,17 +112,17 @@ __device__ inline unsigned block_excl_scan_u32(unsigned v, unsigned* lds, unsign
 // Neighbour search: counting sort of particles by cell
 // (replaces update_particle_grid, runner.hpp:1840-1867)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_cell_count(SimParams p, DevArrays A) {
-  const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= p.n) return;
+__global__ __launch_bounds__(256) void k_cell_count(SceneTab T, DevArrays A) {
+  ALV_SCENE_LINEAR(T, A, i);
+  if (i >= p.first + p.n) return;
   vec3 x = ld3(A.ux, i);
   int ip[3];
   cell_ipos(x, p, ip);
   unsigned k;
   if (within_grid(ip, p)) {
-    k = cell_linear(ip, p);
+    k = p.cell_base + cell_linear(ip, p);
   } else {
-    k = p.num_cells;  // outside the grid: own bin after the last cell
+    k = p.cell_base + p.num_cells;  // outside the grid: own bin after the last cell
     A.anomaly[0] = 1u;
   }
   A.key[i] = k;
@@ -183,9 +183,9 @@ __global__ __launch_bounds__(256) void k_scan_apply(unsigned count, const unsign
   }
 }
 
-__global__ __launch_bounds__(256) void k_scatter(SimParams p, DevArrays A) {
-  const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= p.n) return;
+__global__ __launch_bounds__(256) void k_scatter(SceneTab T, DevArrays A) {
+  ALV_SCENE_LINEAR(T, A, i);
+  if (i >= p.first + p.n) return;
   A.sorted_id[A.cell_start[A.key[i]] + A.slot[i]] = i;
 }
 
@@ -193,9 +193,10 @@ __global__ __launch_bounds__(256) void k_scatter(SimParams p, DevArrays A) {
 // atomic slots of k_cell_count are arrival-ordered.  The bin of particles
 // outside the grid is left unordered: nothing depends on its order (those
 // particles are in no cell and their own neighbour scans are independent).
-__global__ __launch_bounds__(256) void k_cell_sort(SimParams p, DevArrays A) {
+__global__ __launch_bounds__(256) void k_cell_sort(SceneTab T, DevArrays A) {
   const unsigned c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= p.num_cells) return;
+  const SimParams p = T.par[T.cblk_scene[blockIdx.x]];
+  if (c < p.cell_base || c - p.cell_base >= p.num_cells) return;
   const unsigned b = A.cell_start[c], e = A.cell_start[c + 1];
   if (e - b > p.cell_cap) A.anomaly[1] = 1u;
   if (e - b < 2) return;
@@ -237,9 +238,9 @@ __global__ __launch_bounds__(256) void k_cell_sort(SimParams p, DevArrays A) {
 // (compute_particle_boundary_analytic<0>, runner.hpp:1982-2002, with
 // compute_volume_and_boundary_x_analytic, runner.hpp:1738-1771)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gather_boundary(SimParams p, DevArrays A, int kind) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_gather_boundary(SceneTab T, DevArrays A, int kind) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const unsigned i = A.sorted_id[s];
   vec3 x = ld3(A.ux, i);
   vec3 v = ld3(A.uv, i);
@@ -295,9 +296,9 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SimParams p, DevArrays 
 // id, which is ascending sorted index.
 // ---------------------------------------------------------------------------
 template <bool W>
-__global__ __launch_bounds__(256) void k_neighbors(SimParams p, DevArrays A, int with_factor) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int with_factor) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const unsigned K = p.kmax;
   const vec3 xi = ld3(A.x[s]);
   int ip[3];
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(256) void k_neighbors(SimParams p, DevArrays A, int
       int nz = ip[2] + dz;
       if (p.wrap_mask & 4) nz = nz >= p.res[2] ? nz - p.res[2] : (nz < 0 ? nz + p.res[2] : nz);
       if (nz < 0 || nz >= p.res[2]) continue;
-      const unsigned c = row + unsigned(nz);
+      const unsigned c = p.cell_base + row + unsigned(nz);
       if (nz == prev + 1) {
         span_e[nspan - 1] = A.cell_start[c + 1];
       } else {
@@ -534,9 +535,9 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
 // warm_start_divergence_solve_0 (runner.hpp:3230-3250)
 // F: also record the warm-start features of the particle
 template <bool W, bool F>
-__global__ __launch_bounds__(256) void k_ws_div0(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_ws_div0(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   float dadv = density_change<W>(p, A, s);
   const float kv_prev = A.kappa_v[s];
@@ -559,13 +560,17 @@ __global__ __launch_bounds__(256) void k_ws_div0(SimParams p, DevArrays A) {
 
 // Learned warm start: the network's prediction of the converged stored
 // stiffness replaces 0.5 max(kappa_v, -0.5) in warm_start_divergence_solve_0
-// (runner.hpp:3230-3250) and is kept for warm_start_pressure_solve0.  A
-// non-finite prediction falls back to the reference's warm start.  Every
-// lane of the wave runs the matrix-core inference; there is no early return.
-__global__ __launch_bounds__(256) void k_ws_learned(SimParams p, DevArrays A, MlpDev M,
+// (runner.hpp:3230-3250) and is kept for warm_start_pressure_solve0.
+// Converged stiffnesses are never positive (k = rho_dot alpha and
+// (rho* - 1) alpha with rho_dot >= 0, rho* >= 1, alpha <= 0), so predictions
+// are clamped to [bound, 0] with the bounds of the reference's own warm-start
+// clamps (-0.5 and -2.5e-4 in stored form).  A non-finite prediction falls
+// back to the reference's warm start.  Every lane of the wave runs the
+// matrix-core inference; there is no early return.
+__global__ __launch_bounds__(256) void k_ws_learned(SceneTab T, DevArrays A, MlpDev M,
                                                     int use_divergence) {
-  const unsigned s = ALV_TID();
-  const bool ok = s < p.n;
+  ALV_SCENE(T, A, s);
+  const bool ok = s < p.first + p.n;
   float f[kMlpIn];
   const float4 f0 = ok ? A.feat[2 * size_t(s)] : make_float4(0.f, 0.f, 0.f, 0.f);
   const float4 f1 = ok ? A.feat[2 * size_t(s) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -576,16 +581,16 @@ __global__ __launch_bounds__(256) void k_ws_learned(SimParams p, DevArrays A, Ml
   if (!ok) return;
   const float dt = A.ctl->dt;
   if (use_divergence && f[1] > 0.0f) {
-    const float kv = y0 / dt;
-    if (isfinite(kv)) A.kappa_v[s] = kv;
+    const float kv = std_min(std_max(y0, -0.5f), 0.0f) / dt;
+    if (isfinite(y0)) A.kappa_v[s] = kv;
   }
-  A.kpred[s] = y1;
+  A.kpred[s] = isfinite(y1) ? std_min(std_max(y1, -0.00025f), 0.0f) : y1;
 }
 // warm_start_divergence_solve_1 (runner.hpp:3252-3295)
 template <bool W>
-__global__ __launch_bounds__(256) void k_ws_div1(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_ws_div1(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   vec3 dv = stiffness_dv<W, true>(p, A, s, A.kappa_v[s], A.kappa_v, dt);
   add_v(A, s, dv);
@@ -593,9 +598,9 @@ __global__ __launch_bounds__(256) void k_ws_div1(SimParams p, DevArrays A) {
 // compute_velocity_of_density_change (runner.hpp:3297-3314); also stores the
 // first iteration's stiffness k = rho_adv * alpha
 template <bool W>
-__global__ __launch_bounds__(256) void k_vel_density_change(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_vel_density_change(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   float dadv = density_change<W>(p, A, s);
   A.rho_adv[s] = dadv;
@@ -607,10 +612,10 @@ __global__ __launch_bounds__(256) void k_vel_density_change(SimParams p, DevArra
 }
 // divergence_solve_iteration (runner.hpp:3316-3363)
 template <bool W>
-__global__ __launch_bounds__(256) void k_div_iter(SimParams p, DevArrays A) {
+__global__ __launch_bounds__(256) void k_div_iter(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
   if (A.ctl->div_done) return;
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   const float k_i = A.kfac[s];
   A.kappa_v[s] += k_i;
@@ -620,12 +625,12 @@ __global__ __launch_bounds__(256) void k_div_iter(SimParams p, DevArrays A) {
 // compute_divergence_solve_density_error (runner.hpp:3396-3410) + FP64
 // partial sums of the residual
 template <bool W>
-__global__ __launch_bounds__(256) void k_div_err(SimParams p, DevArrays A) {
+__global__ __launch_bounds__(256) void k_div_err(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
+  ALV_SCENE(T, A, s);
   if (A.ctl->div_done) return;
-  const unsigned s = ALV_TID();
   double part = 0.0;
-  if (s < p.n) {
+  if (s < p.first + p.n) {
     float dadv = density_change<W>(p, A, s);
     A.rho_adv[s] = dadv;
     const float k = dadv * A.alpha[s];
@@ -635,13 +640,13 @@ __global__ __launch_bounds__(256) void k_div_err(SimParams p, DevArrays A) {
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) A.partials[xcd_block()] = t;
 }
-__global__ __launch_bounds__(256) void k_div_finalize(SimParams p, DevArrays A,
-                                                      unsigned nblocks) {
+__global__ __launch_bounds__(256) void k_div_finalize(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
+  ALV_SCENE_BLOCK(T, A);
   StepCtl* ctl = A.ctl;
   if (ctl->div_done) return;
   double part = 0.0;
-  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) part += A.partials[b];
+  for (unsigned b = blk_begin + threadIdx.x; b < blk_end; b += blockDim.x) part += A.partials[b];
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) {
     float mean = float(t / double(p.n));
@@ -655,7 +660,7 @@ __global__ __launch_bounds__(256) void k_div_finalize(SimParams p, DevArrays A,
 
 // Control block reset before the solves of a step.
 __global__ void k_ctl_reset(DevArrays A) {
-  StepCtl* ctl = A.ctl;
+  StepCtl* ctl = A.ctl + blockIdx.x;
   ctl->div_iter = 0u;
   ctl->dens_iter = 0u;
   const float mx = FLT_MAX;
@@ -687,12 +692,12 @@ __device__ inline void orthogonal_vectors(vec3 n, vec3& t1, vec3& t2) {
 }
 
 template <bool W>
-__global__ __launch_bounds__(256) void k_viscosity(SimParams p, DevArrays A, int div_finish) {
+__global__ __launch_bounds__(256) void k_viscosity(SceneTab T, DevArrays A, int div_finish) {
   __shared__ float lds[4];
-  const unsigned s = ALV_TID();
+  ALV_SCENE(T, A, s);
   const float dt = A.ctl->dt;
   float cfl = -FLT_MAX;
-  if (s < p.n) {
+  if (s < p.first + p.n) {
     const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
     const unsigned nn = A.nnb[s];
     vec3 da = mk(0.f, 0.f, 0.f);
@@ -768,10 +773,11 @@ __global__ __launch_bounds__(256) void k_viscosity(SimParams p, DevArrays A, int
 }
 
 // Solver::update_dt (solver.hpp:153-169) on the device.
-__global__ __launch_bounds__(256) void k_update_dt(SimParams p, DevArrays A, unsigned nblocks) {
+__global__ __launch_bounds__(256) void k_update_dt(SceneTab T, DevArrays A) {
   __shared__ float lds[4];
+  ALV_SCENE_BLOCK(T, A);
   float m = -FLT_MAX;
-  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) m = std_max(m, A.fpartials[b]);
+  for (unsigned b = blk_begin + threadIdx.x; b < blk_end; b += blockDim.x) m = std_max(m, A.fpartials[b]);
   float pm = block_max_f(m, lds);
   if (threadIdx.x == 0) {
     StepCtl* ctl = A.ctl;
@@ -798,9 +804,9 @@ __global__ __launch_bounds__(256) void k_update_dt(SimParams p, DevArrays A, uns
 }
 
 // integrate_non_pressure_acceleration (runner.hpp:3422-3428)
-__global__ __launch_bounds__(256) void k_integrate_np(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_integrate_np(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   add_v(A, s, dt * ld3(A.a[s]));
 }
@@ -808,9 +814,9 @@ __global__ __launch_bounds__(256) void k_integrate_np(SimParams p, DevArrays A) 
 // warm_start_pressure_solve0 (runner.hpp:3430-3450)
 // L: learned warm start (prediction from k_ws_learned)
 template <bool W, bool L>
-__global__ __launch_bounds__(256) void k_ws_p0(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_ws_p0(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   float dadv = density_adv<W>(p, A, s, dt);
   float k = (dadv > 1.0f) ? (0.5f * std_max(A.kappa[s], -0.00025f) / (dt * dt)) : 0.0f;
@@ -822,18 +828,18 @@ __global__ __launch_bounds__(256) void k_ws_p0(SimParams p, DevArrays A) {
 }
 // warm_start_pressure_solve1 (runner.hpp:3452-3491)
 template <bool W>
-__global__ __launch_bounds__(256) void k_ws_p1(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_ws_p1(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   vec3 dv = stiffness_dv<W, false>(p, A, s, A.kappa[s], A.kappa, dt);
   add_v(A, s, dv);
 }
 // compute_rho_adv (runner.hpp:3525-3541); stores k = (rho_adv - 1) * alpha
 template <bool W>
-__global__ __launch_bounds__(256) void k_rho_adv(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_rho_adv(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   float dadv = density_adv<W>(p, A, s, dt);
   A.rho_adv[s] = dadv;
@@ -844,10 +850,10 @@ __global__ __launch_bounds__(256) void k_rho_adv(SimParams p, DevArrays A) {
 }
 // pressure_solve_iteration (runner.hpp:3543-3588)
 template <bool W>
-__global__ __launch_bounds__(256) void k_p_iter(SimParams p, DevArrays A) {
+__global__ __launch_bounds__(256) void k_p_iter(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
   if (A.ctl->dens_done) return;
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   const float k_i = A.kfac[s];
   A.kappa[s] += k_i;
@@ -857,13 +863,13 @@ __global__ __launch_bounds__(256) void k_p_iter(SimParams p, DevArrays A) {
 // compute_pressure_solve_density_error (runner.hpp:3590-3604) + FP64
 // partial sums of (rho_adv - 1)
 template <bool W>
-__global__ __launch_bounds__(256) void k_p_err(SimParams p, DevArrays A) {
+__global__ __launch_bounds__(256) void k_p_err(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
+  ALV_SCENE(T, A, s);
   if (A.ctl->dens_done) return;
-  const unsigned s = ALV_TID();
   const float dt = A.ctl->dt;
   double part = 0.0;
-  if (s < p.n) {
+  if (s < p.first + p.n) {
     float dadv = density_adv<W>(p, A, s, dt);
     A.rho_adv[s] = dadv;
     const float k = (dadv - 1.0f) * A.alpha[s];
@@ -873,12 +879,13 @@ __global__ __launch_bounds__(256) void k_p_err(SimParams p, DevArrays A) {
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) A.partials[xcd_block()] = t;
 }
-__global__ __launch_bounds__(256) void k_p_finalize(SimParams p, DevArrays A, unsigned nblocks) {
+__global__ __launch_bounds__(256) void k_p_finalize(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
+  ALV_SCENE_BLOCK(T, A);
   StepCtl* ctl = A.ctl;
   if (ctl->dens_done) return;
   double part = 0.0;
-  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) part += A.partials[b];
+  for (unsigned b = blk_begin + threadIdx.x; b < blk_end; b += blockDim.x) part += A.partials[b];
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) {
     float mean = float(t / double(p.n));
@@ -893,9 +900,9 @@ __global__ __launch_bounds__(256) void k_p_finalize(SimParams p, DevArrays A, un
 // pressure_solve_finish + integrate_velocity<wrap> (runner.hpp:3606-3623),
 // then the state goes back to user order.
 template <bool W>
-__global__ __launch_bounds__(256) void k_df_finish(SimParams p, DevArrays A, int dens_finish) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_df_finish(SceneTab T, DevArrays A, int dens_finish) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   const unsigned i = A.sorted_id[s];
   vec3 v = ld3(A.v[s]);
@@ -911,20 +918,20 @@ __global__ __launch_bounds__(256) void k_df_finish(SimParams p, DevArrays A, int
 // compute_velocity_of_density_change (runner.hpp:3297-3314), the density
 // change sum alone (no stiffness factor update): rho_adv <- rho_dot.
 template <bool W>
-__global__ __launch_bounds__(256) void k_density_change(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_density_change(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   A.rho_adv[s] = density_change<W>(p, A, s);
 }
 
 // Training pairs of the step just taken, in sorted order: the features
 // recorded at its divergence warm start and the converged stored stiffness
 // (kappa_v, kappa) it ended with.
-__global__ __launch_bounds__(256) void k_export_pairs(SimParams p, DevArrays A,
+__global__ __launch_bounds__(256) void k_export_pairs(SceneTab T, DevArrays A,
                                                       float* __restrict__ x,
                                                       float* __restrict__ y) {
-  const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= p.n) return;
+  ALV_SCENE_LINEAR(T, A, s);
+  if (s >= p.first + p.n) return;
   const float4 f0 = A.feat[2 * size_t(s)], f1 = A.feat[2 * size_t(s) + 1];
   float* xo = x + size_t(s) * kMlpIn;
   xo[0] = f0.x; xo[1] = f0.y; xo[2] = f0.z; xo[3] = f0.w;
@@ -937,9 +944,9 @@ __global__ __launch_bounds__(256) void k_export_pairs(SimParams p, DevArrays A,
 // integrate_velocity<wrap> (runner.hpp:3613-3623) on the user-order state
 // x <- x + dt v, wrapped on periodic axes.
 template <bool W>
-__global__ __launch_bounds__(256) void k_integrate_velocity(SimParams p, DevArrays A, float dt) {
-  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.n) return;
+__global__ __launch_bounds__(256) void k_integrate_velocity(SceneTab T, DevArrays A, float dt) {
+  ALV_SCENE_LINEAR(T, A, i);
+  if (i >= p.first + p.n) return;
   vec3 x = wrapd<W>(ld3(A.ux, i) + dt * ld3(A.uv, i), p);
   st3(A.ux, i, x);
 }
@@ -949,9 +956,9 @@ __global__ __launch_bounds__(256) void k_integrate_velocity(SimParams p, DevArra
 // ---------------------------------------------------------------------------
 // predict_advection0 (runner.hpp:2609-2639)
 template <bool W>
-__global__ __launch_bounds__(256) void k_ii_predict0(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_ii_predict0(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   add_v(A, s, dt * ld3(A.a[s]));
   A.last_pressure[s] = A.pressure[s] * 0.5f;
@@ -985,9 +992,9 @@ __global__ __launch_bounds__(256) void k_ii_predict0(SimParams p, DevArrays A) {
 }
 // predict_advection1 (runner.hpp:2640-2690)
 template <bool W>
-__global__ __launch_bounds__(256) void k_ii_predict1(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_ii_predict1(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]), d_ii = ld3(A.dii[s]);
   const float dens = A.density[s];
@@ -1037,10 +1044,10 @@ __global__ __launch_bounds__(256) void k_ii_predict1(SimParams p, DevArrays A) {
 // from the second iteration on, neighbour values are read from `pressure`
 // and each particle refreshes its own last_pressure.
 template <bool W>
-__global__ __launch_bounds__(256) void k_ii_iter0(SimParams p, DevArrays A) {
+__global__ __launch_bounds__(256) void k_ii_iter0(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
   if (A.ctl->dens_done) return;
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+  if (s >= p.first + p.n) return;
   const float* lp = (A.ctl->dens_iter == 0u) ? A.last_pressure : A.pressure;
   const vec3 xi = ld3(A.x[s]);
   vec3 acc = mk(0.f, 0.f, 0.f);
@@ -1071,13 +1078,13 @@ __global__ __launch_bounds__(256) void k_ii_iter0(SimParams p, DevArrays A) {
 }
 // pressure_solve_iteration1 (runner.hpp:2716-2772) + FP64 partial sums
 template <bool W>
-__global__ __launch_bounds__(256) void k_ii_iter1(SimParams p, DevArrays A) {
+__global__ __launch_bounds__(256) void k_ii_iter1(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
+  ALV_SCENE(T, A, s);
   if (A.ctl->dens_done) return;
-  const unsigned s = ALV_TID();
   const float dt = A.ctl->dt;
   double part = 0.0;
-  if (s < p.n) {
+  if (s < p.first + p.n) {
     const vec3 xi = ld3(A.x[s]);
     const float lp = A.last_pressure[s];
     const vec3 dpj = ld3(A.dij_pj[s]);
@@ -1134,12 +1141,13 @@ __global__ __launch_bounds__(256) void k_ii_iter1(SimParams p, DevArrays A) {
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) A.partials[xcd_block()] = t;
 }
-__global__ __launch_bounds__(256) void k_ii_finalize(SimParams p, DevArrays A, unsigned nblocks) {
+__global__ __launch_bounds__(256) void k_ii_finalize(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
+  ALV_SCENE_BLOCK(T, A);
   StepCtl* ctl = A.ctl;
   if (ctl->dens_done) return;
   double part = 0.0;
-  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) part += A.partials[b];
+  for (unsigned b = blk_begin + threadIdx.x; b < blk_end; b += blockDim.x) part += A.partials[b];
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) {
     float mean = float(t / double(p.n));
@@ -1152,9 +1160,9 @@ __global__ __launch_bounds__(256) void k_ii_finalize(SimParams p, DevArrays A, u
 }
 // compute_pressure_accels (runner.hpp:2774-2817)
 template <bool W>
-__global__ __launch_bounds__(256) void k_ii_pressure_accel(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_ii_pressure_accel(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const vec3 xi = ld3(A.x[s]);
   const float dens = A.density[s];
   const float dens2 = dens * dens;
@@ -1204,9 +1212,9 @@ __global__ __launch_bounds__(256) void k_ii_pressure_accel(SimParams p, DevArray
 }
 // kinematic_integration<wrap> (runner.hpp:2873-2887) + back to user order
 template <bool W>
-__global__ __launch_bounds__(256) void k_ii_finish(SimParams p, DevArrays A) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_ii_finish(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   const unsigned i = A.sorted_id[s];
   vec3 v = ld3(A.v[s]);
@@ -1224,8 +1232,9 @@ __global__ __launch_bounds__(256) void k_ii_finish(SimParams p, DevArrays A) {
 // ---------------------------------------------------------------------------
 constexpr unsigned kBodyBlocks = 256;
 
-__global__ __launch_bounds__(256) void k_body_partials(SimParams p, DevArrays A) {
+__global__ __launch_bounds__(256) void k_body_partials(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
+  const SimParams p = T.par[0];
   const unsigned b = blockIdx.y;
   if (!A.bnd[b].dynamic) return;
   const unsigned bit = 1u << b;
@@ -1249,8 +1258,9 @@ __global__ __launch_bounds__(256) void k_body_partials(SimParams p, DevArrays A)
   }
 }
 
-__global__ __launch_bounds__(256) void k_body_sums(SimParams p, DevArrays A, unsigned nblk) {
+__global__ __launch_bounds__(256) void k_body_sums(SceneTab T, DevArrays A, unsigned nblk) {
   __shared__ double lds[4];
+  const SimParams p = T.par[0];
   const unsigned b = blockIdx.x;
   if (!A.bnd[b].dynamic) return;
   for (int c = 0; c < 6; ++c) {
@@ -1264,9 +1274,9 @@ __global__ __launch_bounds__(256) void k_body_sums(SimParams p, DevArrays A, uns
 }
 
 // sorted -> user order for the API arrays that are synchronised lazily
-__global__ __launch_bounds__(256) void k_unsort(SimParams p, DevArrays A, unsigned mask) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+__global__ __launch_bounds__(256) void k_unsort(SceneTab T, DevArrays A, unsigned mask) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const unsigned i = A.sorted_id[s];
   if (mask & ALV_F_DENSITY) A.udensity[i] = A.density[s];
   if (mask & ALV_F_ALPHA) A.ualpha[i] = A.alpha[s];
@@ -1275,10 +1285,10 @@ __global__ __launch_bounds__(256) void k_unsort(SimParams p, DevArrays A, unsign
   if (mask & ALV_F_NNB) A.unnb[i] = A.nnb[s];
 }
 
-__global__ __launch_bounds__(256) void k_export_neighbors(SimParams p, DevArrays A,
+__global__ __launch_bounds__(256) void k_export_neighbors(SceneTab T, DevArrays A,
                                                           unsigned* ids, unsigned* counts) {
-  const unsigned s = ALV_TID();
-  if (s >= p.n) return;
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
   const unsigned i = A.sorted_id[s];
   const unsigned nn = A.nnb[s];
   counts[i] = nn;
@@ -1377,33 +1387,62 @@ KernelProfiler* g_profiler = nullptr;
     hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, s, __VA_ARGS__); \
     if (g_profiler) g_profiler->end(#kernel, s);            \
   } while (0)
-#define ALV_LAUNCH(kernel, n, s, ...) ALV_LAUNCH_GRID(kernel, dim3(grid_for(n)), s, __VA_ARGS__)
+// one thread per particle slot of every scene
+#define ALV_LAUNCH(kernel, L, s, ...) ALV_LAUNCH_GRID(kernel, dim3((L).pblocks), s, (L).T, __VA_ARGS__)
 // kernels templated on the periodic wrap and one more flag
-#define ALV_LAUNCH_W2(kernel, flag, n, s, p, ...)                                 \
+#define ALV_LAUNCH_W2(kernel, flag, L, s, ...)                                    \
   do {                                                                            \
     if (g_profiler) g_profiler->begin(#kernel, s);                                \
-    if ((p).wrap_mask)                                                            \
-      hipLaunchKernelGGL((kernel<true, flag>), dim3(grid_for(n)), dim3(kBlock), 0, s, p, \
+    if ((L).p0.wrap_mask)                                                         \
+      hipLaunchKernelGGL((kernel<true, flag>), dim3((L).pblocks), dim3(kBlock), 0, s, (L).T, \
                          __VA_ARGS__);                                            \
     else                                                                          \
-      hipLaunchKernelGGL((kernel<false, flag>), dim3(grid_for(n)), dim3(kBlock), 0, s, p, \
+      hipLaunchKernelGGL((kernel<false, flag>), dim3((L).pblocks), dim3(kBlock), 0, s, (L).T, \
                          __VA_ARGS__);                                            \
     if (g_profiler) g_profiler->end(#kernel, s);                                  \
   } while (0)
 // kernels templated on the periodic wrap: one name for both instances
-#define ALV_LAUNCH_W(kernel, n, s, p, ...)                                        \
+#define ALV_LAUNCH_W(kernel, L, s, ...)                                           \
   do {                                                                            \
     if (g_profiler) g_profiler->begin(#kernel, s);                                \
-    if ((p).wrap_mask)                                                            \
-      hipLaunchKernelGGL(kernel<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, p, __VA_ARGS__); \
+    if ((L).p0.wrap_mask)                                                         \
+      hipLaunchKernelGGL(kernel<true>, dim3((L).pblocks), dim3(kBlock), 0, s, (L).T, __VA_ARGS__); \
     else                                                                          \
-      hipLaunchKernelGGL(kernel<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, p, __VA_ARGS__); \
+      hipLaunchKernelGGL(kernel<false>, dim3((L).pblocks), dim3(kBlock), 0, s, (L).T, __VA_ARGS__); \
     if (g_profiler) g_profiler->end(#kernel, s);                                  \
   } while (0)
+// one block per scene
+#define ALV_LAUNCH_SCENES(kernel, L, s, ...) \
+  ALV_LAUNCH_GRID(kernel, dim3((L).scenes), s, (L).T, __VA_ARGS__)
 #define ALV_LAUNCH1(kernel, s, ...) ALV_LAUNCH_GRID(kernel, dim3(1), s, __VA_ARGS__)
 
-void launch_cell_count(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_cell_count, p.n, s, p, A);
+__global__ __launch_bounds__(256) void k_fill_u32(unsigned* __restrict__ d, unsigned v,
+                                                  size_t n) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) d[i] = v;
+}
+void launch_fill_u32(unsigned* dst, unsigned value, size_t count, hipStream_t s) {
+  if (!count) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(unsigned((count + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     s, dst, value, count);
+}
+
+// The slots of a scene's range past its particles are counted into its
+// outside-the-grid bin, so that after the scan every scene's sorted range
+// starts where its user range starts.
+__global__ void k_reserve_padding(SceneTab T, DevArrays A, unsigned scenes) {
+  const unsigned sc = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sc >= scenes) return;
+  const SimParams p = T.par[sc];
+  A.cell_count[p.cell_base + p.num_cells] += p.span - p.n;
+}
+void launch_reserve_padding(const Launch& L, const DevArrays& A, hipStream_t s) {
+  hipLaunchKernelGGL(k_reserve_padding, dim3((L.scenes + 63) / 64), dim3(64), 0, s, L.T, A,
+                     L.scenes);
+}
+
+void launch_cell_count(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_cell_count, L, s, A);
 }
 void launch_scan(unsigned count, const unsigned* in, unsigned* out, unsigned* tmp,
                  hipStream_t s) {
@@ -1412,109 +1451,101 @@ void launch_scan(unsigned count, const unsigned* in, unsigned* out, unsigned* tm
   ALV_LAUNCH1(k_scan_top, s, nb, tmp);
   ALV_LAUNCH_GRID(k_scan_apply, dim3(nb), s, count, in, out, (const unsigned*)tmp);
 }
-void launch_scatter(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_scatter, p.n, s, p, A);
+void launch_scatter(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_scatter, L, s, A);
 }
-void launch_cell_sort(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_cell_sort, p.num_cells, s, p, A);
+void launch_cell_sort(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_GRID(k_cell_sort, dim3(L.cblocks), s, L.T, A);
 }
-void launch_gather_boundary(const SimParams& p, const DevArrays& A, int kind, hipStream_t s) {
-  ALV_LAUNCH(k_gather_boundary, p.n, s, p, A, kind);
+void launch_gather_boundary(const Launch& L, const DevArrays& A, int kind, hipStream_t s) {
+  ALV_LAUNCH(k_gather_boundary, L, s, A, kind);
 }
-void launch_neighbors(const SimParams& p, const DevArrays& A, int wrap_any, int with_factor,
-                      hipStream_t s) {
-  (void)wrap_any;
-  ALV_LAUNCH_W(k_neighbors, p.n, s, p, A, with_factor);
+void launch_neighbors(const Launch& L, const DevArrays& A, int with_factor, hipStream_t s) {
+  ALV_LAUNCH_W(k_neighbors, L, s, A, with_factor);
 }
-void launch_df_warm_start_div(const SimParams& p, const DevArrays& A, hipStream_t s,
+void launch_df_warm_start_div(const Launch& L, const DevArrays& A, hipStream_t s,
                               int features, const MlpDev* net, int net_divergence) {
   if (features || net)
-    ALV_LAUNCH_W2(k_ws_div0, true, p.n, s, p, A);
+    ALV_LAUNCH_W2(k_ws_div0, true, L, s, A);
   else
-    ALV_LAUNCH_W2(k_ws_div0, false, p.n, s, p, A);
-  if (net) ALV_LAUNCH(k_ws_learned, p.n, s, p, A, *net, net_divergence);
-  ALV_LAUNCH_W(k_ws_div1, p.n, s, p, A);
-  ALV_LAUNCH_W(k_vel_density_change, p.n, s, p, A);
+    ALV_LAUNCH_W2(k_ws_div0, false, L, s, A);
+  if (net) ALV_LAUNCH(k_ws_learned, L, s, A, *net, net_divergence);
+  ALV_LAUNCH_W(k_ws_div1, L, s, A);
+  ALV_LAUNCH_W(k_vel_density_change, L, s, A);
 }
-void launch_df_div_loop_prep(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  (void)p;
-  (void)A;
-  (void)s;
+void launch_df_div_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_W(k_div_iter, L, s, A);
+  ALV_LAUNCH_W(k_div_err, L, s, A);
+  ALV_LAUNCH_SCENES(k_div_finalize, L, s, A);
 }
-void launch_df_div_iteration(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_div_iter, p.n, s, p, A);
-  ALV_LAUNCH_W(k_div_err, p.n, s, p, A);
-  ALV_LAUNCH1(k_div_finalize, s, p, A, grid_for(p.n));
+void launch_df_viscosity(const Launch& L, const DevArrays& A, int div_finish, hipStream_t s) {
+  ALV_LAUNCH_W(k_viscosity, L, s, A, div_finish);
 }
-void launch_df_viscosity(const SimParams& p, const DevArrays& A, int div_finish,
-                         hipStream_t s) {
-  ALV_LAUNCH_W(k_viscosity, p.n, s, p, A, div_finish);
+void launch_update_dt(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_SCENES(k_update_dt, L, s, A);
 }
-void launch_update_dt(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH1(k_update_dt, s, p, A, grid_for(p.n));
+void launch_df_integrate_np(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_integrate_np, L, s, A);
 }
-void launch_df_integrate_np(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_integrate_np, p.n, s, p, A);
-}
-void launch_df_warm_start_dens(const SimParams& p, const DevArrays& A, hipStream_t s,
+void launch_df_warm_start_dens(const Launch& L, const DevArrays& A, hipStream_t s,
                                int learned) {
   if (learned)
-    ALV_LAUNCH_W2(k_ws_p0, true, p.n, s, p, A);
+    ALV_LAUNCH_W2(k_ws_p0, true, L, s, A);
   else
-    ALV_LAUNCH_W2(k_ws_p0, false, p.n, s, p, A);
-  ALV_LAUNCH_W(k_ws_p1, p.n, s, p, A);
-  ALV_LAUNCH_W(k_rho_adv, p.n, s, p, A);
+    ALV_LAUNCH_W2(k_ws_p0, false, L, s, A);
+  ALV_LAUNCH_W(k_ws_p1, L, s, A);
+  ALV_LAUNCH_W(k_rho_adv, L, s, A);
 }
-void launch_df_dens_iteration(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_p_iter, p.n, s, p, A);
-  ALV_LAUNCH_W(k_p_err, p.n, s, p, A);
-  ALV_LAUNCH1(k_p_finalize, s, p, A, grid_for(p.n));
+void launch_df_dens_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_W(k_p_iter, L, s, A);
+  ALV_LAUNCH_W(k_p_err, L, s, A);
+  ALV_LAUNCH_SCENES(k_p_finalize, L, s, A);
 }
-void launch_df_finish(const SimParams& p, const DevArrays& A, int dens_finish, hipStream_t s) {
-  ALV_LAUNCH_W(k_df_finish, p.n, s, p, A, dens_finish);
+void launch_df_finish(const Launch& L, const DevArrays& A, int dens_finish, hipStream_t s) {
+  ALV_LAUNCH_W(k_df_finish, L, s, A, dens_finish);
 }
-void launch_export_pairs(const SimParams& p, const DevArrays& A, float* x, float* y,
+void launch_export_pairs(const Launch& L, const DevArrays& A, float* x, float* y,
                          hipStream_t s) {
-  ALV_LAUNCH(k_export_pairs, p.n, s, p, A, x, y);
+  ALV_LAUNCH(k_export_pairs, L, s, A, x, y);
 }
-void launch_density_change(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_density_change, p.n, s, p, A);
+void launch_density_change(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_W(k_density_change, L, s, A);
 }
-void launch_integrate_velocity(const SimParams& p, const DevArrays& A, float dt, hipStream_t s) {
-  ALV_LAUNCH_W(k_integrate_velocity, p.n, s, p, A, dt);
+void launch_integrate_velocity(const Launch& L, const DevArrays& A, float dt, hipStream_t s) {
+  ALV_LAUNCH_W(k_integrate_velocity, L, s, A, dt);
 }
-void launch_ii_viscosity_predict(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_viscosity, p.n, s, p, A, 0);
+void launch_ii_viscosity_predict(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_W(k_viscosity, L, s, A, 0);
 }
-void launch_ii_predict(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_ii_predict0, p.n, s, p, A);
-  ALV_LAUNCH_W(k_ii_predict1, p.n, s, p, A);
+void launch_ii_predict(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_W(k_ii_predict0, L, s, A);
+  ALV_LAUNCH_W(k_ii_predict1, L, s, A);
 }
-void launch_ii_iteration(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_ii_iter0, p.n, s, p, A);
-  ALV_LAUNCH_W(k_ii_iter1, p.n, s, p, A);
-  ALV_LAUNCH1(k_ii_finalize, s, p, A, grid_for(p.n));
+void launch_ii_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_W(k_ii_iter0, L, s, A);
+  ALV_LAUNCH_W(k_ii_iter1, L, s, A);
+  ALV_LAUNCH_SCENES(k_ii_finalize, L, s, A);
 }
-void launch_ii_finish(const SimParams& p, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_ii_pressure_accel, p.n, s, p, A);
-  ALV_LAUNCH_W(k_ii_finish, p.n, s, p, A);
+void launch_ii_finish(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_W(k_ii_pressure_accel, L, s, A);
+  ALV_LAUNCH_W(k_ii_finish, L, s, A);
 }
-void launch_body_sums(const SimParams& p, const DevArrays& A, hipStream_t s) {
+void launch_body_sums(const Launch& L, const DevArrays& A, hipStream_t s) {
+  const SimParams& p = L.p0;
   if (p.num_boundaries == 0) return;
   const unsigned nblk = std::min(kBodyBlocks, std::max(1u, grid_for(p.n)));
-  ALV_LAUNCH_GRID(k_body_partials, dim3(nblk, p.num_boundaries), s, p, A);
-  ALV_LAUNCH_GRID(k_body_sums, dim3(p.num_boundaries), s, p, A, nblk);
+  ALV_LAUNCH_GRID(k_body_partials, dim3(nblk, p.num_boundaries), s, L.T, A);
+  ALV_LAUNCH_GRID(k_body_sums, dim3(p.num_boundaries), s, L.T, A, nblk);
 }
-void launch_unsort_fields(const SimParams& p, const DevArrays& A, unsigned mask,
-                          hipStream_t s) {
-  ALV_LAUNCH(k_unsort, p.n, s, p, A, mask);
+void launch_unsort_fields(const Launch& L, const DevArrays& A, unsigned mask, hipStream_t s) {
+  ALV_LAUNCH(k_unsort, L, s, A, mask);
 }
-void launch_export_neighbors(const SimParams& p, const DevArrays& A, unsigned* ids,
+void launch_export_neighbors(const Launch& L, const DevArrays& A, unsigned* ids,
                              unsigned* counts, hipStream_t s) {
-  ALV_LAUNCH(k_export_neighbors, p.n, s, p, A, ids, counts);
+  ALV_LAUNCH(k_export_neighbors, L, s, A, ids, counts);
 }
-void launch_ctl_reset(const DevArrays& A, hipStream_t s) {
-  hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(1), 0, s, A);
+void launch_ctl_reset(const Launch& L, const DevArrays& A, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctl_reset, dim3(L.scenes), dim3(1), 0, s, A);
 }
 void launch_volume_field_analytic(const SimParams& p, const BoundaryDev& b, float* volume,
                                   const float* peirce, unsigned begin, unsigned end,

@@ -26,6 +26,12 @@ struct SimParams {
   unsigned kmax, cell_cap, num_cells;
   unsigned num_boundaries;
   unsigned n;  // particles in this step
+  // Where this scene sits in the solver's arrays (one solver can hold many
+  // independent scenes): particles [first, first + n) of a range of `span`
+  // slots, a multiple of the 256-thread block, so that no block straddles two
+  // scenes; cells [cell_base, cell_base + num_cells] (the last one is the bin
+  // of particles outside the grid); boundaries from bnd_base.
+  unsigned first, span, cell_base, bnd_base;
 };
 
 // Geometry of one 32-node cubic Lagrange map (Pile lists, pile.hpp:106-112).

@@ -335,6 +335,25 @@ PYBIND11_MODULE(_alluvion, m) {
         .def("update_particle_neighbors", &S::template update_particle_neighbors<0>)
         .def("update_particle_neighbors_wrap1", &S::template update_particle_neighbors<1>)
         .def("compute_density_change", &S::compute_density_change)
+        .def_property_readonly("num_scenes", &S::num_scenes)
+        .def("set_graph", &S::set_graph, py::arg("enable"), py::arg("slack") = 0)
+        .def("scene_range", &S::scene_range, py::arg("scene"))
+        .def("scene_report",
+             [](S const& s, U k) {
+               const alv_solver_report r = s.scene_report(k);
+               py::dict d;
+               d["num_divergence_solve"] = r.num_divergence_solve;
+               d["num_density_solve"] = r.num_density_solve;
+               d["mean_density_change"] = r.mean_density_change;
+               d["mean_density_error"] = r.mean_density_error;
+               d["max_v2"] = r.max_v2;
+               d["cfl_dt"] = r.cfl_dt;
+               d["utilized_cfl"] = r.utilized_cfl;
+               d["t"] = r.t;
+               d["dt"] = r.dt;
+               return d;
+             },
+             py::arg("scene"))
         .def("set_warm_start_net", &S::set_warm_start_net, py::arg("net").none(true),
              py::arg("divergence") = true, py::arg("density") = true, py::keep_alive<1, 2>())
         .def("set_feature_log", &S::set_feature_log, py::arg("enable"))
@@ -381,6 +400,21 @@ PYBIND11_MODULE(_alluvion, m) {
          py::arg("num_ushers") = 0, py::arg("enable_surface_tension") = false,
          py::arg("enable_vorticity") = false, py::arg("cn") = nullptr, py::arg("cni") = nullptr,
          py::arg("graphical") = false, py::keep_alive<1, 3>())
+      // batch of independent scenes (no counterpart in the reference)
+      .def(py::init([](Store& store, std::vector<Pile<float>*> piles, std::vector<U> num_particles,
+                       std::vector<uint3> grid_res, std::vector<int3> grid_offset,
+                       Const<float> const* cn, ConstiN const* cni) {
+             const size_t n = piles.size();
+             if (num_particles.size() != n || grid_res.size() != n || grid_offset.size() != n)
+               throw std::invalid_argument("one entry per scene in every list");
+             std::vector<SceneSpec<float>> sc(n);
+             for (size_t k = 0; k < n; ++k)
+               sc[k] = SceneSpec<float>{piles[k], num_particles[k], grid_res[k], grid_offset[k]};
+             return new DF(store, sc, cn, cni);
+           }),
+           py::arg("store"), py::arg("piles"), py::arg("num_particles"), py::arg("grid_res"),
+           py::arg("grid_offset"), py::arg("cn") = nullptr, py::arg("cni") = nullptr,
+           py::keep_alive<1, 3>())
       .def_property_readonly("num_divergence_solve",
                              [](DF const& s) { return s.report().num_divergence_solve; })
       .def_property_readonly("num_density_solve", [](DF const& s) { return s.report().num_density_solve; })

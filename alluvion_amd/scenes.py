@@ -79,6 +79,9 @@ class Scene:
     max_num_particles_per_cell: int = 64
     dt: float = 2e-4
     cfl: float = 0.4
+    # time-step range; None = dt (fixed step)
+    min_dt: Optional[float] = None
+    max_dt: Optional[float] = None
     density_change_tolerance: float = 1e-3
     density_error_tolerance: float = 1e-3
     min_divergence_solve: int = 1
@@ -122,6 +125,17 @@ def dam_break_small(seed: int = 1) -> Scene:
         steps=1000,
         boundaries=[BoundarySpec("box", (0.4, 0.4, 0.4, 0.0), -1.0, (64, 64, 64))],
     )
+
+
+def adaptive_step(scene: Scene, cfl: float = 0.4, max_dt: float = 2e-3,
+                  min_dt: float = 1e-6) -> Scene:
+    """The same scene with the CFL-limited time step (Solver::update_dt,
+    solver.hpp:153-169) instead of a fixed one."""
+    scene.cfl = cfl
+    scene.dt = max_dt
+    scene.max_dt = max_dt
+    scene.min_dt = min_dt
+    return scene
 
 
 def dam_break_rigid(seed: int = 2, body_seed: int = 3, nx: int = 128, ny: int = 128,
@@ -216,8 +230,8 @@ def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optiona
     solver = cls(runner, pile, dp, n_max, cn=cn, cni=cni)
     solver.num_particles = scene.n
     solver.dt = scene.dt
-    solver.max_dt = scene.dt
-    solver.min_dt = scene.dt
+    solver.max_dt = scene.dt if scene.max_dt is None else scene.max_dt
+    solver.min_dt = scene.dt if scene.min_dt is None else scene.min_dt
     solver.cfl = scene.cfl
     solver.density_error_tolerance = scene.density_error_tolerance
     solver.min_density_solve = scene.min_density_solve
@@ -229,3 +243,96 @@ def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optiona
     dp.coat(solver.particle_x).set(scene.x)
     dp.coat(solver.particle_v).set(scene.v)
     return dp, pile, solver, cn, cni
+
+
+def dam_break_box(n_target: int, edge: float, seed: int, map_res: int = 64) -> Scene:
+    """A dam break of about `n_target` particles in a cube container of edge
+    `edge`: a square lattice column (x-z side up to the container's inner
+    width) filled to the needed height, first layer at h from the walls."""
+    r, h = 0.0025, 0.01
+    fit = int(np.floor((edge - 2 * h) / (2 * r) + 1e-9)) + 1  # lattice points across
+    side = max(2, min(fit, int(round(n_target ** (1.0 / 3.0)))))
+    ny = max(2, min(fit, int(np.ceil(n_target / (side * side)))))
+    x = lattice_block(side, ny, side, r, -0.5 * edge + h - 2 * r, seed, 0.01 * r)
+    c = int(np.ceil(0.5 * edge / h - 1e-9)) + 1
+    return Scene(
+        name="dam_break_box_%d" % x.shape[0],
+        x=x,
+        v=np.zeros_like(x),
+        grid_offset=(-c, -c, -c),
+        grid_res=(2 * c, 2 * c, 2 * c),
+        steps=500,
+        boundaries=[BoundarySpec("box", (edge, edge, edge, 0.0), -1.0, (map_res,) * 3)],
+    )
+
+
+def batched_dam_breaks(num_scenes: int = 256, seed: int = 6, n_range=(4000, 120000),
+                       edge_range=(0.2, 0.6), map_res: int = 64) -> List[Scene]:
+    """Independent dam breaks: particle counts log-uniform in n_range, cube
+    edges uniform in edge_range (SplitMix64 from `seed`)."""
+    u = uniform(seed, 2 * num_scenes).reshape(num_scenes, 2)
+    lo, hi = np.log(n_range[0]), np.log(n_range[1])
+    out = []
+    for k in range(num_scenes):
+        n = int(round(np.exp(lo + (hi - lo) * u[k, 0])))
+        edge = edge_range[0] + (edge_range[1] - edge_range[0]) * u[k, 1]
+        out.append(dam_break_box(n, edge, seed * 1000 + k, map_res))
+    return out
+
+
+def build_batch(scenes: List[Scene], dp=None, log=None):
+    """One batch solver for several scenes that share the constants of the
+    first one (each keeps its own particles, grid and container).  Returns
+    (dp, piles, solver, cn, cni); every scene's state is uploaded into its
+    range of the particle arrays."""
+    from . import Depot, _alluvion as _al
+
+    if dp is None:
+        dp = Depot(np.float32)
+    s0 = scenes[0]
+    cn, cni = dp.create_cn()
+    cn.set_kernel_radius(s0.kernel_radius)
+    cn.set_particle_attr(s0.particle_radius, s0.particle_mass, s0.density0)
+    cn.gravity = dp.f3(*s0.gravity)
+    cn.viscosity = s0.viscosity
+    cn.boundary_viscosity = s0.boundary_viscosity
+    cn.dfsph_factor_epsilon = s0.dfsph_factor_epsilon
+    cni.max_num_neighbors_per_particle = s0.max_num_neighbors
+    cni.max_num_particles_per_cell = s0.max_num_particles_per_cell
+    runner = dp.Runner()
+    piles = []
+    for k, sc in enumerate(scenes):
+        pile = dp.Pile(dp, runner, 0, cn=cn, cni=cni)
+        for b in sc.boundaries:
+            if b.kind == "box":
+                dist = dp.BoxDistance.create(dp.f3(*b.params[:3]), b.params[3])
+            elif b.kind == "box_shell":
+                dist = dp.BoxShellDistance.create(dp.f3(*b.params[:3]), b.params[3], b.params[4])
+            else:
+                dist = dp.SphereDistance.create(b.params[0])
+            pile.add(dist, _al.uint3(*b.resolution), sign=b.sign, mass=b.mass,
+                     inertia_tensor=dp.f3(*b.inertia), x=dp.f3(*b.x), q=dp.f4(*b.q))
+        piles.append(pile)
+        if log and (k % 32 == 0 or k == len(scenes) - 1):
+            log("batch: %d piles built" % (k + 1))
+    solver = dp.SolverDf(dp, piles, [sc.n for sc in scenes],
+                         [_al.uint3(*sc.grid_res) for sc in scenes],
+                         [_al.int3(*sc.grid_offset) for sc in scenes], cn=cn, cni=cni)
+    solver.dt = s0.dt
+    solver.max_dt = s0.dt if s0.max_dt is None else s0.max_dt
+    solver.min_dt = s0.dt if s0.min_dt is None else s0.min_dt
+    solver.cfl = s0.cfl
+    solver.density_change_tolerance = s0.density_change_tolerance
+    solver.density_error_tolerance = s0.density_error_tolerance
+    solver.min_density_solve = s0.min_density_solve
+    solver.max_density_solve = s0.max_density_solve
+    solver.min_divergence_solve = s0.min_divergence_solve
+    solver.max_divergence_solve = s0.max_divergence_solve
+    X = dp.coat(solver.particle_x)
+    V = dp.coat(solver.particle_v)
+    for k, sc in enumerate(scenes):
+        first, count = solver.scene_range(k)
+        assert count == sc.n
+        X.set(sc.x, offset=first * 12)
+        V.set(sc.v, offset=first * 12)
+    return dp, piles, solver, cn, cni

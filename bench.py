@@ -45,9 +45,7 @@ def kernel_bytes(name, K, nb):
     scalar 4 B, vector 12 B, u32 index 4 B; neighbour-state gathers counted
     once per particle as cache-resident; boundary records 16 B each only for
     (particle, boundary) pairs with non-zero volume, nb per particle).  The
-    neighbour list is counted at 4 B per entry (the index); the stored kernel
-    gradient scale that rides beside it is an implementation choice, not
-    algorithmic traffic."""
+    neighbour list is counted at 4 B per entry."""
     nbr = 4 + 4 * K  # count + neighbour list
     table = {
         "k_neighbors": 12 + 4 * K + 4 + 8 + 4 + 16 * nb,
@@ -237,7 +235,168 @@ class PeriodicStress:
         }
 
 
-WORKLOADS = {"dam_break_1m_rigid": DamBreak, "periodic_8m": PeriodicStress}
+class BatchedScenes:
+    name = "batched_256"
+    metric = "particle-steps/s @256 batched dam breaks (4k-120k particles each), DFSPH 0.1% ρ-err"
+    default_steps = 500
+    default_warmup = 10
+    default_cpu_steps = 30
+    cpu_scenes = 32
+
+    def __init__(self, args):
+        from alluvion_amd import scenes
+
+        self.scs = scenes.batched_dam_breaks(256 // (args.scale ** 3) or 1,
+                                             map_res=args.batch_map_res)
+        n = sum(sc.n for sc in self.scs)
+        log("scene: %d scenes, %d particles" % (len(self.scs), n))
+        t0 = time.perf_counter()
+        self.dp, self.piles, self.solver, _, _ = scenes.build_batch(self.scs, log=log)
+        self.setup_s = time.perf_counter() - t0
+        self.sc = type("Batch", (), {"n": n})()
+        self.iters = 0
+
+    def step(self):
+        self.solver.step()
+        self.iters += self.solver.num_divergence_solve + self.solver.num_density_solve
+
+    def ranges(self):
+        return [self.solver.scene_range(k) for k in range(len(self.scs))]
+
+    def snapshot(self):
+        X = self.dp.coat(self.solver.particle_x).get()
+        V = self.dp.coat(self.solver.particle_v).get()
+        out = []
+        for k in range(min(self.cpu_scenes, len(self.scs))):
+            first, count = self.solver.scene_range(k)
+            out.append((X[first:first + count].copy(), V[first:first + count].copy()))
+        return out
+
+    def oracle(self, state):
+        """The CPU oracle has no batch: it steps the first scenes one after
+        another (their own volume maps, same state)."""
+        import oracle_ffi as of
+
+        orcs = []
+        for k, (x, v) in enumerate(state):
+            orc = of.Oracle(self.scs[k], precision=4)
+            grids = self.piles[k].volume_grids
+            for b, var in enumerate(grids):
+                orc.set_volume_map(b, self.dp.coat(var).get())
+            orc.set("x", x)
+            orc.set("v", v)
+            orcs.append(orc)
+        self.cpu_particles = sum(len(x) for x, _ in state)
+
+        def step():
+            for orc in orcs:
+                orc.step()
+
+        return orcs, step
+
+    def sample_text(self, steps, cores):
+        return ("%d DFSPH steps of the first %d of the 256 scenes (%d particles) on the host "
+                "CPU, one after another (C++/OpenMP FP32 oracle, %d threads), after 1 warm-up "
+                "step" % (steps, self.cpu_scenes, self.cpu_particles, cores))
+
+    def config(self, steps, K, nb):
+        return {
+            "workload": "256 independent DFSPH dam breaks in one batch solver: particle counts "
+                        "log-uniform in 4k-120k, cube edges 0.2-0.6 m, own %d^3 volume map each, "
+                        "dt=2e-4, tolerance 0.1%%" % self.scs[0].boundaries[0].resolution[0],
+            "scenes": len(self.scs),
+            "particles": self.sc.n,
+            "mean_neighbors": round(K, 2),
+            "iterations_per_step_max_over_scenes": round(self.iters / steps, 3),
+            "setup_s": round(self.setup_s, 2),
+        }
+
+
+WORKLOADS = {"dam_break_1m_rigid": DamBreak, "periodic_8m": PeriodicStress,
+             "batched_256": BatchedScenes}
+
+
+def timed_run(make_scene, steps, net=None):
+    """Fresh solver for a scene, `steps` steps (plain, or with the warm-start
+    network); iterations per step, wall ms per step, simulated time."""
+    import alluvion_amd._alluvion as al
+    from alluvion_amd import scenes
+
+    sc = make_scene()
+    dp, pile, solver, _, _ = scenes.build_solver(sc)
+    if net is not None:
+        solver.set_warm_start_net(net)
+    it = 0
+    al.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.step()
+        it += solver.num_divergence_solve + solver.num_density_solve
+    al.device_synchronize()
+    el = time.perf_counter() - t0
+    out = {"iterations_per_step": round(it / steps, 3), "ms_per_step": round(1e3 * el / steps, 4),
+           "particle_steps_per_s": round(sc.n * steps / el, 1), "simulated_s": round(solver.t, 5)}
+    solver.set_warm_start_net(None)
+    del solver, pile, dp
+    return out
+
+
+def learned_comparison(scale, body_resolution):
+    """Plain vs learned warm start.  The network is trained on the GPU on
+    pairs logged from plain runs with the CFL-limited step (cfl 0.4,
+    dt <= 2e-3): 300 steps of the 16,384-particle dam break and 50 steps of
+    the 1M-particle rigid-body dam break.  Then both warm starts run the same
+    steps from the initial state of each scene (the 1M run goes 50 steps
+    past the logged ones); tolerance 0.1 % throughout."""
+    import alluvion_amd._alluvion as al
+    from alluvion_amd import Depot, scenes, warm_start
+
+    small = lambda: scenes.adaptive_step(scenes.dam_break_small())  # noqa: E731
+    big = lambda: scenes.adaptive_step(scenes.dam_break_rigid(  # noqa: E731
+        nx=128 // scale, ny=128 // scale, nz=64 // scale, body_resolution=body_resolution))
+    t0 = time.perf_counter()
+    sets = []
+    for make, steps in ((small, 300), (big, 50)):
+        dp, pile, solver, _, _ = scenes.build_solver(make())
+        sets.append((dp,) + warm_start.collect_pairs(dp, solver, steps))
+        del solver, pile
+    # one data set: the logged pairs back to back
+    count = sum(c for _, _, _, c in sets)
+    D = Depot(np.float32)
+    X = D.create((count * 8,), 1)
+    Y = D.create((count * 2,), 1)
+    off = 0
+    for dp, x, y, c in sets:
+        X.set_from(x, c * 8, off * 8)
+        Y.set_from(y, c * 2, off * 2)
+        off += c
+        dp.remove(x)
+        dp.remove(y)
+    al.device_synchronize()
+    t1 = time.perf_counter()
+    net, losses = warm_start.train(X, Y, count)
+    t2 = time.perf_counter()
+    D.remove(X)
+    D.remove(Y)
+    log("warm start: %d pairs logged in %.2f s, trained in %.2f s (loss %.3f -> %.3f)" % (
+        count, t1 - t0, t2 - t1, float(np.mean(losses[:10])), float(np.mean(losses[-100:]))))
+    runs = []
+    for name, make, steps in (("dam break 16,384 particles", small, 300),
+                              ("dam break 1,048,576 particles + 4 rigid bodies", big, 100)):
+        plain = timed_run(make, steps)
+        learned = timed_run(make, steps, net)
+        runs.append({"scene": name, "steps": steps, "plain": plain, "learned": learned,
+                     "iteration_ratio": round(plain["iterations_per_step"] /
+                                              learned["iterations_per_step"], 3),
+                     "speedup": round(learned["particle_steps_per_s"] /
+                                      plain["particle_steps_per_s"], 3)})
+        log("warm start %s: plain %s learned %s" % (name, plain, learned))
+    return {"time_step": "CFL-limited, cfl 0.4, dt <= 2e-3", "tolerance": "0.1% (divergence, density)",
+            "train": {"pairs": int(count),
+                      "source": "300 plain steps of the 16,384-particle dam break and 50 of the "
+                                "1M-particle one",
+                      "adam_steps": len(losses), "batch": 65536, "seconds": round(t2 - t1, 3)},
+            "runs": runs}
 
 
 def cpu_baseline(wl, state, steps):
@@ -251,8 +410,9 @@ def cpu_baseline(wl, state, steps):
         log("cpu step %d" % i)
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    particles = getattr(wl, "cpu_particles", wl.sc.n)  # the sample's particle count
     return {
-        "value": wl.sc.n * steps / dt,
+        "value": particles * steps / dt,
         "unit": "particle-steps/s",
         "cores": cores,
         "kind": "port",
@@ -272,6 +432,11 @@ def main():
     ap.add_argument("--scale", type=int, default=1,
                     help="divide the lattice dimensions (testing only)")
     ap.add_argument("--body-resolution", type=int, default=128)
+    ap.add_argument("--batch-map-res", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep direct launches instead of tuning the step execution")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the plain-vs-learned warm start comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run 3 device copies of 256 MiB before the steps (PMC scaling)")
     args = ap.parse_args()
@@ -289,15 +454,21 @@ def main():
     dp, solver, n = wl.dp, wl.solver, wl.sc.n
     if args.copy_calibration:
         copy_calibration(dp)
+    tuned = None
+    if not args.no_autotune and hasattr(wl.solver, "set_graph") and args.workload != "periodic_8m":
+        from alluvion_amd import autotune
+        best, timing = autotune.tune_step(wl.solver, log=log)
+        tuned = {"graph": best[0], "slack": best[1],
+                 "ms_per_step": {"%s/%d" % ("graph" if g else "direct", s): round(t, 4)
+                                 for (g, s), t in timing.items()}}
     for i in range(warmup):
         wl.step()
         log("warmup step %d" % i)
     state = None if args.no_cpu_baseline else wl.snapshot()
     if hasattr(wl, "n_div"):
         wl.n_div = wl.n_dens = 0
-    if not args.no_profile:
-        al.profiling_enable(True)
-        al.profiling_reset()
+    if hasattr(wl, "iters"):
+        wl.iters = 0
     al.device_synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
@@ -306,15 +477,28 @@ def main():
             log("timed step %d" % i)
     al.device_synchronize()
     elapsed = time.perf_counter() - t0
+    cfg = wl.config(steps, 0.0, 0.0)  # iteration statistics of the timed steps
+    # Per-kernel times come from a separate window after the timed one: the
+    # HIP events around every launch need direct launches (no graphs).
     stats = {}
     if not args.no_profile:
+        al.profiling_enable(True)
+        al.profiling_reset()
+        for _ in range(min(steps, 50)):
+            wl.step()
+        al.device_synchronize()
         stats = al.profiling_stats()
         al.profiling_enable(False)
     value = n * steps / elapsed
-    nnb = dp.coat(solver.particle_num_neighbors).get()[:n]
+    ranges = wl.ranges() if hasattr(wl, "ranges") else [(0, n)]
+    nnb = dp.coat(solver.particle_num_neighbors).get()
+    nnb = np.concatenate([nnb[a:a + c] for a, c in ranges])
     K = float(nnb.mean())
-    mask = dp.coat(solver.boundary_mask).get()[:n].astype(np.uint64)
+    mask = dp.coat(solver.boundary_mask).get().astype(np.uint64)
+    mask = np.concatenate([mask[a:a + c] for a, c in ranges])
     nb = float(sum(int(((mask >> np.uint64(b)) & np.uint64(1)).sum()) for b in range(32)) / n)
+    cfg.update({k: v for k, v in wl.config(steps, K, nb).items()
+                if k in ("mean_neighbors", "boundary_pairs_per_particle")})
 
     roofline = None
     kernels = {}
@@ -346,6 +530,9 @@ def main():
                 "share_of_gpu_time": round(ms / total_ms, 3),
             }
     log("timed region: %.3f s" % elapsed)
+    learned = None
+    if args.workload == "dam_break_1m_rigid" and not args.no_learned:
+        learned = learned_comparison(args.scale, args.body_resolution)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(wl, state, cpu_steps)
@@ -364,9 +551,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": wl.config(steps, K, nb),
+        "config": cfg,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned_warm_start": learned,
+        "step_execution": tuned,
         "kernels": kernels,
     }
     print(json.dumps(out))

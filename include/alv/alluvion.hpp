@@ -527,10 +527,66 @@ class WarmStartNet {
   alv_mlp* net_ = nullptr;
 };
 
+// One scene of a batch: its pile of static boundaries, particle count and
+// neighbour grid (no counterpart in the reference).
+template <typename TF>
+struct SceneSpec {
+  Pile<TF>* pile;
+  U num_particles;
+  U3 grid_res;
+  I3 grid_offset;
+};
+
 // Solver base (solver.hpp:8-306) over alv_solver.
 template <typename TF>
 class SolverBase {
  public:
+  // a batch of independent scenes in one solver
+  SolverBase(int kind, Store& store, std::vector<SceneSpec<TF>> const& scenes,
+             Const<TF> const* cn, ConstiN const* cni)
+      : max_num_particles(batch_capacity(scenes)), store_(store), pile_(*scenes.at(0).pile) {
+    if (cn)
+      store.copy_cn_external(*cn, *cni);
+    else
+      store.copy_cn();
+    alv_const c = store.device_cn().to_c();
+    alv_consti ci = store.device_cni().to_c();
+    std::vector<alv_scene_desc> d(scenes.size());
+    for (size_t k = 0; k < scenes.size(); ++k) {
+      d[k].pile = scenes[k].pile->handle();
+      d[k].num_particles = scenes[k].num_particles;
+      d[k].grid_res[0] = scenes[k].grid_res.x;
+      d[k].grid_res[1] = scenes[k].grid_res.y;
+      d[k].grid_res[2] = scenes[k].grid_res.z;
+      d[k].grid_offset[0] = scenes[k].grid_offset.x;
+      d[k].grid_offset[1] = scenes[k].grid_offset.y;
+      d[k].grid_offset[2] = scenes[k].grid_offset.z;
+    }
+    check(alv_solver_create_batch(&solver_, kind, U(d.size()), d.data(), &c, &ci));
+    check(alv_solver_get_settings(solver_, &set_));
+    particle_radius = store.device_cn().particle_radius;
+  }
+  static U batch_capacity(std::vector<SceneSpec<TF>> const& scenes) {
+    U n = 0;
+    for (auto const& s : scenes) n += (s.num_particles + 255u) / 256u * 256u;
+    return n;
+  }
+  U num_scenes() const {
+    U n = 0;
+    check(alv_solver_num_scenes(solver_, &n));
+    return n;
+  }
+  std::pair<U, U> scene_range(U k) const {
+    U first = 0, count = 0;
+    check(alv_solver_scene_range(solver_, k, &first, &count));
+    return {first, count};
+  }
+  alv_solver_report scene_report(U k) const {
+    alv_solver_report r;
+    check(alv_solver_scene_report(solver_, k, &r));
+    return r;
+  }
+
   SolverBase(int kind, Runner<TF>& runner, Pile<TF>& pile, Store& store, U max_num_particles,
              U num_ushers, bool enable_surface_tension, bool enable_vorticity,
              Const<TF> const* cn, ConstiN const* cni, bool graphical)
@@ -586,6 +642,8 @@ class SolverBase {
     check(alv_solver_update_particle_neighbors(solver_));
   }
   void compute_all_boundaries() {}
+  // step execution as replayed HIP graphs (no counterpart in the reference)
+  void set_graph(bool enable, U slack = 0) { check(alv_solver_set_graph(solver_, enable, slack)); }
   // learned warm start (no counterpart in the reference); null = plain
   void set_warm_start_net(WarmStartNet const* net, bool divergence = true, bool density = true) {
     check(alv_solver_set_warm_start_net(solver_, net ? net->handle() : nullptr, divergence,
@@ -681,6 +739,10 @@ class SolverDf : public SolverBase<TF> {
            Const<TF> const* cn = nullptr, ConstiN const* cni = nullptr, bool graphical = false)
       : SolverBase<TF>(ALV_SOLVER_DF, runner, pile, store, max_num_particles, num_ushers,
                        enable_surface_tension, enable_vorticity, cn, cni, graphical) {}
+  // batch of independent scenes
+  SolverDf(Store& store, std::vector<SceneSpec<TF>> const& scenes, Const<TF> const* cn = nullptr,
+           ConstiN const* cni = nullptr)
+      : SolverBase<TF>(ALV_SOLVER_DF, store, scenes, cn, cni) {}
 #define ALV_RWS(type, name)                            \
   type get_##name() const { return type(this->set_.name); } \
   void set_##name(type value) {                        \

@@ -189,6 +189,32 @@ int alv_solver_create(alv_solver** out, int kind, alv_pile* pile,
                       unsigned max_num_particles, const alv_const* cn,
                       const alv_consti* cni);
 int alv_solver_destroy(alv_solver* solver);
+
+/* Batch of independent DFSPH or IISPH scenes in one solver.  No counterpart
+ * in the reference, which holds one scene per process (its constants are
+ * process-global, constants.hpp:141-143).  Every scene has its own
+ * particles, neighbour grid and pile of static boundaries; the constants, the
+ * neighbour caps of `cni` and the settings are shared.  One launch of each
+ * kernel advances every scene, and every scene keeps its own time step,
+ * convergence test and iteration count: a converged scene idles while the
+ * others iterate.  Particle arrays hold the scenes one after another; scene k
+ * occupies [first, first + count) (alv_solver_scene_range) of every field. */
+typedef struct {
+  alv_pile* pile;
+  unsigned num_particles;
+  unsigned grid_res[3];
+  int grid_offset[3];
+} alv_scene_desc;
+int alv_solver_create_batch(alv_solver** out, int kind, unsigned num_scenes,
+                            const alv_scene_desc* scenes, const alv_const* cn,
+                            const alv_consti* cni);
+int alv_solver_num_scenes(const alv_solver* solver, unsigned* num_scenes);
+int alv_solver_scene_range(const alv_solver* solver, unsigned scene, unsigned* first,
+                           unsigned* count);
+/* Report of one scene after the last step (t and dt are the scene's own). */
+int alv_solver_scene_report(const alv_solver* solver, unsigned scene,
+                            alv_solver_report* report);
+
 /* Store::copy_cn_external (store.hpp:142-164): constants used from the next
  * step on. */
 int alv_solver_set_const(alv_solver* solver, const alv_const* cn,
@@ -228,6 +254,14 @@ int alv_solver_get_field(alv_solver* solver, int field, void** dptr,
  * (solver_ii.hpp:76-243).  Periodicity follows alv_const.wrap_length; the
  * `wrap` argument of the reference (y only) is implied by it. */
 int alv_solver_step(alv_solver* solver);
+/* Step execution (no counterpart in the reference): with `enable` the DFSPH
+ * step runs as three HIP graphs (neighbour search and divergence warm start,
+ * viscosity to density warm start, final integration), captured on first use
+ * and replayed; each graph holds the last step's Jacobi iteration count plus
+ * `slack` iterations, iterations past convergence return at once, and the
+ * host adds more only when a scene has not converged.  Results are identical
+ * with and without graphs.  Per-kernel profiling uses direct launches. */
+int alv_solver_set_graph(alv_solver* solver, int enable, unsigned slack);
 /* Solver::compute_all_boundaries (solver.hpp:170-185) followed by
  * Solver::update_particle_neighbors<wrap> (solver.hpp:225-260). */
 int alv_solver_update_particle_neighbors(alv_solver* solver);
