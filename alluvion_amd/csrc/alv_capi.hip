@@ -10,6 +10,7 @@
 // iteration, and once at the end of the step to read the report.
 
 #include <float.h>
+#include <stdio.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -24,6 +25,7 @@
 
 #include "../../include/alv/alv.h"
 #include "alv_device.hpp"
+#include "alv_contact.hpp"
 #include "alv_kernels.hpp"
 
 using namespace alv;
@@ -273,9 +275,31 @@ struct alv_pile {
   float gravity[3] = {0.f, 0.f, 0.f};
   unsigned max_num_contacts = 0;
   float* peirce = nullptr;  // device, 120 floats
+  // rigid contacts: collision vertices per body (host, body frame), their
+  // concatenation on the device, and the contact buffers
+  std::vector<std::vector<float>> coll_verts;
+  bool verts_dirty = true;
+  float* verts_dev = nullptr;
+  size_t verts_cap = 0;  // floats
+  ContactJob* jobs_dev = nullptr;
+  size_t jobs_cap = 0;
+  BoundaryDev* bodies_dev = nullptr;
+  size_t bodies_cap = 0;
+  float* inertia_dev = nullptr;
+  size_t inertia_cap = 0;
+  alv_contact* contacts_dev = nullptr;
+  unsigned* count_dev = nullptr;
+  std::vector<alv_contact> contacts;  // host copy of the kept contacts
+  unsigned num_found = 0;
   ~alv_pile() {
     for (float* v : volumes) (void)hipFree(v);
     if (peirce) (void)hipFree(peirce);
+    if (verts_dev) (void)hipFree(verts_dev);
+    if (jobs_dev) (void)hipFree(jobs_dev);
+    if (bodies_dev) (void)hipFree(bodies_dev);
+    if (inertia_dev) (void)hipFree(inertia_dev);
+    if (contacts_dev) (void)hipFree(contacts_dev);
+    if (count_dev) (void)hipFree(count_dev);
   }
 };
 
@@ -739,6 +763,186 @@ int alv_pile_integrate_kinematics(alv_pile* pile, float dt) {
   return ALV_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Grows a device buffer to hold at least `count` elements of T.
+template <class T>
+hipError_t reserve_dev(T** ptr, size_t* cap, size_t count) {
+  if (*ptr && *cap >= count) return hipSuccess;
+  if (*ptr) (void)hipFree(*ptr);
+  *ptr = nullptr;
+  *cap = 0;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), std::max<size_t>(count, 1) * sizeof(T));
+  if (e == hipSuccess) *cap = count;
+  return e;
+}
+
+// Pile::find_contacts (pile.hpp:826-904) on `stream`; see alv.h for the
+// meaning of i and j.  Leaves the kept contacts in pile->contacts.
+int find_contacts(alv_pile* P, int i, int j, float tolerance, hipStream_t stream) {
+  const unsigned nb = unsigned(P->bnd.size());
+  if (i >= int(nb) || j >= int(nb)) return fail(ALV_ERR_ARG, "bad boundary id");
+  P->coll_verts.resize(nb);
+  std::vector<unsigned> vbase(nb + 1, 0);
+  for (unsigned b = 0; b < nb; ++b) vbase[b + 1] = vbase[b] + unsigned(P->coll_verts[b].size() / 3);
+  std::vector<ContactJob> jobs;
+  unsigned candidates = 0;
+  const unsigned i0 = i < 0 ? 0 : unsigned(i), i1 = i < 0 ? nb : unsigned(i) + 1;
+  for (unsigned a = i0; a < i1; ++a) {
+    const unsigned j0 = j < 0 ? 0 : unsigned(j), j1 = j < 0 ? nb : unsigned(j) + 1;
+    for (unsigned b = j0; b < j1; ++b) {
+      if (j < 0 && (a == b || (P->bnd[a].mass == 0.f && P->bnd[b].mass == 0.f))) continue;
+      if (P->bnd[b].type == ALV_DISTANCE_BOX_SHELL) continue;
+      const unsigned nv = vbase[a + 1] - vbase[a];
+      if (nv == 0) continue;
+      ContactJob J{};
+      J.i = a;
+      J.j = b;
+      J.vbase = vbase[a];
+      J.nv = nv;
+      J.cbase = candidates;
+      J.restitution = P->restitution[a] * P->restitution[b];
+      J.friction = P->friction[a] + P->friction[b];
+      jobs.push_back(J);
+      candidates += nv;
+    }
+  }
+  P->num_found = 0;
+  P->contacts.clear();
+  if (jobs.empty()) return ALV_OK;
+  if (P->verts_dirty) {
+    std::vector<float> all;
+    all.reserve(size_t(vbase[nb]) * 3);
+    for (const auto& v : P->coll_verts) all.insert(all.end(), v.begin(), v.end());
+    ALV_HIP(reserve_dev(&P->verts_dev, &P->verts_cap, all.size()));
+    ALV_HIP(hipMemcpyAsync(P->verts_dev, all.data(), all.size() * sizeof(float),
+                           hipMemcpyHostToDevice, stream));
+    ALV_HIP(hipStreamSynchronize(stream));
+    P->verts_dirty = false;
+  }
+  ALV_HIP(reserve_dev(&P->bodies_dev, &P->bodies_cap, nb));
+  ALV_HIP(reserve_dev(&P->inertia_dev, &P->inertia_cap, size_t(nb) * 3));
+  ALV_HIP(reserve_dev(&P->jobs_dev, &P->jobs_cap, jobs.size()));
+  if (!P->count_dev) ALV_HIP(hipMalloc(&P->count_dev, sizeof(unsigned)));
+  if (!P->contacts_dev && P->max_num_contacts)
+    ALV_HIP(hipMalloc(&P->contacts_dev, size_t(P->max_num_contacts) * sizeof(alv_contact)));
+  std::vector<float> inertia(size_t(nb) * 3);
+  for (unsigned b = 0; b < nb; ++b)
+    for (int k = 0; k < 3; ++k) inertia[b * 3 + k] = P->inertia[b][k];
+  ALV_HIP(hipMemcpyAsync(P->bodies_dev, P->bnd.data(), nb * sizeof(BoundaryDev),
+                         hipMemcpyHostToDevice, stream));
+  ALV_HIP(hipMemcpyAsync(P->inertia_dev, inertia.data(), inertia.size() * sizeof(float),
+                         hipMemcpyHostToDevice, stream));
+  ALV_HIP(hipMemcpyAsync(P->jobs_dev, jobs.data(), jobs.size() * sizeof(ContactJob),
+                         hipMemcpyHostToDevice, stream));
+  launch_collision(P->jobs_dev, unsigned(jobs.size()), candidates, P->bodies_dev, P->inertia_dev,
+                   P->verts_dev, tolerance, P->max_num_contacts, P->contacts_dev, P->count_dev,
+                   stream);
+  ALV_HIP(hipGetLastError());
+  unsigned found = 0;
+  ALV_HIP(hipMemcpyAsync(&found, P->count_dev, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  ALV_HIP(hipStreamSynchronize(stream));
+  P->num_found = found;
+  const unsigned kept = std::min(found, P->max_num_contacts);
+  P->contacts.resize(kept);
+  if (kept) {
+    ALV_HIP(hipMemcpyAsync(P->contacts.data(), P->contacts_dev, kept * sizeof(alv_contact),
+                           hipMemcpyDeviceToHost, stream));
+    ALV_HIP(hipStreamSynchronize(stream));
+  }
+  return ALV_OK;
+}
+
+// Pile::solve_contacts (pile.hpp:906-979): sequential impulse sweeps with a
+// penetration correction and dynamic friction, on the host kinematics.
+void solve_contacts(alv_pile* P) {
+  if (P->num_found > P->max_num_contacts)
+    fprintf(stderr, "No. of contacts exceeds %u.\n", P->max_num_contacts);
+  auto v3 = [](const float* a) { return mk(a[0], a[1], a[2]); };
+  auto add3 = [](float* a, vec3 d) {
+    a[0] += d.x;
+    a[1] += d.y;
+    a[2] += d.z;
+  };
+  auto sym = [](vec3 v, vec3 d, vec3 o) {
+    return mk(v.x * d.x + v.y * o.x + v.z * o.y, v.x * o.x + v.y * d.y + v.z * o.z,
+              v.x * o.y + v.y * o.z + v.z * d.z);
+  };
+  for (int sweep = 0; sweep < 5; ++sweep) {
+    for (alv_contact& C : P->contacts) {
+      BoundaryDev& Bi = P->bnd[C.i];
+      BoundaryDev& Bj = P->bnd[C.j];
+      const vec3 n = v3(C.n), t = v3(C.t), cp_i = v3(C.cp_i), cp_j = v3(C.cp_j);
+      const float depth = dot(n, cp_i - cp_j);
+      const vec3 r_i = cp_i - v3(Bi.x), r_j = cp_j - v3(Bj.x);
+      const vec3 u_rel = (v3(Bi.v) + cross(v3(Bi.omega), r_i)) - (v3(Bj.v) + cross(v3(Bj.omega), r_j));
+      const float u_rel_n = dot(n, u_rel);
+      float corr = C.nkninv * (C.goalu - u_rel_n);
+      if (corr < -C.impulse_sum) corr = -C.impulse_sum;
+      if (depth < 0) corr -= C.nkninv * depth;  // unit stiffness
+      vec3 p = corr * n;
+      C.impulse_sum += corr;
+      const float pn = dot(p, n);
+      if (C.friction * pn > C.pmax)
+        p -= C.pmax * t;
+      else if (C.friction * pn < -C.pmax)
+        p += C.pmax * t;
+      else
+        p -= (C.friction * pn) * t;
+      if (Bi.mass != 0.f) {
+        add3(Bi.v, p * (1.0f / Bi.mass));
+        add3(Bi.omega, sym(cross(r_i, p), v3(C.iiwi_diag), v3(C.iiwi_off_diag)));
+      }
+      if (Bj.mass != 0.f) {
+        add3(Bj.v, -p * (1.0f / Bj.mass));
+        add3(Bj.omega, sym(cross(r_j, -p), v3(C.iiwj_diag), v3(C.iiwj_off_diag)));
+      }
+    }
+  }
+}
+
+bool has_collision_vertices(const alv_pile* P) {
+  for (const auto& v : P->coll_verts)
+    if (!v.empty()) return true;
+  return false;
+}
+}  // namespace
+
+extern "C" {
+
+int alv_pile_set_collision_vertices(alv_pile* pile, unsigned b, const float* xyz, unsigned n) {
+  if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
+  if (n && !xyz) return fail(ALV_ERR_ARG, "null vertices");
+  pile->coll_verts.resize(pile->bnd.size());
+  pile->coll_verts[b].assign(xyz, xyz + size_t(n) * 3);
+  pile->verts_dirty = true;
+  return ALV_OK;
+}
+
+int alv_pile_find_contacts(alv_pile* pile, int i, int j, float contact_tolerance,
+                           unsigned* num_contacts) {
+  if (!pile) return fail(ALV_ERR_ARG, "null pile");
+  if (i < 0 && j >= 0) return fail(ALV_ERR_ARG, "j given without i");
+  int rc = find_contacts(pile, i, j, contact_tolerance, nullptr);
+  if (rc == ALV_OK && num_contacts) *num_contacts = pile->num_found;
+  return rc;
+}
+
+int alv_pile_solve_contacts(alv_pile* pile) {
+  if (!pile) return fail(ALV_ERR_ARG, "null pile");
+  solve_contacts(pile);
+  return ALV_OK;
+}
+
+int alv_pile_get_contacts(const alv_pile* pile, alv_contact* out, unsigned capacity, unsigned* n) {
+  if (!pile || (capacity && !out)) return fail(ALV_ERR_ARG, "null argument");
+  const unsigned k = std::min(capacity, unsigned(pile->contacts.size()));
+  if (k) memcpy(out, pile->contacts.data(), k * sizeof(alv_contact));
+  if (n) *n = k;
+  return ALV_OK;
+}
+
 int alv_pile_get_volume_grid(const alv_pile* pile, unsigned b, float** nodes,
                              unsigned* num_nodes) {
   if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
@@ -809,6 +1013,8 @@ static int solver_alloc(alv_solver* S) {
   ALV_ALLOC(A.rho_adv, cap);
   ALV_ALLOC(A.kfac, cap);
   ALV_ALLOC(A.feat, size_t(cap) * 2);
+  ALV_ALLOC(A.tiles, size_t(grid_for(cap)) + 1);
+  A.use_tiles = 1;
   ALV_ALLOC(A.kpred, cap);
   A.kmax = K;
   ALV_ALLOC(A.nbr, nbr_capacity(K, cap));
@@ -824,7 +1030,7 @@ static int solver_alloc(alv_solver* S) {
   ALV_ALLOC(A.bker, BN);
   ALV_ALLOC(A.bforce, BN);
   ALV_ALLOC(A.btorque, BN);
-  if (S->kind == ALV_SOLVER_II) {
+  if (S->kind != ALV_SOLVER_DF) {  // IISPH and ISPH pressure solves
     ALV_ALLOC(A.pressure, cap);
     ALV_ALLOC(A.last_pressure, cap);
     ALV_ALLOC(A.aii, cap);
@@ -877,7 +1083,7 @@ extern "C" {
 int alv_solver_create(alv_solver** out, int kind, alv_pile* pile, unsigned max_num_particles,
                       const alv_const* cn, const alv_consti* cni) {
   if (!out || !pile || !cn || !cni) return fail(ALV_ERR_ARG, "null argument");
-  if (kind != ALV_SOLVER_DF && kind != ALV_SOLVER_II)
+  if (kind != ALV_SOLVER_DF && kind != ALV_SOLVER_II && kind != ALV_SOLVER_I)
     return fail(ALV_ERR_ARG, "unknown solver kind");
   if (max_num_particles == 0) return fail(ALV_ERR_ARG, "max_num_particles is zero");
   if (cni->max_num_neighbors_per_particle == 0)
@@ -942,7 +1148,7 @@ int alv_solver_create_batch(alv_solver** out, int kind, unsigned num_scenes,
                             const alv_scene_desc* descs, const alv_const* cn,
                             const alv_consti* cni) {
   if (!out || !descs || !cn || !cni) return fail(ALV_ERR_ARG, "null argument");
-  if (kind != ALV_SOLVER_DF && kind != ALV_SOLVER_II)
+  if (kind != ALV_SOLVER_DF && kind != ALV_SOLVER_II && kind != ALV_SOLVER_I)
     return fail(ALV_ERR_ARG, "unknown solver kind");
   if (num_scenes == 0 || num_scenes > 65535) return fail(ALV_ERR_ARG, "num_scenes out of range");
   if (cni->max_num_neighbors_per_particle == 0)
@@ -1195,6 +1401,7 @@ int neighbor_phase(alv_solver* S, const Launch& L, int with_factor) {
   launch_scan(L.cells, A.cell_count, A.cell_start, A.scan_tmp, S->stream);
   launch_scatter(L, A, S->stream);
   launch_cell_sort(L, A, S->stream);
+  launch_tiles(L, A, S->stream);
   launch_gather_boundary(L, A, S->kind, S->stream);
   launch_neighbors(L, A, with_factor, S->stream);
   ALV_HIP(hipGetLastError());
@@ -1286,6 +1493,12 @@ int finish_step(alv_solver* S, const Launch& L) {
     }
   }
   alv_pile_integrate_kinematics(pile, S->set.dt);
+  // rigid contacts (solver_df.hpp:356-357, solver_ii.hpp:240-241)
+  if (has_collision_vertices(pile)) {
+    rc = find_contacts(pile, -1, -1, S->cn.contact_tolerance, S->stream);
+    if (rc != ALV_OK) return rc;
+    solve_contacts(pile);
+  }
   S->set.t += S->set.dt;
   return ALV_OK;
 }
@@ -1375,6 +1588,8 @@ int alv_solver_step(alv_solver* S) {
   if (rc != ALV_OK) return rc;
   DevArrays& A = S->A;
   hipStream_t st = S->stream;
+  if (S->kind != ALV_SOLVER_DF && (!A.pressure || !A.last_pressure || !A.aii || !A.pacc))
+    return fail(ALV_ERR_STATE, "pressure-solve arrays are not allocated");
   if (S->kind == ALV_SOLVER_DF) {
     // SolverDf::step (solver_df.hpp:74-359) in three segments; the host
     // checks convergence between them
@@ -1430,6 +1645,21 @@ int alv_solver_step(alv_solver* S) {
       if (L.active) launch_df_finish(L, A, dens, st);
     });
     if (rc != ALV_OK) return rc;
+  } else if (S->kind == ALV_SOLVER_I) {
+    // SolverI::step (solver_i.hpp:79-330), volume-map boundaries
+    rc = neighbor_phase(S, L, 0);
+    if (rc != ALV_OK) return rc;
+    launch_ctl_reset(L, A, st);
+    if (L.active) launch_ii_viscosity_predict(L, A, st);
+    launch_update_dt(L, A, st);
+    if (L.active) {
+      launch_is_prepare(L, A, st);
+      rc = run_loop(S, false, S->set.max_density_solve, &S->last_dens,
+                    [&] { launch_is_iteration(L, A, st); });
+      if (rc != ALV_OK) return rc;
+      launch_is_settle(L, A, st);
+      launch_ii_finish(L, A, st);
+    }
   } else {
     // SolverIi::step (solver_ii.hpp:76-243)
     rc = neighbor_phase(S, L, 0);
@@ -1455,6 +1685,15 @@ int alv_solver_set_graph(alv_solver* S, int enable, unsigned slack) {
   if (!S) return fail(ALV_ERR_ARG, "null solver");
   S->use_graph = enable != 0;
   S->slack = slack;
+  return ALV_OK;
+}
+
+int alv_solver_set_tiles(alv_solver* S, int enable) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  S->A.use_tiles = enable != 0;
+  for (auto& g : S->seg)  // captured graphs hold the old flag
+    g.key = ~0ull;
   return ALV_OK;
 }
 

@@ -9,6 +9,19 @@
 
 namespace alv {
 
+// LDS tile of one 256-particle block: the neighbours of a block whose
+// particles all lie in one x-slab of its scene's grid are inside three
+// contiguous ranges of sorted particles (the slab and the two beside it),
+// which a pass copies into LDS with coalesced loads before the neighbour
+// loops.  Ranges are in increasing order; absent ranges have lo = ~0u.
+constexpr unsigned kTileMax = 2496;  // particles per tile (78 KB of x and v: two blocks per CU)
+struct Tile {
+  unsigned lo[3];
+  unsigned len[3];
+  unsigned total;
+  unsigned ok;  // the block may use its tile
+};
+
 // Device pointer bundle of one solver (passed to kernels by value).
 // "u*" arrays are in user (API) order and hold AoS float3 vectors like the
 // reference's Variable<1,float3>; all other per-particle arrays are in
@@ -45,6 +58,8 @@ struct DevArrays {
   float* kappa_v;
   float* rho_adv;
   float* kfac;  // stiffness k_j of the running Jacobi iteration
+  Tile* tiles;    // [cap / 256] LDS tiles of the neighbour passes
+  int use_tiles;  // passes stage neighbour tiles in LDS
   float4* feat;  // [2 cap] warm-start features (learned warm start, logging)
   float* kpred;  // predicted stored density stiffness (learned warm start)
   uint4* nbr;     // neighbour list: groups of 4 slots in wave tiles (nbr_group)
@@ -145,6 +160,8 @@ void launch_scan(unsigned count, const unsigned* in, unsigned* out, unsigned* tm
                  hipStream_t s);
 void launch_scatter(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_cell_sort(const Launch& L, const DevArrays& A, hipStream_t s);
+// LDS tile ranges of every block (after the sort, before the passes)
+void launch_tiles(const Launch& L, const DevArrays& A, hipStream_t s);
 // gather user -> sorted and compute boundary records
 void launch_gather_boundary(const Launch& L, const DevArrays& A, int solver_kind,
                             hipStream_t s);
@@ -175,6 +192,10 @@ void launch_integrate_velocity(const Launch& L, const DevArrays& A, float dt, hi
 void launch_ii_viscosity_predict(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_ii_predict(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_ii_iteration(const Launch& L, const DevArrays& A, hipStream_t s);
+// SolverI (ISPH)
+void launch_is_prepare(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_is_iteration(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_is_settle(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_ii_finish(const Launch& L, const DevArrays& A, hipStream_t s);
 
 // rigid bodies: per-body FP64 force / torque sums into A.body_sums

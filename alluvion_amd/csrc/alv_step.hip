@@ -233,6 +233,73 @@ __global__ __launch_bounds__(256) void k_cell_sort(SceneTab T, DevArrays A) {
   }
 }
 
+// LDS tile of every particle block.  For a block whose in-grid particles all
+// lie in x-slab x0, the neighbours in slab xs (x0-1 .. x0+1) lie in the cell
+// range from (xs, y0-1, z0-1) to (xs, y1+1, z1+1) in the linear cell order,
+// (y0, z0) and (y1, z1) being the first and last particle's cells, clipped
+// to the grid (with the whole first row when the block starts in row 0 and
+// spans several rows, since row 1's neighbours reach (0, 0)).  Blocks that
+// span several slabs, periodic grids and tiles over kTileMax fall back to
+// global gathers.
+__global__ __launch_bounds__(256) void k_tiles(SceneTab T, DevArrays A, unsigned nblocks) {
+  const unsigned b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= nblocks) return;
+  Tile tl;
+  for (int w = 0; w < 3; ++w) {
+    tl.lo[w] = ~0u;
+    tl.len[w] = 0u;
+  }
+  tl.total = 0u;
+  tl.ok = 0u;
+  const SimParams p = T.par[T.blk_scene[b]];
+  const unsigned s0 = b * kBlock;
+  const unsigned tail = A.cell_start[p.cell_base + p.num_cells];  // first particle outside the grid
+  const unsigned end = min(p.first + p.n, tail);
+  // no tile for periodic grids, for blocks outside the scene, and for a
+  // block that holds particles outside the grid (their in-grid neighbour
+  // cells need not lie in the window of the block's in-grid cells)
+  if (p.wrap_mask != 0 || s0 < p.first || s0 >= end || min(s0 + kBlock, p.first + p.n) > tail) {
+    A.tiles[b] = tl;
+    return;
+  }
+  const unsigned s1 = min(s0 + kBlock, end) - 1u;
+  const unsigned gx = unsigned(p.res[0]), gy = unsigned(p.res[1]), gz = unsigned(p.res[2]);
+  const unsigned c0 = A.key[A.sorted_id[s0]] - p.cell_base;
+  const unsigned c1 = A.key[A.sorted_id[s1]] - p.cell_base;
+  const unsigned x0 = c0 / (gy * gz), x1 = c1 / (gy * gz);
+  if (x0 != x1) {
+    A.tiles[b] = tl;
+    return;
+  }
+  const unsigned y0 = (c0 / gz) % gy, z0 = c0 % gz, y1 = (c1 / gz) % gy, z1 = c1 % gz;
+  const unsigned ys = y0 > 0u ? y0 - 1u : 0u;
+  const unsigned zs = (y0 > 0u || y1 == y0) ? (z0 > 0u ? z0 - 1u : 0u) : 0u;
+  unsigned ye, ze;
+  if (y1 + 1u < gy) {
+    ye = y1 + 1u;
+    ze = min(z1 + 1u, gz - 1u);
+  } else {
+    ye = gy - 1u;
+    ze = gz - 1u;
+  }
+  unsigned k = 0, total = 0;
+  for (int dx = -1; dx <= 1; ++dx) {
+    const int xs = int(x0) + dx;
+    if (xs < 0 || xs >= int(gx)) continue;
+    const unsigned cs = (unsigned(xs) * gy + ys) * gz + zs;
+    const unsigned ce = (unsigned(xs) * gy + ye) * gz + ze;
+    const unsigned lo = A.cell_start[p.cell_base + cs];
+    const unsigned hi = A.cell_start[p.cell_base + ce + 1u];
+    tl.lo[k] = lo;
+    tl.len[k] = hi - lo;
+    total += hi - lo;
+    ++k;
+  }
+  tl.total = total;
+  tl.ok = total <= kTileMax ? 1u : 0u;
+  A.tiles[b] = tl;
+}
+
 // ---------------------------------------------------------------------------
 // Gather user -> sorted order and compute the boundary records
 // (compute_particle_boundary_analytic<0>, runner.hpp:1982-2002, with
@@ -288,6 +355,126 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SceneTab T, DevArrays A
   A.bmask[s] = mask;
 }
 
+// Where the neighbour loops read x_j and v_j: global memory (L2 gathers),
+// or the block's LDS tile.
+struct GlobalXV {
+  const float4* x;
+  const float4* v;
+  __device__ float4 X(unsigned j) const { return x[j]; }
+  __device__ float4 V(unsigned j) const { return v[j]; }
+};
+// LDS slot of sorted particle j: the tile's ranges are in increasing order.
+struct TileMap {
+  unsigned lo1, lo2, base0, base1, base2;
+  __device__ unsigned slot(unsigned j) const {
+    return j >= lo2 ? j - base2 : (j >= lo1 ? j - base1 : j - base0);
+  }
+};
+struct TileXV : TileMap {
+  const float4* x;  // LDS
+  const float4* v;  // LDS
+  __device__ float4 X(unsigned j) const { return x[slot(j)]; }
+  __device__ float4 V(unsigned j) const { return v[slot(j)]; }
+};
+// positions and one scalar (the stiffness of a pass)
+struct GlobalXK {
+  const float4* x;
+  const float* k;
+  __device__ float4 X(unsigned j) const { return x[j]; }
+  __device__ float K(unsigned j) const { return k[j]; }
+};
+struct TileXK : TileMap {
+  const float4* x;  // LDS
+  const float* k;   // LDS
+  __device__ float4 X(unsigned j) const { return x[slot(j)]; }
+  __device__ float K(unsigned j) const { return k[slot(j)]; }
+};
+
+// The block's tile ranges and their LDS bases; false when the block has no
+// usable tile.
+__device__ inline bool tile_map(const DevArrays& A, unsigned blk, Tile& tl, TileMap& m) {
+  if (!A.use_tiles) return false;
+  tl = A.tiles[blk];
+  if (!tl.ok) return false;
+  unsigned off = 0, base[3];
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    base[w] = tl.lo[w] - off;  // slot = j - base
+    off += tl.len[w];
+  }
+  m.lo1 = tl.lo[1];
+  m.lo2 = tl.lo[2];
+  m.base0 = base[0];
+  m.base1 = base[1];
+  m.base2 = base[2];
+  return true;
+}
+
+// Copies the block's tile of x and v into LDS (coalesced) when the block has
+// one; every thread of the block must call it (it holds a barrier).  Returns
+// false when the block uses global gathers instead.
+__device__ inline bool stage_xv(const DevArrays& A, unsigned blk, TileXV& t) {
+  __shared__ float4 lds_xv[2 * kTileMax];
+  Tile tl;
+  if (!tile_map(A, blk, tl, t)) return false;
+  unsigned off = 0;
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    const unsigned lo = tl.lo[w], len = tl.len[w];
+    for (unsigned k = threadIdx.x; k < len; k += kBlock) {
+      lds_xv[off + k] = A.x[lo + k];
+      lds_xv[kTileMax + off + k] = A.v[lo + k];
+    }
+    off += len;
+  }
+  __syncthreads();
+  t.x = lds_xv;
+  t.v = lds_xv + kTileMax;
+  return true;
+}
+// positions and the pass's stiffness array ks into LDS
+__device__ inline bool stage_xk(const DevArrays& A, unsigned blk, const float* ks, TileXK& t) {
+  __shared__ float4 lds_x[kTileMax];
+  __shared__ float lds_k[kTileMax];
+  Tile tl;
+  if (!tile_map(A, blk, tl, t)) return false;
+  unsigned off = 0;
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    const unsigned lo = tl.lo[w], len = tl.len[w];
+    for (unsigned k = threadIdx.x; k < len; k += kBlock) {
+      lds_x[off + k] = A.x[lo + k];
+      lds_k[off + k] = ks[lo + k];
+    }
+    off += len;
+  }
+  __syncthreads();
+  t.x = lds_x;
+  t.k = lds_k;
+  return true;
+}
+
+// positions only (the neighbour search)
+struct TileX : TileMap {
+  const float4* x;  // LDS
+  __device__ float4 X(unsigned j) const { return x[slot(j)]; }
+};
+__device__ inline bool stage_x(const DevArrays& A, unsigned blk, TileX& t) {
+  __shared__ float4 lds_xo[kTileMax];
+  Tile tl;
+  if (!tile_map(A, blk, tl, t)) return false;
+  unsigned off = 0;
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    const unsigned lo = tl.lo[w], len = tl.len[w];
+    for (unsigned k = threadIdx.x; k < len; k += kBlock) lds_xo[off + k] = A.x[lo + k];
+    off += len;
+  }
+  __syncthreads();
+  t.x = lds_xo;
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // Neighbour list (make_neighbor_list<wrap>, runner.hpp:1869-1911) fused with
 // compute_density (runner.hpp:2052-2072) and, for DFSPH,
@@ -298,6 +485,9 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SceneTab T, DevArrays A
 template <bool W>
 __global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int with_factor) {
   ALV_SCENE(T, A, s);
+  // candidate positions from the block's LDS tile when it has one
+  TileX tile_;
+  const bool tiled_ = !W && stage_x(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const unsigned K = p.kmax;
   const vec3 xi = ld3(A.x[s]);
@@ -340,7 +530,10 @@ __global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int 
       for (unsigned j0 = jb; j0 < je; j0 += kCB) {
         float4 xc[kCB];
 #pragma unroll
-        for (int u = 0; u < kCB; ++u) xc[u] = A.x[min(j0 + u, je - 1u)];
+        for (int u = 0; u < kCB; ++u) {
+          const unsigned jc = min(j0 + u, je - 1u);
+          xc[u] = tiled_ ? tile_.X(jc) : A.x[jc];
+        }
 #pragma unroll
         for (int u = 0; u < kCB; ++u) {
           const unsigned j = j0 + u;
@@ -380,7 +573,7 @@ __global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int 
     batch_ids(A, s, nn, t0, jn);
     float4 xj[kNB];
 #pragma unroll
-    for (int u = 0; u < kNB; ++u) xj[u] = A.x[jn[u]];
+    for (int u = 0; u < kNB; ++u) xj[u] = tiled_ ? tile_.X(jn[u]) : A.x[jn[u]];
 #pragma unroll
     for (int u = 0; u < kNB; ++u) {
       if (t0 + u < nn) {
@@ -421,9 +614,10 @@ __device__ inline vec3 rigid_velocity(const BoundaryDev& B, vec3 bx) {
 }
 
 // sum_j V (v_i - v_j) . gradW_ij + sum_b (v_i - v_b) . Psi gradW_b
-template <bool W>
+template <bool W, class Src>
 __device__ inline float velocity_divergence(const SimParams& p, const DevArrays& A,
-                                            unsigned s, vec3 xi, vec3 vi, unsigned nn) {
+                                            unsigned s, vec3 xi, vec3 vi, unsigned nn,
+                                            const Src& src) {
   float acc = 0.0f;
   for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
     unsigned j[kNB];
@@ -431,8 +625,8 @@ __device__ inline float velocity_divergence(const SimParams& p, const DevArrays&
     float4 xj[kNB], vj[kNB];
 #pragma unroll
     for (int u = 0; u < kNB; ++u) {
-      xj[u] = A.x[j[u]];
-      vj[u] = A.v[j[u]];
+      xj[u] = src.X(j[u]);
+      vj[u] = src.V(j[u]);
     }
 #pragma unroll
     for (int u = 0; u < kNB; ++u) {
@@ -455,23 +649,24 @@ __device__ inline float velocity_divergence(const SimParams& p, const DevArrays&
 }
 
 // compute_density_change (runner.hpp:3192-3228)
-template <bool W>
-__device__ inline float density_change(const SimParams& p, const DevArrays& A, unsigned s) {
+template <bool W, class Src>
+__device__ inline float density_change(const SimParams& p, const DevArrays& A, unsigned s,
+                                       const Src& src) {
   const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
   const unsigned nn = A.nnb[s];
-  float da = velocity_divergence<W>(p, A, s, xi, vi, nn);
+  float da = velocity_divergence<W>(p, A, s, xi, vi, nn, src);
   da = std_max(da, 0.0f);
   if (nn < 20u) da = 0.0f;
   return da;
 }
 
 // compute_density_adv (runner.hpp:3493-3523)
-template <bool W>
+template <bool W, class Src>
 __device__ inline float density_adv(const SimParams& p, const DevArrays& A, unsigned s,
-                                    float dt) {
+                                    float dt, const Src& src) {
   const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
   const unsigned nn = A.nnb[s];
-  float delta = velocity_divergence<W>(p, A, s, xi, vi, nn);
+  float delta = velocity_divergence<W>(p, A, s, xi, vi, nn, src);
   return std_max(A.density[s] / p.rho0 + dt * delta, 1.0f);
 }
 
@@ -479,9 +674,9 @@ __device__ inline float density_adv(const SimParams& p, const DevArrays& A, unsi
 // divergence_solve_iteration, warm_start_pressure_solve1,
 // pressure_solve_iteration: runner.hpp:3252-3295, 3316-3363, 3452-3491,
 // 3543-3588).  kj[] holds every particle's stiffness of this pass.
-template <bool W, bool kDivergenceForm>
+template <bool W, bool kDivergenceForm, class Src>
 __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsigned s,
-                                    float k_i, const float* __restrict__ kjs, float dt) {
+                                    float k_i, const Src& src, float dt) {
   const vec3 xi = ld3(A.x[s]);
   const unsigned nn = A.nnb[s];
   vec3 dv = mk(0.f, 0.f, 0.f);
@@ -490,12 +685,12 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
     batch_ids(A, s, nn, t0, j);
     float kj[kNB];
 #pragma unroll
-    for (int u = 0; u < kNB; ++u) kj[u] = kjs[j[u]];
+    for (int u = 0; u < kNB; ++u) kj[u] = src.K(j[u]);
     // positions are gathered only for pairs that contribute
     float4 xj[kNB];
 #pragma unroll
     for (int u = 0; u < kNB; ++u)
-      xj[u] = (fabsf(k_i + kj[u]) > p.eps) ? A.x[j[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+      xj[u] = (fabsf(k_i + kj[u]) > p.eps) ? src.X(j[u]) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int u = 0; u < kNB; ++u) {
       const float k_sum = k_i + kj[u];
@@ -537,9 +732,12 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
 template <bool W, bool F>
 __global__ __launch_bounds__(256) void k_ws_div0(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
+  TileXV tile_;
+  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = density_change<W>(p, A, s);
+  float dadv = tiled_ ? density_change<W>(p, A, s, tile_)
+                      : density_change<W>(p, A, s, GlobalXV{A.x, A.v});
   const float kv_prev = A.kappa_v[s];
   const float kv = (dadv > 0.0f) ? 0.5f * std_max(kv_prev, -0.5f) / dt : 0.0f;
   A.kappa_v[s] = kv;
@@ -590,9 +788,12 @@ __global__ __launch_bounds__(256) void k_ws_learned(SceneTab T, DevArrays A, Mlp
 template <bool W>
 __global__ __launch_bounds__(256) void k_ws_div1(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
+  TileXK tk_;
+  const bool tiled_ = !W && stage_xk(A, s / kBlock, A.kappa_v, tk_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  vec3 dv = stiffness_dv<W, true>(p, A, s, A.kappa_v[s], A.kappa_v, dt);
+  vec3 dv = tiled_ ? stiffness_dv<W, true>(p, A, s, A.kappa_v[s], tk_, dt)
+                   : stiffness_dv<W, true>(p, A, s, A.kappa_v[s], GlobalXK{A.x, A.kappa_v}, dt);
   add_v(A, s, dv);
 }
 // compute_velocity_of_density_change (runner.hpp:3297-3314); also stores the
@@ -600,9 +801,12 @@ __global__ __launch_bounds__(256) void k_ws_div1(SceneTab T, DevArrays A) {
 template <bool W>
 __global__ __launch_bounds__(256) void k_vel_density_change(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
+  TileXV tile_;
+  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = density_change<W>(p, A, s);
+  float dadv = tiled_ ? density_change<W>(p, A, s, tile_)
+                      : density_change<W>(p, A, s, GlobalXV{A.x, A.v});
   A.rho_adv[s] = dadv;
   float inv_dt = 1.0f / dt;
   float alpha = A.alpha[s] * inv_dt;
@@ -615,11 +819,14 @@ template <bool W>
 __global__ __launch_bounds__(256) void k_div_iter(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   if (A.ctl->div_done) return;
+  TileXK tk_;
+  const bool tiled_ = !W && stage_xk(A, s / kBlock, A.kfac, tk_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   const float k_i = A.kfac[s];
   A.kappa_v[s] += k_i;
-  vec3 dv = stiffness_dv<W, true>(p, A, s, k_i, A.kfac, dt);
+  vec3 dv = tiled_ ? stiffness_dv<W, true>(p, A, s, k_i, tk_, dt)
+                   : stiffness_dv<W, true>(p, A, s, k_i, GlobalXK{A.x, A.kfac}, dt);
   add_v(A, s, dv);
 }
 // compute_divergence_solve_density_error (runner.hpp:3396-3410) + FP64
@@ -629,9 +836,12 @@ __global__ __launch_bounds__(256) void k_div_err(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
   ALV_SCENE(T, A, s);
   if (A.ctl->div_done) return;
+  TileXV tile_;
+  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
   double part = 0.0;
   if (s < p.first + p.n) {
-    float dadv = density_change<W>(p, A, s);
+    float dadv = tiled_ ? density_change<W>(p, A, s, tile_)
+                        : density_change<W>(p, A, s, GlobalXV{A.x, A.v});
     A.rho_adv[s] = dadv;
     const float k = dadv * A.alpha[s];
     A.kfac[s] = k;
@@ -673,6 +883,33 @@ __global__ void k_ctl_reset(DevArrays A) {
   ctl->dens_done = dens_go ? 0u : 1u;
 }
 
+// sum_j mu m / rho_j (x_ij . gradW_ij) / (|x_ij|^2 + 0.01 h^2) (v_i - v_j)
+template <bool W, class Src>
+__device__ inline vec3 viscosity_sum(const SimParams& p, const DevArrays& A, unsigned s,
+                                     vec3 xi, vec3 vi, unsigned nn, const Src& src) {
+  vec3 da = mk(0.f, 0.f, 0.f);
+  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+    unsigned j[kNB];
+    batch_ids(A, s, nn, t0, j);
+    float4 xj[kNB], vj[kNB];  // vj.w = density of j
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      xj[u] = src.X(j[u]);
+      vj[u] = src.V(j[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      if (t0 + u < nn) {
+        vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+        da += (p.visc * p.mass / vj[u].w * dot(xixj, gradW(xixj, p)) /
+               (length_sqr(xixj) + 0.01f * p.h2)) *
+              (vi - ld3(vj[u]));
+      }
+    }
+  }
+  return da;
+}
+
 // ---------------------------------------------------------------------------
 // Viscosity (compute_viscosity, runner.hpp:2150-2249) with clear_acceleration
 // (a = g, runner.hpp:1974-1980), optional divergence_solve_finish
@@ -695,31 +932,15 @@ template <bool W>
 __global__ __launch_bounds__(256) void k_viscosity(SceneTab T, DevArrays A, int div_finish) {
   __shared__ float lds[4];
   ALV_SCENE(T, A, s);
+  TileXV tile_;
+  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
   const float dt = A.ctl->dt;
   float cfl = -FLT_MAX;
   if (s < p.first + p.n) {
     const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
     const unsigned nn = A.nnb[s];
-    vec3 da = mk(0.f, 0.f, 0.f);
-    for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
-      unsigned j[kNB];
-      batch_ids(A, s, nn, t0, j);
-      float4 xj[kNB], vj[kNB];  // vj.w = density of j
-#pragma unroll
-      for (int u = 0; u < kNB; ++u) {
-        xj[u] = A.x[j[u]];
-        vj[u] = A.v[j[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < kNB; ++u) {
-        if (t0 + u < nn) {
-          vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
-          da += (p.visc * p.mass / vj[u].w * dot(xixj, gradW(xixj, p)) /
-                 (length_sqr(xixj) + 0.01f * p.h2)) *
-                (vi - ld3(vj[u]));
-        }
-      }
-    }
+    vec3 da = tiled_ ? viscosity_sum<W>(p, A, s, xi, vi, nn, tile_)
+                     : viscosity_sum<W>(p, A, s, xi, vi, nn, GlobalXV{A.x, A.v});
     unsigned mask = A.bmask[s];
     while (mask) {
       const unsigned b = __builtin_ctz(mask);
@@ -816,9 +1037,12 @@ __global__ __launch_bounds__(256) void k_integrate_np(SceneTab T, DevArrays A) {
 template <bool W, bool L>
 __global__ __launch_bounds__(256) void k_ws_p0(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
+  TileXV tile_;
+  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = density_adv<W>(p, A, s, dt);
+  float dadv = tiled_ ? density_adv<W>(p, A, s, dt, tile_)
+                      : density_adv<W>(p, A, s, dt, GlobalXV{A.x, A.v});
   float k = (dadv > 1.0f) ? (0.5f * std_max(A.kappa[s], -0.00025f) / (dt * dt)) : 0.0f;
   if (L && dadv > 1.0f) {
     const float kl = A.kpred[s] / (dt * dt);
@@ -830,18 +1054,24 @@ __global__ __launch_bounds__(256) void k_ws_p0(SceneTab T, DevArrays A) {
 template <bool W>
 __global__ __launch_bounds__(256) void k_ws_p1(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
+  TileXK tk_;
+  const bool tiled_ = !W && stage_xk(A, s / kBlock, A.kappa, tk_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  vec3 dv = stiffness_dv<W, false>(p, A, s, A.kappa[s], A.kappa, dt);
+  vec3 dv = tiled_ ? stiffness_dv<W, false>(p, A, s, A.kappa[s], tk_, dt)
+                   : stiffness_dv<W, false>(p, A, s, A.kappa[s], GlobalXK{A.x, A.kappa}, dt);
   add_v(A, s, dv);
 }
 // compute_rho_adv (runner.hpp:3525-3541); stores k = (rho_adv - 1) * alpha
 template <bool W>
 __global__ __launch_bounds__(256) void k_rho_adv(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
+  TileXV tile_;
+  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = density_adv<W>(p, A, s, dt);
+  float dadv = tiled_ ? density_adv<W>(p, A, s, dt, tile_)
+                      : density_adv<W>(p, A, s, dt, GlobalXV{A.x, A.v});
   A.rho_adv[s] = dadv;
   float alpha = A.alpha[s] * (1.0f / (dt * dt));
   A.alpha[s] = alpha;
@@ -853,11 +1083,14 @@ template <bool W>
 __global__ __launch_bounds__(256) void k_p_iter(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   if (A.ctl->dens_done) return;
+  TileXK tk_;
+  const bool tiled_ = !W && stage_xk(A, s / kBlock, A.kfac, tk_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   const float k_i = A.kfac[s];
   A.kappa[s] += k_i;
-  vec3 dv = stiffness_dv<W, false>(p, A, s, k_i, A.kfac, dt);
+  vec3 dv = tiled_ ? stiffness_dv<W, false>(p, A, s, k_i, tk_, dt)
+                   : stiffness_dv<W, false>(p, A, s, k_i, GlobalXK{A.x, A.kfac}, dt);
   add_v(A, s, dv);
 }
 // compute_pressure_solve_density_error (runner.hpp:3590-3604) + FP64
@@ -867,10 +1100,13 @@ __global__ __launch_bounds__(256) void k_p_err(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
   ALV_SCENE(T, A, s);
   if (A.ctl->dens_done) return;
+  TileXV tile_;
+  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
   const float dt = A.ctl->dt;
   double part = 0.0;
   if (s < p.first + p.n) {
-    float dadv = density_adv<W>(p, A, s, dt);
+    float dadv = tiled_ ? density_adv<W>(p, A, s, dt, tile_)
+                        : density_adv<W>(p, A, s, dt, GlobalXV{A.x, A.v});
     A.rho_adv[s] = dadv;
     const float k = (dadv - 1.0f) * A.alpha[s];
     A.kfac[s] = k;
@@ -920,8 +1156,11 @@ __global__ __launch_bounds__(256) void k_df_finish(SceneTab T, DevArrays A, int 
 template <bool W>
 __global__ __launch_bounds__(256) void k_density_change(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
+  TileXV tile_;
+  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
-  A.rho_adv[s] = density_change<W>(p, A, s);
+  A.rho_adv[s] = tiled_ ? density_change<W>(p, A, s, tile_)
+                        : density_change<W>(p, A, s, GlobalXV{A.x, A.v});
 }
 
 // Training pairs of the step just taken, in sorted order: the features
@@ -1226,6 +1465,136 @@ __global__ __launch_bounds__(256) void k_ii_finish(SceneTab T, DevArrays A) {
 }
 
 // ---------------------------------------------------------------------------
+// ISPH: pressure-Laplacian Jacobi solve (SolverI::step, solver_i.hpp:79-330;
+// kernels runner.hpp:2890-3057).  The Jacobi iterations ping-pong between
+// last_pressure and pressure: iteration k reads buffer k & 1 and writes the
+// other, which is the reference's "iterate, then last_pressure <- pressure"
+// without the copy.  k_is_settle leaves the final values in `pressure`.
+// ---------------------------------------------------------------------------
+// advect_and_init_pressure (runner.hpp:2890-2900)
+__global__ __launch_bounds__(256) void k_is_advect(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
+  const float dt = A.ctl->dt;
+  add_v(A, s, dt * ld3(A.a[s]));
+  A.last_pressure[s] = A.pressure[s] * 0.5f;
+}
+// calculate_isph_diagonal_adv_density (runner.hpp:2901-2949): the Jacobi
+// diagonal (times rho0) into aii, the advected density into adv_density
+template <bool W>
+__global__ __launch_bounds__(256) void k_is_diag(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
+  const float dt = A.ctl->dt;
+  const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
+  const float dens = A.density[s];
+  const float eps = 0.01f * p.h2;
+  float dadv = dens / p.rho0;
+  float diag = 0.f;
+  const unsigned nn = A.nnb[s];
+  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+    unsigned j[kNB];
+    batch_ids(A, s, nn, t0, j);
+    float4 xj[kNB], vj[kNB];
+    float djs[kNB];
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      xj[u] = A.x[j[u]];
+      vj[u] = A.v[j[u]];
+      djs[u] = A.density[j[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      if (t0 + u < nn) {
+        const vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+        const vec3 gw = gradW(xixj, p);
+        const float dj = djs[u];
+        dadv += dt * p.vol * dot(vi - ld3(vj[u]), gw);
+        diag += p.vol * (dens + dj) / (dens * dj) / (length_sqr(xixj) + eps) * dot(xixj, gw);
+      }
+    }
+  }
+  unsigned mask = A.bmask[s];
+  while (mask) {
+    const unsigned b = __builtin_ctz(mask);
+    mask &= mask - 1u;
+    const size_t o = size_t(b) * A.cap + s;
+    const float4 r = A.brec[o], k = A.bker[o];
+    const vec3 g = mk(k.x, k.y, k.z);
+    const vec3 xb = ld3(r);
+    const vec3 xib = xi - xb;
+    dadv += dt * dot(vi - rigid_velocity(A.bnd[b], xb), g);
+    diag += (dens + p.rho0) / (dens * p.rho0) / (length_sqr(xib) + eps) * dot(xib, g);
+  }
+  A.aii[s] = diag * p.rho0;
+  A.adv_density[s] = dadv;
+}
+// isph_solve_iteration (runner.hpp:2996-3042) + FP64 partial sums
+template <bool W>
+__global__ __launch_bounds__(256) void k_is_iter(SceneTab T, DevArrays A) {
+  __shared__ double lds[4];
+  ALV_SCENE(T, A, s);
+  if (A.ctl->dens_done) return;
+  const float dt = A.ctl->dt;
+  const bool odd = (A.ctl->dens_iter & 1u) != 0u;
+  const float* lp = odd ? A.pressure : A.last_pressure;
+  float* out = odd ? A.last_pressure : A.pressure;
+  double part = 0.0;
+  if (s < p.first + p.n) {
+    const vec3 xi = ld3(A.x[s]);
+    const float dens = A.density[s];
+    const float diag = A.aii[s];
+    const float bb = 1.0f - A.adv_density[s];
+    const float dt2 = dt * dt;
+    const float denom = diag * dt2;
+    const float eps = 0.01f * p.h2;
+    const float w = 2.0f / 3.0f;
+    float off = 0.f;
+    const unsigned nn = A.nnb[s];
+    for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+      unsigned j[kNB];
+      batch_ids(A, s, nn, t0, j);
+      float4 xj[kNB];
+      float djs[kNB], lpj[kNB];
+#pragma unroll
+      for (int u = 0; u < kNB; ++u) {
+        xj[u] = A.x[j[u]];
+        djs[u] = A.density[j[u]];
+        lpj[u] = lp[j[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kNB; ++u) {
+        if (t0 + u < nn) {
+          const vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+          const float dj = djs[u];
+          off -= p.vol * (dens + dj) / (dens * dj) / (length_sqr(xixj) + eps) *
+                 dot(xixj, gradW(xixj, p)) * lpj[u];
+        }
+      }
+    }
+    off *= p.rho0;
+    float pr = 0.f;
+    if (fabsf(denom) > 1.0e-9f)
+      pr = std_max((1.0f - w) * lp[s] + w * (bb / denom - off / diag), 0.0f);
+    const float err = pr == 0.f ? 0.f : (denom * pr + off * dt2 - bb);
+    A.density_err[s] = err;
+    out[s] = pr;
+    part = double(err);
+  }
+  double t = block_sum_d(part, lds);
+  if (threadIdx.x == 0) A.partials[xcd_block()] = t;
+}
+// After an even, non-zero number of iterations the result is in
+// last_pressure; with none, `pressure` keeps the previous step's values, as
+// in the reference.
+__global__ __launch_bounds__(256) void k_is_settle(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
+  const unsigned it = A.ctl->dens_iter;
+  if (it != 0u && (it & 1u) == 0u) A.pressure[s] = A.last_pressure[s];
+}
+
+// ---------------------------------------------------------------------------
 // Rigid-body sums (solver_df.hpp:347-353): FP64, deterministic two-level
 // reduction over the (particle, body) pairs with Psi != 0.  Grid (G, B):
 // block (g, b) writes the partial sums of body b over its particle range.
@@ -1457,6 +1826,10 @@ void launch_scatter(const Launch& L, const DevArrays& A, hipStream_t s) {
 void launch_cell_sort(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_GRID(k_cell_sort, dim3(L.cblocks), s, L.T, A);
 }
+void launch_tiles(const Launch& L, const DevArrays& A, hipStream_t s) {
+  if (!A.use_tiles) return;
+  ALV_LAUNCH_GRID(k_tiles, dim3(grid_for(L.pblocks)), s, L.T, A, L.pblocks);
+}
 void launch_gather_boundary(const Launch& L, const DevArrays& A, int kind, hipStream_t s) {
   ALV_LAUNCH(k_gather_boundary, L, s, A, kind);
 }
@@ -1525,6 +1898,17 @@ void launch_ii_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_W(k_ii_iter0, L, s, A);
   ALV_LAUNCH_W(k_ii_iter1, L, s, A);
   ALV_LAUNCH_SCENES(k_ii_finalize, L, s, A);
+}
+void launch_is_prepare(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_is_advect, L, s, A);
+  ALV_LAUNCH_W(k_is_diag, L, s, A);
+}
+void launch_is_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_W(k_is_iter, L, s, A);
+  ALV_LAUNCH_SCENES(k_ii_finalize, L, s, A);
+}
+void launch_is_settle(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_is_settle, L, s, A);
 }
 void launch_ii_finish(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_W(k_ii_pressure_accel, L, s, A);

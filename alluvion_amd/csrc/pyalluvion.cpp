@@ -7,6 +7,8 @@
 // the vector types and the NumericType / VolumeMethod enums.  Everything is
 // a thin layer over alv/alluvion.hpp, which itself calls the C ABI (alv.h).
 #include <pybind11/numpy.h>
+#include <cstring>
+
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
@@ -67,6 +69,9 @@ void declare_variable(py::module& m, py::class_<Store>& store, const std::string
       .def("get_linear_shape", &Var::get_linear_shape)
       .def("get_num_primitives", &Var::get_num_primitives)
       .def("get_num_bytes", &Var::get_num_bytes)
+      .def("read_file", &Var::read_file, py::arg("filename"))
+      .def("write_file", &Var::write_file, py::arg("filename"),
+           py::arg("shape_outermost") = 0, py::arg("offset") = 0)
       .def("get_shape", &Var::get_shape);
 }
 
@@ -173,6 +178,7 @@ PYBIND11_MODULE(_alluvion, m) {
       py::class_<Store>(m, "Store")
           .def(py::init<>())
           .def_static("set_device", &Store::set_device)
+          .def_static("get_alu_info", &Store::get_alu_info, py::arg("filename"))
           .def("has_display", [](Store const&) { return false; })
           .def_property_readonly("cnfloat", &Store::get_cn, py::return_value_policy::reference_internal)
           .def_property_readonly("cni", &Store::get_cni, py::return_value_policy::reference_internal)
@@ -221,20 +227,40 @@ PYBIND11_MODULE(_alluvion, m) {
       .def_static("create", &dg::SphereDistance<float3, float>::create, py::arg("radius"),
                   py::return_value_policy::reference);
 
+  py::class_<Mesh>(m, "Mesh")
+      .def(py::init<>())
+      .def_readonly("vertices", &Mesh::vertices)
+      .def("set_vertices",
+           [](Mesh& mesh, py::array_t<float, py::array::c_style | py::array::forcecast> xyz) {
+             if (xyz.ndim() != 2 || xyz.shape(1) != 3)
+               throw std::invalid_argument("vertices must be [n, 3]");
+             mesh.vertices.resize(size_t(xyz.shape(0)));
+             std::memcpy(mesh.vertices.data(), xyz.data(), mesh.vertices.size() * sizeof(float3));
+           })
+      .def("set_box", &Mesh::set_box, py::arg("widths"), py::arg("n") = 2)
+      .def("set_uv_sphere", &Mesh::set_uv_sphere, py::arg("radius"), py::arg("num_sectors"),
+           py::arg("num_stacks"))
+      .def("translate", &Mesh::translate)
+      .def("rotate", &Mesh::rotate)
+      .def("scale", &Mesh::scale)
+      .def("clear", &Mesh::clear);
+
   using P = Pile<float>;
   py::class_<P>(m, "Pilefloat")
       .def(py::init<Store&, Runner<float>&, U, VolumeMethod, U, Const<float>*, ConstiN*>(),
            py::arg("store"), py::arg("runner"), py::arg("max_num_contacts"),
            py::arg("volume_method") = VolumeMethod::volume_map,
            py::arg("max_num_pellets") = 10000, py::arg("cn") = nullptr, py::arg("cni") = nullptr,
-           py::keep_alive<1, 2>())
+           // the pile reads cn and cni later (contact tolerance, boundary count)
+           py::keep_alive<1, 2>(), py::keep_alive<1, 7>(), py::keep_alive<1, 8>())
       .def("add",
            [](P& p, Dist* distance, U3 resolution, float cell_width, float margin, float sign,
               py::object collision_mesh, float mass, float restitution, float friction,
               float3 inertia_tensor, float3 x, float4 q, py::object display_mesh) {
-             (void)collision_mesh;
              (void)display_mesh;
-             return p.add(distance, resolution, cell_width, margin, sign, mass, restitution,
+             Mesh none;
+             const Mesh& coll = collision_mesh.is_none() ? none : collision_mesh.cast<const Mesh&>();
+             return p.add(distance, resolution, cell_width, margin, sign, coll, mass, restitution,
                           friction, inertia_tensor, x, q);
            },
            py::arg("distance"), py::arg("resolution") = U3{0, 0, 0}, py::arg("cell_width") = 0,
@@ -262,6 +288,54 @@ PYBIND11_MODULE(_alluvion, m) {
       .def("set_gravity", &P::set_gravity)
       .def("integrate_kinematics", &P::integrate_kinematics)
       .def("copy_kinematics_to_device", &P::copy_kinematics_to_device)
+      .def("find_contacts", [](P& p, U i, U j) { return p.find_contacts(i, j); })
+      .def("find_contacts", [](P& p, U i) { return p.find_contacts(i); })
+      .def("find_contacts",
+           [](P& p, py::object particle_x, U max_num_beads) {
+             (void)particle_x;  // pellet contacts are not on this path
+             (void)max_num_beads;
+             return p.find_contacts();
+           })
+      .def("find_contacts", [](P& p) { return p.find_contacts(); })
+      .def("solve_contacts", &P::solve_contacts)
+      .def_property_readonly("num_contacts_pinned",
+                             [](P const& p) { return U(p.get_contacts().size()); })
+      .def("get_contacts",
+           [](P const& p) {
+             py::list out;
+             auto v3 = [](const float* a) { return py::make_tuple(a[0], a[1], a[2]); };
+             for (const Contact& c : p.get_contacts()) {
+               py::dict d;
+               d["i"] = c.i;
+               d["j"] = c.j;
+               d["cp_i"] = v3(c.cp_i);
+               d["cp_j"] = v3(c.cp_j);
+               d["n"] = v3(c.n);
+               d["t"] = v3(c.t);
+               d["iiwi_diag"] = v3(c.iiwi_diag);
+               d["iiwi_off_diag"] = v3(c.iiwi_off_diag);
+               d["iiwj_diag"] = v3(c.iiwj_diag);
+               d["iiwj_off_diag"] = v3(c.iiwj_off_diag);
+               d["friction"] = c.friction;
+               d["nkninv"] = c.nkninv;
+               d["pmax"] = c.pmax;
+               d["goalu"] = c.goalu;
+               d["impulse_sum"] = c.impulse_sum;
+               out.append(d);
+             }
+             return out;
+           })
+      .def("set_collision_vertices",
+           [](P& p, U i, py::array_t<float, py::array::c_style | py::array::forcecast> xyz) {
+             if (xyz.ndim() != 2 || xyz.shape(1) != 3)
+               throw std::invalid_argument("vertices must be [n, 3]");
+             check(alv_pile_set_collision_vertices(p.handle(), i, xyz.data(), U(xyz.shape(0))));
+           })
+      .def("write_file", &P::write_file, py::arg("filename"), py::arg("x_scale") = 1,
+           py::arg("v_scale") = 1, py::arg("omega_scale") = 1)
+      .def("read_file", &P::read_file, py::arg("filename"), py::arg("num_rigids") = -1,
+           py::arg("dst_offset") = 0, py::arg("src_offset") = 0)
+      .def_static("get_size_from_file", &P::get_size_from_file, py::arg("filename"))
       .def_property_readonly("volume_grids",
                              [](P const& p) {
                                std::vector<Variable<1, float>> out;
@@ -337,6 +411,7 @@ PYBIND11_MODULE(_alluvion, m) {
         .def("compute_density_change", &S::compute_density_change)
         .def_property_readonly("num_scenes", &S::num_scenes)
         .def("set_graph", &S::set_graph, py::arg("enable"), py::arg("slack") = 0)
+        .def("set_tiles", &S::set_tiles, py::arg("enable"))
         .def("scene_range", &S::scene_range, py::arg("scene"))
         .def("scene_report",
              [](S const& s, U k) {
@@ -457,6 +532,15 @@ PYBIND11_MODULE(_alluvion, m) {
       .def_property("min_density_solve", &II::get_min_density_solve, &II::set_min_density_solve)
       .def_property("max_density_solve", &II::get_max_density_solve, &II::set_max_density_solve)
       .def_property_readonly("particle_pressure", py::cpp_function(&II::particle_pressure, py::keep_alive<0, 1>()));
+
+  // SolverI: the settings, report and pressure field of SolverIi
+  py::class_<SolverI<float>, II>(m, "SolverIfloat")
+      .def(py::init<Runner<float>&, Pile<float>&, Store&, U, U, bool, bool, Const<float> const*,
+                    ConstiN const*, bool>(),
+           py::arg("runner"), py::arg("pile"), py::arg("store"), py::arg("max_num_particles"),
+           py::arg("num_ushers") = 0, py::arg("enable_surface_tension") = false,
+           py::arg("enable_vorticity") = false, py::arg("cn") = nullptr, py::arg("cni") = nullptr,
+           py::arg("graphical") = false, py::keep_alive<1, 3>());
 
   py::class_<WarmStartNet>(m, "WarmStartNet")
       .def(py::init<unsigned long long>(), py::arg("seed") = 0)

@@ -23,7 +23,7 @@ class Depot(Store):
         self.default_dtype = default_dtype
         super().__init__()
         suffix = self.dtype_to_string(default_dtype)
-        for classname in ["Runner", "Pile", "SolverDf", "SolverIi"]:
+        for classname in ["Runner", "Pile", "SolverDf", "SolverIi", "SolverI"]:
             setattr(self, classname, getattr(_al, f"{classname}{suffix}"))
         for classname in ["Distance", "BoxDistance", "BoxShellDistance", "SphereDistance"]:
             setattr(self, classname, getattr(_al.dg, f"{classname}{suffix}"))

@@ -57,6 +57,10 @@ class BoundarySpec:
     inertia: Tuple[float, float, float] = (1.0, 1.0, 1.0)
     x: Tuple[float, float, float] = (0.0, 0.0, 0.0)
     q: Tuple[float, float, float, float] = (0.0, 0.0, 0.0, 1.0)
+    restitution: float = 1.0
+    friction: float = 0.0
+    # collision vertices in the body frame, (n, 3); None = no contacts
+    collision_vertices: Optional[np.ndarray] = None
 
 
 @dataclass
@@ -91,6 +95,9 @@ class Scene:
     steps: int = 1000
     boundaries: List[BoundarySpec] = field(default_factory=list)
     pile_gravity: Tuple[float, float, float] = (0.0, -9.81, 0.0)
+    # rigid contacts (Const::contact_tolerance, Consti::max_num_contacts)
+    contact_tolerance: float = 0.0
+    max_num_contacts: int = 10000
 
     @property
     def n(self) -> int:
@@ -212,7 +219,8 @@ def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optiona
     cni.max_num_neighbors_per_particle = scene.max_num_neighbors
     cni.max_num_particles_per_cell = scene.max_num_particles_per_cell
     runner = dp.Runner()
-    pile = dp.Pile(dp, runner, 0, cn=cn, cni=cni)
+    cn.contact_tolerance = scene.contact_tolerance
+    pile = dp.Pile(dp, runner, scene.max_num_contacts, cn=cn, cni=cni)
     for b in scene.boundaries:
         if b.kind == "box":
             dist = dp.BoxDistance.create(dp.f3(*b.params[:3]), b.params[3])
@@ -220,13 +228,18 @@ def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optiona
             dist = dp.BoxShellDistance.create(dp.f3(*b.params[:3]), b.params[3], b.params[4])
         else:
             dist = dp.SphereDistance.create(b.params[0])
-        pile.add(dist, _al.uint3(*b.resolution), sign=b.sign, mass=b.mass,
+        mesh = None
+        if b.collision_vertices is not None:
+            mesh = _al.Mesh()
+            mesh.set_vertices(np.asarray(b.collision_vertices, np.float32).reshape(-1, 3))
+        pile.add(dist, _al.uint3(*b.resolution), sign=b.sign, collision_mesh=mesh,
+                 mass=b.mass, restitution=b.restitution, friction=b.friction,
                  inertia_tensor=dp.f3(*b.inertia), x=dp.f3(*b.x), q=dp.f4(*b.q))
         if log:
             log("boundary %s %s built" % (b.kind, b.resolution))
     pile.set_gravity(dp.f3(*scene.pile_gravity))
     n_max = scene.n if max_particles is None else max_particles
-    cls = dp.SolverDf if kind == "df" else dp.SolverIi
+    cls = {"df": dp.SolverDf, "ii": dp.SolverIi, "i": dp.SolverI}[kind]
     solver = cls(runner, pile, dp, n_max, cn=cn, cni=cni)
     solver.num_particles = scene.n
     solver.dt = scene.dt
@@ -333,6 +346,6 @@ def build_batch(scenes: List[Scene], dp=None, log=None):
     for k, sc in enumerate(scenes):
         first, count = solver.scene_range(k)
         assert count == sc.n
-        X.set(sc.x, offset=first * 12)
-        V.set(sc.v, offset=first * 12)
+        X.set(sc.x, offset=first)
+        V.set(sc.v, offset=first)
     return dp, piles, solver, cn, cni

@@ -458,9 +458,10 @@ def main():
     if not args.no_autotune and hasattr(wl.solver, "set_graph") and args.workload != "periodic_8m":
         from alluvion_amd import autotune
         best, timing = autotune.tune_step(wl.solver, log=log)
-        tuned = {"graph": best[0], "slack": best[1],
-                 "ms_per_step": {"%s/%d" % ("graph" if g else "direct", s): round(t, 4)
-                                 for (g, s), t in timing.items()}}
+        tuned = {"lds_tiles": best[0], "graph": best[1], "slack": best[2],
+                 "ms_per_step": {"%s/%s/%d" % ("tiles" if ti else "gathers",
+                                               "graph" if g else "direct", s): round(t, 4)
+                                 for (ti, g, s), t in timing.items()}}
     for i in range(warmup):
         wl.step()
         log("warmup step %d" % i)

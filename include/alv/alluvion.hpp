@@ -7,9 +7,12 @@
 // Errors throw std::runtime_error like the reference (allocator.cu:35-41).
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -102,33 +105,146 @@ class Variable {
   U get_num_bytes() const { return get_linear_shape() * sizeof(M); }
   NumericType get_type() const { return TypeInfo<M>::type; }
   std::vector<U> get_shape() const { return std::vector<U>(shape_.begin(), shape_.end()); }
+  // Offsets count elements of M, as in the reference (variable.hpp:60-82).
   void get_bytes(void* dst, U num_bytes, U offset = 0) const {
     if (num_bytes == 0) return;
-    if (offset + num_bytes > get_num_bytes())
+    const size_t byte_offset = size_t(offset) * sizeof(M);
+    if (byte_offset + num_bytes > get_num_bytes())
       throw std::out_of_range("Retrieving more bytes than allocated");
-    check(alv_memcpy_d2h(dst, static_cast<char*>(ptr_) + offset, num_bytes));
+    check(alv_memcpy_d2h(dst, static_cast<char*>(ptr_) + byte_offset, num_bytes));
   }
   void get_bytes(void* dst) const { get_bytes(dst, get_num_bytes()); }
   void set_bytes(void const* src, U num_bytes, U offset = 0) {
     if (num_bytes == 0) return;
-    if (offset + num_bytes > get_num_bytes())
+    const size_t byte_offset = size_t(offset) * sizeof(M);
+    if (byte_offset + num_bytes > get_num_bytes())
       throw std::out_of_range("Setting more bytes than allocated");
-    check(alv_memcpy_h2d(static_cast<char*>(ptr_) + offset, src, num_bytes));
+    check(alv_memcpy_h2d(static_cast<char*>(ptr_) + byte_offset, src, num_bytes));
   }
   void set_bytes(void const* src) { set_bytes(src, get_num_bytes()); }
   void set_zero() { check(alv_memset(ptr_, 0, get_num_bytes())); }
   void set_same(int value, U num_elements, U offset = 0) {
+    if (num_elements == 0) return;
+    if ((size_t(offset) + num_elements) * sizeof(M) > get_num_bytes())
+      throw std::out_of_range("Setting more bytes than allocated");
     check(alv_memset(static_cast<char*>(ptr_) + offset * sizeof(M), value,
                      num_elements * sizeof(M)));
   }
   void set_same(int value) { check(alv_memset(ptr_, value, get_num_bytes())); }
   void set_from(Variable const& src, U num_elements, U offset = 0) {
+    if (num_elements == 0) return;
+    if ((size_t(offset) + num_elements) * sizeof(M) > get_num_bytes() ||
+        size_t(num_elements) * sizeof(M) > src.get_num_bytes())
+      throw std::out_of_range("Copying more bytes than allocated");
     check(alv_memcpy_d2d(static_cast<char*>(ptr_) + offset * sizeof(M), src.ptr_,
                          num_elements * sizeof(M)));
   }
   void set_from(Variable const& src) {
     check(alv_memcpy_d2d(ptr_, src.ptr_, std::min(get_num_bytes(), src.get_num_bytes())));
   }
+
+  // .alu files (variable.hpp:132-261): D u32 shape items (the outermost
+  // may be smaller than the variable's), a u32 0, the u32 number of
+  // primitives per element, a one-byte type label ('f', 'd', 'i', 'u')
+  // and the raw little-endian elements.  write_file writes
+  // `shape_outermost` outer rows starting at element `offset`; 0 means
+  // the whole outermost extent (the reference sizes the payload before
+  // applying that default and so writes no data then; here the payload
+  // always matches the header).
+  void write_file(const char* filename, U shape_outermost = 0, U offset = 0) const {
+    if (shape_outermost > shape_[0])
+      throw std::runtime_error("Writing more than the outermost shape (" +
+                               std::to_string(shape_outermost) + ">" +
+                               std::to_string(shape_[0]) + ").");
+    if (shape_outermost == 0) shape_outermost = shape_[0];
+    const size_t inner = shape_[0] ? get_linear_shape() / shape_[0] : 0;
+    const size_t linear = inner * shape_outermost;
+    const size_t num_bytes = linear * sizeof(M);
+    if (num_bytes + size_t(offset) * sizeof(M) > get_num_bytes())
+      throw std::runtime_error("Writing more than allocated: " +
+                               std::to_string(num_bytes + size_t(offset) * sizeof(M)) + " " +
+                               std::to_string(get_num_bytes()));
+    std::ofstream stream(filename, std::ios::binary | std::ios::trunc);
+    if (!stream.is_open()) throw std::runtime_error(std::string("Failed to open alu file: ") + filename);
+    for (U i = 0; i < D; ++i) {
+      const U item = i == 0 ? shape_outermost : shape_[i];
+      stream.write(reinterpret_cast<const char*>(&item), sizeof(U));
+    }
+    const U end_of_shape = 0, prims = get_num_primitives_per_element();
+    stream.write(reinterpret_cast<const char*>(&end_of_shape), sizeof(U));
+    stream.write(reinterpret_cast<const char*>(&prims), sizeof(U));
+    const char label = type_label(get_type());
+    stream.write(&label, 1);
+    std::vector<char> host(num_bytes);
+    get_bytes(host.data(), U(num_bytes), offset);
+    stream.write(host.data(), std::streamsize(num_bytes));
+    if (!stream) throw std::runtime_error(std::string("Failed to write alu file: ") + filename);
+  }
+
+  // Reads an .alu file into the first rows of the variable and returns the
+  // file's outermost extent.  Inner extents, the element width and the type
+  // must match; FP32 and FP64 files convert into each other.
+  U read_file(const char* filename) {
+    std::ifstream stream(filename, std::ios::binary);
+    if (!stream.is_open()) throw std::runtime_error(std::string("Failed to open alu file: ") + filename);
+    U outermost = 0;
+    for (U i = 0; i < D; ++i) {
+      U item = 0;
+      stream.read(reinterpret_cast<char*>(&item), sizeof(U));
+      if (!stream || item == 0)
+        throw std::runtime_error(std::string("Shape mismatch when reading ") + filename);
+      if (i == 0) {
+        outermost = item;
+        if (outermost > shape_[0])
+          throw std::runtime_error("Reading more than the outermost shape (" +
+                                   std::to_string(outermost) + ">" + std::to_string(shape_[0]) +
+                                   "): " + filename + ".");
+      } else if (item != shape_[i]) {
+        throw std::runtime_error("Shape mismatch for shape item " + std::to_string(i) + "(" +
+                                 std::to_string(shape_[i]) + "!=" + std::to_string(item) +
+                                 "): " + filename + ".");
+      }
+    }
+    U end_of_shape = 1, prims = 0;
+    stream.read(reinterpret_cast<char*>(&end_of_shape), sizeof(U));
+    if (!stream || end_of_shape != 0)
+      throw std::runtime_error(std::string("Dimension mismatch when reading ") + filename);
+    stream.read(reinterpret_cast<char*>(&prims), sizeof(U));
+    if (!stream || prims != get_num_primitives_per_element())
+      throw std::runtime_error(std::string("Num primitives per unit mismatch when reading ") +
+                               filename + "(" + std::to_string(prims) +
+                               "!=" + std::to_string(get_num_primitives_per_element()) + ").");
+    char label = 0;
+    stream.read(&label, 1);
+    const size_t linear = size_t(get_linear_shape() / shape_[0]) * outermost;
+    const size_t num_prims = linear * prims;
+    std::vector<M> host(linear);
+    const char mine = type_label(get_type());
+    if (label == mine) {
+      stream.read(reinterpret_cast<char*>(host.data()), std::streamsize(linear * sizeof(M)));
+    } else if (label == 'd' && mine == 'f') {
+      std::vector<double> src(num_prims);
+      stream.read(reinterpret_cast<char*>(src.data()), std::streamsize(num_prims * sizeof(double)));
+      float* dst = reinterpret_cast<float*>(host.data());
+      for (size_t i = 0; i < num_prims; ++i) dst[i] = static_cast<float>(src[i]);
+    } else {
+      throw std::runtime_error(std::string("Data type mismatch when reading ") + filename);
+    }
+    if (!stream) throw std::runtime_error(std::string("Truncated alu file: ") + filename);
+    set_bytes(host.data(), U(linear * sizeof(M)));
+    return outermost;
+  }
+
+  static char type_label(NumericType t) {
+    switch (t) {
+      case NumericType::f32: return 'f';
+      case NumericType::f64: return 'd';
+      case NumericType::i32: return 'i';
+      case NumericType::u32: return 'u';
+      default: return '!';
+    }
+  }
+
   std::array<U, D> shape_;
   void* ptr_;
 };
@@ -252,6 +368,24 @@ class Store {
     }
     var.ptr_ = nullptr;
   }
+  // Store::get_alu_info (store.cu:47-66): an .alu file's shape (outermost
+  // first), primitives per element and type label, from its header.
+  static std::tuple<std::vector<U>, U, char> get_alu_info(const char* filename) {
+    std::tuple<std::vector<U>, U, char> info;
+    std::ifstream stream(filename, std::ios::binary);
+    if (!stream.is_open()) throw std::runtime_error(std::string("Failed to open alu file: ") + filename);
+    for (;;) {
+      U item = 0;
+      stream.read(reinterpret_cast<char*>(&item), sizeof(U));
+      if (!stream) throw std::runtime_error(std::string("Truncated alu header: ") + filename);
+      if (item == 0) break;
+      std::get<0>(info).push_back(item);
+    }
+    stream.read(reinterpret_cast<char*>(&std::get<1>(info)), sizeof(U));
+    stream.read(&std::get<2>(info), 1);
+    if (!stream) throw std::runtime_error(std::string("Truncated alu header: ") + filename);
+    return info;
+  }
   Const<float>& get_cn() { return cn_; }
   ConstiN& get_cni() { return cni_; }
   std::tuple<Const<float>, ConstiN> create_cn() { return {Const<float>(), ConstiN()}; }
@@ -374,6 +508,72 @@ class SphereDistance : public Distance<TF3, TF> {
 
 enum class VolumeMethod { volume_map = 0, pellets = 1 };
 
+// Mesh (mesh.hpp:13-42), the part rigid contacts use: the vertex list of a
+// collision mesh and the vertex generators of mesh.cu (set_box :206-276,
+// set_uv_sphere :11-63, translate / rotate / scale :278-294).  Faces,
+// normals and texture coordinates serve the reference's display only.
+struct Mesh {
+  std::vector<float3> vertices;
+  void clear() { vertices.clear(); }
+  // n x n lattice on each face of a box of the given widths, faces in the
+  // order +z, -z, +x, -x, -y, +y
+  void set_box(float3 widths, U n = 2) {
+    clear();
+    const float step = 1.0f / float(n - 1);
+    for (U face = 0; face < 6; ++face)
+      for (U j = 0; j < n; ++j)
+        for (U k = 0; k < n; ++k) {
+          const float a = -0.5f + step * float(k), b = -0.5f + step * float(j);
+          float3 c;
+          switch (face) {
+            case 0: c = float3{a, b, 0.5f}; break;
+            case 1: c = float3{a, b, -0.5f}; break;
+            case 2: c = float3{0.5f, a, b}; break;
+            case 3: c = float3{-0.5f, a, b}; break;
+            case 4: c = float3{a, -0.5f, b}; break;
+            default: c = float3{a, 0.5f, b}; break;
+          }
+          vertices.push_back(float3{c.x * widths.x, c.y * widths.y, c.z * widths.z});
+        }
+  }
+  // (num_stacks + 1) rings from the +z pole to the -z pole, each with
+  // num_sectors + 1 vertices (the first repeated at the end)
+  void set_uv_sphere(float radius, U num_sectors, U num_stacks) {
+    clear();
+    const float pi = 3.14159265358979323846f;
+    const float sector_step = 2 * pi / float(num_sectors);
+    const float stack_step = pi / float(num_stacks);
+    for (U i = 0; i <= num_stacks; ++i) {
+      const float stack_angle = pi / 2 - float(i) * stack_step;
+      const float xy = radius * std::cos(stack_angle), z = radius * std::sin(stack_angle);
+      for (U j = 0; j <= num_sectors; ++j) {
+        const float sector_angle = float(j) * sector_step;
+        vertices.push_back(float3{xy * std::cos(sector_angle), xy * std::sin(sector_angle), z});
+      }
+    }
+  }
+  void translate(float3 dx) {
+    for (float3& v : vertices) v = float3{v.x + dx.x, v.y + dx.y, v.z + dx.z};
+  }
+  void rotate(float4 q) {
+    for (float3& v : vertices) {
+      const float x = (1 - 2 * (q.y * q.y + q.z * q.z)) * v.x + 2 * (q.x * q.y - q.z * q.w) * v.y +
+                      2 * (q.x * q.z + q.y * q.w) * v.z;
+      const float y = 2 * (q.x * q.y + q.z * q.w) * v.x + (1 - 2 * (q.x * q.x + q.z * q.z)) * v.y +
+                      2 * (q.y * q.z - q.x * q.w) * v.z;
+      const float z = 2 * (q.x * q.z - q.y * q.w) * v.x + 2 * (q.y * q.z + q.x * q.w) * v.y +
+                      (1 - 2 * (q.x * q.x + q.y * q.y)) * v.z;
+      v = float3{x, y, z};
+    }
+  }
+  void scale(float s) {
+    for (float3& v : vertices) v = float3{v.x * s, v.y * s, v.z * s};
+  }
+};
+
+// Contact<float3, float> (contact.hpp:7-24)
+using Contact = alv_contact;
+
 // Pile<float> (pile.hpp:20-1066): boundaries with volume maps and rigid
 // kinematics.  Host kinematics are exposed through get/set helpers.
 template <typename TF>
@@ -386,6 +586,7 @@ class Pile {
     (void)runner;
     (void)max_num_pellets;
     check(alv_pile_create(&pile_, max_num_contacts, static_cast<int>(volume_method)));
+    max_num_contacts_ = max_num_contacts;
     cni_->max_num_contacts = max_num_contacts;
     if (cn)
       store.copy_cn_external(*cn_, *cni_);
@@ -397,9 +598,11 @@ class Pile {
   Pile& operator=(Pile const&) = delete;
 
   U add(dg::Distance<float3, TF>* distance, U3 resolution = U3{0, 0, 0}, TF cell_width = 0,
-        TF margin = std::numeric_limits<TF>::infinity(), TF sign = 1, TF mass = 0,
-        TF restitution = 1, TF friction = 0, float3 inertia_tensor = float3{1, 1, 1},
-        float3 x = float3{0, 0, 0}, float4 q = float4{0, 0, 0, 1}) {
+        TF margin = std::numeric_limits<TF>::infinity(), TF sign = 1,
+        Mesh const& collision_mesh = Mesh(), TF mass = 0, TF restitution = 1, TF friction = 0,
+        float3 inertia_tensor = float3{1, 1, 1}, float3 x = float3{0, 0, 0},
+        float4 q = float4{0, 0, 0, 1}, Mesh const& display_mesh = Mesh()) {
+    (void)display_mesh;
     distances_.emplace_back(distance);
     alv_distance d = distance->to_c();
     alv_const cn = cn_->to_c();
@@ -410,6 +613,10 @@ class Pile {
     unsigned id = 0;
     check(alv_pile_add(pile_, &cn, &d, res, cell_width, margin, sign, mass, restitution,
                        friction, inertia, xx, qq, &id));
+    if (!collision_mesh.vertices.empty())
+      check(alv_pile_set_collision_vertices(pile_, id,
+                                            reinterpret_cast<const float*>(collision_mesh.vertices.data()),
+                                            U(collision_mesh.vertices.size())));
     mass_.push_back(mass);
     restitution_.push_back(restitution);
     friction_.push_back(friction);
@@ -437,6 +644,68 @@ class Pile {
   void set_gravity(float3 g) { check(alv_pile_set_gravity(pile_, &g.x)); }
   void integrate_kinematics(TF dt) { check(alv_pile_integrate_kinematics(pile_, dt)); }
   void copy_kinematics_to_device() {}
+
+  // Rigid contacts (pile.hpp:826-979).  find_contacts() over every pair is
+  // the overload the solvers call after integrating the bodies.
+  U find_contacts(U i, U j) { return find(int(i), int(j)); }
+  U find_contacts(U i) { return find(int(i), -1); }
+  U find_contacts() { return find(-1, -1); }
+  void solve_contacts() { check(alv_pile_solve_contacts(pile_)); }
+  std::vector<Contact> get_contacts() const {
+    std::vector<Contact> out(max_num_contacts_);
+    unsigned n = 0;
+    check(alv_pile_get_contacts(pile_, out.data(), U(out.size()), &n));
+    out.resize(n);
+    return out;
+  }
+  void set_collision_vertices(U i, std::vector<float3> const& vertices) {
+    check(alv_pile_set_collision_vertices(pile_, i, reinterpret_cast<const float*>(vertices.data()),
+                                          U(vertices.size())));
+  }
+
+  // Rigid kinematics files (pile.hpp:757-791): per body x*x_scale,
+  // v*v_scale (float3 each), q (float4), omega*omega_scale (float3), 52
+  // bytes a body, no header.
+  static constexpr size_t kKinRecord = sizeof(float) * 13;
+  void write_file(const char* filename, TF x_scale = 1, TF v_scale = 1, TF omega_scale = 1) const {
+    std::ofstream stream(filename, std::ios::binary | std::ios::trunc);
+    if (!stream.is_open()) throw std::runtime_error(std::string("Failed to open file: ") + filename);
+    for (U i = 0; i < get_size(); ++i) {
+      const Kin k = kin(i);
+      const float rec[13] = {k.x.x * x_scale, k.x.y * x_scale, k.x.z * x_scale,
+                             k.v.x * v_scale, k.v.y * v_scale, k.v.z * v_scale,
+                             k.q.x, k.q.y, k.q.z, k.q.w,
+                             k.omega.x * omega_scale, k.omega.y * omega_scale,
+                             k.omega.z * omega_scale};
+      stream.write(reinterpret_cast<const char*>(rec), kKinRecord);
+    }
+    if (!stream) throw std::runtime_error(std::string("Failed to write file: ") + filename);
+  }
+  // Reads up to `num_rigids` records (all bodies when negative), skipping
+  // `src_offset` records of the file, into bodies dst_offset, dst_offset+1,
+  // ...; stops at the end of the file.  Returns the number of bodies set.
+  U read_file(const char* filename, int num_rigids = -1, U dst_offset = 0, U src_offset = 0) {
+    std::ifstream stream(filename, std::ios::binary);
+    if (!stream.is_open()) throw std::runtime_error(std::string("Failed to open file: ") + filename);
+    const U size = get_size();
+    U count = num_rigids >= 0 ? std::min(size, U(num_rigids)) : size;
+    if (dst_offset >= size) return 0;
+    count = std::min(count, size - dst_offset);
+    stream.seekg(std::streamoff(kKinRecord) * src_offset, std::ios_base::beg);
+    U done = 0;
+    for (; done < count; ++done) {
+      float rec[13];
+      stream.read(reinterpret_cast<char*>(rec), kKinRecord);
+      if (!stream) break;
+      check(alv_pile_set_kinematics(pile_, dst_offset + done, rec, rec + 3, rec + 6, rec + 10));
+    }
+    return done;
+  }
+  static U get_size_from_file(const char* filename) {
+    std::ifstream stream(filename, std::ios::binary | std::ios::ate);
+    if (!stream.is_open()) throw std::runtime_error(std::string("Failed to open file: ") + filename);
+    return U(size_t(stream.tellg()) / kKinRecord);
+  }
   Variable<1, float> get_volume_grid(U i) const {
     float* p = nullptr;
     unsigned n = 0;
@@ -464,6 +733,11 @@ class Pile {
     float3 x, v, omega, force, torque;
     float4 q;
   };
+  U find(int i, int j) {
+    unsigned n = 0;
+    check(alv_pile_find_contacts(pile_, i, j, cn_->contact_tolerance, &n));
+    return n;
+  }
   Kin kin(U i) const {
     Kin k;
     check(alv_pile_get_kinematics(pile_, i, &k.x.x, &k.v.x, &k.q.x, &k.omega.x, &k.force.x,
@@ -474,6 +748,7 @@ class Pile {
   Const<TF>* cn_;
   ConstiN* cni_;
   alv_pile* pile_ = nullptr;
+  U max_num_contacts_ = 0;
   std::vector<std::unique_ptr<dg::Distance<float3, TF>>> distances_;
 };
 
@@ -644,6 +919,7 @@ class SolverBase {
   void compute_all_boundaries() {}
   // step execution as replayed HIP graphs (no counterpart in the reference)
   void set_graph(bool enable, U slack = 0) { check(alv_solver_set_graph(solver_, enable, slack)); }
+  void set_tiles(bool enable) { check(alv_solver_set_tiles(solver_, enable)); }
   // learned warm start (no counterpart in the reference); null = plain
   void set_warm_start_net(WarmStartNet const* net, bool divergence = true, bool density = true) {
     check(alv_solver_set_warm_start_net(solver_, net ? net->handle() : nullptr, divergence,
@@ -791,6 +1067,25 @@ class SolverIi : public SolverBase<TF> {
     this->push();
   }
   Variable<1, float> particle_pressure() { return this->template field1<float>(ALV_FIELD_PRESSURE); }
+
+ protected:
+  SolverIi(int kind, Runner<TF>& runner, Pile<TF>& pile, Store& store, U max_num_particles,
+           U num_ushers, bool enable_surface_tension, bool enable_vorticity,
+           Const<TF> const* cn, ConstiN const* cni, bool graphical)
+      : SolverBase<TF>(kind, runner, pile, store, max_num_particles, num_ushers,
+                       enable_surface_tension, enable_vorticity, cn, cni, graphical) {}
+};
+
+// SolverI<float> (solver_i.hpp:8-353): pressure-Laplacian ISPH with a
+// weighted Jacobi solve; the same settings and report as SolverIi.
+template <typename TF>
+class SolverI : public SolverIi<TF> {
+ public:
+  SolverI(Runner<TF>& runner, Pile<TF>& pile, Store& store, U max_num_particles,
+          U num_ushers = 0, bool enable_surface_tension = false, bool enable_vorticity = false,
+          Const<TF> const* cn = nullptr, ConstiN const* cni = nullptr, bool graphical = false)
+      : SolverIi<TF>(ALV_SOLVER_I, runner, pile, store, max_num_particles, num_ushers,
+                     enable_surface_tension, enable_vorticity, cn, cni, graphical) {}
 };
 
 }  // namespace alluvion

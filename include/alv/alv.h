@@ -135,6 +135,39 @@ int alv_pile_get_kinematics(const alv_pile* pile, unsigned b, float x[3],
 int alv_pile_set_gravity(alv_pile* pile, const float g[3]);
 /* Pile::integrate_kinematics (pile.hpp:794-815) */
 int alv_pile_integrate_kinematics(alv_pile* pile, float dt);
+
+/* ---- rigid contacts (contact.hpp:7-24, pile.hpp:826-979) ------------------ */
+/* Contact record; the layout of the reference's Contact<float3, float>. */
+typedef struct alv_contact {
+  unsigned i, j;
+  float cp_i[3], cp_j[3];
+  float n[3], t[3];
+  float iiwi_diag[3], iiwi_off_diag[3];
+  float iiwj_diag[3], iiwj_off_diag[3];
+  float friction, nkninv, pmax, goalu, impulse_sum;
+} alv_contact;
+/* Collision vertices of body b in its own frame (the vertices of the
+ * reference's collision_mesh argument of Pile::add, pile.hpp:322-331):
+ * `xyz` is a host array of 3*n floats; n = 0 removes them. */
+int alv_pile_set_collision_vertices(alv_pile* pile, unsigned b, const float* xyz,
+                                    unsigned n);
+/* Pile::find_contacts (pile.hpp:826-904).  i < 0: every pair (i, j), the
+ * overload the DFSPH step calls; i >= 0, j < 0: body i against all others;
+ * both >= 0: that pair.  Pairs with i == j or two bodies of zero mass are
+ * skipped, as are pairs whose body j is a box shell (the reference's
+ * collision_test has no box-shell variant, runner.hpp:5891-5977).
+ * `contact_tolerance` is Const::contact_tolerance.  Contacts are stored in
+ * (i, j, vertex) order; *num_contacts is the number found, which may exceed
+ * max_num_contacts (only that many are kept). */
+int alv_pile_find_contacts(alv_pile* pile, int i, int j, float contact_tolerance,
+                           unsigned* num_contacts);
+/* Pile::solve_contacts (pile.hpp:906-979): five sequential impulse sweeps
+ * over the stored contacts, changing the host v and omega of the bodies. */
+int alv_pile_solve_contacts(alv_pile* pile);
+/* Copies up to `capacity` stored contacts (as left by the last find or
+ * solve, impulse sums included) to host memory; *n = the number copied. */
+int alv_pile_get_contacts(const alv_pile* pile, alv_contact* out, unsigned capacity,
+                          unsigned* n);
 /* Volume map of boundary b (device pointer) and its grid geometry. */
 int alv_pile_get_volume_grid(const alv_pile* pile, unsigned b, float** nodes,
                              unsigned* num_nodes);
@@ -147,6 +180,7 @@ int alv_pile_get_grid_info(const alv_pile* pile, unsigned b,
 typedef struct alv_solver alv_solver;
 #define ALV_SOLVER_DF 0
 #define ALV_SOLVER_II 1
+#define ALV_SOLVER_I 2 /* SolverI: pressure-Laplacian ISPH (solver_i.hpp:8-353) */
 
 /* Public fields of Solver / SolverDf / SolverIi (solver.hpp:261-272,
  * solver_df.hpp:372-379, solver_ii.hpp:259-262). */
@@ -183,8 +217,9 @@ typedef struct alv_solver_report {
   float dt;
 } alv_solver_report;
 
-/* SolverDf / SolverIi constructors (solver.hpp:19-94, solver_df.hpp:48-67,
- * solver_ii.hpp:40-65).  The pile must outlive the solver. */
+/* SolverDf / SolverIi / SolverI constructors (solver.hpp:19-94,
+ * solver_df.hpp:48-67, solver_ii.hpp:40-65, solver_i.hpp:48-66).  The pile
+ * must outlive the solver. */
 int alv_solver_create(alv_solver** out, int kind, alv_pile* pile,
                       unsigned max_num_particles, const alv_const* cn,
                       const alv_consti* cni);
@@ -262,6 +297,12 @@ int alv_solver_step(alv_solver* solver);
  * host adds more only when a scene has not converged.  Results are identical
  * with and without graphs.  Per-kernel profiling uses direct launches. */
 int alv_solver_set_graph(alv_solver* solver, int enable, unsigned slack);
+/* Neighbour passes that copy each 256-particle block's neighbourhood (three
+ * contiguous ranges of sorted particles) into LDS before their loops (no
+ * counterpart in the reference; on by default).  Blocks whose neighbourhood
+ * does not fit, and periodic grids, read neighbours from global memory.
+ * Results are identical either way. */
+int alv_solver_set_tiles(alv_solver* solver, int enable);
 /* Solver::compute_all_boundaries (solver.hpp:170-185) followed by
  * Solver::update_particle_neighbors<wrap> (solver.hpp:225-260). */
 int alv_solver_update_particle_neighbors(alv_solver* solver);

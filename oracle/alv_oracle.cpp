@@ -613,6 +613,17 @@ struct Boundary {
   V3<T> inertia;
   V3<T> x, v, omega, force, torque, a;
   V4<T> q;
+  // rigid contacts (Pile::add collision_mesh, restitution, friction)
+  T restitution = 1, friction = 0;
+  std::vector<V3<T>> coll;  // collision vertices in the body frame
+};
+
+// Contact<float3, float> (contact.hpp:7-24)
+template <class T>
+struct Contact {
+  unsigned i, j;
+  V3<T> cp_i, cp_j, n, t, iiwi_diag, iiwi_off_diag, iiwj_diag, iiwj_off_diag;
+  T friction, nkninv, pmax, goalu, impulse_sum;
 };
 
 // ---------------------------------------------------------------------------
@@ -620,7 +631,7 @@ struct Boundary {
 // ---------------------------------------------------------------------------
 template <class T>
 struct Solver {
-  int kind;  // 0 DF, 1 II
+  int kind;  // 0 DF, 1 II, 2 ISPH (SolverI)
   unsigned max_n, n;
   Consts<T> c;
   orc_settings set;
@@ -628,6 +639,10 @@ struct Solver {
   T t = 0, dt = 0;
   std::vector<Boundary<T>> bnd;
   V3<T> pile_gravity{0, 0, 0};
+  T contact_tolerance = 0;
+  unsigned max_num_contacts = 0;
+  unsigned num_found = 0;
+  std::vector<Contact<T>> contacts;
 
   std::vector<V3<T>> x, v, a;
   std::vector<T> density, dfsph_factor, kappa, kappa_v, density_adv, cfl_v2;
@@ -1061,6 +1076,134 @@ struct Solver {
       B.torque = {T(ts[0]), T(ts[1]), T(ts[2])};
     }
     for (unsigned b = 0; b < bnd.size(); ++b) integrate_kinematics(bnd[b]);
+    // Pile::find_contacts + solve_contacts (solver_df.hpp:356-357)
+    bool any = false;
+    for (const auto& B : bnd) any |= !B.coll.empty();
+    if (any) {
+      find_contacts(-1, -1);
+      solve_contacts();
+    }
+  }
+
+  // ---- rigid contacts (runner.hpp:3833-3914, pile.hpp:826-979) -------------
+  // R diag(v) R^T stored as diagonal + (01, 02, 12) (runner.hpp:178-199)
+  static void congruent(V3<T> v, V4<T> q, V3<T>& d, V3<T>& o) {
+    T m00 = 1 - 2 * (q.y * q.y + q.z * q.z), m01 = 2 * (q.x * q.y - q.z * q.w),
+      m02 = 2 * (q.x * q.z + q.y * q.w);
+    T m10 = 2 * (q.x * q.y + q.z * q.w), m11 = 1 - 2 * (q.x * q.x + q.z * q.z),
+      m12 = 2 * (q.y * q.z - q.x * q.w);
+    T m20 = 2 * (q.x * q.z - q.y * q.w), m21 = 2 * (q.y * q.z + q.x * q.w),
+      m22 = 1 - 2 * (q.x * q.x + q.y * q.y);
+    d = {v.x * m00 * m00 + v.y * m01 * m01 + v.z * m02 * m02,
+         v.x * m10 * m10 + v.y * m11 * m11 + v.z * m12 * m12,
+         v.x * m20 * m20 + v.y * m21 * m21 + v.z * m22 * m22};
+    o = {v.x * m00 * m10 + v.y * m01 * m11 + v.z * m02 * m12,
+         v.x * m00 * m20 + v.y * m01 * m21 + v.z * m02 * m22,
+         v.x * m10 * m20 + v.y * m11 * m21 + v.z * m12 * m22};
+  }
+  // K = I/m - [r]x A [r]x for symmetric A = (d, o) (runner.hpp:265-288)
+  static void contact_k(V3<T> r, T mass, V3<T> d, V3<T> o, V3<T>& kd, V3<T>& ko) {
+    kd = {0, 0, 0};
+    ko = {0, 0, 0};
+    if (mass == 0) return;
+    T im = 1 / mass;
+    kd = {r.z * r.z * d.y - r.y * r.z * o.z * 2 + r.y * r.y * d.z + im,
+          r.z * r.z * d.x - r.x * r.z * o.y * 2 + r.x * r.x * d.z + im,
+          r.y * r.y * d.x - r.x * r.y * o.x * 2 + r.x * r.x * d.y + im};
+    ko = {-r.z * r.z * o.x + r.x * r.z * o.z + r.y * r.z * o.y - r.x * r.y * d.z,
+          r.y * r.z * o.x - r.x * r.z * d.y - r.y * r.y * o.y + r.x * r.y * o.z,
+          -(r.y * r.z * d.x) + r.x * r.z * o.x + r.x * r.y * o.y - r.x * r.x * o.z};
+  }
+  static V3<T> sym_apply(V3<T> v, V3<T> d, V3<T> o) {
+    return {v.x * d.x + v.y * o.x + v.z * o.y, v.x * o.x + v.y * d.y + v.z * o.z,
+            v.x * o.y + v.y * o.z + v.z * d.z};
+  }
+  // collision_test (analytic distance): vertices of body i against body j
+  void collision_test(unsigned i, unsigned j) {
+    const Boundary<T>& Bi = bnd[i];
+    const Boundary<T>& Bj = bnd[j];
+    for (const V3<T>& vert : Bi.coll) {
+      Contact<T> C{};
+      C.cp_i = rotate(vert, Bi.q) + Bi.x;
+      V3<T> local = rotate(C.cp_i - Bj.x, conjugate(Bj.q));
+      T dist = Bj.sdf.signed_distance(local) * Bj.sign - contact_tolerance;
+      V3<T> nn = Bj.sdf.gradient(local, static_cast<T>(1e-3)) * Bj.sign;
+      nn = nn * (1 / std::sqrt(length_sqr(nn)));
+      if (!(dist < 0)) continue;
+      ++num_found;
+      C.i = i;
+      C.j = j;
+      C.cp_j = rotate(local - dist * nn, Bj.q) + Bj.x;
+      C.n = rotate(nn, Bj.q);
+      C.friction = Bi.friction + Bj.friction;
+      V3<T> r_i = C.cp_i - Bi.x, r_j = C.cp_j - Bj.x;
+      V3<T> u_rel = (Bi.v + cross(Bi.omega, r_i)) - (Bj.v + cross(Bj.omega, r_j));
+      T u_rel_n = dot(C.n, u_rel);
+      C.t = u_rel - u_rel_n * C.n;
+      if (length_sqr(C.t) > static_cast<T>(1e-6)) C.t = C.t * (1 / std::sqrt(length_sqr(C.t)));
+      congruent({1 / Bi.inertia.x, 1 / Bi.inertia.y, 1 / Bi.inertia.z}, Bi.q, C.iiwi_diag,
+                C.iiwi_off_diag);
+      congruent({1 / Bj.inertia.x, 1 / Bj.inertia.y, 1 / Bj.inertia.z}, Bj.q, C.iiwj_diag,
+                C.iiwj_off_diag);
+      V3<T> kdi, koi, kdj, koj;
+      contact_k(r_i, Bi.mass, C.iiwi_diag, C.iiwi_off_diag, kdi, koi);
+      contact_k(r_j, Bj.mass, C.iiwj_diag, C.iiwj_off_diag, kdj, koj);
+      V3<T> kd = kdi + kdj, ko = koi + koj;
+      C.nkninv = 1 / dot(C.n, sym_apply(C.n, kd, ko));
+      C.pmax = 1 / dot(C.t, sym_apply(C.t, kd, ko)) * dot(u_rel, C.t);
+      C.goalu = u_rel_n < 0 ? -(Bi.restitution * Bj.restitution) * u_rel_n : T(0);
+      C.impulse_sum = 0;
+      if (contacts.size() < max_num_contacts) contacts.push_back(C);
+    }
+  }
+  // i < 0: all pairs; j < 0: body i against all others (skipping i == j and
+  // pairs of two massless bodies); box-shell bodies j have no collision test
+  void find_contacts(int i, int j) {
+    num_found = 0;
+    contacts.clear();
+    unsigned nb = unsigned(bnd.size());
+    unsigned i0 = i < 0 ? 0 : unsigned(i), i1 = i < 0 ? nb : unsigned(i) + 1;
+    for (unsigned a = i0; a < i1; ++a) {
+      unsigned j0 = j < 0 ? 0 : unsigned(j), j1 = j < 0 ? nb : unsigned(j) + 1;
+      for (unsigned b = j0; b < j1; ++b) {
+        if (j < 0 && (a == b || (bnd[a].mass == 0 && bnd[b].mass == 0))) continue;
+        if (bnd[b].sdf.type == ORC_BOX_SHELL) continue;
+        collision_test(a, b);
+      }
+    }
+  }
+  // solve_contacts (pile.hpp:906-979)
+  void solve_contacts() {
+    for (int sweep = 0; sweep < 5; ++sweep) {
+      for (Contact<T>& C : contacts) {
+        Boundary<T>& Bi = bnd[C.i];
+        Boundary<T>& Bj = bnd[C.j];
+        T depth = dot(C.n, C.cp_i - C.cp_j);
+        V3<T> r_i = C.cp_i - Bi.x, r_j = C.cp_j - Bj.x;
+        V3<T> u_rel = (Bi.v + cross(Bi.omega, r_i)) - (Bj.v + cross(Bj.omega, r_j));
+        T u_rel_n = dot(C.n, u_rel);
+        T corr = C.nkninv * (C.goalu - u_rel_n);
+        if (corr < -C.impulse_sum) corr = -C.impulse_sum;
+        if (depth < 0) corr -= C.nkninv * depth;
+        V3<T> p = corr * C.n;
+        C.impulse_sum += corr;
+        T pn = dot(p, C.n);
+        if (C.friction * pn > C.pmax)
+          p = p - C.pmax * C.t;
+        else if (C.friction * pn < -C.pmax)
+          p = p + C.pmax * C.t;
+        else
+          p = p - (C.friction * pn) * C.t;
+        if (Bi.mass != 0) {
+          Bi.v += p * (1 / Bi.mass);
+          Bi.omega += sym_apply(cross(r_i, p), C.iiwi_diag, C.iiwi_off_diag);
+        }
+        if (Bj.mass != 0) {
+          Bj.v += (-p) * (1 / Bj.mass);
+          Bj.omega += sym_apply(cross(r_j, -p), C.iiwj_diag, C.iiwj_off_diag);
+        }
+      }
+    }
   }
 
   static V4<T> dq_of(V3<T> w, V4<T> q) {
@@ -1365,6 +1508,13 @@ struct Solver {
     }
     rep.num_density_solve = it;
     rep.mean_density_error = mean;
+    pressure_accels_and_integrate();
+  }
+
+  // compute_pressure_accels, kinematic_integration, rigid sums: the end of
+  // SolverIi::step and SolverI::step (solver_ii.hpp:208-241,
+  // solver_i.hpp:289-327)
+  void pressure_accels_and_integrate() {
     // compute_pressure_accels (runner.hpp:2774-2817)
 #pragma omp parallel for schedule(static)
     for (long ii = 0; ii < long(n); ++ii) {
@@ -1399,6 +1549,90 @@ struct Solver {
     }
     rigid_sums_and_integrate();
     t += dt;
+  }
+
+  // ---- SolverI::step (solver_i.hpp:79-330), volume-map boundaries ------------
+  void step_i() {
+    clear_force();
+    for (unsigned i = 0; i < n; ++i) a[i] = c.g;
+    compute_all_boundaries();
+    update_particle_neighbors();
+    compute_viscosity();
+    update_dt();
+    // advect_and_init_pressure (runner.hpp:2890-2900)
+    for (unsigned i = 0; i < n; ++i) {
+      v[i] += dt * a[i];
+      last_pressure[i] = pressure[i] * static_cast<T>(0.5);
+    }
+    const T eps = static_cast<T>(0.01) * c.h2;
+    // calculate_isph_diagonal_adv_density (runner.hpp:2901-2949); the
+    // diagonal times rho0 is kept in aii
+#pragma omp parallel for schedule(static)
+    for (long ii = 0; ii < long(n); ++ii) {
+      unsigned i = unsigned(ii);
+      V3<T> xi = x[i], vi = v[i];
+      T dens = density[i];
+      T dadv = dens / c.rho0;
+      T diag = 0;
+      for (unsigned s = 0; s < num_neighbors[i]; ++s) {
+        size_t o = size_t(i) * c.kmax + s;
+        unsigned j = nb_id[o];
+        V3<T> xixj = nb_x[o];
+        V3<T> grad_w = displacement_cubic_kernel_grad(xixj, c);
+        T dj = density[j];
+        dadv += dt * c.vol * dot(vi - v[j], grad_w);
+        diag += c.vol * (dens + dj) / (dens * dj) / (length_sqr(xixj) + eps) * dot(xixj, grad_w);
+      }
+      for (unsigned b = 0; b < bnd.size(); ++b) {
+        V4<T> r = brec(b, i), k = bker(b, i);
+        V3<T> g{k.x, k.y, k.z};
+        V3<T> bx{r.x, r.y, r.z};
+        V3<T> xib = xi - bx;
+        dadv += dt * dot(vi - rigid_vel(b, bx), g);
+        diag += (dens + c.rho0) / (dens * c.rho0) / (length_sqr(xib) + eps) * dot(xib, g);
+      }
+      aii[i] = diag * c.rho0;
+      adv_density[i] = dadv;
+    }
+    T mean = std::numeric_limits<T>::max();
+    unsigned it = 0;
+    while ((mean > static_cast<T>(set.density_error_tolerance) ||
+            it < set.min_density_solve) &&
+           it < set.max_density_solve) {
+      // isph_solve_iteration (runner.hpp:2996-3042), weighted Jacobi
+#pragma omp parallel for schedule(static)
+      for (long ii = 0; ii < long(n); ++ii) {
+        unsigned i = unsigned(ii);
+        T dens = density[i];
+        T diag = aii[i];
+        T bb = 1 - adv_density[i];
+        T dt2 = dt * dt;
+        T denom = diag * dt2;
+        const T w = static_cast<T>(2) / 3;
+        T off = 0;
+        for (unsigned s = 0; s < num_neighbors[i]; ++s) {
+          size_t o = size_t(i) * c.kmax + s;
+          unsigned j = nb_id[o];
+          V3<T> xixj = nb_x[o];
+          T dj = density[j];
+          off -= c.vol * (dens + dj) / (dens * dj) / (length_sqr(xixj) + eps) *
+                 dot(xixj, displacement_cubic_kernel_grad(xixj, c)) * last_pressure[j];
+        }
+        off *= c.rho0;
+        T p = 0;
+        if (std::fabs(denom) > static_cast<T>(1.0e-9))
+          p = std::max((1 - w) * last_pressure[i] + w * (bb / denom - off / diag),
+                       static_cast<T>(0));
+        density_err[i] = p == 0 ? 0 : (denom * p + off * dt2 - bb);
+        pressure[i] = p;
+      }
+      for (unsigned i = 0; i < n; ++i) last_pressure[i] = pressure[i];
+      mean = static_cast<T>(sum_of(density_err) / double(n));
+      ++it;
+    }
+    rep.num_density_solve = it;
+    rep.mean_density_error = mean;
+    pressure_accels_and_integrate();
   }
 
   // ---- volume map construction (runner.hpp:1660-1694) -------------------------
@@ -1655,6 +1889,72 @@ int orc_set_rigid(void* h, unsigned b, const double* x, const double* v,
   });
 }
 
+int orc_set_collision(void* h, unsigned b, double restitution, double friction,
+                      const double* xyz, unsigned n) {
+  return dispatch(h, [&](auto& s) {
+    using T = typename std::remove_reference<decltype(s.dt)>::type;
+    if (b >= s.bnd.size()) return -3;
+    auto& B = s.bnd[b];
+    B.restitution = T(restitution);
+    B.friction = T(friction);
+    B.coll.resize(n);
+    for (unsigned k = 0; k < n; ++k)
+      B.coll[k] = {T(xyz[3 * k]), T(xyz[3 * k + 1]), T(xyz[3 * k + 2])};
+    return 0;
+  });
+}
+
+int orc_set_contact_params(void* h, double contact_tolerance, unsigned max_num_contacts) {
+  return dispatch(h, [&](auto& s) {
+    using T = typename std::remove_reference<decltype(s.dt)>::type;
+    s.contact_tolerance = T(contact_tolerance);
+    s.max_num_contacts = max_num_contacts;
+    return 0;
+  });
+}
+
+int orc_find_contacts(void* h, int i, int j, unsigned* num_found) {
+  return dispatch(h, [&](auto& s) {
+    if (i >= int(s.bnd.size()) || j >= int(s.bnd.size()) || (i < 0 && j >= 0)) return -3;
+    s.find_contacts(i, j);
+    if (num_found) *num_found = s.num_found;
+    return 0;
+  });
+}
+
+int orc_solve_contacts(void* h) {
+  return dispatch(h, [&](auto& s) {
+    s.solve_contacts();
+    return 0;
+  });
+}
+
+int orc_get_contacts(void* h, double* out, unsigned capacity, unsigned* n) {
+  return dispatch(h, [&](auto& s) {
+    unsigned k = std::min(capacity, unsigned(s.contacts.size()));
+    for (unsigned c = 0; c < k; ++c) {
+      const auto& C = s.contacts[c];
+      double* o = out + size_t(c) * ORC_CONTACT_FIELDS;
+      o[0] = C.i;
+      o[1] = C.j;
+      const decltype(C.n)* vs[8] = {&C.cp_i, &C.cp_j, &C.n, &C.t, &C.iiwi_diag,
+                                    &C.iiwi_off_diag, &C.iiwj_diag, &C.iiwj_off_diag};
+      for (int f = 0; f < 8; ++f) {
+        o[2 + 3 * f] = vs[f]->x;
+        o[3 + 3 * f] = vs[f]->y;
+        o[4 + 3 * f] = vs[f]->z;
+      }
+      o[26] = C.friction;
+      o[27] = C.nkninv;
+      o[28] = C.pmax;
+      o[29] = C.goalu;
+      o[30] = C.impulse_sum;
+    }
+    if (n) *n = k;
+    return 0;
+  });
+}
+
 int orc_get_rigid(void* h, unsigned b, double* x, double* v, double* q,
                   double* w, double* f, double* tq) {
   return dispatch(h, [&](auto& s) {
@@ -1768,6 +2068,8 @@ int orc_step(void* h) {
   return dispatch(h, [&](auto& s) {
     if (s.kind == 0)
       s.step_df();
+    else if (s.kind == 2)
+      s.step_i();
     else
       s.step_ii();
     return 0;

@@ -92,7 +92,8 @@ typedef struct {
   double q[4]; /* (x, y, z, w) */
 } orc_boundary_desc;
 
-/* kind: 0 = DFSPH (SolverDf), 1 = IISPH (SolverIi).  precision: 4 or 8. */
+/* kind: 0 = DFSPH (SolverDf), 1 = IISPH (SolverIi), 2 = ISPH (SolverI).
+ * precision: 4 or 8. */
 void* orc_create(int precision, int kind, unsigned max_num_particles,
                  const orc_params* params);
 void orc_destroy(void* h);
@@ -116,6 +117,23 @@ int orc_set_rigid(void* h, unsigned b, const double* x, const double* v,
 int orc_get_rigid(void* h, unsigned b, double* x, double* v, double* q,
                   double* omega, double* force, double* torque);
 int orc_set_gravity_pile(void* h, const double* g);
+
+/* Rigid contacts (contact.hpp:7-24, runner.hpp:3833-3914 analytic variant,
+ * pile.hpp:826-979).  Collision vertices of body b in its frame (3*n), with
+ * the body's restitution and friction; Const::contact_tolerance and
+ * Consti::max_num_contacts.  Contacts are found in (i, j, vertex) order:
+ * i < 0 all pairs, j < 0 body i against the others.  A contact record has
+ * ORC_CONTACT_FIELDS doubles: i, j, cp_i, cp_j, n, t, iiwi_diag,
+ * iiwi_off_diag, iiwj_diag, iiwj_off_diag, friction, nkninv, pmax, goalu,
+ * impulse_sum.  The step finds and solves contacts after integrating the
+ * bodies whenever any body has collision vertices. */
+#define ORC_CONTACT_FIELDS 31
+int orc_set_collision(void* h, unsigned b, double restitution, double friction,
+                      const double* xyz, unsigned n);
+int orc_set_contact_params(void* h, double contact_tolerance, unsigned max_num_contacts);
+int orc_find_contacts(void* h, int i, int j, unsigned* num_found);
+int orc_solve_contacts(void* h);
+int orc_get_contacts(void* h, double* out, unsigned capacity, unsigned* n);
 
 /* Named per-particle arrays in working precision (vectors are packed xyz):
  * x v a density dfsph_factor kappa kappa_v density_adv pressure

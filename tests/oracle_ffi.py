@@ -109,6 +109,12 @@ def lib():
         L.orc_set_rigid.argtypes = [vp, ctypes.c_uint, dp, dp, dp, dp]
         L.orc_get_rigid.argtypes = [vp, ctypes.c_uint, dp, dp, dp, dp, dp, dp]
         L.orc_set_gravity_pile.argtypes = [vp, dp]
+        L.orc_set_collision.argtypes = [vp, ctypes.c_uint, ctypes.c_double, ctypes.c_double, dp,
+                                        ctypes.c_uint]
+        L.orc_set_contact_params.argtypes = [vp, ctypes.c_double, ctypes.c_uint]
+        L.orc_find_contacts.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint)]
+        L.orc_solve_contacts.argtypes = [vp]
+        L.orc_get_contacts.argtypes = [vp, dp, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint)]
         L.orc_set_array.argtypes = [vp, cp, vp]
         L.orc_get_array.argtypes = [vp, cp, vp]
         L.orc_compute_all_boundaries.argtypes = [vp]
@@ -146,6 +152,26 @@ _KIND = {"box": 0, "box_shell": 1, "sphere": 2}
 _VEC_ARRAYS = {"x", "v", "a", "dii", "dij_pj", "pressure_accel"}
 
 
+CONTACT_FIELDS = 31
+CONTACT_VECTORS = ("cp_i", "cp_j", "n", "t", "iiwi_diag", "iiwi_off_diag", "iiwj_diag",
+                   "iiwj_off_diag")
+CONTACT_SCALARS = ("friction", "nkninv", "pmax", "goalu", "impulse_sum")
+
+
+def contact_dicts(rows):
+    """Oracle contact rows [n, 31] as the dicts the product's get_contacts
+    returns."""
+    out = []
+    for r in rows:
+        d = {"i": int(r[0]), "j": int(r[1])}
+        for f, name in enumerate(CONTACT_VECTORS):
+            d[name] = tuple(r[2 + 3 * f:5 + 3 * f])
+        for f, name in enumerate(CONTACT_SCALARS):
+            d[name] = r[26 + f]
+        out.append(d)
+    return out
+
+
 class Oracle:
     """One oracle solver (DFSPH or IISPH) in FP32 or FP64."""
 
@@ -171,7 +197,8 @@ class Oracle:
         p.max_num_neighbors_per_particle = scene.max_num_neighbors
         p.max_num_particles_per_cell = scene.max_num_particles_per_cell
         self.max_n = scene.n if max_particles is None else max_particles
-        self.h = self.L.orc_create(precision, 0 if kind == "df" else 1, self.max_n, ctypes.byref(p))
+        self.h = self.L.orc_create(precision, {"df": 0, "ii": 1, "i": 2}[kind], self.max_n,
+                                   ctypes.byref(p))
         if not self.h:
             raise RuntimeError("orc_create failed")
         self.L.orc_set_num_particles(self.h, scene.n)
@@ -206,6 +233,12 @@ class Oracle:
             self.L.orc_add_boundary(self.h, ctypes.byref(d))
         g = np.array(scene.pile_gravity, dtype=np.float64)
         self.L.orc_set_gravity_pile(self.h, _dptr(g))
+        self.L.orc_set_contact_params(self.h, scene.contact_tolerance, scene.max_num_contacts)
+        for k, b in enumerate(scene.boundaries):
+            verts = np.zeros((0, 3)) if b.collision_vertices is None else b.collision_vertices
+            verts = np.ascontiguousarray(np.asarray(verts, np.float32).reshape(-1, 3), np.float64)
+            self.L.orc_set_collision(self.h, k, b.restitution, b.friction, _dptr(verts),
+                                     verts.shape[0])
         full = np.zeros((self.max_n, 3), self.dtype)
         full[: scene.n] = scene.x
         self.set("x", full)
@@ -292,6 +325,22 @@ class Oracle:
                 args.append(_dptr(arr))
                 setattr(self, "_keep_%d" % len(args), arr)
         self.L.orc_set_rigid(self.h, b, *args)
+
+    def find_contacts(self, i=-1, j=-1):
+        n = ctypes.c_uint(0)
+        rc = self.L.orc_find_contacts(self.h, i, j, ctypes.byref(n))
+        if rc != 0:
+            raise ValueError("orc_find_contacts: %d" % rc)
+        return n.value
+
+    def solve_contacts(self):
+        self.L.orc_solve_contacts(self.h)
+
+    def contacts(self, capacity=100000):
+        out = np.zeros((capacity, CONTACT_FIELDS))
+        n = ctypes.c_uint(0)
+        self.L.orc_get_contacts(self.h, _dptr(out), capacity, ctypes.byref(n))
+        return out[:n.value]
 
     def compute_all_boundaries(self):
         self.L.orc_compute_all_boundaries(self.h)

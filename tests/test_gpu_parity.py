@@ -187,13 +187,14 @@ def test_dfsph_step_parity_dam_break(warm):
     assert abs(solver.mean_density_error - rep["mean_density_error"]) <= 1e-6
 
 
-def test_iisph_step_parity_dam_break():
+@pytest.mark.parametrize("kind", ["ii", "i"])
+def test_iisph_and_isph_step_parity_dam_break(kind):
     sc = scenes.dam_break_small()
-    dp, pile, solver = gpu_scene(sc, "ii")
+    dp, pile, solver = gpu_scene(sc, kind)
     for _ in range(20):
         solver.step()
-    orc = oracle_like(sc, dp, pile, kind="ii")
-    sync_state(dp, solver, orc, kind="ii")
+    orc = oracle_like(sc, dp, pile, kind=kind)
+    sync_state(dp, solver, orc, kind=kind)
     solver.step()
     orc.step()
     n = sc.n
@@ -219,7 +220,7 @@ def periodic_lattice(kind):
     return sc
 
 
-@pytest.mark.parametrize("kind", ["df", "ii"])
+@pytest.mark.parametrize("kind", ["df", "ii", "i"])
 def test_step_parity_periodic(kind):
     sc = periodic_lattice(kind)
     dp, pile, solver = gpu_scene(sc, kind)

@@ -60,7 +60,7 @@ def oracle64(sc, kind="df", xv=None):
     return orc
 
 
-@pytest.mark.parametrize("kind", ["df", "ii"])
+@pytest.mark.parametrize("kind", ["df", "ii", "i"])
 def test_lattice_at_rest(kind):
     S = lattice_sum(2 * R, H)
     mass = 0.99 * 1000.0 / S  # 1 % under rest density
@@ -81,7 +81,7 @@ def test_lattice_at_rest(kind):
         assert np.all(orc.get("pressure")[: sc.n] == 0.0)
 
 
-@pytest.mark.parametrize("kind", ["df", "ii"])
+@pytest.mark.parametrize("kind", ["df", "ii", "i"])
 def test_momentum_conserved_without_gravity_and_viscosity(kind):
     sc, x, v = periodic_lattice(jitter=0.3 * R, vamp=0.05, seed=3)
     orc = oracle64(sc, kind, (x, v))
