@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I", INCLUDE,
 ]
-HIP_SOURCES = ["alv_step.hip", "alv_capi.hip", "alv_mlp.hip", "alv_contact.hip"]
+HIP_SOURCES = ["alv_step.hip", "alv_capi.hip", "alv_mlp.hip", "alv_contact.hip", "alv_sample.hip"]
 LIB = os.path.join(PKG, "libalv.so")
 
 

@@ -270,6 +270,10 @@ EventProfiler* g_event_profiler = nullptr;
 struct alv_pile {
   std::vector<BoundaryDev> bnd;  // kinematics live in bnd[b].x/v/q/omega
   std::vector<float*> volumes;
+  std::vector<unsigned char*> cell_nz;  // non-zero flags of the map cells
+  // set when a caller got a writable pointer to a volume map: the flags are
+  // recomputed before the next step
+  mutable bool maps_dirty = false;
   std::vector<float> restitution, friction;
   std::vector<std::array<float, 3>> inertia, force, torque, accel;
   float gravity[3] = {0.f, 0.f, 0.f};
@@ -291,8 +295,16 @@ struct alv_pile {
   unsigned* count_dev = nullptr;
   std::vector<alv_contact> contacts;  // host copy of the kept contacts
   unsigned num_found = 0;
+  // device kinematics (x, v, q, omega of every body) as of the last copy:
+  // the snapshot is taken on the host at each step start and uploaded when
+  // the device copy is asked for
+  std::vector<float> kin_snap;
+  bool kin_dirty = false;
+  float* kin_dev = nullptr;
+  size_t kin_cap = 0;
   ~alv_pile() {
     for (float* v : volumes) (void)hipFree(v);
+    for (unsigned char* f : cell_nz) (void)hipFree(f);
     if (peirce) (void)hipFree(peirce);
     if (verts_dev) (void)hipFree(verts_dev);
     if (jobs_dev) (void)hipFree(jobs_dev);
@@ -300,6 +312,22 @@ struct alv_pile {
     if (inertia_dev) (void)hipFree(inertia_dev);
     if (contacts_dev) (void)hipFree(contacts_dev);
     if (count_dev) (void)hipFree(count_dev);
+    if (kin_dev) (void)hipFree(kin_dev);
+  }
+  // x [B][3], v [B][3], q [B][4], omega [B][3]
+  void snapshot_kinematics() {
+    const size_t B = bnd.size();
+    kin_snap.assign(B * 13, 0.f);
+    for (size_t b = 0; b < B; ++b)
+      for (int a = 0; a < 4; ++a) {
+        if (a < 3) {
+          kin_snap[b * 3 + a] = bnd[b].x[a];
+          kin_snap[B * 3 + b * 3 + a] = bnd[b].v[a];
+          kin_snap[B * 10 + b * 3 + a] = bnd[b].omega[a];
+        }
+        kin_snap[B * 6 + b * 4 + a] = bnd[b].q[a];
+      }
+    kin_dirty = true;
   }
 };
 
@@ -357,6 +385,7 @@ struct alv_solver {
   };
   Segment seg[3];
   int use_graph = 0;
+  int grid_valid = 0;  // a neighbour search has filled the grid (sampling needs it)
   unsigned slack = 0;  // Jacobi iterations enqueued beyond the last step's count
   unsigned chunk_div = 0, chunk_dens = 0, shrink_div = 0, shrink_dens = 0;
   std::vector<void*> allocations;
@@ -647,9 +676,15 @@ int alv_pile_add(alv_pile* pile, const alv_const* cn, const alv_distance* dist,
     launch_volume_field_analytic(p, B, vol, pile->peirce, b, e, 0);
     ALV_HIP(hipGetLastError());
   }
+  unsigned char* flags = nullptr;
+  ALV_HIP(hipMalloc(&flags, std::max<size_t>(size_t(res[0]) * res[1] * res[2], 1)));
+  launch_map_flags(B.geom, vol, flags, nullptr);
+  ALV_HIP(hipGetLastError());
   ALV_HIP(hipDeviceSynchronize());
+  B.cell_nz = flags;
   pile->bnd.push_back(B);
   pile->volumes.push_back(vol);
+  pile->cell_nz.push_back(flags);
   pile->restitution.push_back(restitution);
   pile->friction.push_back(friction);
   std::array<float, 3> I = {inertia ? inertia[0] : 1.f, inertia ? inertia[1] : 1.f,
@@ -935,6 +970,30 @@ int alv_pile_solve_contacts(alv_pile* pile) {
   return ALV_OK;
 }
 
+int alv_pile_copy_kinematics_to_device(alv_pile* pile) {
+  if (!pile) return fail(ALV_ERR_ARG, "null pile");
+  pile->snapshot_kinematics();
+  float *x, *v, *q, *w;
+  return alv_pile_kinematics_device(pile, &x, &v, &q, &w);
+}
+
+int alv_pile_kinematics_device(alv_pile* pile, float** x, float** v, float** q, float** omega) {
+  if (!pile || !x || !v || !q || !omega) return fail(ALV_ERR_ARG, "null argument");
+  const size_t B = pile->bnd.size();
+  if (pile->kin_snap.size() != B * 13) pile->snapshot_kinematics();
+  ALV_HIP(reserve_dev(&pile->kin_dev, &pile->kin_cap, std::max<size_t>(B * 13, 1)));
+  if (pile->kin_dirty && B) {
+    ALV_HIP(hipMemcpy(pile->kin_dev, pile->kin_snap.data(), B * 13 * sizeof(float),
+                      hipMemcpyHostToDevice));
+    pile->kin_dirty = false;
+  }
+  *x = pile->kin_dev;
+  *v = pile->kin_dev + B * 3;
+  *q = pile->kin_dev + B * 6;
+  *omega = pile->kin_dev + B * 10;
+  return ALV_OK;
+}
+
 int alv_pile_get_contacts(const alv_pile* pile, alv_contact* out, unsigned capacity, unsigned* n) {
   if (!pile || (capacity && !out)) return fail(ALV_ERR_ARG, "null argument");
   const unsigned k = std::min(capacity, unsigned(pile->contacts.size()));
@@ -946,7 +1005,10 @@ int alv_pile_get_contacts(const alv_pile* pile, alv_contact* out, unsigned capac
 int alv_pile_get_volume_grid(const alv_pile* pile, unsigned b, float** nodes,
                              unsigned* num_nodes) {
   if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
-  if (nodes) *nodes = pile->volumes[b];
+  if (nodes) {
+    *nodes = pile->volumes[b];
+    pile->maps_dirty = true;  // the caller may write the nodes
+  }
   if (num_nodes) *num_nodes = pile->bnd[b].geom.num_nodes;
   return ALV_OK;
 }
@@ -1338,12 +1400,21 @@ int solver_launch(alv_solver* S, Launch* L) {
   L->cblocks = S->cells_total / kBlock;
   L->scenes = unsigned(ns);
   L->active = active;
+  L->tiles = S->A.use_tiles;
   L->p0 = S->par_host[0];
   return ALV_OK;
 }
 
 int upload_step_inputs(alv_solver* S) {
   size_t nb = 0;
+  for (const auto& sc : S->scenes) {
+    if (sc.pile->maps_dirty) {
+      for (size_t b = 0; b < sc.pile->bnd.size(); ++b)
+        launch_map_flags(sc.pile->bnd[b].geom, sc.pile->volumes[b], sc.pile->cell_nz[b], S->stream);
+      ALV_HIP(hipGetLastError());
+      sc.pile->maps_dirty = false;
+    }
+  }
   for (const auto& sc : S->scenes) {
     const size_t b = sc.pile->bnd.size();
     if (sc.bnd_base + b > S->bnd_capacity)
@@ -1405,6 +1476,7 @@ int neighbor_phase(alv_solver* S, const Launch& L, int with_factor) {
   launch_gather_boundary(L, A, S->kind, S->stream);
   launch_neighbors(L, A, with_factor, S->stream);
   ALV_HIP(hipGetLastError());
+  S->grid_valid = 1;
   return ALV_OK;
 }
 
@@ -1588,6 +1660,7 @@ int alv_solver_step(alv_solver* S) {
   if (rc != ALV_OK) return rc;
   DevArrays& A = S->A;
   hipStream_t st = S->stream;
+  if (!S->batch) S->pile->snapshot_kinematics();  // Pile::copy_kinematics_to_device
   if (S->kind != ALV_SOLVER_DF && (!A.pressure || !A.last_pressure || !A.aii || !A.pacc))
     return fail(ALV_ERR_STATE, "pressure-solve arrays are not allocated");
   if (S->kind == ALV_SOLVER_DF) {
@@ -1972,6 +2045,152 @@ int alv_profiling_kernel(unsigned index, const char** name, double* total_ms,
   if (name) *name = st.name.c_str();
   if (total_ms) *total_ms = st.total_ms;
   if (launches) *launches = st.launches;
+  return ALV_OK;
+}
+
+// ---- sampling and statistics --------------------------------------------
+
+int alv_solver_sample_neighbors(alv_solver* S, const float* sample_x, unsigned ns,
+                                unsigned max_neighbors, float* sample_neighbors,
+                                unsigned* counts) {
+  if (!S || (ns && (!sample_x || !sample_neighbors || !counts)))
+    return fail(ALV_ERR_ARG, "null argument");
+  if (max_neighbors != S->cni.max_num_neighbors_per_particle)
+    return fail(ALV_ERR_ARG, "sample lists must have max_num_neighbors_per_particle entries per sample");
+  if (S->batch) return fail(ALV_ERR_UNSUPPORTED, "sampling a batch solver is not supported");
+  if (!S->grid_valid)
+    return fail(ALV_ERR_STATE, "no neighbour grid yet: step or update_particle_neighbors first");
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  SimParams p = make_params(S->cn, S->cni, unsigned(S->pile->bnd.size()), S->set.num_particles);
+  launch_sample_neighbors(p, S->A, sample_x, ns, reinterpret_cast<float4*>(sample_neighbors),
+                          counts, nullptr);
+  ALV_HIP(hipGetLastError());
+  return ALV_OK;
+}
+
+int alv_solver_sample_all_boundaries(alv_solver* S, const float* sample_x, unsigned ns,
+                                     float* sample_boundary, float* sample_boundary_kernel) {
+  if (!S || (ns && (!sample_x || !sample_boundary || !sample_boundary_kernel)))
+    return fail(ALV_ERR_ARG, "null argument");
+  if (S->batch) return fail(ALV_ERR_UNSUPPORTED, "sampling a batch solver is not supported");
+  alv_pile* P = S->pile;
+  const unsigned nb = unsigned(P->bnd.size());
+  if (nb == 0 || ns == 0) return ALV_OK;
+  ALV_HIP(reserve_dev(&P->bodies_dev, &P->bodies_cap, nb));
+  ALV_HIP(hipMemcpy(P->bodies_dev, P->bnd.data(), nb * sizeof(BoundaryDev), hipMemcpyHostToDevice));
+  SimParams p = make_params(S->cn, S->cni, nb, S->set.num_particles);
+  launch_sample_boundary(p, P->bodies_dev, nb, sample_x, ns,
+                         reinterpret_cast<float4*>(sample_boundary),
+                         reinterpret_cast<float4*>(sample_boundary_kernel), nullptr);
+  ALV_HIP(hipGetLastError());
+  ALV_HIP(hipDeviceSynchronize());  // the body table is reused by later calls
+  return ALV_OK;
+}
+
+int alv_sample_fluid(const alv_const* cn, const float* nbr, const unsigned* counts, unsigned K,
+                     unsigned ns, const float* density, const float* q, unsigned comps,
+                     float* out) {
+  if (!cn || (ns && (!nbr || !counts || !density || !q || !out)))
+    return fail(ALV_ERR_ARG, "null argument");
+  if (comps != 1 && comps != 3) return fail(ALV_ERR_ARG, "components must be 1 or 3");
+  if (K == 0) return fail(ALV_ERR_ARG, "max_neighbors must be positive");
+  alv_consti cni;
+  memset(&cni, 0, sizeof(cni));
+  SimParams p = make_params(*cn, cni, 0, 0);
+  launch_sample_fluid(p, reinterpret_cast<const float4*>(nbr), counts, K, ns, density, q, comps,
+                      out, nullptr);
+  ALV_HIP(hipGetLastError());
+  return ALV_OK;
+}
+
+int alv_sample_fluid_density(const alv_const* cn, const float* nbr, const unsigned* counts,
+                             unsigned K, unsigned ns, float* out) {
+  if (!cn || (ns && (!nbr || !counts || !out))) return fail(ALV_ERR_ARG, "null argument");
+  if (K == 0) return fail(ALV_ERR_ARG, "max_neighbors must be positive");
+  alv_consti cni;
+  memset(&cni, 0, sizeof(cni));
+  SimParams p = make_params(*cn, cni, 0, 0);
+  launch_sample_density(p, reinterpret_cast<const float4*>(nbr), counts, K, ns, nullptr, 0, out,
+                        nullptr);
+  ALV_HIP(hipGetLastError());
+  return ALV_OK;
+}
+
+int alv_sample_position_density(const alv_const* cn, const float* nbr, const unsigned* counts,
+                                unsigned K, unsigned ns, const float* ker, unsigned nb,
+                                float* out) {
+  if (!cn || (ns && (!nbr || !counts || !out)) || (nb && ns && !ker))
+    return fail(ALV_ERR_ARG, "null argument");
+  if (K == 0) return fail(ALV_ERR_ARG, "max_neighbors must be positive");
+  alv_consti cni;
+  memset(&cni, 0, sizeof(cni));
+  SimParams p = make_params(*cn, cni, nb, 0);
+  launch_sample_density(p, reinterpret_cast<const float4*>(nbr), counts, K, ns,
+                        reinterpret_cast<const float4*>(ker), nb, out, nullptr);
+  ALV_HIP(hipGetLastError());
+  return ALV_OK;
+}
+
+namespace {
+int sample_velocity_common(const alv_const* cn, const float* nbr, const unsigned* counts,
+                           unsigned K, unsigned ns, const float* density, const float* v,
+                           const float* sample_v, const float* rec, const float* ker,
+                           unsigned nb, const float* rx, const float* rv, const float* rw,
+                           float* out) {
+  if (!cn || (ns && (!nbr || !counts || !density || !v || !out)))
+    return fail(ALV_ERR_ARG, "null argument");
+  if (nb && (!rec || !ker || !rx || !rv || !rw)) return fail(ALV_ERR_ARG, "null boundary argument");
+  if (K == 0) return fail(ALV_ERR_ARG, "max_neighbors must be positive");
+  alv_consti cni;
+  memset(&cni, 0, sizeof(cni));
+  SimParams p = make_params(*cn, cni, nb, 0);
+  launch_sample_velocity(p, reinterpret_cast<const float4*>(nbr), counts, K, ns, density, v,
+                         sample_v, reinterpret_cast<const float4*>(rec),
+                         reinterpret_cast<const float4*>(ker), nb, rx, rv, rw, out, nullptr);
+  ALV_HIP(hipGetLastError());
+  return ALV_OK;
+}
+}  // namespace
+
+int alv_sample_velocity(const alv_const* cn, const float* nbr, const unsigned* counts, unsigned K,
+                        unsigned ns, const float* density, const float* v, const float* rec,
+                        const float* ker, unsigned nb, const float* rx, const float* rv,
+                        const float* rw, float* out) {
+  return sample_velocity_common(cn, nbr, counts, K, ns, density, v, nullptr, rec, ker, nb, rx, rv,
+                                rw, out);
+}
+
+int alv_sample_vorticity(const alv_const* cn, const float* nbr, const unsigned* counts,
+                         unsigned K, unsigned ns, const float* density, const float* v,
+                         const float* sample_v, const float* rec, const float* ker, unsigned nb,
+                         const float* rx, const float* rv, const float* rw, float* out) {
+  if (ns && !sample_v) return fail(ALV_ERR_ARG, "null sample_v");
+  return sample_velocity_common(cn, nbr, counts, K, ns, density, v, sample_v, rec, ker, nb, rx, rv,
+                                rw, out);
+}
+
+int alv_histogram256(const float* v, const float* mask, unsigned n, float v_min, float v_max,
+                     unsigned* histogram) {
+  if (!histogram || (n && !v)) return fail(ALV_ERR_ARG, "null argument");
+  static unsigned* partial = nullptr;  // process-wide scratch of the partial histograms
+  if (!partial) ALV_HIP(hipMalloc(&partial, histogram_partial_count() * sizeof(unsigned)));
+  const float step_inverse = float(256.0 / (double(v_max) - double(v_min)));
+  launch_histogram256(v, mask, n, v_min, step_inverse, partial, histogram, nullptr);
+  ALV_HIP(hipGetLastError());
+  return ALV_OK;
+}
+
+int alv_kl_divergence(const unsigned* hp, const unsigned* hq, unsigned n_p, unsigned n_q,
+                      float q_lower_bound, unsigned bins, float* result) {
+  if (!hp || !hq || !result) return fail(ALV_ERR_ARG, "null argument");
+  if (n_p == 0) return fail(ALV_ERR_ARG, "n_p must be positive");
+  float* d = nullptr;
+  ALV_HIP(hipMalloc(&d, sizeof(float)));
+  launch_kl_divergence(hp, hq, n_p, n_q, q_lower_bound, bins, d, nullptr);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(result, d, sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(ALV_ERR_HIP, hipGetErrorString(e));
   return ALV_OK;
 }
 

@@ -303,6 +303,45 @@ __device__ inline float grid_sample(const GridGeomDev& g, const float* __restric
 }
 
 // --- neighbour grid helpers (runner.hpp:1827-1838, 689-707) -----------------
+// Boundary record of point x for boundary B
+// (compute_volume_and_boundary_x_analytic, runner.hpp:1738-1771, and
+// compute_particle_boundary_analytic, runner.hpp:1982-2002): rec = (closest
+// boundary point, volume Psi), ker = (Psi gradW, Psi W).  Returns Psi.
+// With Sparse, a point outside the map or in a cell whose nodes are all zero
+// returns 0 at once and leaves rec and ker unset (its volume is exactly 0,
+// so every term of the record is too).
+template <bool Sparse>
+__device__ inline float boundary_record(const BoundaryDev& B, vec3 x, const SimParams& p,
+                                        float4& rec, float4& ker) {
+  const float4 q = make_float4(B.q[0], B.q[1], B.q[2], B.q[3]);
+  vec3 local = rotate(x - mk(B.x[0], B.x[1], B.x[2]), conjugate(q));
+  int ip[3];
+  vec3 inner;
+  const bool in_map = grid_resolve(B.geom, local, ip, inner);
+  if constexpr (Sparse) {
+    if (!in_map) return 0.0f;
+    if (B.cell_nz &&
+        !B.cell_nz[(unsigned(ip[2]) * B.geom.res[1] + unsigned(ip[1])) * B.geom.res[0] + unsigned(ip[0])])
+      return 0.0f;
+  }
+  float d = sdf_eval(B, local) * B.sign;
+  vec3 normal = rotate(sdf_gradient(B, local, p.h) * B.sign, q);
+  float nl2 = length_sqr(normal);
+  normal = normal * (nl2 > 0.0f ? 1.0f / sqrtf(nl2) : 0.0f);
+  vec3 xi_bxj = d * normal;
+  vec3 bx = x - xi_bxj;
+  float vol = 0.0f;
+  if (in_map) {
+    unsigned cell[32];
+    grid_cell_nodes(B.geom.res, ip, cell);
+    vol = grid_interpolate(B.volume, cell, inner);
+  }
+  vec3 kg = gradW(xi_bxj, p);
+  rec = make_float4(bx.x, bx.y, bx.z, vol);
+  ker = make_float4(vol * kg.x, vol * kg.y, vol * kg.z, vol * W_dist(d, p));
+  return vol;
+}
+
 __device__ inline void cell_ipos(vec3 x, const SimParams& p, int ip[3]) {
   ip[0] = int(floorf(x.x / p.h)) - p.off[0];
   ip[1] = int(floorf(x.y / p.h)) - p.off[1];

@@ -130,6 +130,7 @@ struct Launch {
   unsigned scenes;
   unsigned active;   // particles in all scenes
   SimParams p0;      // scene 0 (shared constants, wrap flags)
+  int tiles;         // launch the LDS-tile instances of the neighbour passes
 };
 
 // Particle-kernel prologue: this block's particle index s (blocks remapped
@@ -192,6 +193,32 @@ void launch_integrate_velocity(const Launch& L, const DevArrays& A, float dt, hi
 void launch_ii_viscosity_predict(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_ii_predict(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_ii_iteration(const Launch& L, const DevArrays& A, hipStream_t s);
+// non-zero flags of the volume-map cells (alv_sample.hip)
+void launch_map_flags(const GridGeomDev& g, const float* nodes, unsigned char* out,
+                      hipStream_t s);
+// sampling and statistics (alv_sample.hip)
+void launch_sample_neighbors(const SimParams& p, const DevArrays& A, const float* sample_x,
+                             unsigned ns, float4* out, unsigned* counts, hipStream_t s);
+void launch_sample_boundary(const SimParams& p, const BoundaryDev* bodies, unsigned nb,
+                            const float* sample_x, unsigned ns, float4* rec, float4* ker,
+                            hipStream_t s);
+void launch_sample_fluid(const SimParams& p, const float4* nbr, const unsigned* counts,
+                         unsigned K, unsigned ns, const float* density, const float* q,
+                         unsigned comps, float* out, hipStream_t s);
+void launch_sample_density(const SimParams& p, const float4* nbr, const unsigned* counts,
+                           unsigned K, unsigned ns, const float4* ker, unsigned nb, float* out,
+                           hipStream_t s);
+void launch_sample_velocity(const SimParams& p, const float4* nbr, const unsigned* counts,
+                            unsigned K, unsigned ns, const float* density, const float* v,
+                            const float* sample_v, const float4* rec, const float4* ker,
+                            unsigned nb, const float* rx, const float* rv, const float* rw,
+                            float* out, hipStream_t s);
+void launch_histogram256(const float* v, const float* mask, unsigned n, float v_min,
+                         float step_inverse, unsigned* partial, unsigned* hist, hipStream_t s);
+unsigned histogram_partial_count();
+void launch_kl_divergence(const unsigned* hp, const unsigned* hq, unsigned n_p, unsigned n_q,
+                          float q_lower_bound, unsigned bins, float* out, hipStream_t s);
+
 // SolverI (ISPH)
 void launch_is_prepare(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_is_iteration(const Launch& L, const DevArrays& A, hipStream_t s);

@@ -56,6 +56,13 @@ __device__ inline void batch_ids(const DevArrays& A, unsigned s, unsigned nn, un
   j[3] = t0 + 3 < nn ? g.w : s;
 }
 
+// The w lane of the sorted position carries the stiffness of the next
+// stiffness pass (kappa_v, kfac or kappa, written by the pass that computes
+// it), so a stiffness pass gathers x_j and k_j with one load.
+__device__ inline void set_kw(const DevArrays& A, unsigned s, float k) {
+  reinterpret_cast<float*>(A.x + s)[3] = k;
+}
+
 // The w lane of the padded velocity carries the density, so the viscosity
 // pass gathers v_j and rho_j with one load; velocity updates keep it.
 __device__ inline void add_v(DevArrays& A, unsigned s, vec3 dv) {
@@ -322,29 +329,14 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SceneTab T, DevArrays A
   unsigned mask = 0;
   for (unsigned b = 0; b < p.num_boundaries; ++b) {
     const BoundaryDev& B = A.bnd[b];
-    const float4 q = make_float4(B.q[0], B.q[1], B.q[2], B.q[3]);
-    vec3 local = rotate(x - mk(B.x[0], B.x[1], B.x[2]), conjugate(q));
-    float d = sdf_eval(B, local) * B.sign;
-    vec3 normal = rotate(sdf_gradient(B, local, p.h) * B.sign, q);
-    float nl2 = length_sqr(normal);
-    normal = normal * (nl2 > 0.0f ? 1.0f / sqrtf(nl2) : 0.0f);
-    vec3 xi_bxj = d * normal;
-    vec3 bx = x - xi_bxj;
-    float vol = 0.0f;
-    int ip[3];
-    vec3 inner;
-    if (grid_resolve(B.geom, local, ip, inner)) {
-      unsigned cell[32];
-      grid_cell_nodes(B.geom.res, ip, cell);
-      vol = grid_interpolate(B.volume, cell, inner);
-    }
+    float4 rec, ker;
+    const float vol = boundary_record<true>(B, x, p, rec, ker);
     // Records are stored only where Psi != 0; every contribution of a
     // boundary with Psi = 0 is an exact zero, so the passes skip it.
     if (vol != 0.0f) {
-      vec3 kg = gradW(xi_bxj, p);
       const size_t o = size_t(b) * A.cap + s;
-      A.brec[o] = f4(bx, vol);
-      A.bker[o] = make_float4(vol * kg.x, vol * kg.y, vol * kg.z, vol * W_dist(d, p));
+      A.brec[o] = rec;
+      A.bker[o] = ker;
       mask |= 1u << b;
       if (B.dynamic) {
         A.bforce[o] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -376,18 +368,10 @@ struct TileXV : TileMap {
   __device__ float4 X(unsigned j) const { return x[slot(j)]; }
   __device__ float4 V(unsigned j) const { return v[slot(j)]; }
 };
-// positions and one scalar (the stiffness of a pass)
-struct GlobalXK {
+// positions only (the neighbour search; stiffness passes read k_j in x_j.w)
+struct GlobalX {
   const float4* x;
-  const float* k;
   __device__ float4 X(unsigned j) const { return x[j]; }
-  __device__ float K(unsigned j) const { return k[j]; }
-};
-struct TileXK : TileMap {
-  const float4* x;  // LDS
-  const float* k;   // LDS
-  __device__ float4 X(unsigned j) const { return x[slot(j)]; }
-  __device__ float K(unsigned j) const { return k[slot(j)]; }
 };
 
 // The block's tile ranges and their LDS bases; false when the block has no
@@ -432,27 +416,6 @@ __device__ inline bool stage_xv(const DevArrays& A, unsigned blk, TileXV& t) {
   t.v = lds_xv + kTileMax;
   return true;
 }
-// positions and the pass's stiffness array ks into LDS
-__device__ inline bool stage_xk(const DevArrays& A, unsigned blk, const float* ks, TileXK& t) {
-  __shared__ float4 lds_x[kTileMax];
-  __shared__ float lds_k[kTileMax];
-  Tile tl;
-  if (!tile_map(A, blk, tl, t)) return false;
-  unsigned off = 0;
-#pragma unroll
-  for (int w = 0; w < 3; ++w) {
-    const unsigned lo = tl.lo[w], len = tl.len[w];
-    for (unsigned k = threadIdx.x; k < len; k += kBlock) {
-      lds_x[off + k] = A.x[lo + k];
-      lds_k[off + k] = ks[lo + k];
-    }
-    off += len;
-  }
-  __syncthreads();
-  t.x = lds_x;
-  t.k = lds_k;
-  return true;
-}
 
 // positions only (the neighbour search)
 struct TileX : TileMap {
@@ -482,12 +445,13 @@ __device__ inline bool stage_x(const DevArrays& A, unsigned blk, TileX& t) {
 // reference order (dx outer, dz inner); inside a cell in ascending particle
 // id, which is ascending sorted index.
 // ---------------------------------------------------------------------------
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int with_factor) {
   ALV_SCENE(T, A, s);
   // candidate positions from the block's LDS tile when it has one
   TileX tile_;
-  const bool tiled_ = !W && stage_x(A, s / kBlock, tile_);
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_x(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const unsigned K = p.kmax;
   const vec3 xi = ld3(A.x[s]);
@@ -683,17 +647,13 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
   for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
     unsigned j[kNB];
     batch_ids(A, s, nn, t0, j);
-    float kj[kNB];
-#pragma unroll
-    for (int u = 0; u < kNB; ++u) kj[u] = src.K(j[u]);
-    // positions are gathered only for pairs that contribute
+    // one gather per neighbour: the pass's stiffness k_j rides in x_j.w
     float4 xj[kNB];
 #pragma unroll
-    for (int u = 0; u < kNB; ++u)
-      xj[u] = (fabsf(k_i + kj[u]) > p.eps) ? src.X(j[u]) : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int u = 0; u < kNB; ++u) xj[u] = src.X(j[u]);
 #pragma unroll
     for (int u = 0; u < kNB; ++u) {
-      const float k_sum = k_i + kj[u];
+      const float k_sum = k_i + xj[u].w;
       if (t0 + u < nn && fabsf(k_sum) > p.eps) {
         vec3 d = wrapd<W>(xi - ld3(xj[u]), p);
         if (kDivergenceForm) {
@@ -729,11 +689,12 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
 
 // warm_start_divergence_solve_0 (runner.hpp:3230-3250)
 // F: also record the warm-start features of the particle
-template <bool W, bool F>
+template <bool W, bool F, bool TL>
 __global__ __launch_bounds__(256) void k_ws_div0(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   TileXV tile_;
-  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   float dadv = tiled_ ? density_change<W>(p, A, s, tile_)
@@ -741,6 +702,7 @@ __global__ __launch_bounds__(256) void k_ws_div0(SceneTab T, DevArrays A) {
   const float kv_prev = A.kappa_v[s];
   const float kv = (dadv > 0.0f) ? 0.5f * std_max(kv_prev, -0.5f) / dt : 0.0f;
   A.kappa_v[s] = kv;
+  set_kw(A, s, kv);
   if (F) {
     float psi = 0.0f;
     unsigned mask = A.bmask[s];
@@ -780,29 +742,34 @@ __global__ __launch_bounds__(256) void k_ws_learned(SceneTab T, DevArrays A, Mlp
   const float dt = A.ctl->dt;
   if (use_divergence && f[1] > 0.0f) {
     const float kv = std_min(std_max(y0, -0.5f), 0.0f) / dt;
-    if (isfinite(y0)) A.kappa_v[s] = kv;
+    if (isfinite(y0)) {
+      A.kappa_v[s] = kv;
+      set_kw(A, s, kv);
+    }
   }
   A.kpred[s] = isfinite(y1) ? std_min(std_max(y1, -0.00025f), 0.0f) : y1;
 }
 // warm_start_divergence_solve_1 (runner.hpp:3252-3295)
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_ws_div1(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
-  TileXK tk_;
-  const bool tiled_ = !W && stage_xk(A, s / kBlock, A.kappa_v, tk_);
+  TileX tk_;
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_x(A, s / kBlock, tk_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   vec3 dv = tiled_ ? stiffness_dv<W, true>(p, A, s, A.kappa_v[s], tk_, dt)
-                   : stiffness_dv<W, true>(p, A, s, A.kappa_v[s], GlobalXK{A.x, A.kappa_v}, dt);
+                   : stiffness_dv<W, true>(p, A, s, A.kappa_v[s], GlobalX{A.x}, dt);
   add_v(A, s, dv);
 }
 // compute_velocity_of_density_change (runner.hpp:3297-3314); also stores the
 // first iteration's stiffness k = rho_adv * alpha
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_vel_density_change(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   TileXV tile_;
-  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   float dadv = tiled_ ? density_change<W>(p, A, s, tile_)
@@ -813,31 +780,34 @@ __global__ __launch_bounds__(256) void k_vel_density_change(SceneTab T, DevArray
   A.alpha[s] = alpha;
   const float k = dadv * alpha;
   A.kfac[s] = k;
+  set_kw(A, s, k);
 }
 // divergence_solve_iteration (runner.hpp:3316-3363)
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_div_iter(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   if (A.ctl->div_done) return;
-  TileXK tk_;
-  const bool tiled_ = !W && stage_xk(A, s / kBlock, A.kfac, tk_);
+  TileX tk_;
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_x(A, s / kBlock, tk_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   const float k_i = A.kfac[s];
   A.kappa_v[s] += k_i;
   vec3 dv = tiled_ ? stiffness_dv<W, true>(p, A, s, k_i, tk_, dt)
-                   : stiffness_dv<W, true>(p, A, s, k_i, GlobalXK{A.x, A.kfac}, dt);
+                   : stiffness_dv<W, true>(p, A, s, k_i, GlobalX{A.x}, dt);
   add_v(A, s, dv);
 }
 // compute_divergence_solve_density_error (runner.hpp:3396-3410) + FP64
 // partial sums of the residual
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_div_err(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
   ALV_SCENE(T, A, s);
   if (A.ctl->div_done) return;
   TileXV tile_;
-  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   double part = 0.0;
   if (s < p.first + p.n) {
     float dadv = tiled_ ? density_change<W>(p, A, s, tile_)
@@ -845,6 +815,7 @@ __global__ __launch_bounds__(256) void k_div_err(SceneTab T, DevArrays A) {
     A.rho_adv[s] = dadv;
     const float k = dadv * A.alpha[s];
     A.kfac[s] = k;
+    set_kw(A, s, k);
     part = double(dadv);
   }
   double t = block_sum_d(part, lds);
@@ -928,12 +899,13 @@ __device__ inline void orthogonal_vectors(vec3 n, vec3& t1, vec3& t2) {
   t2 = t2 * (1.0f / sqrtf(length_sqr(t2)));
 }
 
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_viscosity(SceneTab T, DevArrays A, int div_finish) {
   __shared__ float lds[4];
   ALV_SCENE(T, A, s);
   TileXV tile_;
-  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   const float dt = A.ctl->dt;
   float cfl = -FLT_MAX;
   if (s < p.first + p.n) {
@@ -1034,11 +1006,12 @@ __global__ __launch_bounds__(256) void k_integrate_np(SceneTab T, DevArrays A) {
 
 // warm_start_pressure_solve0 (runner.hpp:3430-3450)
 // L: learned warm start (prediction from k_ws_learned)
-template <bool W, bool L>
+template <bool W, bool L, bool TL>
 __global__ __launch_bounds__(256) void k_ws_p0(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   TileXV tile_;
-  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   float dadv = tiled_ ? density_adv<W>(p, A, s, dt, tile_)
@@ -1049,25 +1022,28 @@ __global__ __launch_bounds__(256) void k_ws_p0(SceneTab T, DevArrays A) {
     if (isfinite(kl)) k = kl;
   }
   A.kappa[s] = k;
+  set_kw(A, s, k);
 }
 // warm_start_pressure_solve1 (runner.hpp:3452-3491)
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_ws_p1(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
-  TileXK tk_;
-  const bool tiled_ = !W && stage_xk(A, s / kBlock, A.kappa, tk_);
+  TileX tk_;
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_x(A, s / kBlock, tk_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   vec3 dv = tiled_ ? stiffness_dv<W, false>(p, A, s, A.kappa[s], tk_, dt)
-                   : stiffness_dv<W, false>(p, A, s, A.kappa[s], GlobalXK{A.x, A.kappa}, dt);
+                   : stiffness_dv<W, false>(p, A, s, A.kappa[s], GlobalX{A.x}, dt);
   add_v(A, s, dv);
 }
 // compute_rho_adv (runner.hpp:3525-3541); stores k = (rho_adv - 1) * alpha
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_rho_adv(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   TileXV tile_;
-  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   float dadv = tiled_ ? density_adv<W>(p, A, s, dt, tile_)
@@ -1077,31 +1053,34 @@ __global__ __launch_bounds__(256) void k_rho_adv(SceneTab T, DevArrays A) {
   A.alpha[s] = alpha;
   const float k = (dadv - 1.0f) * alpha;
   A.kfac[s] = k;
+  set_kw(A, s, k);
 }
 // pressure_solve_iteration (runner.hpp:3543-3588)
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_p_iter(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   if (A.ctl->dens_done) return;
-  TileXK tk_;
-  const bool tiled_ = !W && stage_xk(A, s / kBlock, A.kfac, tk_);
+  TileX tk_;
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_x(A, s / kBlock, tk_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
   const float k_i = A.kfac[s];
   A.kappa[s] += k_i;
   vec3 dv = tiled_ ? stiffness_dv<W, false>(p, A, s, k_i, tk_, dt)
-                   : stiffness_dv<W, false>(p, A, s, k_i, GlobalXK{A.x, A.kfac}, dt);
+                   : stiffness_dv<W, false>(p, A, s, k_i, GlobalX{A.x}, dt);
   add_v(A, s, dv);
 }
 // compute_pressure_solve_density_error (runner.hpp:3590-3604) + FP64
 // partial sums of (rho_adv - 1)
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_p_err(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
   ALV_SCENE(T, A, s);
   if (A.ctl->dens_done) return;
   TileXV tile_;
-  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   const float dt = A.ctl->dt;
   double part = 0.0;
   if (s < p.first + p.n) {
@@ -1110,6 +1089,7 @@ __global__ __launch_bounds__(256) void k_p_err(SceneTab T, DevArrays A) {
     A.rho_adv[s] = dadv;
     const float k = (dadv - 1.0f) * A.alpha[s];
     A.kfac[s] = k;
+    set_kw(A, s, k);
     part = double(dadv - 1.0f);
   }
   double t = block_sum_d(part, lds);
@@ -1153,11 +1133,12 @@ __global__ __launch_bounds__(256) void k_df_finish(SceneTab T, DevArrays A, int 
 
 // compute_velocity_of_density_change (runner.hpp:3297-3314), the density
 // change sum alone (no stiffness factor update): rho_adv <- rho_dot.
-template <bool W>
+template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_density_change(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   TileXV tile_;
-  const bool tiled_ = !W && stage_xv(A, s / kBlock, tile_);
+  bool tiled_ = false;
+  if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   A.rho_adv[s] = tiled_ ? density_change<W>(p, A, s, tile_)
                         : density_change<W>(p, A, s, GlobalXV{A.x, A.v});
@@ -1770,6 +1751,36 @@ KernelProfiler* g_profiler = nullptr;
                          __VA_ARGS__);                                            \
     if (g_profiler) g_profiler->end(#kernel, s);                                  \
   } while (0)
+// neighbour passes with an LDS-tile instance (never with the periodic wrap):
+// the gather instances reserve no LDS, so they keep their occupancy
+#define ALV_LAUNCH_WT(kernel, L, s, ...)                                          \
+  do {                                                                            \
+    if (g_profiler) g_profiler->begin(#kernel, s);                                \
+    if ((L).p0.wrap_mask)                                                         \
+      hipLaunchKernelGGL((kernel<true, false>), dim3((L).pblocks), dim3(kBlock), 0, s, (L).T, \
+                         __VA_ARGS__);                                            \
+    else if ((L).tiles)                                                           \
+      hipLaunchKernelGGL((kernel<false, true>), dim3((L).pblocks), dim3(kBlock), 0, s, (L).T, \
+                         __VA_ARGS__);                                            \
+    else                                                                          \
+      hipLaunchKernelGGL((kernel<false, false>), dim3((L).pblocks), dim3(kBlock), 0, s, (L).T, \
+                         __VA_ARGS__);                                            \
+    if (g_profiler) g_profiler->end(#kernel, s);                                  \
+  } while (0)
+#define ALV_LAUNCH_W2T(kernel, flag, L, s, ...)                                   \
+  do {                                                                            \
+    if (g_profiler) g_profiler->begin(#kernel, s);                                \
+    if ((L).p0.wrap_mask)                                                         \
+      hipLaunchKernelGGL((kernel<true, flag, false>), dim3((L).pblocks), dim3(kBlock), 0, s, \
+                         (L).T, __VA_ARGS__);                                     \
+    else if ((L).tiles)                                                           \
+      hipLaunchKernelGGL((kernel<false, flag, true>), dim3((L).pblocks), dim3(kBlock), 0, s, \
+                         (L).T, __VA_ARGS__);                                     \
+    else                                                                          \
+      hipLaunchKernelGGL((kernel<false, flag, false>), dim3((L).pblocks), dim3(kBlock), 0, s, \
+                         (L).T, __VA_ARGS__);                                     \
+    if (g_profiler) g_profiler->end(#kernel, s);                                  \
+  } while (0)
 // kernels templated on the periodic wrap: one name for both instances
 #define ALV_LAUNCH_W(kernel, L, s, ...)                                           \
   do {                                                                            \
@@ -1834,25 +1845,25 @@ void launch_gather_boundary(const Launch& L, const DevArrays& A, int kind, hipSt
   ALV_LAUNCH(k_gather_boundary, L, s, A, kind);
 }
 void launch_neighbors(const Launch& L, const DevArrays& A, int with_factor, hipStream_t s) {
-  ALV_LAUNCH_W(k_neighbors, L, s, A, with_factor);
+  ALV_LAUNCH_WT(k_neighbors, L, s, A, with_factor);
 }
 void launch_df_warm_start_div(const Launch& L, const DevArrays& A, hipStream_t s,
                               int features, const MlpDev* net, int net_divergence) {
   if (features || net)
-    ALV_LAUNCH_W2(k_ws_div0, true, L, s, A);
+    ALV_LAUNCH_W2T(k_ws_div0, true, L, s, A);
   else
-    ALV_LAUNCH_W2(k_ws_div0, false, L, s, A);
+    ALV_LAUNCH_W2T(k_ws_div0, false, L, s, A);
   if (net) ALV_LAUNCH(k_ws_learned, L, s, A, *net, net_divergence);
-  ALV_LAUNCH_W(k_ws_div1, L, s, A);
-  ALV_LAUNCH_W(k_vel_density_change, L, s, A);
+  ALV_LAUNCH_WT(k_ws_div1, L, s, A);
+  ALV_LAUNCH_WT(k_vel_density_change, L, s, A);
 }
 void launch_df_div_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_div_iter, L, s, A);
-  ALV_LAUNCH_W(k_div_err, L, s, A);
+  ALV_LAUNCH_WT(k_div_iter, L, s, A);
+  ALV_LAUNCH_WT(k_div_err, L, s, A);
   ALV_LAUNCH_SCENES(k_div_finalize, L, s, A);
 }
 void launch_df_viscosity(const Launch& L, const DevArrays& A, int div_finish, hipStream_t s) {
-  ALV_LAUNCH_W(k_viscosity, L, s, A, div_finish);
+  ALV_LAUNCH_WT(k_viscosity, L, s, A, div_finish);
 }
 void launch_update_dt(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_SCENES(k_update_dt, L, s, A);
@@ -1863,15 +1874,15 @@ void launch_df_integrate_np(const Launch& L, const DevArrays& A, hipStream_t s) 
 void launch_df_warm_start_dens(const Launch& L, const DevArrays& A, hipStream_t s,
                                int learned) {
   if (learned)
-    ALV_LAUNCH_W2(k_ws_p0, true, L, s, A);
+    ALV_LAUNCH_W2T(k_ws_p0, true, L, s, A);
   else
-    ALV_LAUNCH_W2(k_ws_p0, false, L, s, A);
-  ALV_LAUNCH_W(k_ws_p1, L, s, A);
-  ALV_LAUNCH_W(k_rho_adv, L, s, A);
+    ALV_LAUNCH_W2T(k_ws_p0, false, L, s, A);
+  ALV_LAUNCH_WT(k_ws_p1, L, s, A);
+  ALV_LAUNCH_WT(k_rho_adv, L, s, A);
 }
 void launch_df_dens_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_p_iter, L, s, A);
-  ALV_LAUNCH_W(k_p_err, L, s, A);
+  ALV_LAUNCH_WT(k_p_iter, L, s, A);
+  ALV_LAUNCH_WT(k_p_err, L, s, A);
   ALV_LAUNCH_SCENES(k_p_finalize, L, s, A);
 }
 void launch_df_finish(const Launch& L, const DevArrays& A, int dens_finish, hipStream_t s) {
@@ -1882,13 +1893,13 @@ void launch_export_pairs(const Launch& L, const DevArrays& A, float* x, float* y
   ALV_LAUNCH(k_export_pairs, L, s, A, x, y);
 }
 void launch_density_change(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_density_change, L, s, A);
+  ALV_LAUNCH_WT(k_density_change, L, s, A);
 }
 void launch_integrate_velocity(const Launch& L, const DevArrays& A, float dt, hipStream_t s) {
   ALV_LAUNCH_W(k_integrate_velocity, L, s, A, dt);
 }
 void launch_ii_viscosity_predict(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_viscosity, L, s, A, 0);
+  ALV_LAUNCH_WT(k_viscosity, L, s, A, 0);
 }
 void launch_ii_predict(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_W(k_ii_predict0, L, s, A);

@@ -49,6 +49,9 @@ struct BoundaryDev {
   float outset, radius, max_distance;
   GridGeomDev geom;
   const float* volume;  // device pointer to the volume map nodes
+  // per map cell (x fastest): 1 when any of its 32 nodes is non-zero; a
+  // point in a cell with 0 has volume exactly 0
+  const unsigned char* cell_nz;
   float x[3], v[3], omega[3];
   float q[4];  // (x, y, z, w)
   float mass;

@@ -200,8 +200,80 @@ PYBIND11_MODULE(_alluvion, m) {
   declare_variable<3, U>(m, store, "3Duint");
   declare_variable<1, I>(m, store, "1Dint");
 
+  py::class_<SolverGrid>(m, "SolverGrid");
+  using R = Runner<float>;
+  auto make_list = [](R&, Variable<1, float3> const& sample_x, SolverGrid pid, py::object pid_length,
+                      Variable<2, float4>& nbr, Variable<1, U>& counts, py::object grid_anomaly,
+                      U num_samples) {
+    (void)pid_length;    // the solver's own grid
+    (void)grid_anomaly;  // the solver's grid_anomaly[2] is set on overflow
+    R::launch_make_neighbor_list(sample_x, pid, nbr, counts, num_samples);
+  };
   py::class_<Runner<float>>(m, "Runnerfloat")
       .def(py::init<>())
+      .def("launch_make_neighbor_list", make_list, py::arg("sample_x"), py::arg("pid"),
+           py::arg("pid_length"), py::arg("sample_neighbors"), py::arg("sample_num_neighbors"),
+           py::arg("grid_anomaly"), py::arg("num_samples"))
+      .def("launch_make_neighbor_list_wrap1", make_list, py::arg("sample_x"), py::arg("pid"),
+           py::arg("pid_length"), py::arg("sample_neighbors"), py::arg("sample_num_neighbors"),
+           py::arg("grid_anomaly"), py::arg("num_samples"))
+      .def("launch_sample_fluid", [](R&, Variable<1, float3> const& sx, Variable<1, float3> const& px,
+                                     Variable<1, float> const& pd, Variable<1, float> const& q,
+                                     Variable<2, float4> const& nbr, Variable<1, U> const& cnt,
+                                     Variable<1, float>& out, U ns) {
+             R::launch_sample_fluid(sx, px, pd, q, nbr, cnt, out, ns);
+           })
+      .def("launch_sample_fluid", [](R&, Variable<1, float3> const& sx, Variable<1, float3> const& px,
+                                     Variable<1, float> const& pd, Variable<1, float3> const& q,
+                                     Variable<2, float4> const& nbr, Variable<1, U> const& cnt,
+                                     Variable<1, float3>& out, U ns) {
+             R::launch_sample_fluid(sx, px, pd, q, nbr, cnt, out, ns);
+           })
+      .def("launch_sample_fluid_density",
+           [](R&, Variable<2, float4> const& nbr, Variable<1, U> const& cnt,
+              Variable<1, float>& out, U ns) { R::launch_sample_fluid_density(nbr, cnt, out, ns); })
+      .def("launch_sample_density",
+           [](R&, Variable<1, float3> const& sx, Variable<2, float4> const& nbr,
+              Variable<1, U> const& cnt, Variable<1, float>& out, Variable<2, float4> const& ker,
+              U ns) { R::launch_sample_density(sx, nbr, cnt, out, ker, ns); })
+      .def("launch_sample_velocity",
+           [](R&, Variable<1, float3> const& sx, Variable<1, float3> const& px,
+              Variable<1, float> const& pd, Variable<1, float3> const& pv,
+              Variable<2, float4> const& nbr, Variable<1, U> const& cnt, Variable<1, float3>& sv,
+              Variable<2, float4> const& rec, Variable<2, float4> const& ker,
+              Variable<1, float3> const& rx, Variable<1, float3> const& rv,
+              Variable<1, float3> const& rw, U ns) {
+             R::launch_sample_velocity(sx, px, pd, pv, nbr, cnt, sv, rec, ker, rx, rv, rw, ns);
+           })
+      .def("launch_sample_vorticity",
+           [](R&, Variable<1, float3> const& sx, Variable<1, float3> const& px,
+              Variable<1, float> const& pd, Variable<1, float3> const& pv,
+              Variable<2, float4> const& nbr, Variable<1, U> const& cnt,
+              Variable<1, float3> const& sv, Variable<1, float3>& vort,
+              Variable<2, float4> const& rec, Variable<2, float4> const& ker,
+              Variable<1, float3> const& rx, Variable<1, float3> const& rv,
+              Variable<1, float3> const& rw, U ns) {
+             R::launch_sample_vorticity(sx, px, pd, pv, nbr, cnt, sv, vort, rec, ker, rx, rv, rw, ns);
+           })
+      // the reference's partial-histogram and quantized buffers are accepted and unused
+      .def("launch_histogram256",
+           [](R&, py::object partial, Variable<1, U>& hist, py::object quantized4s,
+              Variable<1, float> const& v, float v_min, float v_max, U n) {
+             (void)partial;
+             (void)quantized4s;
+             R::launch_histogram256(v, v_min, v_max, n, hist);
+           })
+      .def("launch_histogram256_with_mask",
+           [](R&, py::object partial, Variable<1, U>& hist, py::object quantized4s,
+              Variable<1, float> const& mask, Variable<1, float> const& v, float v_min,
+              float v_max, U n) {
+             (void)partial;
+             (void)quantized4s;
+             R::launch_histogram256(v, v_min, v_max, n, hist, &mask);
+           })
+      .def_static("calculate_kl_divergence", &R::calculate_kl_divergence, py::arg("histogram_p"),
+                  py::arg("histogram_q"), py::arg("n_p"), py::arg("n_q"),
+                  py::arg("q_lower_bound"), py::arg("num_bins"))
       .def_static("sum", [](Variable<1, float> const& v, U n, U off) { return Runner<float>::sum(v, n, off); },
                   py::arg("var"), py::arg("n"), py::arg("offset") = 0)
       .def_static("sum", [](Variable<1, float3> const& v, U n, U off) { return Runner<float>::sum(v, n, off); },
@@ -288,6 +360,11 @@ PYBIND11_MODULE(_alluvion, m) {
       .def("set_gravity", &P::set_gravity)
       .def("integrate_kinematics", &P::integrate_kinematics)
       .def("copy_kinematics_to_device", &P::copy_kinematics_to_device)
+      .def_property_readonly("x_device", py::cpp_function(&P::x_device, py::keep_alive<0, 1>()))
+      .def_property_readonly("v_device", py::cpp_function(&P::v_device, py::keep_alive<0, 1>()))
+      .def_property_readonly("q_device", py::cpp_function(&P::q_device, py::keep_alive<0, 1>()))
+      .def_property_readonly("omega_device",
+                             py::cpp_function(&P::omega_device, py::keep_alive<0, 1>()))
       .def("find_contacts", [](P& p, U i, U j) { return p.find_contacts(i, j); })
       .def("find_contacts", [](P& p, U i) { return p.find_contacts(i); })
       .def("find_contacts",
@@ -403,6 +480,9 @@ PYBIND11_MODULE(_alluvion, m) {
         .def_property_readonly("grid_anomaly", py::cpp_function(&S::grid_anomaly, py::keep_alive<0, 1>()))
         .def_property_readonly("boundary_mask", py::cpp_function(&S::boundary_mask, py::keep_alive<0, 1>()))
         .def_property_readonly("pid_length", py::cpp_function(&S::pid_length, py::keep_alive<0, 1>()))
+        .def_property_readonly("pid", py::cpp_function(&S::pid, py::keep_alive<0, 1>()))
+        .def("sample_all_boundaries", &S::sample_all_boundaries, py::arg("sample_x"),
+             py::arg("sample_boundary"), py::arg("sample_boundary_kernel"), py::arg("num_samples"))
         .def("reset_solving_var", &S::reset_solving_var)
         .def("reset_t", &S::reset_t)
         .def("compute_all_boundaries", &S::compute_all_boundaries)

@@ -346,6 +346,14 @@ class Store {
   Store() = default;
   ~Store() {
     for (void* p : owned_) alv_free(p);
+    if (device_constants() == &cn_dev_) device_constants() = nullptr;
+  }
+  // The constants the device code of the last copy_cn / copy_cn_external
+  // uses (the reference keeps them in __constant__ memory,
+  // constants.hpp:141-143); Runner's stand-alone launchers read them.
+  static Const<float> const*& device_constants() {
+    static Const<float> const* current = nullptr;
+    return current;
   }
   static void set_device(int device) { check(alv_set_device(device)); }
   template <U D, typename M>
@@ -395,11 +403,13 @@ class Store {
     cn_dev_ = cn_;
     cni_dev_ = cni_;
     has_dev_ = true;
+    device_constants() = &cn_dev_;
   }
   void copy_cn_external(Const<float> const& cn, ConstiN const& cni) {
     cn_dev_ = cn;
     cni_dev_ = cni;
     has_dev_ = true;
+    device_constants() = &cn_dev_;
   }
   bool has_device_constants() const { return has_dev_; }
   Const<float> const& device_cn() const { return cn_dev_; }
@@ -410,6 +420,12 @@ class Store {
   Const<float> cn_, cn_dev_;
   ConstiN cni_, cni_dev_;
   bool has_dev_ = false;
+};
+
+// The neighbour grid of a solver (its pid / pid_length pair in the
+// reference): what Runner::launch_make_neighbor_list searches.
+struct SolverGrid {
+  alv_solver* solver = nullptr;
 };
 
 // Runner<float> (runner.hpp:4558-6312): reductions.
@@ -431,6 +447,153 @@ class Runner {
                                                      size_t(offset) * sizeof(M)),
                       size_t(n) * TypeInfo<M>::n, &m));
     return TF(m);
+  }
+
+  // ---- sampling and statistics (runner.hpp:4173-4356, 5698-5890) ---------
+  // Sample neighbour lists are Variable<2, float4> [num_samples][K].
+  // make_neighbor_list for sample points (runner.hpp:5604-5620) on a
+  // solver's grid; the periodic wrap is the solver's.
+  static void launch_make_neighbor_list(Variable<1, float3> const& sample_x, SolverGrid pid,
+                                        Variable<2, float4>& sample_neighbors,
+                                        Variable<1, U>& sample_num_neighbors, U num_samples) {
+    if (!pid.solver) throw std::runtime_error("pid does not belong to a solver");
+    if (sample_x.shape_[0] < num_samples) throw std::out_of_range("sample_x too small");
+    check_samples(sample_neighbors, sample_num_neighbors, num_samples);
+    check(alv_solver_sample_neighbors(pid.solver, f(sample_x), num_samples,
+                                      sample_neighbors.shape_[1],
+                                      static_cast<float*>(sample_neighbors.ptr_),
+                                      static_cast<unsigned*>(sample_num_neighbors.ptr_)));
+  }
+  template <typename Q>
+  static void launch_sample_fluid(Variable<1, float3> const& sample_x,
+                                  Variable<1, float3> const& particle_x,
+                                  Variable<1, float> const& particle_density,
+                                  Variable<1, Q> const& particle_quantity,
+                                  Variable<2, float4> const& sample_neighbors,
+                                  Variable<1, U> const& sample_num_neighbors,
+                                  Variable<1, Q>& sample_quantity, U num_samples) {
+    (void)sample_x;
+    (void)particle_x;
+    check_samples(sample_neighbors, sample_num_neighbors, num_samples);
+    if (sample_quantity.shape_[0] < num_samples) throw std::out_of_range("sample_quantity too small");
+    alv_const c = constants();
+    check(alv_sample_fluid(&c, f(sample_neighbors), u(sample_num_neighbors), sample_neighbors.shape_[1],
+                           num_samples, f(particle_density), f(particle_quantity), TypeInfo<Q>::n,
+                           static_cast<float*>(sample_quantity.ptr_)));
+  }
+  static void launch_sample_fluid_density(Variable<2, float4> const& sample_neighbors,
+                                          Variable<1, U> const& sample_num_neighbors,
+                                          Variable<1, float>& sample_density, U num_samples) {
+    check_samples(sample_neighbors, sample_num_neighbors, num_samples);
+    if (sample_density.shape_[0] < num_samples) throw std::out_of_range("sample_density too small");
+    alv_const c = constants();
+    check(alv_sample_fluid_density(&c, f(sample_neighbors), u(sample_num_neighbors),
+                                   sample_neighbors.shape_[1], num_samples,
+                                   static_cast<float*>(sample_density.ptr_)));
+  }
+  static void launch_sample_density(Variable<1, float3> const& sample_x,
+                                    Variable<2, float4> const& sample_neighbors,
+                                    Variable<1, U> const& sample_num_neighbors,
+                                    Variable<1, float>& sample_density,
+                                    Variable<2, float4> const& sample_boundary_kernel,
+                                    U num_samples) {
+    (void)sample_x;
+    check_samples(sample_neighbors, sample_num_neighbors, num_samples);
+    const U nb = sample_boundary_kernel.shape_[0];
+    if (nb && sample_boundary_kernel.shape_[1] != num_samples)
+      throw std::out_of_range("sample_boundary_kernel must be [num_boundaries][num_samples]");
+    if (sample_density.shape_[0] < num_samples) throw std::out_of_range("sample_density too small");
+    alv_const c = constants();
+    check(alv_sample_position_density(&c, f(sample_neighbors), u(sample_num_neighbors),
+                                      sample_neighbors.shape_[1], num_samples,
+                                      f(sample_boundary_kernel), nb,
+                                      static_cast<float*>(sample_density.ptr_)));
+  }
+  static void launch_sample_velocity(
+      Variable<1, float3> const& sample_x, Variable<1, float3> const& particle_x,
+      Variable<1, float> const& particle_density, Variable<1, float3> const& particle_v,
+      Variable<2, float4> const& sample_neighbors, Variable<1, U> const& sample_num_neighbors,
+      Variable<1, float3>& sample_v, Variable<2, float4> const& sample_boundary,
+      Variable<2, float4> const& sample_boundary_kernel, Variable<1, float3> const& rigid_x,
+      Variable<1, float3> const& rigid_v, Variable<1, float3> const& rigid_omega, U num_samples) {
+    (void)sample_x;
+    (void)particle_x;
+    check_samples(sample_neighbors, sample_num_neighbors, num_samples);
+    const U nb = boundaries(sample_boundary, sample_boundary_kernel, rigid_x, rigid_v, rigid_omega,
+                            num_samples);
+    alv_const c = constants();
+    check(alv_sample_velocity(&c, f(sample_neighbors), u(sample_num_neighbors),
+                              sample_neighbors.shape_[1], num_samples, f(particle_density),
+                              f(particle_v), f(sample_boundary), f(sample_boundary_kernel), nb,
+                              f(rigid_x), f(rigid_v), f(rigid_omega),
+                              static_cast<float*>(sample_v.ptr_)));
+  }
+  static void launch_sample_vorticity(
+      Variable<1, float3> const& sample_x, Variable<1, float3> const& particle_x,
+      Variable<1, float> const& particle_density, Variable<1, float3> const& particle_v,
+      Variable<2, float4> const& sample_neighbors, Variable<1, U> const& sample_num_neighbors,
+      Variable<1, float3> const& sample_v, Variable<1, float3>& sample_vorticity,
+      Variable<2, float4> const& sample_boundary, Variable<2, float4> const& sample_boundary_kernel,
+      Variable<1, float3> const& rigid_x, Variable<1, float3> const& rigid_v,
+      Variable<1, float3> const& rigid_omega, U num_samples) {
+    (void)sample_x;
+    (void)particle_x;
+    check_samples(sample_neighbors, sample_num_neighbors, num_samples);
+    const U nb = boundaries(sample_boundary, sample_boundary_kernel, rigid_x, rigid_v, rigid_omega,
+                            num_samples);
+    alv_const c = constants();
+    check(alv_sample_vorticity(&c, f(sample_neighbors), u(sample_num_neighbors),
+                               sample_neighbors.shape_[1], num_samples, f(particle_density),
+                               f(particle_v), f(sample_v), f(sample_boundary),
+                               f(sample_boundary_kernel), nb, f(rigid_x), f(rigid_v),
+                               f(rigid_omega), static_cast<float*>(sample_vorticity.ptr_)));
+  }
+  // The partial histogram and quantized buffers of the reference's signature
+  // are not needed here.
+  static void launch_histogram256(Variable<1, float> const& v, float v_min, float v_max, U n,
+                                  Variable<1, U>& histogram, Variable<1, float> const* mask = nullptr) {
+    if (v.get_linear_shape() < n || histogram.get_linear_shape() < 256 ||
+        (mask && mask->get_linear_shape() < n))
+      throw std::out_of_range("histogram256: buffers too small");
+    check(alv_histogram256(f(v), mask ? f(*mask) : nullptr, n, v_min, v_max,
+                           static_cast<unsigned*>(histogram.ptr_)));
+  }
+  static TF calculate_kl_divergence(Variable<1, U> const& histogram_p,
+                                    Variable<1, U> const& histogram_q, U n_p, U n_q,
+                                    TF q_lower_bound, U num_bins) {
+    if (histogram_p.get_linear_shape() < num_bins || histogram_q.get_linear_shape() < num_bins)
+      throw std::out_of_range("histograms hold fewer than num_bins bins");
+    float r = 0;
+    check(alv_kl_divergence(u(histogram_p), u(histogram_q), n_p, n_q, q_lower_bound, num_bins, &r));
+    return TF(r);
+  }
+
+ private:
+  template <U D, typename M>
+  static const float* f(Variable<D, M> const& v) {
+    return static_cast<const float*>(v.ptr_);
+  }
+  template <U D, typename M>
+  static const unsigned* u(Variable<D, M> const& v) {
+    return static_cast<const unsigned*>(v.ptr_);
+  }
+  static alv_const constants() {
+    Const<float> const* c = Store::device_constants();
+    if (!c) throw std::runtime_error("no device constants: create a solver or call copy_cn first");
+    return c->to_c();
+  }
+  static void check_samples(Variable<2, float4> const& nbr, Variable<1, U> const& counts, U ns) {
+    if (nbr.shape_[0] < ns || counts.shape_[0] < ns)
+      throw std::out_of_range("sample buffers hold fewer than num_samples samples");
+  }
+  static U boundaries(Variable<2, float4> const& rec, Variable<2, float4> const& ker,
+                      Variable<1, float3> const& rx, Variable<1, float3> const& rv,
+                      Variable<1, float3> const& rw, U ns) {
+    const U nb = rec.shape_[0];
+    if (nb && (rec.shape_[1] != ns || ker.shape_[0] != nb || ker.shape_[1] != ns ||
+               rx.shape_[0] < nb || rv.shape_[0] < nb || rw.shape_[0] < nb))
+      throw std::out_of_range("sample boundary buffers must be [num_boundaries][num_samples]");
+    return nb;
   }
 };
 
@@ -643,7 +806,14 @@ class Pile {
   }
   void set_gravity(float3 g) { check(alv_pile_set_gravity(pile_, &g.x)); }
   void integrate_kinematics(TF dt) { check(alv_pile_integrate_kinematics(pile_, dt)); }
-  void copy_kinematics_to_device() {}
+  void copy_kinematics_to_device() { check(alv_pile_copy_kinematics_to_device(pile_)); }
+  // Device copies of the kinematics (x_device_, v_device_, q_device_,
+  // omega_device_ in the reference, pile.hpp:103-110), as of the last solver
+  // step start or copy_kinematics_to_device.
+  Variable<1, float3> x_device() { return Variable<1, float3>({get_size()}, kin_dev(0)); }
+  Variable<1, float3> v_device() { return Variable<1, float3>({get_size()}, kin_dev(1)); }
+  Variable<1, float4> q_device() { return Variable<1, float4>({get_size()}, kin_dev(2)); }
+  Variable<1, float3> omega_device() { return Variable<1, float3>({get_size()}, kin_dev(3)); }
 
   // Rigid contacts (pile.hpp:826-979).  find_contacts() over every pair is
   // the overload the solvers call after integrating the bodies.
@@ -733,6 +903,11 @@ class Pile {
     float3 x, v, omega, force, torque;
     float4 q;
   };
+  float* kin_dev(int which) {
+    float* p[4] = {nullptr, nullptr, nullptr, nullptr};
+    check(alv_pile_kinematics_device(pile_, &p[0], &p[1], &p[2], &p[3]));
+    return p[which];
+  }
   U find(int i, int j) {
     unsigned n = 0;
     check(alv_pile_find_contacts(pile_, i, j, cn_->contact_tolerance, &n));
@@ -988,6 +1163,20 @@ class SolverBase {
     check(alv_solver_get_grid(solver_, count.data(), start.data(), anomaly.data()));
   }
   alv_solver* handle() { return solver_; }
+  SolverGrid pid() { return SolverGrid{solver_}; }
+  // Solver::sample_all_boundaries (solver.hpp:186-202)
+  void sample_all_boundaries(Variable<1, float3> const& sample_x,
+                             Variable<2, float4>& sample_boundary,
+                             Variable<2, float4>& sample_boundary_kernel, U num_samples) {
+    if (sample_x.shape_[0] < num_samples || sample_boundary.shape_[1] != num_samples ||
+        sample_boundary_kernel.shape_[1] != num_samples ||
+        sample_boundary.shape_[0] < pile_.get_size() ||
+        sample_boundary_kernel.shape_[0] < pile_.get_size())
+      throw std::out_of_range("sample boundary buffers must be [num_boundaries][num_samples]");
+    check(alv_solver_sample_all_boundaries(solver_, static_cast<const float*>(sample_x.ptr_),
+                                           num_samples, static_cast<float*>(sample_boundary.ptr_),
+                                           static_cast<float*>(sample_boundary_kernel.ptr_)));
+  }
 
   const U max_num_particles;
   TF particle_radius = 0;

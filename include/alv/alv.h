@@ -136,6 +136,13 @@ int alv_pile_set_gravity(alv_pile* pile, const float g[3]);
 /* Pile::integrate_kinematics (pile.hpp:794-815) */
 int alv_pile_integrate_kinematics(alv_pile* pile, float dt);
 
+/* Device copies of the bodies' kinematics, refreshed at the start of every
+ * solver step (Pile::copy_kinematics_to_device, pile.hpp:751-756): x, v,
+ * omega are [B][3], q is [B][4] (x, y, z, w).  The pointers stay valid
+ * while the pile lives. */
+int alv_pile_copy_kinematics_to_device(alv_pile* pile);
+int alv_pile_kinematics_device(alv_pile* pile, float** x, float** v, float** q, float** omega);
+
 /* ---- rigid contacts (contact.hpp:7-24, pile.hpp:826-979) ------------------ */
 /* Contact record; the layout of the reference's Contact<float3, float>. */
 typedef struct alv_contact {
@@ -409,6 +416,76 @@ int alv_solver_set_feature_log(alv_solver* solver, int enable);
  * particle order): features dev_x [n][8] and the converged stored
  * (kappa_v, kappa) dev_y [n][2]. */
 int alv_solver_export_training_pairs(alv_solver* solver, float* dev_x, float* dev_y);
+
+/* ---- sampling and statistics --------------------------------------------- */
+/* All pointers are device pointers; launches are on the default stream.
+ * Sample neighbour lists use the reference's format: [num_samples][K]
+ * float4, K = max_num_neighbors_per_particle, entry = (x_sample - x_j,
+ * particle id j in the bits of w).
+ *
+ * make_neighbor_list for sample points (runner.hpp:1869-1911, launched by
+ * Runner::launch_make_neighbor_list, runner.hpp:5604-5620): against the grid
+ * and particle positions of the solver's last neighbour search (a step or
+ * update_particle_neighbors must have run).  A particle whose id equals the
+ * sample index is skipped, as in the reference.  Sets grid_anomaly[2] when a
+ * sample has more than K neighbours (the last slot keeps the latest).
+ * max_neighbors (the row length of sample_neighbors) must equal the solver's
+ * max_num_neighbors_per_particle. */
+int alv_solver_sample_neighbors(alv_solver* solver, const float* sample_x,
+                                unsigned num_samples, unsigned max_neighbors,
+                                float* sample_neighbors, unsigned* sample_num_neighbors);
+/* Solver::sample_all_boundaries (solver.hpp:186-202): boundary records of the
+ * sample points ([B][num_samples] float4 each) with the pile's current
+ * kinematics. */
+int alv_solver_sample_all_boundaries(alv_solver* solver, const float* sample_x,
+                                     unsigned num_samples, float* sample_boundary,
+                                     float* sample_boundary_kernel);
+/* sample_fluid (runner.hpp:4173-4192): sum_j m / rho_j W(x_sample - x_j) q_j
+ * for a scalar (components = 1) or 3-vector quantity in particle order. */
+int alv_sample_fluid(const alv_const* cn, const float* sample_neighbors,
+                     const unsigned* sample_num_neighbors, unsigned max_neighbors,
+                     unsigned num_samples, const float* particle_density,
+                     const float* particle_quantity, unsigned components,
+                     float* sample_quantity);
+/* sample_fluid_density (runner.hpp:4193-4209): rho0 sum_j V W */
+int alv_sample_fluid_density(const alv_const* cn, const float* sample_neighbors,
+                             const unsigned* sample_num_neighbors, unsigned max_neighbors,
+                             unsigned num_samples, float* sample_density);
+/* sample_position_density (runner.hpp:2100-2120): as sample_fluid_density
+ * plus the boundary terms sum_b Psi_b W_b of the sample boundary kernels
+ * ([num_boundaries][num_samples] float4). */
+int alv_sample_position_density(const alv_const* cn, const float* sample_neighbors,
+                                const unsigned* sample_num_neighbors, unsigned max_neighbors,
+                                unsigned num_samples, const float* sample_boundary_kernel,
+                                unsigned num_boundaries, float* sample_density);
+/* compute_sample_velocity (runner.hpp:4211-4241) and
+ * compute_sample_vorticity (runner.hpp:4289-4324); boundary terms use the
+ * sample boundary records and the bodies' x, v, omega ([B][3], e.g. the
+ * pile's device kinematics). */
+int alv_sample_velocity(const alv_const* cn, const float* sample_neighbors,
+                        const unsigned* sample_num_neighbors, unsigned max_neighbors,
+                        unsigned num_samples, const float* particle_density,
+                        const float* particle_v, const float* sample_boundary,
+                        const float* sample_boundary_kernel, unsigned num_boundaries,
+                        const float* rigid_x, const float* rigid_v, const float* rigid_omega,
+                        float* sample_v);
+int alv_sample_vorticity(const alv_const* cn, const float* sample_neighbors,
+                         const unsigned* sample_num_neighbors, unsigned max_neighbors,
+                         unsigned num_samples, const float* particle_density,
+                         const float* particle_v, const float* sample_v,
+                         const float* sample_boundary, const float* sample_boundary_kernel,
+                         unsigned num_boundaries, const float* rigid_x, const float* rigid_v,
+                         const float* rigid_omega, float* sample_vorticity);
+/* Runner::launch_histogram256[_with_mask] (runner.hpp:5849-5890): 256-bin
+ * histogram of floor((v - v_min) * 256 / (v_max - v_min)) mod 256 (negative
+ * arguments in bin 0), counting element i only where mask[i] == 1 when a mask
+ * is given; histogram is [256] u32. */
+int alv_histogram256(const float* v, const float* mask, unsigned n, float v_min, float v_max,
+                     unsigned* histogram);
+/* Runner::calculate_kl_divergence (runner.hpp:4803-4815, KLDivergence
+ * :746-762) of two histograms; the result is written to *result (host). */
+int alv_kl_divergence(const unsigned* histogram_p, const unsigned* histogram_q, unsigned n_p,
+                      unsigned n_q, float q_lower_bound, unsigned num_bins, float* result);
 
 #ifdef __cplusplus
 }

@@ -654,6 +654,7 @@ struct Solver {
   std::vector<V3<T>> force, torque;
   // grid and neighbours
   std::vector<unsigned> cell_count, cell_start, cell_ids;
+  std::vector<V3<T>> grid_x;  // positions at the last grid build (the pid grid's entries)
   std::vector<unsigned> nb_id, num_neighbors;
   std::vector<V3<T>> nb_x;
   unsigned anomaly[3] = {0, 0, 0};
@@ -754,6 +755,7 @@ struct Solver {
       if (key[i] == C) continue;
       cell_ids[cell_start[key[i]] + fill[key[i]]++] = i;
     }
+    grid_x.assign(x.begin(), x.begin() + n);
   }
 
   void make_neighbor_list() {
@@ -801,11 +803,16 @@ struct Solver {
   void compute_particle_boundary(unsigned b) {
     // compute_particle_boundary_analytic<0> (runner.hpp:1982-2002) with
     // compute_volume_and_boundary_x_analytic (runner.hpp:1738-1771)
-    Boundary<T>& B = bnd[b];
 #pragma omp parallel for schedule(static)
     for (long ii = 0; ii < long(n); ++ii) {
       unsigned i = unsigned(ii);
-      V3<T> xi = x[i];
+      size_t o = size_t(b) * max_n + i;
+      point_boundary(b, x[i], boundary[o], boundary_kernel[o]);
+    }
+  }
+  void point_boundary(unsigned b, V3<T> xi, V4<T>& rec, V4<T>& ker) const {
+    const Boundary<T>& B = bnd[b];
+    {
       V3<T> local = rotate(xi - B.x, conjugate(B.q));
       T d = B.sdf.signed_distance(local) * B.sign;
       V3<T> normal = rotate(B.sdf.gradient(local, c.h) * B.sign, B.q);
@@ -824,10 +831,102 @@ struct Solver {
         vol = interpolate(B.volume, cells, N);
       }
       V3<T> kg = displacement_cubic_kernel_grad(xi_bxj, c);
-      size_t o = size_t(b) * max_n + i;
-      boundary[o] = {bx.x, bx.y, bx.z, vol};
-      boundary_kernel[o] = {vol * kg.x, vol * kg.y, vol * kg.z,
-                            vol * dist_cubic_kernel(d, c)};
+      rec = {bx.x, bx.y, bx.z, vol};
+      ker = {vol * kg.x, vol * kg.y, vol * kg.z, vol * dist_cubic_kernel(d, c)};
+    }
+  }
+
+  // ---- sampling (runner.hpp:1869-1911 on sample points, 2100-2120,
+  // 4173-4324) ---------------------------------------------------------------
+  // Sample neighbour lists: ids [ns][K], displacements [ns][K] from the grid
+  // positions; a particle whose id equals the sample index is skipped.
+  void sample_neighbors(const V3<T>* sx, unsigned ns, unsigned* ids, V3<T>* xixj,
+                        unsigned* counts) {
+    const unsigned K = c.kmax;
+    for (unsigned i = 0; i < ns; ++i) {
+      int ip[3];
+      ipos_of(sx[i], ip);
+      unsigned cnt = 0;
+      for (int t = 0; t < 27; ++t) {
+        int np[3] = {ip[0] + t / 9 - 1, ip[1] + (t / 3) % 3 - 1, ip[2] + t % 3 - 1};
+        for (int ax = 0; ax < 3; ++ax) {
+          if (!(c.wrap_mask & (1 << ax))) continue;
+          if (np[ax] >= c.res[ax])
+            np[ax] -= c.res[ax];
+          else if (np[ax] < 0)
+            np[ax] += c.res[ax];
+        }
+        if (!within_grid(np)) continue;
+        unsigned cell = linear(np);
+        for (unsigned s = cell_start[cell]; s < cell_start[cell + 1]; ++s) {
+          unsigned j = cell_ids[s];
+          V3<T> d = wrap_disp(sx[i] - grid_x[j]);
+          if (j != i && length_sqr(d) < c.h2) {
+            size_t slot = size_t(i) * K + std::min(K - 1, cnt);
+            ids[slot] = j;
+            xixj[slot] = d;
+            cnt += 1;
+          }
+        }
+      }
+      counts[i] = std::min(K, cnt);
+      if (cnt > K) anomaly[2] = 1;
+    }
+  }
+  // sample_fluid: sum_j m / rho_j W q_j (q has `dim` components)
+  void sample_fluid(const unsigned* ids, const V3<T>* xixj, const unsigned* counts, unsigned ns,
+                    const T* q, unsigned dim, T* out) const {
+    const unsigned K = c.kmax;
+    for (unsigned i = 0; i < ns; ++i) {
+      T acc[3] = {0, 0, 0};
+      for (unsigned k = 0; k < counts[i]; ++k) {
+        size_t o = size_t(i) * K + k;
+        unsigned j = ids[o];
+        T w = c.mass / density[j] * displacement_cubic_kernel(xixj[o], c);
+        for (unsigned a = 0; a < dim; ++a) acc[a] += w * q[size_t(j) * dim + a];
+      }
+      for (unsigned a = 0; a < dim; ++a) out[size_t(i) * dim + a] = acc[a];
+    }
+  }
+  // sample_fluid_density, or sample_position_density with boundary kernels
+  void sample_density(const V3<T>* xixj, const unsigned* counts, unsigned ns, const V4<T>* ker,
+                      unsigned nb, T* out) const {
+    const unsigned K = c.kmax;
+    for (unsigned i = 0; i < ns; ++i) {
+      T dens = 0;
+      for (unsigned k = 0; k < counts[i]; ++k)
+        dens += c.vol * displacement_cubic_kernel(xixj[size_t(i) * K + k], c);
+      for (unsigned b = 0; b < nb; ++b) dens += ker[size_t(b) * ns + i].w;
+      out[i] = dens * c.rho0;
+    }
+  }
+  // compute_sample_velocity, or compute_sample_vorticity when sv is given
+  void sample_velocity(const unsigned* ids, const V3<T>* xixj, const unsigned* counts,
+                       unsigned ns, const V3<T>* sv, const V4<T>* rec, const V4<T>* ker,
+                       unsigned nb, const V3<T>* rx, const V3<T>* rv, const V3<T>* rw,
+                       V3<T>* out) const {
+    const unsigned K = c.kmax;
+    for (unsigned i = 0; i < ns; ++i) {
+      V3<T> acc{0, 0, 0};
+      V3<T> vi = sv ? sv[i] : V3<T>{0, 0, 0};
+      for (unsigned k = 0; k < counts[i]; ++k) {
+        size_t o = size_t(i) * K + k;
+        unsigned j = ids[o];
+        if (sv)
+          acc += c.mass / density[j] * cross(v[j] - vi, displacement_cubic_kernel_grad(xixj[o], c));
+        else
+          acc += (c.mass / density[j] * displacement_cubic_kernel(xixj[o], c)) * v[j];
+      }
+      for (unsigned b = 0; b < nb; ++b) {
+        V4<T> r = rec[size_t(b) * ns + i], kk = ker[size_t(b) * ns + i];
+        V3<T> bx{r.x, r.y, r.z};
+        V3<T> velj = cross(rw[b], bx - rx[b]) + rv[b];
+        if (sv)
+          acc += cross(velj - vi, V3<T>{kk.x, kk.y, kk.z});
+        else
+          acc += velj * kk.w;
+      }
+      out[i] = acc;
     }
   }
   void compute_all_boundaries() {
@@ -1885,6 +1984,111 @@ int orc_set_rigid(void* h, unsigned b, const double* x, const double* v,
     if (v) B.v = {T(v[0]), T(v[1]), T(v[2])};
     if (q) B.q = {T(q[0]), T(q[1]), T(q[2]), T(q[3])};
     if (w) B.omega = {T(w[0]), T(w[1]), T(w[2])};
+    return 0;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+template <class T>
+std::vector<V3<T>> v3s(const double* a, size_t n) {
+  std::vector<V3<T>> out(n);
+  for (size_t i = 0; i < n; ++i) out[i] = {T(a[3 * i]), T(a[3 * i + 1]), T(a[3 * i + 2])};
+  return out;
+}
+template <class T>
+std::vector<V4<T>> v4s(const double* a, size_t n) {
+  std::vector<V4<T>> out(n);
+  for (size_t i = 0; i < n; ++i)
+    out[i] = {T(a[4 * i]), T(a[4 * i + 1]), T(a[4 * i + 2]), T(a[4 * i + 3])};
+  return out;
+}
+}  // namespace
+
+extern "C" {
+
+int orc_sample_neighbors(void* h, const double* sample_x, unsigned ns, unsigned* ids,
+                         double* xixj, unsigned* counts) {
+  return dispatch(h, [&](auto& s) {
+    using T = typename std::remove_reference<decltype(s.dt)>::type;
+    if (s.grid_x.size() != s.n) return -4;
+    auto sx = v3s<T>(sample_x, ns);
+    std::vector<V3<T>> d(size_t(ns) * s.c.kmax, V3<T>{0, 0, 0});
+    s.sample_neighbors(sx.data(), ns, ids, d.data(), counts);
+    for (size_t k = 0; k < d.size(); ++k) {
+      xixj[3 * k] = d[k].x;
+      xixj[3 * k + 1] = d[k].y;
+      xixj[3 * k + 2] = d[k].z;
+    }
+    return 0;
+  });
+}
+
+int orc_sample_boundary(void* h, const double* sample_x, unsigned ns, double* rec, double* ker) {
+  return dispatch(h, [&](auto& s) {
+    using T = typename std::remove_reference<decltype(s.dt)>::type;
+    auto sx = v3s<T>(sample_x, ns);
+    for (unsigned b = 0; b < s.bnd.size(); ++b)
+      for (unsigned i = 0; i < ns; ++i) {
+        V4<T> r, k;
+        s.point_boundary(b, sx[i], r, k);
+        double* ro = rec + 4 * (size_t(b) * ns + i);
+        double* ko = ker + 4 * (size_t(b) * ns + i);
+        ro[0] = r.x; ro[1] = r.y; ro[2] = r.z; ro[3] = r.w;
+        ko[0] = k.x; ko[1] = k.y; ko[2] = k.z; ko[3] = k.w;
+      }
+    return 0;
+  });
+}
+
+int orc_sample_fluid(void* h, const unsigned* ids, const double* xixj, const unsigned* counts,
+                     unsigned ns, const double* quantity, unsigned dim, double* out) {
+  return dispatch(h, [&](auto& s) {
+    using T = typename std::remove_reference<decltype(s.dt)>::type;
+    auto d = v3s<T>(xixj, size_t(ns) * s.c.kmax);
+    std::vector<T> q(size_t(s.n) * dim), o(size_t(ns) * dim);
+    for (size_t k = 0; k < q.size(); ++k) q[k] = T(quantity[k]);
+    s.sample_fluid(ids, d.data(), counts, ns, q.data(), dim, o.data());
+    for (size_t k = 0; k < o.size(); ++k) out[k] = o[k];
+    return 0;
+  });
+}
+
+int orc_sample_density(void* h, const double* xixj, const unsigned* counts, unsigned ns,
+                       const double* ker, unsigned nb, double* out) {
+  return dispatch(h, [&](auto& s) {
+    using T = typename std::remove_reference<decltype(s.dt)>::type;
+    auto d = v3s<T>(xixj, size_t(ns) * s.c.kmax);
+    auto k = nb ? v4s<T>(ker, size_t(nb) * ns) : std::vector<V4<T>>();
+    std::vector<T> o(ns);
+    s.sample_density(d.data(), counts, ns, k.data(), nb, o.data());
+    for (unsigned i = 0; i < ns; ++i) out[i] = o[i];
+    return 0;
+  });
+}
+
+int orc_sample_velocity(void* h, const unsigned* ids, const double* xixj, const unsigned* counts,
+                        unsigned ns, const double* sample_v, const double* rec, const double* ker,
+                        unsigned nb, const double* rigid_x, const double* rigid_v,
+                        const double* rigid_omega, double* out) {
+  return dispatch(h, [&](auto& s) {
+    using T = typename std::remove_reference<decltype(s.dt)>::type;
+    auto d = v3s<T>(xixj, size_t(ns) * s.c.kmax);
+    std::vector<V3<T>> sv = sample_v ? v3s<T>(sample_v, ns) : std::vector<V3<T>>();
+    auto r = nb ? v4s<T>(rec, size_t(nb) * ns) : std::vector<V4<T>>();
+    auto k = nb ? v4s<T>(ker, size_t(nb) * ns) : std::vector<V4<T>>();
+    auto rx = nb ? v3s<T>(rigid_x, nb) : std::vector<V3<T>>();
+    auto rv = nb ? v3s<T>(rigid_v, nb) : std::vector<V3<T>>();
+    auto rw = nb ? v3s<T>(rigid_omega, nb) : std::vector<V3<T>>();
+    std::vector<V3<T>> o(ns);
+    s.sample_velocity(ids, d.data(), counts, ns, sample_v ? sv.data() : nullptr, r.data(),
+                      k.data(), nb, rx.data(), rv.data(), rw.data(), o.data());
+    for (unsigned i = 0; i < ns; ++i) {
+      out[3 * i] = o[i].x;
+      out[3 * i + 1] = o[i].y;
+      out[3 * i + 2] = o[i].z;
+    }
     return 0;
   });
 }

@@ -127,6 +127,23 @@ int orc_set_gravity_pile(void* h, const double* g);
  * iiwi_off_diag, iiwj_diag, iiwj_off_diag, friction, nkninv, pmax, goalu,
  * impulse_sum.  The step finds and solves contacts after integrating the
  * bodies whenever any body has collision vertices. */
+/* Sampling against the grid of the last neighbour search (make_neighbor_list
+ * on sample points, runner.hpp:1869-1911; sample kernels runner.hpp:2100-2120,
+ * 4173-4324).  ids and xixj are [ns][Kmax] (xixj packed xyz), counts [ns];
+ * boundary records [B][ns][4]; quantity [n][dim] in particle order.
+ * orc_sample_velocity computes the sample vorticity when sample_v is given. */
+int orc_sample_neighbors(void* h, const double* sample_x, unsigned ns, unsigned* ids,
+                         double* xixj, unsigned* counts);
+int orc_sample_boundary(void* h, const double* sample_x, unsigned ns, double* rec, double* ker);
+int orc_sample_fluid(void* h, const unsigned* ids, const double* xixj, const unsigned* counts,
+                     unsigned ns, const double* quantity, unsigned dim, double* out);
+int orc_sample_density(void* h, const double* xixj, const unsigned* counts, unsigned ns,
+                       const double* ker, unsigned nb, double* out);
+int orc_sample_velocity(void* h, const unsigned* ids, const double* xixj, const unsigned* counts,
+                        unsigned ns, const double* sample_v, const double* rec, const double* ker,
+                        unsigned nb, const double* rigid_x, const double* rigid_v,
+                        const double* rigid_omega, double* out);
+
 #define ORC_CONTACT_FIELDS 31
 int orc_set_collision(void* h, unsigned b, double restitution, double friction,
                       const double* xyz, unsigned n);

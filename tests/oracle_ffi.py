@@ -112,6 +112,13 @@ def lib():
         L.orc_set_collision.argtypes = [vp, ctypes.c_uint, ctypes.c_double, ctypes.c_double, dp,
                                         ctypes.c_uint]
         L.orc_set_contact_params.argtypes = [vp, ctypes.c_double, ctypes.c_uint]
+        up = ctypes.POINTER(ctypes.c_uint)
+        L.orc_sample_neighbors.argtypes = [vp, dp, ctypes.c_uint, up, dp, up]
+        L.orc_sample_boundary.argtypes = [vp, dp, ctypes.c_uint, dp, dp]
+        L.orc_sample_fluid.argtypes = [vp, up, dp, up, ctypes.c_uint, dp, ctypes.c_uint, dp]
+        L.orc_sample_density.argtypes = [vp, dp, up, ctypes.c_uint, dp, ctypes.c_uint, dp]
+        L.orc_sample_velocity.argtypes = [vp, up, dp, up, ctypes.c_uint, dp, dp, dp, ctypes.c_uint,
+                                          dp, dp, dp, dp]
         L.orc_find_contacts.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint)]
         L.orc_solve_contacts.argtypes = [vp]
         L.orc_get_contacts.argtypes = [vp, dp, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint)]
@@ -325,6 +332,67 @@ class Oracle:
                 args.append(_dptr(arr))
                 setattr(self, "_keep_%d" % len(args), arr)
         self.L.orc_set_rigid(self.h, b, *args)
+
+    # ---- sampling against the grid of the last neighbour search ----
+    def sample_neighbors(self, sample_x):
+        sx = np.ascontiguousarray(sample_x, np.float64).reshape(-1, 3)
+        ns = sx.shape[0]
+        ids = np.zeros((ns, self.kmax), np.uint32)
+        xixj = np.zeros((ns, self.kmax, 3))
+        counts = np.zeros(ns, np.uint32)
+        up = ctypes.POINTER(ctypes.c_uint)
+        rc = self.L.orc_sample_neighbors(self.h, _dptr(sx), ns, ids.ctypes.data_as(up), _dptr(xixj),
+                                         counts.ctypes.data_as(up))
+        if rc != 0:
+            raise RuntimeError("orc_sample_neighbors: %d" % rc)
+        return ids, xixj, counts
+
+    def sample_boundary(self, sample_x):
+        sx = np.ascontiguousarray(sample_x, np.float64).reshape(-1, 3)
+        ns, nb = sx.shape[0], len(self.scene.boundaries)
+        rec = np.zeros((nb, ns, 4))
+        ker = np.zeros((nb, ns, 4))
+        self.L.orc_sample_boundary(self.h, _dptr(sx), ns, _dptr(rec), _dptr(ker))
+        return rec, ker
+
+    def sample_fluid(self, ids, xixj, counts, quantity):
+        q = np.ascontiguousarray(quantity, np.float64)
+        dim = 1 if q.ndim == 1 else q.shape[1]
+        ns = counts.shape[0]
+        out = np.zeros((ns, dim))
+        up = ctypes.POINTER(ctypes.c_uint)
+        self.L.orc_sample_fluid(self.h, np.ascontiguousarray(ids).ctypes.data_as(up),
+                                _dptr(np.ascontiguousarray(xixj, np.float64)),
+                                np.ascontiguousarray(counts).ctypes.data_as(up), ns, _dptr(q), dim,
+                                _dptr(out))
+        return out[:, 0] if q.ndim == 1 else out
+
+    def sample_density(self, xixj, counts, ker=None):
+        ns = counts.shape[0]
+        out = np.zeros(ns)
+        nb = 0 if ker is None else ker.shape[0]
+        k = np.zeros(1) if ker is None else np.ascontiguousarray(ker, np.float64)
+        up = ctypes.POINTER(ctypes.c_uint)
+        self.L.orc_sample_density(self.h, _dptr(np.ascontiguousarray(xixj, np.float64)),
+                                  np.ascontiguousarray(counts).ctypes.data_as(up), ns, _dptr(k), nb,
+                                  _dptr(out))
+        return out
+
+    def sample_velocity(self, ids, xixj, counts, rec, ker, rigid_x, rigid_v, rigid_omega,
+                        sample_v=None):
+        ns = counts.shape[0]
+        nb = rec.shape[0]
+        out = np.zeros((ns, 3))
+        up = ctypes.POINTER(ctypes.c_uint)
+        keep = [np.ascontiguousarray(a, np.float64) for a in (xixj, rec, ker, rigid_x, rigid_v,
+                                                              rigid_omega)]
+        sv = None if sample_v is None else np.ascontiguousarray(sample_v, np.float64)
+        self.L.orc_sample_velocity(self.h, np.ascontiguousarray(ids).ctypes.data_as(up),
+                                   _dptr(keep[0]), np.ascontiguousarray(counts).ctypes.data_as(up),
+                                   ns, ctypes.POINTER(ctypes.c_double)() if sv is None else _dptr(sv),
+                                   _dptr(keep[1]), _dptr(keep[2]), nb, _dptr(keep[3]),
+                                   _dptr(keep[4]), _dptr(keep[5]), _dptr(out))
+        return out
 
     def find_contacts(self, i=-1, j=-1):
         n = ctypes.c_uint(0)
