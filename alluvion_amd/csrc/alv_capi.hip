@@ -1076,7 +1076,7 @@ static int solver_alloc(alv_solver* S) {
   ALV_ALLOC(A.kfac, cap);
   ALV_ALLOC(A.feat, size_t(cap) * 2);
   ALV_ALLOC(A.tiles, size_t(grid_for(cap)) + 1);
-  A.use_tiles = 1;
+  A.use_tiles = 0;  // LDS tiles are opt-in (alv_solver_set_tiles); the autotuner tries them
   ALV_ALLOC(A.kpred, cap);
   A.kmax = K;
   ALV_ALLOC(A.nbr, nbr_capacity(K, cap));

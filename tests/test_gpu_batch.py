@@ -99,6 +99,7 @@ def test_lds_tiles_match_global_gathers():
     sc_b = scenes.dam_break_rigid(nx=32, ny=32, nz=16, body_resolution=16)
     dpa, _, sa, _, _ = scenes.build_solver(sc_a)
     dpb, _, sb, _, _ = scenes.build_solver(sc_b)
+    sa.set_tiles(True)
     sb.set_tiles(False)
     for step in range(25):
         sa.step()
@@ -110,6 +111,7 @@ def test_lds_tiles_match_global_gathers():
                                       dpb.coat(getattr(sb, f)).get(), err_msg=f)
     dpa, _, ba, _, _ = scenes.build_batch(make_scenes())
     dpb, _, bb, _, _ = scenes.build_batch(make_scenes())
+    ba.set_tiles(True)
     bb.set_tiles(False)
     for step in range(15):
         ba.step()
@@ -131,6 +133,7 @@ def test_lds_tiles_with_particles_outside_the_grid():
     assert outside > 0
     dpa, _, sa, _, _ = scenes.build_solver(scene())
     dpb, _, sb, _, _ = scenes.build_solver(scene())
+    sa.set_tiles(True)
     sb.set_tiles(False)
     for step in range(5):
         sa.step()
