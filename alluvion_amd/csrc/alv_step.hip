@@ -462,18 +462,41 @@ __global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int 
   float dens = p.vol * p.w0;
   float sgk = 0.0f;
   vec3 gpi = mk(0.f, 0.f, 0.f);
+  // Distances from x_i to the faces of its own cell.  A neighbour cell whose
+  // box is farther than h holds no neighbour, so it is not scanned; the
+  // threshold carries a margin of 1e-4 h^2 over the rounding of positions
+  // and cell assignment (about 1e-6 h), and skipped cells contribute nothing
+  // to the list, whose order is unchanged.
+  float lo_d[3], hi_d[3];
+  {
+    const float xs[3] = {xi.x, xi.y, xi.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float c0 = float(ip[a] + p.off[a]) * p.h;
+      lo_d[a] = xs[a] - c0;
+      hi_d[a] = c0 + p.h - xs[a];
+    }
+  }
+  const float reach2 = p.h2 * 1.0001f;
   // The 27 cells in the reference order (dx outer, dz inner): for each
   // (dx, dy) row the up-to-3 cells along z are contiguous in the sorted
   // order unless a periodic wrap splits them, so each row is 1-3 spans.
   for (int r = 0; r < 9; ++r) {
-    int nx = ip[0] + r / 3 - 1, ny = ip[1] + r % 3 - 1;
+    const int ddx = r / 3 - 1, ddy = r % 3 - 1;
+    const float ex = ddx < 0 ? lo_d[0] : (ddx > 0 ? hi_d[0] : 0.0f);
+    const float ey = ddy < 0 ? lo_d[1] : (ddy > 0 ? hi_d[1] : 0.0f);
+    const float base = ex * ex + ey * ey;
+    if (!(base < reach2)) continue;
+    const int dz_lo = base + lo_d[2] * lo_d[2] < reach2 ? -1 : 0;
+    const int dz_hi = base + hi_d[2] * hi_d[2] < reach2 ? 1 : 0;
+    int nx = ip[0] + ddx, ny = ip[1] + ddy;
     if (p.wrap_mask & 1) nx = nx >= p.res[0] ? nx - p.res[0] : (nx < 0 ? nx + p.res[0] : nx);
     if (p.wrap_mask & 2) ny = ny >= p.res[1] ? ny - p.res[1] : (ny < 0 ? ny + p.res[1] : ny);
     if (nx < 0 || nx >= p.res[0] || ny < 0 || ny >= p.res[1]) continue;
     const unsigned row = (unsigned(nx) * unsigned(p.res[1]) + unsigned(ny)) * unsigned(p.res[2]);
     unsigned span_b[3], span_e[3], nspan = 0;
     int prev = -2;
-    for (int dz = -1; dz <= 1; ++dz) {
+    for (int dz = dz_lo; dz <= dz_hi; ++dz) {
       int nz = ip[2] + dz;
       if (p.wrap_mask & 4) nz = nz >= p.res[2] ? nz - p.res[2] : (nz < 0 ? nz + p.res[2] : nz);
       if (nz < 0 || nz >= p.res[2]) continue;
@@ -488,7 +511,8 @@ __global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int 
       prev = nz;
     }
     // candidates in batches: all positions of a batch are loaded before
-    // any is tested (one memory latency per batch), in ascending j
+    // any is tested (one memory latency per batch); the tests are branch-free
+    // and give a bit mask of hits, which are then appended in ascending j
     for (unsigned q = 0; q < nspan; ++q) {
       const unsigned jb = span_b[q], je = span_e[q];
       for (unsigned j0 = jb; j0 < je; j0 += kCB) {
@@ -498,23 +522,25 @@ __global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int 
           const unsigned jc = min(j0 + u, je - 1u);
           xc[u] = tiled_ ? tile_.X(jc) : A.x[jc];
         }
+        unsigned hits = 0u;
 #pragma unroll
         for (int u = 0; u < kCB; ++u) {
-          const unsigned j = j0 + u;
-          if (j >= je || j == s) continue;
-          vec3 d = wrapd<W>(xi - ld3(xc[u]), p);
-          float r2 = length_sqr(d);
-          if (r2 < p.h2) {
-            // slot min(cnt, K-1): the last slot keeps the latest neighbour
-            const unsigned slot = min(cnt, K - 1);
-            const unsigned c4 = slot & 3u;
-            grp.x = c4 == 0 ? j : grp.x;
-            grp.y = c4 == 1 ? j : grp.y;
-            grp.z = c4 == 2 ? j : grp.z;
-            grp.w = c4 == 3 ? j : grp.w;
-            if (c4 == 3u && cnt == slot) A.nbr[nbr_group(A, slot >> 2, s)] = grp;
-            ++cnt;
-          }
+          const vec3 d = wrapd<W>(xi - ld3(xc[u]), p);
+          const bool h = (j0 + u < je) & (j0 + u != s) & (length_sqr(d) < p.h2);
+          hits |= unsigned(h) << u;
+        }
+        while (hits) {
+          const unsigned j = j0 + unsigned(__builtin_ctz(hits));
+          hits &= hits - 1u;
+          // slot min(cnt, K-1): the last slot keeps the latest neighbour
+          const unsigned slot = min(cnt, K - 1);
+          const unsigned c4 = slot & 3u;
+          grp.x = c4 == 0 ? j : grp.x;
+          grp.y = c4 == 1 ? j : grp.y;
+          grp.z = c4 == 2 ? j : grp.z;
+          grp.w = c4 == 3 ? j : grp.w;
+          if (c4 == 3u && cnt == slot) A.nbr[nbr_group(A, slot >> 2, s)] = grp;
+          ++cnt;
         }
       }
     }
