@@ -285,6 +285,8 @@ struct alv_pile {
   bool verts_dirty = true;
   float* verts_dev = nullptr;
   size_t verts_cap = 0;  // floats
+  unsigned* vbody_dev = nullptr;  // body of every collision vertex (pellet)
+  size_t vbody_cap = 0;
   ContactJob* jobs_dev = nullptr;
   size_t jobs_cap = 0;
   BoundaryDev* bodies_dev = nullptr;
@@ -300,6 +302,11 @@ struct alv_pile {
   // the device copy is asked for
   std::vector<float> kin_snap;
   bool kin_dirty = false;
+  // VolumeMethod::pellets (pile.hpp:16,124-130): the collision vertices of
+  // every body are its pellets; no volume maps are built
+  int volume_method = ALV_VOLUME_MAP;
+  unsigned max_pellets = 0;
+  std::vector<std::pair<unsigned, unsigned>> ident;  // hint_identical_sequence runs
   float* kin_dev = nullptr;
   size_t kin_cap = 0;
   ~alv_pile() {
@@ -307,6 +314,7 @@ struct alv_pile {
     for (unsigned char* f : cell_nz) (void)hipFree(f);
     if (peirce) (void)hipFree(peirce);
     if (verts_dev) (void)hipFree(verts_dev);
+    if (vbody_dev) (void)hipFree(vbody_dev);
     if (jobs_dev) (void)hipFree(jobs_dev);
     if (bodies_dev) (void)hipFree(bodies_dev);
     if (inertia_dev) (void)hipFree(inertia_dev);
@@ -335,6 +343,11 @@ struct alv_solver {
   int kind = ALV_SOLVER_DF;
   alv_pile* pile = nullptr;
   unsigned cap = 0;
+  // particle-sampled boundaries: pellet rows after the fluid's, and the
+  // launch table of the pellets (one scene, the fluid's constants)
+  unsigned pcap = 0;
+  SimParams* ppar_dev = nullptr;
+  unsigned short* pblk_scene = nullptr;
   alv_const cn;
   alv_consti cni;
   alv_solver_settings set;
@@ -559,12 +572,20 @@ void alv_const_set_particle_attr(alv_const* cn, float radius, float mass, float 
 // Pile
 // ---------------------------------------------------------------------------
 int alv_pile_create(alv_pile** out, unsigned max_num_contacts, int volume_method) {
+  return alv_pile_create_ex(out, max_num_contacts, volume_method, 10000u);
+}
+
+int alv_pile_create_ex(alv_pile** out, unsigned max_num_contacts, int volume_method,
+                       unsigned max_num_pellets) {
   if (!out) return fail(ALV_ERR_ARG, "out is null");
-  if (volume_method != ALV_VOLUME_MAP)
-    return fail(ALV_ERR_UNSUPPORTED, "only VolumeMethod.volume_map is supported");
+  if (volume_method != ALV_VOLUME_MAP && volume_method != ALV_PELLETS)
+    return fail(ALV_ERR_ARG, "unknown volume method");
   alv_pile* p = new (std::nothrow) alv_pile();
   if (!p) return fail(ALV_ERR_STATE, "out of host memory");
   p->max_num_contacts = max_num_contacts;
+  p->volume_method = volume_method;
+  // max_num_pellets_ is 0 for volume maps (pile.hpp:162-163)
+  p->max_pellets = volume_method == ALV_PELLETS ? max_num_pellets : 0u;
   float table[120];
   peirce_table(table);
   hipError_t e = hipMalloc(&p->peirce, sizeof(table));
@@ -652,11 +673,15 @@ int alv_pile_add(alv_pile* pile, const alv_const* cn, const alv_distance* dist,
               size_t(res[0] + 1) * res[1] * (res[2] + 1) +
               size_t(res[0] + 1) * (res[1] + 1) * res[2];
   size_t nodes = nv + 2 * ne;
+  const bool pellets = pile->volume_method == ALV_PELLETS;
+  if (pellets) nodes = 0;  // build_grid builds no volume map (pile.hpp:523)
   if (nodes >= (1ull << 32)) return fail(ALV_ERR_UNSUPPORTED, "volume map too large");
   B.geom.num_nodes = unsigned(nodes);
   float* vol = nullptr;
-  int rc = dev_alloc(&vol, nodes);
-  if (rc != ALV_OK) return rc;
+  if (!pellets) {
+    int rc = dev_alloc(&vol, nodes);
+    if (rc != ALV_OK) return rc;
+  }
   B.volume = vol;
   for (int a = 0; a < 3; ++a) {
     B.x[a] = x ? x[a] : 0.f;
@@ -671,16 +696,18 @@ int alv_pile_add(alv_pile* pile, const alv_const* cn, const alv_distance* dist,
   memset(&cni, 0, sizeof(cni));
   SimParams p = make_params(*cn, cni, 0, 0);
   const unsigned chunk = 1u << 20;
-  for (unsigned b = 0; b < B.geom.num_nodes; b += chunk) {
+  for (unsigned b = 0; !pellets && b < B.geom.num_nodes; b += chunk) {
     unsigned e = std::min(B.geom.num_nodes, b + chunk);
     launch_volume_field_analytic(p, B, vol, pile->peirce, b, e, 0);
     ALV_HIP(hipGetLastError());
   }
   unsigned char* flags = nullptr;
-  ALV_HIP(hipMalloc(&flags, std::max<size_t>(size_t(res[0]) * res[1] * res[2], 1)));
-  launch_map_flags(B.geom, vol, flags, nullptr);
-  ALV_HIP(hipGetLastError());
-  ALV_HIP(hipDeviceSynchronize());
+  if (!pellets) {
+    ALV_HIP(hipMalloc(&flags, std::max<size_t>(size_t(res[0]) * res[1] * res[2], 1)));
+    launch_map_flags(B.geom, vol, flags, nullptr);
+    ALV_HIP(hipGetLastError());
+    ALV_HIP(hipDeviceSynchronize());
+  }
   B.cell_nz = flags;
   pile->bnd.push_back(B);
   pile->volumes.push_back(vol);
@@ -813,14 +840,89 @@ hipError_t reserve_dev(T** ptr, size_t* cap, size_t count) {
   return e;
 }
 
+// Uploads the collision vertices (body frame, concatenated) when they
+// changed; vbase[b] is body b's first vertex.
+int upload_vertices(alv_pile* P, std::vector<unsigned>& vbase, hipStream_t stream) {
+  const unsigned nb = unsigned(P->bnd.size());
+  P->coll_verts.resize(nb);
+  vbase.assign(nb + 1, 0);
+  for (unsigned b = 0; b < nb; ++b) vbase[b + 1] = vbase[b] + unsigned(P->coll_verts[b].size() / 3);
+  if (P->verts_dirty) {
+    std::vector<float> all;
+    all.reserve(size_t(vbase[nb]) * 3);
+    std::vector<unsigned> body;
+    body.reserve(vbase[nb]);
+    for (unsigned b = 0; b < nb; ++b) {
+      all.insert(all.end(), P->coll_verts[b].begin(), P->coll_verts[b].end());
+      body.insert(body.end(), vbase[b + 1] - vbase[b], b);
+    }
+    ALV_HIP(reserve_dev(&P->verts_dev, &P->verts_cap, all.size()));
+    ALV_HIP(reserve_dev(&P->vbody_dev, &P->vbody_cap, body.size()));
+    if (!all.empty()) {
+      ALV_HIP(hipMemcpyAsync(P->verts_dev, all.data(), all.size() * sizeof(float),
+                             hipMemcpyHostToDevice, stream));
+      ALV_HIP(hipMemcpyAsync(P->vbody_dev, body.data(), body.size() * sizeof(unsigned),
+                             hipMemcpyHostToDevice, stream));
+    }
+    ALV_HIP(hipStreamSynchronize(stream));
+    P->verts_dirty = false;
+  }
+  return ALV_OK;
+}
+
+// Runs the contact kernel over `jobs` with the body kinematics of `bodies`
+// and leaves the kept contacts in P->contacts.
+int run_collision(alv_pile* P, const std::vector<ContactJob>& jobs, unsigned candidates,
+                  const BoundaryDev* bodies, const float* verts, float tolerance,
+                  hipStream_t stream) {
+  const unsigned nb = unsigned(P->bnd.size());
+  P->num_found = 0;
+  P->contacts.clear();
+  if (jobs.empty()) return ALV_OK;
+  ALV_HIP(reserve_dev(&P->bodies_dev, &P->bodies_cap, nb));
+  ALV_HIP(reserve_dev(&P->inertia_dev, &P->inertia_cap, size_t(nb) * 5));
+  ALV_HIP(reserve_dev(&P->jobs_dev, &P->jobs_cap, jobs.size()));
+  if (!P->count_dev) ALV_HIP(hipMalloc(&P->count_dev, sizeof(unsigned)));
+  if (!P->contacts_dev && P->max_num_contacts)
+    ALV_HIP(hipMalloc(&P->contacts_dev, size_t(P->max_num_contacts) * sizeof(alv_contact)));
+  std::vector<float> inertia(size_t(nb) * 5);
+  for (unsigned b = 0; b < nb; ++b) {
+    for (int k = 0; k < 3; ++k) inertia[b * 3 + k] = P->inertia[b][k];
+    inertia[nb * 3 + b * 2] = P->restitution[b];
+    inertia[nb * 3 + b * 2 + 1] = P->friction[b];
+  }
+  ALV_HIP(hipMemcpyAsync(P->bodies_dev, bodies, nb * sizeof(BoundaryDev), hipMemcpyHostToDevice,
+                         stream));
+  ALV_HIP(hipMemcpyAsync(P->inertia_dev, inertia.data(), inertia.size() * sizeof(float),
+                         hipMemcpyHostToDevice, stream));
+  ALV_HIP(hipMemcpyAsync(P->jobs_dev, jobs.data(), jobs.size() * sizeof(ContactJob),
+                         hipMemcpyHostToDevice, stream));
+  launch_collision(P->jobs_dev, unsigned(jobs.size()), candidates, nb, P->bodies_dev,
+                   P->inertia_dev, verts, tolerance, P->max_num_contacts, P->contacts_dev,
+                   P->count_dev, stream);
+  ALV_HIP(hipGetLastError());
+  unsigned found = 0;
+  ALV_HIP(hipMemcpyAsync(&found, P->count_dev, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  ALV_HIP(hipStreamSynchronize(stream));
+  P->num_found = found;
+  const unsigned kept = std::min(found, P->max_num_contacts);
+  P->contacts.resize(kept);
+  if (kept) {
+    ALV_HIP(hipMemcpyAsync(P->contacts.data(), P->contacts_dev, kept * sizeof(alv_contact),
+                           hipMemcpyDeviceToHost, stream));
+    ALV_HIP(hipStreamSynchronize(stream));
+  }
+  return ALV_OK;
+}
+
 // Pile::find_contacts (pile.hpp:826-904) on `stream`; see alv.h for the
 // meaning of i and j.  Leaves the kept contacts in pile->contacts.
 int find_contacts(alv_pile* P, int i, int j, float tolerance, hipStream_t stream) {
   const unsigned nb = unsigned(P->bnd.size());
   if (i >= int(nb) || j >= int(nb)) return fail(ALV_ERR_ARG, "bad boundary id");
-  P->coll_verts.resize(nb);
-  std::vector<unsigned> vbase(nb + 1, 0);
-  for (unsigned b = 0; b < nb; ++b) vbase[b + 1] = vbase[b] + unsigned(P->coll_verts[b].size() / 3);
+  std::vector<unsigned> vbase;
+  int rc = upload_vertices(P, vbase, stream);
+  if (rc != ALV_OK) return rc;
   std::vector<ContactJob> jobs;
   unsigned candidates = 0;
   const unsigned i0 = i < 0 ? 0 : unsigned(i), i1 = i < 0 ? nb : unsigned(i) + 1;
@@ -837,56 +939,66 @@ int find_contacts(alv_pile* P, int i, int j, float tolerance, hipStream_t stream
       J.vbase = vbase[a];
       J.nv = nv;
       J.cbase = candidates;
-      J.restitution = P->restitution[a] * P->restitution[b];
-      J.friction = P->friction[a] + P->friction[b];
+      J.nj = 1;
       jobs.push_back(J);
       candidates += nv;
     }
   }
-  P->num_found = 0;
-  P->contacts.clear();
-  if (jobs.empty()) return ALV_OK;
-  if (P->verts_dirty) {
-    std::vector<float> all;
-    all.reserve(size_t(vbase[nb]) * 3);
-    for (const auto& v : P->coll_verts) all.insert(all.end(), v.begin(), v.end());
-    ALV_HIP(reserve_dev(&P->verts_dev, &P->verts_cap, all.size()));
-    ALV_HIP(hipMemcpyAsync(P->verts_dev, all.data(), all.size() * sizeof(float),
-                           hipMemcpyHostToDevice, stream));
-    ALV_HIP(hipStreamSynchronize(stream));
-    P->verts_dirty = false;
-  }
-  ALV_HIP(reserve_dev(&P->bodies_dev, &P->bodies_cap, nb));
-  ALV_HIP(reserve_dev(&P->inertia_dev, &P->inertia_cap, size_t(nb) * 3));
-  ALV_HIP(reserve_dev(&P->jobs_dev, &P->jobs_cap, jobs.size()));
-  if (!P->count_dev) ALV_HIP(hipMalloc(&P->count_dev, sizeof(unsigned)));
-  if (!P->contacts_dev && P->max_num_contacts)
-    ALV_HIP(hipMalloc(&P->contacts_dev, size_t(P->max_num_contacts) * sizeof(alv_contact)));
-  std::vector<float> inertia(size_t(nb) * 3);
+  return run_collision(P, jobs, candidates, P->bnd.data(), P->verts_dev, tolerance, stream);
+}
+
+// Pile::find_contacts(particle_x, max_num_beads), pellet branch
+// (pile.hpp:858-886; collision_test_with_pellets, runner.hpp:3627-3727):
+// the world pellets (`world`, pellet-major, as placed at the step start)
+// against every body j, or against each hinted run of identical bodies in
+// one job, with the kinematics of the last device copy.  Contacts come out
+// in (run, pellet, j) order.  A run uses each body's own distance, which is
+// the run's first body's when the hint holds.
+int find_contacts_pellets(alv_pile* P, const float* world, float tolerance, hipStream_t stream) {
+  const unsigned nb = unsigned(P->bnd.size());
+  std::vector<unsigned> vbase;
+  int rc = upload_vertices(P, vbase, stream);
+  if (rc != ALV_OK) return rc;
+  if (P->kin_snap.size() != size_t(nb) * 13) P->snapshot_kinematics();
+  std::vector<BoundaryDev> bodies(P->bnd);
   for (unsigned b = 0; b < nb; ++b)
-    for (int k = 0; k < 3; ++k) inertia[b * 3 + k] = P->inertia[b][k];
-  ALV_HIP(hipMemcpyAsync(P->bodies_dev, P->bnd.data(), nb * sizeof(BoundaryDev),
-                         hipMemcpyHostToDevice, stream));
-  ALV_HIP(hipMemcpyAsync(P->inertia_dev, inertia.data(), inertia.size() * sizeof(float),
-                         hipMemcpyHostToDevice, stream));
-  ALV_HIP(hipMemcpyAsync(P->jobs_dev, jobs.data(), jobs.size() * sizeof(ContactJob),
-                         hipMemcpyHostToDevice, stream));
-  launch_collision(P->jobs_dev, unsigned(jobs.size()), candidates, P->bodies_dev, P->inertia_dev,
-                   P->verts_dev, tolerance, P->max_num_contacts, P->contacts_dev, P->count_dev,
-                   stream);
-  ALV_HIP(hipGetLastError());
-  unsigned found = 0;
-  ALV_HIP(hipMemcpyAsync(&found, P->count_dev, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-  ALV_HIP(hipStreamSynchronize(stream));
-  P->num_found = found;
-  const unsigned kept = std::min(found, P->max_num_contacts);
-  P->contacts.resize(kept);
-  if (kept) {
-    ALV_HIP(hipMemcpyAsync(P->contacts.data(), P->contacts_dev, kept * sizeof(alv_contact),
-                           hipMemcpyDeviceToHost, stream));
-    ALV_HIP(hipStreamSynchronize(stream));
+    for (int a = 0; a < 4; ++a) {
+      if (a < 3) {
+        bodies[b].x[a] = P->kin_snap[b * 3 + a];
+        bodies[b].v[a] = P->kin_snap[nb * 3 + b * 3 + a];
+        bodies[b].omega[a] = P->kin_snap[nb * 10 + b * 3 + a];
+      }
+      bodies[b].q[a] = P->kin_snap[nb * 6 + b * 4 + a];
+    }
+  std::vector<ContactJob> jobs;
+  unsigned candidates = 0;
+  unsigned jn = 0, seq = 0;
+  while (jn < nb) {
+    unsigned jb = jn, je = jn + 1;
+    if (seq < P->ident.size() && jn == P->ident[seq].first && jn < P->ident[seq].second) {
+      jb = P->ident[seq].first;
+      je = std::min(nb, P->ident[seq].second);
+      ++seq;
+    }
+    if (P->bnd[jb].type != ALV_DISTANCE_BOX_SHELL) {
+      for (unsigned a = 0; a < nb; ++a) {
+        const unsigned nv = vbase[a + 1] - vbase[a];
+        if (nv == 0) continue;
+        ContactJob J{};
+        J.i = a;
+        J.j = jb;
+        J.vbase = vbase[a];
+        J.nv = nv;
+        J.cbase = candidates;
+        J.nj = je - jb;
+        J.flags = kJobWorld | kJobSkip;
+        jobs.push_back(J);
+        candidates += nv * (je - jb);
+      }
+    }
+    jn = je;
   }
-  return ALV_OK;
+  return run_collision(P, jobs, candidates, bodies.data(), world, tolerance, stream);
 }
 
 // Pile::solve_contacts (pile.hpp:906-979): sequential impulse sweeps with a
@@ -937,6 +1049,12 @@ void solve_contacts(alv_pile* P) {
   }
 }
 
+unsigned pellet_count(const alv_pile* P) {
+  size_t n = 0;
+  for (const auto& v : P->coll_verts) n += v.size() / 3;
+  return unsigned(n);
+}
+
 bool has_collision_vertices(const alv_pile* P) {
   for (const auto& v : P->coll_verts)
     if (!v.empty()) return true;
@@ -950,9 +1068,47 @@ int alv_pile_set_collision_vertices(alv_pile* pile, unsigned b, const float* xyz
   if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
   if (n && !xyz) return fail(ALV_ERR_ARG, "null vertices");
   pile->coll_verts.resize(pile->bnd.size());
+  if (pile->volume_method == ALV_PELLETS) {
+    size_t total = n;
+    for (size_t k = 0; k < pile->coll_verts.size(); ++k)
+      if (k != b) total += pile->coll_verts[k].size() / 3;
+    if (total > pile->max_pellets)  // pile.hpp:266-275
+      return fail(ALV_ERR_ARG, "Cannot add pellets because the new pellet count " +
+                                   std::to_string(total) + " exceeds the limit " +
+                                   std::to_string(pile->max_pellets));
+  }
   pile->coll_verts[b].assign(xyz, xyz + size_t(n) * 3);
   pile->verts_dirty = true;
   return ALV_OK;
+}
+
+int alv_pile_num_pellets(const alv_pile* pile, unsigned* num_pellets, unsigned* max_num_pellets) {
+  if (!pile) return fail(ALV_ERR_ARG, "null pile");
+  if (num_pellets) *num_pellets = pile->volume_method == ALV_PELLETS ? pellet_count(pile) : 0u;
+  if (max_num_pellets) *max_num_pellets = pile->max_pellets;
+  return ALV_OK;
+}
+
+int alv_pile_hint_identical_sequence(alv_pile* pile, unsigned begin, unsigned end) {
+  if (!pile) return fail(ALV_ERR_ARG, "null pile");
+  pile->ident.push_back({begin, end});
+  return ALV_OK;
+}
+
+int alv_pile_find_contacts_pellets(alv_pile* pile, const float* particle_x,
+                                   unsigned max_num_beads, float contact_tolerance,
+                                   unsigned* num_contacts) {
+  if (!pile) return fail(ALV_ERR_ARG, "null pile");
+  int rc;
+  if (pile->volume_method == ALV_PELLETS) {
+    if (!particle_x) return fail(ALV_ERR_ARG, "null particle_x");
+    rc = find_contacts_pellets(pile, particle_x + size_t(max_num_beads) * 3, contact_tolerance,
+                               nullptr);
+  } else {
+    rc = find_contacts(pile, -1, -1, contact_tolerance, nullptr);
+  }
+  if (rc == ALV_OK && num_contacts) *num_contacts = pile->num_found;
+  return rc;
 }
 
 int alv_pile_find_contacts(alv_pile* pile, int i, int j, float contact_tolerance,
@@ -1007,7 +1163,7 @@ int alv_pile_get_volume_grid(const alv_pile* pile, unsigned b, float** nodes,
   if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
   if (nodes) {
     *nodes = pile->volumes[b];
-    pile->maps_dirty = true;  // the caller may write the nodes
+    if (pile->volumes[b]) pile->maps_dirty = true;  // the caller may write the nodes
   }
   if (num_nodes) *num_nodes = pile->bnd[b].geom.num_nodes;
   return ALV_OK;
@@ -1040,11 +1196,15 @@ static int solver_alloc(alv_solver* S) {
   DevArrays& A = S->A;
   memset(&A, 0, sizeof(A));
   A.cap = cap;
+  // positions, velocities and pressure hold the pellets after the fluid
+  // (solver.hpp:30-36)
+  const unsigned pcap = S->pcap;
+  const size_t UC = size_t(cap) + pcap;
   int rc = ALV_OK;
 #define ALV_ALLOC(ptr, cnt) \
   if ((rc = S->alloc(&(ptr), (cnt))) != ALV_OK) return rc;
-  ALV_ALLOC(A.ux, size_t(cap) * 3);
-  ALV_ALLOC(A.uv, size_t(cap) * 3);
+  ALV_ALLOC(A.ux, UC * 3);
+  ALV_ALLOC(A.uv, UC * 3);
   ALV_ALLOC(A.ua, size_t(cap) * 3);
   ALV_ALLOC(A.udensity, cap);
   ALV_ALLOC(A.ualpha, cap);
@@ -1052,7 +1212,7 @@ static int solver_alloc(alv_solver* S) {
   ALV_ALLOC(A.ukappa_v, cap);
   ALV_ALLOC(A.urho_adv, cap);
   ALV_ALLOC(A.unnb, cap);
-  ALV_ALLOC(A.upressure, cap);
+  ALV_ALLOC(A.upressure, UC);
   ALV_ALLOC(A.key, cap);
   ALV_ALLOC(A.slot, cap);
   ALV_ALLOC(A.sorted_id, cap);
@@ -1107,6 +1267,27 @@ static int solver_alloc(alv_solver* S) {
   ALV_ALLOC(A.body_sums, std::max<size_t>(1, max_bnd) * 6);
   ALV_ALLOC(A.body_partials, std::max<size_t>(1, max_bnd) * 256 * 6);
   ALV_ALLOC(A.ctl, NS);
+  if (pcap) {
+    A.pcap = pcap;
+    const size_t pspan = size_t(grid_for(pcap)) * kBlock;
+    ALV_ALLOC(A.pkey, pcap);
+    ALV_ALLOC(A.pslot, pcap);
+    ALV_ALLOC(A.psorted_id, pspan);
+    ALV_ALLOC(A.pcell_count, CT);
+    ALV_ALLOC(A.pcell_start, CT + 1);
+    ALV_ALLOC(A.pscan_tmp, (CT + 4095) / 4096 + 1);
+    ALV_ALLOC(A.px, pspan);
+    ALV_ALLOC(A.pv, pspan);
+    ALV_ALLOC(A.pbody, pspan);
+    ALV_ALLOC(A.pp, pspan);
+    ALV_ALLOC(A.pnbr, nbr_capacity(K, cap));
+    ALV_ALLOC(A.pnnb, cap);
+    ALV_ALLOC(A.pbnbr, nbr_capacity(K, pcap));
+    ALV_ALLOC(A.pbnnb, pspan);
+    ALV_ALLOC(S->ppar_dev, 1);
+    ALV_ALLOC(S->pblk_scene, size_t(grid_for(pcap)) + 1);
+    ALV_HIP(hipMemset(S->pblk_scene, 0, (size_t(grid_for(pcap)) + 1) * sizeof(unsigned short)));
+  }
 #undef ALV_ALLOC
   S->bnd_capacity = unsigned(std::max<size_t>(1, all_bnd));
   // scene of every particle block and every cell block
@@ -1152,11 +1333,14 @@ int alv_solver_create(alv_solver** out, int kind, alv_pile* pile, unsigned max_n
     return fail(ALV_ERR_ARG, "max_num_neighbors_per_particle is zero");
   size_t C = size_t(cni->grid_res[0]) * cni->grid_res[1] * cni->grid_res[2];
   if (C == 0 || C > (1ull << 31)) return fail(ALV_ERR_ARG, "grid_res out of range");
+  if (pile->volume_method == ALV_PELLETS && kind != ALV_SOLVER_I)
+    return fail(ALV_ERR_UNSUPPORTED, "pellet boundaries step with the ISPH solver (SolverI) only");
   alv_solver* S = new (std::nothrow) alv_solver();
   if (!S) return fail(ALV_ERR_STATE, "out of host memory");
   S->kind = kind;
   S->pile = pile;
   S->cap = max_num_particles;
+  S->pcap = pile->max_pellets;
   S->cn = *cn;
   S->cni = *cni;
   S->num_cells = unsigned(C);
@@ -1221,6 +1405,8 @@ int alv_solver_create_batch(alv_solver** out, int kind, unsigned num_scenes,
   for (unsigned k = 0; k < num_scenes; ++k) {
     const alv_scene_desc& d = descs[k];
     if (!d.pile) return fail(ALV_ERR_ARG, "scene without a pile");
+    if (d.pile->volume_method == ALV_PELLETS)
+      return fail(ALV_ERR_UNSUPPORTED, "batched scenes use volume-map boundaries");
     if (d.num_particles == 0) return fail(ALV_ERR_ARG, "scene without particles");
     for (const auto& B : d.pile->bnd)
       if (B.dynamic)
@@ -1382,6 +1568,12 @@ int solver_launch(alv_solver* S, Launch* L) {
     p.span = sc.span;
     p.cell_base = sc.cell_base;
     p.bnd_base = sc.bnd_base;
+    if (sc.pile->volume_method == ALV_PELLETS) {
+      p.pellet_mode = 1;
+      p.np = pellet_count(sc.pile);
+      if (p.np > S->pcap)
+        return fail(ALV_ERR_STATE, "pellets were added beyond max_num_pellets of the solver");
+    }
     if (memcmp(&p, &S->par_host[k], sizeof(p)) != 0) {
       S->par_host[k] = p;
       changed = true;
@@ -1458,6 +1650,29 @@ bool all_done(const alv_solver* S, bool divergence) {
     if (!(divergence ? c.div_done : c.dens_done)) return false;
   }
   return true;
+}
+
+// Launch set of the pellets of a pellet solver (one scene): the fluid's
+// constants with n = the pellet count; uploads the pile's pellets.
+int pellet_launch(alv_solver* S, const Launch& L, Launch* Lp) {
+  std::vector<unsigned> vbase;
+  int rc = upload_vertices(S->pile, vbase, S->stream);
+  if (rc != ALV_OK) return rc;
+  S->A.pverts = S->pile->verts_dev;
+  S->A.pvbody = S->pile->vbody_dev;
+  SimParams pp = L.p0;
+  pp.n = pp.np;
+  pp.first = 0;
+  pp.span = grid_for(pp.np) * kBlock;
+  ALV_HIP(hipMemcpyAsync(S->ppar_dev, &pp, sizeof(pp), hipMemcpyHostToDevice, S->stream));
+  // the copy reads pp before this function returns
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  *Lp = L;
+  Lp->T = SceneTab{S->ppar_dev, S->pblk_scene, S->cblk_scene};
+  Lp->pblocks = grid_for(pp.np);
+  Lp->active = pp.np;
+  Lp->p0 = pp;
+  return ALV_OK;
 }
 
 // grid + gather + boundary records + neighbour list + density (+ factor).
@@ -1567,7 +1782,11 @@ int finish_step(alv_solver* S, const Launch& L) {
   alv_pile_integrate_kinematics(pile, S->set.dt);
   // rigid contacts (solver_df.hpp:356-357, solver_ii.hpp:240-241)
   if (has_collision_vertices(pile)) {
-    rc = find_contacts(pile, -1, -1, S->cn.contact_tolerance, S->stream);
+    if (pile->volume_method == ALV_PELLETS)  // pellets as placed at the step start
+      rc = find_contacts_pellets(pile, S->A.ux + size_t(S->cap) * 3, S->cn.contact_tolerance,
+                                 S->stream);
+    else
+      rc = find_contacts(pile, -1, -1, S->cn.contact_tolerance, S->stream);
     if (rc != ALV_OK) return rc;
     solve_contacts(pile);
   }
@@ -1718,6 +1937,27 @@ int alv_solver_step(alv_solver* S) {
       if (L.active) launch_df_finish(L, A, dens, st);
     });
     if (rc != ALV_OK) return rc;
+  } else if (S->kind == ALV_SOLVER_I && L.p0.pellet_mode) {
+    // SolverI::step (solver_i.hpp:79-330) with pellets
+    Launch Lp;
+    rc = pellet_launch(S, L, &Lp);
+    if (rc != ALV_OK) return rc;
+    launch_transform_pellets(Lp, A, st);  // transform_all_pellets
+    launch_pellet_grid(Lp, A, st);
+    rc = neighbor_phase(S, L, 0);
+    if (rc != ALV_OK) return rc;
+    launch_pellet_lists(L, Lp, A, st);
+    launch_ctl_reset(L, A, st);
+    if (L.active) launch_pellet_viscosity(L, A, st);
+    launch_update_dt(L, A, st);
+    if (L.active) {
+      launch_is_prepare_pellets(L, A, st);
+      rc = run_loop(S, false, S->set.max_density_solve, &S->last_dens,
+                    [&] { launch_is_iteration_pellets(L, Lp, A, st); });
+      if (rc != ALV_OK) return rc;
+      launch_is_settle(L, A, st);
+      launch_is_finish_pellets(L, Lp, A, st);
+    }
   } else if (S->kind == ALV_SOLVER_I) {
     // SolverI::step (solver_i.hpp:79-330), volume-map boundaries
     rc = neighbor_phase(S, L, 0);
@@ -1811,10 +2051,36 @@ int alv_solver_update_particle_neighbors(alv_solver* S) {
   if (rc != ALV_OK) return rc;
   rc = upload_step_inputs(S);
   if (rc != ALV_OK) return rc;
+  Launch Lp;
+  if (L.p0.pellet_mode) {
+    // the pellets at their current particle_x rows (solver.hpp:229-259)
+    rc = pellet_launch(S, L, &Lp);
+    if (rc != ALV_OK) return rc;
+    launch_pellet_grid(Lp, S->A, S->stream);
+  }
   rc = neighbor_phase(S, L, S->kind == ALV_SOLVER_DF ? 1 : 0);
   if (rc != ALV_OK) return rc;
+  if (L.p0.pellet_mode) launch_pellet_lists(L, Lp, S->A, S->stream);
+  ALV_HIP(hipGetLastError());
   ALV_HIP(hipStreamSynchronize(S->stream));
   S->dirty |= ALV_F_DENSITY | ALV_F_NNB | ALV_F_ALPHA;
+  return ALV_OK;
+}
+
+int alv_solver_transform_all_pellets(alv_solver* S) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  Launch L;
+  int rc = solver_launch(S, &L);
+  if (rc != ALV_OK) return rc;
+  if (!L.p0.pellet_mode) return ALV_OK;
+  rc = upload_step_inputs(S);
+  if (rc != ALV_OK) return rc;
+  Launch Lp;
+  rc = pellet_launch(S, L, &Lp);
+  if (rc != ALV_OK) return rc;
+  launch_transform_pellets(Lp, S->A, S->stream);
+  ALV_HIP(hipGetLastError());
+  ALV_HIP(hipStreamSynchronize(S->stream));
   return ALV_OK;
 }
 
@@ -1872,8 +2138,8 @@ int alv_solver_get_field(alv_solver* S, int field, void** dptr, size_t* n_elemen
   size_t n = N;
   unsigned comp = 1;
   switch (field) {
-    case ALV_FIELD_X: ptr = A.ux; comp = 3; break;
-    case ALV_FIELD_V: ptr = A.uv; comp = 3; break;
+    case ALV_FIELD_X: ptr = A.ux; comp = 3; n = N + S->pcap; break;
+    case ALV_FIELD_V: ptr = A.uv; comp = 3; n = N + S->pcap; break;
     case ALV_FIELD_A: ptr = A.ua; comp = 3; need = ALV_F_A; break;
     case ALV_FIELD_DENSITY: ptr = A.udensity; need = ALV_F_DENSITY; break;
     case ALV_FIELD_DFSPH_FACTOR: ptr = A.ualpha; need = ALV_F_ALPHA; break;
@@ -1881,7 +2147,7 @@ int alv_solver_get_field(alv_solver* S, int field, void** dptr, size_t* n_elemen
     case ALV_FIELD_KAPPA_V: ptr = A.ukappa_v; break;
     case ALV_FIELD_DENSITY_ADV: ptr = A.urho_adv; need = ALV_F_RHO_ADV; break;
     case ALV_FIELD_NUM_NEIGHBORS: ptr = A.unnb; need = ALV_F_NNB; break;
-    case ALV_FIELD_PRESSURE: ptr = A.upressure; break;
+    case ALV_FIELD_PRESSURE: ptr = A.upressure; n = N + S->pcap; break;
     case ALV_FIELD_PID_LENGTH: ptr = A.cell_count; n = S->num_cells; break;
     case ALV_FIELD_GRID_ANOMALY: ptr = A.anomaly; n = 3; break;
     case ALV_FIELD_CELL_START: ptr = A.cell_start; n = size_t(S->num_cells) + 1; break;
@@ -1925,6 +2191,45 @@ int alv_solver_get_neighbors(alv_solver* S, unsigned* ids, unsigned* counts) {
   if (counts)
     ALV_HIP(hipMemcpy(counts, S->export_counts, n * sizeof(unsigned),
                       hipMemcpyDeviceToHost));
+  return ALV_OK;
+}
+
+int alv_solver_get_pellet_neighbors(alv_solver* S, int which, unsigned* ids, unsigned* counts) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  if (!S->pcap) return fail(ALV_ERR_STATE, "the solver has no pellets");
+  if (which != 0 && which != 1) return fail(ALV_ERR_ARG, "which is 0 or 1");
+  Launch L;
+  int rc = solver_launch(S, &L);
+  if (rc != ALV_OK) return rc;
+  const DevArrays& A = S->A;
+  const unsigned K = S->cni.max_num_neighbors_per_particle, G = nbr_groups(K);
+  const unsigned n = S->set.num_particles, np = L.p0.np;
+  const unsigned rows = which == 0 ? S->cap : S->pcap, live = which == 0 ? n : np;
+  const size_t pspan = size_t(grid_for(S->pcap)) * kBlock;
+  std::vector<unsigned> fsorted(S->cap), psorted(pspan), cnt(which == 0 ? S->cap : pspan);
+  std::vector<uint4> lists(nbr_capacity(K, rows));
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  ALV_HIP(hipMemcpy(fsorted.data(), A.sorted_id, fsorted.size() * sizeof(unsigned),
+                    hipMemcpyDeviceToHost));
+  ALV_HIP(hipMemcpy(psorted.data(), A.psorted_id, psorted.size() * sizeof(unsigned),
+                    hipMemcpyDeviceToHost));
+  ALV_HIP(hipMemcpy(cnt.data(), which == 0 ? A.pnnb : A.pbnnb, cnt.size() * sizeof(unsigned),
+                    hipMemcpyDeviceToHost));
+  ALV_HIP(hipMemcpy(lists.data(), which == 0 ? A.pnbr : A.pbnbr, lists.size() * sizeof(uint4),
+                    hipMemcpyDeviceToHost));
+  if (ids) std::fill(ids, ids + size_t(rows) * K, ~0u);
+  if (counts) std::fill(counts, counts + rows, 0u);
+  for (unsigned s = 0; s < live; ++s) {
+    const unsigned row = which == 0 ? fsorted[s] : psorted[s];
+    const unsigned c = cnt[s];
+    if (counts) counts[row] = c;
+    if (!ids) continue;
+    for (unsigned t = 0; t < c; ++t) {
+      const uint4& g = lists[(size_t(s >> 6) * G + (t >> 2)) * 64u + (s & 63u)];
+      const unsigned j = (&g.x)[t & 3u];
+      ids[size_t(row) * K + t] = which == 0 ? S->cap + psorted[j] : fsorted[j];
+    }
+  }
   return ALV_OK;
 }
 

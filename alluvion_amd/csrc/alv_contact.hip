@@ -65,17 +65,22 @@ __device__ inline void put3(float* dst, vec3 v) {
   dst[2] = v.z;
 }
 
-// Tests candidate c (vertex `vert` of job J); true and a filled record when
-// the vertex is inside body j (dist < 0).
-__device__ bool test_vertex(const ContactJob& J, const BoundaryDev* bodies, const float* inertia,
-                            const float* verts, unsigned vert, float tolerance, alv_contact* out) {
-  const BoundaryDev& Bi = bodies[J.i];
-  const BoundaryDev& Bj = bodies[J.j];
+// Tests vertex `vert` of body i against body j; true and a filled record
+// when the vertex is inside body j (dist < 0).  World vertices (pellets,
+// collision_test_with_pellets, runner.hpp:3627-3727) are already placed;
+// body-frame vertices are placed with body i's kinematics.  `inertia`
+// holds [B][3] principal inertia, then [B][2] (restitution, friction).
+__device__ bool test_vertex(unsigned i, unsigned j, unsigned flags, unsigned nb,
+                            const BoundaryDev* bodies, const float* inertia, const float* verts,
+                            unsigned vert, float tolerance, alv_contact* out) {
+  const BoundaryDev& Bi = bodies[i];
+  const BoundaryDev& Bj = bodies[j];
+  if ((flags & kJobSkip) && (i == j || (Bi.mass == 0.f && Bj.mass == 0.f))) return false;
   const float4 qi = make_float4(Bi.q[0], Bi.q[1], Bi.q[2], Bi.q[3]);
   const float4 qj = make_float4(Bj.q[0], Bj.q[1], Bj.q[2], Bj.q[3]);
   const vec3 xi = mk(Bi.x[0], Bi.x[1], Bi.x[2]), xj = mk(Bj.x[0], Bj.x[1], Bj.x[2]);
-  const vec3 local = mk(verts[3 * vert], verts[3 * vert + 1], verts[3 * vert + 2]);
-  const vec3 cp_i = rotate(local, qi) + xi;
+  const vec3 vx = mk(verts[3 * vert], verts[3 * vert + 1], verts[3 * vert + 2]);
+  const vec3 cp_i = (flags & kJobWorld) ? vx : rotate(vx, qi) + xi;
   const vec3 wrt_j = rotate(cp_i - xj, conjugate(qj));
   const float dist = sdf_eval(Bj, wrt_j) * Bj.sign - tolerance;
   vec3 n = sdf_gradient(Bj, wrt_j, 1e-3f) * Bj.sign;
@@ -90,8 +95,9 @@ __device__ bool test_vertex(const ContactJob& J, const BoundaryDev* bodies, cons
   const float u_rel_n = dot(nw, u_rel);
   vec3 t = u_rel - u_rel_n * nw;
   if (length_sqr(t) > 1e-6f) t = t * (1.0f / sqrtf(length_sqr(t)));
-  const float* Ii = inertia + 3 * J.i;
-  const float* Ij = inertia + 3 * J.j;
+  const float* Ii = inertia + 3 * i;
+  const float* Ij = inertia + 3 * j;
+  const float* rf = inertia + 3 * nb;
   vec3 di, oi, dj, oj;
   congruent_matrix(mk(1 / Ii[0], 1 / Ii[1], 1 / Ii[2]), qi, &di, &oi);
   congruent_matrix(mk(1 / Ij[0], 1 / Ij[1], 1 / Ij[2]), qj, &dj, &oj);
@@ -99,8 +105,8 @@ __device__ bool test_vertex(const ContactJob& J, const BoundaryDev* bodies, cons
   contact_k(r_i, Bi.mass, di, oi, &kdi, &koi);
   contact_k(r_j, Bj.mass, dj, oj, &kdj, &koj);
   const vec3 kd = kdi + kdj, ko = koi + koj;
-  out->i = J.i;
-  out->j = J.j;
+  out->i = i;
+  out->j = j;
   put3(out->cp_i, cp_i);
   put3(out->cp_j, cp_j);
   put3(out->n, nw);
@@ -109,16 +115,16 @@ __device__ bool test_vertex(const ContactJob& J, const BoundaryDev* bodies, cons
   put3(out->iiwi_off_diag, oi);
   put3(out->iiwj_diag, dj);
   put3(out->iiwj_off_diag, oj);
-  out->friction = J.friction;
+  out->friction = rf[2 * i + 1] + rf[2 * j + 1];
   out->nkninv = 1 / dot(nw, sym_apply(nw, kd, ko));
   out->pmax = 1 / dot(t, sym_apply(t, kd, ko)) * dot(u_rel, t);
-  out->goalu = u_rel_n < 0 ? -J.restitution * u_rel_n : 0.0f;
+  out->goalu = u_rel_n < 0 ? -(rf[2 * i] * rf[2 * j]) * u_rel_n : 0.0f;
   out->impulse_sum = 0;
   return true;
 }
 
 __global__ __launch_bounds__(kContactThreads) void k_collision(
-    const ContactJob* jobs, unsigned num_jobs, unsigned num_candidates,
+    const ContactJob* jobs, unsigned num_jobs, unsigned num_candidates, unsigned nb,
     const BoundaryDev* bodies, const float* inertia, const float* verts, float tolerance,
     unsigned max_contacts, alv_contact* contacts, unsigned* count) {
   __shared__ unsigned wave_sum[kContactWaves];
@@ -134,7 +140,9 @@ __global__ __launch_bounds__(kContactThreads) void k_collision(
       unsigned k = 0;
       while (k + 1 < num_jobs && jobs[k + 1].cbase <= c) ++k;
       const ContactJob J = jobs[k];
-      hit = test_vertex(J, bodies, inertia, verts, J.vbase + (c - J.cbase), tolerance, &rec);
+      const unsigned local = c - J.cbase;
+      hit = test_vertex(J.i, J.j + local % J.nj, J.flags, nb, bodies, inertia, verts,
+                        J.vbase + local / J.nj, tolerance, &rec);
     }
     const unsigned long long mask = __ballot(hit);
     const unsigned before = __popcll(mask & ((1ull << lane) - 1ull));
@@ -158,12 +166,12 @@ __global__ __launch_bounds__(kContactThreads) void k_collision(
 }  // namespace
 
 void launch_collision(const ContactJob* jobs, unsigned num_jobs, unsigned num_candidates,
-                      const BoundaryDev* bodies, const float* inertia, const float* verts,
-                      float tolerance, unsigned max_contacts, alv_contact* contacts,
-                      unsigned* count, hipStream_t s) {
+                      unsigned nb, const BoundaryDev* bodies, const float* inertia,
+                      const float* verts, float tolerance, unsigned max_contacts,
+                      alv_contact* contacts, unsigned* count, hipStream_t s) {
   hipLaunchKernelGGL(k_collision, dim3(1), dim3(kContactThreads), 0, s, jobs, num_jobs,
-                     num_candidates, bodies, inertia, verts, tolerance, max_contacts, contacts,
-                     count);
+                     num_candidates, nb, bodies, inertia, verts, tolerance, max_contacts,
+                     contacts, count);
 }
 
 }  // namespace alv

@@ -8,20 +8,26 @@
 
 namespace alv {
 
-// One (i, j) body pair: the collision vertices of body i (vbase .. vbase+nv
-// in the pile's vertex array) against body j's distance.  Candidates are
-// numbered pair by pair, cbase being the first candidate of this pair.
+// One body i against a run of nj consecutive bodies j .. j+nj-1 (nj = 1 for
+// a plain (i, j) pair): the collision vertices vbase .. vbase+nv of the
+// pile's vertex array.  Candidates are numbered job by job, cbase being the
+// first of this job, and inside a job vertex-major: (vertex, j).
+enum : unsigned {
+  kJobWorld = 1u,  // vertices are world positions (pellets) rather than body-frame
+  kJobSkip = 2u,   // reject i == j and pairs of two massless bodies per candidate
+};
 struct ContactJob {
   unsigned i, j;
   unsigned vbase, nv, cbase;
-  float restitution;  // restitution_i * restitution_j
-  float friction;     // friction_i + friction_j
+  unsigned nj;
+  unsigned flags;
   unsigned pad;
 };
 
+// inertia: [B][3] principal inertia, then [B][2] (restitution, friction)
 void launch_collision(const ContactJob* jobs, unsigned num_jobs, unsigned num_candidates,
-                      const BoundaryDev* bodies, const float* inertia, const float* verts,
-                      float tolerance, unsigned max_contacts, alv_contact* contacts,
-                      unsigned* count, hipStream_t s);
+                      unsigned nb, const BoundaryDev* bodies, const float* inertia,
+                      const float* verts, float tolerance, unsigned max_contacts,
+                      alv_contact* contacts, unsigned* count, hipStream_t s);
 
 }  // namespace alv

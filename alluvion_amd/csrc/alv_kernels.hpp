@@ -78,6 +78,27 @@ struct DevArrays {
   float* adv_density;
   float* density_err;
   float4* pacc;
+  // Particle-sampled boundaries (pellets).  The pile's pellets (body
+  // frame, user order) are placed in world space into ux/uv rows
+  // [cap, cap + np) at the step start and sorted by cell on a grid of their
+  // own (same cells as the fluid's); lists hold sorted indices.
+  unsigned pcap;  // max_num_pellets
+  const float* pverts;  // [pcap][3] body-frame pellets (user order)
+  const unsigned* pvbody;  // body of every pellet (user order)
+  unsigned* pkey;
+  unsigned* pslot;
+  unsigned* psorted_id;  // sorted pellet -> user pellet index
+  unsigned* pcell_count;
+  unsigned* pcell_start;
+  unsigned* pscan_tmp;
+  float4* px;  // sorted pellet positions
+  float4* pv;  // sorted pellet velocities
+  unsigned* pbody;  // sorted pellet -> body
+  float* pp;  // pellet pressure (MLS extrapolation), sorted
+  uint4* pnbr;  // pellet lists of the fluid particles (nbr_group layout)
+  unsigned* pnnb;
+  uint4* pbnbr;  // fluid lists of the pellets (nbr_group layout over pellets)
+  unsigned* pbnnb;
   // reductions and control
   double* partials;  // per-block FP64 partial sums
   float* fpartials;  // per-block float maxima
@@ -224,6 +245,24 @@ void launch_is_prepare(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_is_iteration(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_is_settle(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_ii_finish(const Launch& L, const DevArrays& A, hipStream_t s);
+
+// Pellets (Lp: the launch set of the pellets, one thread per pellet).
+// transform_all_pellets: places the pellets (ux/uv rows cap + k) with the
+// body kinematics of A.bnd
+void launch_transform_pellets(const Launch& Lp, const DevArrays& A, hipStream_t s);
+// sorts the pellets at their ux rows into their grid (empty when none)
+void launch_pellet_grid(const Launch& Lp, const DevArrays& A, hipStream_t s);
+// the fluid particles' pellet lists and density with pellets, then the
+// pellets' fluid lists
+void launch_pellet_lists(const Launch& L, const Launch& Lp, const DevArrays& A, hipStream_t s);
+void launch_pellet_viscosity(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_is_prepare_pellets(const Launch& L, const DevArrays& A, hipStream_t s);
+void launch_is_iteration_pellets(const Launch& L, const Launch& Lp, const DevArrays& A,
+                                 hipStream_t s);
+// MLS pressure of the pellets from the final fluid pressure, the pressure
+// accelerations with pellets and the integration
+void launch_is_finish_pellets(const Launch& L, const Launch& Lp, const DevArrays& A,
+                              hipStream_t s);
 
 // rigid bodies: per-body FP64 force / torque sums into A.body_sums
 void launch_body_sums(const Launch& L, const DevArrays& A, hipStream_t s);

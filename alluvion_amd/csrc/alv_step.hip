@@ -56,6 +56,12 @@ __device__ inline void batch_ids(const DevArrays& A, unsigned s, unsigned nn, un
   j[3] = t0 + 3 < nn ? g.w : s;
 }
 
+// slot t of a grouped list (nbr_group layout) of particle s
+__device__ inline unsigned pellet_at(const uint4* list, const DevArrays& A, unsigned s,
+                                     unsigned t) {
+  return reinterpret_cast<const unsigned*>(list + nbr_group(A, t >> 2, s))[t & 3u];
+}
+
 // The w lane of the sorted position carries the stiffness of the next
 // stiffness pass (kappa_v, kfac or kappa, written by the pass that computes
 // it), so a stiffness pass gathers x_j and k_j with one load.
@@ -325,6 +331,17 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SceneTab T, DevArrays A
     A.kappa_v[s] = A.ukappa_v[i];
   } else {
     A.pressure[s] = A.upressure[i];
+  }
+  if (p.pellet_mode) {
+    // no boundary records: the pellet passes add the body forces
+    for (unsigned b = 0; b < p.num_boundaries; ++b) {
+      if (!A.bnd[b].dynamic) continue;
+      const size_t o = size_t(b) * A.cap + s;
+      A.bforce[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+      A.btorque[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    A.bmask[s] = 0u;
+    return;
   }
   unsigned mask = 0;
   for (unsigned b = 0; b < p.num_boundaries; ++b) {
@@ -1405,7 +1422,7 @@ __global__ __launch_bounds__(256) void k_ii_finalize(SceneTab T, DevArrays A) {
   }
 }
 // compute_pressure_accels (runner.hpp:2774-2817)
-template <bool W>
+template <bool W, bool PL>
 __global__ __launch_bounds__(256) void k_ii_pressure_accel(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   if (s >= p.first + p.n) return;
@@ -1437,7 +1454,28 @@ __global__ __launch_bounds__(256) void k_ii_pressure_accel(SceneTab T, DevArrays
       }
     }
   }
-  unsigned mask = A.bmask[s];
+  if constexpr (PL) {
+    // compute_pressure_accels_with_pellets (runner.hpp:2819-2870)
+    const unsigned npn = A.pnnb[s];
+    for (unsigned t = 0; t < npn; ++t) {
+      const unsigned sp = pellet_at(A.pnbr, A, s, t);
+      const vec3 xixj = wrapd<W>(xi - ld3(A.px[sp]), p);
+      const float dpj = A.pp[sp] / (p.rho0 * p.rho0);
+      const vec3 acc = (p.mass * (dpi + dpj)) * gradW(xixj, p);
+      ai -= acc;
+      const unsigned b = A.pbody[sp];
+      const BoundaryDev& B = A.bnd[b];
+      if (B.dynamic) {
+        const size_t o = size_t(b) * A.cap + s;
+        const vec3 f = p.mass * acc;
+        const vec3 tq = cross((xi - xixj) - mk(B.x[0], B.x[1], B.x[2]), f);
+        float4 F = A.bforce[o], T = A.btorque[o];
+        A.bforce[o] = f4(ld3(F) + f, 0.f);
+        A.btorque[o] = f4(ld3(T) + tq, 0.f);
+      }
+    }
+  }
+  unsigned mask = PL ? 0u : A.bmask[s];
   while (mask) {
     const unsigned b = __builtin_ctz(mask);
     mask &= mask - 1u;
@@ -1488,7 +1526,7 @@ __global__ __launch_bounds__(256) void k_is_advect(SceneTab T, DevArrays A) {
 }
 // calculate_isph_diagonal_adv_density (runner.hpp:2901-2949): the Jacobi
 // diagonal (times rho0) into aii, the advected density into adv_density
-template <bool W>
+template <bool W, bool PL>
 __global__ __launch_bounds__(256) void k_is_diag(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
   if (s >= p.first + p.n) return;
@@ -1521,23 +1559,35 @@ __global__ __launch_bounds__(256) void k_is_diag(SceneTab T, DevArrays A) {
       }
     }
   }
-  unsigned mask = A.bmask[s];
-  while (mask) {
-    const unsigned b = __builtin_ctz(mask);
-    mask &= mask - 1u;
-    const size_t o = size_t(b) * A.cap + s;
-    const float4 r = A.brec[o], k = A.bker[o];
-    const vec3 g = mk(k.x, k.y, k.z);
-    const vec3 xb = ld3(r);
-    const vec3 xib = xi - xb;
-    dadv += dt * dot(vi - rigid_velocity(A.bnd[b], xb), g);
-    diag += (dens + p.rho0) / (dens * p.rho0) / (length_sqr(xib) + eps) * dot(xib, g);
+  if constexpr (PL) {
+    // calculate_isph_diagonal_adv_density_with_pellets (runner.hpp:2951-3001)
+    const unsigned npn = A.pnnb[s];
+    for (unsigned t = 0; t < npn; ++t) {
+      const unsigned sp = pellet_at(A.pnbr, A, s, t);
+      const vec3 xixj = wrapd<W>(xi - ld3(A.px[sp]), p);
+      const vec3 g = p.vol * gradW(xixj, p);
+      dadv += dt * dot(vi - ld3(A.pv[sp]), g);
+      diag += (dens + p.rho0) / (dens * p.rho0) / (length_sqr(xixj) + eps) * dot(xixj, g);
+    }
+  } else {
+    unsigned mask = A.bmask[s];
+    while (mask) {
+      const unsigned b = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      const size_t o = size_t(b) * A.cap + s;
+      const float4 r = A.brec[o], k = A.bker[o];
+      const vec3 g = mk(k.x, k.y, k.z);
+      const vec3 xb = ld3(r);
+      const vec3 xib = xi - xb;
+      dadv += dt * dot(vi - rigid_velocity(A.bnd[b], xb), g);
+      diag += (dens + p.rho0) / (dens * p.rho0) / (length_sqr(xib) + eps) * dot(xib, g);
+    }
   }
   A.aii[s] = diag * p.rho0;
   A.adv_density[s] = dadv;
 }
 // isph_solve_iteration (runner.hpp:2996-3042) + FP64 partial sums
-template <bool W>
+template <bool W, bool PL>
 __global__ __launch_bounds__(256) void k_is_iter(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
   ALV_SCENE(T, A, s);
@@ -1579,6 +1629,17 @@ __global__ __launch_bounds__(256) void k_is_iter(SceneTab T, DevArrays A) {
         }
       }
     }
+    if constexpr (PL) {
+      // isph_solve_iteration_with_pellets (runner.hpp:3055-3118): pellets
+      // at rest density with the extrapolated pressure
+      const unsigned npn = A.pnnb[s];
+      for (unsigned t = 0; t < npn; ++t) {
+        const unsigned sp = pellet_at(A.pnbr, A, s, t);
+        const vec3 xixj = wrapd<W>(xi - ld3(A.px[sp]), p);
+        off -= p.vol * (dens + p.rho0) / (dens * p.rho0) / (length_sqr(xixj) + eps) *
+               dot(xixj, gradW(xixj, p)) * A.pp[sp];
+      }
+    }
     off *= p.rho0;
     float pr = 0.f;
     if (fabsf(denom) > 1.0e-9f)
@@ -1616,7 +1677,7 @@ __global__ __launch_bounds__(256) void k_body_partials(SceneTab T, DevArrays A) 
   const unsigned bit = 1u << b;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   for (unsigned s = blockIdx.x * kBlock + threadIdx.x; s < p.n; s += gridDim.x * kBlock) {
-    if (A.bmask[s] & bit) {
+    if (p.pellet_mode || (A.bmask[s] & bit)) {
       const size_t o = size_t(b) * A.cap + s;
       float4 F = A.bforce[o], T = A.btorque[o];
       acc[0] += F.x;
@@ -1750,6 +1811,239 @@ __global__ __launch_bounds__(256) void k_node_positions(GridGeomDev g, float* ou
   out[3 * l] = x.x;
   out[3 * l + 1] = x.y;
   out[3 * l + 2] = x.z;
+}
+
+// ---------------------------------------------------------------------------
+// Particle-sampled boundaries: pellets (Pile with VolumeMethod::pellets,
+// SolverI::step with pellets, solver_i.hpp:79-330).  The pellets of every
+// body are placed at the step start, sorted into a cell grid of their own
+// (the fluid's cells), and the fluid particles keep a second list of the
+// pellets around them; in the reference both lists come from one grid of
+// fluid and pellets, where pellet ids follow every fluid id, so each list
+// in (cell, id) order is the same.
+// ---------------------------------------------------------------------------
+// The 27 cells around x in the reference order (dx outer, dz inner; periodic
+// axes wrapped), each in ascending sorted index: f(j, x - x_j) for every j
+// of `xs` closer than the kernel radius.
+template <bool W, class F>
+__device__ inline void scan_cells(const SimParams& p, const unsigned* cell_start,
+                                  const float4* xs, vec3 x, F&& f) {
+  int ip[3];
+  cell_ipos(x, p, ip);
+  for (int t = 0; t < 27; ++t) {
+    int q[3] = {ip[0] + t / 9 - 1, ip[1] + (t / 3) % 3 - 1, ip[2] + t % 3 - 1};
+    if constexpr (W) {
+      for (int a = 0; a < 3; ++a)
+        if (p.wrap_mask & (1 << a))
+          q[a] = q[a] >= p.res[a] ? q[a] - p.res[a] : (q[a] < 0 ? q[a] + p.res[a] : q[a]);
+    }
+    if (!within_grid(q, p)) continue;
+    const unsigned c = p.cell_base + cell_linear(q, p);
+    const unsigned e = cell_start[c + 1];
+    for (unsigned j = cell_start[c]; j < e; ++j) {
+      const vec3 d = wrapd<W>(x - ld3(xs[j]), p);
+      if (length_sqr(d) < p.h2) f(j, d);
+    }
+  }
+}
+__device__ inline void list_put(uint4* list, const DevArrays& A, unsigned s, unsigned t,
+                                unsigned v) {
+  reinterpret_cast<unsigned*>(list + nbr_group(A, t >> 2, s))[t & 3u] = v;
+}
+
+// transform_pellets (runner.hpp:4071-4085) with the step-start kinematics:
+// pellet k of the pile into ux/uv row cap + k.  One thread per pellet.
+__global__ __launch_bounds__(256) void k_transform_pellets(SceneTab T, DevArrays A) {
+  ALV_SCENE_LINEAR(T, A, k);
+  if (k >= p.n) return;
+  const BoundaryDev& B = A.bnd[A.pvbody[k]];
+  const vec3 r = rotate(ld3(A.pverts, k), make_float4(B.q[0], B.q[1], B.q[2], B.q[3]));
+  st3(A.ux, A.cap + k, r + mk(B.x[0], B.x[1], B.x[2]));
+  st3(A.uv, A.cap + k,
+      cross(mk(B.omega[0], B.omega[1], B.omega[2]), r) + mk(B.v[0], B.v[1], B.v[2]));
+}
+// user -> sorted order of the pellets
+__global__ __launch_bounds__(256) void k_pellet_gather(SceneTab T, DevArrays A) {
+  ALV_SCENE_LINEAR(T, A, s);
+  if (s >= p.n) return;
+  const unsigned k = A.psorted_id[s];
+  A.px[s] = f4(ld3(A.ux, A.cap + k), 0.f);
+  A.pv[s] = f4(ld3(A.uv, A.cap + k), 0.f);
+  A.pbody[s] = A.pvbody[k];
+}
+// The pellet lists of the fluid particles (make_bead_pellet_neighbor_list,
+// runner.hpp:1913-1971; the fluid lists are k_neighbors') and the density
+// with pellets (compute_density_with_pellets, runner.hpp:2074-2098):
+// rho = m (W(0) + sum_fluid W + sum_pellets W), both sums over the lists.
+template <bool W>
+__global__ __launch_bounds__(256) void k_pellet_fluid_lists(SceneTab T, DevArrays A) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
+  const unsigned K = p.kmax;
+  const vec3 xi = ld3(A.x[s]);
+  unsigned cnt = 0;
+  scan_cells<W>(p, A.pcell_start, A.px, xi, [&](unsigned j, vec3) {
+    list_put(A.pnbr, A, s, min(cnt, K - 1), j);
+    ++cnt;
+  });
+  const unsigned npn = min(cnt, K);
+  A.pnnb[s] = npn;
+  if (cnt > K) A.anomaly[2] = 1u;
+  float dens = p.w0;
+  const unsigned nn = A.nnb[s];
+  for (unsigned t = 0; t < nn; ++t)
+    dens += W_d2(length_sqr(wrapd<W>(xi - ld3(A.x[pellet_at(A.nbr, A, s, t)]), p)), p);
+  for (unsigned t = 0; t < npn; ++t)
+    dens += W_d2(length_sqr(wrapd<W>(xi - ld3(A.px[pellet_at(A.pnbr, A, s, t)]), p)), p);
+  const float rho = dens * p.mass;
+  A.density[s] = rho;
+  reinterpret_cast<float*>(A.v + s)[3] = rho;
+}
+// The fluid lists of the pellets (the bead lists of the same search), which
+// the MLS extrapolation reads.  One thread per pellet.
+template <bool W>
+__global__ __launch_bounds__(256) void k_pellet_bead_lists(SceneTab T, DevArrays A) {
+  ALV_SCENE_LINEAR(T, A, s);
+  if (s >= p.n) return;
+  const unsigned K = p.kmax;
+  unsigned cnt = 0;
+  scan_cells<W>(p, A.cell_start, A.x, ld3(A.px[s]), [&](unsigned j, vec3) {
+    list_put(A.pbnbr, A, s, min(cnt, K - 1), j);
+    ++cnt;
+  });
+  A.pbnnb[s] = min(cnt, K);
+  if (cnt > K) A.anomaly[2] = 1u;
+}
+// compute_viscosity_with_pellets (runner.hpp:2251-2297) with
+// clear_acceleration and the per-block CFL maxima, as k_viscosity
+template <bool W>
+__global__ __launch_bounds__(256) void k_viscosity_pellets(SceneTab T, DevArrays A) {
+  __shared__ float lds[4];
+  ALV_SCENE(T, A, s);
+  const float dt = A.ctl->dt;
+  float cfl = -FLT_MAX;
+  if (s < p.first + p.n) {
+    const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
+    vec3 da = viscosity_sum<W>(p, A, s, xi, vi, A.nnb[s], GlobalXV{A.x, A.v});
+    const unsigned npn = A.pnnb[s];
+    for (unsigned t = 0; t < npn; ++t) {
+      const unsigned sp = pellet_at(A.pnbr, A, s, t);
+      const vec3 xixj = wrapd<W>(xi - ld3(A.px[sp]), p);
+      const vec3 acc = (p.vol * p.bvisc * dot(xixj, gradW(xixj, p)) /
+                        (length_sqr(xixj) + 0.01f * p.h2)) *
+                       (vi - ld3(A.pv[sp]));
+      da += acc;
+      const unsigned b = A.pbody[sp];
+      const BoundaryDev& B = A.bnd[b];
+      if (B.dynamic) {
+        const size_t o = size_t(b) * A.cap + s;
+        const vec3 f = (-p.mass) * acc;
+        const vec3 tq = cross((xi - xixj) - mk(B.x[0], B.x[1], B.x[2]), f);
+        float4 F = A.bforce[o], Tq = A.btorque[o];
+        A.bforce[o] = f4(ld3(F) + f, 0.f);
+        A.btorque[o] = f4(ld3(Tq) + tq, 0.f);
+      }
+    }
+    const vec3 a = mk(p.g[0], p.g[1], p.g[2]) + da;
+    A.a[s] = f4(a, 0.f);
+    cfl = length_sqr(vi + a * dt);
+  }
+  float m = block_max_f(cfl, lds);
+  if (threadIdx.x == 0) A.fpartials[xcd_block()] = m;
+}
+
+// Pseudo-inverse of the symmetric 3x3 moment matrix of the MLS plane fit
+// (runner.hpp:2581-2590 takes it from an SVD, runner.hpp:340-574).  For a
+// symmetric matrix V S^+ U^T equals sum_k v_k v_k^T / lambda_k over the
+// eigenpairs with |lambda_k| > 1e-2, the reference's singular-value cut.
+// Eigenpairs by six sweeps of cyclic Jacobi rotations (Jacobi 1846; the
+// rotation of Golub and Van Loan, Matrix Computations, 8.5.2).
+__device__ inline void sym_pinv(vec3 d, vec3 o, float inv[3][3]) {
+  float m[3][3] = {{d.x, o.x, o.y}, {o.x, d.y, o.z}, {o.y, o.z, d.z}};
+  float e[3][3] = {{1.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, {0.f, 0.f, 1.f}};
+#pragma unroll
+  for (int sweep = 0; sweep < 6; ++sweep) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int pp = r < 2 ? 0 : 1, qq = r == 0 ? 1 : 2, k = 3 - pp - qq;
+      const float apq = m[pp][qq];
+      if (apq == 0.f) continue;
+      const float theta = (m[qq][qq] - m[pp][pp]) / (2.f * apq);
+      const float tn = (theta >= 0.f ? 1.f : -1.f) / (fabsf(theta) + sqrtf(theta * theta + 1.f));
+      const float cs = 1.f / sqrtf(tn * tn + 1.f), sn = tn * cs;
+      m[pp][pp] -= tn * apq;
+      m[qq][qq] += tn * apq;
+      m[pp][qq] = m[qq][pp] = 0.f;
+      const float akp = m[k][pp], akq = m[k][qq];
+      m[k][pp] = m[pp][k] = cs * akp - sn * akq;
+      m[k][qq] = m[qq][k] = sn * akp + cs * akq;
+#pragma unroll
+      for (int w = 0; w < 3; ++w) {
+        const float ep = e[w][pp], eq = e[w][qq];
+        e[w][pp] = cs * ep - sn * eq;
+        e[w][qq] = sn * ep + cs * eq;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) inv[r][c] = 0.f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float lam = m[k][k];
+    if (!(fabsf(lam) > 1e-2f)) continue;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) inv[r][c] += e[r][k] * e[c][k] / lam;
+  }
+}
+// extrapolate_pressure_to_pellet (runner.hpp:2543-2607): the weighted plane
+// fit of the fluid neighbours' pressure at each pellet, clamped at 0 (fmax:
+// a pellet without fluid neighbours gets 0).  Inside the Jacobi loop it
+// reads the buffer the next iteration reads; final_pressure reads the
+// solved pressure and also stores the pellets' pressure in user order.
+template <bool W>
+__global__ __launch_bounds__(256) void k_extrapolate(SceneTab T, DevArrays A, int final_pressure) {
+  ALV_SCENE_LINEAR(T, A, s);
+  if (s >= p.n) return;
+  if (!final_pressure && A.ctl->dens_done) return;
+  const float* src = (final_pressure || (A.ctl->dens_iter & 1u) != 0u) ? A.pressure
+                                                                        : A.last_pressure;
+  const vec3 xi = ld3(A.px[s]);
+  const unsigned nn = A.pbnnb[s];
+  float w_sum = 0.f;
+  vec3 center = mk(0.f, 0.f, 0.f);
+  for (unsigned t = 0; t < nn; ++t) {
+    const vec3 xj = ld3(A.x[pellet_at(A.pbnbr, A, s, t)]);
+    const float w = W_d2(length_sqr(wrapd<W>(xi - xj, p)), p) * p.vol;
+    w_sum += w;
+    center += w * xj;
+  }
+  center = mk(center.x / w_sum, center.y / w_sum, center.z / w_sum);
+  float alpha = 0.f;
+  vec3 dg = mk(0.f, 0.f, 0.f), od = mk(0.f, 0.f, 0.f), pxs = mk(0.f, 0.f, 0.f);
+  for (unsigned t = 0; t < nn; ++t) {
+    const unsigned j = pellet_at(A.pbnbr, A, s, t);
+    const vec3 xj = ld3(A.x[j]);
+    const vec3 xt = xj - center;
+    const float w = W_d2(length_sqr(wrapd<W>(xi - xj, p)), p) * p.vol;
+    const float wp = w * src[j];
+    alpha += wp;
+    pxs += wp * xt;
+    dg += w * mk(xt.x * xt.x, xt.y * xt.y, xt.z * xt.z);
+    od += w * mk(xt.x * xt.y, xt.x * xt.z, xt.y * xt.z);
+  }
+  alpha /= w_sum;
+  float inv[3][3];
+  sym_pinv(dg, od, inv);
+  const vec3 plane = mk(inv[0][0] * pxs.x + inv[0][1] * pxs.y + inv[0][2] * pxs.z,
+                        inv[1][0] * pxs.x + inv[1][1] * pxs.y + inv[1][2] * pxs.z,
+                        inv[2][0] * pxs.x + inv[2][1] * pxs.y + inv[2][2] * pxs.z);
+  const float pr = fmaxf(alpha + dot(plane, xi - center), 0.f);
+  A.pp[s] = pr;
+  if (final_pressure) A.upressure[A.cap + A.psorted_id[s]] = pr;
 }
 
 // ---------------------------------------------------------------------------
@@ -1938,17 +2232,17 @@ void launch_ii_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
 }
 void launch_is_prepare(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH(k_is_advect, L, s, A);
-  ALV_LAUNCH_W(k_is_diag, L, s, A);
+  ALV_LAUNCH_W2(k_is_diag, false, L, s, A);
 }
 void launch_is_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_is_iter, L, s, A);
+  ALV_LAUNCH_W2(k_is_iter, false, L, s, A);
   ALV_LAUNCH_SCENES(k_ii_finalize, L, s, A);
 }
 void launch_is_settle(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH(k_is_settle, L, s, A);
 }
 void launch_ii_finish(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_W(k_ii_pressure_accel, L, s, A);
+  ALV_LAUNCH_W2(k_ii_pressure_accel, false, L, s, A);
   ALV_LAUNCH_W(k_ii_finish, L, s, A);
 }
 void launch_body_sums(const Launch& L, const DevArrays& A, hipStream_t s) {
@@ -1984,6 +2278,54 @@ void launch_volume_field_grid(const SimParams& p, const GridGeomDev& g, const fl
 }
 void launch_node_positions(const GridGeomDev& g, float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_node_positions, dim3(grid_for(g.num_nodes)), dim3(kBlock), 0, s, g, out);
+}
+
+void launch_transform_pellets(const Launch& Lp, const DevArrays& A, hipStream_t s) {
+  if (Lp.active) ALV_LAUNCH_GRID(k_transform_pellets, dim3(Lp.pblocks), s, Lp.T, A);
+}
+void launch_pellet_grid(const Launch& Lp, const DevArrays& A, hipStream_t s) {
+  // the pellets' view of the counting sort: their user rows, keys and grid
+  DevArrays Ap = A;
+  Ap.ux = A.ux + size_t(A.cap) * 3;
+  Ap.uv = A.uv + size_t(A.cap) * 3;
+  Ap.key = A.pkey;
+  Ap.slot = A.pslot;
+  Ap.sorted_id = A.psorted_id;
+  Ap.cell_count = A.pcell_count;
+  Ap.cell_start = A.pcell_start;
+  Ap.scan_tmp = A.pscan_tmp;
+  launch_fill_u32(A.pcell_count, 0u, Lp.cells, s);
+  if (Lp.active) launch_cell_count(Lp, Ap, s);
+  launch_reserve_padding(Lp, Ap, s);
+  launch_scan(Lp.cells, Ap.cell_count, Ap.cell_start, Ap.scan_tmp, s);
+  if (Lp.active) {
+    launch_scatter(Lp, Ap, s);
+    launch_cell_sort(Lp, Ap, s);
+    ALV_LAUNCH_GRID(k_pellet_gather, dim3(Lp.pblocks), s, Lp.T, A);
+  }
+}
+void launch_pellet_lists(const Launch& L, const Launch& Lp, const DevArrays& A, hipStream_t s) {
+  if (L.active) ALV_LAUNCH_W(k_pellet_fluid_lists, L, s, A);
+  if (Lp.active) ALV_LAUNCH_W(k_pellet_bead_lists, Lp, s, A);
+}
+void launch_pellet_viscosity(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_W(k_viscosity_pellets, L, s, A);
+}
+void launch_is_prepare_pellets(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH(k_is_advect, L, s, A);
+  ALV_LAUNCH_W2(k_is_diag, true, L, s, A);
+}
+void launch_is_iteration_pellets(const Launch& L, const Launch& Lp, const DevArrays& A,
+                                 hipStream_t s) {
+  if (Lp.active) ALV_LAUNCH_W(k_extrapolate, Lp, s, A, 0);
+  ALV_LAUNCH_W2(k_is_iter, true, L, s, A);
+  ALV_LAUNCH_SCENES(k_ii_finalize, L, s, A);
+}
+void launch_is_finish_pellets(const Launch& L, const Launch& Lp, const DevArrays& A,
+                              hipStream_t s) {
+  if (Lp.active) ALV_LAUNCH_W(k_extrapolate, Lp, s, A, 1);
+  ALV_LAUNCH_W2(k_ii_pressure_accel, true, L, s, A);
+  ALV_LAUNCH_W(k_ii_finish, L, s, A);
 }
 
 }  // namespace alv

@@ -32,6 +32,10 @@ struct SimParams {
   // scenes; cells [cell_base, cell_base + num_cells] (the last one is the bin
   // of particles outside the grid); boundaries from bnd_base.
   unsigned first, span, cell_base, bnd_base;
+  // particle-sampled boundaries (VolumeMethod::pellets): the boundaries are
+  // the np pellets instead of volume maps
+  int pellet_mode;
+  unsigned np;
 };
 
 // Geometry of one 32-node cubic Lagrange map (Pile lists, pile.hpp:106-112).

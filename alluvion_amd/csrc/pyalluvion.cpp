@@ -342,6 +342,30 @@ PYBIND11_MODULE(_alluvion, m) {
            py::arg("inertia_tensor") = float3{1, 1, 1}, py::arg("x") = float3{0, 0, 0},
            py::arg("q") = float4{0, 0, 0, 1}, py::arg("display_mesh") = py::none(),
            py::keep_alive<1, 2>())
+      .def("add_pellets",
+           [](P& p, Dist* distance, U3 resolution, float cell_width, float margin, float sign,
+              py::array_t<float, py::array::c_style | py::array::forcecast> pellets, float mass,
+              float restitution, float friction, float3 inertia_tensor, float3 x, float4 q,
+              py::object display_mesh) {
+             (void)display_mesh;
+             if (pellets.ndim() != 2 || pellets.shape(1) != 3)
+               throw std::invalid_argument("pellets must be [n, 3]");
+             std::vector<float3> pts(size_t(pellets.shape(0)));
+             std::memcpy(pts.data(), pellets.data(), pts.size() * sizeof(float3));
+             return p.add_pellets(distance, resolution, cell_width, margin, sign, pts, mass,
+                                  restitution, friction, inertia_tensor, x, q);
+           },
+           py::arg("distance"), py::arg("resolution") = U3{0, 0, 0}, py::arg("cell_width") = 0,
+           py::arg("margin") = std::numeric_limits<float>::infinity(), py::arg("sign") = 1,
+           py::arg("pellets"), py::arg("mass") = 0, py::arg("restitution") = 1,
+           py::arg("friction") = 0, py::arg("inertia_tensor") = float3{1, 1, 1},
+           py::arg("x") = float3{0, 0, 0}, py::arg("q") = float4{0, 0, 0, 1},
+           py::arg("display_mesh") = py::none(), py::keep_alive<1, 2>())
+      .def_property_readonly("num_pellets", &P::num_pellets)
+      .def_property_readonly("num_collision_pellets", &P::num_pellets)
+      .def_property_readonly("max_num_pellets", &P::max_num_pellets)
+      .def("hint_identical_sequence", &P::hint_identical_sequence, py::arg("begin_id"),
+           py::arg("end_id"))
       .def("get_size", &P::get_size)
       .def_readwrite("mass", &P::mass_)
       .def_readwrite("restitution", &P::restitution_)
@@ -368,11 +392,10 @@ PYBIND11_MODULE(_alluvion, m) {
       .def("find_contacts", [](P& p, U i, U j) { return p.find_contacts(i, j); })
       .def("find_contacts", [](P& p, U i) { return p.find_contacts(i); })
       .def("find_contacts",
-           [](P& p, py::object particle_x, U max_num_beads) {
-             (void)particle_x;  // pellet contacts are not on this path
-             (void)max_num_beads;
-             return p.find_contacts();
-           })
+           [](P& p, const Variable<1, float3>& particle_x, U max_num_beads) {
+             return p.find_contacts(particle_x, max_num_beads);
+           },
+           py::arg("particle_x"), py::arg("max_num_beads"))
       .def("find_contacts", [](P& p) { return p.find_contacts(); })
       .def("solve_contacts", &P::solve_contacts)
       .def_property_readonly("num_contacts_pinned",
@@ -534,6 +557,19 @@ PYBIND11_MODULE(_alluvion, m) {
                std::memcpy(c.mutable_data(), counts.data(), counts.size() * sizeof(U));
                return py::make_tuple(a, c);
              })
+        .def("transform_all_pellets", &S::transform_all_pellets)
+        .def("get_pellet_neighbors",
+             [](S& s, int which, U rows) {
+               std::vector<U> ids, counts;
+               s.get_pellet_neighbors(which, rows, ids, counts);
+               U K = U(ids.size() / std::max<size_t>(1, counts.size()));
+               py::array_t<U> a({size_t(counts.size()), size_t(K)});
+               std::memcpy(a.mutable_data(), ids.data(), ids.size() * sizeof(U));
+               py::array_t<U> c(counts.size());
+               std::memcpy(c.mutable_data(), counts.data(), counts.size() * sizeof(U));
+               return py::make_tuple(a, c);
+             },
+             py::arg("which"), py::arg("rows"))
         .def("get_grid", [](S& s) {
           std::vector<U> cnt, st, an;
           s.get_grid(cnt, st, an);

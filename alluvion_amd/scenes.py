@@ -21,6 +21,9 @@ Scenes:
   0.08 m, 128^3 volume maps) seeded above the fluid.
 * ``periodic_random``: N particles uniform in the periodic unit cube
   [-0.5, 0.5)^3 with uniform random velocities (+-0.5 m/s).
+* ``dam_break_pellets``: a fluid block in a cube container whose walls are
+  two layers of pellets, with a dynamic sphere sampled by pellets on its
+  surface (Fibonacci lattice); ISPH with particle-sampled boundaries.
 """
 from dataclasses import dataclass, field
 from typing import List, Optional, Tuple
@@ -98,6 +101,13 @@ class Scene:
     # rigid contacts (Const::contact_tolerance, Consti::max_num_contacts)
     contact_tolerance: float = 0.0
     max_num_contacts: int = 10000
+    # boundary representation (Pile's VolumeMethod): "volume_map", or
+    # "pellets" (particle-sampled boundaries: each body's collision
+    # vertices are its pellets; ISPH only)
+    volume_method: str = "volume_map"
+    max_num_pellets: int = 10000
+    # hint_identical_sequence runs [begin, end) of identical bodies
+    identical_sequences: List[Tuple[int, int]] = field(default_factory=list)
 
     @property
     def n(self) -> int:
@@ -220,7 +230,10 @@ def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optiona
     cni.max_num_particles_per_cell = scene.max_num_particles_per_cell
     runner = dp.Runner()
     cn.contact_tolerance = scene.contact_tolerance
-    pile = dp.Pile(dp, runner, scene.max_num_contacts, cn=cn, cni=cni)
+    pellets = scene.volume_method == "pellets"
+    method = _al.VolumeMethod.pellets if pellets else _al.VolumeMethod.volume_map
+    pile = dp.Pile(dp, runner, scene.max_num_contacts, volume_method=method,
+                   max_num_pellets=scene.max_num_pellets, cn=cn, cni=cni)
     for b in scene.boundaries:
         if b.kind == "box":
             dist = dp.BoxDistance.create(dp.f3(*b.params[:3]), b.params[3])
@@ -228,6 +241,13 @@ def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optiona
             dist = dp.BoxShellDistance.create(dp.f3(*b.params[:3]), b.params[3], b.params[4])
         else:
             dist = dp.SphereDistance.create(b.params[0])
+        if pellets:
+            pts = np.zeros((0, 3), np.float32) if b.collision_vertices is None else \
+                np.asarray(b.collision_vertices, np.float32).reshape(-1, 3)
+            pile.add_pellets(dist, _al.uint3(*b.resolution), sign=b.sign, pellets=pts,
+                             mass=b.mass, restitution=b.restitution, friction=b.friction,
+                             inertia_tensor=dp.f3(*b.inertia), x=dp.f3(*b.x), q=dp.f4(*b.q))
+            continue
         mesh = None
         if b.collision_vertices is not None:
             mesh = _al.Mesh()
@@ -238,6 +258,8 @@ def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optiona
         if log:
             log("boundary %s %s built" % (b.kind, b.resolution))
     pile.set_gravity(dp.f3(*scene.pile_gravity))
+    for b0, b1 in scene.identical_sequences:
+        pile.hint_identical_sequence(b0, b1)
     n_max = scene.n if max_particles is None else max_particles
     cls = {"df": dp.SolverDf, "ii": dp.SolverIi, "i": dp.SolverI}[kind]
     solver = cls(runner, pile, dp, n_max, cn=cn, cni=cni)
@@ -253,9 +275,73 @@ def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optiona
         solver.density_change_tolerance = scene.density_change_tolerance
         solver.min_divergence_solve = scene.min_divergence_solve
         solver.max_divergence_solve = scene.max_divergence_solve
-    dp.coat(solver.particle_x).set(scene.x)
-    dp.coat(solver.particle_v).set(scene.v)
+    if pellets:  # pellet rows follow the fluid's; the step places them
+        rows = n_max + scene.max_num_pellets
+        for var, src in ((solver.particle_x, scene.x), (solver.particle_v, scene.v)):
+            full = np.zeros((rows, 3), np.float32)
+            full[:scene.n] = src
+            dp.coat(var).set(full)
+    else:
+        dp.coat(solver.particle_x).set(scene.x)
+        dp.coat(solver.particle_v).set(scene.v)
     return dp, pile, solver, cn, cni
+
+
+def box_wall_pellets(edge: float, radius: float, layers: int = 2) -> np.ndarray:
+    """Pellets of a cube container of inner edge `edge` centred at the origin:
+    the lattice points (spacing 2r, offset r from the inner faces) of the
+    `layers` outermost shells outside the inner cube, in x-major order."""
+    d = 2.0 * radius
+    m = int(round(edge / d))
+    k = np.arange(-layers, m + layers)
+    g = (k + 0.5) * d - 0.5 * edge
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X, Y, Z], -1).reshape(-1, 3)
+    inside = np.all(np.abs(pts) < 0.5 * edge, axis=1)
+    return pts[~inside].astype(np.float32)
+
+
+def sphere_pellets(sphere_radius: float, spacing: float) -> np.ndarray:
+    """Fibonacci-lattice pellets on a sphere (Gonzalez 2010, "Measurement of
+    areas on a sphere using Fibonacci and latitude-longitude lattices"),
+    about one per spacing^2 of surface."""
+    n = max(8, int(round(4.0 * np.pi * sphere_radius ** 2 / spacing ** 2)))
+    i = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * i / n
+    phi = np.pi * (1.0 + 5.0 ** 0.5) * i
+    rxy = np.sqrt(1.0 - z * z)
+    return (sphere_radius * np.stack([rxy * np.cos(phi), z, rxy * np.sin(phi)], 1)).astype(
+        np.float32)
+
+
+def dam_break_pellets(nx: int = 16, ny: int = 16, nz: int = 8, seed: int = 7, edge: float = 0.14,
+                      sphere: bool = True, sphere_x: Optional[Tuple[float, float, float]] = None
+                      ) -> Scene:
+    """A fluid block in the corner of a cube container of inner edge `edge`
+    whose walls are two pellet layers (the first fluid layer r from the
+    walls, so fluid and pellets pack at the lattice spacing 2r), and
+    (optionally) a dynamic sphere of radius 0.02 m sampled by surface
+    pellets, by default one spacing above the fluid; ISPH."""
+    r, h = 0.0025, 0.01
+    lo = -0.5 * edge - r  # first lattice layer at lo + 2r = r inside the wall
+    x = lattice_block(nx, ny, nz, r, lo, seed, 0.01 * r)
+    walls = box_wall_pellets(edge, r)
+    bodies = [BoundarySpec("box", (edge, edge, edge, 0.0), -1.0, (4, 4, 4), restitution=0.8,
+                           collision_vertices=walls)]
+    if sphere:
+        R = 0.02
+        m = 0.6 * 1000.0 * 4.0 / 3.0 * np.pi * R ** 3
+        I = 0.4 * m * R * R
+        top = lo + 2 * r * ny
+        sx = (0.01, top + 2 * r + R, 0.0) if sphere_x is None else sphere_x
+        bodies.append(BoundarySpec("sphere", (R,), 1.0, (4, 4, 4), mass=m, inertia=(I, I, I),
+                                   x=tuple(sx), restitution=0.8, friction=0.1,
+                                   collision_vertices=sphere_pellets(R, 2 * r)))
+    cells = int(np.ceil((0.5 * edge + 3 * r * 2) / h)) + 1
+    return Scene(name="dam_break_pellets", x=x, v=np.zeros_like(x), grid_offset=(-cells,) * 3,
+                 grid_res=(2 * cells,) * 3, boundaries=bodies, volume_method="pellets",
+                 max_num_pellets=sum(b.collision_vertices.shape[0] for b in bodies) + 64,
+                 dt=2e-4, min_density_solve=2, max_density_solve=20)
 
 
 def dam_break_box(n_target: int, edge: float, seed: int, map_res: int = 64) -> Scene:

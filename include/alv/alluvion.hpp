@@ -747,8 +747,8 @@ class Pile {
        Const<TF>* cn = nullptr, ConstiN* cni = nullptr)
       : store_(store), cn_(cn ? cn : &store.get_cn()), cni_(cni ? cni : &store.get_cni()) {
     (void)runner;
-    (void)max_num_pellets;
-    check(alv_pile_create(&pile_, max_num_contacts, static_cast<int>(volume_method)));
+    check(alv_pile_create_ex(&pile_, max_num_contacts, static_cast<int>(volume_method),
+                             max_num_pellets));
     max_num_contacts_ = max_num_contacts;
     cni_->max_num_contacts = max_num_contacts;
     if (cn)
@@ -786,6 +786,38 @@ class Pile {
     inertia_tensor_.push_back(inertia_tensor);
     cni_->num_boundaries = get_size();
     return id;
+  }
+  // Pile::add with pellets (pile.hpp:217-283): the pellets (body frame) are
+  // the body's collision vertices; a pellet pile builds no volume map.
+  U add_pellets(dg::Distance<float3, TF>* distance, U3 resolution, TF cell_width, TF margin,
+                TF sign, std::vector<float3> const& pellets, TF mass = 0, TF restitution = 1,
+                TF friction = 0, float3 inertia_tensor = float3{1, 1, 1},
+                float3 x = float3{0, 0, 0}, float4 q = float4{0, 0, 0, 1}) {
+    Mesh m;
+    m.vertices = pellets;
+    return add(distance, resolution, cell_width, margin, sign, m, mass, restitution, friction,
+               inertia_tensor, x, q);
+  }
+  // num_pellets_ / max_num_pellets_ (pile.hpp:124-130)
+  U num_pellets() const {
+    unsigned n = 0, mx = 0;
+    check(alv_pile_num_pellets(pile_, &n, &mx));
+    return n;
+  }
+  U max_num_pellets() const {
+    unsigned n = 0, mx = 0;
+    check(alv_pile_num_pellets(pile_, &n, &mx));
+    return mx;
+  }
+  void hint_identical_sequence(U begin_id, U end_id) {
+    check(alv_pile_hint_identical_sequence(pile_, begin_id, end_id));
+  }
+  // Pile::find_contacts(particle_x, max_num_beads) (pile.hpp:858-904)
+  U find_contacts(Variable<1, float3> const& particle_x, U max_num_beads) {
+    unsigned n = 0;
+    check(alv_pile_find_contacts_pellets(pile_, static_cast<const float*>(particle_x.ptr_),
+                                         max_num_beads, cn_->contact_tolerance, &n));
+    return n;
   }
   U get_size() const {
     unsigned n = 0;
@@ -1089,7 +1121,14 @@ class SolverBase {
   template <U wrap = 0>
   void update_particle_neighbors() {
     sync_const();
+    push();
     check(alv_solver_update_particle_neighbors(solver_));
+  }
+  // Solver::transform_all_pellets (solver.hpp:204-223)
+  void transform_all_pellets() {
+    sync_const();
+    push();
+    check(alv_solver_transform_all_pellets(solver_));
   }
   void compute_all_boundaries() {}
   // step execution as replayed HIP graphs (no counterpart in the reference)
@@ -1153,6 +1192,13 @@ class SolverBase {
                0xffffffffu);
     counts.assign(max_num_particles, 0);
     check(alv_solver_get_neighbors(solver_, ids.data(), counts.data()));
+  }
+  // pellet lists (which 0: pellets near each fluid particle, as particle ids
+  // max_num_particles + k; 1: fluid particles near each pellet)
+  void get_pellet_neighbors(int which, U rows, std::vector<U>& ids, std::vector<U>& counts) {
+    ids.assign(size_t(rows) * store_.device_cni().max_num_neighbors_per_particle, 0xffffffffu);
+    counts.assign(rows, 0);
+    check(alv_solver_get_pellet_neighbors(solver_, which, ids.data(), counts.data()));
   }
   void get_grid(std::vector<U>& count, std::vector<U>& start, std::vector<U>& anomaly) {
     ConstiN const& ci = store_.device_cni();

@@ -109,16 +109,31 @@ typedef struct alv_distance {
   float params[5];
 } alv_distance;
 
-/* VolumeMethod::volume_map is the only method on this path. */
+/* VolumeMethod (pile.hpp:16): volume maps, or particle-sampled boundaries
+ * ("pellets": the collision vertices of every body, no volume maps; only
+ * the ISPH solver steps with them, as in the reference). */
 #define ALV_VOLUME_MAP 0
+#define ALV_PELLETS 1
 
-/* Pile::Pile (pile.hpp:151-192) */
+/* Pile::Pile (pile.hpp:151-192); ALV_PELLETS with max_num_pellets = 10000,
+ * the reference's default. */
 int alv_pile_create(alv_pile** out, unsigned max_num_contacts,
                     int volume_method);
+/* Pile::Pile with max_num_pellets (pile.hpp:151-166) */
+int alv_pile_create_ex(alv_pile** out, unsigned max_num_contacts,
+                       int volume_method, unsigned max_num_pellets);
+/* num_pellets_ and max_num_pellets_ (pile.hpp:124-130) */
+int alv_pile_num_pellets(const alv_pile* pile, unsigned* num_pellets,
+                         unsigned* max_num_pellets);
+/* Pile::hint_identical_sequence (pile.hpp:637-639): bodies [begin, end)
+ * share one shape; the pellet contact search tests them in one run. */
+int alv_pile_hint_identical_sequence(alv_pile* pile, unsigned begin,
+                                     unsigned end);
 int alv_pile_destroy(alv_pile* pile);
-/* Pile::add (pile.hpp:285-352): computes the map domain and resolution
- * (calculate_domain_and_resolution, pile.hpp:40-65) and builds the volume
- * map on the device (update_volume_field, runner.hpp:1660-1694). */
+/* Pile::add (pile.hpp:217-352): computes the map domain and resolution
+ * (calculate_domain_and_resolution, pile.hpp:40-65) and, for volume maps,
+ * builds the map on the device (update_volume_field, runner.hpp:1660-1694;
+ * build_grid, pile.hpp:493-623).  A pellet pile builds no map. */
 int alv_pile_add(alv_pile* pile, const alv_const* cn,
                  const alv_distance* distance, const unsigned resolution[3],
                  float cell_width, float margin, float sign, float mass,
@@ -154,8 +169,10 @@ typedef struct alv_contact {
   float friction, nkninv, pmax, goalu, impulse_sum;
 } alv_contact;
 /* Collision vertices of body b in its own frame (the vertices of the
- * reference's collision_mesh argument of Pile::add, pile.hpp:322-331):
- * `xyz` is a host array of 3*n floats; n = 0 removes them. */
+ * reference's collision_mesh argument of Pile::add, pile.hpp:322-331, or
+ * its pellets, pile.hpp:259-276): `xyz` is a host array of 3*n floats;
+ * n = 0 removes them.  A pellet pile fails with ALV_ERR_ARG when the pellet
+ * count would exceed max_num_pellets. */
 int alv_pile_set_collision_vertices(alv_pile* pile, unsigned b, const float* xyz,
                                     unsigned n);
 /* Pile::find_contacts (pile.hpp:826-904).  i < 0: every pair (i, j), the
@@ -168,6 +185,17 @@ int alv_pile_set_collision_vertices(alv_pile* pile, unsigned b, const float* xyz
  * max_num_contacts (only that many are kept). */
 int alv_pile_find_contacts(alv_pile* pile, int i, int j, float contact_tolerance,
                            unsigned* num_contacts);
+/* Pile::find_contacts(particle_x, max_num_beads), pellet branch
+ * (pile.hpp:858-886, collision_test_with_pellets runner.hpp:3627-3727):
+ * the world pellets, rows max_num_beads .. of the device array particle_x
+ * (float3, the solver's position field), against every body (or hinted run
+ * of identical bodies) with the kinematics of the last device copy; pairs
+ * i == j and of two massless bodies are skipped and box shells have no
+ * test.  Contacts in (run, pellet, j) order.  A volume-map pile runs the
+ * all-pairs search instead, as the reference does. */
+int alv_pile_find_contacts_pellets(alv_pile* pile, const float* particle_x,
+                                   unsigned max_num_beads, float contact_tolerance,
+                                   unsigned* num_contacts);
 /* Pile::solve_contacts (pile.hpp:906-979): five sequential impulse sweeps
  * over the stored contacts, changing the host v and omega of the bodies. */
 int alv_pile_solve_contacts(alv_pile* pile);
@@ -331,6 +359,19 @@ int alv_solver_reset_t(alv_solver* solver);
  * (neighbour particle ids, canonical order) and counts[N]. */
 int alv_solver_get_neighbors(alv_solver* solver, unsigned* ids,
                              unsigned* counts);
+/* Pellet lists of a pellet solver in the reference layout
+ * (particle_boundary_neighbors / particle_num_boundary_neighbors and the
+ * pellets' rows of particle_neighbors, solver.hpp:78-85): which = 0, the
+ * pellets near each fluid particle, ids[max_n*Kmax] as particle ids
+ * max_n + k, counts[max_n]; which = 1, the fluid particles near each pellet,
+ * ids[max_num_pellets*Kmax] as fluid ids, counts[max_num_pellets]. */
+int alv_solver_get_pellet_neighbors(alv_solver* solver, int which,
+                                    unsigned* ids, unsigned* counts);
+/* Solver::transform_all_pellets (solver.hpp:204-223): places the pile's
+ * pellets into rows max_n .. of particle_x / particle_v with the bodies'
+ * current host kinematics (a no-op without pellets).  The ISPH step does it
+ * at its start; update_particle_neighbors uses the rows as they are. */
+int alv_solver_transform_all_pellets(alv_solver* solver);
 /* Cell counts (pid_length) [C], cell offsets [C+1] and grid_anomaly[3] into
  * host buffers. */
 int alv_solver_get_grid(alv_solver* solver, unsigned* cell_count,

@@ -658,14 +658,31 @@ struct Solver {
   std::vector<unsigned> nb_id, num_neighbors;
   std::vector<V3<T>> nb_x;
   unsigned anomaly[3] = {0, 0, 0};
+  // Particle-sampled boundaries (Pile with VolumeMethod::pellets,
+  // pile.hpp:16,217-290): the collision vertices of every body are its
+  // pellets; pellet k of the concatenation occupies particle slot max_n + k
+  // of x, v, pressure, last_pressure and the neighbour lists
+  // (solver.hpp:30-85).
+  bool pellet_mode = false;
+  unsigned max_p = 0, np = 0;
+  std::vector<unsigned> pellet_body;  // pellet_id_to_rigid_id (pile.hpp:624-636)
+  std::vector<unsigned> pnb_id, num_pnb;  // pellet neighbours [max_n + max_p][kmax]
+  std::vector<V3<T>> pnb_x;
+  std::vector<std::pair<unsigned, unsigned>> ident;  // hint_identical_sequence
+  struct Kin {
+    V3<T> x, v, omega;
+    V4<T> q;
+  };
+  std::vector<Kin> kin0;  // body kinematics as copied to the device at the step start
 
+  size_t nt() const { return size_t(max_n) + max_p; }
   unsigned num_cells() const {
     return unsigned(c.res[0]) * unsigned(c.res[1]) * unsigned(c.res[2]);
   }
 
   void alloc() {
-    x.assign(max_n, {0, 0, 0});
-    v.assign(max_n, {0, 0, 0});
+    x.assign(nt(), {0, 0, 0});
+    v.assign(nt(), {0, 0, 0});
     a.assign(max_n, {0, 0, 0});
     density.assign(max_n, 0);
     dfsph_factor.assign(max_n, 0);
@@ -673,17 +690,20 @@ struct Solver {
     kappa_v.assign(max_n, 0);
     density_adv.assign(max_n, 0);
     cfl_v2.assign(max_n, 0);
-    pressure.assign(max_n, 0);
-    last_pressure.assign(max_n, 0);
+    pressure.assign(nt(), 0);
+    last_pressure.assign(nt(), 0);
     aii.assign(max_n, 0);
     adv_density.assign(max_n, 0);
     density_err.assign(max_n, 0);
     dii.assign(max_n, {0, 0, 0});
     dij_pj.assign(max_n, {0, 0, 0});
     pressure_accel.assign(max_n, {0, 0, 0});
-    num_neighbors.assign(max_n, 0);
-    nb_id.assign(size_t(max_n) * c.kmax, 0);
-    nb_x.assign(size_t(max_n) * c.kmax, {0, 0, 0});
+    num_neighbors.assign(nt(), 0);
+    nb_id.assign(nt() * c.kmax, 0);
+    nb_x.assign(nt() * c.kmax, {0, 0, 0});
+    num_pnb.assign(nt(), 0);
+    pnb_id.assign(pellet_mode ? nt() * c.kmax : 0, 0);
+    pnb_x.assign(pellet_mode ? nt() * c.kmax : 0, {0, 0, 0});
     cell_count.assign(num_cells(), 0);
     cell_start.assign(num_cells() + 1, 0);
     realloc_boundary();
@@ -730,8 +750,14 @@ struct Solver {
     unsigned C = num_cells();
     std::fill(cell_count.begin(), cell_count.end(), 0u);
     anomaly[0] = anomaly[1] = anomaly[2] = 0;
-    std::vector<unsigned> key(n);
-    for (unsigned i = 0; i < n; ++i) {
+    // fluid particles, then (pellet mode) the pellets: launch_update_particle_grid
+    // over [0, n) and [max_n, max_n + np) (solver.hpp:229-240)
+    std::vector<unsigned> ids(n);
+    for (unsigned i = 0; i < n; ++i) ids[i] = i;
+    if (pellet_mode)
+      for (unsigned k = 0; k < np; ++k) ids.push_back(max_n + k);
+    std::vector<unsigned> key(nt(), C);
+    for (unsigned i : ids) {
       int ip[3];
       ipos_of(x[i], ip);
       if (within_grid(ip)) {
@@ -751,11 +777,11 @@ struct Solver {
     cell_start[C] = acc;
     cell_ids.assign(acc, 0);
     std::vector<unsigned> fill(C, 0);
-    for (unsigned i = 0; i < n; ++i) {
+    for (unsigned i : ids) {
       if (key[i] == C) continue;
       cell_ids[cell_start[key[i]] + fill[key[i]]++] = i;
     }
-    grid_x.assign(x.begin(), x.begin() + n);
+    grid_x.assign(x.begin(), pellet_mode ? x.end() : x.begin() + n);
   }
 
   void make_neighbor_list() {
@@ -1146,6 +1172,11 @@ struct Solver {
   void update_particle_neighbors() {
     // Solver::update_particle_neighbors (solver.hpp:225-260)
     update_particle_grid();
+    if (pellet_mode) {
+      make_bead_pellet_lists();
+      compute_density_with_pellets();
+      return;
+    }
     make_neighbor_list();
     compute_density();
   }
@@ -1179,7 +1210,10 @@ struct Solver {
     bool any = false;
     for (const auto& B : bnd) any |= !B.coll.empty();
     if (any) {
-      find_contacts(-1, -1);
+      if (pellet_mode)
+        find_contacts_pellets();
+      else
+        find_contacts(-1, -1);
       solve_contacts();
     }
   }
@@ -1630,12 +1664,31 @@ struct Solver {
         T dpj = pressure[j] / dj2;
         ai -= (c.mass * (dpi + dpj)) * displacement_cubic_kernel_grad(nb_x[o], c);
       }
-      for (unsigned b = 0; b < bnd.size(); ++b) {
-        V4<T> r = brec(b, i), k = bker(b, i);
-        V3<T> acc = (c.rho0 * dpi) * V3<T>{k.x, k.y, k.z};
-        V3<T> f = c.mass * acc;
-        ai -= acc;
-        add_force(b, i, {r.x, r.y, r.z}, f);
+      if (pellet_mode) {
+        // compute_pressure_accels_with_pellets (runner.hpp:2819-2870)
+        for (unsigned s = 0; s < num_pnb[i]; ++s) {
+          size_t o = size_t(i) * c.kmax + s;
+          unsigned j = pnb_id[o];
+          V3<T> xixj = pnb_x[o];
+          V3<T> xj = x[i] - xixj;
+          unsigned b = pellet_body[j - max_n];
+          T dj2 = c.rho0 * c.rho0;
+          T dpj = pressure[j] / dj2;
+          V3<T> acc = (c.mass * (dpi + dpj)) * displacement_cubic_kernel_grad(xixj, c);
+          V3<T> f = c.mass * acc;
+          ai -= acc;
+          size_t ob = size_t(b) * max_n + i;
+          force[ob] += f;
+          torque[ob] += cross(xj - bnd[b].x, f);
+        }
+      } else {
+        for (unsigned b = 0; b < bnd.size(); ++b) {
+          V4<T> r = brec(b, i), k = bker(b, i);
+          V3<T> acc = (c.rho0 * dpi) * V3<T>{k.x, k.y, k.z};
+          V3<T> f = c.mass * acc;
+          ai -= acc;
+          add_force(b, i, {r.x, r.y, r.z}, f);
+        }
       }
       pressure_accel[i] = c.rho0 * ai;
     }
@@ -1652,6 +1705,10 @@ struct Solver {
 
   // ---- SolverI::step (solver_i.hpp:79-330), volume-map boundaries ------------
   void step_i() {
+    if (pellet_mode) {
+      step_i_pellets();
+      return;
+    }
     clear_force();
     for (unsigned i = 0; i < n; ++i) a[i] = c.g;
     compute_all_boundaries();
@@ -1731,6 +1788,363 @@ struct Solver {
     }
     rep.num_density_solve = it;
     rep.mean_density_error = mean;
+    pressure_accels_and_integrate();
+  }
+
+  // ---- particle-sampled boundaries (pellets) ----------------------------------
+  // pellet_id_to_rigid_id from the bodies' pellet counts (pile.hpp:624-636)
+  int refresh_pellets() {
+    pellet_body.clear();
+    for (unsigned b = 0; b < bnd.size(); ++b)
+      for (size_t k = 0; k < bnd[b].coll.size(); ++k) pellet_body.push_back(b);
+    if (pellet_body.size() > max_p) return -4;  // pile.hpp:266-275
+    np = unsigned(pellet_body.size());
+    return 0;
+  }
+  void snapshot_kinematics() {
+    kin0.resize(bnd.size());
+    for (unsigned b = 0; b < bnd.size(); ++b) kin0[b] = {bnd[b].x, bnd[b].v, bnd[b].omega, bnd[b].q};
+  }
+  // transform_all_pellets (solver.hpp:204-223, transform_pellets
+  // runner.hpp:4071-4085) with the step-start kinematics
+  void transform_all_pellets() {
+    unsigned k = 0;
+    for (unsigned b = 0; b < bnd.size(); ++b) {
+      const Kin& K0 = kin0[b];
+      for (const V3<T>& local : bnd[b].coll) {
+        V3<T> r = rotate(local, K0.q);
+        x[max_n + k] = r + K0.x;
+        v[max_n + k] = cross(K0.omega, r) + K0.v;
+        ++k;
+      }
+    }
+  }
+  // make_bead_pellet_neighbor_list<wrap> (runner.hpp:1913-1971) over the
+  // fluid particles and the pellets (..._check_contiguous,
+  // runner.hpp:5642-5666): the 27 cells in the reference order, fluid
+  // neighbours into the bead list and pellets into the pellet list
+  void make_bead_pellet_lists() {
+    const unsigned K = c.kmax;
+    int flag2 = 0;
+    const long total = long(n) + long(np);
+#pragma omp parallel for schedule(static) reduction(| : flag2)
+    for (long kk = 0; kk < total; ++kk) {
+      const unsigned i = kk < long(n) ? unsigned(kk) : max_n + unsigned(kk - long(n));
+      V3<T> xi = x[i];
+      int ip[3];
+      ipos_of(xi, ip);
+      unsigned cb = 0, cp = 0;
+      for (int t = 0; t < 27; ++t) {
+        int nq[3] = {ip[0] + t / 9 - 1, ip[1] + (t / 3) % 3 - 1, ip[2] + t % 3 - 1};
+        for (int ax = 0; ax < 3; ++ax) {
+          if (!(c.wrap_mask & (1 << ax))) continue;
+          if (nq[ax] >= c.res[ax])
+            nq[ax] -= c.res[ax];
+          else if (nq[ax] < 0)
+            nq[ax] += c.res[ax];
+        }
+        if (!within_grid(nq)) continue;
+        unsigned cell = linear(nq);
+        for (unsigned q = cell_start[cell]; q < cell_start[cell + 1]; ++q) {
+          unsigned j = cell_ids[q];
+          V3<T> xixj = wrap_disp(xi - x[j]);
+          if (j == i || !(length_sqr(xixj) < c.h2)) continue;
+          if (j < max_n) {
+            size_t slot = size_t(i) * K + std::min(K - 1, cb);
+            nb_id[slot] = j;
+            nb_x[slot] = xixj;
+            ++cb;
+          } else {
+            size_t slot = size_t(i) * K + std::min(K - 1, cp);
+            pnb_id[slot] = j;
+            pnb_x[slot] = xixj;
+            ++cp;
+          }
+        }
+      }
+      num_neighbors[i] = std::min(K, cb);
+      num_pnb[i] = std::min(K, cp);
+      if (cb > K || cp > K) flag2 = 1;
+    }
+    if (flag2) anomaly[2] = 1;
+  }
+  // compute_density_with_pellets (runner.hpp:2074-2098)
+  void compute_density_with_pellets() {
+#pragma omp parallel for schedule(static)
+    for (long ii = 0; ii < long(n); ++ii) {
+      unsigned i = unsigned(ii);
+      T dens = c.w0;
+      for (unsigned s = 0; s < num_neighbors[i]; ++s)
+        dens += displacement_cubic_kernel(nb_x[size_t(i) * c.kmax + s], c);
+      for (unsigned s = 0; s < num_pnb[i]; ++s)
+        dens += displacement_cubic_kernel(pnb_x[size_t(i) * c.kmax + s], c);
+      density[i] = dens * c.mass;
+    }
+  }
+  // compute_viscosity_with_pellets (runner.hpp:2251-2297)
+  void compute_viscosity_with_pellets() {
+#pragma omp parallel for schedule(static)
+    for (long ii = 0; ii < long(n); ++ii) {
+      unsigned i = unsigned(ii);
+      V3<T> vi = v[i], xi = x[i];
+      V3<T> da{0, 0, 0};
+      for (unsigned s = 0; s < num_neighbors[i]; ++s) {
+        size_t o = size_t(i) * c.kmax + s;
+        unsigned j = nb_id[o];
+        V3<T> xixj = nb_x[o];
+        da += (c.visc * c.mass / density[j] * dot(xixj, displacement_cubic_kernel_grad(xixj, c)) /
+               (length_sqr(xixj) + static_cast<T>(0.01) * c.h2)) *
+              (vi - v[j]);
+      }
+      for (unsigned s = 0; s < num_pnb[i]; ++s) {
+        size_t o = size_t(i) * c.kmax + s;
+        unsigned j = pnb_id[o];
+        V3<T> xixj = pnb_x[o];
+        V3<T> acc = (c.vol * c.bvisc * dot(xixj, displacement_cubic_kernel_grad(xixj, c)) /
+                     (length_sqr(xixj) + static_cast<T>(0.01) * c.h2)) *
+                    (vi - v[j]);
+        da += acc;
+        V3<T> px = xi - xixj;
+        unsigned b = pellet_body[j - max_n];
+        V3<T> f = (-c.mass) * acc;
+        size_t ob = size_t(b) * max_n + i;
+        force[ob] += f;
+        torque[ob] += cross(px - bnd[b].x, f);
+      }
+      a[i] += da;
+    }
+  }
+  // Pseudo-inverse of the symmetric 3x3 moment matrix of the MLS plane fit
+  // (runner.hpp:2581-2590 takes it from an SVD, runner.hpp:340-574).  For a
+  // symmetric matrix V S^+ U^T equals sum_k v_k v_k^T / lambda_k over the
+  // eigenpairs with |lambda_k| > 1e-2, the reference's singular-value cut.
+  // Eigenpairs by cyclic Jacobi rotations (Jacobi 1846; the rotation
+  // formulas of Golub and Van Loan, Matrix Computations, 8.5.2), six sweeps.
+  static void sym_pinv(V3<T> d, V3<T> o, T inv[3][3]) {
+    T m[3][3] = {{d.x, o.x, o.y}, {o.x, d.y, o.z}, {o.y, o.z, d.z}};
+    T e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+    for (int sweep = 0; sweep < 6; ++sweep) {
+      for (int r = 0; r < 3; ++r) {
+        const int pp = P[r], qq = Q[r];
+        const T apq = m[pp][qq];
+        if (apq == 0) continue;
+        const T theta = (m[qq][qq] - m[pp][pp]) / (2 * apq);
+        const T tn = (theta >= 0 ? T(1) : T(-1)) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+        const T cs = 1 / std::sqrt(tn * tn + 1), sn = tn * cs;
+        m[pp][pp] -= tn * apq;
+        m[qq][qq] += tn * apq;
+        m[pp][qq] = m[qq][pp] = 0;
+        const int k = 3 - pp - qq;
+        const T akp = m[k][pp], akq = m[k][qq];
+        m[k][pp] = m[pp][k] = cs * akp - sn * akq;
+        m[k][qq] = m[qq][k] = sn * akp + cs * akq;
+        for (int w = 0; w < 3; ++w) {
+          const T ep = e[w][pp], eq = e[w][qq];
+          e[w][pp] = cs * ep - sn * eq;
+          e[w][qq] = sn * ep + cs * eq;
+        }
+      }
+    }
+    for (int r = 0; r < 3; ++r)
+      for (int cc = 0; cc < 3; ++cc) inv[r][cc] = 0;
+    for (int k = 0; k < 3; ++k) {
+      const T lam = m[k][k];
+      if (!(std::fabs(lam) > static_cast<T>(1e-2))) continue;
+      for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc) inv[r][cc] += e[r][k] * e[cc][k] / lam;
+    }
+  }
+  // extrapolate_pressure_to_pellet (runner.hpp:2543-2607): weighted plane
+  // fit of the fluid neighbours' pressure, clamped at 0 (fmax: a pellet
+  // without fluid neighbours gets 0)
+  void extrapolate_pressure(std::vector<T>& pr) {
+#pragma omp parallel for schedule(static)
+    for (long kk = 0; kk < long(np); ++kk) {
+      const unsigned i = max_n + unsigned(kk);
+      const unsigned nn = num_neighbors[i];
+      T w_sum = 0;
+      V3<T> center{0, 0, 0};
+      for (unsigned s = 0; s < nn; ++s) {
+        size_t o = size_t(i) * c.kmax + s;
+        T w = displacement_cubic_kernel(nb_x[o], c) * c.vol;
+        w_sum += w;
+        center += w * x[nb_id[o]];
+      }
+      center = V3<T>{center.x / w_sum, center.y / w_sum, center.z / w_sum};
+      T alpha = 0;
+      V3<T> dg{0, 0, 0}, od{0, 0, 0}, pxs{0, 0, 0};
+      for (unsigned s = 0; s < nn; ++s) {
+        size_t o = size_t(i) * c.kmax + s;
+        unsigned j = nb_id[o];
+        V3<T> xt = x[j] - center;
+        T w = displacement_cubic_kernel(nb_x[o], c) * c.vol;
+        T wp = w * pr[j];
+        alpha += wp;
+        pxs += wp * xt;
+        dg += w * V3<T>{xt.x * xt.x, xt.y * xt.y, xt.z * xt.z};
+        od += w * V3<T>{xt.x * xt.y, xt.x * xt.z, xt.y * xt.z};
+      }
+      alpha /= w_sum;
+      T inv[3][3];
+      sym_pinv(dg, od, inv);
+      V3<T> plane{inv[0][0] * pxs.x + inv[0][1] * pxs.y + inv[0][2] * pxs.z,
+                  inv[1][0] * pxs.x + inv[1][1] * pxs.y + inv[1][2] * pxs.z,
+                  inv[2][0] * pxs.x + inv[2][1] * pxs.y + inv[2][2] * pxs.z};
+      T p = alpha + dot(plane, x[i] - center);
+      pr[i] = std::fmax(p, static_cast<T>(0));
+    }
+  }
+  // collision_test_with_pellets (runner.hpp:3627-3727): a world pellet of
+  // body i against body j, with the device kinematics of the step start
+  void collision_test_pellet(unsigned k, unsigned j) {
+    const unsigned i = pellet_body[k];
+    if (i == j || (bnd[i].mass == 0 && bnd[j].mass == 0)) return;
+    const Boundary<T>& Bi = bnd[i];
+    const Boundary<T>& Bj = bnd[j];
+    const Kin &Ki = kin0[i], &Kj = kin0[j];
+    Contact<T> C{};
+    C.cp_i = x[max_n + k];
+    V3<T> local = rotate(C.cp_i - Kj.x, conjugate(Kj.q));
+    T dist = Bj.sdf.signed_distance(local) * Bj.sign - contact_tolerance;
+    V3<T> nn = Bj.sdf.gradient(local, static_cast<T>(1e-3)) * Bj.sign;
+    nn = nn * (1 / std::sqrt(length_sqr(nn)));
+    if (!(dist < 0)) return;
+    ++num_found;
+    C.i = i;
+    C.j = j;
+    C.cp_j = rotate(local - dist * nn, Kj.q) + Kj.x;
+    C.n = rotate(nn, Kj.q);
+    C.friction = Bi.friction + Bj.friction;
+    V3<T> r_i = C.cp_i - Ki.x, r_j = C.cp_j - Kj.x;
+    V3<T> u_rel = (Ki.v + cross(Ki.omega, r_i)) - (Kj.v + cross(Kj.omega, r_j));
+    T u_rel_n = dot(C.n, u_rel);
+    C.t = u_rel - u_rel_n * C.n;
+    if (length_sqr(C.t) > static_cast<T>(1e-6)) C.t = C.t * (1 / std::sqrt(length_sqr(C.t)));
+    congruent({1 / Bi.inertia.x, 1 / Bi.inertia.y, 1 / Bi.inertia.z}, Ki.q, C.iiwi_diag,
+              C.iiwi_off_diag);
+    congruent({1 / Bj.inertia.x, 1 / Bj.inertia.y, 1 / Bj.inertia.z}, Kj.q, C.iiwj_diag,
+              C.iiwj_off_diag);
+    V3<T> kdi, koi, kdj, koj;
+    contact_k(r_i, Bi.mass, C.iiwi_diag, C.iiwi_off_diag, kdi, koi);
+    contact_k(r_j, Bj.mass, C.iiwj_diag, C.iiwj_off_diag, kdj, koj);
+    V3<T> kd = kdi + kdj, ko = koi + koj;
+    C.nkninv = 1 / dot(C.n, sym_apply(C.n, kd, ko));
+    C.pmax = 1 / dot(C.t, sym_apply(C.t, kd, ko)) * dot(u_rel, C.t);
+    C.goalu = u_rel_n < 0 ? -(Bi.restitution * Bj.restitution) * u_rel_n : T(0);
+    C.impulse_sum = 0;
+    if (contacts.size() < max_num_contacts) contacts.push_back(C);
+  }
+  // Pile::find_contacts(particle_x, max_num_beads), pellet branch
+  // (pile.hpp:858-886): one test per body j, or per hinted run of identical
+  // bodies; contacts in (run, pellet, j) order; box-shell bodies have no
+  // pellet test (runner.hpp:5980-6060)
+  void find_contacts_pellets() {
+    num_found = 0;
+    contacts.clear();
+    const unsigned nb = unsigned(bnd.size());
+    unsigned j = 0, seq = 0;
+    while (j < nb) {
+      unsigned jb = j, je = j + 1;
+      if (seq < ident.size() && j == ident[seq].first && j < ident[seq].second) {
+        jb = ident[seq].first;
+        je = std::min(nb, ident[seq].second);
+        ++seq;
+      }
+      if (bnd[jb].sdf.type != ORC_BOX_SHELL)
+        for (unsigned k = 0; k < np; ++k)
+          for (unsigned jj = jb; jj < je; ++jj) collision_test_pellet(k, jj);
+      j = je;
+    }
+  }
+  // SolverI::step with pellets (solver_i.hpp:79-330, VolumeMethod::pellets)
+  void step_i_pellets() {
+    clear_force();
+    for (unsigned i = 0; i < n; ++i) a[i] = c.g;
+    snapshot_kinematics();
+    transform_all_pellets();
+    update_particle_neighbors();
+    compute_viscosity_with_pellets();
+    update_dt();
+    for (unsigned i = 0; i < n; ++i) {
+      v[i] += dt * a[i];
+      last_pressure[i] = pressure[i] * static_cast<T>(0.5);
+    }
+    const T eps = static_cast<T>(0.01) * c.h2;
+    // calculate_isph_diagonal_adv_density_with_pellets (runner.hpp:2951-3001)
+#pragma omp parallel for schedule(static)
+    for (long ii = 0; ii < long(n); ++ii) {
+      unsigned i = unsigned(ii);
+      V3<T> vi = v[i];
+      T dens = density[i];
+      T dadv = dens / c.rho0;
+      T diag = 0;
+      for (unsigned s = 0; s < num_neighbors[i]; ++s) {
+        size_t o = size_t(i) * c.kmax + s;
+        unsigned j = nb_id[o];
+        V3<T> xixj = nb_x[o];
+        V3<T> grad_w = displacement_cubic_kernel_grad(xixj, c);
+        T dj = density[j];
+        dadv += dt * c.vol * dot(vi - v[j], grad_w);
+        diag += c.vol * (dens + dj) / (dens * dj) / (length_sqr(xixj) + eps) * dot(xixj, grad_w);
+      }
+      for (unsigned s = 0; s < num_pnb[i]; ++s) {
+        size_t o = size_t(i) * c.kmax + s;
+        unsigned j = pnb_id[o];
+        V3<T> xixj = pnb_x[o];
+        V3<T> g = c.vol * displacement_cubic_kernel_grad(xixj, c);
+        dadv += dt * dot(vi - v[j], g);
+        diag += (dens + c.rho0) / (dens * c.rho0) / (length_sqr(xixj) + eps) * dot(xixj, g);
+      }
+      aii[i] = diag * c.rho0;
+      adv_density[i] = dadv;
+    }
+    T mean = std::numeric_limits<T>::max();
+    unsigned it = 0;
+    while ((mean > static_cast<T>(set.density_error_tolerance) || it < set.min_density_solve) &&
+           it < set.max_density_solve) {
+      extrapolate_pressure(last_pressure);
+      // isph_solve_iteration_with_pellets (runner.hpp:3055-3118)
+#pragma omp parallel for schedule(static)
+      for (long ii = 0; ii < long(n); ++ii) {
+        unsigned i = unsigned(ii);
+        T dens = density[i];
+        T diag = aii[i];
+        T bb = 1 - adv_density[i];
+        T dt2 = dt * dt;
+        T denom = diag * dt2;
+        const T w = static_cast<T>(2) / 3;
+        T off = 0;
+        for (unsigned s = 0; s < num_neighbors[i]; ++s) {
+          size_t o = size_t(i) * c.kmax + s;
+          unsigned j = nb_id[o];
+          V3<T> xixj = nb_x[o];
+          T dj = density[j];
+          off -= c.vol * (dens + dj) / (dens * dj) / (length_sqr(xixj) + eps) *
+                 dot(xixj, displacement_cubic_kernel_grad(xixj, c)) * last_pressure[j];
+        }
+        for (unsigned s = 0; s < num_pnb[i]; ++s) {
+          size_t o = size_t(i) * c.kmax + s;
+          unsigned j = pnb_id[o];
+          V3<T> xixj = pnb_x[o];
+          T dj = c.rho0;
+          off -= c.vol * (dens + dj) / (dens * dj) / (length_sqr(xixj) + eps) *
+                 dot(xixj, displacement_cubic_kernel_grad(xixj, c)) * last_pressure[j];
+        }
+        off *= c.rho0;
+        T p = 0;
+        if (std::fabs(denom) > static_cast<T>(1.0e-9))
+          p = std::max((1 - w) * last_pressure[i] + w * (bb / denom - off / diag), static_cast<T>(0));
+        density_err[i] = p == 0 ? 0 : (denom * p + off * dt2 - bb);
+        pressure[i] = p;
+      }
+      for (unsigned i = 0; i < n; ++i) last_pressure[i] = pressure[i];
+      mean = static_cast<T>(sum_of(density_err) / double(n));
+      ++it;
+    }
+    rep.num_density_solve = it;
+    rep.mean_density_error = mean;
+    extrapolate_pressure(pressure);
     pressure_accels_and_integrate();
   }
 
@@ -1839,11 +2253,12 @@ int array_io(Solver<T>& s, const char* name, const void* src, void* dst) {
   bool in = src != nullptr;
   std::string nm(name);
   size_t N = s.max_n;
+  size_t NT = s.nt();
   size_t BN = s.bnd.size() * N;
 #define ALV_ORC_ARR(str, vec, cnt)                     \
   if (nm == str) return copy_vec<T>(vec, cnt, src, in, dst);
-  ALV_ORC_ARR("x", s.x, N)
-  ALV_ORC_ARR("v", s.v, N)
+  ALV_ORC_ARR("x", s.x, NT)
+  ALV_ORC_ARR("v", s.v, NT)
   ALV_ORC_ARR("a", s.a, N)
   ALV_ORC_ARR("density", s.density, N)
   ALV_ORC_ARR("dfsph_factor", s.dfsph_factor, N)
@@ -1851,15 +2266,16 @@ int array_io(Solver<T>& s, const char* name, const void* src, void* dst) {
   ALV_ORC_ARR("kappa_v", s.kappa_v, N)
   ALV_ORC_ARR("density_adv", s.density_adv, N)
   ALV_ORC_ARR("cfl_v2", s.cfl_v2, N)
-  ALV_ORC_ARR("pressure", s.pressure, N)
-  ALV_ORC_ARR("last_pressure", s.last_pressure, N)
+  ALV_ORC_ARR("pressure", s.pressure, NT)
+  ALV_ORC_ARR("last_pressure", s.last_pressure, NT)
   ALV_ORC_ARR("aii", s.aii, N)
   ALV_ORC_ARR("dii", s.dii, N)
   ALV_ORC_ARR("dij_pj", s.dij_pj, N)
   ALV_ORC_ARR("adv_density", s.adv_density, N)
   ALV_ORC_ARR("pressure_accel", s.pressure_accel, N)
   ALV_ORC_ARR("density_err", s.density_err, N)
-  ALV_ORC_ARR("num_neighbors", s.num_neighbors, N)
+  ALV_ORC_ARR("num_neighbors", s.num_neighbors, NT)
+  ALV_ORC_ARR("num_pellet_neighbors", s.num_pnb, NT)
   ALV_ORC_ARR("boundary", s.boundary, BN)
   ALV_ORC_ARR("boundary_kernel", s.boundary_kernel, BN)
   ALV_ORC_ARR("force", s.force, BN)
@@ -2254,6 +2670,70 @@ int orc_get_neighbors(void* h, unsigned* ids, void* xij) {
   });
 }
 
+int orc_enable_pellets(void* h, unsigned max_num_pellets) {
+  return dispatch(h, [&](auto& s) {
+    s.pellet_mode = true;
+    s.max_p = max_num_pellets;
+    s.alloc();
+    return 0;
+  });
+}
+
+int orc_refresh_pellets(void* h, unsigned* num_pellets) {
+  return dispatch(h, [&](auto& s) {
+    int rc = s.refresh_pellets();
+    if (num_pellets) *num_pellets = s.np;
+    return rc;
+  });
+}
+
+int orc_hint_identical_sequence(void* h, unsigned begin, unsigned end) {
+  return dispatch(h, [&](auto& s) {
+    s.ident.push_back({begin, end});
+    return 0;
+  });
+}
+
+int orc_transform_pellets(void* h) {
+  return dispatch(h, [&](auto& s) {
+    if (!s.pellet_mode || s.refresh_pellets() != 0) return -4;
+    s.snapshot_kinematics();
+    s.transform_all_pellets();
+    return 0;
+  });
+}
+
+int orc_find_contacts_pellets(void* h, unsigned* num_found) {
+  return dispatch(h, [&](auto& s) {
+    if (!s.pellet_mode || s.refresh_pellets() != 0) return -4;
+    s.snapshot_kinematics();
+    s.transform_all_pellets();
+    s.find_contacts_pellets();
+    if (num_found) *num_found = s.num_found;
+    return 0;
+  });
+}
+
+int orc_extrapolate_pressure(void* h, int last) {
+  return dispatch(h, [&](auto& s) {
+    if (!s.pellet_mode) return -4;
+    s.extrapolate_pressure(last ? s.last_pressure : s.pressure);
+    return 0;
+  });
+}
+
+int orc_get_lists(void* h, int pellet_list, unsigned* ids, void* xij) {
+  return dispatch(h, [&](auto& s) {
+    size_t cnt = s.nt() * s.c.kmax;
+    if (pellet_list && !s.pellet_mode) return -4;
+    const auto& I = pellet_list ? s.pnb_id : s.nb_id;
+    const auto& X = pellet_list ? s.pnb_x : s.nb_x;
+    if (ids) std::memcpy(ids, I.data(), cnt * sizeof(unsigned));
+    if (xij) std::memcpy(xij, X.data(), cnt * sizeof(X[0]));
+    return 0;
+  });
+}
+
 int orc_get_grid(void* h, unsigned* cell_count, unsigned* cell_start,
                  unsigned* anomaly) {
   return dispatch(h, [&](auto& s) {
@@ -2270,6 +2750,7 @@ int orc_get_grid(void* h, unsigned* cell_count, unsigned* cell_start,
 
 int orc_step(void* h) {
   return dispatch(h, [&](auto& s) {
+    if (s.pellet_mode && (s.kind != 2 || s.refresh_pellets() != 0)) return -4;
     if (s.kind == 0)
       s.step_df();
     else if (s.kind == 2)

@@ -178,6 +178,26 @@ int orc_get_grid(void* h, unsigned* cell_count, unsigned* cell_start,
 int orc_step(void* h);
 int orc_get_report(void* h, orc_report* r);
 
+/* Particle-sampled boundaries (Pile with VolumeMethod::pellets,
+ * pile.hpp:16,217-290; SolverI with pellets, solver_i.hpp:79-330).  After
+ * orc_enable_pellets (which reallocates every array) the collision vertices
+ * of each body (orc_set_collision) are its pellets, in the body frame;
+ * pellet k of their concatenation is particle max_num_particles + k of x, v,
+ * pressure, last_pressure, num_neighbors, num_pellet_neighbors and of the
+ * lists.  orc_refresh_pellets checks the pellet count against the limit
+ * (-4 when over).  orc_transform_pellets and orc_find_contacts_pellets use
+ * the current body kinematics as the step start's.  orc_get_lists:
+ * pellet_list 0 = fluid (bead) lists, 1 = pellet lists, [max_n + max_p][K]. */
+int orc_enable_pellets(void* h, unsigned max_num_pellets);
+int orc_refresh_pellets(void* h, unsigned* num_pellets);
+int orc_hint_identical_sequence(void* h, unsigned begin, unsigned end);
+int orc_transform_pellets(void* h);
+int orc_find_contacts_pellets(void* h, unsigned* num_found);
+/* MLS extrapolation onto the pellets from pressure (last = 0) or
+ * last_pressure (last = 1) (runner.hpp:2543-2607) */
+int orc_extrapolate_pressure(void* h, int last);
+int orc_get_lists(void* h, int pellet_list, unsigned* ids, void* xij);
+
 /* Stand-alone helpers used by the tests. */
 int orc_peirce_constants(double* cosphi, double* sinphi, double* b,
                          double* r, double* c);

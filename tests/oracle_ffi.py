@@ -127,6 +127,13 @@ def lib():
         L.orc_compute_all_boundaries.argtypes = [vp]
         L.orc_update_particle_neighbors.argtypes = [vp]
         L.orc_get_neighbors.argtypes = [vp, vp, vp]
+        L.orc_enable_pellets.argtypes = [vp, ctypes.c_uint]
+        L.orc_refresh_pellets.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
+        L.orc_hint_identical_sequence.argtypes = [vp, ctypes.c_uint, ctypes.c_uint]
+        L.orc_transform_pellets.argtypes = [vp]
+        L.orc_find_contacts_pellets.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
+        L.orc_extrapolate_pressure.argtypes = [vp, ctypes.c_int]
+        L.orc_get_lists.argtypes = [vp, ctypes.c_int, vp, vp]
         L.orc_get_grid.argtypes = [vp, vp, vp, vp]
         L.orc_step.argtypes = [vp]
         L.orc_get_report.argtypes = [vp, ctypes.POINTER(OrcReport)]
@@ -208,6 +215,11 @@ class Oracle:
                                    ctypes.byref(p))
         if not self.h:
             raise RuntimeError("orc_create failed")
+        self.pellets = getattr(scene, "volume_method", "volume_map") == "pellets"
+        self.max_p = scene.max_num_pellets if self.pellets else 0
+        if self.pellets:
+            self.L.orc_enable_pellets(self.h, self.max_p)
+        self.nt = self.max_n + self.max_p
         self.L.orc_set_num_particles(self.h, scene.n)
         self.n = scene.n
         s = OrcSettings()
@@ -246,10 +258,18 @@ class Oracle:
             verts = np.ascontiguousarray(np.asarray(verts, np.float32).reshape(-1, 3), np.float64)
             self.L.orc_set_collision(self.h, k, b.restitution, b.friction, _dptr(verts),
                                      verts.shape[0])
-        full = np.zeros((self.max_n, 3), self.dtype)
+        for b0, b1 in getattr(scene, "identical_sequences", ()):
+            self.L.orc_hint_identical_sequence(self.h, b0, b1)
+        self.np = 0
+        if self.pellets:
+            cnt = ctypes.c_uint(0)
+            if self.L.orc_refresh_pellets(self.h, ctypes.byref(cnt)) != 0:
+                raise ValueError("pellet count exceeds max_num_pellets")
+            self.np = cnt.value
+        full = np.zeros((self.nt, 3), self.dtype)
         full[: scene.n] = scene.x
         self.set("x", full)
-        full = np.zeros((self.max_n, 3), self.dtype)
+        full = np.zeros((self.nt, 3), self.dtype)
         full[: scene.n] = scene.v
         self.set("v", full)
 
@@ -266,6 +286,12 @@ class Oracle:
     # arrays
     def _shape(self, name):
         nb = len(self.scene.boundaries)
+        if name in ("x", "v"):
+            return (self.nt, 3), self.dtype
+        if name in ("pressure", "last_pressure"):
+            return (self.nt,), self.dtype
+        if name in ("num_neighbors", "num_pellet_neighbors"):
+            return (self.nt,), np.uint32
         if name in _VEC_ARRAYS:
             return (self.max_n, 3), self.dtype
         if name in ("boundary", "boundary_kernel"):
@@ -427,6 +453,30 @@ class Oracle:
         self.L.orc_get_neighbors(self.h, ids.ctypes.data, None)
         return ids, self.get("num_neighbors")
 
+    def lists(self, pellet_list=False):
+        """(ids, xixj, counts) of the fluid (bead) or pellet lists,
+        [max_n + max_p] rows."""
+        ids = np.zeros((self.nt, self.kmax), np.uint32)
+        xij = np.zeros((self.nt, self.kmax, 3), self.dtype)
+        rc = self.L.orc_get_lists(self.h, int(pellet_list), ids.ctypes.data, xij.ctypes.data)
+        if rc != 0:
+            raise ValueError("orc_get_lists: %d" % rc)
+        return ids, xij, self.get("num_pellet_neighbors" if pellet_list else "num_neighbors")
+
+    def transform_pellets(self):
+        if self.L.orc_transform_pellets(self.h) != 0:
+            raise ValueError("orc_transform_pellets")
+
+    def find_contacts_pellets(self):
+        n = ctypes.c_uint(0)
+        if self.L.orc_find_contacts_pellets(self.h, ctypes.byref(n)) != 0:
+            raise ValueError("orc_find_contacts_pellets")
+        return n.value
+
+    def extrapolate_pressure(self, last=False):
+        if self.L.orc_extrapolate_pressure(self.h, int(last)) != 0:
+            raise ValueError("orc_extrapolate_pressure")
+
     def grid(self):
         C = int(np.prod(self.scene.grid_res))
         cnt = np.zeros(C, np.uint32)
@@ -436,7 +486,9 @@ class Oracle:
         return cnt, start, an
 
     def step(self):
-        self.L.orc_step(self.h)
+        rc = self.L.orc_step(self.h)
+        if rc != 0:
+            raise ValueError("orc_step: %d" % rc)
 
     def report(self):
         r = OrcReport()

@@ -1,0 +1,115 @@
+"""Particle-sampled boundaries (pellets) on the GPU against the oracle:
+the fluid and pellet lists bit for bit, the density with pellets, ISPH steps
+with pellets (FP32, 1e-5 on the first step), and the pellet contacts."""
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+from test_gpu_contacts import compare_contacts
+from test_oracle_pellets import small_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+def pellet_lists_orc(orc, pellet_list, rows):
+    ids, _, cnt = orc.lists(pellet_list)
+    return ids[rows], cnt[rows]
+
+
+def test_pellet_lists_and_density_match_oracle():
+    from alluvion_amd import scenes
+    sc = small_scene()
+    dp, pile, solver, _, _ = scenes.build_solver(sc, kind="i")
+    assert pile.num_pellets == sum(b.collision_vertices.shape[0] for b in sc.boundaries)
+    solver.transform_all_pellets()
+    solver.update_particle_neighbors()
+    orc = of.Oracle(sc, precision=4, kind="i")
+    orc.transform_pellets()
+    orc.update_particle_neighbors()
+    n, mx, npel = sc.n, orc.max_n, pile.num_pellets
+    K = sc.max_num_neighbors
+    x_g = dp.coat(solver.particle_x).get()
+    np.testing.assert_array_equal(x_g[mx:mx + npel], orc.get("x")[mx:mx + npel])
+    ids, cnt = solver.get_neighbors()
+    oids, ocnt = pellet_lists_orc(orc, False, slice(0, n))
+    np.testing.assert_array_equal(cnt[:n], ocnt)
+    for i in range(n):
+        np.testing.assert_array_equal(ids[i, :cnt[i]], oids[i, :cnt[i]], err_msg=str(i))
+    pids, pcnt = solver.get_pellet_neighbors(0, mx)
+    oids, ocnt = pellet_lists_orc(orc, True, slice(0, n))
+    np.testing.assert_array_equal(pcnt[:n], ocnt)
+    assert ocnt.max() > 0
+    for i in range(n):
+        np.testing.assert_array_equal(pids[i, :pcnt[i]], oids[i, :pcnt[i]], err_msg=str(i))
+    bids, bcnt = solver.get_pellet_neighbors(1, sc.max_num_pellets)
+    oids, ocnt = pellet_lists_orc(orc, False, slice(mx, mx + npel))
+    np.testing.assert_array_equal(bcnt[:npel], ocnt)
+    assert ocnt.max() > 0
+    for k in range(npel):
+        np.testing.assert_array_equal(bids[k, :bcnt[k]], oids[k, :bcnt[k]], err_msg=str(k))
+    assert K == ids.shape[1]
+    dens = dp.coat(solver.particle_density).get()[:n]
+    assert rel(dens, orc.get("density")[:n]) < 1e-6
+    assert np.all(dp.coat(solver.grid_anomaly).get() == 0)
+
+
+def body_state(pile, b):
+    q = pile.get_q(b)
+    return (np.array(list(pile.get_x(b))), np.array(list(pile.get_v(b))),
+            np.array([q.x, q.y, q.z, q.w]), np.array(list(pile.get_omega(b))))
+
+
+def test_isph_steps_with_pellets_match_oracle():
+    from alluvion_amd import scenes
+    sc = small_scene()
+    dp, pile, solver, _, _ = scenes.build_solver(sc, kind="i")
+    orc = of.Oracle(sc, precision=4, kind="i")
+    n, mx, npel = sc.n, orc.max_n, pile.num_pellets
+    tol = [1e-5, 3e-5, 1e-4, 3e-4, 1e-3]
+    for step in range(5):
+        solver.step()
+        orc.step()
+        x_g = dp.coat(solver.particle_x).get()
+        v_g = dp.coat(solver.particle_v).get()
+        p_g = dp.coat(solver.particle_pressure).get()
+        assert rel(x_g[:n], orc.get("x")[:n]) < 1e-6, step
+        assert rel(v_g[:n], orc.get("v")[:n]) < tol[step], step
+        p_o = orc.get("pressure")
+        assert rel(p_g[:n], p_o[:n]) < tol[step] * 10, step
+        assert rel(p_g[mx:mx + npel], p_o[mx:mx + npel]) < tol[step] * 10, step
+        assert solver.num_density_solve == orc.report()["num_density_solve"], step
+        x, v, q, w = body_state(pile, 1)
+        rb = orc.rigid(1)
+        np.testing.assert_allclose(x, rb["x"], rtol=1e-6, atol=1e-8, err_msg=str(step))
+        np.testing.assert_allclose(v, rb["v"], rtol=tol[step] * 10, atol=1e-6, err_msg=str(step))
+        np.testing.assert_allclose(w, rb["omega"], rtol=tol[step] * 10, atol=1e-5, err_msg=str(step))
+    assert np.abs(orc.rigid(1)["v"]).max() > 0  # the fluid pushes the sphere
+    assert p_o[mx:mx + npel].max() > 0  # pellets carry pressure
+
+
+def test_pellet_contacts_match_oracle():
+    from alluvion_amd import scenes
+    sc = small_scene()
+    sc.boundaries[1].x = (0.01, -0.07 + 0.016, 0.0)  # into the floor
+    sc.contact_tolerance = 1e-4
+    dp, pile, solver, _, _ = scenes.build_solver(sc, kind="i")
+    pile.set_v(1, dp.f3(0.3, -0.8, 0.1))
+    pile.set_omega(1, dp.f3(1.0, 0.5, -2.0))
+    orc = of.Oracle(sc, precision=4, kind="i")
+    orc.set_rigid(1, v=[0.3, -0.8, 0.1], omega=[1.0, 0.5, -2.0])
+    solver.transform_all_pellets()
+    pile.copy_kinematics_to_device()
+    n = pile.find_contacts(solver.particle_x, solver.max_num_particles)
+    n_orc = orc.find_contacts_pellets()
+    assert n == n_orc > 5, (n, n_orc)
+    compare_contacts(pile.get_contacts(), of.contact_dicts(orc.contacts()))
+    pile.solve_contacts()
+    orc.solve_contacts()
+    x, v, q, w = body_state(pile, 1)
+    rb = orc.rigid(1)
+    np.testing.assert_allclose(v, rb["v"], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(w, rb["omega"], rtol=1e-5, atol=1e-5)

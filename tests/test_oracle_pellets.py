@@ -1,0 +1,259 @@
+"""Pins the oracle's particle-sampled boundaries (pellets: Pile with
+VolumeMethod::pellets, SolverI::step with pellets, solver_i.hpp:79-330) with
+facts that do not go through its formulas:
+
+* transform_pellets (runner.hpp:4071-4085): world pellets are R(q) p + x and
+  move with v + w x R(q) p;
+* make_bead_pellet_neighbor_list (runner.hpp:1913-1971): the fluid and the
+  pellet lists of every fluid particle and every pellet are the brute-force
+  neighbour sets, in the reference order (27 cells, dx outer, dz inner;
+  ascending id inside a cell);
+* compute_density_with_pellets (runner.hpp:2074-2098): on a perfect
+  periodic lattice where some sites are pellets the density of every fluid
+  particle is the closed lattice sum m (W(0) + sum W), whatever the labels;
+* extrapolate_pressure_to_pellet (runner.hpp:2543-2607): with a moment
+  matrix above the 1e-2 singular-value cut the MLS plane fit reproduces a
+  linear pressure field exactly; below the cut it is the kernel-weighted
+  mean; no fluid neighbours or a negative fit give 0;
+* the pellet coupling of the pressure accelerations
+  (compute_pressure_accels_with_pellets, runner.hpp:2819-2870) conserves
+  linear and angular momentum between fluid and bodies;
+* collision_test_with_pellets (runner.hpp:3627-3727, pile.hpp:858-886):
+  the contacts are exactly the pellets inside another body's distance, in
+  (run, pellet, j) order, with hinted runs of identical bodies.
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+from alluvion_amd import scenes
+from test_oracle_contacts import quat_matrix
+from test_oracle_solver import H, R, cubic_w, lattice_sum, periodic_lattice
+
+
+def box_sdf(p, widths):
+    q = np.abs(p) - 0.5 * np.asarray(widths)
+    return np.linalg.norm(np.maximum(q, 0), axis=1) + np.minimum(q.max(1), 0)
+
+
+def pellet_world(sc, orc):
+    """World pellets as the oracle placed them, and the body of each."""
+    x = orc.get("x")
+    body = np.concatenate([np.full(b.collision_vertices.shape[0], k)
+                           for k, b in enumerate(sc.boundaries)])
+    return x[orc.max_n:orc.max_n + body.size], body
+
+
+def small_scene():
+    """8 x 8 x 6 fluid particles in the corner of the pellet container; the
+    sphere dipped 6 mm into the fluid's top face."""
+    r, edge = 0.0025, 0.14
+    top = -0.5 * edge - r + 2 * r * 8
+    return scenes.dam_break_pellets(nx=8, ny=8, nz=6, edge=edge,
+                                    sphere_x=(-0.045, top + 0.02 - 0.006, -0.05))
+
+
+def cell_order_key(sc, xi, xj, ids):
+    cs = np.floor(xi / sc.kernel_radius).astype(int)
+    cj = np.floor(xj / sc.kernel_radius).astype(int)
+    off = cj - cs + 1
+    assert np.all((off >= 0) & (off <= 2))
+    return (off[:, 0] * 9 + off[:, 1] * 3 + off[:, 2]) * (1 << 30) + ids
+
+
+def test_transform_and_split_lists_match_brute_force():
+    sc = small_scene()
+    q = np.array([0.1, -0.3, 0.2, 0.9])
+    q /= np.linalg.norm(q)
+    orc = of.Oracle(sc, precision=8, kind="i")
+    orc.set_rigid(1, v=[0.1, -0.2, 0.3], q=list(q), omega=[1.0, -2.0, 0.5])
+    orc.transform_pellets()
+    wp, body = pellet_world(sc, orc)
+    loc = sc.boundaries[1].collision_vertices.astype(np.float64)
+    Rm = quat_matrix(q)
+    sel = body == 1
+    np.testing.assert_allclose(wp[sel], loc @ Rm.T + np.array(sc.boundaries[1].x), atol=1e-12)
+    np.testing.assert_allclose(wp[~sel], sc.boundaries[0].collision_vertices, atol=1e-12)
+    vv = orc.get("v")[orc.max_n:orc.max_n + body.size]
+    np.testing.assert_allclose(vv[sel], np.array([0.1, -0.2, 0.3]) + np.cross([1.0, -2.0, 0.5],
+                                                                               loc @ Rm.T),
+                               atol=1e-12)
+    assert np.all(vv[~sel] == 0)
+
+    orc.update_particle_neighbors()
+    bids, _, bcnt = orc.lists(False)
+    pids, _, pcnt = orc.lists(True)
+    n, mx = sc.n, orc.max_n
+    x = orc.get("x")
+    allx = np.concatenate([x[:n], wp])
+    allid = np.concatenate([np.arange(n), mx + np.arange(body.size)])
+    rows = list(range(n)) + [mx + k for k in range(0, body.size, 7)]
+    for i in rows:
+        d2 = ((allx - x[i]) ** 2).sum(1)
+        near = allid[(d2 < H * H) & (allid != i)]
+        want_b, want_p = near[near < mx], near[near >= mx]
+        got_b, got_p = bids[i, :bcnt[i]], pids[i, :pcnt[i]]
+        assert set(got_b.tolist()) == set(want_b.tolist()), i
+        assert set(got_p.tolist()) == set(want_p.tolist()), i
+        for got in (got_b, got_p):
+            if got.size:
+                key = cell_order_key(sc, x[i], x[got], got)
+                assert np.all(np.diff(key) > 0), i
+    assert pcnt[:n].max() > 0 and bcnt[mx:mx + body.size].max() > 0
+
+
+def test_lattice_density_with_pellet_sites_is_the_lattice_sum():
+    S = lattice_sum(2 * R, H)
+    mass = 1000.0 / S
+    sc, x, v = periodic_lattice(m=20, mass=mass)
+    site = np.arange(x.shape[0])
+    pel = (site % 7 == 3)
+    sc.x = x[~pel].astype(np.float32)
+    sc.v = v[~pel].astype(np.float32)
+    sc.volume_method = "pellets"
+    sc.max_num_pellets = int(pel.sum())
+    sc.boundaries = [scenes.BoundarySpec("box", (0.05, 0.05, 0.05, 0.0), 1.0, (4, 4, 4),
+                                         collision_vertices=x[pel])]
+    orc = of.Oracle(sc, precision=8, kind="i")
+    orc.set("x", np.concatenate([x[~pel], x[pel]]))
+    orc.transform_pellets()
+    orc.update_particle_neighbors()
+    # pellets travel as FP32 (as the product's pile stores them): their
+    # sites are off the lattice by ~1e-9 m, which moves rho by ~1e-7
+    np.testing.assert_allclose(orc.get("density")[:sc.n], mass * S, rtol=1e-6)
+    _, _, pcnt = orc.lists(True)
+    assert pcnt[:sc.n].min() > 0  # every particle sees pellets
+
+
+def mls_scene(h, radius, pellet_x):
+    """A 6^3 fluid lattice block (spacing 2r) with one static body whose
+    pellets are `pellet_x`."""
+    d = 2 * radius
+    g = np.arange(6) * d
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    x = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    c = int(np.ceil((7 * d + h) / h)) + 2
+    sc = scenes.Scene(name="mls", x=x.astype(np.float32), v=np.zeros_like(x, np.float32),
+                      particle_radius=radius, kernel_radius=h,
+                      particle_mass=1000.0 * d ** 3, grid_offset=(-c,) * 3, grid_res=(2 * c,) * 3,
+                      boundaries=[scenes.BoundarySpec("box", (0.1, 0.1, 0.1, 0.0), 1.0, (4, 4, 4),
+                                                      collision_vertices=np.asarray(pellet_x))],
+                      volume_method="pellets", max_num_pellets=len(pellet_x))
+    return sc
+
+
+def test_mls_extrapolation_reproduces_linear_fields_and_weighted_means():
+    # large scale (spacing 1, h = 2): the moment matrix sum vol W x_t x_t^T
+    # is O(1), above the 1e-2 cut
+    r, h = 0.5, 2.0
+    pel = np.array([[2.5, 2.5, 5.5], [2.5, -0.5, 2.5], [1.5, 3.5, 2.5], [25.0, 25.0, 25.0]])
+    sc = mls_scene(h, r, pel)
+    orc = of.Oracle(sc, precision=8, kind="i")
+    orc.transform_pellets()
+    orc.update_particle_neighbors()
+    x = orc.get("x")
+    n, mx = sc.n, orc.max_n
+    g = np.array([1.0, -2.0, 0.5])
+    pr = np.zeros(mx + 4)
+    pr[:n] = 5.0 + x[:n] @ g
+    orc.set("pressure", pr)
+    orc.extrapolate_pressure(last=False)
+    got = orc.get("pressure")[mx:mx + 4]
+    np.testing.assert_allclose(got[:3], 5.0 + pel[:3] @ g, rtol=1e-10)
+    assert got[3] == 0.0  # no fluid neighbours: 0/0 -> fmax(NaN, 0) = 0
+    # a fit that is negative at the pellet is clamped; elsewhere reproduced
+    pr[:n] = 0.05 + x[:n, 1]
+    orc.set("pressure", pr)
+    orc.extrapolate_pressure(last=False)
+    got = orc.get("pressure")[mx:mx + 4]
+    np.testing.assert_allclose(got[0], 0.05 + 2.5, rtol=1e-10)
+    assert got[1] == 0.0  # 0.05 - 0.5 < 0
+
+    # the weights' scale of a real scene (r = 2.5 mm) leaves every
+    # eigenvalue below the cut: the extrapolation is the weighted mean
+    r, h = 0.0025, 0.01
+    pel = np.array([[0.005, 0.005, 0.0135], [0.0051, -0.0015, 0.006]])
+    sc = mls_scene(h, r, pel)
+    orc = of.Oracle(sc, precision=8, kind="i")
+    orc.transform_pellets()
+    orc.update_particle_neighbors()
+    x = orc.get("x")
+    n, mx = sc.n, orc.max_n
+    pr = np.zeros(mx + 2)
+    pr[:n] = 100.0 + 1e4 * x[:n] @ np.array([1.0, 2.0, -0.5]) + 50 * np.sin(np.arange(n))
+    orc.set("last_pressure", pr)
+    orc.extrapolate_pressure(last=True)
+    got = orc.get("last_pressure")[mx:mx + 2]
+    for k in range(2):
+        w = cubic_w(np.linalg.norm(x[:n] - x[mx + k], axis=1) / h, h)
+        assert (w > 0).sum() > 5
+        np.testing.assert_allclose(got[k], (w * pr[:n]).sum() / w.sum(), rtol=1e-10)
+
+
+def test_pellet_pressure_coupling_conserves_momentum():
+    # fluid and bodies at rest: viscosity adds nothing, so the bodies' forces
+    # are the pressure coupling, whose pair terms are central and opposite
+    sc = small_scene()
+    orc = of.Oracle(sc, precision=8, kind="i")
+    x0 = orc.get("x")[:sc.n].copy()
+    orc.step()
+    n, mx = sc.n, orc.max_n
+    mass, rho0 = sc.particle_mass, sc.density0
+    acc = orc.get("pressure_accel")[:n]
+    F = orc.get("force")[:, :n].sum(1)
+    Tq = orc.get("torque")[:, :n].sum(1)
+    assert np.abs(F[1]).max() > 0 and np.abs(F[0]).max() > 0
+    fluid_p = (mass / rho0) * acc.sum(0)
+    np.testing.assert_allclose(fluid_p + F.sum(0), 0, atol=1e-12 * np.abs(fluid_p).max())
+    bx = np.array([b.x for b in sc.boundaries], np.float64)
+    fluid_l = (mass / rho0) * np.cross(x0, acc).sum(0)
+    body_l = (Tq + np.cross(bx, F)).sum(0)
+    np.testing.assert_allclose(fluid_l + body_l, 0, atol=1e-11 * np.abs(fluid_l).max())
+    assert orc.report()["num_density_solve"] >= sc.min_density_solve
+
+
+@pytest.mark.parametrize("runs", [False, True])
+def test_pellet_contacts_are_the_pellets_inside_other_bodies(runs):
+    sc = small_scene()
+    s1 = sc.boundaries[1]
+    s1.x = (0.01, -0.07 + 0.016, 0.0)  # 4 mm into the floor
+    extra = []
+    if runs:  # an identical second sphere, also touching the floor
+        s2 = scenes.BoundarySpec(s1.kind, s1.params, s1.sign, s1.resolution, mass=s1.mass,
+                                 inertia=s1.inertia, x=(-0.03, -0.07 + 0.017, 0.02), q=s1.q,
+                                 restitution=s1.restitution, friction=s1.friction,
+                                 collision_vertices=s1.collision_vertices)
+        extra = [s2]
+        sc.identical_sequences = [(1, 3)]
+    sc.boundaries = sc.boundaries + extra
+    sc.max_num_pellets = sum(b.collision_vertices.shape[0] for b in sc.boundaries)
+    sc.contact_tolerance = 1e-4
+    orc = of.Oracle(sc, precision=8, kind="i")
+    found = orc.find_contacts_pellets()
+    rows = orc.contacts()
+    wp, body = pellet_world(sc, orc)
+    nb = len(sc.boundaries)
+    runs_l = [(0, 1), (1, 3)] if runs else [(j, j + 1) for j in range(nb)]
+    want = []
+    for jb, je in runs_l:
+        for k in range(wp.shape[0]):
+            i = body[k]
+            for j in range(jb, je):
+                Bj, Bi = sc.boundaries[j], sc.boundaries[i]
+                if i == j or (Bi.mass == 0 and Bj.mass == 0):
+                    continue
+                local = wp[k] - np.array(Bj.x)  # unrotated bodies
+                if Bj.kind == "box":
+                    dist = box_sdf(local[None], Bj.params[:3])[0] * Bj.sign
+                else:
+                    dist = np.linalg.norm(local) - Bj.params[0]
+                if dist - sc.contact_tolerance < 0:
+                    want.append((i, j, k))
+    assert found == len(want) == len(rows) > 5
+    np.testing.assert_array_equal(rows[:, 0], [w[0] for w in want])
+    np.testing.assert_array_equal(rows[:, 1], [w[1] for w in want])
+    np.testing.assert_allclose(rows[:, 2:5], wp[[w[2] for w in want]], atol=1e-12)
+    # floor contacts push up; container pellets inside a sphere push out of it
+    floor = rows[:, 1] == 0
+    np.testing.assert_allclose(rows[floor, 8:11], np.tile([0, 1, 0], (floor.sum(), 1)), atol=1e-9)
+    assert (~floor).sum() > 0
