@@ -2367,10 +2367,70 @@ int alv_solver_sample_neighbors(alv_solver* S, const float* sample_x, unsigned n
     return fail(ALV_ERR_STATE, "no neighbour grid yet: step or update_particle_neighbors first");
   ALV_HIP(hipStreamSynchronize(S->stream));
   SimParams p = make_params(S->cn, S->cni, unsigned(S->pile->bnd.size()), S->set.num_particles);
+  p.pellet_mode = S->par_host[0].pellet_mode;  // the grid of the last search has pellets
   launch_sample_neighbors(p, S->A, sample_x, ns, reinterpret_cast<float4*>(sample_neighbors),
                           counts, nullptr);
   ALV_HIP(hipGetLastError());
   return ALV_OK;
+}
+
+int alv_solver_sample_bead_pellet_neighbors(alv_solver* S, const float* sample_x,
+                                            unsigned num_samples, unsigned offset,
+                                            unsigned max_num_beads, unsigned max_neighbors,
+                                            float* bead_neighbors, unsigned* bead_counts,
+                                            float* pellet_neighbors, unsigned* pellet_counts) {
+  if (!S || (num_samples && (!sample_x || !bead_neighbors || !bead_counts || !pellet_neighbors ||
+                             !pellet_counts)))
+    return fail(ALV_ERR_ARG, "null argument");
+  if (max_neighbors != S->cni.max_num_neighbors_per_particle)
+    return fail(ALV_ERR_ARG, "lists must have max_num_neighbors_per_particle entries per row");
+  if (S->batch) return fail(ALV_ERR_UNSUPPORTED, "sampling a batch solver is not supported");
+  if (!S->grid_valid)
+    return fail(ALV_ERR_STATE, "no neighbour grid yet: step or update_particle_neighbors first");
+  ALV_HIP(hipStreamSynchronize(S->stream));
+  SimParams p = make_params(S->cn, S->cni, unsigned(S->pile->bnd.size()), S->set.num_particles);
+  p.pellet_mode = S->par_host[0].pellet_mode;
+  launch_sample_bead_pellet(p, S->A, sample_x, num_samples, offset, max_num_beads,
+                            reinterpret_cast<float4*>(bead_neighbors), bead_counts,
+                            reinterpret_cast<float4*>(pellet_neighbors), pellet_counts, nullptr);
+  ALV_HIP(hipGetLastError());
+  return ALV_OK;
+}
+
+namespace {
+int pellet_sums(const alv_const* cn, int what, const float* nbr, const unsigned* counts,
+                const float* pnbr, const unsigned* pcounts, unsigned K, unsigned n, float* out) {
+  if (!cn || (n && (!pnbr || !pcounts || !out || (what != 0 && (!nbr || !counts)))))
+    return fail(ALV_ERR_ARG, "null argument");
+  if (K == 0) return fail(ALV_ERR_ARG, "max_neighbors must be positive");
+  alv_consti cni;
+  memset(&cni, 0, sizeof(cni));
+  SimParams p = make_params(*cn, cni, 0, 0);
+  launch_pellet_sums(p, what, reinterpret_cast<const float4*>(nbr), counts,
+                     reinterpret_cast<const float4*>(pnbr), pcounts, K, n, out, nullptr);
+  ALV_HIP(hipGetLastError());
+  return ALV_OK;
+}
+}  // namespace
+
+int alv_particle_boundary_with_pellets(const alv_const* cn, const float* pellet_neighbors,
+                                       const unsigned* pellet_counts, unsigned K, unsigned n,
+                                       float* boundary_kernel) {
+  return pellet_sums(cn, 0, nullptr, nullptr, pellet_neighbors, pellet_counts, K, n,
+                     boundary_kernel);
+}
+
+int alv_density_with_pellets(const alv_const* cn, const float* neighbors, const unsigned* counts,
+                             const float* pellet_neighbors, const unsigned* pellet_counts,
+                             unsigned K, unsigned n, float* density) {
+  return pellet_sums(cn, 1, neighbors, counts, pellet_neighbors, pellet_counts, K, n, density);
+}
+
+int alv_dfsph_factor_with_pellets(const alv_const* cn, const float* neighbors,
+                                  const unsigned* counts, const float* pellet_neighbors,
+                                  const unsigned* pellet_counts, unsigned K, unsigned n,
+                                  float* factor) {
+  return pellet_sums(cn, 2, neighbors, counts, pellet_neighbors, pellet_counts, K, n, factor);
 }
 
 int alv_solver_sample_all_boundaries(alv_solver* S, const float* sample_x, unsigned ns,

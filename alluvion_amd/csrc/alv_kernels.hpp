@@ -220,6 +220,18 @@ void launch_map_flags(const GridGeomDev& g, const float* nodes, unsigned char* o
 // sampling and statistics (alv_sample.hip)
 void launch_sample_neighbors(const SimParams& p, const DevArrays& A, const float* sample_x,
                              unsigned ns, float4* out, unsigned* counts, hipStream_t s);
+// make_bead_pellet_neighbor_list on samples offset .. offset+ns (rows
+// indexed by the sample's particle index) against a pellet solver's grids
+void launch_sample_bead_pellet(const SimParams& p, const DevArrays& A, const float* sample_x,
+                               unsigned ns, unsigned offset, unsigned max_beads, float4* bead,
+                               unsigned* bead_counts, float4* pel, unsigned* pel_counts,
+                               hipStream_t s);
+// what: 0 boundary kernel with pellets (float4 out), 1 density with
+// pellets, 2 DFSPH factor with pellets
+void launch_pellet_sums(const SimParams& p, int what, const float4* bead,
+                        const unsigned* bead_counts, const float4* pel,
+                        const unsigned* pel_counts, unsigned K, unsigned n, float* out,
+                        hipStream_t s);
 void launch_sample_boundary(const SimParams& p, const BoundaryDev* bodies, unsigned nb,
                             const float* sample_x, unsigned ns, float4* rec, float4* ker,
                             hipStream_t s);

@@ -23,11 +23,15 @@ __device__ inline vec3 ld3f(const float* a, unsigned i) { return mk(a[3 * i], a[
 
 // make_neighbor_list<wrap> on sample points.  As in the reference, a
 // particle whose id equals the sample's index is not taken as its neighbour.
+// With pellets (pcell_start non-null) each cell's pellets follow its fluid
+// particles, as ids max_n + k follow every fluid id in the reference's grid.
 template <bool W>
 __global__ __launch_bounds__(kSampleBlock) void k_sample_neighbors(
     SimParams p, const unsigned* __restrict__ cell_start, const float4* __restrict__ xs,
-    const unsigned* __restrict__ sorted_id, const float* __restrict__ sample_x, unsigned ns,
-    float4* __restrict__ out, unsigned* __restrict__ counts, unsigned* anomaly) {
+    const unsigned* __restrict__ sorted_id, const unsigned* __restrict__ pcell_start,
+    const float4* __restrict__ pxs, const unsigned* __restrict__ psorted_id, unsigned pid_base,
+    const float* __restrict__ sample_x, unsigned ns, float4* __restrict__ out,
+    unsigned* __restrict__ counts, unsigned* anomaly) {
   const unsigned i = blockIdx.x * kSampleBlock + threadIdx.x;
   if (i >= ns) return;
   const unsigned K = p.kmax;
@@ -35,6 +39,13 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_neighbors(
   int ip[3];
   cell_ipos(x, p, ip);
   unsigned cnt = 0;
+  auto visit = [&](unsigned id, vec3 pos) {
+    const vec3 d = W ? wrap_disp(x - pos, p) : x - pos;
+    if (id != i && length_sqr(d) < p.h2) {
+      out[size_t(i) * K + min(cnt, K - 1u)] = make_float4(d.x, d.y, d.z, __uint_as_float(id));
+      ++cnt;
+    }
+  };
   for (int c = 0; c < 27; ++c) {
     int n[3] = {ip[0] + c / 9 - 1, ip[1] + (c / 3) % 3 - 1, ip[2] + c % 3 - 1};
 #pragma unroll
@@ -42,18 +53,109 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_neighbors(
       if (p.wrap_mask & (1 << a)) n[a] = n[a] >= p.res[a] ? n[a] - p.res[a] : (n[a] < 0 ? n[a] + p.res[a] : n[a]);
     if (!within_grid(n, p)) continue;
     const unsigned cell = p.cell_base + cell_linear(n, p);
-    const unsigned je = cell_start[cell + 1];
-    for (unsigned j = cell_start[cell]; j < je; ++j) {
-      const unsigned id = sorted_id[j];
-      const vec3 d = W ? wrap_disp(x - ld3(xs[j]), p) : x - ld3(xs[j]);
-      if (id != i && length_sqr(d) < p.h2) {
-        out[size_t(i) * K + min(cnt, K - 1u)] = make_float4(d.x, d.y, d.z, __uint_as_float(id));
-        ++cnt;
-      }
-    }
+    for (unsigned j = cell_start[cell], je = cell_start[cell + 1]; j < je; ++j)
+      visit(sorted_id[j], ld3(xs[j]));
+    if (pcell_start)
+      for (unsigned j = pcell_start[cell], je = pcell_start[cell + 1]; j < je; ++j)
+        visit(pid_base + psorted_id[j], ld3(pxs[j]));
   }
   counts[i] = min(cnt, K);
   if (cnt > K) anomaly[2] = 1u;
+}
+
+// make_bead_pellet_neighbor_list<wrap> (runner.hpp:1913-1971): samples
+// offset .. offset+ns of sample_x, their neighbours split by id into beads
+// (id < max_beads) and pellets, rows indexed by the sample's particle index.
+template <bool W>
+__global__ __launch_bounds__(kSampleBlock) void k_sample_bead_pellet(
+    SimParams p, const unsigned* __restrict__ cell_start, const float4* __restrict__ xs,
+    const unsigned* __restrict__ sorted_id, const unsigned* __restrict__ pcell_start,
+    const float4* __restrict__ pxs, const unsigned* __restrict__ psorted_id, unsigned pid_base,
+    const float* __restrict__ sample_x, unsigned ns, unsigned offset, unsigned max_beads,
+    float4* __restrict__ bead, unsigned* __restrict__ bead_counts, float4* __restrict__ pel,
+    unsigned* __restrict__ pel_counts, unsigned* anomaly) {
+  const unsigned i0 = blockIdx.x * kSampleBlock + threadIdx.x;
+  if (i0 >= ns) return;
+  const unsigned i = i0 + offset;
+  const unsigned K = p.kmax;
+  const vec3 x = ld3f(sample_x, i);
+  int ip[3];
+  cell_ipos(x, p, ip);
+  unsigned cb = 0, cp = 0;
+  auto visit = [&](unsigned id, vec3 pos) {
+    const vec3 d = W ? wrap_disp(x - pos, p) : x - pos;
+    if (id != i && length_sqr(d) < p.h2) {
+      const float4 e = make_float4(d.x, d.y, d.z, __uint_as_float(id));
+      if (id < max_beads)
+        bead[size_t(i) * K + min(cb++, K - 1u)] = e;
+      else
+        pel[size_t(i) * K + min(cp++, K - 1u)] = e;
+    }
+  };
+  for (int c = 0; c < 27; ++c) {
+    int n[3] = {ip[0] + c / 9 - 1, ip[1] + (c / 3) % 3 - 1, ip[2] + c % 3 - 1};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (p.wrap_mask & (1 << a)) n[a] = n[a] >= p.res[a] ? n[a] - p.res[a] : (n[a] < 0 ? n[a] + p.res[a] : n[a]);
+    if (!within_grid(n, p)) continue;
+    const unsigned cell = p.cell_base + cell_linear(n, p);
+    for (unsigned j = cell_start[cell], je = cell_start[cell + 1]; j < je; ++j)
+      visit(sorted_id[j], ld3(xs[j]));
+    if (pcell_start)
+      for (unsigned j = pcell_start[cell], je = pcell_start[cell + 1]; j < je; ++j)
+        visit(pid_base + psorted_id[j], ld3(pxs[j]));
+  }
+  bead_counts[i] = min(cb, K);
+  pel_counts[i] = min(cp, K);
+  if (cb > K || cp > K) anomaly[2] = 1u;
+}
+
+// Kernel sums over reference-layout lists (xixj, id) of particles with
+// pellets.  compute_particle_boundary_with_pellets (runner.hpp:2027-2050):
+// vol (sum grad W, sum W) over the pellet list;
+// compute_density_with_pellets (runner.hpp:2074-2098): m (W(0) + sum W +
+// sum W) over both lists; compute_dfsph_factor_with_pellets
+// (runner.hpp:3156-3190): the DFSPH factor with the pellet list's entries
+// taken as the boundary's (vol grad W) vectors, as the reference reads them.
+enum { kPelletBoundary = 0, kPelletDensity = 1, kPelletFactor = 2 };
+__global__ __launch_bounds__(kSampleBlock) void k_pellet_sums(
+    SimParams p, int what, const float4* __restrict__ bead, const unsigned* __restrict__ bead_counts,
+    const float4* __restrict__ pel, const unsigned* __restrict__ pel_counts, unsigned K,
+    unsigned n, float* __restrict__ out) {
+  const unsigned i = blockIdx.x * kSampleBlock + threadIdx.x;
+  if (i >= n) return;
+  const unsigned nb = bead ? min(bead_counts[i], K) : 0u;
+  const unsigned np_ = min(pel_counts[i], K);
+  if (what == kPelletBoundary) {
+    vec3 g = mk(0.f, 0.f, 0.f);
+    float w = 0.f;
+    for (unsigned t = 0; t < np_; ++t) {
+      const vec3 d = ld3(pel[size_t(i) * K + t]);
+      w += W_d2(length_sqr(d), p);
+      g += gradW(d, p);
+    }
+    reinterpret_cast<float4*>(out)[i] = make_float4(p.vol * g.x, p.vol * g.y, p.vol * g.z, p.vol * w);
+  } else if (what == kPelletDensity) {
+    float dens = p.w0;
+    for (unsigned t = 0; t < nb; ++t) dens += W_d2(length_sqr(ld3(bead[size_t(i) * K + t])), p);
+    for (unsigned t = 0; t < np_; ++t) dens += W_d2(length_sqr(ld3(pel[size_t(i) * K + t])), p);
+    out[i] = dens * p.mass;
+  } else {
+    float sgk = 0.f;
+    vec3 gpi = mk(0.f, 0.f, 0.f);
+    for (unsigned t = 0; t < nb; ++t) {
+      const vec3 gp = (-p.vol) * gradW(ld3(bead[size_t(i) * K + t]), p);
+      sgk += length_sqr(gp);
+      gpi -= gp;
+    }
+    for (unsigned t = 0; t < np_; ++t) {
+      const vec3 gp = -ld3(pel[size_t(i) * K + t]);
+      sgk += length_sqr(gp);
+      gpi -= gp;
+    }
+    sgk += length_sqr(gpi);
+    out[i] = sgk > p.eps ? -1.0f / sgk : 0.0f;
+  }
 }
 
 // boundary records of sample points for every boundary, written in full
@@ -247,12 +349,40 @@ void launch_map_flags(const GridGeomDev& g, const float* nodes, unsigned char* o
 void launch_sample_neighbors(const SimParams& p, const DevArrays& A, const float* sample_x,
                              unsigned ns, float4* out, unsigned* counts, hipStream_t s) {
   if (ns == 0) return;
+  const unsigned* pc = p.pellet_mode ? A.pcell_start : nullptr;
   if (p.wrap_mask)
     hipLaunchKernelGGL(k_sample_neighbors<true>, dim3(sample_grid(ns)), dim3(kSampleBlock), 0, s,
-                       p, A.cell_start, A.x, A.sorted_id, sample_x, ns, out, counts, A.anomaly);
+                       p, A.cell_start, A.x, A.sorted_id, pc, A.px, A.psorted_id, A.cap,
+                       sample_x, ns, out, counts, A.anomaly);
   else
     hipLaunchKernelGGL(k_sample_neighbors<false>, dim3(sample_grid(ns)), dim3(kSampleBlock), 0, s,
-                       p, A.cell_start, A.x, A.sorted_id, sample_x, ns, out, counts, A.anomaly);
+                       p, A.cell_start, A.x, A.sorted_id, pc, A.px, A.psorted_id, A.cap,
+                       sample_x, ns, out, counts, A.anomaly);
+}
+void launch_sample_bead_pellet(const SimParams& p, const DevArrays& A, const float* sample_x,
+                               unsigned ns, unsigned offset, unsigned max_beads, float4* bead,
+                               unsigned* bead_counts, float4* pel, unsigned* pel_counts,
+                               hipStream_t s) {
+  if (ns == 0) return;
+  const unsigned* pc = p.pellet_mode ? A.pcell_start : nullptr;
+  if (p.wrap_mask)
+    hipLaunchKernelGGL(k_sample_bead_pellet<true>, dim3(sample_grid(ns)), dim3(kSampleBlock), 0,
+                       s, p, A.cell_start, A.x, A.sorted_id, pc, A.px, A.psorted_id, A.cap,
+                       sample_x, ns, offset, max_beads, bead, bead_counts, pel, pel_counts,
+                       A.anomaly);
+  else
+    hipLaunchKernelGGL(k_sample_bead_pellet<false>, dim3(sample_grid(ns)), dim3(kSampleBlock), 0,
+                       s, p, A.cell_start, A.x, A.sorted_id, pc, A.px, A.psorted_id, A.cap,
+                       sample_x, ns, offset, max_beads, bead, bead_counts, pel, pel_counts,
+                       A.anomaly);
+}
+void launch_pellet_sums(const SimParams& p, int what, const float4* bead,
+                        const unsigned* bead_counts, const float4* pel,
+                        const unsigned* pel_counts, unsigned K, unsigned n, float* out,
+                        hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_pellet_sums, dim3(sample_grid(n)), dim3(kSampleBlock), 0, s, p, what, bead,
+                     bead_counts, pel, pel_counts, K, n, out);
 }
 void launch_sample_boundary(const SimParams& p, const BoundaryDev* bodies, unsigned nb,
                             const float* sample_x, unsigned ns, float4* rec, float4* ker,

@@ -209,8 +209,48 @@ PYBIND11_MODULE(_alluvion, m) {
     (void)grid_anomaly;  // the solver's grid_anomaly[2] is set on overflow
     R::launch_make_neighbor_list(sample_x, pid, nbr, counts, num_samples);
   };
+  auto bead_pellet = [](R&, Variable<1, float3> const& sample_x, SolverGrid pid,
+                        py::object pid_length, Variable<2, float4>& bead, Variable<1, U>& nbead,
+                        Variable<2, float4>& pel, Variable<1, U>& npel, py::object grid_anomaly,
+                        U max_num_beads, U num_samples, U offset) {
+    (void)pid_length;
+    (void)grid_anomaly;
+    R::launch_make_bead_pellet_neighbor_list(sample_x, pid, bead, nbead, pel, npel,
+                                             max_num_beads, num_samples, offset);
+  };
   py::class_<Runner<float>>(m, "Runnerfloat")
       .def(py::init<>())
+      .def("launch_make_bead_pellet_neighbor_list", bead_pellet, py::arg("sample_x"),
+           py::arg("pid"), py::arg("pid_length"), py::arg("sample_bead_neighbors"),
+           py::arg("sample_num_bead_neighbors"), py::arg("sample_pellet_neighbors"),
+           py::arg("sample_num_pellet_neighbors"), py::arg("grid_anomaly"),
+           py::arg("max_num_beads"), py::arg("num_samples"), py::arg("offset") = 0)
+      .def("launch_make_bead_pellet_neighbor_list_wrap1", bead_pellet, py::arg("sample_x"),
+           py::arg("pid"), py::arg("pid_length"), py::arg("sample_bead_neighbors"),
+           py::arg("sample_num_bead_neighbors"), py::arg("sample_pellet_neighbors"),
+           py::arg("sample_num_pellet_neighbors"), py::arg("grid_anomaly"),
+           py::arg("max_num_beads"), py::arg("num_samples"), py::arg("offset") = 0)
+      .def("launch_compute_particle_boundary_with_pellets",
+           [](R&, Variable<1, float4>& out, Variable<2, float4> const& pel,
+              Variable<1, U> const& npel, U n) {
+             R::launch_compute_particle_boundary_with_pellets(out, pel, npel, n);
+           },
+           py::arg("particle_boundary_kernel_combined"), py::arg("sample_pellet_neighbors"),
+           py::arg("sample_num_pellet_neighbors"), py::arg("num_particles"))
+      .def("launch_compute_density_with_pellets",
+           [](R&, Variable<2, float4> const& nbr, Variable<1, U> const& cnt,
+              Variable<2, float4> const& pel, Variable<1, U> const& npel, Variable<1, float>& out,
+              U n) { R::launch_compute_density_with_pellets(nbr, cnt, pel, npel, out, n); },
+           py::arg("particle_neighbors"), py::arg("particle_num_neighbors"),
+           py::arg("particle_boundary_neighbors"), py::arg("particle_num_boundary_neighbors"),
+           py::arg("particle_density"), py::arg("num_particles"))
+      .def("launch_compute_dfsph_factor_with_pellets",
+           [](R&, Variable<2, float4> const& nbr, Variable<1, U> const& cnt,
+              Variable<2, float4> const& pel, Variable<1, U> const& npel, Variable<1, float>& out,
+              U n) { R::launch_compute_dfsph_factor_with_pellets(nbr, cnt, pel, npel, out, n); },
+           py::arg("particle_neighbors"), py::arg("particle_num_neighbors"),
+           py::arg("particle_boundary_neighbors"), py::arg("particle_num_boundary_neighbors"),
+           py::arg("particle_dfsph_factor"), py::arg("num_particles"))
       .def("launch_make_neighbor_list", make_list, py::arg("sample_x"), py::arg("pid"),
            py::arg("pid_length"), py::arg("sample_neighbors"), py::arg("sample_num_neighbors"),
            py::arg("grid_anomaly"), py::arg("num_samples"))

@@ -464,6 +464,64 @@ class Runner {
                                       static_cast<float*>(sample_neighbors.ptr_),
                                       static_cast<unsigned*>(sample_num_neighbors.ptr_)));
   }
+  // make_bead_pellet_neighbor_list (runner.hpp:5622-5641)
+  static void launch_make_bead_pellet_neighbor_list(
+      Variable<1, float3> const& sample_x, SolverGrid pid, Variable<2, float4>& bead_neighbors,
+      Variable<1, U>& num_bead_neighbors, Variable<2, float4>& pellet_neighbors,
+      Variable<1, U>& num_pellet_neighbors, U max_num_beads, U num_samples, U offset = 0) {
+    if (!pid.solver) throw std::runtime_error("pid does not belong to a solver");
+    if (sample_x.shape_[0] < offset + num_samples) throw std::out_of_range("sample_x too small");
+    check_samples(bead_neighbors, num_bead_neighbors, offset + num_samples);
+    check_samples(pellet_neighbors, num_pellet_neighbors, offset + num_samples);
+    check(alv_solver_sample_bead_pellet_neighbors(
+        pid.solver, f(sample_x), num_samples, offset, max_num_beads, bead_neighbors.shape_[1],
+        static_cast<float*>(bead_neighbors.ptr_), static_cast<unsigned*>(num_bead_neighbors.ptr_),
+        static_cast<float*>(pellet_neighbors.ptr_),
+        static_cast<unsigned*>(num_pellet_neighbors.ptr_)));
+  }
+  // compute_particle_boundary_with_pellets (runner.hpp:5693-5703)
+  static void launch_compute_particle_boundary_with_pellets(
+      Variable<1, float4>& boundary_kernel_combined, Variable<2, float4> const& pellet_neighbors,
+      Variable<1, U> const& num_pellet_neighbors, U num_particles) {
+    check_samples(pellet_neighbors, num_pellet_neighbors, num_particles);
+    if (boundary_kernel_combined.shape_[0] < num_particles)
+      throw std::out_of_range("particle_boundary_kernel_combined too small");
+    alv_const c = constants();
+    check(alv_particle_boundary_with_pellets(&c, f(pellet_neighbors), u(num_pellet_neighbors),
+                                             pellet_neighbors.shape_[1], num_particles,
+                                             static_cast<float*>(boundary_kernel_combined.ptr_)));
+  }
+  // compute_density_with_pellets (runner.hpp:5681-5692)
+  static void launch_compute_density_with_pellets(
+      Variable<2, float4> const& neighbors, Variable<1, U> const& num_neighbors,
+      Variable<2, float4> const& pellet_neighbors, Variable<1, U> const& num_pellet_neighbors,
+      Variable<1, float>& density, U num_particles) {
+    check_samples(neighbors, num_neighbors, num_particles);
+    check_samples(pellet_neighbors, num_pellet_neighbors, num_particles);
+    if (pellet_neighbors.shape_[1] != neighbors.shape_[1])
+      throw std::invalid_argument("the two lists must have the same row length");
+    if (density.shape_[0] < num_particles) throw std::out_of_range("particle_density too small");
+    alv_const c = constants();
+    check(alv_density_with_pellets(&c, f(neighbors), u(num_neighbors), f(pellet_neighbors),
+                                   u(num_pellet_neighbors), neighbors.shape_[1], num_particles,
+                                   static_cast<float*>(density.ptr_)));
+  }
+  // compute_dfsph_factor_with_pellets (runner.hpp:3156-3190; launched by
+  // SolverPellet, solver_pellet.hpp:77-87)
+  static void launch_compute_dfsph_factor_with_pellets(
+      Variable<2, float4> const& neighbors, Variable<1, U> const& num_neighbors,
+      Variable<2, float4> const& pellet_neighbors, Variable<1, U> const& num_pellet_neighbors,
+      Variable<1, float>& factor, U num_particles) {
+    check_samples(neighbors, num_neighbors, num_particles);
+    check_samples(pellet_neighbors, num_pellet_neighbors, num_particles);
+    if (pellet_neighbors.shape_[1] != neighbors.shape_[1])
+      throw std::invalid_argument("the two lists must have the same row length");
+    if (factor.shape_[0] < num_particles) throw std::out_of_range("particle_dfsph_factor too small");
+    alv_const c = constants();
+    check(alv_dfsph_factor_with_pellets(&c, f(neighbors), u(num_neighbors), f(pellet_neighbors),
+                                        u(num_pellet_neighbors), neighbors.shape_[1],
+                                        num_particles, static_cast<float*>(factor.ptr_)));
+  }
   template <typename Q>
   static void launch_sample_fluid(Variable<1, float3> const& sample_x,
                                   Variable<1, float3> const& particle_x,

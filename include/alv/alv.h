@@ -468,13 +468,45 @@ int alv_solver_export_training_pairs(alv_solver* solver, float* dev_x, float* de
  * Runner::launch_make_neighbor_list, runner.hpp:5604-5620): against the grid
  * and particle positions of the solver's last neighbour search (a step or
  * update_particle_neighbors must have run).  A particle whose id equals the
- * sample index is skipped, as in the reference.  Sets grid_anomaly[2] when a
- * sample has more than K neighbours (the last slot keeps the latest).
+ * sample index is skipped, as in the reference.  A pellet solver's pellets
+ * are in the grid too (ids max_n + k, after the cell's fluid particles).
+ * Sets grid_anomaly[2] when a sample has more than K neighbours (the last
+ * slot keeps the latest).
  * max_neighbors (the row length of sample_neighbors) must equal the solver's
  * max_num_neighbors_per_particle. */
 int alv_solver_sample_neighbors(alv_solver* solver, const float* sample_x,
                                 unsigned num_samples, unsigned max_neighbors,
                                 float* sample_neighbors, unsigned* sample_num_neighbors);
+/* make_bead_pellet_neighbor_list<wrap> (runner.hpp:1913-1971; Runner
+ * launcher runner.hpp:5622-5641) against the solver's grid of the last
+ * neighbour search: samples offset .. offset+num_samples-1 of the device
+ * array sample_x (float3), their neighbours within the kernel radius split
+ * by id into beads (id < max_num_beads) and pellets (ids max_n + k), in the
+ * reference order, skipping the id equal to the sample's index; rows
+ * (float4 xixj + id bits, max_neighbors each) and counts are indexed by the
+ * sample's index (offset included). */
+int alv_solver_sample_bead_pellet_neighbors(alv_solver* solver, const float* sample_x,
+                                            unsigned num_samples, unsigned offset,
+                                            unsigned max_num_beads, unsigned max_neighbors,
+                                            float* bead_neighbors, unsigned* bead_counts,
+                                            float* pellet_neighbors, unsigned* pellet_counts);
+/* Sums over reference-layout lists (device float4 rows of K entries):
+ * compute_particle_boundary_with_pellets (runner.hpp:2027-2050):
+ * boundary_kernel[i] = vol (sum grad W, sum W) over the pellet list;
+ * compute_density_with_pellets (runner.hpp:2074-2098): m (W(0) + sum W +
+ * sum W) over both lists; compute_dfsph_factor_with_pellets
+ * (runner.hpp:3156-3190): the DFSPH factor, the pellet entries' vectors
+ * taken as -(grad p_j) as the reference reads them. */
+int alv_particle_boundary_with_pellets(const alv_const* cn, const float* pellet_neighbors,
+                                       const unsigned* pellet_counts, unsigned K, unsigned n,
+                                       float* boundary_kernel);
+int alv_density_with_pellets(const alv_const* cn, const float* neighbors, const unsigned* counts,
+                             const float* pellet_neighbors, const unsigned* pellet_counts,
+                             unsigned K, unsigned n, float* density);
+int alv_dfsph_factor_with_pellets(const alv_const* cn, const float* neighbors,
+                                  const unsigned* counts, const float* pellet_neighbors,
+                                  const unsigned* pellet_counts, unsigned K, unsigned n,
+                                  float* factor);
 /* Solver::sample_all_boundaries (solver.hpp:186-202): boundary records of the
  * sample points ([B][num_samples] float4 each) with the pile's current
  * kinematics. */

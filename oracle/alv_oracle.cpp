@@ -2057,6 +2057,90 @@ struct Solver {
       j = je;
     }
   }
+  // make_bead_pellet_neighbor_list<wrap> (runner.hpp:1913-1971) on sample
+  // points offset .. offset+ns of sx (rows indexed by the sample's index)
+  void sample_bead_pellet(const V3<T>* sx, unsigned ns, unsigned offset, unsigned max_beads,
+                          unsigned* bid, V3<T>* bx, unsigned* bc, unsigned* pid, V3<T>* px,
+                          unsigned* pc) const {
+    const unsigned K = c.kmax;
+    for (unsigned i0 = 0; i0 < ns; ++i0) {
+      const unsigned i = i0 + offset;
+      V3<T> xi = sx[i];
+      int ip[3];
+      ipos_of(xi, ip);
+      unsigned cb = 0, cp = 0;
+      for (int t = 0; t < 27; ++t) {
+        int nq[3] = {ip[0] + t / 9 - 1, ip[1] + (t / 3) % 3 - 1, ip[2] + t % 3 - 1};
+        for (int ax = 0; ax < 3; ++ax) {
+          if (!(c.wrap_mask & (1 << ax))) continue;
+          if (nq[ax] >= c.res[ax])
+            nq[ax] -= c.res[ax];
+          else if (nq[ax] < 0)
+            nq[ax] += c.res[ax];
+        }
+        if (!within_grid(nq)) continue;
+        unsigned cell = linear(nq);
+        for (unsigned q = cell_start[cell]; q < cell_start[cell + 1]; ++q) {
+          unsigned j = cell_ids[q];
+          V3<T> d = wrap_disp(xi - grid_x[j]);
+          if (j == i || !(length_sqr(d) < c.h2)) continue;
+          if (j < max_beads) {
+            size_t o = size_t(i) * K + std::min(K - 1, cb++);
+            bid[o] = j;
+            bx[o] = d;
+          } else {
+            size_t o = size_t(i) * K + std::min(K - 1, cp++);
+            pid[o] = j;
+            px[o] = d;
+          }
+        }
+      }
+      bc[i] = std::min(K, cb);
+      pc[i] = std::min(K, cp);
+    }
+  }
+  // sums over lists: 0 compute_particle_boundary_with_pellets
+  // (runner.hpp:2027-2050), 1 compute_density_with_pellets
+  // (runner.hpp:2074-2098), 2 compute_dfsph_factor_with_pellets
+  // (runner.hpp:3156-3190, the pellet entries read as grad_wvol)
+  void pellet_sums(int what, unsigned K, unsigned nrows, const V3<T>* bx, const unsigned* bc,
+                   const V3<T>* px, const unsigned* pc, T* out) const {
+    for (unsigned i = 0; i < nrows; ++i) {
+      const unsigned nb = bc ? std::min(bc[i], K) : 0u, np_ = std::min(pc[i], K);
+      if (what == 0) {
+        V3<T> g{0, 0, 0};
+        T w = 0;
+        for (unsigned t = 0; t < np_; ++t) {
+          w += displacement_cubic_kernel(px[size_t(i) * K + t], c);
+          g += displacement_cubic_kernel_grad(px[size_t(i) * K + t], c);
+        }
+        out[4 * i] = c.vol * g.x;
+        out[4 * i + 1] = c.vol * g.y;
+        out[4 * i + 2] = c.vol * g.z;
+        out[4 * i + 3] = c.vol * w;
+      } else if (what == 1) {
+        T dens = c.w0;
+        for (unsigned t = 0; t < nb; ++t) dens += displacement_cubic_kernel(bx[size_t(i) * K + t], c);
+        for (unsigned t = 0; t < np_; ++t) dens += displacement_cubic_kernel(px[size_t(i) * K + t], c);
+        out[i] = dens * c.mass;
+      } else {
+        T sgk = 0;
+        V3<T> gpi{0, 0, 0};
+        for (unsigned t = 0; t < nb; ++t) {
+          V3<T> gp = (-c.vol) * displacement_cubic_kernel_grad(bx[size_t(i) * K + t], c);
+          sgk += length_sqr(gp);
+          gpi -= gp;
+        }
+        for (unsigned t = 0; t < np_; ++t) {
+          V3<T> gp = T(-1) * px[size_t(i) * K + t];
+          sgk += length_sqr(gp);
+          gpi -= gp;
+        }
+        sgk += length_sqr(gpi);
+        out[i] = sgk > c.eps ? -1 / sgk : T(0);
+      }
+    }
+  }
   // SolverI::step with pellets (solver_i.hpp:79-330, VolumeMethod::pellets)
   void step_i_pellets() {
     clear_force();
@@ -2428,7 +2512,7 @@ int orc_sample_neighbors(void* h, const double* sample_x, unsigned ns, unsigned*
                          double* xixj, unsigned* counts) {
   return dispatch(h, [&](auto& s) {
     using T = typename std::remove_reference<decltype(s.dt)>::type;
-    if (s.grid_x.size() != s.n) return -4;
+    if (s.grid_x.size() != (s.pellet_mode ? s.nt() : size_t(s.n))) return -4;  // no grid yet
     auto sx = v3s<T>(sample_x, ns);
     std::vector<V3<T>> d(size_t(ns) * s.c.kmax, V3<T>{0, 0, 0});
     s.sample_neighbors(sx.data(), ns, ids, d.data(), counts);
@@ -2718,6 +2802,48 @@ int orc_extrapolate_pressure(void* h, int last) {
   return dispatch(h, [&](auto& s) {
     if (!s.pellet_mode) return -4;
     s.extrapolate_pressure(last ? s.last_pressure : s.pressure);
+    return 0;
+  });
+}
+
+int orc_sample_bead_pellet(void* h, const double* sample_x, unsigned ns, unsigned offset,
+                           unsigned max_beads, unsigned* bead_ids, double* bead_xixj,
+                           unsigned* bead_counts, unsigned* pel_ids, double* pel_xixj,
+                           unsigned* pel_counts) {
+  return dispatch(h, [&](auto& s) {
+    using T = std::remove_cv_t<std::remove_reference_t<decltype(s.c.h)>>;
+    const size_t rows = size_t(offset) + ns, K = s.c.kmax;
+    std::vector<V3<T>> sx(rows), bx(rows * K, V3<T>{0, 0, 0}), px(rows * K, V3<T>{0, 0, 0});
+    for (size_t i = 0; i < rows; ++i)
+      sx[i] = {T(sample_x[3 * i]), T(sample_x[3 * i + 1]), T(sample_x[3 * i + 2])};
+    s.sample_bead_pellet(sx.data(), ns, offset, max_beads, bead_ids, bx.data(), bead_counts,
+                         pel_ids, px.data(), pel_counts);
+    for (size_t o = 0; o < rows * K; ++o) {
+      bead_xixj[3 * o] = bx[o].x;
+      bead_xixj[3 * o + 1] = bx[o].y;
+      bead_xixj[3 * o + 2] = bx[o].z;
+      pel_xixj[3 * o] = px[o].x;
+      pel_xixj[3 * o + 1] = px[o].y;
+      pel_xixj[3 * o + 2] = px[o].z;
+    }
+    return 0;
+  });
+}
+
+int orc_pellet_sums(void* h, int what, unsigned K, unsigned n, const double* bead_xixj,
+                    const unsigned* bead_counts, const double* pel_xixj,
+                    const unsigned* pel_counts, double* out) {
+  return dispatch(h, [&](auto& s) {
+    using T = std::remove_cv_t<std::remove_reference_t<decltype(s.c.h)>>;
+    std::vector<V3<T>> bx(size_t(n) * K), px(size_t(n) * K);
+    for (size_t o = 0; o < size_t(n) * K; ++o) {
+      if (bead_xixj) bx[o] = {T(bead_xixj[3 * o]), T(bead_xixj[3 * o + 1]), T(bead_xixj[3 * o + 2])};
+      px[o] = {T(pel_xixj[3 * o]), T(pel_xixj[3 * o + 1]), T(pel_xixj[3 * o + 2])};
+    }
+    std::vector<T> res(size_t(n) * (what == 0 ? 4 : 1));
+    s.pellet_sums(what, K, n, bx.data(), bead_xixj ? bead_counts : nullptr, px.data(), pel_counts,
+                  res.data());
+    for (size_t k = 0; k < res.size(); ++k) out[k] = double(res[k]);
     return 0;
   });
 }

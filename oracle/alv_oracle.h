@@ -197,6 +197,22 @@ int orc_find_contacts_pellets(void* h, unsigned* num_found);
  * last_pressure (last = 1) (runner.hpp:2543-2607) */
 int orc_extrapolate_pressure(void* h, int last);
 int orc_get_lists(void* h, int pellet_list, unsigned* ids, void* xij);
+/* make_bead_pellet_neighbor_list on samples offset .. offset+ns-1 of
+ * sample_x [offset+ns][3] against the last grid; outputs [offset+ns][K]
+ * (xixj packed xyz), rows indexed by the sample's index. */
+int orc_sample_bead_pellet(void* h, const double* sample_x, unsigned ns, unsigned offset,
+                           unsigned max_beads, unsigned* bead_ids, double* bead_xixj,
+                           unsigned* bead_counts, unsigned* pel_ids, double* pel_xixj,
+                           unsigned* pel_counts);
+/* List sums of n rows of K entries: what 0 = boundary kernel with pellets
+ * (out [n][4]), 1 = density with pellets, 2 = DFSPH factor with pellets
+ * (out [n]).  bead_xixj may be null for what 0.  The DFSPH factor with
+ * pellets (runner.hpp:3156-3190) takes the pellet entries' vectors as
+ * grad_wvol, as the reference does: parity unpinned beyond its fluid part
+ * (no closed form or fixture covers that reading). */
+int orc_pellet_sums(void* h, int what, unsigned K, unsigned n, const double* bead_xixj,
+                    const unsigned* bead_counts, const double* pel_xixj,
+                    const unsigned* pel_counts, double* out);
 
 /* Stand-alone helpers used by the tests. */
 int orc_peirce_constants(double* cosphi, double* sinphi, double* b,

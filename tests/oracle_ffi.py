@@ -134,6 +134,10 @@ def lib():
         L.orc_find_contacts_pellets.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
         L.orc_extrapolate_pressure.argtypes = [vp, ctypes.c_int]
         L.orc_get_lists.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.orc_sample_bead_pellet.argtypes = [vp, dp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                             up, dp, up, up, dp, up]
+        L.orc_pellet_sums.argtypes = [vp, ctypes.c_int, ctypes.c_uint, ctypes.c_uint, dp, up, dp,
+                                      up, dp]
         L.orc_get_grid.argtypes = [vp, vp, vp, vp]
         L.orc_step.argtypes = [vp]
         L.orc_get_report.argtypes = [vp, ctypes.POINTER(OrcReport)]
@@ -160,6 +164,10 @@ def lib():
 
 def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _uptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))
 
 
 _KIND = {"box": 0, "box_shell": 1, "sphere": 2}
@@ -462,6 +470,36 @@ class Oracle:
         if rc != 0:
             raise ValueError("orc_get_lists: %d" % rc)
         return ids, xij, self.get("num_pellet_neighbors" if pellet_list else "num_neighbors")
+
+    def sample_bead_pellet(self, sample_x, offset=0, num_samples=None, max_beads=None):
+        """(bead ids, xixj, counts, pellet ids, xixj, counts), rows
+        [offset + num_samples] indexed by the sample's index."""
+        sx = np.ascontiguousarray(sample_x, np.float64).reshape(-1, 3)
+        ns = sx.shape[0] - offset if num_samples is None else num_samples
+        rows, K = offset + ns, self.kmax
+        out = [np.zeros((rows, K), np.uint32), np.zeros((rows, K, 3)), np.zeros(rows, np.uint32),
+               np.zeros((rows, K), np.uint32), np.zeros((rows, K, 3)), np.zeros(rows, np.uint32)]
+        mb = self.max_n if max_beads is None else max_beads
+        rc = self.L.orc_sample_bead_pellet(self.h, _dptr(sx), ns, offset, mb,
+                                           *[_uptr(a) if a.dtype == np.uint32 else _dptr(a)
+                                             for a in out])
+        if rc != 0:
+            raise ValueError("orc_sample_bead_pellet: %d" % rc)
+        return out
+
+    def pellet_sums(self, what, pel_xixj, pel_counts, bead_xixj=None, bead_counts=None):
+        n, K = pel_counts.shape[0], self.kmax
+        pel = np.ascontiguousarray(pel_xixj, np.float64).reshape(n, K, 3)
+        pc = np.ascontiguousarray(pel_counts, np.uint32)
+        bx = None if bead_xixj is None else np.ascontiguousarray(bead_xixj, np.float64).reshape(n, K, 3)
+        bc = None if bead_counts is None else np.ascontiguousarray(bead_counts, np.uint32)
+        out = np.zeros((n, 4) if what == 0 else n)
+        rc = self.L.orc_pellet_sums(self.h, what, K, n, None if bx is None else _dptr(bx),
+                                    None if bc is None else _uptr(bc), _dptr(pel), _uptr(pc),
+                                    _dptr(out))
+        if rc != 0:
+            raise ValueError("orc_pellet_sums: %d" % rc)
+        return out
 
     def transform_pellets(self):
         if self.L.orc_transform_pellets(self.h) != 0:

@@ -113,3 +113,59 @@ def test_pellet_contacts_match_oracle():
     rb = orc.rigid(1)
     np.testing.assert_allclose(v, rb["v"], rtol=1e-5, atol=1e-6)
     np.testing.assert_allclose(w, rb["omega"], rtol=1e-5, atol=1e-5)
+
+
+def test_runner_pellet_lists_and_sums_match_oracle():
+    from alluvion_amd import scenes
+    sc = small_scene()
+    dp, pile, solver, _, _ = scenes.build_solver(sc, kind="i")
+    solver.transform_all_pellets()
+    solver.update_particle_neighbors()
+    orc = of.Oracle(sc, precision=4, kind="i")
+    orc.transform_pellets()
+    orc.update_particle_neighbors()
+    runner = dp.Runner()
+    mx, K = solver.max_num_particles, sc.max_num_neighbors
+    off, ns = 100, 60
+    rows = off + ns
+    bead, pel = dp.create((rows, K), 4), dp.create((rows, K), 4)
+    nb, npl = dp.create((rows,), 1, np.uint32), dp.create((rows,), 1, np.uint32)
+    for c in (nb, npl):
+        dp.coat(c).set(np.zeros(rows, np.uint32))
+    runner.launch_make_bead_pellet_neighbor_list(solver.particle_x, solver.pid, solver.pid_length,
+                                                 bead, nb, pel, npl, solver.grid_anomaly, mx,
+                                                 ns, off)
+    bid, bxx, bc, pid, pxx, pc = orc.sample_bead_pellet(orc.get("x")[:rows], offset=off)
+    g_b = dp.coat(bead).get().reshape(rows, K, 4)
+    g_p = dp.coat(pel).get().reshape(rows, K, 4)
+    np.testing.assert_array_equal(dp.coat(nb).get(), bc)
+    np.testing.assert_array_equal(dp.coat(npl).get(), pc)
+    assert pc[off:].max() > 0
+    for i in range(off, rows):
+        np.testing.assert_array_equal(g_b[i, :bc[i], 3].view(np.uint32), bid[i, :bc[i]])
+        np.testing.assert_array_equal(g_p[i, :pc[i], 3].view(np.uint32), pid[i, :pc[i]])
+        np.testing.assert_allclose(g_b[i, :bc[i], :3], bxx[i, :bc[i]], atol=1e-9)
+        np.testing.assert_allclose(g_p[i, :pc[i], :3], pxx[i, :pc[i]], atol=1e-9)
+    # sums over the GPU lists against the oracle's sums over its lists
+    dens, fac = dp.create((rows,), 1), dp.create((rows,), 1)
+    bk = dp.create((rows,), 4)
+    runner.launch_compute_density_with_pellets(bead, nb, pel, npl, dens, rows)
+    runner.launch_compute_dfsph_factor_with_pellets(bead, nb, pel, npl, fac, rows)
+    runner.launch_compute_particle_boundary_with_pellets(bk, pel, npl, rows)
+    s = slice(off, rows)
+    assert rel(dp.coat(dens).get()[s], orc.pellet_sums(1, pxx[s], pc[s], bxx[s], bc[s])) < 1e-6
+    assert rel(dp.coat(fac).get()[s], orc.pellet_sums(2, pxx[s], pc[s], bxx[s], bc[s])) < 1e-5
+    assert rel(dp.coat(bk).get().reshape(rows, 4)[s], orc.pellet_sums(0, pxx[s], pc[s])) < 1e-5
+    # sample lists of a pellet solver hold the pellets too
+    sx = dp.create_coated((ns,), 3)
+    pts = orc.get("x")[off:rows].astype(np.float32)
+    sx.set(pts)
+    nbr, cnt = dp.create((ns, K), 4), dp.create((ns,), 1, np.uint32)
+    runner.launch_make_neighbor_list(sx, solver.pid, solver.pid_length, nbr, cnt,
+                                     solver.grid_anomaly, ns)
+    ids_o, xixj_o, cnt_o = orc.sample_neighbors(pts)
+    np.testing.assert_array_equal(dp.coat(cnt).get(), cnt_o)
+    lists = dp.coat(nbr).get().reshape(ns, K, 4)
+    assert (ids_o >= mx).any()
+    for k in range(ns):
+        np.testing.assert_array_equal(lists[k, :cnt_o[k], 3].view(np.uint32), ids_o[k, :cnt_o[k]])

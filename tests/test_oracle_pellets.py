@@ -257,3 +257,75 @@ def test_pellet_contacts_are_the_pellets_inside_other_bodies(runs):
     floor = rows[:, 1] == 0
     np.testing.assert_allclose(rows[floor, 8:11], np.tile([0, 1, 0], (floor.sum(), 1)), atol=1e-9)
     assert (~floor).sum() > 0
+
+
+def test_sample_bead_pellet_lists_and_list_sums():
+    # samples: particles of the scene (offset rows, self skipped) and points
+    sc = small_scene()
+    orc = of.Oracle(sc, precision=8, kind="i")
+    orc.transform_pellets()
+    orc.update_particle_neighbors()
+    x = orc.get("x")
+    n, mx = sc.n, orc.max_n
+    wp, body = pellet_world(sc, orc)
+    off = 100
+    sx = x[:off + 60]
+    bid, bxx, bc, pid, pxx, pc = orc.sample_bead_pellet(sx, offset=off)
+    allx = np.concatenate([x[:n], wp])
+    allid = np.concatenate([np.arange(n), mx + np.arange(body.size)])
+    assert np.all(bc[:off] == 0) and np.all(pc[:off] == 0)
+    for i in range(off, off + 60):
+        d = sx[i] - allx
+        near = allid[((d ** 2).sum(1) < H * H) & (allid != i)]
+        assert set(bid[i, :bc[i]].tolist()) == set(near[near < mx].tolist()), i
+        assert set(pid[i, :pc[i]].tolist()) == set(near[near >= mx].tolist()), i
+        np.testing.assert_allclose(pxx[i, :pc[i]], sx[i] - allx[pid[i, :pc[i]] - mx + n],
+                                   atol=1e-15)
+    # the particles' own lists equal the samples' at the same points
+    ids_b, _, cnt_b = orc.lists(False)
+    np.testing.assert_array_equal(bc[off:off + 60], cnt_b[off:off + 60])
+
+    # list sums: density with pellets equals the solver's; the boundary
+    # kernel with pellets plus the fluid terms is the full kernel sum
+    rows = slice(off, off + 60)
+    dens = orc.pellet_sums(1, pxx[rows], pc[rows], bxx[rows], bc[rows])
+    np.testing.assert_allclose(dens, orc.get("density")[rows], rtol=1e-12)
+    bk = orc.pellet_sums(0, pxx[rows], pc[rows])
+    vol = sc.particle_mass / sc.density0
+    fluid_w = np.array([cubic_w(np.linalg.norm(bxx[i, :bc[i]], axis=1) / H, H).sum()
+                        for i in range(off, off + 60)])
+    np.testing.assert_allclose(vol * (cubic_w(0.0, H) + fluid_w) + bk[:, 3],
+                               dens / sc.particle_mass * vol, rtol=1e-12)
+    pel_w = np.array([cubic_w(np.linalg.norm(pxx[i, :pc[i]], axis=1) / H, H).sum()
+                      for i in range(off, off + 60)])
+    np.testing.assert_allclose(bk[:, 3], vol * pel_w, rtol=1e-12, atol=1e-12)
+    assert bk[:, 3].max() > 0
+
+
+def test_boundary_kernel_with_pellets_balances_on_a_lattice():
+    # perfect periodic lattice, every 7th site a pellet: the fluid and pellet
+    # gradient sums cancel and the kernel sums add to the lattice sum
+    S = lattice_sum(2 * R, H)
+    sc, x, v = periodic_lattice(m=20, mass=1000.0 / S)
+    x = x.astype(np.float32).astype(np.float64)
+    pel = (np.arange(x.shape[0]) % 7 == 3)
+    sc.x, sc.v = x[~pel].astype(np.float32), v[~pel].astype(np.float32)
+    sc.volume_method, sc.max_num_pellets = "pellets", int(pel.sum())
+    sc.boundaries = [scenes.BoundarySpec("box", (0.05, 0.05, 0.05, 0.0), 1.0, (4, 4, 4),
+                                         collision_vertices=x[pel])]
+    orc = of.Oracle(sc, precision=8, kind="i")
+    orc.set("x", np.concatenate([x[~pel], x[pel]]))
+    orc.transform_pellets()
+    orc.update_particle_neighbors()
+    n = sc.n
+    _, bxx, bc = orc.lists(False)
+    _, pxx, pc = orc.lists(True)
+    bk = orc.pellet_sums(0, pxx[:n], pc[:n])
+    vol = sc.particle_mass / sc.density0
+    gw_f = np.zeros((n, 3))
+    for i in range(n):
+        g, _ = of.cubic_kernel(bxx[i, :bc[i]], H)
+        gw_f[i] = g.sum(0)
+    # FP32 sites break the lattice's symmetry at ~1e-6 of a single gradient
+    np.testing.assert_allclose(bk[:, :3] + vol * gw_f, 0, atol=2e-5 * np.abs(bk[:, :3]).max())
+    assert np.abs(bk[:, :3]).max() > 0
