@@ -462,16 +462,29 @@ __device__ inline bool stage_x(const DevArrays& A, unsigned blk, TileX& t) {
 // reference order (dx outer, dz inner); inside a cell in ascending particle
 // id, which is ascending sorted index.
 // ---------------------------------------------------------------------------
+// Candidates staged per wave: the 64 particles of a wave mostly lie in one
+// (x, y) column of cells, so for each (dx, dy) row their candidate spans lie
+// in one range of about 64 + 2 cells' particles, which the wave copies into
+// LDS with coalesced loads; the lanes then read their own spans from LDS
+// instead of gathering from L2.  A row whose range exceeds kStage (a wave
+// across two columns) reads global memory.
+constexpr unsigned kStage = 128;
+
 template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int with_factor) {
+  // LDS staging for the gather instances without the periodic wrap (their
+  // rows are single spans); the others keep the lane-private scan
+  constexpr bool kWaveStage = !W && !TL;
+  __shared__ float4 stage_[kWaveStage ? 4 : 1][kWaveStage ? kStage : 1];
   ALV_SCENE(T, A, s);
   // candidate positions from the block's LDS tile when it has one
   TileX tile_;
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_x(A, s / kBlock, tile_);
-  if (s >= p.first + p.n) return;
+  const bool live = s < p.first + p.n;
+  if (!kWaveStage && !live) return;
   const unsigned K = p.kmax;
-  const vec3 xi = ld3(A.x[s]);
+  const vec3 xi = live ? ld3(A.x[s]) : mk(0.f, 0.f, 0.f);
   int ip[3];
   cell_ipos(xi, p, ip);
   unsigned cnt = 0;
@@ -495,6 +508,37 @@ __global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int 
     }
   }
   const float reach2 = p.h2 * 1.0001f;
+  // candidates [j0, je) in batches: all positions of a batch are loaded
+  // before any is tested (one memory latency per batch); the tests are
+  // branch-free and give a bit mask of hits, which are then appended in
+  // ascending j
+  auto scan_span = [&](unsigned jb, unsigned je, auto&& X) {
+    for (unsigned j0 = jb; j0 < je; j0 += kCB) {
+      float4 xc[kCB];
+#pragma unroll
+      for (int u = 0; u < kCB; ++u) xc[u] = X(min(j0 + u, je - 1u));
+      unsigned hits = 0u;
+#pragma unroll
+      for (int u = 0; u < kCB; ++u) {
+        const vec3 d = wrapd<W>(xi - ld3(xc[u]), p);
+        const bool h = (j0 + u < je) & (j0 + u != s) & (length_sqr(d) < p.h2);
+        hits |= unsigned(h) << u;
+      }
+      while (hits) {
+        const unsigned j = j0 + unsigned(__builtin_ctz(hits));
+        hits &= hits - 1u;
+        // slot min(cnt, K-1): the last slot keeps the latest neighbour
+        const unsigned slot = min(cnt, K - 1);
+        const unsigned c4 = slot & 3u;
+        grp.x = c4 == 0 ? j : grp.x;
+        grp.y = c4 == 1 ? j : grp.y;
+        grp.z = c4 == 2 ? j : grp.z;
+        grp.w = c4 == 3 ? j : grp.w;
+        if (c4 == 3u && cnt == slot) A.nbr[nbr_group(A, slot >> 2, s)] = grp;
+        ++cnt;
+      }
+    }
+  };
   // The 27 cells in the reference order (dx outer, dz inner): for each
   // (dx, dy) row the up-to-3 cells along z are contiguous in the sorted
   // order unless a periodic wrap splits them, so each row is 1-3 spans.
@@ -503,65 +547,67 @@ __global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int 
     const float ex = ddx < 0 ? lo_d[0] : (ddx > 0 ? hi_d[0] : 0.0f);
     const float ey = ddy < 0 ? lo_d[1] : (ddy > 0 ? hi_d[1] : 0.0f);
     const float base = ex * ex + ey * ey;
-    if (!(base < reach2)) continue;
     const int dz_lo = base + lo_d[2] * lo_d[2] < reach2 ? -1 : 0;
     const int dz_hi = base + hi_d[2] * hi_d[2] < reach2 ? 1 : 0;
     int nx = ip[0] + ddx, ny = ip[1] + ddy;
     if (p.wrap_mask & 1) nx = nx >= p.res[0] ? nx - p.res[0] : (nx < 0 ? nx + p.res[0] : nx);
     if (p.wrap_mask & 2) ny = ny >= p.res[1] ? ny - p.res[1] : (ny < 0 ? ny + p.res[1] : ny);
-    if (nx < 0 || nx >= p.res[0] || ny < 0 || ny >= p.res[1]) continue;
+    const bool row_ok = live && base < reach2 && nx >= 0 && nx < p.res[0] && ny >= 0 &&
+                        ny < p.res[1];
     const unsigned row = (unsigned(nx) * unsigned(p.res[1]) + unsigned(ny)) * unsigned(p.res[2]);
-    unsigned span_b[3], span_e[3], nspan = 0;
-    int prev = -2;
-    for (int dz = dz_lo; dz <= dz_hi; ++dz) {
-      int nz = ip[2] + dz;
-      if (p.wrap_mask & 4) nz = nz >= p.res[2] ? nz - p.res[2] : (nz < 0 ? nz + p.res[2] : nz);
-      if (nz < 0 || nz >= p.res[2]) continue;
-      const unsigned c = p.cell_base + row + unsigned(nz);
-      if (nz == prev + 1) {
-        span_e[nspan - 1] = A.cell_start[c + 1];
-      } else {
-        span_b[nspan] = A.cell_start[c];
-        span_e[nspan] = A.cell_start[c + 1];
-        ++nspan;
+    if constexpr (kWaveStage) {
+      // one span: cells nz_lo .. nz_hi of the row, clipped to the grid
+      const int z0 = max(ip[2] + dz_lo, 0), z1 = min(ip[2] + dz_hi, p.res[2] - 1);
+      unsigned jb = 0u, je = 0u;
+      if (row_ok && z0 <= z1) {
+        jb = A.cell_start[p.cell_base + row + unsigned(z0)];
+        je = A.cell_start[p.cell_base + row + unsigned(z1) + 1u];
       }
-      prev = nz;
-    }
-    // candidates in batches: all positions of a batch are loaded before
-    // any is tested (one memory latency per batch); the tests are branch-free
-    // and give a bit mask of hits, which are then appended in ascending j
-    for (unsigned q = 0; q < nspan; ++q) {
-      const unsigned jb = span_b[q], je = span_e[q];
-      for (unsigned j0 = jb; j0 < je; j0 += kCB) {
-        float4 xc[kCB];
+      unsigned lo = jb < je ? jb : ~0u, hi = jb < je ? je : 0u;
 #pragma unroll
-        for (int u = 0; u < kCB; ++u) {
-          const unsigned jc = min(j0 + u, je - 1u);
-          xc[u] = tiled_ ? tile_.X(jc) : A.x[jc];
+      for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, unsigned(__shfl_xor(int(lo), o, 64)));
+        hi = max(hi, unsigned(__shfl_xor(int(hi), o, 64)));
+      }
+      const bool fits = hi > lo && hi - lo <= kStage;
+      float4* st = stage_[threadIdx.x >> 6];
+      __syncthreads();  // the previous row's reads are done
+      if (fits)
+        for (unsigned k = threadIdx.x & 63u; k < hi - lo; k += 64u) st[k] = A.x[lo + k];
+      __syncthreads();
+      if (jb < je) {
+        if (fits)
+          scan_span(jb, je, [&](unsigned j) { return st[j - lo]; });
+        else
+          scan_span(jb, je, [&](unsigned j) { return A.x[j]; });
+      }
+    } else {
+      if (!row_ok) continue;
+      unsigned span_b[3], span_e[3], nspan = 0;
+      int prev = -2;
+      for (int dz = dz_lo; dz <= dz_hi; ++dz) {
+        int nz = ip[2] + dz;
+        if (p.wrap_mask & 4) nz = nz >= p.res[2] ? nz - p.res[2] : (nz < 0 ? nz + p.res[2] : nz);
+        if (nz < 0 || nz >= p.res[2]) continue;
+        const unsigned c = p.cell_base + row + unsigned(nz);
+        if (nz == prev + 1) {
+          span_e[nspan - 1] = A.cell_start[c + 1];
+        } else {
+          span_b[nspan] = A.cell_start[c];
+          span_e[nspan] = A.cell_start[c + 1];
+          ++nspan;
         }
-        unsigned hits = 0u;
-#pragma unroll
-        for (int u = 0; u < kCB; ++u) {
-          const vec3 d = wrapd<W>(xi - ld3(xc[u]), p);
-          const bool h = (j0 + u < je) & (j0 + u != s) & (length_sqr(d) < p.h2);
-          hits |= unsigned(h) << u;
-        }
-        while (hits) {
-          const unsigned j = j0 + unsigned(__builtin_ctz(hits));
-          hits &= hits - 1u;
-          // slot min(cnt, K-1): the last slot keeps the latest neighbour
-          const unsigned slot = min(cnt, K - 1);
-          const unsigned c4 = slot & 3u;
-          grp.x = c4 == 0 ? j : grp.x;
-          grp.y = c4 == 1 ? j : grp.y;
-          grp.z = c4 == 2 ? j : grp.z;
-          grp.w = c4 == 3 ? j : grp.w;
-          if (c4 == 3u && cnt == slot) A.nbr[nbr_group(A, slot >> 2, s)] = grp;
-          ++cnt;
-        }
+        prev = nz;
+      }
+      for (unsigned q = 0; q < nspan; ++q) {
+        if (tiled_)
+          scan_span(span_b[q], span_e[q], [&](unsigned j) { return tile_.X(j); });
+        else
+          scan_span(span_b[q], span_e[q], [&](unsigned j) { return A.x[j]; });
       }
     }
   }
+  if (!live) return;
   // the group holding the last slot is incomplete, or was rewritten by
   // overflowing neighbours after it was stored
   if (cnt > 0 && ((min(cnt, K) & 3u) != 0u || cnt > K))
