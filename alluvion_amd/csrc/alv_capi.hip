@@ -2413,6 +2413,27 @@ int pellet_sums(const alv_const* cn, int what, const float* nbr, const unsigned*
 }
 }  // namespace
 
+int alv_sample_with_pellets(const alv_const* cn, int what, const float* neighbors,
+                            const unsigned* counts, const float* pellet_neighbors,
+                            const unsigned* pellet_counts, unsigned K, unsigned num_samples,
+                            const float* particle_density, const float* particle_v,
+                            const float* sample_v, float* out) {
+  if (!cn || what < 0 || what > 2) return fail(ALV_ERR_ARG, "bad argument");
+  if (num_samples && (!neighbors || !counts || !pellet_neighbors || !pellet_counts || !out ||
+                      (what > 0 && (!particle_density || !particle_v)) ||
+                      (what == 2 && !sample_v)))
+    return fail(ALV_ERR_ARG, "null argument");
+  if (K == 0) return fail(ALV_ERR_ARG, "max_neighbors must be positive");
+  alv_consti cni;
+  memset(&cni, 0, sizeof(cni));
+  SimParams p = make_params(*cn, cni, 0, 0);
+  launch_sample_with_pellets(p, what, reinterpret_cast<const float4*>(neighbors), counts,
+                             reinterpret_cast<const float4*>(pellet_neighbors), pellet_counts, K,
+                             num_samples, particle_density, particle_v, sample_v, out, nullptr);
+  ALV_HIP(hipGetLastError());
+  return ALV_OK;
+}
+
 int alv_particle_boundary_with_pellets(const alv_const* cn, const float* pellet_neighbors,
                                        const unsigned* pellet_counts, unsigned K, unsigned n,
                                        float* boundary_kernel) {

@@ -226,6 +226,12 @@ void launch_sample_bead_pellet(const SimParams& p, const DevArrays& A, const flo
                                unsigned ns, unsigned offset, unsigned max_beads, float4* bead,
                                unsigned* bead_counts, float4* pel, unsigned* pel_counts,
                                hipStream_t s);
+// sampled density (0), velocity (1) or vorticity (2) with pellet lists
+void launch_sample_with_pellets(const SimParams& p, int what, const float4* nbr,
+                                const unsigned* counts, const float4* pel,
+                                const unsigned* pcounts, unsigned K, unsigned ns,
+                                const float* density, const float* v, const float* sample_v,
+                                float* out, hipStream_t s);
 // what: 0 boundary kernel with pellets (float4 out), 1 density with
 // pellets, 2 DFSPH factor with pellets
 void launch_pellet_sums(const SimParams& p, int what, const float4* bead,

@@ -110,6 +110,52 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_bead_pellet(
   if (cb > K || cp > K) anomaly[2] = 1u;
 }
 
+// Sampling with pellet lists (runner.hpp:2122-2147 sample_position_density_
+// with_pellets, 4242-4277 compute_sample_velocity_with_pellets, 4325-4359
+// compute_sample_vorticity_with_pellets): fluid entries weighted m / rho_j,
+// pellet entries by the particle volume; velocities of both from the user
+// order rows of v (pellets at rows max_n + k).  what 0: density, 1:
+// velocity, 2: vorticity (sample_v is the samples' velocity).
+__global__ __launch_bounds__(kSampleBlock) void k_sample_with_pellets(
+    SimParams p, int what, const float4* __restrict__ nbr, const unsigned* __restrict__ counts,
+    const float4* __restrict__ pel, const unsigned* __restrict__ pcounts, unsigned K,
+    unsigned ns, const float* __restrict__ density, const float* __restrict__ v,
+    const float* __restrict__ sample_v, float* __restrict__ out) {
+  const unsigned i = blockIdx.x * kSampleBlock + threadIdx.x;
+  if (i >= ns) return;
+  const unsigned nb = min(counts[i], K), np_ = min(pcounts[i], K);
+  if (what == 0) {
+    float dens = 0.f;
+    for (unsigned t = 0; t < nb; ++t) dens += W_d2(length_sqr(ld3(nbr[size_t(i) * K + t])), p);
+    for (unsigned t = 0; t < np_; ++t) dens += W_d2(length_sqr(ld3(pel[size_t(i) * K + t])), p);
+    out[i] = dens * p.mass;
+    return;
+  }
+  const vec3 vi = what == 2 ? ld3f(sample_v, i) : mk(0.f, 0.f, 0.f);
+  vec3 r = mk(0.f, 0.f, 0.f);
+  for (unsigned t = 0; t < nb; ++t) {
+    const float4 e = nbr[size_t(i) * K + t];
+    const unsigned j = __float_as_uint(e.w);
+    const vec3 d = ld3(e);
+    if (what == 1)
+      r += (p.mass / density[j] * W_d2(length_sqr(d), p)) * ld3f(v, j);
+    else
+      r += (p.mass / density[j]) * cross(ld3f(v, j) - vi, gradW(d, p));
+  }
+  for (unsigned t = 0; t < np_; ++t) {
+    const float4 e = pel[size_t(i) * K + t];
+    const unsigned j = __float_as_uint(e.w);
+    const vec3 d = ld3(e);
+    if (what == 1)
+      r += (p.vol * W_d2(length_sqr(d), p)) * ld3f(v, j);
+    else
+      r += p.vol * cross(ld3f(v, j) - vi, gradW(d, p));
+  }
+  out[3 * i] = r.x;
+  out[3 * i + 1] = r.y;
+  out[3 * i + 2] = r.z;
+}
+
 // Kernel sums over reference-layout lists (xixj, id) of particles with
 // pellets.  compute_particle_boundary_with_pellets (runner.hpp:2027-2050):
 // vol (sum grad W, sum W) over the pellet list;
@@ -375,6 +421,15 @@ void launch_sample_bead_pellet(const SimParams& p, const DevArrays& A, const flo
                        s, p, A.cell_start, A.x, A.sorted_id, pc, A.px, A.psorted_id, A.cap,
                        sample_x, ns, offset, max_beads, bead, bead_counts, pel, pel_counts,
                        A.anomaly);
+}
+void launch_sample_with_pellets(const SimParams& p, int what, const float4* nbr,
+                                const unsigned* counts, const float4* pel,
+                                const unsigned* pcounts, unsigned K, unsigned ns,
+                                const float* density, const float* v, const float* sample_v,
+                                float* out, hipStream_t s) {
+  if (ns == 0) return;
+  hipLaunchKernelGGL(k_sample_with_pellets, dim3(sample_grid(ns)), dim3(kSampleBlock), 0, s, p,
+                     what, nbr, counts, pel, pcounts, K, ns, density, v, sample_v, out);
 }
 void launch_pellet_sums(const SimParams& p, int what, const float4* bead,
                         const unsigned* bead_counts, const float4* pel,

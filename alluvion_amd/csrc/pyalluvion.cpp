@@ -230,6 +230,39 @@ PYBIND11_MODULE(_alluvion, m) {
            py::arg("sample_num_bead_neighbors"), py::arg("sample_pellet_neighbors"),
            py::arg("sample_num_pellet_neighbors"), py::arg("grid_anomaly"),
            py::arg("max_num_beads"), py::arg("num_samples"), py::arg("offset") = 0)
+      .def("launch_sample_density_with_pellets",
+           [](R&, Variable<1, float3> const& sx, Variable<2, float4> const& nbr,
+              Variable<1, U> const& cnt, Variable<2, float4> const& pel, Variable<1, U> const& pcnt,
+              Variable<1, float>& out, U ns) {
+             R::launch_sample_density_with_pellets(sx, nbr, cnt, pel, pcnt, out, ns);
+           },
+           py::arg("sample_x"), py::arg("sample_neighbors"), py::arg("sample_num_neighbors"),
+           py::arg("sample_pellet_neighbors"), py::arg("sample_num_pellet_neighbors"),
+           py::arg("sample_density"), py::arg("num_samples"))
+      .def("launch_sample_velocity_with_pellets",
+           [](R&, Variable<1, float3> const& sx, Variable<1, float3> const& px,
+              Variable<1, float> const& pd, Variable<1, float3> const& pv,
+              Variable<2, float4> const& nbr, Variable<1, U> const& cnt, Variable<1, float3>& sv,
+              Variable<2, float4> const& pel, Variable<1, U> const& pcnt, U ns) {
+             R::launch_sample_velocity_with_pellets(sx, px, pd, pv, nbr, cnt, sv, pel, pcnt, ns);
+           },
+           py::arg("sample_x"), py::arg("particle_x"), py::arg("particle_density"),
+           py::arg("particle_v"), py::arg("sample_neighbors"), py::arg("sample_num_neighbors"),
+           py::arg("sample_v"), py::arg("sample_pellet_neighbors"),
+           py::arg("sample_num_pellet_neighbors"), py::arg("num_samples"))
+      .def("launch_sample_vorticity_with_pellets",
+           [](R&, Variable<1, float3> const& sx, Variable<1, float3> const& px,
+              Variable<1, float> const& pd, Variable<1, float3> const& pv,
+              Variable<2, float4> const& nbr, Variable<1, U> const& cnt,
+              Variable<1, float3> const& sv, Variable<1, float3>& vort,
+              Variable<2, float4> const& pel, Variable<1, U> const& pcnt, U ns) {
+             R::launch_sample_vorticity_with_pellets(sx, px, pd, pv, nbr, cnt, sv, vort, pel, pcnt,
+                                                     ns);
+           },
+           py::arg("sample_x"), py::arg("particle_x"), py::arg("particle_density"),
+           py::arg("particle_v"), py::arg("sample_neighbors"), py::arg("sample_num_neighbors"),
+           py::arg("sample_v"), py::arg("sample_vorticity"), py::arg("sample_pellet_neighbors"),
+           py::arg("sample_num_pellet_neighbors"), py::arg("num_samples"))
       .def("launch_compute_particle_boundary_with_pellets",
            [](R&, Variable<1, float4>& out, Variable<2, float4> const& pel,
               Variable<1, U> const& npel, U n) {

@@ -479,6 +479,57 @@ class Runner {
         static_cast<float*>(pellet_neighbors.ptr_),
         static_cast<unsigned*>(num_pellet_neighbors.ptr_)));
   }
+  // sample_position_density_with_pellets, compute_sample_velocity_with_
+  // pellets, compute_sample_vorticity_with_pellets (runner.hpp:5747-5846)
+  static void launch_sample_density_with_pellets(
+      Variable<1, float3> const& sample_x, Variable<2, float4> const& nbr,
+      Variable<1, U> const& cnt, Variable<2, float4> const& pel, Variable<1, U> const& pcnt,
+      Variable<1, float>& sample_density, U num_samples) {
+    (void)sample_x;
+    pellet_samples(nbr, cnt, pel, pcnt, num_samples);
+    if (sample_density.shape_[0] < num_samples) throw std::out_of_range("sample_density too small");
+    alv_const c = constants();
+    check(alv_sample_with_pellets(&c, 0, f(nbr), u(cnt), f(pel), u(pcnt), nbr.shape_[1],
+                                  num_samples, nullptr, nullptr, nullptr,
+                                  static_cast<float*>(sample_density.ptr_)));
+  }
+  static void launch_sample_velocity_with_pellets(
+      Variable<1, float3> const& sample_x, Variable<1, float3> const& particle_x,
+      Variable<1, float> const& particle_density, Variable<1, float3> const& particle_v,
+      Variable<2, float4> const& nbr, Variable<1, U> const& cnt, Variable<1, float3>& sample_v,
+      Variable<2, float4> const& pel, Variable<1, U> const& pcnt, U num_samples) {
+    (void)sample_x;
+    (void)particle_x;
+    pellet_samples(nbr, cnt, pel, pcnt, num_samples);
+    if (sample_v.shape_[0] < num_samples) throw std::out_of_range("sample_v too small");
+    alv_const c = constants();
+    check(alv_sample_with_pellets(&c, 1, f(nbr), u(cnt), f(pel), u(pcnt), nbr.shape_[1],
+                                  num_samples, f(particle_density), f(particle_v), nullptr,
+                                  static_cast<float*>(sample_v.ptr_)));
+  }
+  static void launch_sample_vorticity_with_pellets(
+      Variable<1, float3> const& sample_x, Variable<1, float3> const& particle_x,
+      Variable<1, float> const& particle_density, Variable<1, float3> const& particle_v,
+      Variable<2, float4> const& nbr, Variable<1, U> const& cnt,
+      Variable<1, float3> const& sample_v, Variable<1, float3>& sample_vorticity,
+      Variable<2, float4> const& pel, Variable<1, U> const& pcnt, U num_samples) {
+    (void)sample_x;
+    (void)particle_x;
+    pellet_samples(nbr, cnt, pel, pcnt, num_samples);
+    if (sample_v.shape_[0] < num_samples || sample_vorticity.shape_[0] < num_samples)
+      throw std::out_of_range("sample_v / sample_vorticity too small");
+    alv_const c = constants();
+    check(alv_sample_with_pellets(&c, 2, f(nbr), u(cnt), f(pel), u(pcnt), nbr.shape_[1],
+                                  num_samples, f(particle_density), f(particle_v), f(sample_v),
+                                  static_cast<float*>(sample_vorticity.ptr_)));
+  }
+  static void pellet_samples(Variable<2, float4> const& nbr, Variable<1, U> const& cnt,
+                             Variable<2, float4> const& pel, Variable<1, U> const& pcnt, U ns) {
+    check_samples(nbr, cnt, ns);
+    check_samples(pel, pcnt, ns);
+    if (pel.shape_[1] != nbr.shape_[1])
+      throw std::invalid_argument("the two lists must have the same row length");
+  }
   // compute_particle_boundary_with_pellets (runner.hpp:5693-5703)
   static void launch_compute_particle_boundary_with_pellets(
       Variable<1, float4>& boundary_kernel_combined, Variable<2, float4> const& pellet_neighbors,

@@ -497,6 +497,18 @@ int alv_solver_sample_bead_pellet_neighbors(alv_solver* solver, const float* sam
  * sum W) over both lists; compute_dfsph_factor_with_pellets
  * (runner.hpp:3156-3190): the DFSPH factor, the pellet entries' vectors
  * taken as -(grad p_j) as the reference reads them. */
+/* Sampling with pellet lists: what 0 = sample_position_density_with_pellets
+ * (runner.hpp:2122-2147, out[ns] = m (sum W + sum W)), 1 =
+ * compute_sample_velocity_with_pellets (runner.hpp:4242-4277), 2 =
+ * compute_sample_vorticity_with_pellets (runner.hpp:4325-4359; sample_v the
+ * samples' velocity); out is float3 [ns] for 1 and 2.  Fluid entries are
+ * weighted m / rho_j, pellet entries by the particle volume; particle_v
+ * holds the pellets' velocities at rows max_n + k (the solver's field). */
+int alv_sample_with_pellets(const alv_const* cn, int what, const float* neighbors,
+                            const unsigned* counts, const float* pellet_neighbors,
+                            const unsigned* pellet_counts, unsigned K, unsigned num_samples,
+                            const float* particle_density, const float* particle_v,
+                            const float* sample_v, float* out);
 int alv_particle_boundary_with_pellets(const alv_const* cn, const float* pellet_neighbors,
                                        const unsigned* pellet_counts, unsigned K, unsigned n,
                                        float* boundary_kernel);

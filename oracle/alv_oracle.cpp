@@ -2830,6 +2830,54 @@ int orc_sample_bead_pellet(void* h, const double* sample_x, unsigned ns, unsigne
   });
 }
 
+int orc_sample_with_pellets(void* h, int what, const unsigned* ids, const double* xixj,
+                            const unsigned* counts, const unsigned* pids, const double* pxixj,
+                            const unsigned* pcounts, unsigned ns, const double* density,
+                            const double* v, const double* sample_v, double* out) {
+  return dispatch(h, [&](auto& s) {
+    using T = std::remove_cv_t<std::remove_reference_t<decltype(s.c.h)>>;
+    const unsigned K = s.c.kmax;
+    const auto& c = s.c;
+    for (unsigned i = 0; i < ns; ++i) {
+      const unsigned nb = std::min(counts[i], K), np_ = std::min(pcounts[i], K);
+      auto d3 = [&](const double* a, size_t o) { return V3<T>{T(a[3 * o]), T(a[3 * o + 1]), T(a[3 * o + 2])}; };
+      if (what == 0) {
+        // sample_position_density_with_pellets (runner.hpp:2122-2147)
+        T dens = 0;
+        for (unsigned t = 0; t < nb; ++t) dens += displacement_cubic_kernel(d3(xixj, size_t(i) * K + t), c);
+        for (unsigned t = 0; t < np_; ++t) dens += displacement_cubic_kernel(d3(pxixj, size_t(i) * K + t), c);
+        out[i] = double(dens * c.mass);
+        continue;
+      }
+      // compute_sample_velocity / vorticity_with_pellets (runner.hpp:4242-4359)
+      const V3<T> vi = what == 2 ? d3(sample_v, i) : V3<T>{0, 0, 0};
+      V3<T> r{0, 0, 0};
+      for (unsigned t = 0; t < nb; ++t) {
+        const size_t o = size_t(i) * K + t;
+        const unsigned j = ids[o];
+        const V3<T> d = d3(xixj, o);
+        if (what == 1)
+          r += (c.mass / T(density[j]) * displacement_cubic_kernel(d, c)) * d3(v, j);
+        else
+          r += (c.mass / T(density[j])) * cross(d3(v, j) - vi, displacement_cubic_kernel_grad(d, c));
+      }
+      for (unsigned t = 0; t < np_; ++t) {
+        const size_t o = size_t(i) * K + t;
+        const unsigned j = pids[o];
+        const V3<T> d = d3(pxixj, o);
+        if (what == 1)
+          r += (c.vol * displacement_cubic_kernel(d, c)) * d3(v, j);
+        else
+          r += c.vol * cross(d3(v, j) - vi, displacement_cubic_kernel_grad(d, c));
+      }
+      out[3 * i] = r.x;
+      out[3 * i + 1] = r.y;
+      out[3 * i + 2] = r.z;
+    }
+    return 0;
+  });
+}
+
 int orc_pellet_sums(void* h, int what, unsigned K, unsigned n, const double* bead_xixj,
                     const unsigned* bead_counts, const double* pel_xixj,
                     const unsigned* pel_counts, double* out) {

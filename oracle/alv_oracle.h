@@ -210,6 +210,14 @@ int orc_sample_bead_pellet(void* h, const double* sample_x, unsigned ns, unsigne
  * pellets (runner.hpp:3156-3190) takes the pellet entries' vectors as
  * grad_wvol, as the reference does: parity unpinned beyond its fluid part
  * (no closed form or fixture covers that reading). */
+/* Sampling with pellet lists (what 0 density, 1 velocity, 2 vorticity;
+ * runner.hpp:2122-2147, 4242-4359): lists [ns][K] (xixj packed xyz),
+ * density and v indexed by particle id (v [max_n + max_p][3]), sample_v
+ * [ns][3] for the vorticity; out [ns] or [ns][3]. */
+int orc_sample_with_pellets(void* h, int what, const unsigned* ids, const double* xixj,
+                            const unsigned* counts, const unsigned* pids, const double* pxixj,
+                            const unsigned* pcounts, unsigned ns, const double* density,
+                            const double* v, const double* sample_v, double* out);
 int orc_pellet_sums(void* h, int what, unsigned K, unsigned n, const double* bead_xixj,
                     const unsigned* bead_counts, const double* pel_xixj,
                     const unsigned* pel_counts, double* out);

@@ -136,6 +136,8 @@ def lib():
         L.orc_get_lists.argtypes = [vp, ctypes.c_int, vp, vp]
         L.orc_sample_bead_pellet.argtypes = [vp, dp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                                              up, dp, up, up, dp, up]
+        L.orc_sample_with_pellets.argtypes = [vp, ctypes.c_int, up, dp, up, up, dp, up,
+                                              ctypes.c_uint, dp, dp, dp, dp]
         L.orc_pellet_sums.argtypes = [vp, ctypes.c_int, ctypes.c_uint, ctypes.c_uint, dp, up, dp,
                                       up, dp]
         L.orc_get_grid.argtypes = [vp, vp, vp, vp]
@@ -485,6 +487,27 @@ class Oracle:
                                              for a in out])
         if rc != 0:
             raise ValueError("orc_sample_bead_pellet: %d" % rc)
+        return out
+
+    def sample_with_pellets(self, what, lists, pel_lists, density=None, v=None, sample_v=None):
+        """what 0 density, 1 velocity, 2 vorticity from (ids, xixj, counts)
+        fluid and pellet sample lists; density [max_n], v [max_n + max_p, 3]."""
+        ids, xixj, cnt = (np.ascontiguousarray(a) for a in lists)
+        pids, pxixj, pcnt = (np.ascontiguousarray(a) for a in pel_lists)
+        ns = cnt.shape[0]
+        c = lambda a, t: np.ascontiguousarray(a, t)
+        dens = c(np.zeros(1) if density is None else density, np.float64)
+        vv = c(np.zeros((1, 3)) if v is None else v, np.float64)
+        sv = c(np.zeros((ns, 3)) if sample_v is None else sample_v, np.float64)
+        out = np.zeros(ns if what == 0 else (ns, 3))
+        rc = self.L.orc_sample_with_pellets(self.h, what, _uptr(c(ids, np.uint32)),
+                                            _dptr(c(xixj, np.float64)), _uptr(c(cnt, np.uint32)),
+                                            _uptr(c(pids, np.uint32)),
+                                            _dptr(c(pxixj, np.float64)),
+                                            _uptr(c(pcnt, np.uint32)), ns, _dptr(dens),
+                                            _dptr(vv), _dptr(sv), _dptr(out))
+        if rc != 0:
+            raise ValueError("orc_sample_with_pellets: %d" % rc)
         return out
 
     def pellet_sums(self, what, pel_xixj, pel_counts, bead_xixj=None, bead_counts=None):

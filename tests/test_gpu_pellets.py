@@ -169,3 +169,49 @@ def test_runner_pellet_lists_and_sums_match_oracle():
     assert (ids_o >= mx).any()
     for k in range(ns):
         np.testing.assert_array_equal(lists[k, :cnt_o[k], 3].view(np.uint32), ids_o[k, :cnt_o[k]])
+
+
+def test_sampling_with_pellets_matches_oracle():
+    from alluvion_amd import scenes
+    sc = small_scene()
+    dp, pile, solver, _, _ = scenes.build_solver(sc, kind="i")
+    for _ in range(3):
+        solver.step()
+    solver.update_particle_neighbors()  # grid of the current positions
+    orc = of.Oracle(sc, precision=4, kind="i")
+    runner = dp.Runner()
+    mx, K = solver.max_num_particles, sc.max_num_neighbors
+    x = dp.coat(solver.particle_x).get()
+    rng = np.random.default_rng(4)
+    ns = 200
+    pts = (x[rng.choice(sc.n, ns)] + rng.uniform(-0.004, 0.004, (ns, 3))).astype(np.float32)
+    sx = dp.create_coated((ns,), 3)
+    sx.set(pts)
+    bead, pel = dp.create((ns, K), 4), dp.create((ns, K), 4)
+    nb, npl = dp.create((ns,), 1, np.uint32), dp.create((ns,), 1, np.uint32)
+    runner.launch_make_bead_pellet_neighbor_list(sx, solver.pid, solver.pid_length, bead, nb, pel,
+                                                 npl, solver.grid_anomaly, mx, ns, 0)
+    g_b = dp.coat(bead).get().reshape(ns, K, 4)
+    g_p = dp.coat(pel).get().reshape(ns, K, 4)
+    lists = (np.ascontiguousarray(g_b[..., 3]).view(np.uint32), g_b[..., :3].astype(np.float64),
+             dp.coat(nb).get())
+    pl = (np.ascontiguousarray(g_p[..., 3]).view(np.uint32), g_p[..., :3].astype(np.float64),
+          dp.coat(npl).get())
+    assert pl[2].max() > 0
+    dens = dp.coat(solver.particle_density).get().astype(np.float64)
+    v = dp.coat(solver.particle_v).get().astype(np.float64)
+    assert np.abs(v[mx:mx + pile.num_pellets]).max() > 0  # the sphere's pellets move
+    out1 = dp.create((ns,), 1)
+    runner.launch_sample_density_with_pellets(sx, bead, nb, pel, npl, out1, ns)
+    assert rel(dp.coat(out1).get(), orc.sample_with_pellets(0, lists, pl)) < 1e-6
+    sv = dp.create((ns,), 3)
+    runner.launch_sample_velocity_with_pellets(sx, solver.particle_x, solver.particle_density,
+                                               solver.particle_v, bead, nb, sv, pel, npl, ns)
+    sv_o = orc.sample_with_pellets(1, lists, pl, dens, v)
+    assert rel(dp.coat(sv).get(), sv_o) < 1e-5
+    vort = dp.create((ns,), 3)
+    runner.launch_sample_vorticity_with_pellets(sx, solver.particle_x, solver.particle_density,
+                                                solver.particle_v, bead, nb, sv, vort, pel, npl,
+                                                ns)
+    svg = dp.coat(sv).get().astype(np.float64)
+    assert rel(dp.coat(vort).get(), orc.sample_with_pellets(2, lists, pl, dens, v, sample_v=svg)) < 1e-4

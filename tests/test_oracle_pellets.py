@@ -329,3 +329,51 @@ def test_boundary_kernel_with_pellets_balances_on_a_lattice():
     # FP32 sites break the lattice's symmetry at ~1e-6 of a single gradient
     np.testing.assert_allclose(bk[:, :3] + vol * gw_f, 0, atol=2e-5 * np.abs(bk[:, :3]).max())
     assert np.abs(bk[:, :3]).max() > 0
+
+
+def test_sampling_with_pellets_on_a_lattice():
+    # perfect periodic lattice with every 7th site a pellet: at interior sites
+    # the sampled density is m S, a uniform velocity and a rigid rotation are
+    # reproduced (fluid weights m / rho_j = 1/S, pellet weights vol = 1/S),
+    # and the vorticity of the rotation is 2 c w, c = sum_d (1/S) dW/dx d_x
+    S = lattice_sum(2 * R, H)
+    mass = 1000.0 / S
+    sc, x, v = periodic_lattice(m=20, mass=mass)
+    x = x.astype(np.float32).astype(np.float64)
+    L = sc.wrap_length[0]
+    pel = (np.arange(x.shape[0]) % 7 == 3)
+    sc.x, sc.v = x[~pel].astype(np.float32), v[~pel].astype(np.float32)
+    sc.volume_method, sc.max_num_pellets = "pellets", int(pel.sum())
+    sc.boundaries = [scenes.BoundarySpec("box", (0.05, 0.05, 0.05, 0.0), 1.0, (4, 4, 4),
+                                         collision_vertices=x[pel])]
+    orc = of.Oracle(sc, precision=8, kind="i")
+    orc.set("x", np.concatenate([x[~pel], x[pel]]))
+    orc.transform_pellets()
+    orc.update_particle_neighbors()
+    n, mx, npl = sc.n, orc.max_n, int(pel.sum())
+    allx = orc.get("x")
+    inner = np.nonzero(np.all(np.abs(allx[:n]) < 0.5 * L - 2 * H, axis=1))[0]
+    idx = inner[inner >= 100][:50]  # sample k never sits on particle k
+    samples = allx[idx]
+    bid, bxx, bc, pid, pxx, pc = orc.sample_bead_pellet(samples)
+    lists, pl = (bid, bxx, bc), (pid, pxx, pc)
+    assert pc.min() > 0
+    np.testing.assert_allclose(orc.sample_with_pellets(0, lists, pl), mass * S, rtol=1e-6)
+    dens = orc.get("density")
+    vv = np.zeros((orc.nt, 3))
+    U = np.array([0.4, -0.2, 0.9])
+    vv[:n] = U
+    vv[mx:mx + npl] = U
+    np.testing.assert_allclose(orc.sample_with_pellets(1, lists, pl, dens, vv),
+                               np.tile(U, (50, 1)), rtol=2e-6)
+    w = np.array([0.5, -1.5, 2.0])
+    vv[:n] = np.cross(w, allx[:n])
+    vv[mx:mx + npl] = np.cross(w, allx[mx:mx + npl])
+    sv = orc.sample_with_pellets(1, lists, pl, dens, vv)
+    np.testing.assert_allclose(sv, np.cross(w, samples), rtol=0, atol=1e-6 * np.abs(sv).max())
+    vort = orc.sample_with_pellets(2, lists, pl, dens, vv, sample_v=sv)
+    g = np.arange(-2, 3) * 2 * R
+    D = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    gw, _ = of.cubic_kernel(D, H)
+    c = np.sum(gw[:, 0] * D[:, 0]) / S
+    np.testing.assert_allclose(vort, np.tile(2 * c * w, (50, 1)), rtol=1e-5)
