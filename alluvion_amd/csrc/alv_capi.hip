@@ -1216,6 +1216,7 @@ static int solver_alloc(alv_solver* S) {
   ALV_ALLOC(A.key, cap);
   ALV_ALLOC(A.slot, cap);
   ALV_ALLOC(A.sorted_id, cap);
+  ALV_ALLOC(A.sort_tmp, cap);
   const size_t CT = S->cells_total;
   ALV_ALLOC(A.cell_count, CT);
   ALV_ALLOC(A.cell_start, CT + 1);
@@ -1273,6 +1274,7 @@ static int solver_alloc(alv_solver* S) {
     ALV_ALLOC(A.pkey, pcap);
     ALV_ALLOC(A.pslot, pcap);
     ALV_ALLOC(A.psorted_id, pspan);
+    ALV_ALLOC(A.psort_tmp, pspan);
     ALV_ALLOC(A.pcell_count, CT);
     ALV_ALLOC(A.pcell_start, CT + 1);
     ALV_ALLOC(A.pscan_tmp, (CT + 4095) / 4096 + 1);

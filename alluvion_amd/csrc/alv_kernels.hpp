@@ -44,6 +44,7 @@ struct DevArrays {
   unsigned* key;
   unsigned* slot;
   unsigned* sorted_id;  // sorted position -> user index
+  unsigned* sort_tmp;   // the scatter's arrival-ordered user indices
   unsigned* cell_count;  // [num_cells + 1]; last bin = outside the grid
   unsigned* cell_start;  // [num_cells + 2]
   unsigned* scan_tmp;
@@ -88,6 +89,7 @@ struct DevArrays {
   unsigned* pkey;
   unsigned* pslot;
   unsigned* psorted_id;  // sorted pellet -> user pellet index
+  unsigned* psort_tmp;
   unsigned* pcell_count;
   unsigned* pcell_start;
   unsigned* pscan_tmp;

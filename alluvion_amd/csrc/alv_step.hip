@@ -199,51 +199,29 @@ __global__ __launch_bounds__(256) void k_scan_apply(unsigned count, const unsign
 __global__ __launch_bounds__(256) void k_scatter(SceneTab T, DevArrays A) {
   ALV_SCENE_LINEAR(T, A, i);
   if (i >= p.first + p.n) return;
-  A.sorted_id[A.cell_start[A.key[i]] + A.slot[i]] = i;
+  A.sort_tmp[A.cell_start[A.key[i]] + A.slot[i]] = i;
 }
 
-// Makes the order inside each cell canonical (ascending particle id); the
-// atomic slots of k_cell_count are arrival-ordered.  The bin of particles
-// outside the grid is left unordered: nothing depends on its order (those
-// particles are in no cell and their own neighbour scans are independent).
+// Makes the order inside each cell canonical (ascending particle id): the
+// atomic slots of k_cell_count are arrival-ordered, so every particle takes
+// the position of its rank among its cell's particles in the scatter's
+// copy.  The bin of particles outside the grid keeps the arrival order:
+// nothing depends on it (those particles are in no cell and their own
+// neighbour scans are independent).
 __global__ __launch_bounds__(256) void k_cell_sort(SceneTab T, DevArrays A) {
-  const unsigned c = blockIdx.x * kBlock + threadIdx.x;
-  const SimParams p = T.par[T.cblk_scene[blockIdx.x]];
-  if (c < p.cell_base || c - p.cell_base >= p.num_cells) return;
-  const unsigned b = A.cell_start[c], e = A.cell_start[c + 1];
-  if (e - b > p.cell_cap) A.anomaly[1] = 1u;
-  if (e - b < 2) return;
-  unsigned* ids = A.sorted_id;
-  const unsigned m = e - b;
-  if (m <= 32) {
-    unsigned loc[32];
-    for (unsigned t = 0; t < m; ++t) loc[t] = ids[b + t];
-    for (unsigned t = 1; t < m; ++t) {
-      unsigned v = loc[t];
-      unsigned u = t;
-      while (u > 0 && loc[u - 1] > v) {
-        loc[u] = loc[u - 1];
-        --u;
-      }
-      loc[u] = v;
-    }
-    for (unsigned t = 0; t < m; ++t) ids[b + t] = loc[t];
-  } else {
-    // over-full cell (an anomaly): shell sort in place, gaps 3k+1
-    unsigned gap = 1;
-    while (gap < m / 3) gap = 3 * gap + 1;
-    for (; gap > 0; gap /= 3) {
-      for (unsigned t = gap; t < m; ++t) {
-        unsigned v = ids[b + t];
-        unsigned u = t;
-        while (u >= gap && ids[b + u - gap] > v) {
-          ids[b + u] = ids[b + u - gap];
-          u -= gap;
-        }
-        ids[b + u] = v;
-      }
-    }
+  ALV_SCENE_LINEAR(T, A, i);
+  if (i >= p.first + p.n) return;
+  const unsigned k = A.key[i];
+  const unsigned b = A.cell_start[k];
+  if (k == p.cell_base + p.num_cells) {
+    A.sorted_id[b + A.slot[i]] = i;
+    return;
   }
+  const unsigned e = A.cell_start[k + 1];
+  if (e - b > p.cell_cap) A.anomaly[1] = 1u;
+  unsigned rank = 0;
+  for (unsigned t = b; t < e; ++t) rank += A.sort_tmp[t] < i ? 1u : 0u;
+  A.sorted_id[b + rank] = i;
 }
 
 // LDS tile of every particle block.  For a block whose in-grid particles all
@@ -2201,7 +2179,7 @@ void launch_scatter(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH(k_scatter, L, s, A);
 }
 void launch_cell_sort(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_GRID(k_cell_sort, dim3(L.cblocks), s, L.T, A);
+  ALV_LAUNCH(k_cell_sort, L, s, A);
 }
 void launch_tiles(const Launch& L, const DevArrays& A, hipStream_t s) {
   if (!A.use_tiles) return;
@@ -2337,6 +2315,7 @@ void launch_pellet_grid(const Launch& Lp, const DevArrays& A, hipStream_t s) {
   Ap.key = A.pkey;
   Ap.slot = A.pslot;
   Ap.sorted_id = A.psorted_id;
+  Ap.sort_tmp = A.psort_tmp;
   Ap.cell_count = A.pcell_count;
   Ap.cell_start = A.pcell_start;
   Ap.scan_tmp = A.pscan_tmp;
