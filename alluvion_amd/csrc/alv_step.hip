@@ -448,8 +448,10 @@ __device__ inline bool stage_x(const DevArrays& A, unsigned blk, TileX& t) {
 // across two columns) reads global memory.
 constexpr unsigned kStage = 128;
 
+// Bounded to 64 VGPRs (8 waves per SIMD; a few bytes of spills), measured
+// slightly faster than 73 VGPRs at 6 waves.
 template <bool W, bool TL>
-__global__ __launch_bounds__(256) void k_neighbors(SceneTab T, DevArrays A, int with_factor) {
+__global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, int with_factor) {
   // LDS staging for the gather instances without the periodic wrap (their
   // rows are single spans); the others keep the lane-private scan
   constexpr bool kWaveStage = !W && !TL;
