@@ -501,9 +501,12 @@ __global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, i
 #pragma unroll
       for (int u = 0; u < kCB; ++u) {
         const vec3 d = wrapd<W>(xi - ld3(xc[u]), p);
-        const bool h = (j0 + u < je) & (j0 + u != s) & (length_sqr(d) < p.h2);
-        hits |= unsigned(h) << u;
+        hits |= unsigned(length_sqr(d) < p.h2) << u;
       }
+      // the batch's slots past the span and the particle itself, once per
+      // batch rather than per candidate
+      if (je - j0 < kCB) hits &= (1u << (je - j0)) - 1u;
+      if (s - j0 < kCB) hits &= ~(1u << (s - j0));
       while (hits) {
         const unsigned j = j0 + unsigned(__builtin_ctz(hits));
         hits &= hits - 1u;
