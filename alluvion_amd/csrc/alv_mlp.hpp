@@ -69,9 +69,12 @@ __device__ inline f32x4 bias_tile(const float* b, int m, int g, int nrows) {
   return c;
 }
 
-__device__ inline float mlp_silu(float z) { return z / (1.0f + expf(-z)); }
+// SiLU z sigma(z) with the hardware exponential and reciprocal (about 1 ulp
+// each): the network computes in bfloat16, whose rounding is 2^-8.
+__device__ inline float mlp_sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float mlp_silu(float z) { return z * mlp_sigmoid(z); }
 __device__ inline float mlp_silu_grad(float z) {
-  const float s = 1.0f / (1.0f + expf(-z));
+  const float s = mlp_sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -147,24 +150,64 @@ __device__ inline f32x4 mlp_infer_tile(const MlpDev& M, bf16x8 u, int lane) {
 
 // Wave-level inference for 64 samples held one per lane (raw features f).
 // Every lane of the wave must call it (MFMA is wave-wide).  Returns the
-// de-normalised outputs of this lane's sample.
+// de-normalised outputs of this lane's sample.  The four 16-sample tiles go
+// through each layer together, so every weight operand is loaded once per
+// wave; the arithmetic per tile is mlp_forward_tile's.
 __device__ inline void mlp_infer_wave(const MlpDev& M, const float (&f)[kMlpIn], float& out0,
                                       float& out1) {
   const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  const bf16x8* ops = M.ops;
+  const float* P = M.params;
   float xn[kMlpIn];
 #pragma unroll
   for (int k = 0; k < kMlpIn; ++k) xn[k] = (f[k] - M.norm[k]) * M.norm[kMlpIn + k];
+  bf16x8 u[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k) {
+      const float v = __shfl(xn[k], t * 16 + c, 64);
+      u[t][k] = (__bf16)(g == 0 ? v : 0.0f);
+    }
+  f32x4 z[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const bf16x8 w = ops[kOpsF1 + m * 64 + lane];
+    const f32x4 b = bias_tile(P + kOffB1, m, g, kMlpHid);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) z[t][m] = mfma16x16x32(w, u[t], b);
+  }
+  bf16x8 a[4][2];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f32x4 act[4];
+    silu_tiles(z[t], act);
+    tiles_to_operands(act, a[t]);
+  }
+  const int F[2] = {kOpsF2, kOpsF3}, B[2] = {kOffB2, kOffB3};
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const bf16x8 w0 = ops[F[layer] + (m * 2 + 0) * 64 + lane];
+      const bf16x8 w1 = ops[F[layer] + (m * 2 + 1) * 64 + lane];
+      const f32x4 b = bias_tile(P + B[layer], m, g, kMlpHid);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) z[t][m] = mfma16x16x32(w1, a[t][1], mfma16x16x32(w0, a[t][0], b));
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      f32x4 act[4];
+      silu_tiles(z[t], act);
+      tiles_to_operands(act, a[t]);
+    }
+  }
+  const bf16x8 w0 = ops[kOpsF4 + 0 * 64 + lane], w1 = ops[kOpsF4 + 1 * 64 + lane];
+  const f32x4 b = bias_tile(P + kOffB4, 0, g, kMlpOut);
   float r0 = 0.0f, r1 = 0.0f;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    bf16x8 u;
-    const int src = t * 16 + c;
-#pragma unroll
-    for (int k = 0; k < kMlpIn; ++k) {
-      const float v = __shfl(xn[k], src, 64);
-      u[k] = (__bf16)(g == 0 ? v : 0.0f);
-    }
-    const f32x4 y = mlp_infer_tile(M, u, lane);
+    const f32x4 y = mfma16x16x32(w1, a[t][1], mfma16x16x32(w0, a[t][0], b));
     // lanes 0-15 hold the outputs of samples t*16 + lane
     const float y0 = __shfl(y[0], c, 64), y1 = __shfl(y[1], c, 64);
     if (g == t) {
