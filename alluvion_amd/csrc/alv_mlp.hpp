@@ -69,12 +69,15 @@ __device__ inline f32x4 bias_tile(const float* b, int m, int g, int nrows) {
   return c;
 }
 
-// SiLU z sigma(z) with the hardware exponential and reciprocal (about 1 ulp
-// each): the network computes in bfloat16, whose rounding is 2^-8.
-__device__ inline float mlp_sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
-__device__ inline float mlp_silu(float z) { return z * mlp_sigmoid(z); }
+__device__ inline float mlp_silu(float z) { return z / (1.0f + expf(-z)); }
+// The same with the hardware exponential and reciprocal (about 1 ulp each;
+// the network computes in bfloat16, rounding 2^-8), for the per-step
+// inference; training keeps the correctly rounded form.
+__device__ inline float mlp_silu_fast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 __device__ inline float mlp_silu_grad(float z) {
-  const float s = mlp_sigmoid(z);
+  const float s = 1.0f / (1.0f + expf(-z));
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -89,11 +92,12 @@ __device__ inline void tiles_to_operands(const f32x4 (&t)[4], bf16x8 (&op)[2]) {
     }
 }
 
+template <bool kFast = false>
 __device__ inline void silu_tiles(const f32x4 (&z)[4], f32x4 (&a)[4]) {
 #pragma unroll
   for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[m][r] = mlp_silu(z[m][r]);
+    for (int r = 0; r < 4; ++r) a[m][r] = kFast ? mlp_silu_fast(z[m][r]) : mlp_silu(z[m][r]);
 }
 
 // One hidden layer: z = W x + b for 64 outputs from two k-steps of input.
@@ -181,7 +185,7 @@ __device__ inline void mlp_infer_wave(const MlpDev& M, const float (&f)[kMlpIn],
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     f32x4 act[4];
-    silu_tiles(z[t], act);
+    silu_tiles<true>(z[t], act);
     tiles_to_operands(act, a[t]);
   }
   const int F[2] = {kOpsF2, kOpsF3}, B[2] = {kOffB2, kOffB3};
@@ -198,7 +202,7 @@ __device__ inline void mlp_infer_wave(const MlpDev& M, const float (&f)[kMlpIn],
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       f32x4 act[4];
-      silu_tiles(z[t], act);
+      silu_tiles<true>(z[t], act);
       tiles_to_operands(act, a[t]);
     }
   }

@@ -335,7 +335,8 @@ def timed_run(make_scene, steps, net=None):
     al.device_synchronize()
     el = time.perf_counter() - t0
     out = {"iterations_per_step": round(it / steps, 3), "ms_per_step": round(1e3 * el / steps, 4),
-           "particle_steps_per_s": round(sc.n * steps / el, 1), "simulated_s": round(solver.t, 5)}
+           "particle_steps_per_s": round(sc.n * steps / el, 1), "simulated_s": round(solver.t, 5),
+           "simulated_s_per_wall_s": round(solver.t / el, 4)}
     solver.set_warm_start_net(None)
     del solver, pile, dp
     return out
@@ -389,7 +390,11 @@ def learned_comparison(scale, body_resolution):
                      "iteration_ratio": round(plain["iterations_per_step"] /
                                               learned["iterations_per_step"], 3),
                      "speedup": round(learned["particle_steps_per_s"] /
-                                      plain["particle_steps_per_s"], 3)})
+                                      plain["particle_steps_per_s"], 3),
+                     # physical progress per wall second: the warm start
+                     # changes the trajectory and with it the CFL time step
+                     "simulated_time_speedup": round(learned["simulated_s_per_wall_s"] /
+                                                     plain["simulated_s_per_wall_s"], 3)})
         log("warm start %s: plain %s learned %s" % (name, plain, learned))
     return {"time_step": "CFL-limited, cfl 0.4, dt <= 2e-3", "tolerance": "0.1% (divergence, density)",
             "train": {"pairs": int(count),
