@@ -2200,6 +2200,8 @@ int alv_solver_get_pellet_neighbors(alv_solver* S, int which, unsigned* ids, uns
   if (!S) return fail(ALV_ERR_ARG, "null solver");
   if (!S->pcap) return fail(ALV_ERR_STATE, "the solver has no pellets");
   if (which != 0 && which != 1) return fail(ALV_ERR_ARG, "which is 0 or 1");
+  if (!S->grid_valid)
+    return fail(ALV_ERR_STATE, "no neighbour search yet: step or update_particle_neighbors first");
   Launch L;
   int rc = solver_launch(S, &L);
   if (rc != ALV_OK) return rc;
@@ -2223,13 +2225,16 @@ int alv_solver_get_pellet_neighbors(alv_solver* S, int which, unsigned* ids, uns
   if (counts) std::fill(counts, counts + rows, 0u);
   for (unsigned s = 0; s < live; ++s) {
     const unsigned row = which == 0 ? fsorted[s] : psorted[s];
+    if (row >= rows) return fail(ALV_ERR_STATE, "pellet lists are out of date");
     const unsigned c = cnt[s];
     if (counts) counts[row] = c;
     if (!ids) continue;
-    for (unsigned t = 0; t < c; ++t) {
+    for (unsigned t = 0; t < c && t < K; ++t) {
       const uint4& g = lists[(size_t(s >> 6) * G + (t >> 2)) * 64u + (s & 63u)];
       const unsigned j = (&g.x)[t & 3u];
-      ids[size_t(row) * K + t] = which == 0 ? S->cap + psorted[j] : fsorted[j];
+      const std::vector<unsigned>& map = which == 0 ? psorted : fsorted;
+      if (j >= map.size()) return fail(ALV_ERR_STATE, "pellet lists are out of date");
+      ids[size_t(row) * K + t] = which == 0 ? S->cap + map[j] : map[j];
     }
   }
   return ALV_OK;
