@@ -155,26 +155,108 @@ def adaptive_step(scene: Scene, cfl: float = 0.4, max_dt: float = 2e-3,
     return scene
 
 
-def dam_break_rigid(seed: int = 2, body_seed: int = 3, nx: int = 128, ny: int = 128,
-                    nz: int = 64, body_resolution: int = 128) -> Scene:
-    r, h = 0.0025, 0.01
-    x = lattice_block(nx, ny, nz, r, -0.5 + h - 2 * r, seed, 0.01 * r)
-    u = uniform(body_seed, 12).reshape(4, 3)
+SPHERE_RADIUS, BOX_EDGE = 0.05, 0.08
+SPHERE_DENSITY, BOX_DENSITY = 500.0, 800.0
+
+
+def body_distance(p: np.ndarray, body: BoundarySpec) -> np.ndarray:
+    """Signed distance (negative inside) of points p [n, 3] to an unrotated
+    sphere or box body: the closed forms of dg/sphere_distance.hpp:18-21 and
+    dg/box_distance.hpp:23-28."""
+    d = p.astype(np.float64) - np.asarray(body.x, np.float64)
+    if body.kind == "sphere":
+        return np.linalg.norm(d, axis=1) - body.params[0]
+    q = np.abs(d) - 0.5 * np.asarray(body.params[:3], np.float64)
+    outside = np.linalg.norm(np.maximum(q, 0.0), axis=1)
+    return outside + np.minimum(q.max(axis=1), 0.0)
+
+
+def floating_bodies(x_range, z_range, y_top, body_seed, resolution) -> List[BoundarySpec]:
+    """Two spheres (r = 0.05 m, 500 kg/m^3) and two cubes (edge 0.08 m,
+    800 kg/m^3) floating at their equilibrium depth in a fluid whose top
+    lattice layer is at y_top: a sphere's centre on the surface (half
+    submerged), a cube's 0.3 edge below it (80 % submerged).  The fluid's
+    x range is cut into four slots, sphere, cube, sphere, cube; each centre
+    is seeded uniformly (SplitMix64) inside its slot with one kernel radius
+    of clearance to the slot's and the fluid's sides."""
+    h = 0.01
+    u = uniform(body_seed, 8).reshape(4, 2)
+    slot = (x_range[1] - x_range[0]) / 4.0
     bodies = []
-    sphere_r, box_a = 0.05, 0.08
-    rho_sphere, rho_box = 500.0, 800.0
     for b in range(4):
-        pos = (-0.35 + 0.7 * u[b, 0], 0.2 + 0.2 * u[b, 1], -0.35 + 0.7 * u[b, 2])
-        res = (body_resolution,) * 3
-        if b < 2:
-            m = rho_sphere * 4.0 / 3.0 * np.pi * sphere_r ** 3
-            inertia = (0.4 * m * sphere_r ** 2,) * 3
-            bodies.append(BoundarySpec("sphere", (sphere_r,), 1.0, res, m, inertia, pos))
+        sphere = b % 2 == 0
+        half = SPHERE_RADIUS if sphere else 0.5 * BOX_EDGE
+        room_x = slot - 2.0 * (half + h)
+        room_z = (z_range[1] - z_range[0]) - 2.0 * (half + h)
+        if room_x <= 0 or room_z <= 0:
+            raise ValueError("the fluid block is too small to float the bodies")
+        cx = x_range[0] + (b + 0.5) * slot + (u[b, 0] - 0.5) * room_x
+        cz = 0.5 * (z_range[0] + z_range[1]) + (u[b, 1] - 0.5) * room_z
+        cy = y_top if sphere else y_top - 0.3 * BOX_EDGE
+        res = (resolution,) * 3
+        if sphere:
+            m = SPHERE_DENSITY * 4.0 / 3.0 * np.pi * SPHERE_RADIUS ** 3
+            inertia = (0.4 * m * SPHERE_RADIUS ** 2,) * 3
+            bodies.append(BoundarySpec("sphere", (SPHERE_RADIUS,), 1.0, res, m, inertia,
+                                       (cx, cy, cz)))
         else:
-            m = rho_box * box_a ** 3
-            inertia = (m * box_a ** 2 / 6.0,) * 3
-            bodies.append(BoundarySpec("box", (box_a, box_a, box_a, 0.0), 1.0, res, m, inertia,
-                                       pos))
+            m = BOX_DENSITY * BOX_EDGE ** 3
+            inertia = (m * BOX_EDGE ** 2 / 6.0,) * 3
+            bodies.append(BoundarySpec("box", (BOX_EDGE, BOX_EDGE, BOX_EDGE, 0.0), 1.0, res, m,
+                                       inertia, (cx, cy, cz)))
+    return bodies
+
+
+def dam_break_rigid(seed: int = 2, body_seed: int = 3, nx: int = 128, ny: int = 128,
+                    nz: int = 64, body_resolution: int = 128,
+                    floating: Optional[bool] = None) -> Scene:
+    """The 1 M-particle dam break with four dynamic bodies.
+
+    floating (the default whenever the fluid block is wide enough): the
+    bodies float partly submerged in the fluid's top (``floating_bodies``);
+    lattice particles closer than h to a body are dropped, as the reference's
+    scene helpers drop particles closer than h to a boundary, and the lattice
+    continues upward in its own index order until exactly nx * ny * nz
+    particles remain (the first nx * ny * nz lattice sites, jitter included,
+    are the plain block's).  Otherwise the bodies are seeded in the air above
+    the fluid."""
+    r, h = 0.0025, 0.01
+    d = 2 * r
+    box_min = -0.5 + h - 2 * r
+    lo = box_min + d  # first lattice layer
+    n = nx * ny * nz
+    x_range, z_range = (lo, lo + (nx - 1) * d), (lo, lo + (nz - 1) * d)
+    if floating is None:
+        floating = (x_range[1] - x_range[0]) >= 4 * (2 * SPHERE_RADIUS + 2 * h) and \
+            (z_range[1] - z_range[0]) >= 2 * SPHERE_RADIUS + 2 * h
+    if floating:
+        bodies = floating_bodies(x_range, z_range, lo + (ny - 1) * d, body_seed, body_resolution)
+        extra = 1
+        while True:
+            x = lattice_block(nx, ny + extra, nz, r, box_min, seed, 0.01 * r)
+            keep = np.ones(x.shape[0], bool)
+            for b in bodies:
+                keep &= body_distance(x, b) >= h
+            if keep.sum() >= n:
+                x = x[keep][:n]
+                break
+            extra += 1
+    else:
+        x = lattice_block(nx, ny, nz, r, box_min, seed, 0.01 * r)
+        u = uniform(body_seed, 12).reshape(4, 3)
+        bodies = []
+        for b in range(4):
+            pos = (-0.35 + 0.7 * u[b, 0], 0.2 + 0.2 * u[b, 1], -0.35 + 0.7 * u[b, 2])
+            res = (body_resolution,) * 3
+            if b < 2:
+                m = SPHERE_DENSITY * 4.0 / 3.0 * np.pi * SPHERE_RADIUS ** 3
+                inertia = (0.4 * m * SPHERE_RADIUS ** 2,) * 3
+                bodies.append(BoundarySpec("sphere", (SPHERE_RADIUS,), 1.0, res, m, inertia, pos))
+            else:
+                m = BOX_DENSITY * BOX_EDGE ** 3
+                inertia = (m * BOX_EDGE ** 2 / 6.0,) * 3
+                bodies.append(BoundarySpec("box", (BOX_EDGE, BOX_EDGE, BOX_EDGE, 0.0), 1.0, res,
+                                           m, inertia, pos))
     container = BoundarySpec("box", (1.0, 1.0, 1.0, 0.0), -1.0, (64, 64, 64))
     return Scene(
         name="dam_break_1m_rigid",

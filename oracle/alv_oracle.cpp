@@ -29,6 +29,24 @@
 
 namespace orc {
 
+// Planted mistakes (orc_plant): a test switches one on to show that its
+// check fails on a plausible error in the oracle.  0 = the reference's
+// algorithm.
+enum Plant {
+  kPlantNone = 0,
+  kPlantFactorNoGradSum = 1,     // DFSPH factor without |sum_j grad p_j|^2
+  kPlantFactorBoundarySign = 2,  // boundary gradient subtracted in the factor
+  kPlantDivWarmClamp = 3,        // divergence warm start clamped at -0.25
+  kPlantDensWarmClamp = 4,       // density warm start clamped at -5e-4
+  kPlantDivFinish = 5,           // stored kappa_v = kappa_v dt^2
+  kPlantDensFinish = 6,          // stored kappa = kappa dt
+  kPlantIiOmega = 7,             // IISPH relaxation weight 1/2
+  kPlantIiAiiNoDji = 8,          // IISPH a_ii without the d_ji term
+  kPlantIiDiiDensity = 9,        // IISPH d_ii with 1/rho instead of 1/rho^2
+  kPlantCflRadius = 10,          // CFL length scale r instead of 2r
+};
+static int g_plant = kPlantNone;
+
 template <class T>
 struct V3 {
   T x, y, z;
@@ -1001,9 +1019,12 @@ struct Solver {
       }
       for (unsigned b = 0; b < bnd.size(); ++b) {
         V4<T> k = bker(b, i);
-        grad_p_i += V3<T>{k.x, k.y, k.z};
+        if (g_plant == kPlantFactorBoundarySign)
+          grad_p_i -= V3<T>{k.x, k.y, k.z};
+        else
+          grad_p_i += V3<T>{k.x, k.y, k.z};
       }
-      sum_grad_p_k += length_sqr(grad_p_i);
+      if (g_plant != kPlantFactorNoGradSum) sum_grad_p_k += length_sqr(grad_p_i);
       dfsph_factor[i] = sum_grad_p_k > c.eps ? -1 / sum_grad_p_k : T(0);
     }
   }
@@ -1158,7 +1179,7 @@ struct Solver {
     T pile_max = pile_cfl_v2();
     T max_v2 = std::max(pmax, pile_max);
     T max_v = std::sqrt(max_v2);
-    T length_scale = c.radius * 2;
+    T length_scale = g_plant == kPlantCflRadius ? c.radius : c.radius * 2;
     T cfl_dt = static_cast<T>(set.cfl) * length_scale / max_v;
     dt = std::max(std::min(cfl_dt, static_cast<T>(set.max_dt)),
                   static_cast<T>(set.min_dt));
@@ -1386,12 +1407,12 @@ struct Solver {
       compute_dfsph_factor();
       // warm_start_divergence_solve_0 (runner.hpp:3230-3250)
       std::vector<T> kv_new(n);
+      const T div_clamp = static_cast<T>(g_plant == kPlantDivWarmClamp ? -0.25 : -0.5);
 #pragma omp parallel for schedule(static)
       for (long ii = 0; ii < long(n); ++ii) {
         unsigned i = unsigned(ii);
         T dadv = density_change(i);
-        kv_new[i] = dadv > 0 ? static_cast<T>(0.5) *
-                                   std::max(kappa_v[i], static_cast<T>(-0.5)) / dt
+        kv_new[i] = dadv > 0 ? static_cast<T>(0.5) * std::max(kappa_v[i], div_clamp) / dt
                              : static_cast<T>(0);
       }
       for (unsigned i = 0; i < n; ++i) kappa_v[i] = kv_new[i];
@@ -1441,7 +1462,7 @@ struct Solver {
       // divergence_solve_finish (runner.hpp:3412-3420)
       for (unsigned i = 0; i < n; ++i) {
         dfsph_factor[i] *= dt;
-        kappa_v[i] *= dt;
+        kappa_v[i] *= g_plant == kPlantDivFinish ? dt * dt : dt;
       }
     }
     // clear_acceleration (runner.hpp:1974-1980)
@@ -1453,13 +1474,12 @@ struct Solver {
     if (set.enable_density_solve) {
       // warm_start_pressure_solve0 (runner.hpp:3430-3450)
       std::vector<T> k_new(n);
+      const T dens_clamp = static_cast<T>(g_plant == kPlantDensWarmClamp ? -0.0005 : -0.00025);
 #pragma omp parallel for schedule(static)
       for (long ii = 0; ii < long(n); ++ii) {
         unsigned i = unsigned(ii);
         T dadv = density_adv_of(i, dt);
-        k_new[i] = dadv > 1.0 ? (static_cast<T>(0.5) *
-                                 std::max(kappa[i], static_cast<T>(-0.00025)) /
-                                 (dt * dt))
+        k_new[i] = dadv > 1.0 ? (static_cast<T>(0.5) * std::max(kappa[i], dens_clamp) / (dt * dt))
                               : static_cast<T>(0);
       }
       for (unsigned i = 0; i < n; ++i) kappa[i] = k_new[i];
@@ -1508,7 +1528,7 @@ struct Solver {
       rep.num_density_solve = it;
       rep.mean_density_error = mean;
       // pressure_solve_finish (runner.hpp:3606-3611)
-      for (unsigned i = 0; i < n; ++i) kappa[i] *= dt * dt;
+      for (unsigned i = 0; i < n; ++i) kappa[i] *= g_plant == kPlantDensFinish ? dt : dt * dt;
     }
     integrate_velocity();
     rigid_sums_and_integrate();
@@ -1536,7 +1556,7 @@ struct Solver {
       v[i] += dt * a[i];
       last_pressure[i] = pressure[i] * static_cast<T>(0.5);
       T dens = density[i];
-      T dens2 = dens * dens;
+      T dens2 = g_plant == kPlantIiDiiDensity ? dens * c.rho0 : dens * dens;
       V3<T> d{0, 0, 0};
       for (unsigned s = 0; s < num_neighbors[i]; ++s)
         d -= (c.vol / dens2) *
@@ -1565,7 +1585,7 @@ struct Solver {
         V3<T> grad_w = displacement_cubic_kernel_grad(nb_x[o], c);
         V3<T> dji = (dpi * c.rho0) * grad_w;
         dadv += dt * c.vol * dot(vi - v[j], grad_w);
-        a_ii += c.vol * dot(d_ii - dji, grad_w);
+        a_ii += c.vol * dot(g_plant == kPlantIiAiiNoDji ? d_ii : d_ii - dji, grad_w);
       }
       for (unsigned b = 0; b < bnd.size(); ++b) {
         V4<T> r = brec(b, i), k = bker(b, i);
@@ -1611,7 +1631,7 @@ struct Solver {
         T dt2 = dt * dt;
         T a_ii = aii[i];
         T denom = a_ii * dt2;
-        const T w = static_cast<T>(2) / 3;
+        const T w = g_plant == kPlantIiOmega ? static_cast<T>(0.5) : static_cast<T>(2) / 3;
         T p = 0;
         T sum_tmp = 0;
         for (unsigned s = 0; s < num_neighbors[i]; ++s) {
@@ -1648,6 +1668,19 @@ struct Solver {
   // SolverIi::step and SolverI::step (solver_ii.hpp:208-241,
   // solver_i.hpp:289-327)
   void pressure_accels_and_integrate() {
+    compute_pressure_accels();
+    // kinematic_integration<wrap> (runner.hpp:2873-2887)
+    for (unsigned i = 0; i < n; ++i) {
+      V3<T> vv = v[i];
+      vv += pressure_accel[i] * dt;
+      x[i] = wrap_pos(x[i] + vv * dt);
+      v[i] = vv;
+    }
+    rigid_sums_and_integrate();
+    t += dt;
+  }
+
+  void compute_pressure_accels() {
     // compute_pressure_accels (runner.hpp:2774-2817)
 #pragma omp parallel for schedule(static)
     for (long ii = 0; ii < long(n); ++ii) {
@@ -1692,15 +1725,6 @@ struct Solver {
       }
       pressure_accel[i] = c.rho0 * ai;
     }
-    // kinematic_integration<wrap> (runner.hpp:2873-2887)
-    for (unsigned i = 0; i < n; ++i) {
-      V3<T> vv = v[i];
-      vv += pressure_accel[i] * dt;
-      x[i] = wrap_pos(x[i] + vv * dt);
-      v[i] = vv;
-    }
-    rigid_sums_and_integrate();
-    t += dt;
   }
 
   // ---- SolverI::step (solver_i.hpp:79-330), volume-map boundaries ------------
@@ -2741,6 +2765,41 @@ int orc_compute_density_change(void* h) {
 int orc_integrate_velocity(void* h) {
   return dispatch(h, [&](auto& s) {
     s.integrate_velocity();
+    return 0;
+  });
+}
+
+int orc_plant(int code) {
+  orc::g_plant = code;
+  return 0;
+}
+
+int orc_compute_dfsph_factor(void* h) {
+  return dispatch(h, [&](auto& s) {
+    s.compute_dfsph_factor();
+    return 0;
+  });
+}
+
+int orc_apply_stiffness(void* h, int divergence_form, const double* k) {
+  return dispatch(h, [&](auto& s) {
+    using T = typename std::remove_reference<decltype(s.dt)>::type;
+    std::vector<T> kk(s.n);
+    for (unsigned i = 0; i < s.n; ++i) kk[i] = static_cast<T>(k[i]);
+    std::vector<orc::V3<T>> dvs(s.n);
+#pragma omp parallel for schedule(static)
+    for (long ii = 0; ii < long(s.n); ++ii) {
+      unsigned i = unsigned(ii);
+      dvs[i] = s.kappa_dv(i, kk[i], s.dt, divergence_form != 0, [&](unsigned j) { return kk[j]; });
+    }
+    for (unsigned i = 0; i < s.n; ++i) s.v[i] += dvs[i];
+    return 0;
+  });
+}
+
+int orc_compute_pressure_accels(void* h) {
+  return dispatch(h, [&](auto& s) {
+    s.compute_pressure_accels();
     return 0;
   });
 }

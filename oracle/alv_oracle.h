@@ -173,6 +173,23 @@ int orc_get_neighbors(void* h, unsigned* ids, void* xij);
 int orc_compute_density_change(void* h);
 /* x += dt v, wrapped on periodic axes */
 int orc_integrate_velocity(void* h);
+/* Planted mistakes for the oracle's own tests: a non-zero code switches one
+ * deliberate error into the algorithm (see enum Plant in alv_oracle.cpp),
+ * process-wide, until orc_plant(0). */
+int orc_plant(int code);
+/* compute_dfsph_factor (runner.hpp:3124-3154) on the current lists and
+ * boundary records, into dfsph_factor */
+int orc_compute_dfsph_factor(void* h);
+/* The velocity change of one stiffness pass (the shared body of
+ * warm_start_divergence_solve_1 / divergence_solve_iteration
+ * (divergence_form != 0, runner.hpp:3252-3363) or warm_start_pressure_solve1
+ * / pressure_solve_iteration (runner.hpp:3452-3588)) for the stiffness field
+ * k[n] (FP64 values), applied to v with the current dt; boundary forces
+ * accumulate as in the step. */
+int orc_apply_stiffness(void* h, int divergence_form, const double* k);
+/* compute_pressure_accels (runner.hpp:2774-2817) from the current pressure,
+ * into pressure_accel, without integrating */
+int orc_compute_pressure_accels(void* h);
 int orc_get_grid(void* h, unsigned* cell_count, unsigned* cell_start,
                  unsigned* anomaly);
 int orc_step(void* h);

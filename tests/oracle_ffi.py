@@ -157,6 +157,10 @@ def lib():
         L.orc_sum_f32.argtypes = [vp, ctypes.c_size_t]
         L.orc_compute_density_change.argtypes = [vp]
         L.orc_integrate_velocity.argtypes = [vp]
+        L.orc_plant.argtypes = [ctypes.c_int]
+        L.orc_compute_dfsph_factor.argtypes = [vp]
+        L.orc_apply_stiffness.argtypes = [vp, ctypes.c_int, dp]
+        L.orc_compute_pressure_accels.argtypes = [vp]
         L.orc_mlp_num_params.restype = ctypes.c_int
         L.orc_mlp_forward.argtypes = [vp, ctypes.c_size_t, vp, ctypes.c_int, vp]
         L.orc_mlp_loss_grad.argtypes = [vp, ctypes.c_size_t, vp, vp, ctypes.c_int, dp, vp]
@@ -233,7 +237,9 @@ class Oracle:
         self.L.orc_set_num_particles(self.h, scene.n)
         self.n = scene.n
         s = OrcSettings()
-        s.dt = s.max_dt = s.min_dt = scene.dt
+        s.dt = scene.dt
+        s.max_dt = scene.dt if scene.max_dt is None else scene.max_dt
+        s.min_dt = scene.dt if scene.min_dt is None else scene.min_dt
         s.cfl = scene.cfl
         s.density_change_tolerance = scene.density_change_tolerance
         s.density_error_tolerance = scene.density_error_tolerance
@@ -458,6 +464,18 @@ class Oracle:
     def integrate_velocity(self):
         self.L.orc_integrate_velocity(self.h)
 
+    def compute_dfsph_factor(self):
+        self.L.orc_compute_dfsph_factor(self.h)
+
+    def apply_stiffness(self, k, divergence_form=True):
+        """One stiffness pass's velocity change for the field k [n]."""
+        kk = np.ascontiguousarray(k, np.float64)
+        assert kk.shape == (self.n,)
+        self.L.orc_apply_stiffness(self.h, int(divergence_form), _dptr(kk))
+
+    def compute_pressure_accels(self):
+        self.L.orc_compute_pressure_accels(self.h)
+
     def neighbors(self):
         ids = np.zeros((self.max_n, self.kmax), np.uint32)
         self.L.orc_get_neighbors(self.h, ids.ctypes.data, None)
@@ -555,6 +573,33 @@ class Oracle:
         r = OrcReport()
         self.L.orc_get_report(self.h, ctypes.byref(r))
         return {f: getattr(r, f) for f, _ in OrcReport._fields_}
+
+
+class planted:
+    """Context manager switching one planted mistake into the oracle (the
+    codes of enum Plant in oracle/alv_oracle.cpp), so that a test can show
+    that its check catches it."""
+    FACTOR_NO_GRAD_SUM = 1
+    FACTOR_BOUNDARY_SIGN = 2
+    DIV_WARM_CLAMP = 3
+    DENS_WARM_CLAMP = 4
+    DIV_FINISH = 5
+    DENS_FINISH = 6
+    II_OMEGA = 7
+    II_AII_NO_DJI = 8
+    II_DII_DENSITY = 9
+    CFL_RADIUS = 10
+
+    def __init__(self, code):
+        self.code = code
+
+    def __enter__(self):
+        lib().orc_plant(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_plant(0)
+        return False
 
 
 def peirce_constants():

@@ -1,0 +1,247 @@
+"""The workloads the benchmark runs, on the GPU against the oracle, one step
+at a time from a synchronised state:
+
+* the 1 M-particle dam break with four floating rigid bodies, scaled to
+  98,304 particles with 32^3 body maps over three steps, and at full size
+  (1,048,576 particles, 128^3 body maps) for one step;
+* eight scenes of the 256-scene batched draw (its largest and smallest
+  included, 64^3 maps) against one oracle per scene, and the whole 256-scene
+  batch by properties: every scene converged or at its iteration cap, and
+  sampled scenes bit-identical to the same scenes run alone;
+* the CFL-limited time step (Solver::update_dt, solver.hpp:153-169): dt,
+  cfl_dt and the velocity maxima, with static and with moving bodies;
+* a box-shell container (dg/box_shell_distance.hpp:36-59): its volume map
+  and a step.
+
+Fields within 1e-5 relative (max-norm), positions within 1e-6 m, iteration
+counts equal.  Body force and torque sums within 1e-5 of the sum of the
+per-particle magnitudes (the rounding bound of a sum); body positions and
+orientations within 1e-6, velocities within what that force error moves
+them in one step.
+"""
+import numpy as np
+import pytest
+
+import oracle_ffi as of
+from alluvion_amd import scenes
+from gpu_sync import REL, body_state, compare_fields, oracle_for, rel_err, sync
+
+pytestmark = pytest.mark.gpu
+
+
+def config2_scaled():
+    """The floating-body dam break on a 128 x 24 x 32 lattice (same fluid
+    width, four bodies at the same places in x, 32^3 body maps)."""
+    return scenes.dam_break_rigid(nx=128, ny=24, nz=32, body_resolution=32)
+
+
+def check_bodies(pile, orc, before, dt, tag):
+    """Force and torque sums and the integrated kinematics of every dynamic
+    body after a step; `before` holds the synchronised pre-step states."""
+    F = orc.get("force")
+    Tq = orc.get("torque")
+    moved = 0.0
+    for b in range(pile.get_size()):
+        if orc.scene.boundaries[b].mass == 0:
+            continue
+        m = orc.scene.boundaries[b].mass
+        o = orc.rigid(b)
+        g = body_state(pile, b)
+        fscale = np.abs(F[b]).sum(axis=0).max()
+        tscale = np.abs(Tq[b]).sum(axis=0).max()
+        gf = np.array(list(pile.get_force(b)))
+        gt = np.array(list(pile.get_torque(b)))
+        errs = (np.abs(gf - o["force"]).max() / max(fscale, 1e-30),
+                np.abs(gt - o["torque"]).max() / max(tscale, 1e-30))
+        print(tag, "body", b, "force %.3g torque %.3g (|F| %.3g of sum|f| %.3g)" %
+              (errs[0], errs[1], np.abs(o["force"]).max(), fscale), flush=True)
+        assert errs[0] < 1e-5 and errs[1] < 1e-5, (tag, b, errs)
+        assert np.abs(g["x"] - o["x"]).max() < 1e-6, (tag, b)
+        assert np.abs(g["q"] - o["q"]).max() < 1e-6, (tag, b)
+        dv_tol = 2e-5 * fscale / m * dt + 1e-6 * np.abs(o["v"]).max() + 1e-9
+        assert np.abs(g["v"] - o["v"]).max() < dv_tol, (tag, b)
+        moved = max(moved, np.abs(o["force"]).max())
+        assert np.all(np.isfinite(g["omega"]))
+    return moved
+
+
+def step_pair(dp, solver, pile, orc, kind="df"):
+    before = [body_state(pile, b) for b in range(pile.get_size())]
+    dt0 = float(solver.dt)
+    solver.step()
+    orc.step()
+    rep = orc.report()
+    assert solver.num_density_solve == rep["num_density_solve"]
+    if kind == "df":
+        assert solver.num_divergence_solve == rep["num_divergence_solve"]
+    return before, rep, dt0
+
+
+def test_config2_scaled_floating_bodies_per_step():
+    sc = config2_scaled()
+    dp, pile, solver, _, _ = scenes.build_solver(sc)
+    for _ in range(10):
+        solver.step()
+    orc = oracle_for(sc, dp, pile.volume_grids)
+    forces = []
+    for step in range(3):
+        sync(dp, solver, pile, orc)
+        before, rep, _ = step_pair(dp, solver, pile, orc)
+        compare_fields(dp, solver, orc, tag="config2 scaled step %d" % step)
+        forces.append(check_bodies(pile, orc, before, float(solver.dt), "step %d" % step))
+    # the bodies float: the fluid carries them (non-zero boundary forces)
+    assert min(forces) > 0.1
+
+
+def test_config2_full_size_one_step():
+    sc = scenes.dam_break_rigid()
+    assert sc.n == 1_048_576
+    dp, pile, solver, _, _ = scenes.build_solver(sc)
+    for _ in range(20):
+        solver.step()
+    orc = oracle_for(sc, dp, pile.volume_grids)
+    sync(dp, solver, pile, orc)
+    before, rep, _ = step_pair(dp, solver, pile, orc)
+    compare_fields(dp, solver, orc, tag="config2 1M")
+    assert check_bodies(pile, orc, before, float(solver.dt), "config2 1M") > 0.1
+
+
+def batch_sample():
+    """Eight of the 256 batched scenes: the largest, the smallest and the
+    first six others, with their 64^3 maps."""
+    draw = scenes.batched_dam_breaks()
+    counts = np.array([s.n for s in draw])
+    pick = [int(np.argmax(counts)), int(np.argmin(counts))]
+    pick += [k for k in range(len(draw)) if k not in pick][:6]
+    return [draw[k] for k in pick], counts
+
+
+def test_config4_batch_scenes_match_per_scene_oracles():
+    scs, counts = batch_sample()
+    assert max(s.n for s in scs) == counts.max() and min(s.n for s in scs) == counts.min()
+    dp, piles, batch, _, _ = scenes.build_batch(scs)
+    oracles = [oracle_for(sc, dp, piles[k].volume_grids) for k, sc in enumerate(scs)]
+    for step in range(3):
+        for k, orc in enumerate(oracles):
+            first, _ = batch.scene_range(k)
+            if step:
+                sync(dp, batch, None, orc, first=first, dt=batch.scene_report(k)["dt"])
+        batch.step()
+        for k, orc in enumerate(oracles):
+            orc.step()
+            rep, orep = batch.scene_report(k), orc.report()
+            assert rep["num_divergence_solve"] == orep["num_divergence_solve"], (step, k)
+            assert rep["num_density_solve"] == orep["num_density_solve"], (step, k)
+            first, _ = batch.scene_range(k)
+            compare_fields(dp, batch, orc, first=first,
+                           tag="batch scene %d (%d particles) step %d" % (k, scs[k].n, step))
+
+
+def test_config4_full_batch_properties():
+    """All 256 scenes (about 8.7 M particles) for five steps: every scene's
+    solves end converged or at their caps, and three scenes run alone give
+    bit-identical positions, velocities and densities."""
+    scs = scenes.batched_dam_breaks()
+    dp, piles, batch, _, _ = scenes.build_batch(scs)
+    tol = scs[0].density_error_tolerance
+    div_tol = scs[0].density_change_tolerance
+    counts = [s.n for s in scs]
+    sample = [0, int(np.argmax(counts)), len(scs) - 1]
+    alone = [scenes.build_solver(scenes.batched_dam_breaks()[k]) for k in sample]
+    for step in range(5):
+        batch.step()
+        for k in range(len(scs)):
+            r = batch.scene_report(k)
+            assert (r["mean_density_error"] <= tol or
+                    r["num_density_solve"] == scs[k].max_density_solve), (step, k)
+            assert (r["mean_density_change"] <= div_tol / r["dt"] or
+                    r["num_divergence_solve"] == scs[k].max_divergence_solve), (step, k)
+        for sdp, _, ssolver, _, _ in alone:
+            ssolver.step()
+    X = dp.coat(batch.particle_x).get()
+    V = dp.coat(batch.particle_v).get()
+    D = dp.coat(batch.particle_density).get()
+    for k, (sdp, _, ssolver, _, _) in zip(sample, alone):
+        first, count = batch.scene_range(k)
+        np.testing.assert_array_equal(X[first:first + count],
+                                      sdp.coat(ssolver.particle_x).get()[:count])
+        np.testing.assert_array_equal(V[first:first + count],
+                                      sdp.coat(ssolver.particle_v).get()[:count])
+        np.testing.assert_array_equal(D[first:first + count],
+                                      sdp.coat(ssolver.particle_density).get()[:count])
+
+
+def check_cfl(solver, rep, tag):
+    for key in ("particle_max_v2", "pile_max_v2", "max_v2", "cfl_dt", "utilized_cfl"):
+        g, o = float(getattr(solver, key)), rep[key]
+        print(tag, key, g, o, flush=True)
+        assert abs(g - o) <= 1e-5 * max(abs(o), 1e-12), (tag, key, g, o)
+    assert abs(float(solver.dt) - rep["dt"]) <= 1e-6 * rep["dt"], (tag, solver.dt, rep["dt"])
+
+
+def test_cfl_time_step_matches_oracle():
+    sc = scenes.adaptive_step(scenes.dam_break_small())
+    dp, pile, solver, _, _ = scenes.build_solver(sc)
+    for _ in range(60):
+        solver.step()
+    orc = oracle_for(sc, dp, pile.volume_grids)
+    dts = []
+    for step in range(2):
+        sync(dp, solver, pile, orc)
+        _, rep, dt0 = step_pair(dp, solver, pile, orc)
+        compare_fields(dp, solver, orc, tag="cfl step %d" % step)
+        check_cfl(solver, rep, "cfl step %d" % step)
+        dts.append(rep["dt"])
+    # the CFL bound, not the cap, sets the step here
+    assert max(dts) < sc.max_dt and rep["cfl_dt"] < sc.max_dt
+
+
+def test_cfl_time_step_with_moving_bodies():
+    sc = scenes.adaptive_step(config2_scaled())
+    dp, pile, solver, _, _ = scenes.build_solver(sc)
+    for _ in range(30):
+        solver.step()
+    orc = oracle_for(sc, dp, pile.volume_grids)
+    sync(dp, solver, pile, orc)
+    before, rep, _ = step_pair(dp, solver, pile, orc)
+    compare_fields(dp, solver, orc, tag="cfl bodies")
+    check_cfl(solver, rep, "cfl bodies")
+    assert rep["pile_max_v2"] > 0
+    check_bodies(pile, orc, before, rep["dt"], "cfl bodies")
+
+
+def box_shell_scene():
+    sc = scenes.dam_break_small()
+    sc.boundaries = [scenes.BoundarySpec("box_shell", (0.4, 0.4, 0.4, 0.02, 0.0), 1.0,
+                                         (64, 64, 64))]
+    return sc
+
+
+def test_box_shell_volume_map_matches_oracle():
+    sc = box_shell_scene()
+    sc.boundaries[0].resolution = (8, 8, 8)
+    dp, pile, solver, _, _ = scenes.build_solver(sc)
+    gpu = dp.coat(pile.volume_grids[0]).get()
+    orc = of.Oracle(sc, precision=4)
+    orc.build_volume_map(0)
+    ref = orc.volume_map(0)
+    scale = 4 * np.pi / 3 * sc.kernel_radius ** 3
+    assert gpu.shape == ref.shape
+    err = np.abs(gpu.astype(np.float64) - ref).max() / scale
+    print("box-shell volume map max error / ball volume: %.3g" % err)
+    assert err <= 1e-5, err
+    assert ref.max() > 0.5 * scale  # nodes inside the shell carry volume
+
+
+def test_box_shell_container_step_matches_oracle():
+    sc = box_shell_scene()
+    dp, pile, solver, _, _ = scenes.build_solver(sc)
+    for _ in range(5):
+        solver.step()
+    orc = oracle_for(sc, dp, pile.volume_grids)
+    for step in range(2):
+        sync(dp, solver, pile, orc)
+        step_pair(dp, solver, pile, orc)
+        compare_fields(dp, solver, orc, tag="box shell step %d" % step)
+    # particles near the walls see the shell
+    assert np.count_nonzero(orc.get("boundary_kernel")[0, :sc.n, 3]) > 1000

@@ -68,6 +68,7 @@ def test_field_op_and_field_gen_kats():
     from alluvion_amd import LIB_PATH
     L = ctypes.CDLL(LIB_PATH)
     fp = ctypes.POINTER(ctypes.c_float)
+    up = ctypes.POINTER(ctypes.c_uint)
     # field_op: node positions on the reference's 11^3 grid
     dmin = np.array([-23.1, 12.23, -31.44], np.float32)
     dmax = np.array([45.1, 103.2, 12.4], np.float32)
@@ -76,7 +77,9 @@ def test_field_op_and_field_gen_kats():
     nn = of.num_nodes(res)
     out = np.zeros((nn, 3), np.float32)
     n_out = ctypes.c_uint(0)
-    rc = L.alv_node_positions(dmin.ctypes.data_as(fp), res.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+    L.alv_node_positions.argtypes = [fp, up, fp, fp, up]
+    L.alv_node_positions.restype = ctypes.c_int
+    rc = L.alv_node_positions(dmin.ctypes.data_as(fp), res.ctypes.data_as(up),
                               cs.ctypes.data_as(fp), out.ctypes.data_as(fp), ctypes.byref(n_out))
     assert rc == 0 and n_out.value == nn
     want = np.array([of.node_position(dmin.astype(np.float64), res, cs.astype(np.float64), l)
@@ -102,14 +105,21 @@ def test_field_op_and_field_gen_kats():
             ("wrap_length", ctypes.c_float * 3)]
 
     cn = AlvConst()
-    L.alv_const_set_kernel_radius(ctypes.byref(cn), ctypes.c_float(r))
-    L.alv_const_set_particle_attr(ctypes.byref(cn), ctypes.c_float(0.25), ctypes.c_float(0.2),
-                                  ctypes.c_float(1.0))
+    assert ctypes.sizeof(AlvConst) == 23 * 4  # alv_const: 23 floats
+    cp = ctypes.POINTER(AlvConst)
+    L.alv_const_set_kernel_radius.argtypes = [cp, ctypes.c_float]
+    L.alv_const_set_kernel_radius.restype = None
+    L.alv_const_set_particle_attr.argtypes = [cp, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    L.alv_const_set_particle_attr.restype = None
+    L.alv_volume_from_distance_map.argtypes = [cp, fp, fp, up, fp, ctypes.c_float, fp, fp,
+                                               ctypes.c_uint]
+    L.alv_volume_from_distance_map.restype = ctypes.c_int
+    L.alv_const_set_kernel_radius(ctypes.byref(cn), r)
+    L.alv_const_set_particle_attr(ctypes.byref(cn), 0.25, 0.2, 1.0)
     vol = np.zeros(nn, np.float32)
     rc = L.alv_volume_from_distance_map(
-        ctypes.byref(cn), dmin.ctypes.data_as(fp), dmax.ctypes.data_as(fp),
-        res.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), cs.ctypes.data_as(fp),
-        ctypes.c_float(1.0), dist.ctypes.data_as(fp), vol.ctypes.data_as(fp), ctypes.c_uint(nn))
+        ctypes.byref(cn), dmin.ctypes.data_as(fp), dmax.ctypes.data_as(fp), res.ctypes.data_as(up),
+        cs.ctypes.data_as(fp), 1.0, dist.ctypes.data_as(fp), vol.ctypes.data_as(fp), nn)
     assert rc == 0
     ref = of.volume_from_distance_map(4, r, dmin, dmax, res, cs, 1.0, dist.astype(np.float64))
     scale = 4 * np.pi / 3 * r ** 3

@@ -6,6 +6,7 @@ import pytest
 
 import oracle_ffi as of
 from test_gpu_contacts import compare_contacts
+from gpu_sync import sync
 from test_oracle_pellets import small_scene
 
 pytestmark = pytest.mark.gpu
@@ -69,24 +70,26 @@ def test_isph_steps_with_pellets_match_oracle():
     dp, pile, solver, _, _ = scenes.build_solver(sc, kind="i")
     orc = of.Oracle(sc, precision=4, kind="i")
     n, mx, npel = sc.n, orc.max_n, pile.num_pellets
-    tol = [1e-5, 3e-5, 1e-4, 3e-4, 1e-3]
     for step in range(5):
+        if step:  # each step from the GPU's state, bodies and pellets included
+            sync(dp, solver, pile, orc, kind="i")
         solver.step()
         orc.step()
         x_g = dp.coat(solver.particle_x).get()
         v_g = dp.coat(solver.particle_v).get()
         p_g = dp.coat(solver.particle_pressure).get()
         assert rel(x_g[:n], orc.get("x")[:n]) < 1e-6, step
-        assert rel(v_g[:n], orc.get("v")[:n]) < tol[step], step
+        assert rel(v_g[:n], orc.get("v")[:n]) < 1e-5, step
         p_o = orc.get("pressure")
-        assert rel(p_g[:n], p_o[:n]) < tol[step] * 10, step
-        assert rel(p_g[mx:mx + npel], p_o[mx:mx + npel]) < tol[step] * 10, step
+        # the pressure is a Jacobi solution: one FP32 rounding per sweep
+        assert rel(p_g[:n], p_o[:n]) < 1e-4, step
+        assert rel(p_g[mx:mx + npel], p_o[mx:mx + npel]) < 1e-4, step
         assert solver.num_density_solve == orc.report()["num_density_solve"], step
         x, v, q, w = body_state(pile, 1)
         rb = orc.rigid(1)
         np.testing.assert_allclose(x, rb["x"], rtol=1e-6, atol=1e-8, err_msg=str(step))
-        np.testing.assert_allclose(v, rb["v"], rtol=tol[step] * 10, atol=1e-6, err_msg=str(step))
-        np.testing.assert_allclose(w, rb["omega"], rtol=tol[step] * 10, atol=1e-5, err_msg=str(step))
+        np.testing.assert_allclose(v, rb["v"], rtol=1e-4, atol=1e-6, err_msg=str(step))
+        np.testing.assert_allclose(w, rb["omega"], rtol=1e-4, atol=1e-5, err_msg=str(step))
     assert np.abs(orc.rigid(1)["v"]).max() > 0  # the fluid pushes the sphere
     assert p_o[mx:mx + npel].max() > 0  # pellets carry pressure
 
