@@ -17,6 +17,11 @@ CANDIDATES = ((False, False, 0), (True, False, 0), (True, True, 0), (True, True,
               (True, True, 2), (False, True, 1))
 
 
+def steps_taken(steps=5, candidates=CANDIDATES):
+    """Simulation steps that tune_step runs with these arguments."""
+    return len(candidates) * (steps + 1)
+
+
 def tune_step(solver, steps=5, candidates=CANDIDATES, log=None):
     """Returns (best (tiles, graph, slack), {(tiles, graph, slack): ms per step})."""
     timing = {}

@@ -364,6 +364,19 @@ struct alv_solver {
   unsigned last_div = 2, last_dens = 3;
   unsigned* export_ids = nullptr;
   unsigned* export_counts = nullptr;
+  // Reference-layout copies made on request (user order, owned here):
+  // particle_neighbors [cap][K] float4, particle_boundary /
+  // particle_boundary_kernel [B][cap] float4, particle_force /
+  // particle_torque [B][cap] float3, and the IISPH arrays.
+  float4* export_nbr = nullptr;
+  float4* export_brec = nullptr;
+  float4* export_bker = nullptr;
+  float* export_force = nullptr;
+  float* export_torque = nullptr;
+  float* export_ii_scalar[4] = {};  // last_pressure, aii, adv_density, density_err
+  float* export_ii_vec[3] = {};  // dii, dij_pj, pressure_accel
+  unsigned export_stale = ~0u;  // kX* bits of the copies older than the state
+  int record_static_forces = 0;  // accumulate particle forces of static boundaries too
   alv_mlp* net = nullptr;  // learned warm start (not owned)
   int net_div = 0, net_dens = 0;  // which warm starts use it
   int log_features = 0;
@@ -408,6 +421,13 @@ struct alv_solver {
     for (void* p : allocations) (void)hipFree(p);
     if (export_ids) (void)hipFree(export_ids);
     if (export_counts) (void)hipFree(export_counts);
+    for (void* p : {(void*)export_nbr, (void*)export_brec, (void*)export_bker,
+                    (void*)export_force, (void*)export_torque})
+      if (p) (void)hipFree(p);
+    for (float* p : export_ii_scalar)
+      if (p) (void)hipFree(p);
+    for (float* p : export_ii_vec)
+      if (p) (void)hipFree(p);
     if (bnd_dev) (void)hipFree(bnd_dev);
     if (ctl_host) (void)hipHostFree(ctl_host);
     if (bnd_host) (void)hipHostFree(bnd_host);
@@ -530,6 +550,7 @@ int alv_sum_f32(const float* d, size_t n, double* out) {
   *out = s;
   return ALV_OK;
 }
+
 int alv_max_f32(const float* d, size_t n, float* out) {
   if (!out) return fail(ALV_ERR_ARG, "out is null");
   const unsigned nb = 1024;
@@ -603,12 +624,19 @@ int alv_pile_destroy(alv_pile* pile) {
   return ALV_OK;
 }
 
-int alv_pile_add(alv_pile* pile, const alv_const* cn, const alv_distance* dist,
-                 const unsigned resolution[3], float cell_width, float margin, float sign,
-                 float mass, float restitution, float friction, const float inertia[3],
-                 const float x[3], const float q[4], unsigned* boundary_id) {
+}  // extern "C"
+
+namespace {
+// Pile::add and Pile::replace (pile.hpp:217-352, 353-492): builds boundary
+// `slot` (appended when slot < 0) with its map domain, resolution and, for
+// volume maps, its map on the device.
+int place_boundary(alv_pile* pile, int slot, const alv_const* cn, const alv_distance* dist,
+                   const unsigned resolution[3], float cell_width, float margin, float sign,
+                   float mass, float restitution, float friction, const float inertia[3],
+                   const float x[3], const float q[4], unsigned* boundary_id) {
   if (!pile || !cn || !dist) return fail(ALV_ERR_ARG, "null argument");
-  if (pile->bnd.size() >= 32) return fail(ALV_ERR_UNSUPPORTED, "at most 32 boundaries");
+  if (slot < 0 && pile->bnd.size() >= 32) return fail(ALV_ERR_UNSUPPORTED, "at most 32 boundaries");
+  if (slot >= 0 && size_t(slot) >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
   BoundaryDev B;
   memset(&B, 0, sizeof(B));
   B.type = dist->type;
@@ -691,6 +719,7 @@ int alv_pile_add(alv_pile* pile, const alv_const* cn, const alv_distance* dist,
   for (int a = 0; a < 4; ++a) B.q[a] = q ? q[a] : (a == 3 ? 1.f : 0.f);
   B.mass = mass;
   B.dynamic = mass != 0.f ? 1 : 0;
+  B.accum = B.dynamic;
   // build the volume map (update_volume_field, analytic variant)
   alv_consti cni;
   memset(&cni, 0, sizeof(cni));
@@ -709,18 +738,105 @@ int alv_pile_add(alv_pile* pile, const alv_const* cn, const alv_distance* dist,
     ALV_HIP(hipDeviceSynchronize());
   }
   B.cell_nz = flags;
-  pile->bnd.push_back(B);
-  pile->volumes.push_back(vol);
-  pile->cell_nz.push_back(flags);
-  pile->restitution.push_back(restitution);
-  pile->friction.push_back(friction);
   std::array<float, 3> I = {inertia ? inertia[0] : 1.f, inertia ? inertia[1] : 1.f,
                             inertia ? inertia[2] : 1.f};
-  pile->inertia.push_back(I);
-  pile->force.push_back({0.f, 0.f, 0.f});
-  pile->torque.push_back({0.f, 0.f, 0.f});
-  pile->accel.push_back({0.f, 0.f, 0.f});
-  if (boundary_id) *boundary_id = unsigned(pile->bnd.size() - 1);
+  if (slot < 0) {
+    pile->bnd.push_back(B);
+    pile->volumes.push_back(vol);
+    pile->cell_nz.push_back(flags);
+    pile->restitution.push_back(restitution);
+    pile->friction.push_back(friction);
+    pile->inertia.push_back(I);
+    pile->force.push_back({0.f, 0.f, 0.f});
+    pile->torque.push_back({0.f, 0.f, 0.f});
+    pile->accel.push_back({0.f, 0.f, 0.f});
+    slot = int(pile->bnd.size() - 1);
+  } else {
+    // the solvers read the boundary table at every step start, so the new
+    // map and kinematics take effect from the next step
+    ALV_HIP(hipDeviceSynchronize());
+    if (pile->volumes[slot]) (void)hipFree(pile->volumes[slot]);
+    if (pile->cell_nz[slot]) (void)hipFree(pile->cell_nz[slot]);
+    pile->bnd[slot] = B;
+    pile->volumes[slot] = vol;
+    pile->cell_nz[slot] = flags;
+    pile->restitution[slot] = restitution;
+    pile->friction[slot] = friction;
+    pile->inertia[slot] = I;
+    pile->force[slot] = {0.f, 0.f, 0.f};
+    pile->torque[slot] = {0.f, 0.f, 0.f};
+    pile->accel[slot] = {0.f, 0.f, 0.f};
+  }
+  if (boundary_id) *boundary_id = unsigned(slot);
+  return ALV_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int alv_pile_add(alv_pile* pile, const alv_const* cn, const alv_distance* dist,
+                 const unsigned resolution[3], float cell_width, float margin, float sign,
+                 float mass, float restitution, float friction, const float inertia[3],
+                 const float x[3], const float q[4], unsigned* boundary_id) {
+  return place_boundary(pile, -1, cn, dist, resolution, cell_width, margin, sign, mass,
+                        restitution, friction, inertia, x, q, boundary_id);
+}
+
+int alv_pile_replace(alv_pile* pile, unsigned b, const alv_const* cn, const alv_distance* dist,
+                     const unsigned resolution[3], float cell_width, float margin, float sign,
+                     float mass, float restitution, float friction, const float inertia[3],
+                     const float x[3], const float q[4]) {
+  return place_boundary(pile, int(b), cn, dist, resolution, cell_width, margin, sign, mass,
+                        restitution, friction, inertia, x, q, nullptr);
+}
+
+int alv_pile_set_body(alv_pile* pile, unsigned b, float mass, float restitution, float friction,
+                      const float inertia[3]) {
+  if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
+  pile->bnd[b].mass = mass;
+  pile->bnd[b].dynamic = mass != 0.f ? 1 : 0;
+  pile->bnd[b].accum = pile->bnd[b].dynamic;
+  pile->restitution[b] = restitution;
+  pile->friction[b] = friction;
+  if (inertia) pile->inertia[b] = {inertia[0], inertia[1], inertia[2]};
+  return ALV_OK;
+}
+
+int alv_pile_get_body(const alv_pile* pile, unsigned b, float* mass, float* restitution,
+                      float* friction, float inertia[3]) {
+  if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
+  if (mass) *mass = pile->bnd[b].mass;
+  if (restitution) *restitution = pile->restitution[b];
+  if (friction) *friction = pile->friction[b];
+  if (inertia)
+    for (int a = 0; a < 3; ++a) inertia[a] = pile->inertia[b][a];
+  return ALV_OK;
+}
+
+int alv_pile_compute_mask(alv_pile* pile, unsigned b, float distance_threshold,
+                          const float* sample_x, float* mask, unsigned num_samples) {
+  if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
+  if (num_samples && (!sample_x || !mask)) return fail(ALV_ERR_ARG, "null argument");
+  // the host kinematics of the body, as Pile::compute_mask passes x_(i), q_(i)
+  launch_boundary_mask(pile->bnd[b], sample_x, distance_threshold, mask, num_samples, nullptr);
+  ALV_HIP(hipGetLastError());
+  ALV_HIP(hipDeviceSynchronize());
+  return ALV_OK;
+}
+
+int alv_pile_calculate_cfl_v2(const alv_pile* pile, float* max_v2) {
+  if (!pile || !max_v2) return fail(ALV_ERR_ARG, "null argument");
+  // Pile::calculate_cfl_v2 (pile.hpp:816-825): |omega x (max_distance, 0, 0) + v|^2
+  float m = -FLT_MAX;
+  for (const BoundaryDev& B : pile->bnd) {
+    const float r[3] = {B.max_distance, 0.f, 0.f};
+    const float c[3] = {B.omega[1] * r[2] - B.omega[2] * r[1] + B.v[0],
+                        B.omega[2] * r[0] - B.omega[0] * r[2] + B.v[1],
+                        B.omega[0] * r[1] - B.omega[1] * r[0] + B.v[2]};
+    const float v2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    if (v2 > m) m = v2;
+  }
+  *max_v2 = m;
   return ALV_OK;
 }
 
@@ -1614,6 +1730,10 @@ int upload_step_inputs(alv_solver* S) {
     if (sc.bnd_base + b > S->bnd_capacity)
       return fail(ALV_ERR_STATE, "boundaries were added after the solver was created");
     if (b) memcpy(S->bnd_host + sc.bnd_base, sc.pile->bnd.data(), b * sizeof(BoundaryDev));
+    for (size_t k = 0; k < b; ++k) {
+      BoundaryDev& B = S->bnd_host[sc.bnd_base + k];
+      B.accum = B.dynamic || S->record_static_forces;
+    }
     nb = std::max<size_t>(nb, sc.bnd_base + b);
   }
   if (nb)
@@ -1734,6 +1854,7 @@ int finish_step(alv_solver* S, const Launch& L) {
   if (rc != ALV_OK) return rc;
   if (g_event_profiler) g_event_profiler->collect();
   S->dirty = ALV_F_DENSITY | ALV_F_ALPHA | ALV_F_RHO_ADV | ALV_F_A | ALV_F_NNB;
+  S->export_stale = ~0u;
   if (S->batch) {
     // every scene keeps its own step, time and report; the solver's report
     // is the maximum iteration counts over the scenes
@@ -1868,9 +1989,95 @@ int finish_loop(alv_solver* S, bool divergence, unsigned issued, unsigned max_it
   }
   return ALV_OK;
 }
+
+// Staleness bits of the reference-layout copies (alv_solver::export_stale).
+enum : unsigned { kXNbr = 1u, kXBnd = 2u, kXIi = 4u };
+
+// Reference-layout arrays made on request (ALV_FIELD_NEIGHBORS ..
+// ALV_FIELD_PRESSURE_ACCEL): allocated on first use, rebuilt when the state
+// moved on since the last copy.
+int export_field(alv_solver* S, int field, void** dptr, size_t* n_elements,
+                 unsigned* n_components) {
+  const size_t cap = S->cap;
+  const size_t K = S->cni.max_num_neighbors_per_particle;
+  size_t nbnd = 1;
+  for (const auto& sc : S->scenes) nbnd = std::max<size_t>(nbnd, sc.pile->bnd.size());
+  const bool ii_field = field >= ALV_FIELD_LAST_PRESSURE;
+  if (ii_field && S->kind != ALV_SOLVER_II)
+    return fail(ALV_ERR_ARG, "the IISPH arrays belong to SolverIi");
+  unsigned bit = field == ALV_FIELD_NEIGHBORS ? kXNbr : ii_field ? kXIi : kXBnd;
+  if (bit == kXNbr && !S->export_nbr) ALV_HIP(hipMalloc(&S->export_nbr, cap * K * sizeof(float4)));
+  if (bit == kXBnd && !S->export_brec) {
+    ALV_HIP(hipMalloc(&S->export_brec, nbnd * cap * sizeof(float4)));
+    ALV_HIP(hipMalloc(&S->export_bker, nbnd * cap * sizeof(float4)));
+    ALV_HIP(hipMalloc(&S->export_force, nbnd * cap * 3 * sizeof(float)));
+    ALV_HIP(hipMalloc(&S->export_torque, nbnd * cap * 3 * sizeof(float)));
+  }
+  if (bit == kXIi && !S->export_ii_scalar[0]) {
+    for (float*& p : S->export_ii_scalar) ALV_HIP(hipMalloc(&p, cap * sizeof(float)));
+    for (float*& p : S->export_ii_vec) ALV_HIP(hipMalloc(&p, cap * 3 * sizeof(float)));
+  }
+  if (S->export_stale & bit) {
+    // rows of unused slots stay zero
+    if (bit == kXNbr) ALV_HIP(hipMemsetAsync(S->export_nbr, 0, cap * K * sizeof(float4), S->stream));
+    if (bit == kXBnd) {
+      ALV_HIP(hipMemsetAsync(S->export_brec, 0, nbnd * cap * sizeof(float4), S->stream));
+      ALV_HIP(hipMemsetAsync(S->export_bker, 0, nbnd * cap * sizeof(float4), S->stream));
+      ALV_HIP(hipMemsetAsync(S->export_force, 0, nbnd * cap * 3 * sizeof(float), S->stream));
+      ALV_HIP(hipMemsetAsync(S->export_torque, 0, nbnd * cap * 3 * sizeof(float), S->stream));
+    }
+    if (bit == kXIi) {
+      for (float* p : S->export_ii_scalar)
+        ALV_HIP(hipMemsetAsync(p, 0, cap * sizeof(float), S->stream));
+      for (float* p : S->export_ii_vec)
+        ALV_HIP(hipMemsetAsync(p, 0, cap * 3 * sizeof(float), S->stream));
+    }
+    Launch L;
+    int rc = solver_launch(S, &L);
+    if (rc != ALV_OK) return rc;
+    if (L.active && S->grid_valid) {
+      if (bit == kXNbr) launch_export_neighbor_rows(L, S->A, S->export_nbr, S->stream);
+      if (bit == kXBnd)
+        launch_export_boundary(L, S->A, S->export_brec, S->export_bker, S->export_force,
+                               S->export_torque, S->stream);
+      if (bit == kXIi) launch_export_iisph(L, S->A, S->export_ii_scalar, S->export_ii_vec, S->stream);
+      ALV_HIP(hipGetLastError());
+    }
+    ALV_HIP(hipStreamSynchronize(S->stream));
+    S->export_stale &= ~bit;
+  }
+  void* ptr = nullptr;
+  size_t n = cap;
+  unsigned comp = 1;
+  switch (field) {
+    case ALV_FIELD_NEIGHBORS: ptr = S->export_nbr; n = cap * K; comp = 4; break;
+    case ALV_FIELD_BOUNDARY: ptr = S->export_brec; n = nbnd * cap; comp = 4; break;
+    case ALV_FIELD_BOUNDARY_KERNEL: ptr = S->export_bker; n = nbnd * cap; comp = 4; break;
+    case ALV_FIELD_FORCE: ptr = S->export_force; n = nbnd * cap; comp = 3; break;
+    case ALV_FIELD_TORQUE: ptr = S->export_torque; n = nbnd * cap; comp = 3; break;
+    case ALV_FIELD_LAST_PRESSURE: ptr = S->export_ii_scalar[0]; break;
+    case ALV_FIELD_AII: ptr = S->export_ii_scalar[1]; break;
+    case ALV_FIELD_ADV_DENSITY: ptr = S->export_ii_scalar[2]; break;
+    case ALV_FIELD_DENSITY_ERR: ptr = S->export_ii_scalar[3]; break;
+    case ALV_FIELD_DII: ptr = S->export_ii_vec[0]; comp = 3; break;
+    case ALV_FIELD_DIJ_PJ: ptr = S->export_ii_vec[1]; comp = 3; break;
+    case ALV_FIELD_PRESSURE_ACCEL: ptr = S->export_ii_vec[2]; comp = 3; break;
+    default: return fail(ALV_ERR_ARG, "unknown field");
+  }
+  *dptr = ptr;
+  if (n_elements) *n_elements = n;
+  if (n_components) *n_components = comp;
+  return ALV_OK;
+}
 }  // namespace
 
 extern "C" {
+
+int alv_solver_set_static_boundary_forces(alv_solver* S, int enable) {
+  if (!S) return fail(ALV_ERR_ARG, "null solver");
+  S->record_static_forces = enable ? 1 : 0;
+  return ALV_OK;
+}
 
 int alv_solver_step(alv_solver* S) {
   if (!S) return fail(ALV_ERR_ARG, "null solver");
@@ -2066,6 +2273,7 @@ int alv_solver_update_particle_neighbors(alv_solver* S) {
   ALV_HIP(hipGetLastError());
   ALV_HIP(hipStreamSynchronize(S->stream));
   S->dirty |= ALV_F_DENSITY | ALV_F_NNB | ALV_F_ALPHA;
+  S->export_stale = ~0u;
   return ALV_OK;
 }
 
@@ -2154,6 +2362,19 @@ int alv_solver_get_field(alv_solver* S, int field, void** dptr, size_t* n_elemen
     case ALV_FIELD_GRID_ANOMALY: ptr = A.anomaly; n = 3; break;
     case ALV_FIELD_CELL_START: ptr = A.cell_start; n = size_t(S->num_cells) + 1; break;
     case ALV_FIELD_BOUNDARY_MASK: ptr = A.bmask; n = S->set.num_particles; break;
+    case ALV_FIELD_NEIGHBORS:
+    case ALV_FIELD_BOUNDARY:
+    case ALV_FIELD_BOUNDARY_KERNEL:
+    case ALV_FIELD_FORCE:
+    case ALV_FIELD_TORQUE:
+    case ALV_FIELD_LAST_PRESSURE:
+    case ALV_FIELD_AII:
+    case ALV_FIELD_ADV_DENSITY:
+    case ALV_FIELD_DENSITY_ERR:
+    case ALV_FIELD_DII:
+    case ALV_FIELD_DIJ_PJ:
+    case ALV_FIELD_PRESSURE_ACCEL:
+      return export_field(S, field, dptr, n_elements, n_components);
     default: return fail(ALV_ERR_ARG, "unknown field");
   }
   if (need & S->dirty) {

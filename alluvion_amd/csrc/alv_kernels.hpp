@@ -257,6 +257,9 @@ void launch_sample_velocity(const SimParams& p, const float4* nbr, const unsigne
 void launch_histogram256(const float* v, const float* mask, unsigned n, float v_min,
                          float step_inverse, unsigned* partial, unsigned* hist, hipStream_t s);
 unsigned histogram_partial_count();
+// compute_boundary_mask (analytic): mask[i] *= (sign d(local x_i) >= threshold)
+void launch_boundary_mask(const BoundaryDev& B, const float* x, float threshold, float* mask,
+                          unsigned n, hipStream_t s);
 void launch_kl_divergence(const unsigned* hp, const unsigned* hq, unsigned n_p, unsigned n_q,
                           float q_lower_bound, unsigned bins, float* out, hipStream_t s);
 
@@ -292,6 +295,18 @@ void launch_unsort_fields(const Launch& L, const DevArrays& A, unsigned field_ma
 // reference-layout neighbour ids (user order) for inspection: ids[i*kmax+t]
 void launch_export_neighbors(const Launch& L, const DevArrays& A, unsigned* ids,
                              unsigned* counts, hipStream_t s);
+// reference-layout neighbour rows (x_i - x_j, bits of j), [cap][kmax] float4
+void launch_export_neighbor_rows(const Launch& L, const DevArrays& A, float4* out,
+                                 hipStream_t s);
+// boundary records, boundary kernels ([B][cap] float4) and per-particle
+// boundary forces and torques ([B][cap] float3) in user order; null outputs
+// are skipped
+void launch_export_boundary(const Launch& L, const DevArrays& A, float4* rec, float4* ker,
+                            float* force, float* torque, hipStream_t s);
+// IISPH arrays in user order: scalar = last_pressure, aii, adv_density,
+// density_err; vec = dii, dij_pj, pressure_accel (float3); nulls skipped
+void launch_export_iisph(const Launch& L, const DevArrays& A, float* const scalar[4],
+                         float* const vec[3], hipStream_t s);
 // reset the control block for a fresh solve
 void launch_ctl_reset(const Launch& L, const DevArrays& A, hipStream_t s);
 

@@ -480,6 +480,27 @@ void launch_histogram256(const float* v, const float* mask, unsigned n, float v_
   hipLaunchKernelGGL(k_histogram_merge, dim3(1), dim3(256), 0, s, partial, kHistBlocks, hist);
 }
 unsigned histogram_partial_count() { return kHistBlocks * 256u; }
+
+// compute_boundary_mask, analytic variant (runner.hpp:4385-4397): mask[i] is
+// multiplied by (sign * d(R(q)^T (x_i - x_b)) >= threshold), d the
+// boundary's signed distance.
+__global__ __launch_bounds__(256) void k_boundary_mask(BoundaryDev B, const float* __restrict__ x,
+                                                      float threshold, float* __restrict__ mask,
+                                                      unsigned n) {
+  const unsigned i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = make_float4(B.q[0], B.q[1], B.q[2], B.q[3]);
+  const vec3 xi = mk(x[3 * size_t(i)], x[3 * size_t(i) + 1], x[3 * size_t(i) + 2]);
+  const vec3 local = rotate(xi - mk(B.x[0], B.x[1], B.x[2]), conjugate(q));
+  const float d = sdf_eval(B, local) * B.sign;
+  mask[i] *= (d >= threshold) ? 1.0f : 0.0f;
+}
+void launch_boundary_mask(const BoundaryDev& B, const float* x, float threshold, float* mask,
+                          unsigned n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_boundary_mask, dim3((n + 255u) / 256u), dim3(256), 0, s, B, x, threshold,
+                     mask, n);
+}
 void launch_kl_divergence(const unsigned* hp, const unsigned* hq, unsigned n_p, unsigned n_q,
                           float q_lower_bound, unsigned bins, float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_kl_divergence, dim3(1), dim3(256), 0, s, hp, hq, n_p, n_q, q_lower_bound,

@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SceneTab T, DevArrays A
   if (p.pellet_mode) {
     // no boundary records: the pellet passes add the body forces
     for (unsigned b = 0; b < p.num_boundaries; ++b) {
-      if (!A.bnd[b].dynamic) continue;
+      if (!A.bnd[b].accum) continue;
       const size_t o = size_t(b) * A.cap + s;
       A.bforce[o] = make_float4(0.f, 0.f, 0.f, 0.f);
       A.btorque[o] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SceneTab T, DevArrays A
       A.brec[o] = rec;
       A.bker[o] = ker;
       mask |= 1u << b;
-      if (B.dynamic) {
+      if (B.accum) {
         A.bforce[o] = make_float4(0.f, 0.f, 0.f, 0.f);
         A.btorque[o] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
@@ -747,7 +747,7 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
       vec3 acc = k_i * (-mk(k.x, k.y, k.z));
       dv -= dt * acc;
       const BoundaryDev& B = A.bnd[b];
-      if (B.dynamic) {
+      if (B.accum) {
         vec3 f = p.mass * acc;
         vec3 tq = cross(ld3(A.brec[o]) - mk(B.x[0], B.x[1], B.x[2]), f);
         float4 F = A.bforce[o], T = A.btorque[o];
@@ -790,37 +790,70 @@ __global__ __launch_bounds__(256) void k_ws_div0(SceneTab T, DevArrays A) {
   }
 }
 
-// Learned warm start: the network's prediction of the converged stored
-// stiffness replaces 0.5 max(kappa_v, -0.5) in warm_start_divergence_solve_0
-// (runner.hpp:3230-3250) and is kept for warm_start_pressure_solve0.
-// Converged stiffnesses are never positive (k = rho_dot alpha and
-// (rho* - 1) alpha with rho_dot >= 0, rho* >= 1, alpha <= 0), so predictions
-// are clamped to [bound, 0] with the bounds of the reference's own warm-start
-// clamps (-0.5 and -2.5e-4 in stored form).  A non-finite prediction falls
-// back to the reference's warm start.  Every lane of the wave runs the
-// matrix-core inference; there is no early return.
-__global__ __launch_bounds__(256) void k_ws_learned(SceneTab T, DevArrays A, MlpDev M,
-                                                    int use_divergence) {
+// warm_start_divergence_solve_0 (runner.hpp:3230-3250) with the learned
+// warm start fused in: the density change and the eight features are
+// computed as in k_ws_div0, then the whole wave runs the network on its 64
+// particles (matrix cores; the forward weight operands are staged in LDS once
+// per block and shared by its four waves).  The prediction of the converged
+// stored stiffness replaces 0.5 max(kappa_v, -0.5) and the density one is
+// kept for warm_start_pressure_solve0.  Converged stiffnesses are never
+// positive (k = rho_dot alpha and (rho* - 1) alpha with rho_dot >= 0,
+// rho* >= 1, alpha <= 0), so predictions are clamped to [bound, 0] with the
+// bounds of the reference's own warm-start clamps (-0.5 and -2.5e-4 in
+// stored form); a non-finite prediction falls back to the reference's warm
+// start.  F: also log the features for training.
+template <bool W, bool F>
+__global__ __launch_bounds__(256) void k_ws_div0_learned(SceneTab T, DevArrays A, MlpDev M,
+                                                         int use_divergence) {
+  constexpr int kOps = kOpsT4;  // forward operands F1 .. F4
+  __shared__ bf16x8 ops_lds[kOps];
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(M.ops);
+    uint4* dst = reinterpret_cast<uint4*>(ops_lds);
+    for (int k = threadIdx.x; k < kOps; k += kBlock) dst[k] = src[k];
+  }
+  __syncthreads();
   ALV_SCENE(T, A, s);
-  const bool ok = s < p.first + p.n;
-  float f[kMlpIn];
-  const float4 f0 = ok ? A.feat[2 * size_t(s)] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const float4 f1 = ok ? A.feat[2 * size_t(s) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
-  f[0] = f0.x; f[1] = f0.y; f[2] = f0.z; f[3] = f0.w;
-  f[4] = f1.x; f[5] = f1.y; f[6] = f1.z; f[7] = f1.w;
-  float y0, y1;
-  mlp_infer_wave(M, f, y0, y1);
-  if (!ok) return;
+  const bool live = s < p.first + p.n;
   const float dt = A.ctl->dt;
-  if (use_divergence && f[1] > 0.0f) {
-    const float kv = std_min(std_max(y0, -0.5f), 0.0f) / dt;
-    if (isfinite(y0)) {
-      A.kappa_v[s] = kv;
-      set_kw(A, s, kv);
+  float f[kMlpIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float dadv = 0.0f, kv = 0.0f;
+  if (live) {
+    dadv = density_change<W>(p, A, s, GlobalXV{A.x, A.v});
+    const float kv_prev = A.kappa_v[s];
+    kv = (dadv > 0.0f) ? 0.5f * std_max(kv_prev, -0.5f) / dt : 0.0f;
+    float psi = 0.0f;
+    unsigned mask = A.bmask[s];
+    while (mask) {
+      const unsigned b = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      psi += A.brec[size_t(b) * A.cap + s].w;
+    }
+    const vec3 v = ld3(A.v[s]);
+    f[0] = A.density[s] / p.rho0 - 1.0f;
+    f[1] = dadv;
+    f[2] = float(A.nnb[s]);
+    f[3] = psi;
+    f[4] = kv_prev;
+    f[5] = A.kappa[s];
+    f[6] = sqrtf(length_sqr(v));
+    f[7] = dt;
+    if (F) {
+      A.feat[2 * size_t(s)] = make_float4(f[0], f[1], f[2], f[3]);
+      A.feat[2 * size_t(s) + 1] = make_float4(f[4], f[5], f[6], f[7]);
     }
   }
+  MlpDev Ml = M;
+  Ml.ops = ops_lds;
+  float y0, y1;
+  mlp_infer_wave(Ml, f, y0, y1);  // every lane of the wave, live or not
+  if (!live) return;
+  if (use_divergence && dadv > 0.0f && isfinite(y0)) kv = std_min(std_max(y0, -0.5f), 0.0f) / dt;
+  A.kappa_v[s] = kv;
+  set_kw(A, s, kv);
   A.kpred[s] = isfinite(y1) ? std_min(std_max(y1, -0.00025f), 0.0f) : y1;
 }
+
 // warm_start_divergence_solve_1 (runner.hpp:3252-3295)
 template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_ws_div1(SceneTab T, DevArrays A) {
@@ -1012,7 +1045,7 @@ __global__ __launch_bounds__(256) void k_viscosity(SceneTab T, DevArrays A, int 
           as[q] = (p.bvisc * vol * dot(xix, gw) / (length_sqr(xix) + 0.01f * p.h2)) * (vi - vq);
         }
         da += as[0] + as[1] + as[2] + as[3];
-        if (B.dynamic) {
+        if (B.accum) {
           vec3 f[4], tq[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
@@ -1077,7 +1110,7 @@ __global__ __launch_bounds__(256) void k_integrate_np(SceneTab T, DevArrays A) {
 }
 
 // warm_start_pressure_solve0 (runner.hpp:3430-3450)
-// L: learned warm start (prediction from k_ws_learned)
+// L: learned warm start (prediction from k_ws_div0_learned)
 template <bool W, bool L, bool TL>
 __global__ __launch_bounds__(256) void k_ws_p0(SceneTab T, DevArrays A) {
   ALV_SCENE(T, A, s);
@@ -1494,7 +1527,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure_accel(SceneTab T, DevArrays
       ai -= acc;
       const unsigned b = A.pbody[sp];
       const BoundaryDev& B = A.bnd[b];
-      if (B.dynamic) {
+      if (B.accum) {
         const size_t o = size_t(b) * A.cap + s;
         const vec3 f = p.mass * acc;
         const vec3 tq = cross((xi - xixj) - mk(B.x[0], B.x[1], B.x[2]), f);
@@ -1513,7 +1546,7 @@ __global__ __launch_bounds__(256) void k_ii_pressure_accel(SceneTab T, DevArrays
     vec3 acc = (p.rho0 * dpi) * mk(k.x, k.y, k.z);
     ai -= acc;
     const BoundaryDev& B = A.bnd[b];
-    if (B.dynamic) {
+    if (B.accum) {
       vec3 f = p.mass * acc;
       vec3 tq = cross(ld3(A.brec[o]) - mk(B.x[0], B.x[1], B.x[2]), f);
       float4 F = A.bforce[o], T = A.btorque[o];
@@ -1765,6 +1798,101 @@ __global__ __launch_bounds__(256) void k_export_neighbors(SceneTab T, DevArrays 
   }
 }
 
+// The neighbour list in the reference's layout (make_neighbor_list,
+// runner.hpp:1896-1899): row i (user order) holds (x_i - x_j, bits of j)
+// per neighbour, from the positions of the last neighbour search (the
+// sorted positions keep them until the next search); slots past the count
+// are zero.
+template <bool W>
+__global__ __launch_bounds__(256) void k_export_neighbor_rows(SceneTab T, DevArrays A,
+                                                              float4* __restrict__ out) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
+  const unsigned i = A.sorted_id[s];
+  const unsigned nn = A.nnb[s];
+  const vec3 xi = ld3(A.x[s]);
+  float4* row = out + size_t(i) * p.kmax;
+  for (unsigned t = 0; t < p.kmax; ++t) {
+    if (t >= nn) {
+      row[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
+    }
+    const uint4 g = A.nbr[nbr_group(A, t >> 2, s)];
+    const unsigned j = (t & 3u) == 0 ? g.x : (t & 3u) == 1 ? g.y : (t & 3u) == 2 ? g.z : g.w;
+    const vec3 d = wrapd<W>(xi - ld3(A.x[j]), p);
+    row[t] = make_float4(d.x, d.y, d.z, __uint_as_float(A.sorted_id[j]));
+  }
+}
+
+// Boundary records and per-particle boundary forces in the reference's
+// [B][max_num_particles] layout and user order (particle_boundary,
+// particle_boundary_kernel: float4; particle_force, particle_torque:
+// float3).  The step skips the pairs whose volume is 0; their records are
+// computed here in full (closest boundary point, volume 0, kernel terms 0)
+// at the positions and boundary kinematics of the last neighbour search, as
+// the reference's boundary pass writes them for every pair
+// (compute_particle_boundary_analytic, runner.hpp:1982-2002).  Forces of
+// boundaries that accumulate none are zero.  Null outputs are skipped.
+__global__ __launch_bounds__(256) void k_export_boundary(SceneTab T, DevArrays A,
+                                                         float4* __restrict__ rec,
+                                                         float4* __restrict__ ker,
+                                                         float* __restrict__ force,
+                                                         float* __restrict__ torque) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
+  const unsigned i = A.sorted_id[s];
+  const unsigned mask = p.pellet_mode ? 0u : A.bmask[s];
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (unsigned b = 0; b < p.num_boundaries; ++b) {
+    const size_t o = size_t(b) * A.cap + s;
+    const size_t u = size_t(b) * A.cap + i;
+    const bool seen = (mask >> b) & 1u;
+    if (rec || ker) {
+      float4 r = zero, k = zero;
+      if (seen) {
+        r = A.brec[o];
+        k = A.bker[o];
+      } else if (!p.pellet_mode) {
+        boundary_record<false>(A.bnd[b], ld3(A.x[s]), p, r, k);
+      }
+      if (rec) rec[u] = r;
+      if (ker) ker[u] = k;
+    }
+    const bool has_force = A.bnd[b].accum && (seen || p.pellet_mode);
+    if (force) {
+      const float4 F = has_force ? A.bforce[o] : zero;
+      force[3 * u] = F.x;
+      force[3 * u + 1] = F.y;
+      force[3 * u + 2] = F.z;
+    }
+    if (torque) {
+      const float4 Q = has_force ? A.btorque[o] : zero;
+      torque[3 * u] = Q.x;
+      torque[3 * u + 1] = Q.y;
+      torque[3 * u + 2] = Q.z;
+    }
+  }
+}
+
+// IISPH per-particle arrays in user order (SolverIi's particle_last_pressure,
+// particle_aii, particle_adv_density, particle_density_err; particle_dii,
+// particle_dij_pj, particle_pressure_accel as float3).  Null outputs are
+// skipped.
+__global__ __launch_bounds__(256) void k_export_iisph(SceneTab T, DevArrays A, float* last_p,
+                                                      float* aii, float* adv, float* err,
+                                                      float* dii, float* dij, float* pacc) {
+  ALV_SCENE(T, A, s);
+  if (s >= p.first + p.n) return;
+  const unsigned i = A.sorted_id[s];
+  if (last_p) last_p[i] = A.last_pressure[s];
+  if (aii) aii[i] = A.aii[s];
+  if (adv) adv[i] = A.adv_density[s];
+  if (err) err[i] = A.density_err[s];
+  if (dii) st3(dii, i, ld3(A.dii[s]));
+  if (dij) st3(dij, i, ld3(A.dij_pj[s]));
+  if (pacc) st3(pacc, i, ld3(A.pacc[s]));
+}
+
 // ---------------------------------------------------------------------------
 // Volume maps (update_volume_field, runner.hpp:1616-1694).  Peirce constants
 // are packed as cosphi[30], sinphi[30], b[30], r[15], c[15].
@@ -1964,7 +2092,7 @@ __global__ __launch_bounds__(256) void k_viscosity_pellets(SceneTab T, DevArrays
       da += acc;
       const unsigned b = A.pbody[sp];
       const BoundaryDev& B = A.bnd[b];
-      if (B.dynamic) {
+      if (B.accum) {
         const size_t o = size_t(b) * A.cap + s;
         const vec3 f = (-p.mass) * acc;
         const vec3 tq = cross((xi - xixj) - mk(B.x[0], B.x[1], B.x[2]), f);
@@ -2198,11 +2326,14 @@ void launch_neighbors(const Launch& L, const DevArrays& A, int with_factor, hipS
 }
 void launch_df_warm_start_div(const Launch& L, const DevArrays& A, hipStream_t s,
                               int features, const MlpDev* net, int net_divergence) {
-  if (features || net)
+  if (net && features)
+    ALV_LAUNCH_W2(k_ws_div0_learned, true, L, s, A, *net, net_divergence);
+  else if (net)
+    ALV_LAUNCH_W2(k_ws_div0_learned, false, L, s, A, *net, net_divergence);
+  else if (features)
     ALV_LAUNCH_W2T(k_ws_div0, true, L, s, A);
   else
     ALV_LAUNCH_W2T(k_ws_div0, false, L, s, A);
-  if (net) ALV_LAUNCH(k_ws_learned, L, s, A, *net, net_divergence);
   ALV_LAUNCH_WT(k_ws_div1, L, s, A);
   ALV_LAUNCH_WT(k_vel_density_change, L, s, A);
 }
@@ -2287,6 +2418,19 @@ void launch_unsort_fields(const Launch& L, const DevArrays& A, unsigned mask, hi
 void launch_export_neighbors(const Launch& L, const DevArrays& A, unsigned* ids,
                              unsigned* counts, hipStream_t s) {
   ALV_LAUNCH(k_export_neighbors, L, s, A, ids, counts);
+}
+void launch_export_neighbor_rows(const Launch& L, const DevArrays& A, float4* out,
+                                 hipStream_t s) {
+  ALV_LAUNCH_W(k_export_neighbor_rows, L, s, A, out);
+}
+void launch_export_boundary(const Launch& L, const DevArrays& A, float4* rec, float4* ker,
+                            float* force, float* torque, hipStream_t s) {
+  ALV_LAUNCH(k_export_boundary, L, s, A, rec, ker, force, torque);
+}
+void launch_export_iisph(const Launch& L, const DevArrays& A, float* const scalar[4],
+                         float* const vec[3], hipStream_t s) {
+  ALV_LAUNCH(k_export_iisph, L, s, A, scalar[0], scalar[1], scalar[2], scalar[3], vec[0],
+             vec[1], vec[2]);
 }
 void launch_ctl_reset(const Launch& L, const DevArrays& A, hipStream_t s) {
   hipLaunchKernelGGL(k_ctl_reset, dim3(L.scenes), dim3(1), 0, s, A);

@@ -59,7 +59,10 @@ struct BoundaryDev {
   float x[3], v[3], omega[3];
   float q[4];  // (x, y, z, w)
   float mass;
-  int dynamic;  // mass != 0: forces are accumulated for it
+  int dynamic;  // mass != 0: the body moves and its force sums are reduced
+  // per-particle forces and torques are accumulated for it: dynamic bodies,
+  // and every boundary when the solver records static boundaries' forces
+  int accum;
 };
 
 // Device-resident solver control block: time step, convergence bookkeeping

@@ -91,14 +91,6 @@ py::class_<T> vec3_class(py::module& m, const char* name) {
       });
 }
 
-// Indexable view of one kinematic quantity of a pile (PinnedVariable x_, v_,
-// q_, omega_ in the reference, pile.hpp:103-110).
-template <typename T>
-struct KinView {
-  Pile<float>* pile;
-  int which;  // 0 x, 1 v, 2 q, 3 omega
-};
-
 }  // namespace
 
 PYBIND11_MODULE(_alluvion, m) {
@@ -391,6 +383,95 @@ PYBIND11_MODULE(_alluvion, m) {
       .def("clear", &Mesh::clear);
 
   using P = Pile<float>;
+  // Pile.x / v / q / omega: the reference binds its pinned kinematics
+  // variables (pile.hpp:103-110, pyalluvion.cu:467-470), so pile.x[i] reads
+  // and pile.x[i] = f3 writes body i.  This view does the same on the
+  // library's host kinematics.
+  struct PileKinematics {
+    P* pile;
+    int which;  // 0 x, 1 v, 2 q, 3 omega
+    U size() const { return pile->get_size(); }
+    U index(long i) const {
+      long n = long(size());
+      if (i < 0) i += n;
+      if (i < 0 || i >= n) throw py::index_error("body index out of range");
+      return U(i);
+    }
+    py::object get(long i) const {
+      U b = index(i);
+      switch (which) {
+        case 0: return py::cast(pile->get_x(b));
+        case 1: return py::cast(pile->get_v(b));
+        case 2: return py::cast(pile->get_q(b));
+        default: return py::cast(pile->get_omega(b));
+      }
+    }
+    void set(long i, py::object value) const {
+      U b = index(i);
+      switch (which) {
+        case 0: pile->set_x(b, value.cast<float3>()); break;
+        case 1: pile->set_v(b, value.cast<float3>()); break;
+        case 2: pile->set_q(b, value.cast<float4>()); break;
+        default: pile->set_omega(b, value.cast<float3>()); break;
+      }
+    }
+  };
+  py::class_<PileKinematics>(m, "PileKinematics")
+      .def("__len__", &PileKinematics::size)
+      .def("__getitem__", &PileKinematics::get)
+      .def("__setitem__", &PileKinematics::set)
+      .def("__iter__",
+           [](PileKinematics const& k) {
+             py::list out;
+             for (U i = 0; i < k.size(); ++i) out.append(k.get(long(i)));
+             return py::iter(out);
+           })
+      .def("__repr__", [](PileKinematics const& k) {
+        py::list out;
+        for (U i = 0; i < k.size(); ++i) out.append(k.get(long(i)));
+        return py::repr(out);
+      });
+  auto kin_property = [](int which) {
+    return py::cpp_function([which](P& p) { return PileKinematics{&p, which}; },
+                            py::keep_alive<0, 1>());
+  };
+  auto kin_setter = [](int which) {
+    return py::cpp_function([which](P& p, py::sequence values) {
+      PileKinematics k{&p, which};
+      if (py::len(values) != k.size())
+        throw std::invalid_argument("one value per body expected");
+      for (U i = 0; i < k.size(); ++i) k.set(long(i), values[i]);
+    });
+  };
+  // mass / restitution / friction / inertia_tensor lists (the reference's
+  // std::vector members, pile.hpp:97-101): read as lists, assigned as lists
+  auto body_getter = [](int which) {
+    return py::cpp_function([which](P const& p) {
+      py::list out;
+      for (U i = 0; i < p.get_size(); ++i) {
+        const auto b = p.get_body(i);
+        if (which == 0) out.append(b.mass);
+        if (which == 1) out.append(b.restitution);
+        if (which == 2) out.append(b.friction);
+        if (which == 3) out.append(b.inertia_tensor);
+      }
+      return out;
+    });
+  };
+  auto body_setter = [](int which) {
+    return py::cpp_function([which](P& p, py::sequence values) {
+      if (py::len(values) != p.get_size())
+        throw std::invalid_argument("one value per body expected");
+      for (U i = 0; i < p.get_size(); ++i) {
+        auto b = p.get_body(i);
+        if (which == 0) b.mass = values[i].cast<float>();
+        if (which == 1) b.restitution = values[i].cast<float>();
+        if (which == 2) b.friction = values[i].cast<float>();
+        if (which == 3) b.inertia_tensor = values[i].cast<float3>();
+        p.set_body(i, b);
+      }
+    });
+  };
   py::class_<P>(m, "Pilefloat")
       .def(py::init<Store&, Runner<float>&, U, VolumeMethod, U, Const<float>*, ConstiN*>(),
            py::arg("store"), py::arg("runner"), py::arg("max_num_contacts"),
@@ -440,10 +521,47 @@ PYBIND11_MODULE(_alluvion, m) {
       .def("hint_identical_sequence", &P::hint_identical_sequence, py::arg("begin_id"),
            py::arg("end_id"))
       .def("get_size", &P::get_size)
-      .def_readwrite("mass", &P::mass_)
-      .def_readwrite("restitution", &P::restitution_)
-      .def_readwrite("friction", &P::friction_)
-      .def_readwrite("inertia_tensor", &P::inertia_tensor_)
+      .def_property("mass", body_getter(0), body_setter(0))
+      .def_property("restitution", body_getter(1), body_setter(1))
+      .def_property("friction", body_getter(2), body_setter(2))
+      .def_property("inertia_tensor", body_getter(3), body_setter(3))
+      .def_property("x", kin_property(0), kin_setter(0))
+      .def_property("v", kin_property(1), kin_setter(1))
+      .def_property("q", kin_property(2), kin_setter(2))
+      .def_property("omega", kin_property(3), kin_setter(3))
+      .def("replace",
+           [](P& p, U i, Dist* distance, U3 resolution, float cell_width, float margin,
+              float sign, py::object collision_mesh, float mass, float restitution,
+              float friction, float3 inertia_tensor, float3 x, float4 q,
+              py::object display_mesh) {
+             (void)display_mesh;
+             Mesh none;
+             const Mesh& coll = collision_mesh.is_none() ? none : collision_mesh.cast<const Mesh&>();
+             p.replace(i, distance, resolution, cell_width, margin, sign, coll, mass, restitution,
+                       friction, inertia_tensor, x, q);
+           },
+           py::arg("i"), py::arg("distance"), py::arg("resolution") = U3{0, 0, 0},
+           py::arg("cell_width") = 0,
+           py::arg("margin") = std::numeric_limits<float>::infinity(), py::arg("sign") = 1,
+           py::arg("collision_mesh") = py::none(), py::arg("mass") = 0,
+           py::arg("restitution") = 1, py::arg("friction") = 0,
+           py::arg("inertia_tensor") = float3{1, 1, 1}, py::arg("x") = float3{0, 0, 0},
+           py::arg("q") = float4{0, 0, 0, 1}, py::arg("display_mesh") = py::none(),
+           py::keep_alive<1, 3>())
+      .def("calculate_cfl_v2", &P::calculate_cfl_v2)
+      .def("compute_mask", &P::compute_mask, py::arg("i"), py::arg("distance_threshold"),
+           py::arg("sample_x"), py::arg("mask"), py::arg("num_samples"))
+      .def("get_matrix",
+           [](P const& p, U i) {
+             // glm::mat4 is column-major: element (row r, column c) is m[4c + r]
+             const auto m = p.get_matrix(i);
+             py::array_t<float> out({4, 4});
+             auto a = out.mutable_unchecked<2>();
+             for (int r = 0; r < 4; ++r)
+               for (int c = 0; c < 4; ++c) a(r, c) = m[4 * c + r];
+             return out;
+           },
+           py::arg("i"))
       .def("get_x", &P::get_x)
       .def("get_v", &P::get_v)
       .def("get_q", &P::get_q)
@@ -575,6 +693,12 @@ PYBIND11_MODULE(_alluvion, m) {
         .def_property_readonly("particle_num_neighbors", py::cpp_function(&S::particle_num_neighbors, py::keep_alive<0, 1>()))
         .def_property_readonly("grid_anomaly", py::cpp_function(&S::grid_anomaly, py::keep_alive<0, 1>()))
         .def_property_readonly("boundary_mask", py::cpp_function(&S::boundary_mask, py::keep_alive<0, 1>()))
+        .def_property_readonly("particle_neighbors", py::cpp_function(&S::particle_neighbors, py::keep_alive<0, 1>()))
+        .def_property_readonly("particle_boundary", py::cpp_function(&S::particle_boundary, py::keep_alive<0, 1>()))
+        .def_property_readonly("particle_boundary_kernel", py::cpp_function(&S::particle_boundary_kernel, py::keep_alive<0, 1>()))
+        .def_property_readonly("particle_force", py::cpp_function(&S::particle_force, py::keep_alive<0, 1>()))
+        .def_property_readonly("particle_torque", py::cpp_function(&S::particle_torque, py::keep_alive<0, 1>()))
+        .def("set_static_boundary_forces", &S::set_static_boundary_forces, py::arg("enable"))
         .def_property_readonly("pid_length", py::cpp_function(&S::pid_length, py::keep_alive<0, 1>()))
         .def_property_readonly("pid", py::cpp_function(&S::pid, py::keep_alive<0, 1>()))
         .def("sample_all_boundaries", &S::sample_all_boundaries, py::arg("sample_x"),
@@ -720,7 +844,14 @@ PYBIND11_MODULE(_alluvion, m) {
                     &II::set_density_error_tolerance)
       .def_property("min_density_solve", &II::get_min_density_solve, &II::set_min_density_solve)
       .def_property("max_density_solve", &II::get_max_density_solve, &II::set_max_density_solve)
-      .def_property_readonly("particle_pressure", py::cpp_function(&II::particle_pressure, py::keep_alive<0, 1>()));
+      .def_property_readonly("particle_pressure", py::cpp_function(&II::particle_pressure, py::keep_alive<0, 1>()))
+      .def_property_readonly("particle_last_pressure", py::cpp_function(&II::particle_last_pressure, py::keep_alive<0, 1>()))
+      .def_property_readonly("particle_aii", py::cpp_function(&II::particle_aii, py::keep_alive<0, 1>()))
+      .def_property_readonly("particle_dii", py::cpp_function(&II::particle_dii, py::keep_alive<0, 1>()))
+      .def_property_readonly("particle_dij_pj", py::cpp_function(&II::particle_dij_pj, py::keep_alive<0, 1>()))
+      .def_property_readonly("particle_adv_density", py::cpp_function(&II::particle_adv_density, py::keep_alive<0, 1>()))
+      .def_property_readonly("particle_pressure_accel", py::cpp_function(&II::particle_pressure_accel, py::keep_alive<0, 1>()))
+      .def_property_readonly("particle_density_err", py::cpp_function(&II::particle_density_err, py::keep_alive<0, 1>()));
 
   // SolverI: the settings, report and pressure field of SolverIi
   py::class_<SolverI<float>, II>(m, "SolverIfloat")

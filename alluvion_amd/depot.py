@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _alluvion as _al
 from ._alluvion import Store
-from .variable import variable_class_dict
+from .variable import coat as coat_variable
 
 
 class Depot(Store):
@@ -76,8 +76,4 @@ class Depot(Store):
 
     @classmethod
     def coat(cls, var):
-        name = type(var).__name__
-        if name.startswith("Coated"):
-            return var
-        coated = variable_class_dict[name](var)
-        return coated
+        return coat_variable(var)

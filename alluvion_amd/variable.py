@@ -1,55 +1,73 @@
-"""Coated variables: numpy ``get``/``set`` on device variables.
+"""Numpy access to device variables.
 
-Mirror of the reference's alluvion/variable.py:6-59.
+``Depot.coat(var)`` returns the variable as an instance of a subclass of its
+own bound class that also carries ``get`` (device -> new numpy array),
+``set`` (numpy -> device, at an element offset) and ``type_enum_to_dtype``;
+these are the calls reference scripts make on coated variables
+(alluvion/variable.py:6-59 and depot.py:112-117 of the reference).
+
+Design: one mixin, ``NumpyAccess``, holds the three methods; the coated
+subclass of a bound class is made the first time a variable of that class
+is coated and cached.  The copies go through the bound class's
+``get_bytes`` / ``set_bytes`` (a byte view of the numpy buffer), so no extra
+staging buffer is made.
 """
 import numpy as np
 
 from . import _alluvion as _al
 
-
-def type_enum_to_dtype_func(self):
-    numeric_type = self.get_type()
-    if numeric_type == _al.NumericType.f32:
-        return np.float32
-    if numeric_type == _al.NumericType.f64:
-        return np.float64
-    if numeric_type == _al.NumericType.i32:
-        return np.int32
-    if numeric_type == _al.NumericType.u32:
-        return np.uint32
-    return numeric_type
+# NumericType of the bound classes -> numpy scalar type
+_NUMPY_TYPE = {
+    _al.NumericType.f32: np.float32,
+    _al.NumericType.f64: np.float64,
+    _al.NumericType.i32: np.int32,
+    _al.NumericType.u32: np.uint32,
+}
 
 
-def get_func(self, shape=None):
-    dtype = self.type_enum_to_dtype()
-    if shape is None:
-        shape = self.get_shape()
-    if not isinstance(shape, (list, tuple)):
-        shape = [shape]
-    dst = np.empty(int(np.prod(shape)) * self.get_num_primitives_per_element(), dtype)
-    self.get_bytes(dst.view(np.ubyte))
-    if self.get_num_primitives_per_element() == 1:
-        return dst.reshape(shape)
-    return dst.reshape(*shape, -1)
+class NumpyAccess:
+    """Mixin for bound Variable classes: copies to and from numpy arrays."""
+
+    is_graphical = False
+    is_pinned = False
+
+    def type_enum_to_dtype(self):
+        kind = self.get_type()
+        return _NUMPY_TYPE.get(kind, kind)
+
+    def get(self, shape=None):
+        """A new array with the variable's contents.  The shape is the
+        variable's (or `shape`), with a trailing axis for the components
+        of vector elements (float3 -> [..., 3])."""
+        if shape is None:
+            shape = self.get_shape()
+        dims = list(shape) if isinstance(shape, (list, tuple)) else [int(shape)]
+        comps = self.get_num_primitives_per_element()
+        out = np.empty(dims + ([comps] if comps > 1 else []), self.type_enum_to_dtype())
+        self.get_bytes(out.reshape(-1).view(np.ubyte))
+        return out
+
+    def set(self, src, offset=0):
+        """Writes `src` (converted to the variable's scalar type) starting
+        at element `offset`."""
+        data = np.ascontiguousarray(src, dtype=self.type_enum_to_dtype())
+        self.set_bytes(data.reshape(-1).view(np.ubyte), offset)
 
 
-def set_func(self, src, offset=0):
-    dtype = self.type_enum_to_dtype()
-    src = np.ascontiguousarray(src)
-    if src.dtype != dtype:
-        src = src.astype(dtype)
-    self.set_bytes(src.reshape(-1).view(np.ubyte), offset)
+_coated_classes = {}
 
 
-variable_class_dict = {}
-for _name in dir(_al):
-    if not _name.startswith("Variable"):
-        continue
-    variable_class_dict[_name] = type(
-        "Coated" + _name, (getattr(_al, _name),), {
-            "type_enum_to_dtype": type_enum_to_dtype_func,
-            "get": get_func,
-            "set": set_func,
-            "is_graphical": False,
-            "is_pinned": False,
-        })
+def coated_class(base):
+    """The subclass of bound class `base` that adds NumpyAccess."""
+    cls = _coated_classes.get(base)
+    if cls is None:
+        cls = type("Coated" + base.__name__, (NumpyAccess, base), {})
+        _coated_classes[base] = cls
+    return cls
+
+
+def coat(var):
+    """`var` with numpy access (itself if it already has it)."""
+    if isinstance(var, NumpyAccess):
+        return var
+    return coated_class(type(var))(var)

@@ -40,6 +40,78 @@ def collect_pairs(dp, solver, steps, on_step=None):
     return X, Y, steps * n
 
 
+class Snapshot:
+    """Device copy of a DFSPH solver's step-to-step state (positions,
+    velocities, stored stiffnesses), its pile's body kinematics, time and
+    time step, to take one step twice from the same state."""
+
+    def __init__(self, dp, solver, pile):
+        self.dp, self.solver, self.pile = dp, solver, pile
+        self.fields = ("particle_x", "particle_v", "particle_kappa", "particle_kappa_v")
+        self.copies = []
+        for name in self.fields:
+            var = getattr(solver, name)
+            self.copies.append(dp.create(var.get_shape(), var.get_num_primitives_per_element()))
+
+    def save(self):
+        for name, copy in zip(self.fields, self.copies):
+            copy.set_from(getattr(self.solver, name))
+        p = self.pile
+        self.bodies = [(p.get_x(b), p.get_v(b), p.get_q(b), p.get_omega(b))
+                       for b in range(p.get_size())]
+        self.t, self.dt = self.solver.t, self.solver.dt
+
+    def restore(self):
+        for name, copy in zip(self.fields, self.copies):
+            getattr(self.solver, name).set_from(copy)
+        p = self.pile
+        for b, (x, v, q, w) in enumerate(self.bodies):
+            p.set_x(b, x)
+            p.set_v(b, v)
+            p.set_q(b, q)
+            p.set_omega(b, w)
+        self.solver.t, self.solver.dt = self.t, self.dt
+
+    def release(self):
+        for copy in self.copies:
+            self.dp.remove(copy)
+        self.copies = []
+
+
+def replay_compare(dp, solver, pile, net, steps, on_step=None):
+    """Iterations of the learned and the plain warm start from the same
+    states.  Each step of a plain-warm-start trajectory is taken twice from
+    one snapshot: first with the network, then, after restoring the
+    snapshot, with the reference's warm start, from whose result the
+    trajectory continues.  Returns per-step arrays (learned iterations,
+    plain iterations, learned ms, plain ms); each step ends with the
+    solver's synchronisation, so the wall times are per step."""
+    import time
+
+    import numpy as np
+
+    snap = Snapshot(dp, solver, pile)
+    its = np.zeros((steps, 2), np.int64)
+    ms = np.zeros((steps, 2))
+    try:
+        for i in range(steps):
+            snap.save()
+            for k, use in enumerate((net, None)):
+                if k == 1:
+                    snap.restore()
+                solver.set_warm_start_net(use)
+                t0 = time.perf_counter()
+                solver.step()
+                ms[i, k] = 1e3 * (time.perf_counter() - t0)
+                its[i, k] = solver.num_divergence_solve + solver.num_density_solve
+            if on_step is not None:
+                on_step(i)
+    finally:
+        solver.set_warm_start_net(None)
+        snap.release()
+    return its[:, 0], its[:, 1], ms[:, 0], ms[:, 1]
+
+
 def train(X, Y, count, seed=1, schedule=((3000, 1e-3), (1000, 2e-4)), batch=65536,
           net=None):
     """Fits the input/output normalisation to the data set and trains with

@@ -3,9 +3,12 @@
 Default workload (the headline metric): DFSPH dam break with 1,048,576 fluid
 particles (128 x 128 x 64 jittered lattice, r = 2.5 mm) in a 1 m cube
 container plus four dynamic rigid bodies (two spheres r = 0.05 m, two cubes of
-edge 0.08 m, 128^3 volume maps), dt = 2e-4 s, divergence and density
-tolerance 0.1 %.  One step is SolverDf::step (neighbour search, boundaries,
-both Jacobi solves, viscosity, integration, rigid coupling).
+edge 0.08 m, 128^3 volume maps) floating partly submerged in the fluid,
+dt = 2e-4 s, divergence and density tolerance 0.1 %.  One step is
+SolverDf::step (neighbour search, boundaries, both Jacobi solves, viscosity,
+integration, rigid coupling).  The timed window starts after --start-step
+simulated steps (400 of the scene's 500 by default: pre-roll, autotuning and
+warm-up steps together), when the dam break has developed.
 
 ``--workload periodic_8m`` is the roofline stress run: 8,388,608 particles
 uniform in a periodic unit cube with velocities uniform in +-0.5 m/s; one step
@@ -93,6 +96,57 @@ def measured_traffic(workload, kernel):
     return int(ent["fetch_bytes"] + ent["write_bytes"])
 
 
+def stream_copy_gbs(dp, mib=1024, reps=10):
+    """Measured copy ceiling: device-to-device copies of `mib` MiB
+    (Variable.set_from, the runtime's copy kernel), 2 x bytes per copy over
+    the mean time of `reps` copies after one untimed copy."""
+    import alluvion_amd._alluvion as al
+
+    n = mib * (1 << 20) // 4
+    src = dp.create((n,), 1)
+    dst = dp.create((n,), 1)
+    src.set_same(1)  # any bytes: only the traffic matters
+    dst.set_from(src)
+    al.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.set_from(src)
+    al.device_synchronize()
+    el = (time.perf_counter() - t0) / reps
+    dp.remove(src)
+    dp.remove(dst)
+    gbs = 2.0 * n * 4 / el / 1e9
+    log("stream copy: %d MiB in %.3f ms = %.1f GB/s" % (mib, el * 1e3, gbs))
+    return round(gbs, 1)
+
+
+def host_cpu():
+    """lscpu's model name, sockets, cores per socket and threads per core,
+    and the CPUs this process may run on."""
+    import subprocess
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+
+    def num(key):
+        try:
+            return int(info.get(key, ""))
+        except ValueError:
+            return None
+
+    sockets, cores, threads = num("Socket(s)"), num("Core(s) per socket"), num("Thread(s) per core")
+    return {"model": info.get("Model name"), "sockets": sockets, "cores_per_socket": cores,
+            "threads_per_core": threads,
+            "physical_cores": sockets * cores if sockets and cores else None,
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
 def copy_calibration(dp):
     """Plain device-to-device copies of a known size (Variable.set_from), so
     that a PMC run of this program can scale its HBM counters."""
@@ -113,9 +167,10 @@ def copy_calibration(dp):
 class DamBreak:
     name = "dam_break_1m_rigid"
     metric = METRIC
-    default_steps = 200
+    default_steps = 100
     default_warmup = 20
     default_cpu_steps = 48
+    default_start_step = 400
 
     def __init__(self, args):
         from alluvion_amd import scenes
@@ -133,6 +188,18 @@ class DamBreak:
         self.solver.step()  # ends with a stream synchronisation (report read-back)
         self.n_div += self.solver.num_divergence_solve
         self.n_dens += self.solver.num_density_solve
+
+    def body_forces(self):
+        """Force sums of the last step on the dynamic bodies, and each as a
+        fraction of the body's weight."""
+        out = []
+        for b, spec in enumerate(self.sc.boundaries):
+            if spec.mass == 0:
+                continue
+            f = np.array(list(self.pile.get_force(b)), np.float64)
+            out.append({"body": b, "kind": spec.kind, "force_N": [round(float(c), 5) for c in f],
+                        "lift_over_weight": round(float(f[1] / (spec.mass * 9.81)), 4)})
+        return out
 
     def snapshot(self):
         dp, solver, pile, n = self.dp, self.solver, self.pile, self.sc.n
@@ -165,8 +232,8 @@ class DamBreak:
     def config(self, steps, K, nb):
         return {
             "workload": "DFSPH dam break, 1,048,576 particles (128x128x64 lattice, r=2.5mm) in a "
-                        "1 m cube + 4 dynamic rigid bodies (128^3 volume maps), dt=2e-4, "
-                        "tolerance 0.1%, plain warm start",
+                        "1 m cube + 4 dynamic rigid bodies floating partly submerged (128^3 "
+                        "volume maps), dt=2e-4, tolerance 0.1%, plain warm start",
             "particles": self.sc.n,
             "mean_neighbors": round(K, 2),
             "boundary_pairs_per_particle": round(nb, 4),
@@ -316,39 +383,19 @@ WORKLOADS = {"dam_break_1m_rigid": DamBreak, "periodic_8m": PeriodicStress,
              "batched_256": BatchedScenes}
 
 
-def timed_run(make_scene, steps, net=None):
-    """Fresh solver for a scene, `steps` steps (plain, or with the warm-start
-    network); iterations per step, wall ms per step, simulated time."""
-    import alluvion_amd._alluvion as al
-    from alluvion_amd import scenes
+def learned_comparison(scale, body_resolution, replay_steps=300):
+    """Plain vs learned warm start, measured by replay.
 
-    sc = make_scene()
-    dp, pile, solver, _, _ = scenes.build_solver(sc)
-    if net is not None:
-        solver.set_warm_start_net(net)
-    it = 0
-    al.device_synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        solver.step()
-        it += solver.num_divergence_solve + solver.num_density_solve
-    al.device_synchronize()
-    el = time.perf_counter() - t0
-    out = {"iterations_per_step": round(it / steps, 3), "ms_per_step": round(1e3 * el / steps, 4),
-           "particle_steps_per_s": round(sc.n * steps / el, 1), "simulated_s": round(solver.t, 5),
-           "simulated_s_per_wall_s": round(solver.t / el, 4)}
-    solver.set_warm_start_net(None)
-    del solver, pile, dp
-    return out
-
-
-def learned_comparison(scale, body_resolution):
-    """Plain vs learned warm start.  The network is trained on the GPU on
-    pairs logged from plain runs with the CFL-limited step (cfl 0.4,
-    dt <= 2e-3): 300 steps of the 16,384-particle dam break and 50 steps of
-    the 1M-particle rigid-body dam break.  Then both warm starts run the same
-    steps from the initial state of each scene (the 1M run goes 50 steps
-    past the logged ones); tolerance 0.1 % throughout."""
+    The network is trained on the GPU on pairs logged from plain runs with
+    the CFL-limited step (cfl 0.4, dt <= 2e-3): 300 steps of the
+    16,384-particle dam break and 50 steps of the 1M-particle floating-body
+    dam break.  Then, on each scene, every one of `replay_steps` further steps
+    is taken twice from the same state, with the network and with the
+    reference's warm start (alluvion_amd.warm_start.replay_compare), first
+    with the CFL-limited step, then with the fixed dt = 2e-4 of the configs.
+    Tolerance 0.1 % and the reference's minimum iteration counts (1
+    divergence, 2 density) throughout, so no step can take fewer than 3
+    iterations: the largest possible ratio is plain / 3."""
     import alluvion_amd._alluvion as al
     from alluvion_amd import Depot, scenes, warm_start
 
@@ -356,18 +403,21 @@ def learned_comparison(scale, body_resolution):
     big = lambda: scenes.adaptive_step(scenes.dam_break_rigid(  # noqa: E731
         nx=128 // scale, ny=128 // scale, nz=64 // scale, body_resolution=body_resolution))
     t0 = time.perf_counter()
-    sets = []
-    for make, steps in ((small, 300), (big, 50)):
-        dp, pile, solver, _, _ = scenes.build_solver(make())
-        sets.append((dp,) + warm_start.collect_pairs(dp, solver, steps))
-        del solver, pile
+    runs = []
+    for name, make, steps in (("dam break 16,384 particles", small, 300),
+                              ("dam break 1,048,576 particles + 4 floating bodies", big, 50)):
+        sc = make()
+        dp, pile, solver, _, _ = scenes.build_solver(sc)
+        runs.append((name, sc, dp, pile, solver) + warm_start.collect_pairs(dp, solver, steps))
+        log("warm start: %d pairs logged on %s" % (runs[-1][-1], name))
     # one data set: the logged pairs back to back
-    count = sum(c for _, _, _, c in sets)
+    count = sum(r[-1] for r in runs)
     D = Depot(np.float32)
     X = D.create((count * 8,), 1)
     Y = D.create((count * 2,), 1)
     off = 0
-    for dp, x, y, c in sets:
+    for r in runs:
+        dp, x, y, c = r[2], r[5], r[6], r[7]
         X.set_from(x, c * 8, off * 8)
         Y.set_from(y, c * 2, off * 2)
         off += c
@@ -381,31 +431,41 @@ def learned_comparison(scale, body_resolution):
     D.remove(Y)
     log("warm start: %d pairs logged in %.2f s, trained in %.2f s (loss %.3f -> %.3f)" % (
         count, t1 - t0, t2 - t1, float(np.mean(losses[:10])), float(np.mean(losses[-100:]))))
-    runs = []
-    for name, make, steps in (("dam break 16,384 particles", small, 300),
-                              ("dam break 1,048,576 particles + 4 rigid bodies", big, 100)):
-        plain = timed_run(make, steps)
-        learned = timed_run(make, steps, net)
-        runs.append({"scene": name, "steps": steps, "plain": plain, "learned": learned,
-                     "iteration_ratio": round(plain["iterations_per_step"] /
-                                              learned["iterations_per_step"], 3),
-                     "speedup": round(learned["particle_steps_per_s"] /
-                                      plain["particle_steps_per_s"], 3),
-                     # physical progress per wall second: the warm start
-                     # changes the trajectory and with it the CFL time step
-                     "simulated_time_speedup": round(learned["simulated_s_per_wall_s"] /
-                                                     plain["simulated_s_per_wall_s"], 3)})
-        log("warm start %s: plain %s learned %s" % (name, plain, learned))
-    return {"time_step": "CFL-limited, cfl 0.4, dt <= 2e-3", "tolerance": "0.1% (divergence, density)",
+    results = []
+    floor = 3  # min_divergence_solve + min_density_solve
+    for name, sc, dp, pile, solver, *_ in runs:
+        for rule in ("CFL-limited, cfl 0.4, dt <= 2e-3", "fixed dt = 2e-4"):
+            if rule.startswith("fixed"):
+                solver.min_dt = solver.max_dt = solver.dt = 2e-4
+            lit, pit, lms, pms = warm_start.replay_compare(dp, solver, pile, net, replay_steps)
+            res = {"scene": name, "time_step": rule, "steps": replay_steps,
+                   "plain_iterations_per_step": round(float(pit.mean()), 3),
+                   "learned_iterations_per_step": round(float(lit.mean()), 3),
+                   "iteration_ratio": round(float(pit.mean() / lit.mean()), 3),
+                   "ratio_ceiling_at_min_counts": round(float(pit.mean() / floor), 3),
+                   "steps_learned_fewer_or_equal": round(float((lit <= pit).mean()), 3),
+                   "plain_ms_per_step": round(float(np.median(pms)), 4),
+                   "learned_ms_per_step": round(float(np.median(lms)), 4),
+                   "speedup": round(float(np.median(pms) / np.median(lms)), 3)}
+            results.append(res)
+            log("warm start replay %s, %s: %s" % (name, rule, res))
+        del solver, pile
+    return {"tolerance": "0.1% (divergence, density)",
+            "method": "replay: each step taken from one state with the network and with the "
+                      "reference warm start; the trajectory continues from the plain one",
             "train": {"pairs": int(count),
-                      "source": "300 plain steps of the 16,384-particle dam break and 50 of the "
-                                "1M-particle one",
+                      "source": "300 plain CFL steps of the 16,384-particle dam break and 50 of "
+                                "the 1M-particle one",
                       "adam_steps": len(losses), "batch": 65536, "seconds": round(t2 - t1, 3)},
-            "runs": runs}
+            "runs": results}
 
 
 def cpu_baseline(wl, state, steps):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as of
+
+    native = of.use_native_build()  # -march=native for this host, built here
+    log("cpu oracle build: %s" % (native or "portable (x86-64-v3)"))
     orc, step = wl.oracle(state)
     log("cpu oracle ready")
     step()  # warm-up (first touch of the neighbour arrays)
@@ -422,6 +482,9 @@ def cpu_baseline(wl, state, steps):
         "cores": cores,
         "kind": "port",
         "sample": wl.sample_text(steps, cores),
+        "host": host_cpu(),
+        "build": "g++ -O3 %s -fopenmp -ffp-contract=off, FP32" % (
+            "-march=native" if native else "-march=x86-64-v3"),
     }
 
 
@@ -444,6 +507,11 @@ def main():
                     help="skip the plain-vs-learned warm start comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run 3 device copies of 256 MiB before the steps (PMC scaling)")
+    ap.add_argument("--start-step", type=int, default=None,
+                    help="simulated steps before the timed window (pre-roll, autotuning and "
+                         "warm-up together); default 400 for the dam break, 0 otherwise")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip the measured device-copy (STREAM copy) ceiling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (one-GPU solver core); "
@@ -459,9 +527,20 @@ def main():
     dp, solver, n = wl.dp, wl.solver, wl.sc.n
     if args.copy_calibration:
         copy_calibration(dp)
+    start_step = args.start_step if args.start_step is not None else getattr(
+        wl, "default_start_step", 0)
+    do_tune = not args.no_autotune and hasattr(wl.solver, "set_graph") and \
+        args.workload != "periodic_8m"
+    from alluvion_amd import autotune
+    tune_steps = autotune.steps_taken() if do_tune else 0
+    preroll = max(0, start_step - tune_steps - warmup)
+    start_step = preroll + tune_steps + warmup  # the first timed step
+    for i in range(preroll):
+        wl.step()
+        if i % 50 == 0:
+            log("pre-roll step %d of %d" % (i, preroll))
     tuned = None
-    if not args.no_autotune and hasattr(wl.solver, "set_graph") and args.workload != "periodic_8m":
-        from alluvion_amd import autotune
+    if do_tune:
         best, timing = autotune.tune_step(wl.solver, log=log)
         tuned = {"lds_tiles": best[0], "graph": best[1], "slack": best[2],
                  "ms_per_step": {"%s/%s/%d" % ("tiles" if ti else "gathers",
@@ -536,6 +615,13 @@ def main():
                 "share_of_gpu_time": round(ms / total_ms, 3),
             }
     log("timed region: %.3f s" % elapsed)
+    cfg["timed_steps"] = "%d-%d" % (start_step, start_step + steps)
+    if hasattr(wl, "body_forces"):
+        cfg["body_forces_last_step"] = wl.body_forces()
+    if roofline is not None and not args.no_stream:
+        roofline["stream_copy_gbs"] = stream_copy_gbs(dp)
+        roofline["frac_of_stream_copy"] = round(roofline["achieved"] /
+                                                roofline["stream_copy_gbs"], 4)
     learned = None
     if args.workload == "dam_break_1m_rigid" and not args.no_learned:
         learned = learned_comparison(args.scale, args.body_resolution)

@@ -945,6 +945,74 @@ class Pile {
   void set_omega(U i, float3 w) {
     check(alv_pile_set_kinematics(pile_, i, nullptr, nullptr, nullptr, &w.x));
   }
+  // mass_, restitution_, friction_, inertia_tensor_ of body i (pile.hpp:97-101)
+  struct Body {
+    TF mass, restitution, friction;
+    float3 inertia_tensor;
+  };
+  Body get_body(U i) const {
+    Body b;
+    check(alv_pile_get_body(pile_, i, &b.mass, &b.restitution, &b.friction, &b.inertia_tensor.x));
+    return b;
+  }
+  void set_body(U i, Body const& b) {
+    check(alv_pile_set_body(pile_, i, b.mass, b.restitution, b.friction, &b.inertia_tensor.x));
+  }
+  // Pile::replace (pile.hpp:423-492)
+  void replace(U i, dg::Distance<float3, TF>* distance, U3 resolution = U3{0, 0, 0},
+               TF cell_width = 0, TF margin = std::numeric_limits<TF>::infinity(), TF sign = 1,
+               Mesh const& collision_mesh = Mesh(), TF mass = 0, TF restitution = 1,
+               TF friction = 0, float3 inertia_tensor = float3{1, 1, 1},
+               float3 x = float3{0, 0, 0}, float4 q = float4{0, 0, 0, 1},
+               Mesh const& display_mesh = Mesh()) {
+    (void)display_mesh;
+    if (i >= distances_.size()) throw std::out_of_range("no boundary " + std::to_string(i));
+    alv_distance d = distance->to_c();
+    alv_const cn = cn_->to_c();
+    unsigned res[3] = {resolution.x, resolution.y, resolution.z};
+    float inertia[3] = {inertia_tensor.x, inertia_tensor.y, inertia_tensor.z};
+    float xx[3] = {x.x, x.y, x.z};
+    float qq[4] = {q.x, q.y, q.z, q.w};
+    check(alv_pile_replace(pile_, i, &cn, &d, res, cell_width, margin, sign, mass, restitution,
+                           friction, inertia, xx, qq));
+    distances_[i].reset(distance);
+    check(alv_pile_set_collision_vertices(
+        pile_, i, reinterpret_cast<const float*>(collision_mesh.vertices.data()),
+        U(collision_mesh.vertices.size())));
+    mass_[i] = mass;
+    restitution_[i] = restitution;
+    friction_[i] = friction;
+    inertia_tensor_[i] = inertia_tensor;
+  }
+  // Pile::calculate_cfl_v2 (pile.hpp:816-825)
+  TF calculate_cfl_v2() const {
+    float v2 = 0;
+    check(alv_pile_calculate_cfl_v2(pile_, &v2));
+    return v2;
+  }
+  // Pile::compute_mask (pile.hpp:1048-1065)
+  void compute_mask(U i, TF distance_threshold, Variable<1, float3> const& sample_x,
+                    Variable<1, TF>& mask, U num_samples) {
+    if (sample_x.get_linear_shape() < num_samples || mask.get_linear_shape() < num_samples)
+      throw std::out_of_range("sample_x and mask must hold num_samples elements");
+    check(alv_pile_compute_mask(pile_, i, distance_threshold,
+                                static_cast<const float*>(sample_x.ptr_),
+                                static_cast<float*>(mask.ptr_), num_samples));
+  }
+  // Pile::get_matrix (pile.hpp:1010-1031): the body's rigid transform,
+  // column-major like the reference's glm::mat4 (m[12..14] = x).
+  std::array<float, 16> get_matrix(U i) const {
+    const float4 q = get_q(i);
+    const float3 t = get_x(i);
+    return {1 - 2 * (q.y * q.y + q.z * q.z), 2 * (q.x * q.y + q.z * q.w),
+            2 * (q.x * q.z - q.y * q.w),     0,
+            2 * (q.x * q.y - q.z * q.w),     1 - 2 * (q.x * q.x + q.z * q.z),
+            2 * (q.y * q.z + q.x * q.w),     0,
+            2 * (q.x * q.z + q.y * q.w),     2 * (q.y * q.z - q.x * q.w),
+            1 - 2 * (q.x * q.x + q.y * q.y), 0,
+            t.x,                             t.y,
+            t.z,                             1};
+  }
   void set_gravity(float3 g) { check(alv_pile_set_gravity(pile_, &g.x)); }
   void integrate_kinematics(TF dt) { check(alv_pile_integrate_kinematics(pile_, dt)); }
   void copy_kinematics_to_device() { check(alv_pile_copy_kinematics_to_device(pile_)); }
@@ -1289,6 +1357,32 @@ class SolverBase {
   Variable<1, U> particle_num_neighbors() { return field1<U>(ALV_FIELD_NUM_NEIGHBORS); }
   Variable<1, U> grid_anomaly() { return field1<U>(ALV_FIELD_GRID_ANOMALY); }
   Variable<1, U> boundary_mask() { return field1<U>(ALV_FIELD_BOUNDARY_MASK); }
+  // [rows][n / rows] view of a reference-layout field (the rows are
+  // boundaries, or particles for the neighbour list)
+  template <typename M>
+  Variable<2, M> field2(int f, bool rows_are_particles) {
+    void* p = nullptr;
+    size_t n = 0;
+    unsigned c = 0;
+    check(alv_solver_get_field(solver_, f, &p, &n, &c));
+    U rows = rows_are_particles ? max_num_particles : U(n / max_num_particles);
+    return Variable<2, M>({rows, U(n / rows)}, p);
+  }
+  // particle_neighbors (solver.hpp:74-79): [max_num_particles][K] float4
+  // (x_i - x_j, bits of j), made from the last neighbour search on request
+  Variable<2, float4> particle_neighbors() { return field2<float4>(ALV_FIELD_NEIGHBORS, true); }
+  // particle_boundary, particle_boundary_kernel, particle_force,
+  // particle_torque (solver.hpp:39-46): [num_boundaries][max_num_particles]
+  Variable<2, float4> particle_boundary() { return field2<float4>(ALV_FIELD_BOUNDARY, false); }
+  Variable<2, float4> particle_boundary_kernel() {
+    return field2<float4>(ALV_FIELD_BOUNDARY_KERNEL, false);
+  }
+  Variable<2, float3> particle_force() { return field2<float3>(ALV_FIELD_FORCE, false); }
+  Variable<2, float3> particle_torque() { return field2<float3>(ALV_FIELD_TORQUE, false); }
+  // static boundaries accumulate particle_force / particle_torque too
+  void set_static_boundary_forces(bool enable) {
+    check(alv_solver_set_static_boundary_forces(solver_, enable));
+  }
   Variable<3, U> pid_length() {
     void* p = nullptr;
     check(alv_solver_get_field(solver_, ALV_FIELD_PID_LENGTH, &p, nullptr, nullptr));
@@ -1411,6 +1505,22 @@ class SolverIi : public SolverBase<TF> {
     this->push();
   }
   Variable<1, float> particle_pressure() { return this->template field1<float>(ALV_FIELD_PRESSURE); }
+  // the IISPH arrays (solver_ii.hpp:53-62), user order
+  Variable<1, float> particle_last_pressure() {
+    return this->template field1<float>(ALV_FIELD_LAST_PRESSURE);
+  }
+  Variable<1, float> particle_aii() { return this->template field1<float>(ALV_FIELD_AII); }
+  Variable<1, float3> particle_dii() { return this->template field1<float3>(ALV_FIELD_DII); }
+  Variable<1, float3> particle_dij_pj() { return this->template field1<float3>(ALV_FIELD_DIJ_PJ); }
+  Variable<1, float> particle_adv_density() {
+    return this->template field1<float>(ALV_FIELD_ADV_DENSITY);
+  }
+  Variable<1, float3> particle_pressure_accel() {
+    return this->template field1<float3>(ALV_FIELD_PRESSURE_ACCEL);
+  }
+  Variable<1, float> particle_density_err() {
+    return this->template field1<float>(ALV_FIELD_DENSITY_ERR);
+  }
 
  protected:
   SolverIi(int kind, Runner<TF>& runner, Pile<TF>& pile, Store& store, U max_num_particles,

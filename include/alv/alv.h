@@ -140,6 +140,32 @@ int alv_pile_add(alv_pile* pile, const alv_const* cn,
                  float restitution, float friction, const float inertia[3],
                  const float x[3], const float q[4], unsigned* boundary_id);
 int alv_pile_get_size(const alv_pile* pile, unsigned* size);
+/* Pile::replace (pile.hpp:423-492): boundary b gets a new distance, map and
+ * mass properties, x and q, with v = omega = 0, as alv_pile_add would build
+ * it; solvers use it from their next step.  (Collision vertices are set
+ * separately, alv_pile_set_collision_vertices.) */
+int alv_pile_replace(alv_pile* pile, unsigned b, const alv_const* cn,
+                     const alv_distance* distance, const unsigned resolution[3],
+                     float cell_width, float margin, float sign, float mass,
+                     float restitution, float friction, const float inertia[3],
+                     const float x[3], const float q[4]);
+/* mass_, restitution_, friction_, inertia_tensor_ of body b (pile.hpp:96-102,
+ * read-write in the reference's bindings, pyalluvion.cu:466-473) */
+int alv_pile_set_body(alv_pile* pile, unsigned b, float mass, float restitution,
+                      float friction, const float inertia[3]);
+int alv_pile_get_body(const alv_pile* pile, unsigned b, float* mass, float* restitution,
+                      float* friction, float inertia[3]);
+/* Pile::calculate_cfl_v2 (pile.hpp:816-825): the largest
+ * |omega x (max_distance, 0, 0) + v|^2 over the bodies (lowest float with
+ * none). */
+int alv_pile_calculate_cfl_v2(const alv_pile* pile, float* max_v2);
+/* Pile::compute_mask (pile.hpp:1048-1065, analytic distances,
+ * compute_boundary_mask runner.hpp:4385-4397): for the device arrays
+ * sample_x [n] float3 and mask [n] float, mask[i] *= (sign * d >= threshold)
+ * with d the signed distance of sample i in body b's frame (its current
+ * host kinematics). */
+int alv_pile_compute_mask(alv_pile* pile, unsigned b, float distance_threshold,
+                          const float* sample_x, float* mask, unsigned num_samples);
 /* Host kinematics x_, v_, q_, omega_ (pile.hpp:103-110) */
 int alv_pile_set_kinematics(alv_pile* pile, unsigned b, const float x[3],
                             const float v[3], const float q[4],
@@ -317,6 +343,33 @@ int alv_solver_get_report(const alv_solver* solver,
 /* Per-particle u32 bit mask of the boundaries with non-zero volume
  * (bit b = boundary b), in cell-sorted order; for statistics. */
 #define ALV_FIELD_BOUNDARY_MASK 13
+/* Reference-layout arrays made on request from the last step (or neighbour
+ * search), in user order; the buffers are allocated on first request and
+ * belong to the solver.
+ * particle_neighbors (solver.hpp:78-85, filled by make_neighbor_list,
+ * runner.hpp:1896-1899): [max_n][K] float4 (x_i - x_j, bits of j), slots
+ * past the count zero; */
+#define ALV_FIELD_NEIGHBORS 14
+/* particle_boundary, particle_boundary_kernel (solver.hpp:39-42,
+ * runner.hpp:1982-2025): [B][max_n] float4 (x_b, Psi) and (Psi grad W,
+ * Psi W), zero where the particle does not see boundary b; */
+#define ALV_FIELD_BOUNDARY 15
+#define ALV_FIELD_BOUNDARY_KERNEL 16
+/* particle_force, particle_torque (solver.hpp:43-46): [B][max_n] float3,
+ * the step's per-particle forces on each boundary; rows of static
+ * boundaries are zero unless alv_solver_set_static_boundary_forces is on; */
+#define ALV_FIELD_FORCE 17
+#define ALV_FIELD_TORQUE 18
+/* SolverIi's particle_last_pressure, particle_aii, particle_adv_density,
+ * particle_density_err ([max_n] float) and particle_dii, particle_dij_pj,
+ * particle_pressure_accel ([max_n] float3) (solver_ii.hpp:50-64). */
+#define ALV_FIELD_LAST_PRESSURE 19
+#define ALV_FIELD_AII 20
+#define ALV_FIELD_ADV_DENSITY 21
+#define ALV_FIELD_DENSITY_ERR 22
+#define ALV_FIELD_DII 23
+#define ALV_FIELD_DIJ_PJ 24
+#define ALV_FIELD_PRESSURE_ACCEL 25
 int alv_solver_get_field(alv_solver* solver, int field, void** dptr,
                          size_t* n_elements, unsigned* n_components);
 
@@ -338,6 +391,12 @@ int alv_solver_set_graph(alv_solver* solver, int enable, unsigned slack);
  * does not fit, and periodic grids, read neighbours from global memory.
  * Results are identical either way. */
 int alv_solver_set_tiles(alv_solver* solver, int enable);
+/* The reference accumulates per-particle boundary forces and torques for
+ * every boundary in every pressure and viscosity pass (runner.hpp:3252-3363,
+ * 2150-2249) but only moving bodies use them; here only bodies with mass
+ * accumulate by default.  With `enable`, static boundaries accumulate too
+ * from the next step on (ALV_FIELD_FORCE / ALV_FIELD_TORQUE rows). */
+int alv_solver_set_static_boundary_forces(alv_solver* solver, int enable);
 /* Solver::compute_all_boundaries (solver.hpp:170-185) followed by
  * Solver::update_particle_neighbors<wrap> (solver.hpp:225-260). */
 int alv_solver_update_particle_neighbors(alv_solver* solver);

@@ -86,6 +86,23 @@ def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
+def use_native_build(timeout=300):
+    """Builds the oracle for this host's CPU (-march=native, the CPU
+    baseline's build) and uses it for the rest of the process; returns the
+    library path, or None when the build failed (the portable build is then
+    used).  Must run before the first oracle call."""
+    global LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    path = os.path.join(ORACLE_DIR, "_build", "native", "liboracle.so")
+    try:
+        subprocess.run(["make", "-s", "-C", ORACLE_DIR, "native"], check=True, timeout=timeout)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    LIB_PATH = path
+    return path
+
+
 def lib():
     global _lib
     if _lib is None:

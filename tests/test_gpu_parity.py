@@ -105,7 +105,7 @@ def test_field_op_and_field_gen_kats():
             ("wrap_length", ctypes.c_float * 3)]
 
     cn = AlvConst()
-    assert ctypes.sizeof(AlvConst) == 23 * 4  # alv_const: 23 floats
+    assert ctypes.sizeof(AlvConst) == 20 * 4  # alv_const: 20 floats
     cp = ctypes.POINTER(AlvConst)
     L.alv_const_set_kernel_radius.argtypes = [cp, ctypes.c_float]
     L.alv_const_set_kernel_radius.restype = None
