@@ -21,6 +21,7 @@
 #include <new>
 #include <string>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/alv/alv.h"
@@ -533,11 +534,54 @@ __global__ __launch_bounds__(256) void k_max_f32(const float* d, size_t n, float
   if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
 }
 
+// Final step of the allocation-free sum: one lane adds the block partials in
+// index order, the order in which alv_sum_f32 adds them on the host, so both
+// entry points give the same bits.
+__global__ __launch_bounds__(64) void k_sum_partials(double* part, unsigned nb) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (unsigned k = 0; k < nb; ++k) s += part[k];
+  part[nb] = s;
+}
+
+// Scratch of alv_sum_f32_noalloc: the block partials and the result.  Every
+// GPU the library runs on holds its own copy until the process ends.  Host
+// code must never pass `scratch` itself to a kernel or a copy (on the host it
+// names a stand-in, not device memory); it uses the per-device address from
+// hipGetSymbolAddress (scratch_address below).
+static __device__ __align__(256) double scratch[ALV_SUM_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr unsigned kSumBlocks = 1024;  // block partials of k_sum_f32
+static_assert(ALV_SUM_NOALLOC_SCRATCH_DOUBLES >= kSumBlocks + 1, "scratch too small");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int scratch_address(double** ptr) {
+  int dev = 0;
+  ALV_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(ALV_ERR_UNSUPPORTED, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    ALV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(ALV_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return ALV_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int alv_sum_f32(const float* d, size_t n, double* out) {
   if (!out) return fail(ALV_ERR_ARG, "out is null");
-  const unsigned nb = 1024;
+  const unsigned nb = kSumBlocks;
   double* part = nullptr;
   ALV_HIP(hipMalloc(&part, nb * sizeof(double)));
   hipLaunchKernelGGL(k_sum_f32, dim3(nb), dim3(256), 0, 0, d, n, part);
@@ -551,6 +595,25 @@ int alv_sum_f32(const float* d, size_t n, double* out) {
   return ALV_OK;
 }
 
+int alv_sum_f32_noalloc(const float* d, size_t n, double* out) {
+  if (!out) return fail(ALV_ERR_ARG, "out is null");
+  if (n && !d) return fail(ALV_ERR_ARG, "d is null");
+  if (n > ALV_SUM_NOALLOC_MAX_ELEMENTS)
+    return fail(ALV_ERR_ARG, "alv_sum_f32_noalloc: n = " + std::to_string(n) +
+                                 " exceeds the maximum of " +
+                                 std::to_string((unsigned long long)ALV_SUM_NOALLOC_MAX_ELEMENTS) +
+                                 " elements");
+  double* part = nullptr;
+  int rc = scratch_address(&part);
+  if (rc != ALV_OK) return rc;
+  // Every block writes its partial on every call, so nothing left by an
+  // earlier call is read.
+  hipLaunchKernelGGL(k_sum_f32, dim3(kSumBlocks), dim3(256), 0, 0, d, n, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, 0, part, kSumBlocks);
+  ALV_HIP(hipGetLastError());
+  ALV_HIP(hipMemcpy(out, part + kSumBlocks, sizeof(double), hipMemcpyDeviceToHost));
+  return ALV_OK;
+}
 int alv_max_f32(const float* d, size_t n, float* out) {
   if (!out) return fail(ALV_ERR_ARG, "out is null");
   const unsigned nb = 1024;

@@ -56,6 +56,20 @@ int alv_memset(void* dptr, int value, size_t bytes);
  * the sum accumulates in FP64. */
 int alv_sum_f32(const float* d, size_t n, double* out);
 int alv_max_f32(const float* d, size_t n, float* out);
+/* alv_sum_f32 without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (tight stepping loops
+ * that must not allocate, harnesses that forbid hipMalloc).  The block
+ * partials and the result live in a fixed device array of the library
+ * (ALV_SUM_NOALLOC_SCRATCH_DOUBLES doubles, 8,200 bytes per GPU, held until
+ * the process ends); the caller passes the input d (device, n floats) and
+ * the host result *out.  Same kernels and summation order as alv_sum_f32, so
+ * the same bits.  n above ALV_SUM_NOALLOC_MAX_ELEMENTS fails with ALV_ERR_ARG.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (several host threads) race.  Runs on the null stream and waits for
+ * the result like alv_sum_f32. */
+#define ALV_SUM_NOALLOC_SCRATCH_DOUBLES 1025
+#define ALV_SUM_NOALLOC_MAX_ELEMENTS (1ull << 36)
+int alv_sum_f32_noalloc(const float* d, size_t n, double* out);
 
 /* ---- constants: Const<float> and Consti (constants.hpp:40-137) ----------- */
 typedef struct alv_const {
