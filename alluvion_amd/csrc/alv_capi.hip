@@ -162,6 +162,9 @@ SimParams make_params(const alv_const& cn, const alv_consti& cni, unsigned num_b
   memset(&p, 0, sizeof(p));
   p.h = cn.kernel_radius;
   p.h2 = cn.kernel_radius_sqr;
+  // IEEE single-precision divisions on the host: RN(1/h), RN(1/h^2)
+  p.inv_h = 1.0f / p.h;
+  p.inv_h2 = 1.0f / p.h2;
   p.k = cn.cubic_kernel_k;
   p.l = cn.cubic_kernel_l;
   p.w0 = cn.cubic_kernel_zero;

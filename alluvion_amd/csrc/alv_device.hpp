@@ -3,8 +3,10 @@
 // Own vector type and inline functions used by every kernel.  Expression
 // order follows the reference kernels (kennychufk/alluvion runner.hpp) so that
 // the FP32 results are reproducible; all code is compiled with
-// -ffp-contract=off (no fused multiply-add), and FP32 division and sqrt are
-// the correctly rounded HIP defaults.
+// -ffp-contract=off (no implicit fused multiply-add), FP32 sqrt and division
+// by a per-pair value are the correctly rounded HIP defaults, and division by
+// the kernel radius (a constant of the solve) goes through div_known, which
+// is correctly rounded too.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -67,9 +69,28 @@ __device__ inline void st3(float* p, unsigned i, vec3 v) {
 }
 __device__ inline float4 f4(vec3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
+// Correctly rounded x / d for a divisor known in advance, from its correctly
+// rounded reciprocal rd = RN(1/d) (computed once on the host): q = RN(x rd)
+// is within one unit in the last place of x/d, the remainder x - q d is
+// exact with one fused multiply-add, and RN(q + (x - q d) rd) = RN(x/d)
+// (Markstein's theorem: P. Markstein, "IA-64 and Elementary Functions",
+// 2000, section 8; N. Brisebarre, J.-M. Muller, S. K. Raina, "Accelerating
+// correctly rounded floating-point division when the divisor is known in
+// advance", IEEE Trans. Computers 53(8), 2004), for any x whose quotient
+// neither overflows nor underflows.  Three instructions instead of the ten of
+// the general correctly rounded division.  The kernel functions below divide
+// by h and h^2 this way; their results are the bits of the IEEE divisions
+// (a quotient below the normal range only occurs for particles closer than
+// 1e-19 h, where W and gradW no longer depend on its last bits).
+__device__ inline float div_known(float x, float d, float rd) {
+  const float q = x * rd;
+  const float r = __builtin_fmaf(-q, d, x);
+  return __builtin_fmaf(r, rd, q);
+}
+
 // --- cubic spline kernel (runner.hpp:596-638) ------------------------------
 __device__ inline float W_dist(float r, const SimParams& p) {
-  float q = fabsf(r) / p.h;
+  float q = div_known(fabsf(r), p.h, p.inv_h);
   float conj = 1.0f - q;
   float result = 0.0f;
   if (q <= 0.5f)
@@ -81,8 +102,8 @@ __device__ inline float W_dist(float r, const SimParams& p) {
 // Both branch values are computed and one is selected, so the neighbour
 // loops stay branch-free (the selected value is the same FP32 result).
 __device__ inline float W_d2(float r2, const SimParams& p) {
-  float q2 = r2 / p.h2;
-  float q = sqrtf(r2) / p.h;
+  float q2 = div_known(r2, p.h2, p.inv_h2);
+  float q = div_known(sqrtf(r2), p.h, p.inv_h);
   float conj = 1.0f - q;
   float inner = (6.0f * q - 6.0f) * q2 + 1.0f;
   float outer = 2.0f * conj * conj * conj;
@@ -93,8 +114,8 @@ __device__ inline float W_d2(float r2, const SimParams& p) {
 __device__ inline float gradW_scale(vec3 d, const SimParams& p) {
   float rl2 = length_sqr(d);
   float rl = sqrtf(rl2);
-  float q = rl / p.h;
-  float s_in = (q * 3.0f - 2.0f) / p.h;
+  float q = div_known(rl, p.h, p.inv_h);
+  float s_in = div_known(q * 3.0f - 2.0f, p.h, p.inv_h);
   float s_out = (1.0f - q) * (q - 1.0f) / rl;
   float scale = q <= 0.5f ? s_in : (q <= 1.0f ? s_out : 0.0f);
   return p.l * scale;

@@ -20,6 +20,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "alv_device.hpp"
 #include "alv_kernels.hpp"
@@ -445,17 +446,36 @@ __device__ inline bool stage_x(const DevArrays& A, unsigned blk, TileX& t) {
 // in one range of about 64 + 2 cells' particles, which the wave copies into
 // LDS with coalesced loads; the lanes then read their own spans from LDS
 // instead of gathering from L2.  A row whose range exceeds kStage (a wave
-// across two columns) reads global memory.
-constexpr unsigned kStage = 128;
+// across two columns) reads global memory.  The stage keeps kCB slots of
+// slack past kStage, so that a batch may read up to kCB - 1 entries past its
+// span's end (masked out of its hits) without clamping its addresses.
+constexpr unsigned kStage = 120;
+// List slots per lane that the staged search keeps in LDS ([slot][lane]) until
+// the density and factor pass, which stores them to the list four at a time
+// for the whole wave (one coalesced 1 KB store per group); slots from
+// kListLds on are written to the list during the search.
+constexpr unsigned kListLds = 32;
+static_assert(kListLds % 4u == 0u, "LDS slots end on a group boundary");
 
-// Bounded to 64 VGPRs (8 waves per SIMD; a few bytes of spills), measured
-// slightly faster than 73 VGPRs at 6 waves.
+// Makes a wave's LDS writes visible to the wave's other lanes (the stage is
+// private to a wave, so no workgroup barrier is needed).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The staged instance holds 40 KB of LDS per block (4 blocks, 4 waves per
+// SIMD); the others are bounded to 64 VGPRs for 8 waves per SIMD.
 template <bool W, bool TL>
-__global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, int with_factor) {
-  // LDS staging for the gather instances without the periodic wrap (their
-  // rows are single spans); the others keep the lane-private scan
+__global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab T, DevArrays A,
+                                                                          int with_factor) {
+  // LDS staging and the LDS list for the gather instance without the
+  // periodic wrap (its rows are single spans); the others keep the
+  // lane-private scan
   constexpr bool kWaveStage = !W && !TL;
-  __shared__ float4 stage_[kWaveStage ? 4 : 1][kWaveStage ? kStage : 1];
+  __shared__ float4 stage_[kWaveStage ? 4 : 1][kWaveStage ? kStage + kCB : 1];
+  __shared__ unsigned list_[kWaveStage ? 4 : 1][kWaveStage ? kListLds : 1][64];
   ALV_SCENE(T, A, s);
   // candidate positions from the block's LDS tile when it has one
   TileX tile_;
@@ -464,6 +484,8 @@ __global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, i
   const bool live = s < p.first + p.n;
   if (!kWaveStage && !live) return;
   const unsigned K = p.kmax;
+  // this lane's LDS list slots: slot t at lst[t * 64]
+  unsigned* lst = &list_[kWaveStage ? threadIdx.x >> 6 : 0][0][kWaveStage ? threadIdx.x & 63u : 0];
   const vec3 xi = live ? ld3(A.x[s]) : mk(0.f, 0.f, 0.f);
   int ip[3];
   cell_ipos(xi, p, ip);
@@ -473,11 +495,17 @@ __global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, i
   float sgk = 0.0f;
   vec3 gpi = mk(0.f, 0.f, 0.f);
   // Distances from x_i to the faces of its own cell.  A neighbour cell whose
-  // box is farther than h holds no neighbour, so it is not scanned; the
-  // threshold carries a margin of 1e-4 h^2 over the rounding of positions
-  // and cell assignment (about 1e-6 h), and skipped cells contribute nothing
-  // to the list, whose order is unchanged.
+  // box is farther than h holds no neighbour, so it is not scanned; skipped
+  // cells contribute nothing to the list, whose order is unchanged.  The
+  // reach carries a margin over rounding: a particle's cell comes from the
+  // rounded x/h, so it can lie outside its cell's box by a few units in the
+  // last place of its coordinate, and the face coordinate and the differences
+  // are rounded too.  Those errors scale with the coordinates' magnitude
+  // (about 2^-23 |x| in all), so the margin is 2^-20 of the largest
+  // coordinate (plus h), with a floor of 5e-5 h for coordinates near the
+  // origin.
   float lo_d[3], hi_d[3];
+  float mag = p.h;
   {
     const float xs[3] = {xi.x, xi.y, xi.z};
 #pragma unroll
@@ -485,18 +513,39 @@ __global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, i
       const float c0 = float(ip[a] + p.off[a]) * p.h;
       lo_d[a] = xs[a] - c0;
       hi_d[a] = c0 + p.h - xs[a];
+      mag = fmaxf(mag, fabsf(xs[a]) + p.h);
     }
   }
-  const float reach2 = p.h2 * 1.0001f;
-  // candidates [j0, je) in batches: all positions of a batch are loaded
+  const float reach = p.h * 1.00005f + mag * 9.5367431640625e-7f;
+  const float reach2 = reach * reach;
+  // Appends neighbour j at slot min(cnt, K-1) (the last slot keeps the
+  // latest neighbour): LDS slots below kListLds in the staged instance, else
+  // the register group, stored when complete.
+  auto append = [&](unsigned j) {
+    const unsigned slot = min(cnt, K - 1);
+    if (kWaveStage && slot < kListLds) {
+      lst[slot * 64u] = j;
+    } else {
+      const unsigned c4 = slot & 3u;
+      grp.x = c4 == 0 ? j : grp.x;
+      grp.y = c4 == 1 ? j : grp.y;
+      grp.z = c4 == 2 ? j : grp.z;
+      grp.w = c4 == 3 ? j : grp.w;
+      if (c4 == 3u && cnt == slot) A.nbr[nbr_group(A, slot >> 2, s)] = grp;
+    }
+    ++cnt;
+  };
+  // candidates [jb, je) in batches: all positions of a batch are loaded
   // before any is tested (one memory latency per batch); the tests are
   // branch-free and give a bit mask of hits, which are then appended in
-  // ascending j
-  auto scan_span = [&](unsigned jb, unsigned je, auto&& X) {
+  // ascending j.  X(j0, u) is candidate j0 + u; kClamp: X must not be read
+  // past je - 1 (global memory), else reads past the span are harmless.
+  auto scan_span = [&](unsigned jb, unsigned je, auto&& X, auto clamp_tag) {
+    constexpr bool kClamp = decltype(clamp_tag)::value;
     for (unsigned j0 = jb; j0 < je; j0 += kCB) {
       float4 xc[kCB];
 #pragma unroll
-      for (int u = 0; u < kCB; ++u) xc[u] = X(min(j0 + u, je - 1u));
+      for (int u = 0; u < kCB; ++u) xc[u] = kClamp ? X(min(j0 + u, je - 1u), 0u) : X(j0, u);
       unsigned hits = 0u;
 #pragma unroll
       for (int u = 0; u < kCB; ++u) {
@@ -510,18 +559,12 @@ __global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, i
       while (hits) {
         const unsigned j = j0 + unsigned(__builtin_ctz(hits));
         hits &= hits - 1u;
-        // slot min(cnt, K-1): the last slot keeps the latest neighbour
-        const unsigned slot = min(cnt, K - 1);
-        const unsigned c4 = slot & 3u;
-        grp.x = c4 == 0 ? j : grp.x;
-        grp.y = c4 == 1 ? j : grp.y;
-        grp.z = c4 == 2 ? j : grp.z;
-        grp.w = c4 == 3 ? j : grp.w;
-        if (c4 == 3u && cnt == slot) A.nbr[nbr_group(A, slot >> 2, s)] = grp;
-        ++cnt;
+        append(j);
       }
     }
   };
+  using Clamp = std::integral_constant<bool, true>;
+  using NoClamp = std::integral_constant<bool, false>;
   // The 27 cells in the reference order (dx outer, dz inner): for each
   // (dx, dy) row the up-to-3 cells along z are contiguous in the sorted
   // order unless a periodic wrap splits them, so each row is 1-3 spans.
@@ -554,15 +597,15 @@ __global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, i
       }
       const bool fits = hi > lo && hi - lo <= kStage;
       float4* st = stage_[threadIdx.x >> 6];
-      __syncthreads();  // the previous row's reads are done
+      wave_lds_sync();  // the previous row's reads are done
       if (fits)
         for (unsigned k = threadIdx.x & 63u; k < hi - lo; k += 64u) st[k] = A.x[lo + k];
-      __syncthreads();
+      wave_lds_sync();
       if (jb < je) {
         if (fits)
-          scan_span(jb, je, [&](unsigned j) { return st[j - lo]; });
+          scan_span(jb, je, [&](unsigned j0, unsigned u) { return st[j0 - lo + u]; }, NoClamp());
         else
-          scan_span(jb, je, [&](unsigned j) { return A.x[j]; });
+          scan_span(jb, je, [&](unsigned j, unsigned) { return A.x[j]; }, Clamp());
       }
     } else {
       if (!row_ok) continue;
@@ -584,17 +627,21 @@ __global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, i
       }
       for (unsigned q = 0; q < nspan; ++q) {
         if (tiled_)
-          scan_span(span_b[q], span_e[q], [&](unsigned j) { return tile_.X(j); });
+          scan_span(span_b[q], span_e[q], [&](unsigned j, unsigned) { return tile_.X(j); },
+                    Clamp());
         else
-          scan_span(span_b[q], span_e[q], [&](unsigned j) { return A.x[j]; });
+          scan_span(span_b[q], span_e[q], [&](unsigned j, unsigned) { return A.x[j]; },
+                    Clamp());
       }
     }
   }
   if (!live) return;
-  // the group holding the last slot is incomplete, or was rewritten by
-  // overflowing neighbours after it was stored
-  if (cnt > 0 && ((min(cnt, K) & 3u) != 0u || cnt > K))
-    A.nbr[nbr_group(A, (min(cnt, K) - 1u) >> 2, s)] = grp;
+  // The group holding the last slot is incomplete, or was rewritten by
+  // overflowing neighbours after it was stored (register groups only; the
+  // LDS slots are stored below).
+  const unsigned nn = min(cnt, K);
+  if (nn > 0 && (!kWaveStage || nn - 1u >= kListLds) && ((nn & 3u) != 0u || cnt > K))
+    A.nbr[nbr_group(A, (nn - 1u) >> 2, s)] = grp;
   if (cnt > K) A.anomaly[2] = 1u;
   // Density and factor sums over the finished list, in list order.  The
   // list holds the first K-1 neighbours in search order and the last one
@@ -602,11 +649,21 @@ __global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, i
   // Doing the kernel arithmetic here, not inside the candidate scan, keeps
   // every lane of a wave busy: in the scan some lane finds a neighbour at
   // almost every candidate, so the whole wave would execute the kernel
-  // evaluation for every candidate.
-  const unsigned nn = min(cnt, K);
+  // evaluation for every candidate.  The lanes walk their lists in step, so
+  // the LDS slots of group q are stored for the whole wave at once.
   for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
     unsigned jn[kNB];
-    batch_ids(A, s, nn, t0, jn);
+    if (kWaveStage && t0 < kListLds) {
+      const uint4 g = make_uint4(lst[t0 * 64u], lst[(t0 + 1u) * 64u], lst[(t0 + 2u) * 64u],
+                                 lst[(t0 + 3u) * 64u]);
+      A.nbr[nbr_group(A, t0 >> 2, s)] = g;
+      jn[0] = t0 + 0 < nn ? g.x : s;
+      jn[1] = t0 + 1 < nn ? g.y : s;
+      jn[2] = t0 + 2 < nn ? g.z : s;
+      jn[3] = t0 + 3 < nn ? g.w : s;
+    } else {
+      batch_ids(A, s, nn, t0, jn);
+    }
     float4 xj[kNB];
 #pragma unroll
     for (int u = 0; u < kNB; ++u) xj[u] = tiled_ ? tile_.X(jn[u]) : A.x[jn[u]];
@@ -631,7 +688,7 @@ __global__ __launch_bounds__(256, 8) void k_neighbors(SceneTab T, DevArrays A, i
     dens += bk.w;
     gpi += mk(bk.x, bk.y, bk.z);
   }
-  A.nnb[s] = min(cnt, K);
+  A.nnb[s] = nn;
   const float rho = dens * p.rho0;
   A.density[s] = rho;
   reinterpret_cast<float*>(A.v + s)[3] = rho;

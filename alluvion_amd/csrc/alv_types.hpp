@@ -14,6 +14,8 @@ enum { ALV_DIST_BOX = 0, ALV_DIST_BOX_SHELL = 1, ALV_DIST_SPHERE = 2 };
 struct SimParams {
   // kernel constants (set_kernel_radius, constants.hpp:100-120)
   float h, h2, k, l, w0;
+  // correctly rounded 1/h and 1/h^2 (divisions by h and h^2 use div_known)
+  float inv_h, inv_h2;
   // particle attributes (set_particle_attr, constants.hpp:122-128)
   float radius, vol, mass, rho0;
   float visc, bvisc, eps;

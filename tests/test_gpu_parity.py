@@ -167,6 +167,40 @@ def test_neighbor_search_periodic_bit_exact():
     neighbor_parity(sc)
 
 
+def test_neighbor_search_far_from_origin_bit_exact():
+    """A random cloud 300 m from the origin (30,000 cells of h): the rounding
+    of x/h and of the cell faces is about 3e-5 m there, far above a fixed
+    margin of 1e-4 h^2, so the search's skip of cells farther than h must
+    scale its margin with the coordinates.  Lists, counts and offsets equal
+    the oracle's (which scans all 27 cells), and every list equals brute force
+    with the FP32 predicate ((dx dx + dy dy) + dz dz < h^2)."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(11)
+    origin = np.array([301.27, -187.9, 256.3])
+    edge = 0.15
+    n = 24_000
+    x = (origin + rng.random((n, 3)) * edge).astype(np.float32)
+    h = 0.01
+    off = tuple(int(np.floor(o / h)) - 1 for o in origin)
+    res = int(np.ceil(edge / h)) + 3
+    sc = scenes.Scene(name="far_cloud", x=x, v=np.zeros_like(x), gravity=(0.0, 0.0, 0.0),
+                      grid_offset=off, grid_res=(res, res, res), boundaries=[])
+    dp, solver, orc = neighbor_parity(sc)
+    ids, cnt = solver.get_neighbors()
+    assert int(np.asarray(solver.get_grid()[2])[0]) == 0  # nothing outside the grid
+    h32 = np.float32(h)
+    tree = cKDTree(x.astype(np.float64))
+    pairs = tree.query_ball_point(x.astype(np.float64), r=h * 1.001)
+    for i in range(n):
+        cand = np.array([j for j in pairs[i] if j != i], np.int64)
+        d = x[i][None, :] - x[cand]
+        d2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        want = np.sort(cand[d2 < h32 * h32])
+        got = np.sort(ids[i, : cnt[i]].astype(np.int64))
+        assert np.array_equal(got, want), i
+
+
 @pytest.mark.parametrize("warm", [0, 40])
 def test_dfsph_step_parity_dam_break(warm):
     sc = scenes.dam_break_small()
