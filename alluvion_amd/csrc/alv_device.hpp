@@ -88,6 +88,46 @@ __device__ inline float div_known(float x, float d, float rd) {
   return __builtin_fmaf(r, rd, q);
 }
 
+// Square root rounded to nearest, from the hardware square root (within one
+// unit in the last place, v_sqrt_f32): the estimate's neighbours s - 1 ulp
+// and s + 1 ulp are tested with exact remainders x - s' s (fused
+// multiply-add); a float x never lies within (s - ulp/2)^2 - ulp^2/4 .. of a
+// midpoint square, so the tests pick RN(sqrt(x)).  Checked on the CPU for
+// every float in [2^-96, 2^32] and every estimate within one ulp.  Smaller
+// arguments (squared distances of particles closer than 1e-29 h) may round
+// differently, but the kernel values below do not depend on those bits.
+// Nine instructions instead of the fifteen of the general correctly rounded
+// sqrtf, which also scales denormal arguments.
+__device__ inline float sqrt_rn(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
+  const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
+  const float r_dn = __builtin_fmaf(-s_dn, s, x);
+  const float r_up = __builtin_fmaf(-s_up, s, x);
+  const float t = r_dn <= 0.0f ? s_dn : s;
+  return r_up > 0.0f ? s_up : t;
+}
+
+// a / b for normal operands with a normal quotient: one Newton step on the
+// hardware reciprocal (v_rcp_f32, within one ulp), then two remainder
+// corrections (Markstein).  Eight instructions instead of the sixteen of the
+// general correctly rounded division.  It is not correctly rounded for every
+// (a, b): when 1/b lies next to a rounding midpoint the refined reciprocal
+// can miss RN(1/b) by more than half a unit.  The kernels use it only for
+// the cubic-spline gradient's (1 - q)(q - 1) / r, whose operands follow from
+// r and h alone, and only when the host has checked that whole domain for
+// the solve's h (SimParams::div_exact, set by the host library).
+__device__ inline float div_rn(float a, float b) {
+  const float y0 = __builtin_amdgcn_rcpf(b);
+  const float e = __builtin_fmaf(-b, y0, 1.0f);
+  const float y = __builtin_fmaf(e, y0, y0);
+  const float q0 = a * y;
+  const float r0 = __builtin_fmaf(-b, q0, a);
+  const float q1 = __builtin_fmaf(r0, y, q0);
+  const float r1 = __builtin_fmaf(-b, q1, a);
+  return __builtin_fmaf(r1, y, q1);
+}
+
 // --- cubic spline kernel (runner.hpp:596-638) ------------------------------
 __device__ inline float W_dist(float r, const SimParams& p) {
   float q = div_known(fabsf(r), p.h, p.inv_h);
@@ -103,7 +143,7 @@ __device__ inline float W_dist(float r, const SimParams& p) {
 // loops stay branch-free (the selected value is the same FP32 result).
 __device__ inline float W_d2(float r2, const SimParams& p) {
   float q2 = div_known(r2, p.h2, p.inv_h2);
-  float q = div_known(sqrtf(r2), p.h, p.inv_h);
+  float q = div_known(sqrt_rn(r2), p.h, p.inv_h);
   float conj = 1.0f - q;
   float inner = (6.0f * q - 6.0f) * q2 + 1.0f;
   float outer = 2.0f * conj * conj * conj;
@@ -113,10 +153,11 @@ __device__ inline float W_d2(float r2, const SimParams& p) {
 // gradW(d) = gradW_scale(d) * d
 __device__ inline float gradW_scale(vec3 d, const SimParams& p) {
   float rl2 = length_sqr(d);
-  float rl = sqrtf(rl2);
+  float rl = sqrt_rn(rl2);
   float q = div_known(rl, p.h, p.inv_h);
   float s_in = div_known(q * 3.0f - 2.0f, p.h, p.inv_h);
-  float s_out = (1.0f - q) * (q - 1.0f) / rl;
+  const float num = (1.0f - q) * (q - 1.0f);
+  float s_out = p.div_exact ? div_rn(num, rl) : num / rl;
   float scale = q <= 0.5f ? s_in : (q <= 1.0f ? s_out : 0.0f);
   return p.l * scale;
 }

@@ -319,6 +319,9 @@ void launch_volume_field_grid(const SimParams& p, const GridGeomDev& g,
                               const float* peirce, unsigned begin, unsigned end,
                               hipStream_t s);
 void launch_node_positions(const GridGeomDev& g, float* out, hipStream_t s);
+// element-wise device arithmetic of the kernel functions (alv_check_math)
+void launch_check_math(const SimParams& p, int op, const float* a, const float* b, float* out,
+                       size_t n, hipStream_t s);
 
 // field sync masks for launch_unsort_fields
 enum : unsigned {

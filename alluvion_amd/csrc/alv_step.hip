@@ -556,10 +556,23 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
       // batch rather than per candidate
       if (je - j0 < kCB) hits &= (1u << (je - j0)) - 1u;
       if (s - j0 < kCB) hits &= ~(1u << (s - j0));
-      while (hits) {
-        const unsigned j = j0 + unsigned(__builtin_ctz(hits));
-        hits &= hits - 1u;
-        append(j);
+      if (kWaveStage && cnt + kCB <= min(kListLds, K)) {
+        // Every candidate of the batch is written, without a branch, to the
+        // slot after the batch's earlier hits: a miss is overwritten by the
+        // next hit (or lies past the new count), so the slots cnt ..
+        // cnt + hits - 1 end up holding the hits in ascending j.  The batch
+        // stays below the LDS slots' end and below the cap K - 1, where the
+        // latest neighbour replaces the last slot.
+#pragma unroll
+        for (int u = 0; u < kCB; ++u)
+          lst[(cnt + unsigned(__builtin_popcount(hits & ((1u << u) - 1u)))) * 64u] = j0 + u;
+        cnt += unsigned(__builtin_popcount(hits));
+      } else {
+        while (hits) {
+          const unsigned j = j0 + unsigned(__builtin_ctz(hits));
+          hits &= hits - 1u;
+          append(j);
+        }
       }
     }
   };
@@ -2505,6 +2518,30 @@ void launch_volume_field_grid(const SimParams& p, const GridGeomDev& g, const fl
   if (end <= begin) return;
   hipLaunchKernelGGL(k_volume_grid, dim3(grid_for(end - begin)), dim3(kBlock), 0, s, p, g,
                      distance, volume, sign, peirce, begin, end);
+}
+// The device arithmetic the kernel functions are built on, element-wise
+// (alv_check_math): 0 sqrt_rn(a), 1 div_rn(a, b), 2 div_known(a, b, RN(1/b)),
+// 3 gradW_scale of the displacement (a, 0, 0), 4 W_d2(a).
+__global__ __launch_bounds__(256) void k_check_math(SimParams p, int op, const float* a,
+                                                    const float* b, float* out, size_t n) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float x = a[i];
+  float r = 0.0f;
+  switch (op) {
+    case 0: r = sqrt_rn(x); break;
+    case 1: r = div_rn(x, b[i]); break;
+    case 2: r = div_known(x, b[i], 1.0f / b[i]); break;
+    case 3: r = gradW_scale(mk(x, 0.0f, 0.0f), p); break;
+    default: r = W_d2(x, p); break;
+  }
+  out[i] = r;
+}
+void launch_check_math(const SimParams& p, int op, const float* a, const float* b, float* out,
+                       size_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_check_math, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     s, p, op, a, b, out, n);
 }
 void launch_node_positions(const GridGeomDev& g, float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_node_positions, dim3(grid_for(g.num_nodes)), dim3(kBlock), 0, s, g, out);

@@ -1,0 +1,44 @@
+"""ctypes view of the C ABI pieces the tests call directly (no pybind11):
+the alv_const record and the declared signatures of the functions used."""
+import ctypes
+
+from alluvion_amd import LIB_PATH
+
+_FIELDS = ("kernel_radius", "kernel_radius_sqr", "cubic_kernel_k", "cubic_kernel_l",
+           "cubic_kernel_zero", "particle_radius", "particle_vol", "particle_mass", "density0",
+           "viscosity", "boundary_viscosity")
+
+
+class AlvConst(ctypes.Structure):
+    """alv_const (include/alv/alv.h): 20 floats."""
+    _fields_ = [(k, ctypes.c_float) for k in _FIELDS] + [("gravity", ctypes.c_float * 3)] + [
+        (k, ctypes.c_float) for k in ("boundary_epsilon", "dfsph_factor_epsilon",
+                                      "contact_tolerance")] + [
+        ("wrap_length", ctypes.c_float * 3)]
+
+
+assert ctypes.sizeof(AlvConst) == 20 * 4
+
+
+def library():
+    L = ctypes.CDLL(LIB_PATH)
+    cp = ctypes.POINTER(AlvConst)
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.alv_const_set_kernel_radius.argtypes = [cp, ctypes.c_float]
+    L.alv_const_set_kernel_radius.restype = None
+    L.alv_const_set_particle_attr.argtypes = [cp, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    L.alv_const_set_particle_attr.restype = None
+    L.alv_check_math.argtypes = [cp, ctypes.c_int, fp, fp, fp, ctypes.c_size_t]
+    L.alv_check_math.restype = ctypes.c_int
+    L.alv_kernel_division_exact.argtypes = [cp, ctypes.POINTER(ctypes.c_int)]
+    L.alv_kernel_division_exact.restype = ctypes.c_int
+    L.alv_last_error.argtypes = []
+    L.alv_last_error.restype = ctypes.c_char_p
+    return L
+
+
+def const_for(L, h):
+    cn = AlvConst()
+    L.alv_const_set_kernel_radius(ctypes.byref(cn), h)
+    L.alv_const_set_particle_attr(ctypes.byref(cn), h / 4, 1.0, 1000.0)
+    return cn
