@@ -157,60 +157,6 @@ void peirce_table(float out[120]) {
   }
 }
 
-// div_rn (alv_device.hpp) emulated on the host from a reciprocal estimate y0.
-float div_rn_from(float a, float b, float y0) {
-  const float e = std::fmaf(-b, y0, 1.0f);
-  const float y = std::fmaf(e, y0, y0);
-  const float q0 = a * y;
-  const float r0 = std::fmaf(-b, q0, a);
-  const float q1 = std::fmaf(r0, y, q0);
-  const float r1 = std::fmaf(-b, q1, a);
-  return std::fmaf(r1, y, q1);
-}
-
-// Whether div_rn gives the IEEE quotient of the cubic-spline gradient's
-// (1 - q)(q - 1) / r for every float r whose q = RN(r / h) lies in (1/2, 1]
-// (the only branch that divides by r; a and r follow from r and h alone),
-// from each of the three reciprocal estimates within one ulp of RN(1/r).
-// About 10 M quotients per radius, once per radius in the process.
-bool gradient_division_exact(float h) {
-  static std::mutex mutex;
-  static std::map<uint32_t, bool> checked;
-  uint32_t hb;
-  memcpy(&hb, &h, 4);
-  std::lock_guard<std::mutex> lock(mutex);
-  auto it = checked.find(hb);
-  if (it != checked.end()) return it->second;
-  bool ok = std::isfinite(h) && h > 0.0f;
-  const float half = 0.5f * h;
-  uint32_t lo, hi;
-  memcpy(&lo, &half, 4);
-  hi = hb;
-  for (uint32_t u = lo > 8u ? lo - 8u : 0u; ok && u <= hi + 8u; ++u) {
-    float r;
-    memcpy(&r, &u, 4);
-    const float q = r / h;
-    if (!(q > 0.5f && q <= 1.0f)) continue;
-    const float a = (1.0f - q) * (q - 1.0f);
-    const float want = a / r;
-    const float y = 1.0f / r;
-    uint32_t yb, wb;
-    memcpy(&yb, &y, 4);
-    memcpy(&wb, &want, 4);
-    for (int k = -1; k <= 1 && ok; ++k) {
-      const uint32_t y0b = yb + uint32_t(k);
-      float y0, got;
-      memcpy(&y0, &y0b, 4);
-      got = div_rn_from(a, r, y0);
-      uint32_t gb;
-      memcpy(&gb, &got, 4);
-      ok = gb == wb;
-    }
-  }
-  checked[hb] = ok;
-  return ok;
-}
-
 SimParams make_params(const alv_const& cn, const alv_consti& cni, unsigned num_boundaries,
                       unsigned n) {
   SimParams p;
@@ -220,7 +166,6 @@ SimParams make_params(const alv_const& cn, const alv_consti& cni, unsigned num_b
   // IEEE single-precision divisions on the host: RN(1/h), RN(1/h^2)
   p.inv_h = 1.0f / p.h;
   p.inv_h2 = 1.0f / p.h2;
-  p.div_exact = gradient_division_exact(p.h) ? 1 : 0;
   p.k = cn.cubic_kernel_k;
   p.l = cn.cubic_kernel_l;
   p.w0 = cn.cubic_kernel_zero;
@@ -2627,8 +2572,8 @@ int alv_node_positions(const float dmin[3], const unsigned res[3], const float c
 
 int alv_check_math(const alv_const* cn, int op, const float* a, const float* b, float* out,
                    size_t n) {
-  if (!cn || op < 0 || op > 4) return fail(ALV_ERR_ARG, "bad argument");
-  if (n && (!a || !out || ((op == 1 || op == 2) && !b))) return fail(ALV_ERR_ARG, "null argument");
+  if (!cn || op < 0 || op > 3) return fail(ALV_ERR_ARG, "bad argument");
+  if (n && (!a || !out || (op == 1 && !b))) return fail(ALV_ERR_ARG, "null argument");
   if (!n) return ALV_OK;
   alv_consti cni;
   memset(&cni, 0, sizeof(cni));
@@ -2644,12 +2589,6 @@ int alv_check_math(const alv_const* cn, int op, const float* a, const float* b, 
   if (e == hipSuccess) e = hipMemcpy(out, d + 2 * n, n * sizeof(float), hipMemcpyDeviceToHost);
   (void)hipFree(d);
   if (e != hipSuccess) return fail(ALV_ERR_HIP, hipGetErrorString(e));
-  return ALV_OK;
-}
-
-int alv_kernel_division_exact(const alv_const* cn, int* exact) {
-  if (!cn || !exact) return fail(ALV_ERR_ARG, "null argument");
-  *exact = gradient_division_exact(cn->kernel_radius) ? 1 : 0;
   return ALV_OK;
 }
 

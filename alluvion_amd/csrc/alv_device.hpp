@@ -108,26 +108,6 @@ __device__ inline float sqrt_rn(float x) {
   return r_up > 0.0f ? s_up : t;
 }
 
-// a / b for normal operands with a normal quotient: one Newton step on the
-// hardware reciprocal (v_rcp_f32, within one ulp), then two remainder
-// corrections (Markstein).  Eight instructions instead of the sixteen of the
-// general correctly rounded division.  It is not correctly rounded for every
-// (a, b): when 1/b lies next to a rounding midpoint the refined reciprocal
-// can miss RN(1/b) by more than half a unit.  The kernels use it only for
-// the cubic-spline gradient's (1 - q)(q - 1) / r, whose operands follow from
-// r and h alone, and only when the host has checked that whole domain for
-// the solve's h (SimParams::div_exact, set by the host library).
-__device__ inline float div_rn(float a, float b) {
-  const float y0 = __builtin_amdgcn_rcpf(b);
-  const float e = __builtin_fmaf(-b, y0, 1.0f);
-  const float y = __builtin_fmaf(e, y0, y0);
-  const float q0 = a * y;
-  const float r0 = __builtin_fmaf(-b, q0, a);
-  const float q1 = __builtin_fmaf(r0, y, q0);
-  const float r1 = __builtin_fmaf(-b, q1, a);
-  return __builtin_fmaf(r1, y, q1);
-}
-
 // --- cubic spline kernel (runner.hpp:596-638) ------------------------------
 __device__ inline float W_dist(float r, const SimParams& p) {
   float q = div_known(fabsf(r), p.h, p.inv_h);
@@ -150,14 +130,16 @@ __device__ inline float W_d2(float r2, const SimParams& p) {
   float result = q <= 0.5f ? inner : (q <= 1.0f ? outer : 0.0f);
   return result * p.k;
 }
-// gradW(d) = gradW_scale(d) * d
+// gradW(d) = gradW_scale(d) * d.  (1 - q)(q - 1) / r is an IEEE division: a
+// faster sequence from the hardware reciprocal is not correctly rounded for
+// every radius, and choosing it at run time inside the neighbour loops cost
+// more than it saved.
 __device__ inline float gradW_scale(vec3 d, const SimParams& p) {
   float rl2 = length_sqr(d);
   float rl = sqrt_rn(rl2);
   float q = div_known(rl, p.h, p.inv_h);
   float s_in = div_known(q * 3.0f - 2.0f, p.h, p.inv_h);
-  const float num = (1.0f - q) * (q - 1.0f);
-  float s_out = p.div_exact ? div_rn(num, rl) : num / rl;
+  float s_out = (1.0f - q) * (q - 1.0f) / rl;
   float scale = q <= 0.5f ? s_in : (q <= 1.0f ? s_out : 0.0f);
   return p.l * scale;
 }

@@ -206,22 +206,27 @@ __global__ __launch_bounds__(256) void k_scatter(SceneTab T, DevArrays A) {
 // Makes the order inside each cell canonical (ascending particle id): the
 // atomic slots of k_cell_count are arrival-ordered, so every particle takes
 // the position of its rank among its cell's particles in the scatter's
-// copy.  The bin of particles outside the grid keeps the arrival order:
-// nothing depends on it (those particles are in no cell and their own
-// neighbour scans are independent).
+// copy.  One thread per sorted slot t: the scatter's copy holds particle
+// sort_tmp[t] there, and its cell's slots [b, e) lie around t, so the rank
+// loop and the write stay within a few cache lines of the thread's
+// neighbours' (a thread per particle, in particle order, touches a random
+// cell when particle order and space are unrelated).  The bin of particles
+// outside the grid keeps the arrival order: nothing depends on it (those
+// particles are in no cell and their own neighbour scans are independent).
 __global__ __launch_bounds__(256) void k_cell_sort(SceneTab T, DevArrays A) {
-  ALV_SCENE_LINEAR(T, A, i);
-  if (i >= p.first + p.n) return;
+  ALV_SCENE_LINEAR(T, A, t);
+  if (t >= p.first + p.n) return;
+  const unsigned i = A.sort_tmp[t];
   const unsigned k = A.key[i];
-  const unsigned b = A.cell_start[k];
   if (k == p.cell_base + p.num_cells) {
-    A.sorted_id[b + A.slot[i]] = i;
+    A.sorted_id[t] = i;
     return;
   }
+  const unsigned b = A.cell_start[k];
   const unsigned e = A.cell_start[k + 1];
   if (e - b > p.cell_cap) A.anomaly[1] = 1u;
   unsigned rank = 0;
-  for (unsigned t = b; t < e; ++t) rank += A.sort_tmp[t] < i ? 1u : 0u;
+  for (unsigned u = b; u < e; ++u) rank += A.sort_tmp[u] < i ? 1u : 0u;
   A.sorted_id[b + rank] = i;
 }
 
@@ -2520,8 +2525,8 @@ void launch_volume_field_grid(const SimParams& p, const GridGeomDev& g, const fl
                      distance, volume, sign, peirce, begin, end);
 }
 // The device arithmetic the kernel functions are built on, element-wise
-// (alv_check_math): 0 sqrt_rn(a), 1 div_rn(a, b), 2 div_known(a, b, RN(1/b)),
-// 3 gradW_scale of the displacement (a, 0, 0), 4 W_d2(a).
+// (alv_check_math): 0 sqrt_rn(a), 1 div_known(a, b, RN(1/b)), 2 gradW_scale
+// of the displacement (a, 0, 0), 3 W_d2(a).
 __global__ __launch_bounds__(256) void k_check_math(SimParams p, int op, const float* a,
                                                     const float* b, float* out, size_t n) {
   const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -2530,9 +2535,8 @@ __global__ __launch_bounds__(256) void k_check_math(SimParams p, int op, const f
   float r = 0.0f;
   switch (op) {
     case 0: r = sqrt_rn(x); break;
-    case 1: r = div_rn(x, b[i]); break;
-    case 2: r = div_known(x, b[i], 1.0f / b[i]); break;
-    case 3: r = gradW_scale(mk(x, 0.0f, 0.0f), p); break;
+    case 1: r = div_known(x, b[i], 1.0f / b[i]); break;
+    case 2: r = gradW_scale(mk(x, 0.0f, 0.0f), p); break;
     default: r = W_d2(x, p); break;
   }
   out[i] = r;

@@ -16,9 +16,6 @@ struct SimParams {
   float h, h2, k, l, w0;
   // correctly rounded 1/h and 1/h^2 (divisions by h and h^2 use div_known)
   float inv_h, inv_h2;
-  // 1 when the host has checked that div_rn gives the IEEE quotient of the
-  // cubic-spline gradient's (1 - q)(q - 1) / r for every r of its branch
-  int div_exact;
   // particle attributes (set_particle_attr, constants.hpp:122-128)
   float radius, vol, mass, rho0;
   float visc, bvisc, eps;

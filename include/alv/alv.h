@@ -463,15 +463,11 @@ int alv_profiling_kernel(unsigned index, const char** name, double* total_ms,
 /* ---- device arithmetic checks (no counterpart in the reference) ---------- */
 /* The kernels' correctly rounded arithmetic, element-wise on the GPU over
  * host buffers a[n] (and b[n]), for callers that check it against IEEE
- * results: op 0 sqrt(a), op 1 a / b by the fast division of the cubic-spline
- * gradient, op 2 a / b by the known-divisor division, op 3 the cubic-spline
- * gradient scale of the displacement (a, 0, 0), op 4 the cubic spline W of
- * the squared distance a, with the kernel radius of cn. */
+ * results: op 0 sqrt(a), op 1 a / b by the known-divisor division, op 2 the
+ * cubic-spline gradient scale of the displacement (a, 0, 0), op 3 the cubic
+ * spline W of the squared distance a, with the kernel radius of cn. */
 int alv_check_math(const alv_const* cn, int op, const float* a, const float* b, float* out,
                    size_t n);
-/* Whether the kernels use the fast division for cn's kernel radius (the
- * host checked every quotient of the gradient's dividing branch): 1 or 0. */
-int alv_kernel_division_exact(const alv_const* cn, int* exact);
 
 /* ---- stand-alone volume-map kernels (tests/field_op.cu, field_gen.cu) ----- */
 /* index_to_node_position over all nodes of a grid (runner.hpp:1131-1178):

@@ -30,8 +30,6 @@ def library():
     L.alv_const_set_particle_attr.restype = None
     L.alv_check_math.argtypes = [cp, ctypes.c_int, fp, fp, fp, ctypes.c_size_t]
     L.alv_check_math.restype = ctypes.c_int
-    L.alv_kernel_division_exact.argtypes = [cp, ctypes.POINTER(ctypes.c_int)]
-    L.alv_kernel_division_exact.restype = ctypes.c_int
     L.alv_last_error.argtypes = []
     L.alv_last_error.restype = ctypes.c_char_p
     return L

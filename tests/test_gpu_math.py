@@ -1,11 +1,10 @@
 """The GPU's arithmetic under the kernel functions, bit for bit against IEEE
 single precision computed by numpy (correctly rounded, no fused operations):
-the square root over random and edge arguments, the gradient division over
-its whole branch at h = 0.01 (fast division) and h = 2^-7 (the IEEE division
-the kernels fall back to there), the known-divisor division by h and h^2,
-and the cubic spline W and gradient scale over every distance in (0, h]
-sampled densely.  The CPU checks of the same sequences are in
-tests/test_round_check.py."""
+the square root over random and edge arguments, the known-divisor division by
+h and h^2, and the cubic spline W and gradient scale over every distance of
+the gradient's dividing branch (1/2 < r/h <= 1) and a dense sample of (0, h].
+The CPU checks of the same sequences are in tests/test_round_check.py and
+tests/test_div_known.py."""
 import ctypes
 
 import numpy as np
@@ -84,22 +83,13 @@ def test_sqrt_matches_ieee():
 
 
 @pytest.mark.parametrize("h", [0.01, 2.0 ** -7])
-def test_gradient_division_and_scale(h):
+def test_gradient_scale(h):
     L = alv_ctypes.library()
     cn = alv_ctypes.const_for(L, h)
-    flag = ctypes.c_int(-1)
-    assert L.alv_kernel_division_exact(ctypes.byref(cn), ctypes.byref(flag)) == 0
-    r = gradient_branch(h)
-    q = r / F32(h)
-    a = (F32(1.0) - q) * (q - F32(1.0))
-    if flag.value:
-        assert_bits(run(L, cn, 1, a, r), a / r, "gradient division h=%g" % h)
-    # the kernel's gradient scale: every r of the dividing branch, and a
-    # dense sample of (0, h]
-    rs = np.concatenate([r, (np.arange(1, 1 << 20, dtype=np.float64) / (1 << 20) *
-                             float(h)).astype(F32)])
-    assert_bits(run(L, cn, 3, rs), numpy_grad_scale(rs, h, cn.cubic_kernel_l),
-                "gradient scale h=%g (fast division %d)" % (h, flag.value))
+    rs = np.concatenate([gradient_branch(h), (np.arange(1, 1 << 20, dtype=np.float64) /
+                                               (1 << 20) * float(h)).astype(F32)])
+    assert_bits(run(L, cn, 2, rs), numpy_grad_scale(rs, h, cn.cubic_kernel_l),
+                "gradient scale h=%g" % h)
 
 
 @pytest.mark.parametrize("h", [0.01, 0.037])
@@ -109,8 +99,8 @@ def test_known_divisor_division_and_w(h):
     rng = np.random.default_rng(6)
     a = (rng.random(1 << 22) * 4 * h).astype(F32)
     d = np.full_like(a, F32(h))
-    assert_bits(run(L, cn, 2, a, d), a / F32(h), "x / h")
+    assert_bits(run(L, cn, 1, a, d), a / F32(h), "x / h")
     d2 = np.full_like(a, F32(cn.kernel_radius_sqr))
-    assert_bits(run(L, cn, 2, a * a, d2), (a * a) / F32(cn.kernel_radius_sqr), "x / h^2")
+    assert_bits(run(L, cn, 1, a * a, d2), (a * a) / F32(cn.kernel_radius_sqr), "x / h^2")
     r2 = (rng.random(1 << 22) * F32(cn.kernel_radius_sqr) * 1.1).astype(F32)
-    assert_bits(run(L, cn, 4, r2), numpy_w(r2, h, cn.kernel_radius_sqr, cn.cubic_kernel_k), "W")
+    assert_bits(run(L, cn, 3, r2), numpy_w(r2, h, cn.kernel_radius_sqr, cn.cubic_kernel_k), "W")
