@@ -1866,7 +1866,10 @@ int pellet_launch(alv_solver* S, const Launch& L, Launch* Lp) {
 
 // grid + gather + boundary records + neighbour list + density (+ factor).
 // Only kernels (no memset nodes), so the sequence can be captured in a graph.
-int neighbor_phase(alv_solver* S, const Launch& L, int with_factor) {
+// gather_state: also gather the solver state the step's passes read in
+// sorted order (stiffnesses or pressure); a neighbour update on its own
+// leaves it out, since only a step reads it, after its own neighbour phase.
+int neighbor_phase(alv_solver* S, const Launch& L, int with_factor, bool gather_state = true) {
   DevArrays& A = S->A;
   launch_fill_u32(A.cell_count, 0u, L.cells, S->stream);
   launch_fill_u32(A.anomaly, 0u, 3, S->stream);
@@ -1877,7 +1880,9 @@ int neighbor_phase(alv_solver* S, const Launch& L, int with_factor) {
   launch_scatter(L, A, S->stream);
   launch_cell_sort(L, A, S->stream);
   launch_tiles(L, A, S->stream);
-  launch_gather_boundary(L, A, S->kind, S->stream);
+  const int kind = !gather_state ? ALV_KIND_NONE
+                   : S->kind == ALV_SOLVER_DF ? ALV_KIND_DF : ALV_KIND_II;
+  launch_gather_boundary(L, A, kind, S->stream);
   launch_neighbors(L, A, with_factor, S->stream);
   ALV_HIP(hipGetLastError());
   S->grid_valid = 1;
@@ -2334,7 +2339,7 @@ int alv_solver_update_particle_neighbors(alv_solver* S) {
     if (rc != ALV_OK) return rc;
     launch_pellet_grid(Lp, S->A, S->stream);
   }
-  rc = neighbor_phase(S, L, S->kind == ALV_SOLVER_DF ? 1 : 0);
+  rc = neighbor_phase(S, L, S->kind == ALV_SOLVER_DF ? 1 : 0, false);
   if (rc != ALV_OK) return rc;
   if (L.p0.pellet_mode) launch_pellet_lists(L, Lp, S->A, S->stream);
   ALV_HIP(hipGetLastError());

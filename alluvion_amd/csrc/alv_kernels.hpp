@@ -334,6 +334,8 @@ enum : unsigned {
 };
 
 // field codes written by launch_unsort_fields / gathered in gather_boundary
-enum { ALV_KIND_DF = 0, ALV_KIND_II = 1 };
+// solver state gathered into sorted order by k_gather_boundary: DFSPH
+// stiffnesses, IISPH/ISPH pressure, or none (positions and velocities only)
+enum { ALV_KIND_DF = 0, ALV_KIND_II = 1, ALV_KIND_NONE = 2 };
 
 }  // namespace alv

@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SceneTab T, DevArrays A
   if (kind == ALV_KIND_DF) {
     A.kappa[s] = A.ukappa[i];
     A.kappa_v[s] = A.ukappa_v[i];
-  } else {
+  } else if (kind == ALV_KIND_II) {
     A.pressure[s] = A.upressure[i];
   }
   if (p.pellet_mode) {
