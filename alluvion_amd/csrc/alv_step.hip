@@ -869,14 +869,16 @@ __global__ __launch_bounds__(256) void k_ws_div0(SceneTab T, DevArrays A) {
 // warm start fused in: the density change and the eight features are
 // computed as in k_ws_div0, then the whole wave runs the network on its 64
 // particles (matrix cores; the forward weight operands are staged in LDS once
-// per block and shared by its four waves).  The prediction of the converged
-// stored stiffness replaces 0.5 max(kappa_v, -0.5) and the density one is
-// kept for warm_start_pressure_solve0.  Converged stiffnesses are never
-// positive (k = rho_dot alpha and (rho* - 1) alpha with rho_dot >= 0,
-// rho* >= 1, alpha <= 0), so predictions are clamped to [bound, 0] with the
-// bounds of the reference's own warm-start clamps (-0.5 and -2.5e-4 in
-// stored form); a non-finite prediction falls back to the reference's warm
-// start.  F: also log the features for training.
+// per block and shared by its four waves).  The network's two outputs are
+// corrections to the reference's warm starts 0.5 max(kappa_v, -0.5) and
+// 0.5 max(kappa, -2.5e-4) (stored form, from the previous step's stored
+// stiffnesses); the corrected divergence stiffness is used here and the
+// density one is kept for warm_start_pressure_solve0.  Converged
+// stiffnesses are never positive (k = rho_dot alpha and (rho* - 1) alpha
+// with rho_dot >= 0, rho* >= 1, alpha <= 0), so the results are clamped to
+// [bound, 0] with the bounds of the reference's own warm-start clamps; a
+// non-finite prediction falls back to the reference's warm start.  F: also
+// log the features for training.
 template <bool W, bool F>
 __global__ __launch_bounds__(256) void k_ws_div0_learned(SceneTab T, DevArrays A, MlpDev M,
                                                          int use_divergence) {
@@ -923,10 +925,14 @@ __global__ __launch_bounds__(256) void k_ws_div0_learned(SceneTab T, DevArrays A
   float y0, y1;
   mlp_infer_wave(Ml, f, y0, y1);  // every lane of the wave, live or not
   if (!live) return;
-  if (use_divergence && dadv > 0.0f && isfinite(y0)) kv = std_min(std_max(y0, -0.5f), 0.0f) / dt;
+  // The network predicts corrections to the reference's warm starts (in
+  // stored form): an uncertain network (outputs near 0) falls back to them.
+  if (use_divergence && dadv > 0.0f && isfinite(y0))
+    kv = std_min(std_max(0.5f * std_max(f[4], -0.5f) + y0, -0.5f), 0.0f) / dt;
   A.kappa_v[s] = kv;
   set_kw(A, s, kv);
-  A.kpred[s] = isfinite(y1) ? std_min(std_max(y1, -0.00025f), 0.0f) : y1;
+  A.kpred[s] =
+      isfinite(y1) ? std_min(std_max(0.5f * std_max(f[5], -0.00025f) + y1, -0.00025f), 0.0f) : y1;
 }
 
 // warm_start_divergence_solve_1 (runner.hpp:3252-3295)
@@ -1336,9 +1342,12 @@ __global__ __launch_bounds__(256) void k_export_pairs(SceneTab T, DevArrays A,
   float* xo = x + size_t(s) * kMlpIn;
   xo[0] = f0.x; xo[1] = f0.y; xo[2] = f0.z; xo[3] = f0.w;
   xo[4] = f1.x; xo[5] = f1.y; xo[6] = f1.z; xo[7] = f1.w;
+  // targets: the converged stored stiffnesses minus the reference's warm
+  // starts from the previous ones (features 4 and 5), which the learned warm
+  // start adds back
   const unsigned i = A.sorted_id[s];
-  y[size_t(s) * 2 + 0] = A.ukappa_v[i];
-  y[size_t(s) * 2 + 1] = A.ukappa[i];
+  y[size_t(s) * 2 + 0] = A.ukappa_v[i] - 0.5f * std_max(f1.x, -0.5f);
+  y[size_t(s) * 2 + 1] = A.ukappa[i] - 0.5f * std_max(f1.y, -0.00025f);
 }
 
 // integrate_velocity<wrap> (runner.hpp:3613-3623) on the user-order state

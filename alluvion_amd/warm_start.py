@@ -1,12 +1,15 @@
 """Learned warm start for the DFSPH solver: data collection and training.
 
 The network (``_alluvion.WarmStartNet``, 8 -> 64 -> 64 -> 64 -> 2, SiLU,
-bfloat16 on the matrix cores) predicts each particle's converged stored
-stiffness (kappa_v, kappa) from eight local features.  Training pairs come
-from the solver itself: with the feature log on, every step records the
-features at its divergence warm start and, after the step, the converged
-stiffness it ended with.  Training runs on the GPU with the library's own
-forward/backward kernels and Adam.
+bfloat16 on the matrix cores) predicts, for each particle, how far its
+converged stored stiffnesses (kappa_v, kappa) lie from the reference's warm
+starts (half the previous step's, clamped), from eight local features; the
+solver adds the prediction to the reference's warm start, so a network that
+predicts 0 reproduces it.  Training pairs come from the solver itself: with
+the feature log on, every step records the features at its divergence warm
+start and, after the step, the converged stiffnesses minus those warm
+starts.  Training runs on the GPU with the library's own forward/backward
+kernels and Adam.
 
 Typical use::
 
@@ -22,22 +25,28 @@ iterations, never accuracy.
 from . import _alluvion
 
 
-def collect_pairs(dp, solver, steps, on_step=None):
-    """Runs `steps` solver steps with the feature log on and returns device
-    buffers (X [pairs][8], Y [pairs][2]) and the number of pairs."""
+def collect_pairs(dp, solver, steps, on_step=None, stride=1):
+    """Runs `steps` solver steps and logs the pairs of every `stride`-th one
+    (the feature log is on for those steps only); returns device buffers
+    (X [pairs][8], Y [pairs][2]) and the number of pairs."""
     n = solver.num_particles
-    X = dp.create((steps * n * 8,), 1)
-    Y = dp.create((steps * n * 2,), 1)
-    solver.set_feature_log(True)
+    logged = (steps + stride - 1) // stride
+    X = dp.create((logged * n * 8,), 1)
+    Y = dp.create((logged * n * 2,), 1)
+    k = 0
     try:
         for i in range(steps):
+            log = i % stride == 0
+            solver.set_feature_log(log)
             solver.step()
-            solver.export_training_pairs(X, Y, i * n)
+            if log:
+                solver.export_training_pairs(X, Y, k * n)
+                k += 1
             if on_step is not None:
                 on_step(i)
     finally:
         solver.set_feature_log(False)
-    return X, Y, steps * n
+    return X, Y, logged * n
 
 
 class Snapshot:

@@ -388,41 +388,49 @@ def learned_comparison(scale, body_resolution, replay_steps=300):
 
     The network is trained on the GPU on pairs logged from plain runs with
     the CFL-limited step (cfl 0.4, dt <= 2e-3): 300 steps of the
-    16,384-particle dam break and 50 steps of the 1M-particle floating-body
-    dam break.  Then, on each scene, every one of `replay_steps` further steps
-    is taken twice from the same state, with the network and with the
-    reference's warm start (alluvion_amd.warm_start.replay_compare), first
-    with the CFL-limited step, then with the fixed dt = 2e-4 of the configs.
-    Tolerance 0.1 % and the reference's minimum iteration counts (1
-    divergence, 2 density) throughout, so no step can take fewer than 3
-    iterations: the largest possible ratio is plain / 3."""
+    16,384-particle dam break, and every 5th of 350 steps of a 1M-particle
+    floating-body dam break drawn with other seeds (lattice jitter 12, bodies
+    13), so that the pairs cover the whole flow and none comes from the run
+    the comparison replays.  Then, on each scene (the 16,384-particle run
+    continued; the benchmark's own 1M scene after 50 plain steps), every one
+    of `replay_steps` further steps is taken twice from the same state, with
+    the network and with the reference's warm start
+    (alluvion_amd.warm_start.replay_compare), first with the CFL-limited step,
+    then with the fixed dt = 2e-4 of the configs.  Tolerance 0.1 % and the
+    reference's minimum iteration counts (1 divergence, 2 density)
+    throughout, so no step can take fewer than 3 iterations: the largest
+    possible ratio is plain / 3."""
     import alluvion_amd._alluvion as al
     from alluvion_amd import Depot, scenes, warm_start
 
-    small = lambda: scenes.adaptive_step(scenes.dam_break_small())  # noqa: E731
-    big = lambda: scenes.adaptive_step(scenes.dam_break_rigid(  # noqa: E731
-        nx=128 // scale, ny=128 // scale, nz=64 // scale, body_resolution=body_resolution))
+    def big(seed, body_seed):
+        return scenes.adaptive_step(scenes.dam_break_rigid(
+            seed=seed, body_seed=body_seed, nx=128 // scale, ny=128 // scale, nz=64 // scale,
+            body_resolution=body_resolution))
+
     t0 = time.perf_counter()
-    runs = []
-    for name, make, steps in (("dam break 16,384 particles", small, 300),
-                              ("dam break 1,048,576 particles + 4 floating bodies", big, 50)):
-        sc = make()
-        dp, pile, solver, _, _ = scenes.build_solver(sc)
-        runs.append((name, sc, dp, pile, solver) + warm_start.collect_pairs(dp, solver, steps))
-        log("warm start: %d pairs logged on %s" % (runs[-1][-1], name))
+    logs = []
+    sc_small = scenes.adaptive_step(scenes.dam_break_small())
+    dp_s, pile_s, solver_s, _, _ = scenes.build_solver(sc_small)
+    logs.append((dp_s,) + warm_start.collect_pairs(dp_s, solver_s, 300))
+    log("warm start: %d pairs logged on the 16,384-particle dam break" % logs[-1][-1])
+    sc_train = big(12, 13)
+    dp_t, pile_t, solver_t, _, _ = scenes.build_solver(sc_train)
+    logs.append((dp_t,) + warm_start.collect_pairs(dp_t, solver_t, 350, stride=5))
+    log("warm start: %d pairs logged on the 1M-particle training run" % logs[-1][-1])
     # one data set: the logged pairs back to back
-    count = sum(r[-1] for r in runs)
+    count = sum(r[-1] for r in logs)
     D = Depot(np.float32)
     X = D.create((count * 8,), 1)
     Y = D.create((count * 2,), 1)
     off = 0
-    for r in runs:
-        dp, x, y, c = r[2], r[5], r[6], r[7]
+    for dp, x, y, c in logs:
         X.set_from(x, c * 8, off * 8)
         Y.set_from(y, c * 2, off * 2)
         off += c
         dp.remove(x)
         dp.remove(y)
+    del solver_t, pile_t
     al.device_synchronize()
     t1 = time.perf_counter()
     net, losses = warm_start.train(X, Y, count)
@@ -431,9 +439,15 @@ def learned_comparison(scale, body_resolution, replay_steps=300):
     D.remove(Y)
     log("warm start: %d pairs logged in %.2f s, trained in %.2f s (loss %.3f -> %.3f)" % (
         count, t1 - t0, t2 - t1, float(np.mean(losses[:10])), float(np.mean(losses[-100:]))))
+    sc_big = big(2, 3)
+    dp_b, pile_b, solver_b, _, _ = scenes.build_solver(sc_big)
+    for _ in range(50):
+        solver_b.step()
+    runs = [("dam break 16,384 particles", dp_s, pile_s, solver_s),
+            ("dam break 1,048,576 particles + 4 floating bodies", dp_b, pile_b, solver_b)]
     results = []
     floor = 3  # min_divergence_solve + min_density_solve
-    for name, sc, dp, pile, solver, *_ in runs:
+    for name, dp, pile, solver in runs:
         for rule in ("CFL-limited, cfl 0.4, dt <= 2e-3", "fixed dt = 2e-4"):
             if rule.startswith("fixed"):
                 solver.min_dt = solver.max_dt = solver.dt = 2e-4
@@ -449,13 +463,14 @@ def learned_comparison(scale, body_resolution, replay_steps=300):
                    "speedup": round(float(np.median(pms) / np.median(lms)), 3)}
             results.append(res)
             log("warm start replay %s, %s: %s" % (name, rule, res))
-        del solver, pile
     return {"tolerance": "0.1% (divergence, density)",
             "method": "replay: each step taken from one state with the network and with the "
                       "reference warm start; the trajectory continues from the plain one",
+            "target": "correction to the reference warm start (stored stiffness)",
             "train": {"pairs": int(count),
-                      "source": "300 plain CFL steps of the 16,384-particle dam break and 50 of "
-                                "the 1M-particle one",
+                      "source": "300 plain CFL steps of the 16,384-particle dam break and every "
+                                "5th of 350 plain CFL steps of a 1M-particle floating-body dam "
+                                "break drawn with other seeds (12, 13)",
                       "adam_steps": len(losses), "batch": 65536, "seconds": round(t2 - t1, 3)},
             "runs": results}
 

@@ -109,10 +109,10 @@ def test_training_fits_a_smooth_map():
 
 
 def test_feature_log_and_learned_step():
-    """Features are recorded with their meaning; a step with a network
-    switched in meets the unchanged convergence test, and with an all-zero
-    network (every prediction 0) matches the oracle's step with zero warm
-    starts."""
+    """Features are recorded with their meaning and the targets are the
+    converged stiffnesses minus the reference's warm starts; with an all-zero
+    network (every correction 0) a step matches the oracle's step with the
+    reference's warm starts."""
     import alluvion_amd._alluvion as al
     sc = scenes.dam_break_small()
     dp, pile, solver, cn, cni = scenes.build_solver(sc, "df")
@@ -129,17 +129,23 @@ def test_feature_log_and_learned_step():
     solver.export_training_pairs(X, Y)
     f = dp.coat(X).get().reshape(n, 8)
     y = dp.coat(Y).get().reshape(n, 2)
-    # the pairs come in an internal order: match them by (kappa_v, kappa)
+    # the pairs come in an internal order: match them by the previous
+    # stiffnesses (features 4, 5) to the converged ones minus the reference's
+    # warm starts
     kv_now = dp.coat(solver.particle_kappa_v).get()[:n]
     k_now = dp.coat(solver.particle_kappa).get()[:n]
-    assert sorted(map(tuple, y)) == sorted(zip(kv_now, k_now))
+    f32 = np.float32
+    want = sorted(zip(kv_prev, k_prev, kv_now - f32(0.5) * np.maximum(kv_prev, f32(-0.5)),
+                      k_now - f32(0.5) * np.maximum(k_prev, f32(-0.00025))))
+    got = sorted(zip(f[:, 4], f[:, 5], y[:, 0], y[:, 1]))
+    assert got == want
     assert np.allclose(np.sort(f[:, 4]), np.sort(kv_prev))
     assert np.allclose(np.sort(f[:, 5]), np.sort(k_prev))
     assert np.allclose(np.sort(f[:, 6]), np.sort(np.linalg.norm(v_prev, axis=1)), rtol=1e-6)
     assert np.all(f[:, 7] == np.float32(sc.dt))
     assert np.all(f[:, 2] == np.floor(f[:, 2])) and f[:, 2].max() <= sc.max_num_neighbors
     assert np.all(f[:, 1] >= 0)
-    # an all-zero network predicts 0 for both stiffnesses
+    # an all-zero network predicts no correction: the reference's warm starts
     net = al.WarmStartNet(0)
     net.set_params(np.zeros(of.MLP_PARAMS, np.float32))
     solver.set_warm_start_net(net)
@@ -148,8 +154,8 @@ def test_feature_log_and_learned_step():
         orc.set_volume_map(b, dp.coat(var).get())
     orc.set("x", dp.coat(solver.particle_x).get()[:n])
     orc.set("v", dp.coat(solver.particle_v).get()[:n])
-    orc.set("kappa", np.zeros(n, np.float32))
-    orc.set("kappa_v", np.zeros(n, np.float32))
+    orc.set("kappa", dp.coat(solver.particle_kappa).get()[:n])
+    orc.set("kappa_v", dp.coat(solver.particle_kappa_v).get()[:n])
     solver.step()
     orc.step()
     rep = orc.report()
