@@ -13,8 +13,8 @@ time; the tuning steps are ordinary simulation steps.
 import time
 
 # (LDS tiles, graph, slack)
-CANDIDATES = ((False, False, 0), (True, False, 0), (True, True, 0), (True, True, 1),
-              (True, True, 2), (False, True, 1))
+CANDIDATES = ((False, False, 0), (False, True, 0), (False, True, 1), (False, True, 2),
+              (True, False, 0), (True, True, 0), (True, True, 1))
 
 
 def steps_taken(steps=5, candidates=CANDIDATES):

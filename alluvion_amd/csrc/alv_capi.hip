@@ -763,6 +763,16 @@ int place_boundary(alv_pile* pile, int slot, const alv_const* cn, const alv_dist
     B.geom.res[a] = res[a];
     B.geom.cs[a] = (B.geom.dmax[a] - B.geom.dmin[a]) / float(res[a]);
   }
+  {
+    // the farthest corner of the map's box from the body origin, with a
+    // relative margin far above the rounding of the kernels' distance test
+    double r2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double lo = B.geom.dmin[a], hi = B.geom.dmax[a];
+      r2 += std::max(lo * lo, hi * hi);
+    }
+    B.map_radius2 = float(r2 * (1.0 + 1e-4) + 1e-12);
+  }
   size_t nv = size_t(res[0] + 1) * (res[1] + 1) * (res[2] + 1);
   size_t ne = size_t(res[0]) * (res[1] + 1) * (res[2] + 1) +
               size_t(res[0] + 1) * res[1] * (res[2] + 1) +

@@ -351,14 +351,18 @@ __device__ inline float grid_sample(const GridGeomDev& g, const float* __restric
 // (compute_volume_and_boundary_x_analytic, runner.hpp:1738-1771, and
 // compute_particle_boundary_analytic, runner.hpp:1982-2002): rec = (closest
 // boundary point, volume Psi), ker = (Psi gradW, Psi W).  Returns Psi.
-// With Sparse, a point outside the map or in a cell whose nodes are all zero
-// returns 0 at once and leaves rec and ker unset (its volume is exactly 0,
-// so every term of the record is too).
+// With Sparse, a point outside the map (first by a sphere around the body
+// origin that holds the map's box, before the rotation into the body frame)
+// or in a cell whose nodes are all zero returns 0 at once and leaves rec and
+// ker unset (its volume is exactly 0, so every term of the record is too).
 template <bool Sparse>
 __device__ inline float boundary_record(const BoundaryDev& B, vec3 x, const SimParams& p,
                                         float4& rec, float4& ker) {
   const float4 q = make_float4(B.q[0], B.q[1], B.q[2], B.q[3]);
-  vec3 local = rotate(x - mk(B.x[0], B.x[1], B.x[2]), conjugate(q));
+  const vec3 rel = x - mk(B.x[0], B.x[1], B.x[2]);
+  // farther from the body origin than any point of the map: outside it
+  if (Sparse && length_sqr(rel) > B.map_radius2) return 0.0f;
+  vec3 local = rotate(rel, conjugate(q));
   int ip[3];
   vec3 inner;
   const bool in_map = grid_resolve(B.geom, local, ip, inner);

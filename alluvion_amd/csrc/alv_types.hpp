@@ -53,6 +53,9 @@ struct BoundaryDev {
   float sign;
   float half[3], half_outer[3];
   float outset, radius, max_distance;
+  // square of a radius around the body origin that holds the map's box (a
+  // point farther from x lies outside the map)
+  float map_radius2;
   GridGeomDev geom;
   const float* volume;  // device pointer to the volume map nodes
   // per map cell (x fastest): 1 when any of its 32 nodes is non-zero; a
