@@ -523,6 +523,10 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
   }
   const float reach = p.h * 1.00005f + mag * 9.5367431640625e-7f;
   const float reach2 = reach * reach;
+  // The particle itself lies in row 4 (dx = dy = 0), unless a periodic axis
+  // has fewer than 3 cells and another row wraps onto its own.
+  const bool self_any_row =
+      W && (((p.wrap_mask & 1) && p.res[0] < 3) || ((p.wrap_mask & 2) && p.res[1] < 3));
   // Appends neighbour j at slot min(cnt, K-1) (the last slot keeps the
   // latest neighbour): LDS slots below kListLds in the staged instance, else
   // the register group, stored when complete.
@@ -545,7 +549,7 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
   // branch-free and give a bit mask of hits, which are then appended in
   // ascending j.  X(j0, u) is candidate j0 + u; kClamp: X must not be read
   // past je - 1 (global memory), else reads past the span are harmless.
-  auto scan_span = [&](unsigned jb, unsigned je, auto&& X, auto clamp_tag) {
+  auto scan_span = [&](unsigned jb, unsigned je, auto&& X, auto clamp_tag, bool own_row) {
     constexpr bool kClamp = decltype(clamp_tag)::value;
     for (unsigned j0 = jb; j0 < je; j0 += kCB) {
       float4 xc[kCB];
@@ -557,10 +561,10 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
         const vec3 d = wrapd<W>(xi - ld3(xc[u]), p);
         hits |= unsigned(length_sqr(d) < p.h2) << u;
       }
-      // the batch's slots past the span and the particle itself, once per
-      // batch rather than per candidate
-      if (je - j0 < kCB) hits &= (1u << (je - j0)) - 1u;
-      if (s - j0 < kCB) hits &= ~(1u << (s - j0));
+      // the batch's slots past the span and, in the particle's own row, the
+      // particle itself, once per batch rather than per candidate
+      hits &= __builtin_amdgcn_ubfe(0xffu, 0u, min(je - j0, kCB));
+      if (own_row && s - j0 < kCB) hits &= ~(1u << (s - j0));
       if (kWaveStage && cnt + kCB <= min(kListLds, K)) {
         // Every candidate of the batch is written, without a branch, to the
         // slot after the batch's earlier hits: a miss is overwritten by the
@@ -621,9 +625,11 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
       wave_lds_sync();
       if (jb < je) {
         if (fits)
-          scan_span(jb, je, [&](unsigned j0, unsigned u) { return st[j0 - lo + u]; }, NoClamp());
+          scan_span(jb, je, [&](unsigned j0, unsigned u) { return st[j0 - lo + u]; }, NoClamp(),
+                    r == 4 || self_any_row);
         else
-          scan_span(jb, je, [&](unsigned j, unsigned) { return A.x[j]; }, Clamp());
+          scan_span(jb, je, [&](unsigned j, unsigned) { return A.x[j]; }, Clamp(),
+                    r == 4 || self_any_row);
       }
     } else {
       if (!row_ok) continue;
@@ -646,10 +652,10 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
       for (unsigned q = 0; q < nspan; ++q) {
         if (tiled_)
           scan_span(span_b[q], span_e[q], [&](unsigned j, unsigned) { return tile_.X(j); },
-                    Clamp());
+                    Clamp(), r == 4 || self_any_row);
         else
           scan_span(span_b[q], span_e[q], [&](unsigned j, unsigned) { return A.x[j]; },
-                    Clamp());
+                    Clamp(), r == 4 || self_any_row);
       }
     }
   }
