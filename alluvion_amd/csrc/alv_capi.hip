@@ -1456,7 +1456,7 @@ static int solver_alloc(alv_solver* S) {
     ALV_ALLOC(A.density_err, cap);
     ALV_ALLOC(A.pacc, cap);
   }
-  ALV_ALLOC(A.partials, grid_for(cap) + 1);
+  ALV_ALLOC(A.partials, size_t(kLPP) * grid_for(cap) + 1);  // per block of the passes
   ALV_ALLOC(A.fpartials, grid_for(cap) + 1);
   ALV_ALLOC(A.body_sums, std::max<size_t>(1, max_bnd) * 6);
   ALV_ALLOC(A.body_partials, std::max<size_t>(1, max_bnd) * 256 * 6);
@@ -2587,8 +2587,9 @@ int alv_node_positions(const float dmin[3], const unsigned res[3], const float c
 
 int alv_check_math(const alv_const* cn, int op, const float* a, const float* b, float* out,
                    size_t n) {
-  if (!cn || op < 0 || op > 3) return fail(ALV_ERR_ARG, "bad argument");
-  if (n && (!a || !out || (op == 1 && !b))) return fail(ALV_ERR_ARG, "null argument");
+  if (!cn || op < 0 || op > 6) return fail(ALV_ERR_ARG, "bad argument");
+  if (n && (!a || !out || ((op == 1 || op == 5) && !b)))
+    return fail(ALV_ERR_ARG, "null argument");
   if (!n) return ALV_OK;
   alv_consti cni;
   memset(&cni, 0, sizeof(cni));

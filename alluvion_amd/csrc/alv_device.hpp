@@ -3,10 +3,9 @@
 // Own vector type and inline functions used by every kernel.  Expression
 // order follows the reference kernels (kennychufk/alluvion runner.hpp) so that
 // the FP32 results are reproducible; all code is compiled with
-// -ffp-contract=off (no implicit fused multiply-add), FP32 sqrt and division
-// by a per-pair value are the correctly rounded HIP defaults, and division by
-// the kernel radius (a constant of the solve) goes through div_known, which
-// is correctly rounded too.
+// -ffp-contract=off (no implicit fused multiply-add), and the square roots
+// and divisions of the kernel functions (sqrt_rn, div_known, div_rn) are
+// correctly rounded.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -108,6 +107,24 @@ __device__ inline float sqrt_rn(float x) {
   return r_up > 0.0f ? s_up : t;
 }
 
+// Reciprocal rounded to nearest for a normal y whose reciprocal is normal:
+// one Newton step with an exact fused-multiply-add residual, from the
+// hardware estimate (v_rcp_f32, within one ulp).  By Markstein's reciprocal
+// theorem the step gives RN(1/y) from every estimate within one ulp, except
+// for a significand of all ones with the estimate on the power of two below
+// 1/y; checked on the CPU for every significand and every estimate within one
+// ulp, and on the GPU for every significand (the estimate depends on the
+// significand only).
+__device__ inline float rcp_rn(float y) {
+  const float r0 = __builtin_amdgcn_rcpf(y);
+  const float e = __builtin_fmaf(-y, r0, 1.0f);
+  return __builtin_fmaf(e, r0, r0);
+}
+// x / y rounded to nearest: the known-divisor sequence with y's correctly
+// rounded reciprocal (six instructions instead of the general sequence's
+// scaling and fix-up), for normal y and quotients in the normal range.
+__device__ inline float div_rn(float x, float y) { return div_known(x, y, rcp_rn(y)); }
+
 // --- cubic spline kernel (runner.hpp:596-638) ------------------------------
 __device__ inline float W_dist(float r, const SimParams& p) {
   float q = div_known(fabsf(r), p.h, p.inv_h);
@@ -130,20 +147,21 @@ __device__ inline float W_d2(float r2, const SimParams& p) {
   float result = q <= 0.5f ? inner : (q <= 1.0f ? outer : 0.0f);
   return result * p.k;
 }
-// gradW(d) = gradW_scale(d) * d.  (1 - q)(q - 1) / r is an IEEE division: a
-// faster sequence from the hardware reciprocal is not correctly rounded for
-// every radius, and choosing it at run time inside the neighbour loops cost
-// more than it saved.
+// gradW(d) = gradW_scale(d) * d.  The outer branch divides by r, which lies
+// in (h/2, h] where it is selected (elsewhere its quotient is discarded):
+// by div_rn, or by the compiler's general IEEE sequence (same bits).
+template <bool kRn = true>
 __device__ inline float gradW_scale(vec3 d, const SimParams& p) {
   float rl2 = length_sqr(d);
   float rl = sqrt_rn(rl2);
   float q = div_known(rl, p.h, p.inv_h);
   float s_in = div_known(q * 3.0f - 2.0f, p.h, p.inv_h);
-  float s_out = (1.0f - q) * (q - 1.0f) / rl;
+  float s_out = kRn ? div_rn((1.0f - q) * (q - 1.0f), rl) : (1.0f - q) * (q - 1.0f) / rl;
   float scale = q <= 0.5f ? s_in : (q <= 1.0f ? s_out : 0.0f);
   return p.l * scale;
 }
-__device__ inline vec3 gradW(vec3 d, const SimParams& p) { return gradW_scale(d, p) * d; }
+__device__ inline vec3 gradW(vec3 d, const SimParams& p) { return gradW_scale<false>(d, p) * d; }
+__device__ inline vec3 gradW_rn(vec3 d, const SimParams& p) { return gradW_scale<true>(d, p) * d; }
 
 // --- quaternion rotation (q = x, y, z, w) -----------------------------------
 __host__ __device__ inline vec3 rotate(vec3 v, float4 q) {

@@ -111,6 +111,10 @@ struct DevArrays {
 };
 
 constexpr unsigned kBlock = 256;
+// Lanes per particle of the density-change passes that gather neighbours
+// from global memory (2 or 4; the lanes of a particle lie in one quad).
+constexpr unsigned kLPP = 2;
+static_assert(kLPP == 2u || kLPP == 4u, "a particle's lanes lie in one quad");
 
 // Neighbour slots are stored in groups of four (one uint4 per particle and
 // group).  Each 64-particle wave owns one contiguous [groups][64] tile of
@@ -166,6 +170,21 @@ struct Launch {
   (A).ctl += scene_;                                     \
   (A).bnd += p.bnd_base
 #define ALV_SCENE(T, A, s) ALV_SCENE_AT(T, A, s, xcd_block())
+// Prologue of the density-change passes: with ONE (their LDS-tile
+// instances) one lane per particle as ALV_SCENE, else kLPP lanes per
+// particle, on a grid of kLPP blocks per particle block (each block 256 /
+// kLPP particles); q is the lane within the particle's lanes and qblk_ the
+// remapped block index.
+#define ALV_SCENE_PASS(T, A, s, q, ONE)                                               \
+  constexpr unsigned lpp_ = (ONE) ? 1u : kLPP;                                        \
+  const unsigned qblk_ = xcd_block();                                                 \
+  const unsigned q = threadIdx.x & (lpp_ - 1u);                                       \
+  const unsigned s = (qblk_ / lpp_) * kBlock + (qblk_ % lpp_) * (kBlock / lpp_) +     \
+                     threadIdx.x / lpp_;                                              \
+  const unsigned scene_ = (T).blk_scene[qblk_ / lpp_];                                \
+  const SimParams p = (T).par[scene_];                                                \
+  (A).ctl += scene_;                                                                  \
+  (A).bnd += p.bnd_base
 #define ALV_SCENE_LINEAR(T, A, s) ALV_SCENE_AT(T, A, s, blockIdx.x)
 // One block per scene (reductions over a scene's blocks).
 #define ALV_SCENE_BLOCK(T, A)                            \

@@ -50,7 +50,11 @@ constexpr unsigned kCB = 8;  // candidate batch of the neighbour search
 __device__ inline void batch_ids(const DevArrays& A, unsigned s, unsigned nn, unsigned t0,
                                  unsigned (&j)[kNB]) {
   static_assert(kNB == 4, "one neighbour group per batch");
-  const uint4 g = A.nbr[nbr_group(A, t0 >> 2, s)];
+  uint4 g = A.nbr[nbr_group(A, t0 >> 2, s)];
+  // The group stays one 16-byte load issued before the gathers: otherwise the
+  // compiler may load each later slot on its own under its count test, with
+  // a wait for each.
+  asm volatile("" : "+v"(g.x), "+v"(g.y), "+v"(g.z), "+v"(g.w));
   j[0] = t0 + 0 < nn ? g.x : s;
   j[1] = t0 + 1 < nn ? g.y : s;
   j[2] = t0 + 2 < nn ? g.z : s;
@@ -777,6 +781,114 @@ __device__ inline float density_change(const SimParams& p, const DevArrays& A, u
   return da;
 }
 
+// ---------------------------------------------------------------------------
+// Several lanes per particle.  The gathers of the neighbour passes are bound
+// by the vector L1's tag lookups: a wave64 gather is looked up four lanes at
+// a time, one lookup per distinct cache line of the four.  With one particle
+// per lane, the four lanes hold four particles whose lists drift apart, so
+// their j-th neighbours lie on two or three lines.  With kLPP lanes per
+// particle, lane q of a particle takes list slots kLPP k + q: consecutive
+// slots, mostly consecutive particles of one cell, on fewer lines.  Each lane
+// evaluates its slots' terms; every lane of the particle then adds the terms
+// in slot order (DPP broadcasts within the quad), so the sum is the
+// sequential sum over the list.
+// ---------------------------------------------------------------------------
+// The value of lane Q of this lane's particle.  Every lane of the quad is
+// active where it is called, so bound_ctrl never applies; it lets the
+// compiler fold the move into the add (v_add_f32_dpp).
+template <int Q>
+__device__ inline float lane_of_particle(float v) {
+  constexpr int ctl = kLPP == 4u ? Q * 0x55 : (Q | Q << 2 | (2 + Q) << 4 | (2 + Q) << 6);
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctl, 0xf, 0xf, true));
+}
+// acc + (the particle's lanes' t, in lane order)
+__device__ inline float add_in_slot_order(float acc, float t) {
+  acc += lane_of_particle<0>(t);
+  acc += lane_of_particle<1>(t);
+  if constexpr (kLPP == 4u) {
+    acc += lane_of_particle<2>(t);
+    acc += lane_of_particle<3>(t);
+  }
+  return acc;
+}
+// Slots t0 + q and t0 + kLPP + q of particle s's list, or s for a slot past
+// the count; t0 is a multiple of 2 kLPP below the count.  A group past the
+// list's last one is read from the last one (a valid address) and ignored.
+// The loads are issued before either slot is used (see batch_ids).
+__device__ inline void lane_slots(const DevArrays& A, unsigned s, unsigned q, unsigned t0,
+                                  unsigned nn, unsigned& ja, unsigned& jb) {
+  if constexpr (kLPP == 4u) {
+    const unsigned* g = reinterpret_cast<const unsigned*>(A.nbr + nbr_group(A, t0 >> 2, s)) + q;
+    const unsigned gb = min((t0 >> 2) + 1u, nbr_groups(A.kmax) - 1u) - (t0 >> 2);
+    ja = g[0];
+    jb = g[gb * 256u];  // 64 groups of 4 slots per group index
+    asm volatile("" : "+v"(ja), "+v"(jb));
+  } else {
+    uint4 g = A.nbr[nbr_group(A, t0 >> 2, s)];
+    asm volatile("" : "+v"(g.x), "+v"(g.y), "+v"(g.z), "+v"(g.w));
+    ja = q ? g.y : g.x;
+    jb = q ? g.w : g.z;
+  }
+  ja = t0 + q < nn ? ja : s;
+  jb = t0 + kLPP + q < nn ? jb : s;
+}
+
+// velocity_divergence over the fluid list, kLPP lanes per particle; the
+// boundary terms are added by every lane of the particle.  A slot past the
+// count holds the particle itself, whose term is an exact zero (x_i - x_i =
+// +0, so gradW is a signed zero and so is the term), so every slot's term is
+// added without a test: a sum that starts at +0 never becomes -0 under
+// round-to-nearest, and x + (+-0) = x for every other x.
+template <bool W>
+__device__ inline float velocity_divergence_q(const SimParams& p, const DevArrays& A,
+                                              unsigned s, unsigned q, vec3 xi, vec3 vi,
+                                              unsigned nn) {
+  float acc = 0.0f;
+  for (unsigned t0 = 0; t0 < nn; t0 += 2u * kLPP) {
+    unsigned ja, jb;
+    lane_slots(A, s, q, t0, nn, ja, jb);
+    const float4 xa = A.x[ja], va = A.v[ja], xb = A.x[jb], vb = A.v[jb];
+    const float ta = p.vol * dot(vi - ld3(va), gradW_rn(wrapd<W>(xi - ld3(xa), p), p));
+    const float tb = p.vol * dot(vi - ld3(vb), gradW_rn(wrapd<W>(xi - ld3(xb), p), p));
+    acc = add_in_slot_order(acc, ta);
+    acc = add_in_slot_order(acc, tb);
+  }
+  unsigned mask = A.bmask[s];
+  while (mask) {
+    const unsigned b = __builtin_ctz(mask);
+    mask &= mask - 1u;
+    const size_t o = size_t(b) * A.cap + s;
+    float4 r = A.brec[o], k = A.bker[o];
+    vec3 velj = rigid_velocity(A.bnd[b], ld3(r));
+    acc += dot(vi - velj, mk(k.x, k.y, k.z));
+  }
+  return acc;
+}
+template <bool W>
+__device__ inline float density_change_q(const SimParams& p, const DevArrays& A, unsigned s,
+                                         unsigned q) {
+  const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
+  const unsigned nn = A.nnb[s];
+  float da = velocity_divergence_q<W>(p, A, s, q, xi, vi, nn);
+  da = std_max(da, 0.0f);
+  if (nn < 20u) da = 0.0f;
+  return da;
+}
+
+// The density-change passes run with one lane per particle when they read
+// neighbours from the block's LDS tile (TL), else with kLPP lanes per
+// particle (global gathers); q is the lane within the particle's lanes.
+template <bool W, bool TL>
+__device__ inline float pass_density_change(const SimParams& p, const DevArrays& A,
+                                            unsigned s, unsigned q, bool tiled,
+                                            const TileXV& tile) {
+  if constexpr (TL)
+    return tiled ? density_change<W>(p, A, s, tile)
+                 : density_change<W>(p, A, s, GlobalXV{A.x, A.v});
+  else
+    return density_change_q<W>(p, A, s, q);
+}
+
 // compute_density_adv (runner.hpp:3493-3523)
 template <bool W, class Src>
 __device__ inline float density_adv(const SimParams& p, const DevArrays& A, unsigned s,
@@ -785,6 +897,19 @@ __device__ inline float density_adv(const SimParams& p, const DevArrays& A, unsi
   const unsigned nn = A.nnb[s];
   float delta = velocity_divergence<W>(p, A, s, xi, vi, nn, src);
   return std_max(A.density[s] / p.rho0 + dt * delta, 1.0f);
+}
+template <bool W, bool TL>
+__device__ inline float pass_density_adv(const SimParams& p, const DevArrays& A, unsigned s,
+                                         unsigned q, float dt, bool tiled, const TileXV& tile) {
+  if constexpr (TL) {
+    return tiled ? density_adv<W>(p, A, s, dt, tile)
+                 : density_adv<W>(p, A, s, dt, GlobalXV{A.x, A.v});
+  } else {
+    const vec3 xi = ld3(A.x[s]), vi = ld3(A.v[s]);
+    const unsigned nn = A.nnb[s];
+    float delta = velocity_divergence_q<W>(p, A, s, q, xi, vi, nn);
+    return std_max(A.density[s] / p.rho0 + dt * delta, 1.0f);
+  }
 }
 
 // Velocity change of one stiffness pass (warm_start_divergence_solve_1,
@@ -844,14 +969,14 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
 // F: also record the warm-start features of the particle
 template <bool W, bool F, bool TL>
 __global__ __launch_bounds__(256) void k_ws_div0(SceneTab T, DevArrays A) {
-  ALV_SCENE(T, A, s);
+  ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = tiled_ ? density_change<W>(p, A, s, tile_)
-                      : density_change<W>(p, A, s, GlobalXV{A.x, A.v});
+  const float dadv = pass_density_change<W, TL>(p, A, s, q, tiled_, tile_);
+  if (q != 0u) return;
   const float kv_prev = A.kappa_v[s];
   const float kv = (dadv > 0.0f) ? 0.5f * std_max(kv_prev, -0.5f) / dt : 0.0f;
   A.kappa_v[s] = kv;
@@ -958,14 +1083,14 @@ __global__ __launch_bounds__(256) void k_ws_div1(SceneTab T, DevArrays A) {
 // first iteration's stiffness k = rho_adv * alpha
 template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_vel_density_change(SceneTab T, DevArrays A) {
-  ALV_SCENE(T, A, s);
+  ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = tiled_ ? density_change<W>(p, A, s, tile_)
-                      : density_change<W>(p, A, s, GlobalXV{A.x, A.v});
+  const float dadv = pass_density_change<W, TL>(p, A, s, q, tiled_, tile_);
+  if (q != 0u) return;
   A.rho_adv[s] = dadv;
   float inv_dt = 1.0f / dt;
   float alpha = A.alpha[s] * inv_dt;
@@ -995,31 +1120,36 @@ __global__ __launch_bounds__(256) void k_div_iter(SceneTab T, DevArrays A) {
 template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_div_err(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
-  ALV_SCENE(T, A, s);
+  ALV_SCENE_PASS(T, A, s, q, TL);
   if (A.ctl->div_done) return;
   TileXV tile_;
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   double part = 0.0;
   if (s < p.first + p.n) {
-    float dadv = tiled_ ? density_change<W>(p, A, s, tile_)
-                        : density_change<W>(p, A, s, GlobalXV{A.x, A.v});
-    A.rho_adv[s] = dadv;
-    const float k = dadv * A.alpha[s];
-    A.kfac[s] = k;
-    set_kw(A, s, k);
-    part = double(dadv);
+    const float dadv = pass_density_change<W, TL>(p, A, s, q, tiled_, tile_);
+    if (q == 0u) {
+      A.rho_adv[s] = dadv;
+      const float k = dadv * A.alpha[s];
+      A.kfac[s] = k;
+      set_kw(A, s, k);
+      part = double(dadv);
+    }
   }
   double t = block_sum_d(part, lds);
-  if (threadIdx.x == 0) A.partials[xcd_block()] = t;
+  if (threadIdx.x == 0) A.partials[qblk_] = t;
 }
-__global__ __launch_bounds__(256) void k_div_finalize(SceneTab T, DevArrays A) {
+// lanes: blocks per particle block of the partial sums' pass (1 or kLPP)
+__global__ __launch_bounds__(256) void k_div_finalize(SceneTab T, DevArrays A, unsigned lanes) {
   __shared__ double lds[4];
   ALV_SCENE_BLOCK(T, A);
   StepCtl* ctl = A.ctl;
   if (ctl->div_done) return;
   double part = 0.0;
-  for (unsigned b = blk_begin + threadIdx.x; b < blk_end; b += blockDim.x) part += A.partials[b];
+  // unrolled so that the loads run ahead of the (ordered) additions
+#pragma unroll 8
+  for (unsigned b = lanes * blk_begin + threadIdx.x; b < lanes * blk_end; b += blockDim.x)
+    part += A.partials[b];
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) {
     float mean = float(t / double(p.n));
@@ -1200,14 +1330,14 @@ __global__ __launch_bounds__(256) void k_integrate_np(SceneTab T, DevArrays A) {
 // L: learned warm start (prediction from k_ws_div0_learned)
 template <bool W, bool L, bool TL>
 __global__ __launch_bounds__(256) void k_ws_p0(SceneTab T, DevArrays A) {
-  ALV_SCENE(T, A, s);
+  ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = tiled_ ? density_adv<W>(p, A, s, dt, tile_)
-                      : density_adv<W>(p, A, s, dt, GlobalXV{A.x, A.v});
+  const float dadv = pass_density_adv<W, TL>(p, A, s, q, dt, tiled_, tile_);
+  if (q != 0u) return;
   float k = (dadv > 1.0f) ? (0.5f * std_max(A.kappa[s], -0.00025f) / (dt * dt)) : 0.0f;
   if (L && dadv > 1.0f) {
     const float kl = A.kpred[s] / (dt * dt);
@@ -1232,14 +1362,14 @@ __global__ __launch_bounds__(256) void k_ws_p1(SceneTab T, DevArrays A) {
 // compute_rho_adv (runner.hpp:3525-3541); stores k = (rho_adv - 1) * alpha
 template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_rho_adv(SceneTab T, DevArrays A) {
-  ALV_SCENE(T, A, s);
+  ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  float dadv = tiled_ ? density_adv<W>(p, A, s, dt, tile_)
-                      : density_adv<W>(p, A, s, dt, GlobalXV{A.x, A.v});
+  const float dadv = pass_density_adv<W, TL>(p, A, s, q, dt, tiled_, tile_);
+  if (q != 0u) return;
   A.rho_adv[s] = dadv;
   float alpha = A.alpha[s] * (1.0f / (dt * dt));
   A.alpha[s] = alpha;
@@ -1268,7 +1398,7 @@ __global__ __launch_bounds__(256) void k_p_iter(SceneTab T, DevArrays A) {
 template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_p_err(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
-  ALV_SCENE(T, A, s);
+  ALV_SCENE_PASS(T, A, s, q, TL);
   if (A.ctl->dens_done) return;
   TileXV tile_;
   bool tiled_ = false;
@@ -1276,24 +1406,29 @@ __global__ __launch_bounds__(256) void k_p_err(SceneTab T, DevArrays A) {
   const float dt = A.ctl->dt;
   double part = 0.0;
   if (s < p.first + p.n) {
-    float dadv = tiled_ ? density_adv<W>(p, A, s, dt, tile_)
-                        : density_adv<W>(p, A, s, dt, GlobalXV{A.x, A.v});
-    A.rho_adv[s] = dadv;
-    const float k = (dadv - 1.0f) * A.alpha[s];
-    A.kfac[s] = k;
-    set_kw(A, s, k);
-    part = double(dadv - 1.0f);
+    const float dadv = pass_density_adv<W, TL>(p, A, s, q, dt, tiled_, tile_);
+    if (q == 0u) {
+      A.rho_adv[s] = dadv;
+      const float k = (dadv - 1.0f) * A.alpha[s];
+      A.kfac[s] = k;
+      set_kw(A, s, k);
+      part = double(dadv - 1.0f);
+    }
   }
   double t = block_sum_d(part, lds);
-  if (threadIdx.x == 0) A.partials[xcd_block()] = t;
+  if (threadIdx.x == 0) A.partials[qblk_] = t;
 }
-__global__ __launch_bounds__(256) void k_p_finalize(SceneTab T, DevArrays A) {
+// lanes: blocks per particle block of the partial sums' pass (1 or kLPP)
+__global__ __launch_bounds__(256) void k_p_finalize(SceneTab T, DevArrays A, unsigned lanes) {
   __shared__ double lds[4];
   ALV_SCENE_BLOCK(T, A);
   StepCtl* ctl = A.ctl;
   if (ctl->dens_done) return;
   double part = 0.0;
-  for (unsigned b = blk_begin + threadIdx.x; b < blk_end; b += blockDim.x) part += A.partials[b];
+  // unrolled so that the loads run ahead of the (ordered) additions
+#pragma unroll 8
+  for (unsigned b = lanes * blk_begin + threadIdx.x; b < lanes * blk_end; b += blockDim.x)
+    part += A.partials[b];
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) {
     float mean = float(t / double(p.n));
@@ -1327,13 +1462,13 @@ __global__ __launch_bounds__(256) void k_df_finish(SceneTab T, DevArrays A, int 
 // change sum alone (no stiffness factor update): rho_adv <- rho_dot.
 template <bool W, bool TL>
 __global__ __launch_bounds__(256) void k_density_change(SceneTab T, DevArrays A) {
-  ALV_SCENE(T, A, s);
+  ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   if (s >= p.first + p.n) return;
-  A.rho_adv[s] = tiled_ ? density_change<W>(p, A, s, tile_)
-                        : density_change<W>(p, A, s, GlobalXV{A.x, A.v});
+  const float dadv = pass_density_change<W, TL>(p, A, s, q, tiled_, tile_);
+  if (q == 0u) A.rho_adv[s] = dadv;
 }
 
 // Training pairs of the step just taken, in sorted order: the features
@@ -2334,6 +2469,39 @@ KernelProfiler* g_profiler = nullptr;
                          __VA_ARGS__);                                            \
     if (g_profiler) g_profiler->end(#kernel, s);                                  \
   } while (0)
+// density-change passes (ALV_SCENE_PASS): the tile instance on one block per
+// particle block, the others on kLPP blocks per particle block
+#define ALV_PASS_GRID(L, tiled) dim3((tiled) ? (L).pblocks : kLPP * (L).pblocks)
+#define ALV_LAUNCH_WTP(kernel, L, s, ...)                                         \
+  do {                                                                            \
+    if (g_profiler) g_profiler->begin(#kernel, s);                                \
+    if ((L).p0.wrap_mask)                                                         \
+      hipLaunchKernelGGL((kernel<true, false>), ALV_PASS_GRID(L, false), dim3(kBlock), 0, s, \
+                         (L).T, __VA_ARGS__);                                     \
+    else if ((L).tiles)                                                           \
+      hipLaunchKernelGGL((kernel<false, true>), ALV_PASS_GRID(L, true), dim3(kBlock), 0, s, \
+                         (L).T, __VA_ARGS__);                                     \
+    else                                                                          \
+      hipLaunchKernelGGL((kernel<false, false>), ALV_PASS_GRID(L, false), dim3(kBlock), 0, s, \
+                         (L).T, __VA_ARGS__);                                     \
+    if (g_profiler) g_profiler->end(#kernel, s);                                  \
+  } while (0)
+#define ALV_LAUNCH_W2TP(kernel, flag, L, s, ...)                                  \
+  do {                                                                            \
+    if (g_profiler) g_profiler->begin(#kernel, s);                                \
+    if ((L).p0.wrap_mask)                                                         \
+      hipLaunchKernelGGL((kernel<true, flag, false>), ALV_PASS_GRID(L, false), dim3(kBlock), 0, \
+                         s, (L).T, __VA_ARGS__);                                  \
+    else if ((L).tiles)                                                           \
+      hipLaunchKernelGGL((kernel<false, flag, true>), ALV_PASS_GRID(L, true), dim3(kBlock), 0, \
+                         s, (L).T, __VA_ARGS__);                                  \
+    else                                                                          \
+      hipLaunchKernelGGL((kernel<false, flag, false>), ALV_PASS_GRID(L, false), dim3(kBlock), \
+                         0, s, (L).T, __VA_ARGS__);                               \
+    if (g_profiler) g_profiler->end(#kernel, s);                                  \
+  } while (0)
+// blocks per particle block of the partial-sum passes just launched
+inline unsigned pass_lanes(const Launch& L) { return (L.tiles && !L.p0.wrap_mask) ? 1u : kLPP; }
 #define ALV_LAUNCH_W2T(kernel, flag, L, s, ...)                                   \
   do {                                                                            \
     if (g_profiler) g_profiler->begin(#kernel, s);                                \
@@ -2421,16 +2589,16 @@ void launch_df_warm_start_div(const Launch& L, const DevArrays& A, hipStream_t s
   else if (net)
     ALV_LAUNCH_W2(k_ws_div0_learned, false, L, s, A, *net, net_divergence);
   else if (features)
-    ALV_LAUNCH_W2T(k_ws_div0, true, L, s, A);
+    ALV_LAUNCH_W2TP(k_ws_div0, true, L, s, A);
   else
-    ALV_LAUNCH_W2T(k_ws_div0, false, L, s, A);
+    ALV_LAUNCH_W2TP(k_ws_div0, false, L, s, A);
   ALV_LAUNCH_WT(k_ws_div1, L, s, A);
-  ALV_LAUNCH_WT(k_vel_density_change, L, s, A);
+  ALV_LAUNCH_WTP(k_vel_density_change, L, s, A);
 }
 void launch_df_div_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_WT(k_div_iter, L, s, A);
-  ALV_LAUNCH_WT(k_div_err, L, s, A);
-  ALV_LAUNCH_SCENES(k_div_finalize, L, s, A);
+  ALV_LAUNCH_WTP(k_div_err, L, s, A);
+  ALV_LAUNCH_SCENES(k_div_finalize, L, s, A, pass_lanes(L));
 }
 void launch_df_viscosity(const Launch& L, const DevArrays& A, int div_finish, hipStream_t s) {
   ALV_LAUNCH_WT(k_viscosity, L, s, A, div_finish);
@@ -2444,16 +2612,16 @@ void launch_df_integrate_np(const Launch& L, const DevArrays& A, hipStream_t s) 
 void launch_df_warm_start_dens(const Launch& L, const DevArrays& A, hipStream_t s,
                                int learned) {
   if (learned)
-    ALV_LAUNCH_W2T(k_ws_p0, true, L, s, A);
+    ALV_LAUNCH_W2TP(k_ws_p0, true, L, s, A);
   else
-    ALV_LAUNCH_W2T(k_ws_p0, false, L, s, A);
+    ALV_LAUNCH_W2TP(k_ws_p0, false, L, s, A);
   ALV_LAUNCH_WT(k_ws_p1, L, s, A);
-  ALV_LAUNCH_WT(k_rho_adv, L, s, A);
+  ALV_LAUNCH_WTP(k_rho_adv, L, s, A);
 }
 void launch_df_dens_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_WT(k_p_iter, L, s, A);
-  ALV_LAUNCH_WT(k_p_err, L, s, A);
-  ALV_LAUNCH_SCENES(k_p_finalize, L, s, A);
+  ALV_LAUNCH_WTP(k_p_err, L, s, A);
+  ALV_LAUNCH_SCENES(k_p_finalize, L, s, A, pass_lanes(L));
 }
 void launch_df_finish(const Launch& L, const DevArrays& A, int dens_finish, hipStream_t s) {
   ALV_LAUNCH_W(k_df_finish, L, s, A, dens_finish);
@@ -2463,7 +2631,7 @@ void launch_export_pairs(const Launch& L, const DevArrays& A, float* x, float* y
   ALV_LAUNCH(k_export_pairs, L, s, A, x, y);
 }
 void launch_density_change(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_WT(k_density_change, L, s, A);
+  ALV_LAUNCH_WTP(k_density_change, L, s, A);
 }
 void launch_integrate_velocity(const Launch& L, const DevArrays& A, float dt, hipStream_t s) {
   ALV_LAUNCH_W(k_integrate_velocity, L, s, A, dt);
@@ -2551,8 +2719,12 @@ __global__ __launch_bounds__(256) void k_check_math(SimParams p, int op, const f
   switch (op) {
     case 0: r = sqrt_rn(x); break;
     case 1: r = div_known(x, b[i], 1.0f / b[i]); break;
-    case 2: r = gradW_scale(mk(x, 0.0f, 0.0f), p); break;
-    default: r = W_d2(x, p); break;
+    case 2: r = gradW_scale<true>(mk(x, 0.0f, 0.0f), p); break;
+    case 6: r = gradW_scale<false>(mk(x, 0.0f, 0.0f), p); break;
+    case 3: r = W_d2(x, p); break;
+    case 4: r = rcp_rn(x); break;
+    case 5: r = div_rn(x, b[i]); break;
+    default: break;
   }
   out[i] = r;
 }

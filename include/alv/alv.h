@@ -465,7 +465,10 @@ int alv_profiling_kernel(unsigned index, const char** name, double* total_ms,
  * host buffers a[n] (and b[n]), for callers that check it against IEEE
  * results: op 0 sqrt(a), op 1 a / b by the known-divisor division, op 2 the
  * cubic-spline gradient scale of the displacement (a, 0, 0), op 3 the cubic
- * spline W of the squared distance a, with the kernel radius of cn. */
+ * spline W of the squared distance a, with the kernel radius of cn; op 4
+ * 1 / a, op 5 a / b by the general division (the reciprocal refined to the
+ * correctly rounded one); op 6 the gradient scale of op 2 with the
+ * compiler's IEEE division in its outer branch. */
 int alv_check_math(const alv_const* cn, int op, const float* a, const float* b, float* out,
                    size_t n);
 

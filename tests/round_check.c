@@ -48,3 +48,58 @@ long long check_sqrt(float lo, float hi, float* bad) {
   }
   return miss;
 }
+
+/* the sequence of rcp_rn, from the hardware estimate r0 */
+static float rcp_from(float y, float r0) {
+  const float e = fmaf(-y, r0, 1.0f);
+  return fmaf(e, r0, r0);
+}
+
+/* Every significand y in [1, 2) and every estimate r0 within one ulp of 1/y
+ * (|r0 y - 1| < ulp(r0) y, exact in double): counts the (y, r0) whose Newton
+ * step misses RN(1/y) = 1.0f / y, the last one's y in *bad and r0 in
+ * *bad_r0, and the estimates tried in *tried.  The reciprocal's significand
+ * depends on y's significand only, so one binade covers every normal y whose
+ * reciprocal is normal. */
+long long check_rcp(float* bad, float* bad_r0, long long* tried) {
+  long long miss = 0, n = 0;
+  for (uint32_t m = 0; m < (1u << 23); ++m) {
+    const float y = from_bits(0x3f800000u | m);
+    const float want = 1.0f / y;
+    for (int k = -2; k <= 2; ++k) {
+      const float r0 = from_bits(bits(want) + (uint32_t)k);
+      const double ulp = ldexp(1.0, ilogb(r0) - 23);
+      if (fabs((double)r0 * (double)y - 1.0) >= ulp * (double)y) continue;
+      ++n;
+      if (bits(rcp_from(y, r0)) != bits(want)) {
+        *bad = y;
+        *bad_r0 = r0;
+        ++miss;
+      }
+    }
+  }
+  *tried = n;
+  return miss;
+}
+
+/* The gradient's outer branch (1 - q)(q - 1) / r, q = RN(r / h), for every
+ * float r in (h/2, h], by the known-divisor sequence with r's correctly
+ * rounded reciprocal (div_rn); returns mismatches against the IEEE division,
+ * the first one's r in *bad. */
+long long check_div_gradient(float h, float* bad) {
+  long long miss = 0;
+  for (uint32_t u = bits(0.5f * h) + 1u; u <= bits(h); ++u) {
+    const float r = from_bits(u);
+    const float q = r / h;
+    const float x = (1.0f - q) * (q - 1.0f);
+    const float want = x / r;
+    const float rr = 1.0f / r;
+    const float q0 = x * rr;
+    const float rem = fmaf(-q0, r, x);
+    if (bits(fmaf(rem, rr, q0)) != bits(want)) {
+      if (!miss) *bad = r;
+      ++miss;
+    }
+  }
+  return miss;
+}

@@ -1,10 +1,11 @@
 """The GPU's arithmetic under the kernel functions, bit for bit against IEEE
 single precision computed by numpy (correctly rounded, no fused operations):
-the square root over random and edge arguments, the known-divisor division by
-h and h^2, and the cubic spline W and gradient scale over every distance of
-the gradient's dividing branch (1/2 < r/h <= 1) and a dense sample of (0, h].
-The CPU checks of the same sequences are in tests/test_round_check.py and
-tests/test_div_known.py."""
+the square root over random and edge arguments, the reciprocal over every
+significand (three binades) and the general division over random pairs, the
+known-divisor division by h and h^2, and the cubic spline W and gradient
+scale over every distance of the gradient's dividing branch (1/2 < r/h <= 1)
+and a dense sample of (0, h].  The CPU checks of the same sequences are in
+tests/test_round_check.py and tests/test_div_known.py."""
 import ctypes
 
 import numpy as np
@@ -82,14 +83,35 @@ def test_sqrt_matches_ieee():
     assert_bits(run(L, cn, 0, x), np.sqrt(x), "sqrt")
 
 
+def test_reciprocal_every_significand():
+    L = alv_ctypes.library()
+    cn = alv_ctypes.const_for(L, 0.01)
+    sig = np.arange(1 << 23, dtype=np.uint32)
+    for e in (127, 127 - 7, 127 + 10):  # [1, 2), [2^-7, 2^-6), [2^10, 2^11)
+        y = ((np.uint32(e) << np.uint32(23)) | sig).view(F32)
+        assert_bits(run(L, cn, 4, y), F32(1.0) / y, "1 / y, exponent %d" % (e - 127))
+
+
+def test_division_random_pairs():
+    L = alv_ctypes.library()
+    cn = alv_ctypes.const_for(L, 0.01)
+    rng = np.random.default_rng(7)
+    n = 1 << 22
+    a = (rng.uniform(1.0, 2.0, n) * 2.0 ** rng.integers(-40, 40, n)
+         * rng.choice([-1.0, 1.0], n)).astype(F32)
+    b = (rng.uniform(1.0, 2.0, n) * 2.0 ** rng.integers(-40, 40, n)).astype(F32)
+    assert_bits(run(L, cn, 5, a, b), a / b, "a / b")
+
+
+@pytest.mark.parametrize("op", [2, 6])
 @pytest.mark.parametrize("h", [0.01, 2.0 ** -7])
-def test_gradient_scale(h):
+def test_gradient_scale(h, op):
     L = alv_ctypes.library()
     cn = alv_ctypes.const_for(L, h)
     rs = np.concatenate([gradient_branch(h), (np.arange(1, 1 << 20, dtype=np.float64) /
                                                (1 << 20) * float(h)).astype(F32)])
-    assert_bits(run(L, cn, 2, rs), numpy_grad_scale(rs, h, cn.cubic_kernel_l),
-                "gradient scale h=%g" % h)
+    assert_bits(run(L, cn, op, rs), numpy_grad_scale(rs, h, cn.cubic_kernel_l),
+                "gradient scale h=%g (op %d)" % (h, op))
 
 
 @pytest.mark.parametrize("h", [0.01, 0.037])
