@@ -1456,7 +1456,7 @@ static int solver_alloc(alv_solver* S) {
     ALV_ALLOC(A.density_err, cap);
     ALV_ALLOC(A.pacc, cap);
   }
-  ALV_ALLOC(A.partials, size_t(kLPP) * grid_for(cap) + 1);  // per block of the passes
+  ALV_ALLOC(A.partials, grid_for(cap) + 1);
   ALV_ALLOC(A.fpartials, grid_for(cap) + 1);
   ALV_ALLOC(A.body_sums, std::max<size_t>(1, max_bnd) * 6);
   ALV_ALLOC(A.body_partials, std::max<size_t>(1, max_bnd) * 256 * 6);

@@ -172,16 +172,14 @@ struct Launch {
 #define ALV_SCENE(T, A, s) ALV_SCENE_AT(T, A, s, xcd_block())
 // Prologue of the density-change passes: with ONE (their LDS-tile
 // instances) one lane per particle as ALV_SCENE, else kLPP lanes per
-// particle, on a grid of kLPP blocks per particle block (each block 256 /
-// kLPP particles); q is the lane within the particle's lanes and qblk_ the
-// remapped block index.
+// particle in blocks of kLPP * 256 threads (one block per particle block);
+// q is the lane within the particle's lanes.
 #define ALV_SCENE_PASS(T, A, s, q, ONE)                                               \
   constexpr unsigned lpp_ = (ONE) ? 1u : kLPP;                                        \
   const unsigned qblk_ = xcd_block();                                                 \
   const unsigned q = threadIdx.x & (lpp_ - 1u);                                       \
-  const unsigned s = (qblk_ / lpp_) * kBlock + (qblk_ % lpp_) * (kBlock / lpp_) +     \
-                     threadIdx.x / lpp_;                                              \
-  const unsigned scene_ = (T).blk_scene[qblk_ / lpp_];                                \
+  const unsigned s = qblk_ * kBlock + threadIdx.x / lpp_;                             \
+  const unsigned scene_ = (T).blk_scene[qblk_];                                       \
   const SimParams p = (T).par[scene_];                                                \
   (A).ctl += scene_;                                                                  \
   (A).bnd += p.bnd_base

@@ -968,7 +968,7 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
 // warm_start_divergence_solve_0 (runner.hpp:3230-3250)
 // F: also record the warm-start features of the particle
 template <bool W, bool F, bool TL>
-__global__ __launch_bounds__(256) void k_ws_div0(SceneTab T, DevArrays A) {
+__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock) void k_ws_div0(SceneTab T, DevArrays A) {
   ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
@@ -1082,7 +1082,7 @@ __global__ __launch_bounds__(256) void k_ws_div1(SceneTab T, DevArrays A) {
 // compute_velocity_of_density_change (runner.hpp:3297-3314); also stores the
 // first iteration's stiffness k = rho_adv * alpha
 template <bool W, bool TL>
-__global__ __launch_bounds__(256) void k_vel_density_change(SceneTab T, DevArrays A) {
+__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock) void k_vel_density_change(SceneTab T, DevArrays A) {
   ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
@@ -1118,8 +1118,8 @@ __global__ __launch_bounds__(256) void k_div_iter(SceneTab T, DevArrays A) {
 // compute_divergence_solve_density_error (runner.hpp:3396-3410) + FP64
 // partial sums of the residual
 template <bool W, bool TL>
-__global__ __launch_bounds__(256) void k_div_err(SceneTab T, DevArrays A) {
-  __shared__ double lds[4];
+__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock) void k_div_err(SceneTab T, DevArrays A) {
+  __shared__ double lds[kLPP * kBlock / 64];
   ALV_SCENE_PASS(T, A, s, q, TL);
   if (A.ctl->div_done) return;
   TileXV tile_;
@@ -1139,17 +1139,13 @@ __global__ __launch_bounds__(256) void k_div_err(SceneTab T, DevArrays A) {
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) A.partials[qblk_] = t;
 }
-// lanes: blocks per particle block of the partial sums' pass (1 or kLPP)
-__global__ __launch_bounds__(256) void k_div_finalize(SceneTab T, DevArrays A, unsigned lanes) {
+__global__ __launch_bounds__(256) void k_div_finalize(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
   ALV_SCENE_BLOCK(T, A);
   StepCtl* ctl = A.ctl;
   if (ctl->div_done) return;
   double part = 0.0;
-  // unrolled so that the loads run ahead of the (ordered) additions
-#pragma unroll 8
-  for (unsigned b = lanes * blk_begin + threadIdx.x; b < lanes * blk_end; b += blockDim.x)
-    part += A.partials[b];
+  for (unsigned b = blk_begin + threadIdx.x; b < blk_end; b += blockDim.x) part += A.partials[b];
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) {
     float mean = float(t / double(p.n));
@@ -1329,7 +1325,7 @@ __global__ __launch_bounds__(256) void k_integrate_np(SceneTab T, DevArrays A) {
 // warm_start_pressure_solve0 (runner.hpp:3430-3450)
 // L: learned warm start (prediction from k_ws_div0_learned)
 template <bool W, bool L, bool TL>
-__global__ __launch_bounds__(256) void k_ws_p0(SceneTab T, DevArrays A) {
+__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock) void k_ws_p0(SceneTab T, DevArrays A) {
   ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
@@ -1361,7 +1357,7 @@ __global__ __launch_bounds__(256) void k_ws_p1(SceneTab T, DevArrays A) {
 }
 // compute_rho_adv (runner.hpp:3525-3541); stores k = (rho_adv - 1) * alpha
 template <bool W, bool TL>
-__global__ __launch_bounds__(256) void k_rho_adv(SceneTab T, DevArrays A) {
+__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock) void k_rho_adv(SceneTab T, DevArrays A) {
   ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
@@ -1396,8 +1392,8 @@ __global__ __launch_bounds__(256) void k_p_iter(SceneTab T, DevArrays A) {
 // compute_pressure_solve_density_error (runner.hpp:3590-3604) + FP64
 // partial sums of (rho_adv - 1)
 template <bool W, bool TL>
-__global__ __launch_bounds__(256) void k_p_err(SceneTab T, DevArrays A) {
-  __shared__ double lds[4];
+__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock) void k_p_err(SceneTab T, DevArrays A) {
+  __shared__ double lds[kLPP * kBlock / 64];
   ALV_SCENE_PASS(T, A, s, q, TL);
   if (A.ctl->dens_done) return;
   TileXV tile_;
@@ -1418,17 +1414,13 @@ __global__ __launch_bounds__(256) void k_p_err(SceneTab T, DevArrays A) {
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) A.partials[qblk_] = t;
 }
-// lanes: blocks per particle block of the partial sums' pass (1 or kLPP)
-__global__ __launch_bounds__(256) void k_p_finalize(SceneTab T, DevArrays A, unsigned lanes) {
+__global__ __launch_bounds__(256) void k_p_finalize(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
   ALV_SCENE_BLOCK(T, A);
   StepCtl* ctl = A.ctl;
   if (ctl->dens_done) return;
   double part = 0.0;
-  // unrolled so that the loads run ahead of the (ordered) additions
-#pragma unroll 8
-  for (unsigned b = lanes * blk_begin + threadIdx.x; b < lanes * blk_end; b += blockDim.x)
-    part += A.partials[b];
+  for (unsigned b = blk_begin + threadIdx.x; b < blk_end; b += blockDim.x) part += A.partials[b];
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) {
     float mean = float(t / double(p.n));
@@ -1461,7 +1453,7 @@ __global__ __launch_bounds__(256) void k_df_finish(SceneTab T, DevArrays A, int 
 // compute_velocity_of_density_change (runner.hpp:3297-3314), the density
 // change sum alone (no stiffness factor update): rho_adv <- rho_dot.
 template <bool W, bool TL>
-__global__ __launch_bounds__(256) void k_density_change(SceneTab T, DevArrays A) {
+__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock) void k_density_change(SceneTab T, DevArrays A) {
   ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
@@ -2469,20 +2461,20 @@ KernelProfiler* g_profiler = nullptr;
                          __VA_ARGS__);                                            \
     if (g_profiler) g_profiler->end(#kernel, s);                                  \
   } while (0)
-// density-change passes (ALV_SCENE_PASS): the tile instance on one block per
-// particle block, the others on kLPP blocks per particle block
-#define ALV_PASS_GRID(L, tiled) dim3((tiled) ? (L).pblocks : kLPP * (L).pblocks)
+// density-change passes (ALV_SCENE_PASS): one block per particle block, of
+// 256 threads for the tile instance and kLPP * 256 for the others
+#define ALV_PASS_BLOCK(tiled) dim3((tiled) ? kBlock : kLPP * kBlock)
 #define ALV_LAUNCH_WTP(kernel, L, s, ...)                                         \
   do {                                                                            \
     if (g_profiler) g_profiler->begin(#kernel, s);                                \
     if ((L).p0.wrap_mask)                                                         \
-      hipLaunchKernelGGL((kernel<true, false>), ALV_PASS_GRID(L, false), dim3(kBlock), 0, s, \
+      hipLaunchKernelGGL((kernel<true, false>), dim3((L).pblocks), ALV_PASS_BLOCK(false), 0, s, \
                          (L).T, __VA_ARGS__);                                     \
     else if ((L).tiles)                                                           \
-      hipLaunchKernelGGL((kernel<false, true>), ALV_PASS_GRID(L, true), dim3(kBlock), 0, s, \
+      hipLaunchKernelGGL((kernel<false, true>), dim3((L).pblocks), ALV_PASS_BLOCK(true), 0, s, \
                          (L).T, __VA_ARGS__);                                     \
     else                                                                          \
-      hipLaunchKernelGGL((kernel<false, false>), ALV_PASS_GRID(L, false), dim3(kBlock), 0, s, \
+      hipLaunchKernelGGL((kernel<false, false>), dim3((L).pblocks), ALV_PASS_BLOCK(false), 0, s, \
                          (L).T, __VA_ARGS__);                                     \
     if (g_profiler) g_profiler->end(#kernel, s);                                  \
   } while (0)
@@ -2490,18 +2482,16 @@ KernelProfiler* g_profiler = nullptr;
   do {                                                                            \
     if (g_profiler) g_profiler->begin(#kernel, s);                                \
     if ((L).p0.wrap_mask)                                                         \
-      hipLaunchKernelGGL((kernel<true, flag, false>), ALV_PASS_GRID(L, false), dim3(kBlock), 0, \
+      hipLaunchKernelGGL((kernel<true, flag, false>), dim3((L).pblocks), ALV_PASS_BLOCK(false), 0, \
                          s, (L).T, __VA_ARGS__);                                  \
     else if ((L).tiles)                                                           \
-      hipLaunchKernelGGL((kernel<false, flag, true>), ALV_PASS_GRID(L, true), dim3(kBlock), 0, \
+      hipLaunchKernelGGL((kernel<false, flag, true>), dim3((L).pblocks), ALV_PASS_BLOCK(true), 0, \
                          s, (L).T, __VA_ARGS__);                                  \
     else                                                                          \
-      hipLaunchKernelGGL((kernel<false, flag, false>), ALV_PASS_GRID(L, false), dim3(kBlock), \
+      hipLaunchKernelGGL((kernel<false, flag, false>), dim3((L).pblocks), ALV_PASS_BLOCK(false), \
                          0, s, (L).T, __VA_ARGS__);                               \
     if (g_profiler) g_profiler->end(#kernel, s);                                  \
   } while (0)
-// blocks per particle block of the partial-sum passes just launched
-inline unsigned pass_lanes(const Launch& L) { return (L.tiles && !L.p0.wrap_mask) ? 1u : kLPP; }
 #define ALV_LAUNCH_W2T(kernel, flag, L, s, ...)                                   \
   do {                                                                            \
     if (g_profiler) g_profiler->begin(#kernel, s);                                \
@@ -2598,7 +2588,7 @@ void launch_df_warm_start_div(const Launch& L, const DevArrays& A, hipStream_t s
 void launch_df_div_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_WT(k_div_iter, L, s, A);
   ALV_LAUNCH_WTP(k_div_err, L, s, A);
-  ALV_LAUNCH_SCENES(k_div_finalize, L, s, A, pass_lanes(L));
+  ALV_LAUNCH_SCENES(k_div_finalize, L, s, A);
 }
 void launch_df_viscosity(const Launch& L, const DevArrays& A, int div_finish, hipStream_t s) {
   ALV_LAUNCH_WT(k_viscosity, L, s, A, div_finish);
@@ -2621,7 +2611,7 @@ void launch_df_warm_start_dens(const Launch& L, const DevArrays& A, hipStream_t 
 void launch_df_dens_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_WT(k_p_iter, L, s, A);
   ALV_LAUNCH_WTP(k_p_err, L, s, A);
-  ALV_LAUNCH_SCENES(k_p_finalize, L, s, A, pass_lanes(L));
+  ALV_LAUNCH_SCENES(k_p_finalize, L, s, A);
 }
 void launch_df_finish(const Launch& L, const DevArrays& A, int dens_finish, hipStream_t s) {
   ALV_LAUNCH_W(k_df_finish, L, s, A, dens_finish);
