@@ -1186,14 +1186,16 @@ __device__ inline vec3 viscosity_sum(const SimParams& p, const DevArrays& A, uns
       xj[u] = src.X(j[u]);
       vj[u] = src.V(j[u]);
     }
+    // An empty slot holds the particle itself: x_ij = 0 gives a zero
+    // numerator and a zero velocity difference, so its term is an exact zero
+    // and every slot is added without a test (da never becomes -0).  Both
+    // divisors are normal (rho_j, |x_ij|^2 + 0.01 h^2): div_rn.
 #pragma unroll
     for (int u = 0; u < kNB; ++u) {
-      if (t0 + u < nn) {
-        vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
-        da += (p.visc * p.mass / vj[u].w * dot(xixj, gradW(xixj, p)) /
-               (length_sqr(xixj) + 0.01f * p.h2)) *
-              (vi - ld3(vj[u]));
-      }
+      vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+      da += div_rn(div_rn(p.visc * p.mass, vj[u].w) * dot(xixj, gradW_rn(xixj, p)),
+                   length_sqr(xixj) + 0.01f * p.h2) *
+            (vi - ld3(vj[u]));
     }
   }
   return da;
