@@ -695,16 +695,18 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
     float4 xj[kNB];
 #pragma unroll
     for (int u = 0; u < kNB; ++u) xj[u] = tiled_ ? tile_.X(jn[u]) : A.x[jn[u]];
+    // An empty slot holds the particle itself (d = 0): its W term is
+    // replaced by +0 and its gradient terms are signed zeros, so every slot
+    // is added without a branch (sums that start at +0 never become -0).
 #pragma unroll
     for (int u = 0; u < kNB; ++u) {
-      if (t0 + u < nn) {
-        vec3 d = wrapd<W>(xi - ld3(xj[u]), p);
-        dens += p.vol * W_d2(length_sqr(d), p);
-        if (with_factor) {
-          vec3 gp = (-p.vol) * gradW(d, p);
-          sgk += length_sqr(gp);
-          gpi -= gp;
-        }
+      vec3 d = wrapd<W>(xi - ld3(xj[u]), p);
+      const float w = W_d2(length_sqr(d), p);
+      dens += p.vol * (t0 + u < nn ? w : 0.0f);
+      if (with_factor) {
+        vec3 gp = (-p.vol) * gradW_rn(d, p);
+        sgk += length_sqr(gp);
+        gpi -= gp;
       }
     }
   }
