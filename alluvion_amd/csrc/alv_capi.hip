@@ -275,6 +275,7 @@ EventProfiler* g_event_profiler = nullptr;
 struct alv_pile {
   std::vector<BoundaryDev> bnd;  // kinematics live in bnd[b].x/v/q/omega
   std::vector<float*> volumes;
+  std::vector<float*> distances;  // distance maps (ALV_DISTANCE_MAP), else null
   std::vector<unsigned char*> cell_nz;  // non-zero flags of the map cells
   // set when a caller got a writable pointer to a volume map: the flags are
   // recomputed before the next step
@@ -316,6 +317,7 @@ struct alv_pile {
   size_t kin_cap = 0;
   ~alv_pile() {
     for (float* v : volumes) (void)hipFree(v);
+    for (float* d : distances) (void)hipFree(d);
     for (unsigned char* f : cell_nz) (void)hipFree(f);
     if (peirce) (void)hipFree(peirce);
     if (verts_dev) (void)hipFree(verts_dev);
@@ -742,6 +744,20 @@ int place_boundary(alv_pile* pile, int slot, const alv_const* cn, const alv_dist
       aabb_max[a] = B.radius;
     }
     B.max_distance = B.radius;
+  } else if (dist->type == ALV_DISTANCE_MAP) {
+    if (!dist->signed_distance) return fail(ALV_ERR_ARG, "distance map without a function");
+    if (pile->volume_method == ALV_PELLETS)
+      return fail(ALV_ERR_UNSUPPORTED, "distance maps in a pellet pile");
+    // the farthest corner of the bounding box from the body origin
+    float r2 = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      aabb_min[a] = dist->aabb_min[a];
+      aabb_max[a] = dist->aabb_max[a];
+      if (!(aabb_max[a] > aabb_min[a])) return fail(ALV_ERR_ARG, "empty bounding box");
+      const float m = std::max(std::fabs(aabb_min[a]), std::fabs(aabb_max[a]));
+      r2 += m * m;
+    }
+    B.max_distance = sqrtf(r2);
   } else {
     return fail(ALV_ERR_UNSUPPORTED, "distance type not supported");
   }
@@ -788,6 +804,36 @@ int place_boundary(alv_pile* pile, int slot, const alv_const* cn, const alv_dist
     if (rc != ALV_OK) return rc;
   }
   B.volume = vol;
+  float* dmap = nullptr;
+  if (dist->type == ALV_DISTANCE_MAP) {
+    // construct_distance_grid (pile.hpp:79-95): the distance at every node
+    int rc = dev_alloc(&dmap, nodes);
+    if (rc != ALV_OK) {
+      (void)hipFree(vol);
+      return rc;
+    }
+    const unsigned chunk = 1u << 18;
+    std::vector<float> pts(size_t(chunk) * 3), val(chunk);
+    for (unsigned b0 = 0; b0 < B.geom.num_nodes && rc == ALV_OK; b0 += chunk) {
+      const unsigned m = std::min(chunk, B.geom.num_nodes - b0);
+      for (unsigned k = 0; k < m; ++k) {
+        const vec3 xn = node_position(B.geom, b0 + k);
+        pts[3 * size_t(k)] = xn.x;
+        pts[3 * size_t(k) + 1] = xn.y;
+        pts[3 * size_t(k) + 2] = xn.z;
+      }
+      dist->signed_distance(dist->context, pts.data(), m, val.data());
+      if (hipMemcpy(dmap + b0, val.data(), size_t(m) * sizeof(float), hipMemcpyHostToDevice) !=
+          hipSuccess)
+        rc = fail(ALV_ERR_HIP, "distance map upload");
+    }
+    if (rc != ALV_OK) {
+      (void)hipFree(vol);
+      (void)hipFree(dmap);
+      return rc;
+    }
+  }
+  B.distance = dmap;
   for (int a = 0; a < 3; ++a) {
     B.x[a] = x ? x[a] : 0.f;
     B.v[a] = 0.f;
@@ -804,7 +850,10 @@ int place_boundary(alv_pile* pile, int slot, const alv_const* cn, const alv_dist
   const unsigned chunk = 1u << 20;
   for (unsigned b = 0; !pellets && b < B.geom.num_nodes; b += chunk) {
     unsigned e = std::min(B.geom.num_nodes, b + chunk);
-    launch_volume_field_analytic(p, B, vol, pile->peirce, b, e, 0);
+    if (dmap)  // update_volume_field from the distance map (runner.hpp:1616-1658)
+      launch_volume_field_grid(p, B.geom, dmap, vol, sign, pile->peirce, b, e, 0);
+    else
+      launch_volume_field_analytic(p, B, vol, pile->peirce, b, e, 0);
     ALV_HIP(hipGetLastError());
   }
   unsigned char* flags = nullptr;
@@ -820,6 +869,7 @@ int place_boundary(alv_pile* pile, int slot, const alv_const* cn, const alv_dist
   if (slot < 0) {
     pile->bnd.push_back(B);
     pile->volumes.push_back(vol);
+    pile->distances.push_back(dmap);
     pile->cell_nz.push_back(flags);
     pile->restitution.push_back(restitution);
     pile->friction.push_back(friction);
@@ -833,9 +883,11 @@ int place_boundary(alv_pile* pile, int slot, const alv_const* cn, const alv_dist
     // map and kinematics take effect from the next step
     ALV_HIP(hipDeviceSynchronize());
     if (pile->volumes[slot]) (void)hipFree(pile->volumes[slot]);
+    if (pile->distances[slot]) (void)hipFree(pile->distances[slot]);
     if (pile->cell_nz[slot]) (void)hipFree(pile->cell_nz[slot]);
     pile->bnd[slot] = B;
     pile->volumes[slot] = vol;
+    pile->distances[slot] = dmap;
     pile->cell_nz[slot] = flags;
     pile->restitution[slot] = restitution;
     pile->friction[slot] = friction;
@@ -1359,6 +1411,15 @@ int alv_pile_get_volume_grid(const alv_pile* pile, unsigned b, float** nodes,
     if (pile->volumes[b]) pile->maps_dirty = true;  // the caller may write the nodes
   }
   if (num_nodes) *num_nodes = pile->bnd[b].geom.num_nodes;
+  return ALV_OK;
+}
+
+int alv_pile_get_distance_grid(const alv_pile* pile, unsigned b, float** nodes,
+                               unsigned* num_nodes) {
+  if (!pile || b >= pile->bnd.size()) return fail(ALV_ERR_ARG, "bad boundary id");
+  const bool map = pile->distances[b] != nullptr;
+  if (nodes) *nodes = pile->distances[b];
+  if (num_nodes) *num_nodes = map ? pile->bnd[b].geom.num_nodes : 0u;
   return ALV_OK;
 }
 

@@ -82,10 +82,29 @@ __device__ bool test_vertex(unsigned i, unsigned j, unsigned flags, unsigned nb,
   const vec3 vx = mk(verts[3 * vert], verts[3 * vert + 1], verts[3 * vert + 2]);
   const vec3 cp_i = (flags & kJobWorld) ? vx : rotate(vx, qi) + xi;
   const vec3 wrt_j = rotate(cp_i - xj, conjugate(qj));
-  const float dist = sdf_eval(Bj, wrt_j) * Bj.sign - tolerance;
-  vec3 n = sdf_gradient(Bj, wrt_j, 1e-3f) * Bj.sign;
-  n = n * (1.0f / sqrtf(length_sqr(n)));
+  float dist;
+  vec3 n;
+  if (Bj.type == ALV_DIST_MAP) {
+    // collision_find_dist_normal (runner.hpp:1794-1825): outside the map
+    // only the sign is known (no contact for a positive sign)
+    dist = Bj.sign;
+    n = mk(0.f, 0.f, 0.f);
+    int ip[3];
+    vec3 inner;
+    if (grid_resolve(Bj.geom, wrt_j, ip, inner)) {
+      unsigned cell[32];
+      grid_cell_nodes(Bj.geom.res, ip, cell);
+      vec3 g;
+      dist = Bj.sign * grid_interpolate_and_derive(Bj.distance, cell, inner, Bj.geom.cs, g) -
+             tolerance;
+      n = g * Bj.sign;
+    }
+  } else {
+    dist = sdf_eval(Bj, wrt_j) * Bj.sign - tolerance;
+    n = sdf_gradient(Bj, wrt_j, 1e-3f) * Bj.sign;
+  }
   if (!(dist < 0)) return false;
+  n = n * (1.0f / sqrtf(length_sqr(n)));
   const vec3 cp_j = rotate(wrt_j - dist * n, qj) + xj;
   const vec3 nw = rotate(n, qj);
   const vec3 r_i = cp_i - xi, r_j = cp_j - xj;

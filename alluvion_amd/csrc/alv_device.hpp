@@ -8,6 +8,7 @@
 // correctly rounded.
 #pragma once
 
+#include <float.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -188,7 +189,12 @@ __device__ inline float box_dist(vec3 x, vec3 half) {
   vec3 cl = mk(fmaxf(diff.x, 0.0f), fmaxf(diff.y, 0.0f), fmaxf(diff.z, 0.0f));
   return sqrtf(length_sqr(cl)) + fminf(fmaxf(diff.x, fmaxf(diff.y, diff.z)), 0.0f);
 }
+__device__ inline float map_distance(const BoundaryDev& b, vec3 x);
+// Signed distance of boundary b at x (body frame, before the boundary's
+// sign); a distance map gives FLT_MAX outside the map
+// (interpolate_distance_without_intermediates, runner.hpp:1596-1614).
 __device__ inline float sdf_eval(const BoundaryDev& b, vec3 x) {
+  if (b.type == ALV_DIST_MAP) return map_distance(b, x);
   if (b.type == ALV_DIST_BOX) return box_dist(x, mk(b.half[0], b.half[1], b.half[2])) - b.outset;
   if (b.type == ALV_DIST_BOX_SHELL) {
     float inner = box_dist(x, mk(b.half[0], b.half[1], b.half[2])) - b.outset;
@@ -352,6 +358,77 @@ __device__ inline float grid_interpolate(const float* __restrict__ nodes,
   return phi;
 }
 
+// Value and gradient (in world units: d/dx = (2 / cell size) d/dxi) of the
+// 32-node element at xi (interpolate_and_derive, runner.hpp:1773-1791).  The
+// shape functions' derivatives follow from the closed forms above:
+//   corner: dN/dx = (1/64)(1+b y)(1+c z)[a (9 r^2 - 19) + 18 x (1+a x)];
+//   edge along u: dN/du = (9/64)(1+p v1)(1+q v2)(3 s (1 - u^2) - 2 u (1+3 s u)),
+//   dN/dv1 = (9/64)(1-u^2)(1+3 s u) p (1+q v2), likewise dN/dv2.
+__device__ inline float grid_interpolate_and_derive(const float* __restrict__ nodes,
+                                                    const unsigned* cell, vec3 xi,
+                                                    const float cs[3], vec3& grad) {
+  const float x = xi.x, y = xi.y, z = xi.z;
+  const float k64 = 1.0f / 64.0f;
+  const float k964 = 9.0f / 64.0f;
+  const float F = k64 * (9.0f * (x * x + y * y + z * z) - 19.0f);
+  float phi = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    float a = (c & 1) ? 1.0f : -1.0f;
+    float b = (c & 2) ? 1.0f : -1.0f;
+    float cc = (c & 4) ? 1.0f : -1.0f;
+    float A = 1.0f + a * x, B = 1.0f + b * y, C = 1.0f + cc * z;
+    const float v = nodes[cell[c]];
+    phi += v * (F * (A * B) * C);
+    gx += v * ((a * F + A * (k64 * (18.0f * x))) * (B * C));
+    gy += v * ((b * F + B * (k64 * (18.0f * y))) * (A * C));
+    gz += v * ((cc * F + C * (k64 * (18.0f * z))) * (A * B));
+  }
+#pragma unroll
+  for (int axis = 0; axis < 3; ++axis) {
+    float u = axis == 0 ? x : (axis == 1 ? y : z);
+    float v1 = axis == 0 ? y : x;
+    float v2 = axis == 2 ? y : z;
+    float one_m_u2 = 1.0f - u * u;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float pp, qq;
+      if (axis == 0) {
+        pp = (e >> 1) ? 1.0f : -1.0f;
+        qq = (e & 1) ? 1.0f : -1.0f;
+      } else if (axis == 1) {
+        pp = (e & 1) ? 1.0f : -1.0f;
+        qq = (e >> 1) ? 1.0f : -1.0f;
+      } else {
+        pp = (e >> 1) ? 1.0f : -1.0f;
+        qq = (e & 1) ? 1.0f : -1.0f;
+      }
+      float P = 1.0f + pp * v1, Q = 1.0f + qq * v2;
+#pragma unroll
+      for (int s2 = 0; s2 < 2; ++s2) {
+        float sg = s2 ? 1.0f : -1.0f;
+        float L = 1.0f + 3.0f * sg * u;
+        int n = 8 + axis * 8 + 2 * e + s2;
+        const float v = nodes[cell[n]];
+        float G = k964 * (one_m_u2 * L);
+        phi += v * (G * (P * Q));
+        // derivatives along the edge's axis and the two transverse axes
+        float du = k964 * (3.0f * sg * one_m_u2 - 2.0f * u * L) * (P * Q);
+        float dv1 = G * (pp * Q);
+        float dv2 = G * (P * qq);
+        float dx = axis == 0 ? du : dv1;
+        float dy = axis == 1 ? du : (axis == 0 ? dv1 : dv2);
+        float dz = axis == 2 ? du : dv2;
+        gx += v * dx;
+        gy += v * dy;
+        gz += v * dz;
+      }
+    }
+  }
+  grad = mk(gx * (2.0f / cs[0]), gy * (2.0f / cs[1]), gz * (2.0f / cs[2]));
+  return phi;
+}
+
 // Interpolates the 32-node map at x; FLT_MAX outside (used by the volume
 // field builder for distance maps, runner.hpp:1596-1614).
 __device__ inline float grid_sample(const GridGeomDev& g, const float* __restrict__ nodes,
@@ -364,11 +441,17 @@ __device__ inline float grid_sample(const GridGeomDev& g, const float* __restric
   return grid_interpolate(nodes, cell, inner);
 }
 
+__device__ inline float map_distance(const BoundaryDev& b, vec3 x) {
+  return grid_sample(b.geom, b.distance, x, FLT_MAX);
+}
+
 // --- neighbour grid helpers (runner.hpp:1827-1838, 689-707) -----------------
 // Boundary record of point x for boundary B
 // (compute_volume_and_boundary_x_analytic, runner.hpp:1738-1771, and
-// compute_particle_boundary_analytic, runner.hpp:1982-2002): rec = (closest
-// boundary point, volume Psi), ker = (Psi gradW, Psi W).  Returns Psi.
+// compute_particle_boundary_analytic, runner.hpp:1982-2002; for a distance
+// map compute_volume_and_boundary_x, runner.hpp:1696-1736, and
+// compute_particle_boundary, runner.hpp:2005-2025): rec = (closest boundary
+// point, volume Psi), ker = (Psi gradW, Psi W).  Returns Psi.
 // With Sparse, a point outside the map (first by a sphere around the body
 // origin that holds the map's box, before the rotation into the body frame)
 // or in a cell whose nodes are all zero returns 0 at once and leaves rec and
@@ -390,18 +473,36 @@ __device__ inline float boundary_record(const BoundaryDev& B, vec3 x, const SimP
         !B.cell_nz[(unsigned(ip[2]) * B.geom.res[1] + unsigned(ip[1])) * B.geom.res[0] + unsigned(ip[0])])
       return 0.0f;
   }
-  float d = sdf_eval(B, local) * B.sign;
-  vec3 normal = rotate(sdf_gradient(B, local, p.h) * B.sign, q);
+  float d;
+  vec3 normal;
+  float vol = 0.0f;
+  if (B.type == ALV_DIST_MAP) {
+    // compute_volume_and_boundary_x (runner.hpp:1696-1736): distance and
+    // normal from the distance map; outside the map the record is x itself
+    // with zero volume
+    d = 0.0f;
+    normal = mk(0.f, 0.f, 0.f);
+    if (in_map) {
+      unsigned cell[32];
+      grid_cell_nodes(B.geom.res, ip, cell);
+      vec3 g;
+      d = grid_interpolate_and_derive(B.distance, cell, inner, B.geom.cs, g) * B.sign;
+      normal = rotate(g * B.sign, q);
+      vol = grid_interpolate(B.volume, cell, inner);
+    }
+  } else {
+    d = sdf_eval(B, local) * B.sign;
+    normal = rotate(sdf_gradient(B, local, p.h) * B.sign, q);
+    if (in_map) {
+      unsigned cell[32];
+      grid_cell_nodes(B.geom.res, ip, cell);
+      vol = grid_interpolate(B.volume, cell, inner);
+    }
+  }
   float nl2 = length_sqr(normal);
   normal = normal * (nl2 > 0.0f ? 1.0f / sqrtf(nl2) : 0.0f);
   vec3 xi_bxj = d * normal;
   vec3 bx = x - xi_bxj;
-  float vol = 0.0f;
-  if (in_map) {
-    unsigned cell[32];
-    grid_cell_nodes(B.geom.res, ip, cell);
-    vol = grid_interpolate(B.volume, cell, inner);
-  }
   vec3 kg = gradW(xi_bxj, p);
   rec = make_float4(bx.x, bx.y, bx.z, vol);
   ker = make_float4(vol * kg.x, vol * kg.y, vol * kg.z, vol * W_dist(d, p));

@@ -9,7 +9,9 @@
 
 namespace alv {
 
-enum { ALV_DIST_BOX = 0, ALV_DIST_BOX_SHELL = 1, ALV_DIST_SPHERE = 2 };
+// ALV_DIST_MAP: a signed distance sampled on the boundary's map nodes (the
+// reference's grid path, used for MeshDistance)
+enum { ALV_DIST_BOX = 0, ALV_DIST_BOX_SHELL = 1, ALV_DIST_SPHERE = 2, ALV_DIST_MAP = 3 };
 
 struct SimParams {
   // kernel constants (set_kernel_radius, constants.hpp:100-120)
@@ -58,6 +60,7 @@ struct BoundaryDev {
   float map_radius2;
   GridGeomDev geom;
   const float* volume;  // device pointer to the volume map nodes
+  const float* distance;  // ALV_DIST_MAP: device pointer to the distance map nodes
   // per map cell (x fastest): 1 when any of its 32 nodes is non-zero; a
   // point in a cell with 0 has volume exactly 0
   const unsigned char* cell_nz;

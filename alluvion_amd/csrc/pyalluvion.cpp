@@ -363,6 +363,27 @@ PYBIND11_MODULE(_alluvion, m) {
   py::class_<dg::SphereDistance<float3, float>, Dist>(dg, "SphereDistancefloat")
       .def_static("create", &dg::SphereDistance<float3, float>::create, py::arg("radius"),
                   py::return_value_policy::reference);
+  // A distance sampled on the map nodes (the reference's grid path, which it
+  // takes for MeshDistance): signed_distance maps an [n, 3] float32 array of
+  // body-frame points to n distances.
+  py::class_<dg::MapDistance<float3, float>, Dist>(dg, "MapDistancefloat")
+      .def_static(
+          "create",
+          [](float3 aabb_min, float3 aabb_max, py::function signed_distance) {
+            auto fn = [signed_distance](const float* pts, unsigned n, float* out) {
+              py::gil_scoped_acquire gil;
+              py::array_t<float> arr({py::ssize_t(n), py::ssize_t(3)});
+              std::memcpy(arr.mutable_data(), pts, size_t(n) * 3 * sizeof(float));
+              py::array_t<float, py::array::c_style | py::array::forcecast> d =
+                  signed_distance(arr);
+              if (d.size() != py::ssize_t(n))
+                throw std::invalid_argument("signed_distance must return one value per point");
+              std::memcpy(out, d.data(), size_t(n) * sizeof(float));
+            };
+            return dg::MapDistance<float3, float>::create(aabb_min, aabb_max, fn);
+          },
+          py::arg("aabb_min"), py::arg("aabb_max"), py::arg("signed_distance"),
+          py::return_value_policy::reference);
 
   py::class_<Mesh>(m, "Mesh")
       .def(py::init<>())
@@ -633,6 +654,7 @@ PYBIND11_MODULE(_alluvion, m) {
                                for (U i = 0; i < p.get_size(); ++i) out.push_back(p.get_volume_grid(i));
                                return out;
                              })
+      .def_property_readonly("distance_grids", &P::get_distance_grids)
       .def_property_readonly("domain_min_list",
                              [](P const& p) {
                                std::vector<float3> out;

@@ -25,7 +25,8 @@ class Depot(Store):
         suffix = self.dtype_to_string(default_dtype)
         for classname in ["Runner", "Pile", "SolverDf", "SolverIi", "SolverI"]:
             setattr(self, classname, getattr(_al, f"{classname}{suffix}"))
-        for classname in ["Distance", "BoxDistance", "BoxShellDistance", "SphereDistance"]:
+        for classname in ["Distance", "BoxDistance", "BoxShellDistance", "SphereDistance",
+                          "MapDistance"]:
             setattr(self, classname, getattr(_al.dg, f"{classname}{suffix}"))
         self.cn = getattr(self, f"cn{suffix}")
         self.create_cn = getattr(self, f"create_cn{suffix}")

@@ -64,6 +64,10 @@ class BoundarySpec:
     friction: float = 0.0
     # collision vertices in the body frame, (n, 3); None = no contacts
     collision_vertices: Optional[np.ndarray] = None
+    # sample the shape's signed distance onto the boundary's map nodes and
+    # take the reference's grid path (as for MeshDistance) instead of the
+    # analytic one
+    sampled: bool = False
 
 
 @dataclass
@@ -157,6 +161,40 @@ def adaptive_step(scene: Scene, cfl: float = 0.4, max_dt: float = 2e-3,
 
 SPHERE_RADIUS, BOX_EDGE = 0.05, 0.08
 SPHERE_DENSITY, BOX_DENSITY = 500.0, 800.0
+
+
+def shape_distance(kind: str, params, p: np.ndarray) -> np.ndarray:
+    """Signed distance (negative inside, body frame) of points p [n, 3] to
+    an analytic shape, in FP64: dg/box_distance.hpp:23-28,
+    dg/box_shell_distance.hpp:36-59, dg/sphere_distance.hpp:18-21."""
+    d = np.asarray(p, np.float64)
+
+    def box(half, outset):
+        q = np.abs(d) - np.asarray(half, np.float64)
+        return (np.linalg.norm(np.maximum(q, 0.0), axis=1) + np.minimum(q.max(axis=1), 0.0)
+                - outset)
+
+    if kind == "sphere":
+        return np.linalg.norm(d, axis=1) - params[0]
+    half = 0.5 * np.asarray(params[:3], np.float64)
+    if kind == "box":
+        return box(half, params[3])
+    inner = box(half, params[4])
+    outer = box(half + params[3], params[4])
+    return np.maximum(outer, -inner)
+
+
+def shape_aabb(kind: str, params):
+    """Bounding box (body frame) of an analytic shape, as its Distance
+    computes it."""
+    if kind == "sphere":
+        r = params[0]
+        return (-r, -r, -r), (r, r, r)
+    half = 0.5 * np.asarray(params[:3], np.float64)
+    grow = params[3] if kind == "box" else params[3] + params[4]
+    lo = tuple(float(v) for v in -half - grow)
+    hi = tuple(float(v) for v in half + grow)
+    return lo, hi
 
 
 def body_distance(p: np.ndarray, body: BoundarySpec) -> np.ndarray:
@@ -317,7 +355,13 @@ def build_solver(scene: Scene, kind: str = "df", dp=None, max_particles: Optiona
     pile = dp.Pile(dp, runner, scene.max_num_contacts, volume_method=method,
                    max_num_pellets=scene.max_num_pellets, cn=cn, cni=cni)
     for b in scene.boundaries:
-        if b.kind == "box":
+        if b.sampled:
+            lo, hi = shape_aabb(b.kind, b.params)
+            dist = dp.MapDistance.create(
+                dp.f3(*lo), dp.f3(*hi),
+                lambda pts, kind=b.kind, params=b.params:
+                    shape_distance(kind, params, pts).astype(np.float32))
+        elif b.kind == "box":
             dist = dp.BoxDistance.create(dp.f3(*b.params[:3]), b.params[3])
         elif b.kind == "box_shell":
             dist = dp.BoxShellDistance.create(dp.f3(*b.params[:3]), b.params[3], b.params[4])

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -776,6 +777,42 @@ class SphereDistance : public Distance<TF3, TF> {
   }
   TF radius_;
 };
+// A distance sampled on the boundary's map nodes, as the reference does for
+// MeshDistance (construct_distance_grid, pile.hpp:79-95): `fn` writes the
+// signed distance of n points (xyz triples, body frame) to out[n]; the
+// boundary records, the volume map and the contacts then use the
+// interpolated map and its gradient.
+template <typename TF3, typename TF>
+class MapDistance : public Distance<TF3, TF> {
+ public:
+  using Fn = std::function<void(const float* points, unsigned n, float* out)>;
+  MapDistance(TF3 aabb_min, TF3 aabb_max, Fn fn) : fn_(std::move(fn)) {
+    this->aabb_min_ = aabb_min;
+    this->aabb_max_ = aabb_max;
+  }
+  static MapDistance* create(TF3 aabb_min, TF3 aabb_max, Fn fn) {
+    return new MapDistance(aabb_min, aabb_max, std::move(fn));
+  }
+  alv_distance to_c() const override {
+    alv_distance d{};
+    d.type = ALV_DISTANCE_MAP;
+    d.aabb_min[0] = this->aabb_min_.x;
+    d.aabb_min[1] = this->aabb_min_.y;
+    d.aabb_min[2] = this->aabb_min_.z;
+    d.aabb_max[0] = this->aabb_max_.x;
+    d.aabb_max[1] = this->aabb_max_.y;
+    d.aabb_max[2] = this->aabb_max_.z;
+    d.signed_distance = &MapDistance::call;
+    d.context = const_cast<MapDistance*>(this);
+    return d;
+  }
+  Fn fn_;
+
+ private:
+  static void call(void* self, const float* points, unsigned n, float* out) {
+    static_cast<MapDistance*>(self)->fn_(points, n, out);
+  }
+};
 }  // namespace dg
 
 enum class VolumeMethod { volume_map = 0, pellets = 1 };
@@ -1090,6 +1127,18 @@ class Pile {
     unsigned n = 0;
     check(alv_pile_get_volume_grid(pile_, i, &p, &n));
     return Variable<1, float>({n}, p);
+  }
+  // Pile::get_distance_grids (pile.hpp:981-988): empty for analytic distances
+  Variable<1, float> get_distance_grid(U i) const {
+    float* p = nullptr;
+    unsigned n = 0;
+    check(alv_pile_get_distance_grid(pile_, i, &p, &n));
+    return Variable<1, float>({n}, p);
+  }
+  std::vector<Variable<1, float>> get_distance_grids() const {
+    std::vector<Variable<1, float>> out;
+    for (U i = 0; i < get_size(); ++i) out.push_back(get_distance_grid(i));
+    return out;
   }
   struct GridInfo {
     float3 domain_min, domain_max, cell_size;

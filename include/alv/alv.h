@@ -112,15 +112,27 @@ void alv_const_set_particle_attr(alv_const* cn, float radius, float mass,
 typedef struct alv_pile alv_pile;
 
 /* Analytic distances (dg/box_distance.hpp, dg/box_shell_distance.hpp,
- * dg/sphere_distance.hpp). */
+ * dg/sphere_distance.hpp), and a distance sampled on the boundary's map
+ * nodes (ALV_DISTANCE_MAP: the reference's grid path, which it takes for
+ * MeshDistance, pile.hpp:79-95,508-521, runner.hpp:1696-1736,2005-2025). */
 #define ALV_DISTANCE_BOX 0
 #define ALV_DISTANCE_BOX_SHELL 1
 #define ALV_DISTANCE_SPHERE 2
+#define ALV_DISTANCE_MAP 3
 typedef struct alv_distance {
   int type;
   /* box: widths[3], outset; box shell: inner widths[3], thickness, outset;
    * sphere: radius */
   float params[5];
+  /* ALV_DISTANCE_MAP: the distance's bounding box (body frame), and a
+   * function that writes the signed distance of n points (xyz triples, body
+   * frame) to out[n].  Pile::add calls it on the nodes of the boundary's map
+   * (in chunks, before it returns), as construct_distance_grid evaluates the
+   * mesh distance; the map is then interpolated with its gradient.  Unused
+   * for the analytic types. */
+  float aabb_min[3], aabb_max[3];
+  void (*signed_distance)(void* context, const float* points, unsigned n, float* out);
+  void* context;
 } alv_distance;
 
 /* VolumeMethod (pile.hpp:16): volume maps, or particle-sampled boundaries
@@ -246,6 +258,11 @@ int alv_pile_get_contacts(const alv_pile* pile, alv_contact* out, unsigned capac
 /* Volume map of boundary b (device pointer) and its grid geometry. */
 int alv_pile_get_volume_grid(const alv_pile* pile, unsigned b, float** nodes,
                              unsigned* num_nodes);
+/* Distance map of boundary b (Pile::get_distance_grids, pile.hpp:981-988):
+ * device pointer and node count; null and 0 for an analytic distance, whose
+ * reference distance grid is empty. */
+int alv_pile_get_distance_grid(const alv_pile* pile, unsigned b, float** nodes,
+                               unsigned* num_nodes);
 int alv_pile_get_grid_info(const alv_pile* pile, unsigned b,
                            float domain_min[3], float domain_max[3],
                            unsigned resolution[3], float cell_size[3],

@@ -27,6 +27,12 @@
 #include <string>
 #include <vector>
 
+template <class T>
+static void volume_from_distance_impl(double h, const double* dmin, const double* dmax,
+                                      const unsigned* res, const double* cs, double sign,
+                                      const double* dist, unsigned begin, unsigned end,
+                                      double* out);
+
 namespace orc {
 
 // Planted mistakes (orc_plant): a test switches one on to show that its
@@ -421,6 +427,14 @@ struct Sdf {
       aabb_max = {half.x + th + outset, half.y + th + outset,
                   half.z + th + outset};
       max_distance = std::sqrt(length_sqr(half_outer)) + outset;
+    } else if (type == ORC_MAP) {
+      // the distance comes from the boundary's map; only its box is kept
+      aabb_min = {T(d.aabb_min[0]), T(d.aabb_min[1]), T(d.aabb_min[2])};
+      aabb_max = {T(d.aabb_max[0]), T(d.aabb_max[1]), T(d.aabb_max[2])};
+      V3<T> far{std::max(std::fabs(aabb_min.x), std::fabs(aabb_max.x)),
+                std::max(std::fabs(aabb_min.y), std::fabs(aabb_max.y)),
+                std::max(std::fabs(aabb_min.z), std::fabs(aabb_max.z))};
+      max_distance = std::sqrt(length_sqr(far));
     } else {
       radius = T(d.p[0]);
       outset = 0;
@@ -618,6 +632,25 @@ T interpolate(const Nodes& nodes, const unsigned* cells, const T* N) {
   return phi;
 }
 
+// interpolate_and_derive (runner.hpp:1773-1791): the value and the
+// gradient, scaled from the local coordinate to world units by 2 / cell size
+template <class T, class Nodes>
+T interpolate_and_derive(const Nodes& nodes, const unsigned* cells, const T* N,
+                         const T* dNx, const T* dNy, const T* dNz, V3<T> cs,
+                         V3<T>& grad) {
+  T phi = 0;
+  V3<T> g{0, 0, 0};
+  for (int j = 0; j < 32; ++j) {
+    T c = nodes[cells[j]];
+    phi += c * N[j];
+    g.x += c * dNx[j];
+    g.y += c * dNy[j];
+    g.z += c * dNz[j];
+  }
+  grad = {g.x * (T(2) / cs.x), g.y * (T(2) / cs.y), g.z * (T(2) / cs.z)};
+  return phi;
+}
+
 // ---------------------------------------------------------------------------
 // Boundary description (Pile, pile.hpp:40-78, 285-352)
 // ---------------------------------------------------------------------------
@@ -634,6 +667,42 @@ struct Boundary {
   // rigid contacts (Pile::add collision_mesh, restitution, friction)
   T restitution = 1, friction = 0;
   std::vector<V3<T>> coll;  // collision vertices in the body frame
+  std::vector<T> distance;  // ORC_MAP: distance map nodes
+  bool is_map() const { return sdf.type == ORC_MAP; }
+  // Distance (before the sign) and its gradient at body-frame point `local`
+  // from the distance map; false outside the map.
+  bool map_distance(V3<T> local, T& d, V3<T>& grad) const {
+    int ip[3];
+    V3<T> inner;
+    if (!resolve(geom, local, ip, inner)) return false;
+    T N[32], dNx[32], dNy[32], dNz[32];
+    unsigned cells[32];
+    shape_functions<T>(inner, N, dNx, dNy, dNz);
+    cell_nodes(geom.res, ip, cells);
+    d = interpolate_and_derive(distance, cells, N, dNx, dNy, dNz, geom.cs, grad);
+    return true;
+  }
+  // Collision distance and normal of a body-frame point
+  // (collision_find_dist_normal, runner.hpp:1794-1825, for a map; the
+  // analytic variant otherwise); the normal is unit when dist < 0.
+  T contact_dist_normal(V3<T> local, T tolerance, V3<T>& n) const {
+    T dist;
+    if (is_map()) {
+      T d;
+      V3<T> g;
+      dist = sign;
+      n = {0, 0, 0};
+      if (map_distance(local, d, g)) {
+        dist = sign * d - tolerance;
+        n = g * sign;
+      }
+    } else {
+      dist = sdf.signed_distance(local) * sign - tolerance;
+      n = sdf.gradient(local, static_cast<T>(1e-3)) * sign;
+    }
+    if (dist < 0) n = n * (1 / std::sqrt(length_sqr(n)));
+    return dist;
+  }
 };
 
 // Contact<float3, float> (contact.hpp:7-24)
@@ -856,6 +925,35 @@ struct Solver {
   }
   void point_boundary(unsigned b, V3<T> xi, V4<T>& rec, V4<T>& ker) const {
     const Boundary<T>& B = bnd[b];
+    if (B.is_map()) {
+      // compute_volume_and_boundary_x (runner.hpp:1696-1736) with
+      // compute_particle_boundary (runner.hpp:2005-2025): outside the map
+      // the record is the particle itself with zero volume
+      V3<T> local = rotate(xi - B.x, conjugate(B.q));
+      V3<T> bx = xi, xi_bxj{0, 0, 0};
+      T d = 0, vol = 0;
+      int ip[3];
+      V3<T> inner;
+      if (resolve(B.geom, local, ip, inner)) {
+        T N[32], dNx[32], dNy[32], dNz[32];
+        unsigned cells[32];
+        shape_functions<T>(inner, N, dNx, dNy, dNz);
+        cell_nodes(B.geom.res, ip, cells);
+        V3<T> g;
+        d = interpolate_and_derive(B.distance, cells, N, dNx, dNy, dNz, B.geom.cs, g) *
+            B.sign;
+        V3<T> normal = rotate(g * B.sign, B.q);
+        T nl2 = length_sqr(normal);
+        normal = normal * (nl2 > 0 ? 1 / std::sqrt(nl2) : T(0));
+        xi_bxj = d * normal;
+        bx = xi - xi_bxj;
+        vol = interpolate(B.volume, cells, N);
+      }
+      V3<T> kg = displacement_cubic_kernel_grad(xi_bxj, c);
+      rec = {bx.x, bx.y, bx.z, vol};
+      ker = {vol * kg.x, vol * kg.y, vol * kg.z, vol * dist_cubic_kernel(d, c)};
+      return;
+    }
     {
       V3<T> local = rotate(xi - B.x, conjugate(B.q));
       T d = B.sdf.signed_distance(local) * B.sign;
@@ -1280,9 +1378,8 @@ struct Solver {
       Contact<T> C{};
       C.cp_i = rotate(vert, Bi.q) + Bi.x;
       V3<T> local = rotate(C.cp_i - Bj.x, conjugate(Bj.q));
-      T dist = Bj.sdf.signed_distance(local) * Bj.sign - contact_tolerance;
-      V3<T> nn = Bj.sdf.gradient(local, static_cast<T>(1e-3)) * Bj.sign;
-      nn = nn * (1 / std::sqrt(length_sqr(nn)));
+      V3<T> nn;
+      T dist = Bj.contact_dist_normal(local, contact_tolerance, nn);
       if (!(dist < 0)) continue;
       ++num_found;
       C.i = i;
@@ -2030,9 +2127,8 @@ struct Solver {
     Contact<T> C{};
     C.cp_i = x[max_n + k];
     V3<T> local = rotate(C.cp_i - Kj.x, conjugate(Kj.q));
-    T dist = Bj.sdf.signed_distance(local) * Bj.sign - contact_tolerance;
-    V3<T> nn = Bj.sdf.gradient(local, static_cast<T>(1e-3)) * Bj.sign;
-    nn = nn * (1 / std::sqrt(length_sqr(nn)));
+    V3<T> nn;
+    T dist = Bj.contact_dist_normal(local, contact_tolerance, nn);
     if (!(dist < 0)) return;
     ++num_found;
     C.i = i;
@@ -2260,6 +2356,17 @@ struct Solver {
   void build_volume_map(unsigned b, unsigned begin, unsigned end) {
     Boundary<T>& B = bnd[b];
     end = std::min(end, B.geom.num_nodes);
+    if (B.is_map()) {
+      // update_volume_field from the distance map (runner.hpp:1616-1658)
+      const double dmin[3] = {double(B.geom.dmin.x), double(B.geom.dmin.y), double(B.geom.dmin.z)};
+      const double dmax[3] = {double(B.geom.dmax.x), double(B.geom.dmax.y), double(B.geom.dmax.z)};
+      const double cs[3] = {double(B.geom.cs.x), double(B.geom.cs.y), double(B.geom.cs.z)};
+      std::vector<double> dist(B.distance.begin(), B.distance.end()), out(B.geom.num_nodes);
+      volume_from_distance_impl<T>(double(c.h), dmin, dmax, B.geom.res, cs, double(B.sign),
+                                   dist.data(), begin, end, out.data());
+      for (unsigned l = begin; l < end; ++l) B.volume[l] = T(out[l]);
+      return;
+    }
 #pragma omp parallel for schedule(dynamic, 64)
     for (long ll = long(begin); ll < long(end); ++ll) {
       unsigned l = unsigned(ll);
@@ -2485,6 +2592,16 @@ int orc_set_volume_map(void* h, unsigned b, const void* nodes) {
     if (b >= s.bnd.size()) return -3;
     auto& vol = s.bnd[b].volume;
     std::memcpy(vol.data(), nodes, vol.size() * sizeof(vol[0]));
+    return 0;
+  });
+}
+
+int orc_set_distance_map(void* h, unsigned b, const void* nodes) {
+  return dispatch(h, [&](auto& s) {
+    if (b >= s.bnd.size() || !s.bnd[b].is_map()) return -3;
+    auto& d = s.bnd[b].distance;
+    d.resize(s.bnd[b].geom.num_nodes);
+    std::memcpy(d.data(), nodes, d.size() * sizeof(d[0]));
     return 0;
   });
 }
@@ -3058,6 +3175,25 @@ int orc_shape_functions(const double* xi, double* N, double* dN) {
   return 0;
 }
 
+int orc_node_positions(const double* dmin, const unsigned* res, const double* cs,
+                       double* out) {
+  GridGeom<double> g{};
+  g.dmin = {dmin[0], dmin[1], dmin[2]};
+  g.cs = {cs[0], cs[1], cs[2]};
+  for (int a = 0; a < 3; ++a) g.res[a] = res[a];
+  const unsigned nv = (res[0] + 1) * (res[1] + 1) * (res[2] + 1);
+  const unsigned ne = res[0] * (res[1] + 1) * (res[2] + 1) + (res[0] + 1) * res[1] * (res[2] + 1) +
+                      (res[0] + 1) * (res[1] + 1) * res[2];
+  g.num_nodes = nv + 2 * ne;
+  for (unsigned l = 0; l < g.num_nodes; ++l) {
+    V3<double> x = node_position(g, l);
+    out[3 * size_t(l)] = x.x;
+    out[3 * size_t(l) + 1] = x.y;
+    out[3 * size_t(l) + 2] = x.z;
+  }
+  return 0;
+}
+
 int orc_node_position(const double* dmin, const unsigned* res,
                       const double* cs, unsigned l, double* out) {
   GridGeom<double> g{};
@@ -3149,6 +3285,36 @@ int orc_volume_from_distance_map(int precision, double h, const double* dmin,
   else
     volume_from_distance_impl<double>(h, dmin, dmax, res, cs, sign, dist,
                                       begin, end, out);
+  return 0;
+}
+
+int orc_map_value_gradient(const double* dmin, const double* dmax, const unsigned* res,
+                           const double* cs, const double* nodes, size_t n,
+                           const double* points, double* value, double* gradient,
+                           int* inside) {
+  GridGeom<double> g{};
+  g.dmin = {dmin[0], dmin[1], dmin[2]};
+  g.dmax = {dmax[0], dmax[1], dmax[2]};
+  g.cs = {cs[0], cs[1], cs[2]};
+  for (int a = 0; a < 3; ++a) g.res[a] = res[a];
+  for (size_t i = 0; i < n; ++i) {
+    V3<double> x{points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+    int ip[3];
+    V3<double> inner, gr{0, 0, 0};
+    double v = 0;
+    inside[i] = resolve(g, x, ip, inner) ? 1 : 0;
+    if (inside[i]) {
+      double N[32], dNx[32], dNy[32], dNz[32];
+      unsigned cells[32];
+      shape_functions<double>(inner, N, dNx, dNy, dNz);
+      cell_nodes(g.res, ip, cells);
+      v = interpolate_and_derive(nodes, cells, N, dNx, dNy, dNz, g.cs, gr);
+    }
+    value[i] = v;
+    gradient[3 * i] = gr.x;
+    gradient[3 * i + 1] = gr.y;
+    gradient[3 * i + 2] = gr.z;
+  }
   return 0;
 }
 

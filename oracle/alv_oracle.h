@@ -75,8 +75,11 @@ typedef struct {
 } orc_report;
 
 /* Analytic distance types (reference dg/box_distance.hpp,
- * dg/box_shell_distance.hpp, dg/sphere_distance.hpp). */
-enum { ORC_BOX = 0, ORC_BOX_SHELL = 1, ORC_SPHERE = 2 };
+ * dg/box_shell_distance.hpp, dg/sphere_distance.hpp), and a distance sampled
+ * on the boundary's map nodes (ORC_MAP: the reference's grid path for
+ * MeshDistance, runner.hpp:1696-1736,1773-1825,2005-2025; the node values
+ * are set with orc_set_distance_map). */
+enum { ORC_BOX = 0, ORC_BOX_SHELL = 1, ORC_SPHERE = 2, ORC_MAP = 3 };
 
 typedef struct {
   int type;
@@ -90,6 +93,7 @@ typedef struct {
   double inertia[3];
   double x[3];
   double q[4]; /* (x, y, z, w) */
+  double aabb_min[3], aabb_max[3]; /* ORC_MAP: the distance's bounding box */
 } orc_boundary_desc;
 
 /* kind: 0 = DFSPH (SolverDf), 1 = IISPH (SolverIi), 2 = ISPH (SolverI).
@@ -112,6 +116,15 @@ int orc_get_grid_info(void* h, unsigned b, double* domain_min,
 int orc_build_volume_map(void* h, unsigned b, unsigned begin, unsigned end);
 int orc_set_volume_map(void* h, unsigned b, const void* nodes);
 int orc_get_volume_map(void* h, unsigned b, void* out);
+/* Distance map of an ORC_MAP boundary (working precision, one value per map
+ * node); orc_build_volume_map then builds the volume map from it. */
+int orc_set_distance_map(void* h, unsigned b, const void* nodes);
+/* interpolate_and_derive (runner.hpp:1773-1791) on a stand-alone map:
+ * value and gradient at each of n points (nodes in double). */
+int orc_map_value_gradient(const double* domain_min, const double* domain_max,
+                           const unsigned* res, const double* cell_size,
+                           const double* nodes, size_t n, const double* points,
+                           double* value, double* gradient, int* inside);
 int orc_set_rigid(void* h, unsigned b, const double* x, const double* v,
                   const double* q, const double* omega);
 int orc_get_rigid(void* h, unsigned b, double* x, double* v, double* q,
@@ -252,6 +265,9 @@ int orc_shape_functions(const double* xi, double* N, double* dN);
 /* Node position of node index l (runner.hpp:1131-1178). */
 int orc_node_position(const double* domain_min, const unsigned* res,
                       const double* cell_size, unsigned l, double* out);
+/* Positions of all nodes (out[3 * num_nodes]). */
+int orc_node_positions(const double* domain_min, const unsigned* res,
+                       const double* cell_size, double* out);
 /* Indices of the 32 nodes of cell ipos (runner.hpp:1197-1257). */
 int orc_cell_nodes(const unsigned* res, const int* ipos, unsigned* out);
 /* Volume map from a sampled distance map (the grid variant,

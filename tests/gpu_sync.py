@@ -28,12 +28,17 @@ def body_state(pile, b):
             "omega": np.array(list(pile.get_omega(b)), np.float64)}
 
 
-def oracle_for(scene, dp, piles_grids, kind="df", precision=4, max_particles=None):
+def oracle_for(scene, dp, piles_grids, kind="df", precision=4, max_particles=None,
+               distance_grids=None):
     """An oracle of `scene` using the GPU-built volume maps (device
-    variables, one per boundary)."""
+    variables, one per boundary) and, for sampled boundaries, the GPU's
+    distance maps."""
     orc = of.Oracle(scene, precision=precision, kind=kind, max_particles=max_particles)
     for b, var in enumerate(piles_grids):
         orc.set_volume_map(b, dp.coat(var).get())
+    for b, spec in enumerate(scene.boundaries):
+        if getattr(spec, "sampled", False):
+            orc.set_distance_map(b, dp.coat(distance_grids[b]).get())
     return orc
 
 

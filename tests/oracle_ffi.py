@@ -9,6 +9,8 @@ import subprocess
 
 import numpy as np
 
+from alluvion_amd.scenes import shape_aabb as scenes_shape_aabb
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
@@ -76,6 +78,8 @@ class OrcBoundaryDesc(ctypes.Structure):
         ("inertia", ctypes.c_double * 3),
         ("x", ctypes.c_double * 3),
         ("q", ctypes.c_double * 4),
+        ("aabb_min", ctypes.c_double * 3),
+        ("aabb_max", ctypes.c_double * 3),
     ]
 
 
@@ -122,6 +126,10 @@ def lib():
         L.orc_get_grid_info.argtypes = [vp, ctypes.c_uint, dp, dp, ctypes.POINTER(ctypes.c_uint), dp]
         L.orc_build_volume_map.argtypes = [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint]
         L.orc_set_volume_map.argtypes = [vp, ctypes.c_uint, vp]
+        L.orc_set_distance_map.argtypes = [vp, ctypes.c_uint, vp]
+        L.orc_map_value_gradient.argtypes = [dp, dp, ctypes.POINTER(ctypes.c_uint), dp, dp,
+                                             ctypes.c_size_t, dp, dp, dp,
+                                             ctypes.POINTER(ctypes.c_int)]
         L.orc_get_volume_map.argtypes = [vp, ctypes.c_uint, vp]
         L.orc_set_rigid.argtypes = [vp, ctypes.c_uint, dp, dp, dp, dp]
         L.orc_get_rigid.argtypes = [vp, ctypes.c_uint, dp, dp, dp, dp, dp, dp]
@@ -164,6 +172,7 @@ def lib():
         L.orc_cubic_kernel.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_size_t, dp, dp]
         L.orc_shape_functions.argtypes = [dp, dp, dp]
         L.orc_node_position.argtypes = [dp, ctypes.POINTER(ctypes.c_uint), dp, ctypes.c_uint, dp]
+        L.orc_node_positions.argtypes = [dp, ctypes.POINTER(ctypes.c_uint), dp, dp]
         L.orc_cell_nodes.argtypes = [ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_uint)]
         L.orc_volume_from_distance_map.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp,
@@ -194,6 +203,7 @@ def _uptr(a):
 
 
 _KIND = {"box": 0, "box_shell": 1, "sphere": 2}
+ORC_MAP = 3
 _VEC_ARRAYS = {"x", "v", "a", "dii", "dij_pj", "pressure_accel"}
 
 
@@ -270,9 +280,13 @@ class Oracle:
         self.L.orc_set_settings(self.h, ctypes.byref(s))
         for b in scene.boundaries:
             d = OrcBoundaryDesc()
-            d.type = _KIND[b.kind]
+            d.type = ORC_MAP if getattr(b, "sampled", False) else _KIND[b.kind]
             for i, v in enumerate(b.params):
                 d.p[i] = v
+            lo, hi = scenes_shape_aabb(b.kind, b.params)
+            for a in range(3):
+                d.aabb_min[a] = lo[a]
+                d.aabb_max[a] = hi[a]
             d.sign = b.sign
             for a in range(3):
                 d.resolution[a] = b.resolution[a]
@@ -375,6 +389,11 @@ class Oracle:
         a = np.ascontiguousarray(nodes, dtype=self.dtype)
         assert a.size == self.num_nodes(b)
         self.L.orc_set_volume_map(self.h, b, a.ctypes.data)
+
+    def set_distance_map(self, b, nodes):
+        a = np.ascontiguousarray(nodes, dtype=self.dtype)
+        assert a.size == self.num_nodes(b)
+        assert self.L.orc_set_distance_map(self.h, b, a.ctypes.data) == 0
 
     def rigid(self, b):
         out = [np.zeros(k) for k in (3, 3, 4, 3, 3, 3)]
@@ -650,12 +669,38 @@ def node_position(dmin, res, cs, l):
     return out
 
 
+def node_positions(dmin, res, cs):
+    """All node positions [num_nodes, 3] (index_to_node_position)."""
+    dmin = np.ascontiguousarray(dmin, dtype=np.float64)
+    cs = np.ascontiguousarray(cs, dtype=np.float64)
+    r = (ctypes.c_uint * 3)(*[int(v) for v in res])
+    out = np.zeros((num_nodes(res), 3))
+    lib().orc_node_positions(_dptr(dmin), r, _dptr(cs), _dptr(out))
+    return out
+
+
 def cell_nodes(res, ipos):
     r = (ctypes.c_uint * 3)(*[int(v) for v in res])
     ip = (ctypes.c_int * 3)(*[int(v) for v in ipos])
     out = (ctypes.c_uint * 32)()
     lib().orc_cell_nodes(r, ip, out)
     return np.array(list(out), dtype=np.int64)
+
+
+def map_value_gradient(dmin, dmax, res, cs, nodes, points):
+    """interpolate_and_derive on a stand-alone map (double): values [n],
+    gradients [n, 3], inside flags [n]."""
+    dmin, dmax, cs = [np.ascontiguousarray(a, dtype=np.float64) for a in (dmin, dmax, cs)]
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    n = pts.shape[0]
+    r = (ctypes.c_uint * 3)(*[int(v) for v in res])
+    val, grad = np.zeros(n), np.zeros((n, 3))
+    inside = np.zeros(n, np.int32)
+    lib().orc_map_value_gradient(_dptr(dmin), _dptr(dmax), r, _dptr(cs), _dptr(nodes), n,
+                                 _dptr(pts), _dptr(val), _dptr(grad),
+                                 inside.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return val, grad, inside.astype(bool)
 
 
 def num_nodes(res):
