@@ -481,9 +481,9 @@ void launch_histogram256(const float* v, const float* mask, unsigned n, float v_
 }
 unsigned histogram_partial_count() { return kHistBlocks * 256u; }
 
-// compute_boundary_mask, analytic variant (runner.hpp:4385-4397): mask[i] is
-// multiplied by (sign * d(R(q)^T (x_i - x_b)) >= threshold), d the
-// boundary's signed distance.
+// compute_boundary_mask (runner.hpp:4368-4397): mask[i] is multiplied by
+// (sign * d(R(q)^T (x_i - x_b)) >= threshold), d the boundary's signed
+// distance (for a distance map the interpolated map, FLT_MAX outside it).
 __global__ __launch_bounds__(256) void k_boundary_mask(BoundaryDev B, const float* __restrict__ x,
                                                       float threshold, float* __restrict__ mask,
                                                       unsigned n) {

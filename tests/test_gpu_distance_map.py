@@ -52,6 +52,24 @@ def test_distance_grids_hold_the_sampled_distance():
     want = scenes.shape_distance("sphere", (0.05,), of.node_positions(dmin, res, cs))
     # node positions are computed in FP32 by the library, in FP64 here
     np.testing.assert_allclose(got, want, rtol=0, atol=2e-7)
+    # compute_mask against the sampled sphere: sign * (interpolated map) >=
+    # threshold, the map interpolated by the oracle from the GPU's nodes;
+    # points off the map have no distance (FLT_MAX) and keep their mask
+    rng = np.random.default_rng(9)
+    sx = (np.array([0.1, 0.1, 0.1]) + rng.uniform(-0.09, 0.09, (4096, 3))).astype(np.float32)
+    thr = 0.004
+    mask_var = dp.create_coated((4096,), 1, np.float32)
+    mask_var.set(np.ones(4096, np.float32))
+    sx_var = dp.create_coated((4096,), 3, np.float32)
+    sx_var.set(sx)
+    pile.compute_mask(1, thr, sx_var, mask_var, 4096)
+    mask = mask_var.get()
+    val, _, inside = of.map_value_gradient(dmin, dmax, res, cs, got,
+                                           sx.astype(np.float64) - np.array([0.1, 0.1, 0.1]))
+    d = np.where(inside, val, np.finfo(np.float32).max)
+    clear = np.abs(d - thr) > 1e-5
+    np.testing.assert_array_equal(mask[clear], (d[clear] >= thr).astype(np.float32))
+    assert (mask == 0).sum() > 100 and (mask == 1).sum() > 100
 
 
 def test_volume_map_from_the_distance_map_matches_oracle():
