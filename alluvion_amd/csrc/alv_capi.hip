@@ -2252,11 +2252,15 @@ int alv_solver_step(alv_solver* S) {
         base ^ ((unsigned long long)cdiv << 8) ^ (div << 1) ^ (S->log_features << 2) ^
         ((S->net != nullptr) << 3) ^ (S->net_div << 4) ^
         ((unsigned long long)(uintptr_t)S->net << 32);
+    // the reference's warm start 0 rides on the neighbour search's walk over
+    // the lists unless the learned warm start or feature logging replaces it
+    const int fuse_ws = div && !S->net && !S->log_features;
     rc = run_segment(S, 0, keyA, [&] {
-      if (neighbor_phase(S, L, div) != ALV_OK) return;
+      if (neighbor_phase(S, L, fuse_ws ? 2 : div) != ALV_OK) return;
       launch_ctl_reset(L, A, st);
       if (L.active && div) {
-        launch_df_warm_start_div(L, A, st, S->log_features, S->net ? &mv : nullptr, S->net_div);
+        launch_df_warm_start_div(L, A, st, S->log_features, S->net ? &mv : nullptr, S->net_div,
+                                 fuse_ws);
         for (unsigned k = 0; k < cdiv; ++k) launch_df_div_iteration(L, A, st);
       }
     });

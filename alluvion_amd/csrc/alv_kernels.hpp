@@ -206,14 +206,17 @@ void launch_tiles(const Launch& L, const DevArrays& A, hipStream_t s);
 // gather user -> sorted and compute boundary records
 void launch_gather_boundary(const Launch& L, const DevArrays& A, int solver_kind,
                             hipStream_t s);
-// neighbour list + density (+ DFSPH factor when with_factor)
+// neighbour list + density; mode 1: + DFSPH factor; mode 2: + factor and the
+// divergence solve's warm start 0
 void launch_neighbors(const Launch& L, const DevArrays& A, int with_factor, hipStream_t s);
 
 // DFSPH
 // features: record the eight warm-start features; net: learned warm start
 // (null = the reference's warm start)
+// fused: warm start 0 already ran in the neighbour search (launch_neighbors
+// mode 2)
 void launch_df_warm_start_div(const Launch& L, const DevArrays& A, hipStream_t s,
-                              int features, const MlpDev* net, int net_divergence);
+                              int features, const MlpDev* net, int net_divergence, int fused);
 void launch_df_div_iteration(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_df_viscosity(const Launch& L, const DevArrays& A, int div_finish,
                          hipStream_t s);

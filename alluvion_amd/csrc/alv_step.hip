@@ -352,6 +352,12 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SceneTab T, DevArrays A
   A.bmask[s] = mask;
 }
 
+// velocity of a rigid boundary at world point bx
+__device__ inline vec3 rigid_velocity(const BoundaryDev& B, vec3 bx) {
+  return cross(mk(B.omega[0], B.omega[1], B.omega[2]), bx - mk(B.x[0], B.x[1], B.x[2])) +
+         mk(B.v[0], B.v[1], B.v[2]);
+}
+
 // Where the neighbour loops read x_j and v_j: global memory (L2 gathers),
 // or the block's LDS tile.
 struct GlobalXV {
@@ -445,10 +451,13 @@ __device__ inline bool stage_x(const DevArrays& A, unsigned blk, TileX& t) {
 
 // ---------------------------------------------------------------------------
 // Neighbour list (make_neighbor_list<wrap>, runner.hpp:1869-1911) fused with
-// compute_density (runner.hpp:2052-2072) and, for DFSPH,
-// compute_dfsph_factor (runner.hpp:3124-3154).  Cells are visited in the
-// reference order (dx outer, dz inner); inside a cell in ascending particle
-// id, which is ascending sorted index.
+// compute_density (runner.hpp:2052-2072) and, for DFSPH (mode 1),
+// compute_dfsph_factor (runner.hpp:3124-3154); mode 2 also runs the
+// divergence solve's warm start 0 (compute_density_change and
+// warm_start_divergence_solve_0, runner.hpp:3192-3250) on the same walk over
+// the list, reusing each neighbour's kernel gradient.  Cells are visited in
+// the reference order (dx outer, dz inner); inside a cell in ascending
+// particle id, which is ascending sorted index.
 // ---------------------------------------------------------------------------
 // Candidates staged per wave: the 64 particles of a wave mostly lie in one
 // (x, y) column of cells, so for each (dx, dy) row their candidate spans lie
@@ -679,6 +688,9 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
   // almost every candidate, so the whole wave would execute the kernel
   // evaluation for every candidate.  The lanes walk their lists in step, so
   // the LDS slots of group q are stored for the whole wave at once.
+  const bool ws = with_factor == 2;  // divergence warm start 0 on this walk
+  const vec3 vi = ws ? ld3(A.v[s]) : mk(0.f, 0.f, 0.f);
+  float div = 0.0f;  // sum_j V (v_i - v_j) . gradW_ij, in list order
   for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
     unsigned jn[kNB];
     if (kWaveStage && t0 < kListLds) {
@@ -692,21 +704,28 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
     } else {
       batch_ids(A, s, nn, t0, jn);
     }
-    float4 xj[kNB];
+    float4 xj[kNB], vj[kNB];
 #pragma unroll
     for (int u = 0; u < kNB; ++u) xj[u] = tiled_ ? tile_.X(jn[u]) : A.x[jn[u]];
+    if (ws) {
+#pragma unroll
+      for (int u = 0; u < kNB; ++u) vj[u] = A.v[jn[u]];
+    }
     // An empty slot holds the particle itself (d = 0): its W term is
-    // replaced by +0 and its gradient terms are signed zeros, so every slot
-    // is added without a branch (sums that start at +0 never become -0).
+    // replaced by +0 and its gradient and divergence terms are signed zeros,
+    // so every slot is added without a branch (sums that start at +0 never
+    // become -0).
 #pragma unroll
     for (int u = 0; u < kNB; ++u) {
       vec3 d = wrapd<W>(xi - ld3(xj[u]), p);
       const float w = W_d2(length_sqr(d), p);
       dens += p.vol * (t0 + u < nn ? w : 0.0f);
       if (with_factor) {
-        vec3 gp = (-p.vol) * gradW_rn(d, p);
+        const vec3 g = gradW_rn(d, p);
+        vec3 gp = (-p.vol) * g;
         sgk += length_sqr(gp);
         gpi -= gp;
+        if (ws) div += p.vol * dot(vi - ld3(vj[u]), g);
       }
     }
   }
@@ -714,9 +733,11 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
   while (mask) {
     const unsigned b = __builtin_ctz(mask);
     mask &= mask - 1u;
-    float4 bk = A.bker[size_t(b) * A.cap + s];
+    const size_t o = size_t(b) * A.cap + s;
+    float4 bk = A.bker[o];
     dens += bk.w;
     gpi += mk(bk.x, bk.y, bk.z);
+    if (ws) div += dot(vi - rigid_velocity(A.bnd[b], ld3(A.brec[o])), mk(bk.x, bk.y, bk.z));
   }
   A.nnb[s] = nn;
   const float rho = dens * p.rho0;
@@ -726,15 +747,21 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
     sgk += length_sqr(gpi);
     A.alpha[s] = sgk > p.eps ? -1.0f / sgk : 0.0f;
   }
+  if (ws) {
+    // density_change, then the warm start of the stored stiffness
+    float da = std_max(div, 0.0f);
+    if (nn < 20u) da = 0.0f;
+    const float dt = A.ctl->dt;
+    const float kv_prev = A.kappa_v[s];
+    const float kv = (da > 0.0f) ? 0.5f * std_max(kv_prev, -0.5f) / dt : 0.0f;
+    A.kappa_v[s] = kv;
+    set_kw(A, s, kv);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // DFSPH building blocks
 // ---------------------------------------------------------------------------
-__device__ inline vec3 rigid_velocity(const BoundaryDev& B, vec3 bx) {
-  return cross(mk(B.omega[0], B.omega[1], B.omega[2]), bx - mk(B.x[0], B.x[1], B.x[2])) +
-         mk(B.v[0], B.v[1], B.v[2]);
-}
 
 // sum_j V (v_i - v_j) . gradW_ij + sum_b (v_i - v_b) . Psi gradW_b
 template <bool W, class Src>
@@ -2577,15 +2604,17 @@ void launch_neighbors(const Launch& L, const DevArrays& A, int with_factor, hipS
   ALV_LAUNCH_WT(k_neighbors, L, s, A, with_factor);
 }
 void launch_df_warm_start_div(const Launch& L, const DevArrays& A, hipStream_t s,
-                              int features, const MlpDev* net, int net_divergence) {
-  if (net && features)
-    ALV_LAUNCH_W2(k_ws_div0_learned, true, L, s, A, *net, net_divergence);
-  else if (net)
-    ALV_LAUNCH_W2(k_ws_div0_learned, false, L, s, A, *net, net_divergence);
-  else if (features)
-    ALV_LAUNCH_W2TP(k_ws_div0, true, L, s, A);
-  else
-    ALV_LAUNCH_W2TP(k_ws_div0, false, L, s, A);
+                              int features, const MlpDev* net, int net_divergence, int fused) {
+  if (!fused) {  // else warm start 0 ran in the neighbour search (mode 2)
+    if (net && features)
+      ALV_LAUNCH_W2(k_ws_div0_learned, true, L, s, A, *net, net_divergence);
+    else if (net)
+      ALV_LAUNCH_W2(k_ws_div0_learned, false, L, s, A, *net, net_divergence);
+    else if (features)
+      ALV_LAUNCH_W2TP(k_ws_div0, true, L, s, A);
+    else
+      ALV_LAUNCH_W2TP(k_ws_div0, false, L, s, A);
+  }
   ALV_LAUNCH_WT(k_ws_div1, L, s, A);
   ALV_LAUNCH_WTP(k_vel_density_change, L, s, A);
 }
