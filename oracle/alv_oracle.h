@@ -238,8 +238,8 @@ int orc_sample_bead_pellet(void* h, const double* sample_x, unsigned ns, unsigne
  * (out [n][4]), 1 = density with pellets, 2 = DFSPH factor with pellets
  * (out [n]).  bead_xixj may be null for what 0.  The DFSPH factor with
  * pellets (runner.hpp:3156-3190) takes the pellet entries' vectors as
- * grad_wvol, as the reference does: parity unpinned beyond its fluid part
- * (no closed form or fixture covers that reading). */
+ * grad_wvol, as the reference does (pinned by equivalence: pellet entries
+ * V gradW(r) give the factor of fluid neighbours at r). */
 /* Sampling with pellet lists (what 0 density, 1 velocity, 2 vorticity;
  * runner.hpp:2122-2147, 4242-4359): lists [ns][K] (xixj packed xyz),
  * density and v indexed by particle id (v [max_n + max_p][3]), sample_v

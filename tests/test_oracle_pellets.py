@@ -377,3 +377,35 @@ def test_sampling_with_pellets_on_a_lattice():
     gw, _ = of.cubic_kernel(D, H)
     c = np.sum(gw[:, 0] * D[:, 0]) / S
     np.testing.assert_allclose(vort, np.tile(2 * c * w, (50, 1)), rtol=1e-5)
+
+
+def test_dfsph_factor_pellet_entries_enter_as_fluid_kernel_gradients():
+    # compute_dfsph_factor_with_pellets (runner.hpp:3156-3190) takes each
+    # pellet entry's vector g as V gradW: a row whose pellet entries are
+    # g_k = V gradW(r_k) has the factor of the same row with fluid neighbours
+    # at r_k appended to its fluid list (same operations in the same order,
+    # so the same bits).  A flipped sign, a dropped |g|^2 term or a pellet
+    # left out of grad_p_i changes the factor.
+    sc = small_scene()
+    orc = of.Oracle(sc, precision=8, kind="i")
+    K = orc.kmax
+    vol = sc.particle_mass / sc.density0
+    rng = np.random.default_rng(21)
+    n = 40
+    fluid = rng.uniform(-0.7, 0.7, (n, K, 3)) * H
+    pel_r = rng.uniform(-0.7, 0.7, (n, K, 3)) * H
+    nf = rng.integers(5, K // 2, n)
+    npel = rng.integers(1, K // 2 - 1, n)
+    g, _ = of.cubic_kernel(pel_r.reshape(-1, 3), H)
+    pel_g = (vol * g).reshape(n, K, 3)
+    a_pel = orc.pellet_sums(2, pel_g, npel, fluid, nf)
+    merged = fluid.copy()
+    for i in range(n):
+        merged[i, nf[i]:nf[i] + npel[i]] = pel_r[i, :npel[i]]
+    a_fluid = orc.pellet_sums(2, np.zeros_like(pel_g), np.zeros(n, np.uint32), merged,
+                              nf + npel)
+    np.testing.assert_array_equal(a_pel, a_fluid)
+    assert np.all(a_pel < 0)
+    # the same rows with the pellet vectors negated differ
+    a_neg = orc.pellet_sums(2, -pel_g, npel, fluid, nf)
+    assert np.all(a_neg != a_pel)
