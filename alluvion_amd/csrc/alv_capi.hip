@@ -1993,6 +1993,18 @@ void fill_report(alv_solver_report* r, const StepCtl& c) {
 }
 
 int finish_step(alv_solver* S, const Launch& L) {
+  // rigid force sums (solver_df.hpp:347-353) are enqueued with the control
+  // block copy, so one wait brings back both
+  alv_pile* pile = S->pile;
+  bool any_dynamic = false;
+  for (const auto& B : pile->bnd) any_dynamic |= B.dynamic != 0;
+  const bool body_sums = !S->batch && any_dynamic && L.active;
+  if (body_sums) {
+    launch_body_sums(L, S->A, S->stream);
+    ALV_HIP(hipGetLastError());
+    ALV_HIP(hipMemcpyAsync(S->body_host, S->A.body_sums, pile->bnd.size() * 6 * sizeof(double),
+                           hipMemcpyDeviceToHost, S->stream));
+  }
   int rc = read_ctl(S);
   if (rc != ALV_OK) return rc;
   if (g_event_profiler) g_event_profiler->collect();
@@ -2027,16 +2039,7 @@ int finish_step(alv_solver* S, const Launch& L) {
   fill_report(&S->rep, c);
   if (c.div_iter) S->last_div = c.div_iter;
   if (c.dens_iter) S->last_dens = c.dens_iter;
-  // rigid force sums (solver_df.hpp:347-353)
-  alv_pile* pile = S->pile;
-  bool any_dynamic = false;
-  for (const auto& B : pile->bnd) any_dynamic |= B.dynamic != 0;
-  if (any_dynamic && L.active) {
-    launch_body_sums(L, S->A, S->stream);
-    ALV_HIP(hipGetLastError());
-    ALV_HIP(hipMemcpyAsync(S->body_host, S->A.body_sums, pile->bnd.size() * 6 * sizeof(double),
-                           hipMemcpyDeviceToHost, S->stream));
-    ALV_HIP(hipStreamSynchronize(S->stream));
+  if (body_sums) {
     for (size_t b = 0; b < pile->bnd.size(); ++b) {
       if (!pile->bnd[b].dynamic) continue;
       for (int a = 0; a < 3; ++a) {
