@@ -2276,10 +2276,12 @@ int alv_solver_step(alv_solver* S) {
     const unsigned long long keyB = base ^ ((unsigned long long)cdens << 8) ^ (div << 1) ^
                                     (dens << 2) ^ (learned_dens << 3);
     rc = run_segment(S, 1, keyB, [&] {
-      if (L.active) launch_df_viscosity(L, A, div, st);
+      // with the divergence solve, its density-change passes computed the
+      // viscosity
+      if (L.active && !div) launch_df_viscosity(L, A, st);
       launch_update_dt(L, A, st);
       if (L.active) {
-        launch_df_integrate_np(L, A, st);
+        launch_df_integrate_np(L, A, div, st);
         if (dens) {
           launch_df_warm_start_dens(L, A, st, learned_dens);
           for (unsigned k = 0; k < cdens; ++k) launch_df_dens_iteration(L, A, st);

@@ -92,10 +92,10 @@ __device__ bool test_vertex(unsigned i, unsigned j, unsigned flags, unsigned nb,
     int ip[3];
     vec3 inner;
     if (grid_resolve(Bj.geom, wrt_j, ip, inner)) {
-      unsigned cell[32];
-      grid_cell_nodes(Bj.geom.res, ip, cell);
       vec3 g;
-      dist = Bj.sign * grid_interpolate_and_derive(Bj.distance, cell, inner, Bj.geom.cs, g) -
+      float unused;
+      dist = Bj.sign * grid_map_eval<false>(Bj.distance, nullptr, Bj.geom.res, ip, inner,
+                                            Bj.geom.cs, g, unused) -
              tolerance;
       n = g * Bj.sign;
     }

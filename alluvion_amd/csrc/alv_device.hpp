@@ -189,12 +189,11 @@ __device__ inline float box_dist(vec3 x, vec3 half) {
   vec3 cl = mk(fmaxf(diff.x, 0.0f), fmaxf(diff.y, 0.0f), fmaxf(diff.z, 0.0f));
   return sqrtf(length_sqr(cl)) + fminf(fmaxf(diff.x, fmaxf(diff.y, diff.z)), 0.0f);
 }
-__device__ inline float map_distance(const BoundaryDev& b, vec3 x);
-// Signed distance of boundary b at x (body frame, before the boundary's
-// sign); a distance map gives FLT_MAX outside the map
-// (interpolate_distance_without_intermediates, runner.hpp:1596-1614).
+// Signed distance of an analytic boundary b at x (body frame, before the
+// boundary's sign).  A distance map is sampled with map_distance instead:
+// keeping its 32-node interpolation out of this function keeps it out of the
+// register count of every kernel that evaluates an analytic distance.
 __device__ inline float sdf_eval(const BoundaryDev& b, vec3 x) {
-  if (b.type == ALV_DIST_MAP) return map_distance(b, x);
   if (b.type == ALV_DIST_BOX) return box_dist(x, mk(b.half[0], b.half[1], b.half[2])) - b.outset;
   if (b.type == ALV_DIST_BOX_SHELL) {
     float inner = box_dist(x, mk(b.half[0], b.half[1], b.half[2])) - b.outset;
@@ -358,61 +357,96 @@ __device__ inline float grid_interpolate(const float* __restrict__ nodes,
   return phi;
 }
 
-// Value and gradient (in world units: d/dx = (2 / cell size) d/dxi) of the
-// 32-node element at xi (interpolate_and_derive, runner.hpp:1773-1791).  The
-// shape functions' derivatives follow from the closed forms above:
+// A distance map's distance and its gradient in world units (d/dx = (2 /
+// cell size) d/dxi) at local coordinate xi of cell ip (interpolate_and_derive,
+// runner.hpp:1773-1791) and, with Vol, the volume (grid_interpolate over
+// `vol`), in one pass over the 32 nodes.  The shape functions' derivatives
+// follow from the closed forms above:
 //   corner: dN/dx = (1/64)(1+b y)(1+c z)[a (9 r^2 - 19) + 18 x (1+a x)];
 //   edge along u: dN/du = (9/64)(1+p v1)(1+q v2)(3 s (1 - u^2) - 2 u (1+3 s u)),
 //   dN/dv1 = (9/64)(1-u^2)(1+3 s u) p (1+q v2), likewise dN/dv2.
-__device__ inline float grid_interpolate_and_derive(const float* __restrict__ nodes,
-                                                    const unsigned* cell, vec3 xi,
-                                                    const float cs[3], vec3& grad) {
+// The volume uses grid_interpolate's expressions and node order, so it is
+// that function's value bit for bit.  The edge nodes are visited one axis at a
+// time in a loop that is not unrolled, so at most 16 node loads are in flight
+// rather than 64: fully unrolled, the interpolations set the register count
+// (and so the occupancy) of every kernel that holds them, analytic
+// boundaries included.
+template <bool Vol>
+__device__ inline float grid_map_eval(const float* __restrict__ dist,
+                                      const float* __restrict__ vol, const unsigned res[3],
+                                      const int ip[3], vec3 xi, const float cs[3], vec3& grad,
+                                      float& volume) {
   const float x = xi.x, y = xi.y, z = xi.z;
   const float k64 = 1.0f / 64.0f;
   const float k964 = 9.0f / 64.0f;
   const float F = k64 * (9.0f * (x * x + y * y + z * z) - 19.0f);
-  float phi = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  const unsigned rx = res[0], ry = res[1], rz = res[2];
+  const unsigned i = ip[0], j = ip[1], k = ip[2];
+  const unsigned sx = rx + 1, sxy = (rx + 1) * (ry + 1);
+  float phi = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f, pv = 0.0f;
+  {
+    float dn[8], vn[8];
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    float a = (c & 1) ? 1.0f : -1.0f;
-    float b = (c & 2) ? 1.0f : -1.0f;
-    float cc = (c & 4) ? 1.0f : -1.0f;
-    float A = 1.0f + a * x, B = 1.0f + b * y, C = 1.0f + cc * z;
-    const float v = nodes[cell[c]];
-    phi += v * (F * (A * B) * C);
-    gx += v * ((a * F + A * (k64 * (18.0f * x))) * (B * C));
-    gy += v * ((b * F + B * (k64 * (18.0f * y))) * (A * C));
-    gz += v * ((cc * F + C * (k64 * (18.0f * z))) * (A * B));
+    for (int c = 0; c < 8; ++c) {
+      const unsigned n = sxy * (k + ((c >> 2) & 1)) + sx * (j + ((c >> 1) & 1)) + (i + (c & 1));
+      dn[c] = dist[n];
+      if (Vol) vn[c] = vol[n];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      float a = (c & 1) ? 1.0f : -1.0f;
+      float b = (c & 2) ? 1.0f : -1.0f;
+      float cc = (c & 4) ? 1.0f : -1.0f;
+      float A = 1.0f + a * x, B = 1.0f + b * y, C = 1.0f + cc * z;
+      const float v = dn[c];
+      phi += v * (F * (A * B) * C);
+      gx += v * ((a * F + A * (k64 * (18.0f * x))) * (B * C));
+      gy += v * ((b * F + B * (k64 * (18.0f * y))) * (A * C));
+      gz += v * ((cc * F + C * (k64 * (18.0f * z))) * (A * B));
+      if (Vol) pv += vn[c] * (F * (A * B) * C);
+    }
   }
-#pragma unroll
+  // edge nodes of axis a: base_a + 2 (So (o + (e & 1)) + Sm (m + (e >> 1)) + c) + s
+  const unsigned nv = sxy * (rz + 1);
+  const unsigned ne_x = rx * (ry + 1) * (rz + 1), ne_y = (rx + 1) * ry * (rz + 1);
+#pragma unroll 1
   for (int axis = 0; axis < 3; ++axis) {
-    float u = axis == 0 ? x : (axis == 1 ? y : z);
-    float v1 = axis == 0 ? y : x;
-    float v2 = axis == 2 ? y : z;
-    float one_m_u2 = 1.0f - u * u;
+    const float u = axis == 0 ? x : (axis == 1 ? y : z);
+    const float v1 = axis == 0 ? y : x;
+    const float v2 = axis == 2 ? y : z;
+    const float one_m_u2 = 1.0f - u * u;
+    unsigned base, So, Sm, o, m, c0;
+    if (axis == 0) {
+      base = nv, So = rx * (ry + 1), Sm = rx, o = k, m = j, c0 = i;
+    } else if (axis == 1) {
+      base = nv + 2 * ne_x, So = ry * (rz + 1), Sm = ry, o = i, m = k, c0 = j;
+    } else {
+      base = nv + 2 * (ne_x + ne_y), So = rz * (rx + 1), Sm = rz, o = j, m = i, c0 = k;
+    }
+    float dn[8], vn[8];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      float pp, qq;
-      if (axis == 0) {
-        pp = (e >> 1) ? 1.0f : -1.0f;
-        qq = (e & 1) ? 1.0f : -1.0f;
-      } else if (axis == 1) {
-        pp = (e & 1) ? 1.0f : -1.0f;
-        qq = (e >> 1) ? 1.0f : -1.0f;
-      } else {
-        pp = (e >> 1) ? 1.0f : -1.0f;
-        qq = (e & 1) ? 1.0f : -1.0f;
+      const unsigned n = base + 2 * (So * (o + (e & 1)) + Sm * (m + (e >> 1)) + c0);
+      dn[2 * e] = dist[n];
+      dn[2 * e + 1] = dist[n + 1];
+      if (Vol) {
+        vn[2 * e] = vol[n];
+        vn[2 * e + 1] = vol[n + 1];
       }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int pb = axis == 1 ? (e & 1) : (e >> 1), qb = axis == 1 ? (e >> 1) : (e & 1);
+      const float pp = pb ? 1.0f : -1.0f, qq = qb ? 1.0f : -1.0f;
       float P = 1.0f + pp * v1, Q = 1.0f + qq * v2;
 #pragma unroll
       for (int s2 = 0; s2 < 2; ++s2) {
         float sg = s2 ? 1.0f : -1.0f;
         float L = 1.0f + 3.0f * sg * u;
-        int n = 8 + axis * 8 + 2 * e + s2;
-        const float v = nodes[cell[n]];
+        const float v = dn[2 * e + s2];
         float G = k964 * (one_m_u2 * L);
         phi += v * (G * (P * Q));
-        // derivatives along the edge's axis and the two transverse axes
+        if (Vol) pv += vn[2 * e + s2] * (G * (P * Q));
         float du = k964 * (3.0f * sg * one_m_u2 - 2.0f * u * L) * (P * Q);
         float dv1 = G * (pp * Q);
         float dv2 = G * (P * qq);
@@ -426,6 +460,7 @@ __device__ inline float grid_interpolate_and_derive(const float* __restrict__ no
     }
   }
   grad = mk(gx * (2.0f / cs[0]), gy * (2.0f / cs[1]), gz * (2.0f / cs[2]));
+  if (Vol) volume = pv;
   return phi;
 }
 
@@ -483,12 +518,10 @@ __device__ inline float boundary_record(const BoundaryDev& B, vec3 x, const SimP
     d = 0.0f;
     normal = mk(0.f, 0.f, 0.f);
     if (in_map) {
-      unsigned cell[32];
-      grid_cell_nodes(B.geom.res, ip, cell);
       vec3 g;
-      d = grid_interpolate_and_derive(B.distance, cell, inner, B.geom.cs, g) * B.sign;
+      d = grid_map_eval<true>(B.distance, B.volume, B.geom.res, ip, inner, B.geom.cs, g, vol) *
+          B.sign;
       normal = rotate(g * B.sign, q);
-      vol = grid_interpolate(B.volume, cell, inner);
     }
   } else {
     d = sdf_eval(B, local) * B.sign;

@@ -218,10 +218,13 @@ void launch_neighbors(const Launch& L, const DevArrays& A, int with_factor, hipS
 void launch_df_warm_start_div(const Launch& L, const DevArrays& A, hipStream_t s,
                               int features, const MlpDev* net, int net_divergence, int fused);
 void launch_df_div_iteration(const Launch& L, const DevArrays& A, hipStream_t s);
-void launch_df_viscosity(const Launch& L, const DevArrays& A, int div_finish,
-                         hipStream_t s);
+// the viscosity of a step without a divergence solve (with one, that solve's
+// density-change passes compute it)
+void launch_df_viscosity(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_update_dt(const Launch& L, const DevArrays& A, hipStream_t s);
-void launch_df_integrate_np(const Launch& L, const DevArrays& A, hipStream_t s);
+// fin: the divergence solve ran (and computed the viscosity): also finish it
+// and add the bodies' viscous reactions
+void launch_df_integrate_np(const Launch& L, const DevArrays& A, int fin, hipStream_t s);
 void launch_df_warm_start_dens(const Launch& L, const DevArrays& A, hipStream_t s,
                                int learned);
 // training pairs of the last step in sorted order: x [n][8], y [n][2]

@@ -492,7 +492,7 @@ __global__ __launch_bounds__(256) void k_boundary_mask(BoundaryDev B, const floa
   const float4 q = make_float4(B.q[0], B.q[1], B.q[2], B.q[3]);
   const vec3 xi = mk(x[3 * size_t(i)], x[3 * size_t(i) + 1], x[3 * size_t(i) + 2]);
   const vec3 local = rotate(xi - mk(B.x[0], B.x[1], B.x[2]), conjugate(q));
-  const float d = sdf_eval(B, local) * B.sign;
+  const float d = (B.type == ALV_DIST_MAP ? map_distance(B, local) : sdf_eval(B, local)) * B.sign;
   mask[i] *= (d >= threshold) ? 1.0f : 0.0f;
 }
 void launch_boundary_mask(const BoundaryDev& B, const float* x, float threshold, float* mask,

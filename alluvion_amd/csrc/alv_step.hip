@@ -81,6 +81,14 @@ __device__ inline void add_v(DevArrays& A, unsigned s, vec3 dv) {
   A.v[s] = make_float4(v.x + dv.x, v.y + dv.y, v.z + dv.z, v.w);
 }
 __device__ inline float std_min(float a, float b) { return (b < a) ? b : a; }
+// The viscosity's coefficient of a neighbour, mu m / rho_j, which
+// compute_viscosity (runner.hpp:2150-2249) divides out for every pair: it is
+// stored with the particle's velocity (v.w) when its density is computed, so
+// one correctly rounded division per particle gives the quotient of every
+// pair.
+__device__ inline float visc_coefficient(const SimParams& p, float rho) {
+  return (p.visc * p.mass) / rho;
+}
 
 // ---------------------------------------------------------------------------
 // wave / block reductions (wave64)
@@ -742,7 +750,7 @@ __global__ __launch_bounds__(256, (!W && !TL) ? 4 : 8) void k_neighbors(SceneTab
   A.nnb[s] = nn;
   const float rho = dens * p.rho0;
   A.density[s] = rho;
-  reinterpret_cast<float*>(A.v + s)[3] = rho;
+  reinterpret_cast<float*>(A.v + s)[3] = visc_coefficient(p, rho);
   if (with_factor) {
     sgk += length_sqr(gpi);
     A.alpha[s] = sgk > p.eps ? -1.0f / sgk : 0.0f;
@@ -868,20 +876,39 @@ __device__ inline void lane_slots(const DevArrays& A, unsigned s, unsigned q, un
 // +0, so gradW is a signed zero and so is the term), so every slot's term is
 // added without a test: a sum that starts at +0 never becomes -0 under
 // round-to-nearest, and x + (+-0) = x for every other x.
-template <bool W>
+// V: also the fluid part of the viscosity, sum_j mu m / rho_j (x_ij .
+// gradW_ij) / (|x_ij|^2 + 0.01 h^2) (v_i - v_j), into *visc (every lane of
+// the particle holds it): the same gathers (mu m / rho_j in v_j.w) and the
+// same gradW_ij, term for term viscosity_sum's, added in list order.
+template <bool W, bool V = false>
 __device__ inline float velocity_divergence_q(const SimParams& p, const DevArrays& A,
                                               unsigned s, unsigned q, vec3 xi, vec3 vi,
-                                              unsigned nn) {
+                                              unsigned nn, vec3* visc = nullptr) {
   float acc = 0.0f;
+  vec3 av = mk(0.f, 0.f, 0.f);
+  const float e2 = 0.01f * p.h2;
   for (unsigned t0 = 0; t0 < nn; t0 += 2u * kLPP) {
     unsigned ja, jb;
     lane_slots(A, s, q, t0, nn, ja, jb);
     const float4 xa = A.x[ja], va = A.v[ja], xb = A.x[jb], vb = A.v[jb];
-    const float ta = p.vol * dot(vi - ld3(va), gradW_rn(wrapd<W>(xi - ld3(xa), p), p));
-    const float tb = p.vol * dot(vi - ld3(vb), gradW_rn(wrapd<W>(xi - ld3(xb), p), p));
-    acc = add_in_slot_order(acc, ta);
-    acc = add_in_slot_order(acc, tb);
+    const vec3 da = wrapd<W>(xi - ld3(xa), p), db = wrapd<W>(xi - ld3(xb), p);
+    const vec3 ga = gradW_rn(da, p), gb = gradW_rn(db, p);
+    const vec3 ua = vi - ld3(va), ub = vi - ld3(vb);
+    acc = add_in_slot_order(acc, p.vol * dot(ua, ga));
+    acc = add_in_slot_order(acc, p.vol * dot(ub, gb));
+    if constexpr (V) {
+      // an empty slot's term is an exact zero here too (x_ij = 0)
+      const vec3 wa = div_rn(va.w * dot(da, ga), length_sqr(da) + e2) * ua;
+      const vec3 wb = div_rn(vb.w * dot(db, gb), length_sqr(db) + e2) * ub;
+      av.x = add_in_slot_order(av.x, wa.x);
+      av.y = add_in_slot_order(av.y, wa.y);
+      av.z = add_in_slot_order(av.z, wa.z);
+      av.x = add_in_slot_order(av.x, wb.x);
+      av.y = add_in_slot_order(av.y, wb.y);
+      av.z = add_in_slot_order(av.z, wb.z);
+    }
   }
+  if constexpr (V) *visc = av;
   unsigned mask = A.bmask[s];
   while (mask) {
     const unsigned b = __builtin_ctz(mask);
@@ -992,6 +1019,138 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
     }
   }
   return dv;
+}
+
+// sum_j mu m / rho_j (x_ij . gradW_ij) / (|x_ij|^2 + 0.01 h^2) (v_i - v_j)
+template <bool W, class Src>
+__device__ inline vec3 viscosity_sum(const SimParams& p, const DevArrays& A, unsigned s,
+                                     vec3 xi, vec3 vi, unsigned nn, const Src& src) {
+  vec3 da = mk(0.f, 0.f, 0.f);
+  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
+    unsigned j[kNB];
+    batch_ids(A, s, nn, t0, j);
+    float4 xj[kNB], vj[kNB];  // vj.w = mu m / rho_j
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      xj[u] = src.X(j[u]);
+      vj[u] = src.V(j[u]);
+    }
+    // An empty slot holds the particle itself: x_ij = 0 gives a zero
+    // numerator and a zero velocity difference, so its term is an exact zero
+    // and every slot is added without a test (da never becomes -0).  The
+    // divisor |x_ij|^2 + 0.01 h^2 is normal: div_rn.
+#pragma unroll
+    for (int u = 0; u < kNB; ++u) {
+      vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
+      da += div_rn(vj[u].w * dot(xixj, gradW_rn(xixj, p)),
+                   length_sqr(xixj) + 0.01f * p.h2) *
+            (vi - ld3(vj[u]));
+    }
+  }
+  return da;
+}
+
+__device__ inline void orthogonal_vectors(vec3 n, vec3& t1, vec3& t2) {
+  vec3 v = mk(1.f, 0.f, 0.f);
+  if (fabsf(dot(v, n)) > 0.999f) {
+    v.x = 0.f;
+    v.y = 1.f;
+  }
+  t1 = cross(n, v);
+  t2 = cross(n, t1);
+  t1 = t1 * (1.0f / sqrtf(length_sqr(t1)));
+  t2 = t2 * (1.0f / sqrtf(length_sqr(t2)));
+}
+
+// The boundary terms of compute_viscosity: four points around the closest
+// boundary point of every boundary the particle sees, added to the running
+// sum da in the reference's order; with Force, also each body's reaction
+// (its per-particle force and torque accumulators) where the body
+// accumulates forces.
+template <bool Force>
+__device__ inline vec3 boundary_viscosity(const SimParams& p, const DevArrays& A, unsigned s,
+                                          vec3 xi, vec3 vi, vec3 da) {
+  unsigned mask = A.bmask[s];
+  while (mask) {
+    const unsigned b = __builtin_ctz(mask);
+    mask &= mask - 1u;
+    const size_t o = size_t(b) * A.cap + s;
+    const BoundaryDev& B = A.bnd[b];
+    float4 rec = A.brec[o];
+    vec3 bx = ld3(rec);
+    vec3 r_x = mk(B.x[0], B.x[1], B.x[2]);
+    vec3 normal = bx - xi;
+    float nl2 = length_sqr(normal);
+    if (nl2 > 0.0f) {
+      normal = normal * (1.0f / sqrtf(nl2));
+      vec3 t1, t2;
+      orthogonal_vectors(normal, t1, t2);
+      float dist = p.h - sqrtf(nl2);
+      vec3 xs[4] = {bx - t1 * dist, bx + t1 * dist, bx - t2 * dist, bx + t2 * dist};
+      float vol = 0.25f * rec.w;
+      vec3 as[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        vec3 xix = xi - xs[q];
+        vec3 gw = gradW(xix, p);
+        vec3 vq = rigid_velocity(B, xs[q]);
+        as[q] = (p.bvisc * vol * dot(xix, gw) / (length_sqr(xix) + 0.01f * p.h2)) * (vi - vq);
+      }
+      da += as[0] + as[1] + as[2] + as[3];
+      if (Force && B.accum) {
+        vec3 f[4], tq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          f[q] = (-p.mass) * as[q];
+          tq[q] = cross(xs[q] - r_x, f[q]);
+        }
+        float4 F = A.bforce[o], T = A.btorque[o];
+        A.bforce[o] = f4(ld3(F) + (f[0] + f[1] + f[2] + f[3]), 0.f);
+        A.btorque[o] = f4(ld3(T) + (tq[0] + tq[1] + tq[2] + tq[3]), 0.f);
+      }
+    }
+  }
+  return da;
+}
+
+// a = g + da with da the fluid sum plus the boundary terms (no reactions),
+// stored; returns the particle's CFL term |v + a dt|^2.
+__device__ inline float viscosity_finish(const SimParams& p, const DevArrays& A, unsigned s,
+                                         vec3 xi, vec3 vi, vec3 da, float dt) {
+  da = boundary_viscosity<false>(p, A, s, xi, vi, da);
+  const vec3 a = mk(p.g[0], p.g[1], p.g[2]) + da;
+  A.a[s] = f4(a, 0.f);
+  return length_sqr(vi + a * dt);
+}
+
+// The density change as pass_density_change and, over the same walk of the
+// list, the fluid part of the viscosity (viscosity_sum's terms in its order;
+// from an LDS tile, one lane per particle, the two sums walk the list
+// separately).  xi and vi are the particle's position and velocity.
+template <bool W, bool TL>
+__device__ inline float pass_density_change_visc(const SimParams& p, const DevArrays& A,
+                                                 unsigned s, unsigned q, bool tiled,
+                                                 const TileXV& tile, vec3& xi, vec3& vi,
+                                                 vec3& visc) {
+  xi = ld3(A.x[s]);
+  vi = ld3(A.v[s]);
+  const unsigned nn = A.nnb[s];
+  float da;
+  if constexpr (TL) {
+    if (tiled) {
+      da = velocity_divergence<W>(p, A, s, xi, vi, nn, tile);
+      visc = viscosity_sum<W>(p, A, s, xi, vi, nn, tile);
+    } else {
+      const GlobalXV g{A.x, A.v};
+      da = velocity_divergence<W>(p, A, s, xi, vi, nn, g);
+      visc = viscosity_sum<W>(p, A, s, xi, vi, nn, g);
+    }
+  } else {
+    da = velocity_divergence_q<W, true>(p, A, s, q, xi, vi, nn, &visc);
+  }
+  da = std_max(da, 0.0f);
+  if (nn < 20u) da = 0.0f;
+  return da;
 }
 
 // warm_start_divergence_solve_0 (runner.hpp:3230-3250)
@@ -1109,24 +1268,39 @@ __global__ __launch_bounds__(256) void k_ws_div1(SceneTab T, DevArrays A) {
   add_v(A, s, dv);
 }
 // compute_velocity_of_density_change (runner.hpp:3297-3314); also stores the
-// first iteration's stiffness k = rho_adv * alpha
+// first iteration's stiffness k = rho_adv * alpha.  Every density-change pass
+// of the divergence solve also computes the viscosity (compute_viscosity's
+// a = g + fluid and boundary terms, without the bodies' reactions) and the
+// per-block CFL maxima from the velocities it sees: the last pass that runs
+// sees the solve's final velocities, which k_viscosity would read.  Both
+// passes are bounded to 64 VGPRs (8 waves per SIMD); unbounded, the viscosity
+// terms take 66 (7 waves per SIMD), which measured 2 % slower.
 template <bool W, bool TL>
-__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock) void k_vel_density_change(SceneTab T, DevArrays A) {
+__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock, 8) void k_vel_density_change(SceneTab T, DevArrays A) {
+  __shared__ float lds[kLPP * kBlock / 64];
   ALV_SCENE_PASS(T, A, s, q, TL);
   TileXV tile_;
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
-  if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
-  const float dadv = pass_density_change<W, TL>(p, A, s, q, tiled_, tile_);
-  if (q != 0u) return;
-  A.rho_adv[s] = dadv;
-  float inv_dt = 1.0f / dt;
-  float alpha = A.alpha[s] * inv_dt;
-  A.alpha[s] = alpha;
-  const float k = dadv * alpha;
-  A.kfac[s] = k;
-  set_kw(A, s, k);
+  float cfl = -FLT_MAX;
+  if (s < p.first + p.n) {
+    vec3 xi, vi, visc;
+    const float dadv =
+        pass_density_change_visc<W, TL>(p, A, s, q, tiled_, tile_, xi, vi, visc);
+    if (q == 0u) {
+      A.rho_adv[s] = dadv;
+      float inv_dt = 1.0f / dt;
+      float alpha = A.alpha[s] * inv_dt;
+      A.alpha[s] = alpha;
+      const float k = dadv * alpha;
+      A.kfac[s] = k;
+      set_kw(A, s, k);
+      cfl = viscosity_finish(p, A, s, xi, vi, visc, dt);
+    }
+  }
+  const float m = block_max_f(cfl, lds);
+  if (threadIdx.x == 0) A.fpartials[qblk_] = m;
 }
 // divergence_solve_iteration (runner.hpp:3316-3363)
 template <bool W, bool TL>
@@ -1146,27 +1320,38 @@ __global__ __launch_bounds__(256) void k_div_iter(SceneTab T, DevArrays A) {
 }
 // compute_divergence_solve_density_error (runner.hpp:3396-3410) + FP64
 // partial sums of the residual
+// (and the viscosity, as k_vel_density_change)
 template <bool W, bool TL>
-__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock) void k_div_err(SceneTab T, DevArrays A) {
+__global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock, 8) void k_div_err(SceneTab T, DevArrays A) {
   __shared__ double lds[kLPP * kBlock / 64];
+  __shared__ float ldsm[kLPP * kBlock / 64];
   ALV_SCENE_PASS(T, A, s, q, TL);
   if (A.ctl->div_done) return;
   TileXV tile_;
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
+  const float dt = A.ctl->dt;
   double part = 0.0;
+  float cfl = -FLT_MAX;
   if (s < p.first + p.n) {
-    const float dadv = pass_density_change<W, TL>(p, A, s, q, tiled_, tile_);
+    vec3 xi, vi, visc;
+    const float dadv =
+        pass_density_change_visc<W, TL>(p, A, s, q, tiled_, tile_, xi, vi, visc);
     if (q == 0u) {
       A.rho_adv[s] = dadv;
       const float k = dadv * A.alpha[s];
       A.kfac[s] = k;
       set_kw(A, s, k);
       part = double(dadv);
+      cfl = viscosity_finish(p, A, s, xi, vi, visc, dt);
     }
   }
-  double t = block_sum_d(part, lds);
-  if (threadIdx.x == 0) A.partials[qblk_] = t;
+  const double t = block_sum_d(part, lds);
+  const float m = block_max_f(cfl, ldsm);
+  if (threadIdx.x == 0) {
+    A.partials[qblk_] = t;
+    A.fpartials[qblk_] = m;
+  }
 }
 __global__ __launch_bounds__(256) void k_div_finalize(SceneTab T, DevArrays A) {
   __shared__ double lds[4];
@@ -1201,55 +1386,17 @@ __global__ void k_ctl_reset(DevArrays A) {
   ctl->dens_done = dens_go ? 0u : 1u;
 }
 
-// sum_j mu m / rho_j (x_ij . gradW_ij) / (|x_ij|^2 + 0.01 h^2) (v_i - v_j)
-template <bool W, class Src>
-__device__ inline vec3 viscosity_sum(const SimParams& p, const DevArrays& A, unsigned s,
-                                     vec3 xi, vec3 vi, unsigned nn, const Src& src) {
-  vec3 da = mk(0.f, 0.f, 0.f);
-  for (unsigned t0 = 0; t0 < nn; t0 += kNB) {
-    unsigned j[kNB];
-    batch_ids(A, s, nn, t0, j);
-    float4 xj[kNB], vj[kNB];  // vj.w = density of j
-#pragma unroll
-    for (int u = 0; u < kNB; ++u) {
-      xj[u] = src.X(j[u]);
-      vj[u] = src.V(j[u]);
-    }
-    // An empty slot holds the particle itself: x_ij = 0 gives a zero
-    // numerator and a zero velocity difference, so its term is an exact zero
-    // and every slot is added without a test (da never becomes -0).  Both
-    // divisors are normal (rho_j, |x_ij|^2 + 0.01 h^2): div_rn.
-#pragma unroll
-    for (int u = 0; u < kNB; ++u) {
-      vec3 xixj = wrapd<W>(xi - ld3(xj[u]), p);
-      da += div_rn(div_rn(p.visc * p.mass, vj[u].w) * dot(xixj, gradW_rn(xixj, p)),
-                   length_sqr(xixj) + 0.01f * p.h2) *
-            (vi - ld3(vj[u]));
-    }
-  }
-  return da;
-}
-
 // ---------------------------------------------------------------------------
 // Viscosity (compute_viscosity, runner.hpp:2150-2249) with clear_acceleration
-// (a = g, runner.hpp:1974-1980), optional divergence_solve_finish
-// (runner.hpp:3412-3420) and the per-block maxima of calculate_cfl_v2
-// (runner.hpp:2533-2541).
+// (a = g, runner.hpp:1974-1980) and the per-block maxima of calculate_cfl_v2
+// (runner.hpp:2533-2541).  With the divergence solve on, the DFSPH step
+// computes it in that solve's density-change passes instead (the last one that
+// runs sees the final velocities) and adds the bodies' reactions in
+// k_integrate_np; k_viscosity serves the step without a divergence solve and
+// IISPH.
 // ---------------------------------------------------------------------------
-__device__ inline void orthogonal_vectors(vec3 n, vec3& t1, vec3& t2) {
-  vec3 v = mk(1.f, 0.f, 0.f);
-  if (fabsf(dot(v, n)) > 0.999f) {
-    v.x = 0.f;
-    v.y = 1.f;
-  }
-  t1 = cross(n, v);
-  t2 = cross(n, t1);
-  t1 = t1 * (1.0f / sqrtf(length_sqr(t1)));
-  t2 = t2 * (1.0f / sqrtf(length_sqr(t2)));
-}
-
 template <bool W, bool TL>
-__global__ __launch_bounds__(256) void k_viscosity(SceneTab T, DevArrays A, int div_finish) {
+__global__ __launch_bounds__(256) void k_viscosity(SceneTab T, DevArrays A) {
   __shared__ float lds[4];
   ALV_SCENE(T, A, s);
   TileXV tile_;
@@ -1262,53 +1409,10 @@ __global__ __launch_bounds__(256) void k_viscosity(SceneTab T, DevArrays A, int 
     const unsigned nn = A.nnb[s];
     vec3 da = tiled_ ? viscosity_sum<W>(p, A, s, xi, vi, nn, tile_)
                      : viscosity_sum<W>(p, A, s, xi, vi, nn, GlobalXV{A.x, A.v});
-    unsigned mask = A.bmask[s];
-    while (mask) {
-      const unsigned b = __builtin_ctz(mask);
-      mask &= mask - 1u;
-      const size_t o = size_t(b) * A.cap + s;
-      const BoundaryDev& B = A.bnd[b];
-      float4 rec = A.brec[o];
-      vec3 bx = ld3(rec);
-      vec3 r_x = mk(B.x[0], B.x[1], B.x[2]);
-      vec3 normal = bx - xi;
-      float nl2 = length_sqr(normal);
-      if (nl2 > 0.0f) {
-        normal = normal * (1.0f / sqrtf(nl2));
-        vec3 t1, t2;
-        orthogonal_vectors(normal, t1, t2);
-        float dist = p.h - sqrtf(nl2);
-        vec3 xs[4] = {bx - t1 * dist, bx + t1 * dist, bx - t2 * dist, bx + t2 * dist};
-        float vol = 0.25f * rec.w;
-        vec3 as[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          vec3 xix = xi - xs[q];
-          vec3 gw = gradW(xix, p);
-          vec3 vq = rigid_velocity(B, xs[q]);
-          as[q] = (p.bvisc * vol * dot(xix, gw) / (length_sqr(xix) + 0.01f * p.h2)) * (vi - vq);
-        }
-        da += as[0] + as[1] + as[2] + as[3];
-        if (B.accum) {
-          vec3 f[4], tq[4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            f[q] = (-p.mass) * as[q];
-            tq[q] = cross(xs[q] - r_x, f[q]);
-          }
-          float4 F = A.bforce[o], T = A.btorque[o];
-          A.bforce[o] = f4(ld3(F) + (f[0] + f[1] + f[2] + f[3]), 0.f);
-          A.btorque[o] = f4(ld3(T) + (tq[0] + tq[1] + tq[2] + tq[3]), 0.f);
-        }
-      }
-    }
+    da = boundary_viscosity<true>(p, A, s, xi, vi, da);
     vec3 a = mk(p.g[0], p.g[1], p.g[2]) + da;
     A.a[s] = f4(a, 0.f);
     cfl = length_sqr(vi + a * dt);
-    if (div_finish) {
-      A.alpha[s] *= dt;
-      A.kappa_v[s] *= dt;
-    }
   }
   float m = block_max_f(cfl, lds);
   if (threadIdx.x == 0) A.fpartials[xcd_block()] = m;
@@ -1340,16 +1444,29 @@ __global__ __launch_bounds__(256) void k_update_dt(SceneTab T, DevArrays A) {
     ctl->pile_max_v2 = pile_max;
     ctl->max_v2 = max_v2;
     ctl->cfl_dt = cfl_dt;
+    ctl->div_dt = ctl->dt;
     ctl->dt = dt;
     ctl->utilized_cfl = dt * max_v / ls;
   }
 }
 
-// integrate_non_pressure_acceleration (runner.hpp:3422-3428)
-__global__ __launch_bounds__(256) void k_integrate_np(SceneTab T, DevArrays A) {
+// integrate_non_pressure_acceleration (runner.hpp:3422-3428).  With fin (a
+// DFSPH step whose divergence solve computed the viscosity), first the rest
+// of what follows that solve in the reference: divergence_solve_finish
+// (runner.hpp:3412-3420) with the time step the solve ran with, and the
+// bodies' viscous reactions (compute_viscosity), from the same velocities,
+// which this kernel has not yet changed.
+__global__ __launch_bounds__(256) void k_integrate_np(SceneTab T, DevArrays A, int fin) {
   ALV_SCENE(T, A, s);
   if (s >= p.first + p.n) return;
   const float dt = A.ctl->dt;
+  if (fin) {
+    const float dts = A.ctl->div_dt;
+    A.alpha[s] *= dts;
+    A.kappa_v[s] *= dts;
+    if (A.bmask[s])
+      boundary_viscosity<true>(p, A, s, ld3(A.x[s]), ld3(A.v[s]), mk(0.f, 0.f, 0.f));
+  }
   add_v(A, s, dt * ld3(A.a[s]));
 }
 
@@ -2302,7 +2419,7 @@ __global__ __launch_bounds__(256) void k_pellet_fluid_lists(SceneTab T, DevArray
     dens += W_d2(length_sqr(wrapd<W>(xi - ld3(A.px[pellet_at(A.pnbr, A, s, t)]), p)), p);
   const float rho = dens * p.mass;
   A.density[s] = rho;
-  reinterpret_cast<float*>(A.v + s)[3] = rho;
+  reinterpret_cast<float*>(A.v + s)[3] = visc_coefficient(p, rho);
 }
 // The fluid lists of the pellets (the bead lists of the same search), which
 // the MLS extrapolation reads.  One thread per pellet.
@@ -2623,14 +2740,14 @@ void launch_df_div_iteration(const Launch& L, const DevArrays& A, hipStream_t s)
   ALV_LAUNCH_WTP(k_div_err, L, s, A);
   ALV_LAUNCH_SCENES(k_div_finalize, L, s, A);
 }
-void launch_df_viscosity(const Launch& L, const DevArrays& A, int div_finish, hipStream_t s) {
-  ALV_LAUNCH_WT(k_viscosity, L, s, A, div_finish);
+void launch_df_viscosity(const Launch& L, const DevArrays& A, hipStream_t s) {
+  ALV_LAUNCH_WT(k_viscosity, L, s, A);
 }
 void launch_update_dt(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_SCENES(k_update_dt, L, s, A);
 }
-void launch_df_integrate_np(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH(k_integrate_np, L, s, A);
+void launch_df_integrate_np(const Launch& L, const DevArrays& A, int fin, hipStream_t s) {
+  ALV_LAUNCH(k_integrate_np, L, s, A, fin);
 }
 void launch_df_warm_start_dens(const Launch& L, const DevArrays& A, hipStream_t s,
                                int learned) {
@@ -2660,7 +2777,7 @@ void launch_integrate_velocity(const Launch& L, const DevArrays& A, float dt, hi
   ALV_LAUNCH_W(k_integrate_velocity, L, s, A, dt);
 }
 void launch_ii_viscosity_predict(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_WT(k_viscosity, L, s, A, 0);
+  ALV_LAUNCH_WT(k_viscosity, L, s, A);
 }
 void launch_ii_predict(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_W(k_ii_predict0, L, s, A);

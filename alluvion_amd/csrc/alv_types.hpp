@@ -87,7 +87,8 @@ struct StepCtl {
   float mean_density_change, mean_density_error;
   float particle_max_v2, pile_max_v2, max_v2, cfl_dt, utilized_cfl;
   float length_scale;
-  unsigned pad[2];
+  float div_dt;  // the time step of the step's divergence solve (set by update_dt)
+  unsigned pad;
 };
 
 }  // namespace alv

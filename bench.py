@@ -43,25 +43,28 @@ METRIC = "particle-steps/s @1M DFSPH 0.1% ρ-err; learned vs plain warm start"
 METRIC_STRESS = "particle-steps/s @8M periodic: neighbour search + density + divergence sums"
 
 
-def kernel_bytes(name, K, nb, fused_ws=False):
+def kernel_bytes(name, K, nb, df_fused=False):
     """Algorithmic bytes per particle of one launch (minimal widths: FP32
     scalar 4 B, vector 12 B, u32 index 4 B; neighbour-state gathers counted
     once per particle as cache-resident; boundary records 16 B each only for
     (particle, boundary) pairs with non-zero volume, nb per particle).  The
-    neighbour list is counted at 4 B per entry.  fused_ws: the DFSPH step's
-    neighbour search also runs the divergence warm start (its own velocity,
-    the stored stiffness read and written, the stiffness copy into x.w, and
-    the boundary points)."""
+    neighbour list is counted at 4 B per entry.  df_fused: the DFSPH step with
+    a divergence solve, whose neighbour search also runs the divergence warm
+    start (its own velocity, the stored stiffness read and written, the
+    stiffness copy into x.w, and the boundary points) and whose divergence
+    passes also compute the viscosity (rho_j and the acceleration written),
+    the bodies' reactions and the solve's finish scaling (factor and stored
+    stiffness read and written, the boundary mask) moving to k_integrate_np."""
     nbr = 4 + 4 * K  # count + neighbour list
     table = {
-        "k_neighbors": 12 + 4 * K + 4 + 8 + 4 + 16 * nb + ((12 + 8 + 4 + 16 * nb) if fused_ws else 0),
+        "k_neighbors": 12 + 4 * K + 4 + 8 + 4 + 16 * nb + ((12 + 8 + 4 + 16 * nb) if df_fused else 0),
         "k_ws_div0": nbr + 12 + 12 + 8 + 4 + 32 * nb,
         "k_ws_div1": nbr + 4 + 12 + 24 + 4 + 16 * nb,
-        "k_vel_density_change": nbr + 12 + 12 + 8 + 4 + 4 + 4 + 32 * nb,
+        "k_vel_density_change": nbr + 12 + 12 + 8 + 4 + 4 + 4 + 32 * nb + (16 if df_fused else 0),
         "k_div_iter": nbr + 4 + 12 + 8 + 24 + 4 + 16 * nb,
-        "k_div_err": nbr + 12 + 12 + 4 + 4 + 4 + 4 + 32 * nb,
+        "k_div_err": nbr + 12 + 12 + 4 + 4 + 4 + 4 + 32 * nb + (16 if df_fused else 0),
         "k_viscosity": nbr + 12 + 12 + 12 + 16 + 4 + 16 * nb,
-        "k_integrate_np": 36,
+        "k_integrate_np": 36 + (20 if df_fused else 0),
         "k_ws_p0": nbr + 12 + 12 + 4 + 8 + 4 + 32 * nb,
         "k_ws_p1": nbr + 4 + 12 + 24 + 4 + 16 * nb,
         "k_rho_adv": nbr + 12 + 12 + 4 + 8 + 4 + 4 + 4 + 32 * nb,
@@ -169,7 +172,7 @@ def copy_calibration(dp):
 
 class DamBreak:
     name = "dam_break_1m_rigid"
-    fused_ws = True  # DFSPH: the neighbour search runs the divergence warm start
+    df_fused = True  # DFSPH: warm start in the search, viscosity in the divergence passes
     metric = METRIC
     default_steps = 100
     default_warmup = 20
@@ -308,7 +311,7 @@ class PeriodicStress:
 
 class BatchedScenes:
     name = "batched_256"
-    fused_ws = True
+    df_fused = True
     metric = "particle-steps/s @256 batched dam breaks (4k-120k particles each), DFSPH 0.1% ρ-err"
     default_steps = 500
     default_warmup = 10
@@ -612,13 +615,13 @@ def main():
         for name, (ms, launches) in stats.items():
             ent = {"total_ms": round(ms, 3), "launches": int(launches),
                    "avg_us": round(1000.0 * ms / max(launches, 1), 2)}
-            bpp = kernel_bytes(name, K, nb, getattr(wl, "fused_ws", False))
+            bpp = kernel_bytes(name, K, nb, getattr(wl, "df_fused", False))
             if bpp is not None and launches and ms > 0:
                 ent["algorithmic_gbs"] = round(bpp * n / (ms / 1000.0 / launches) / 1e9, 1)
             kernels[name] = ent
         dom = max(stats.items(), key=lambda kv: kv[1][0])
         name, (ms, launches) = dom
-        bpp = kernel_bytes(name, K, nb, getattr(wl, "fused_ws", False))
+        bpp = kernel_bytes(name, K, nb, getattr(wl, "df_fused", False))
         if bpp is not None and launches:
             avg_s = ms / 1000.0 / launches
             achieved = bpp * n / avg_s / 1e9
