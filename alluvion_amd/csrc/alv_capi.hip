@@ -361,8 +361,14 @@ struct alv_solver {
   alv_solver_report rep;
   DevArrays A;
   hipStream_t stream = nullptr;
-  StepCtl* ctl_host = nullptr;  // pinned
-  BoundaryDev* bnd_host = nullptr;  // pinned upload buffer
+  // Step inputs: the control blocks [NS] followed by the boundaries
+  // [bnd_capacity], in one pinned, coherent host block and one device block,
+  // so one copy uploads both; the kernels also publish the control blocks
+  // back into the host block (publish_ctl).
+  void* step_host = nullptr;
+  void* step_dev = nullptr;
+  StepCtl* ctl_host = nullptr;  // views into step_host / step_dev
+  BoundaryDev* bnd_host = nullptr;
   BoundaryDev* bnd_dev = nullptr;
   double* body_host = nullptr;  // pinned [B*6]
   unsigned bnd_capacity = 0;
@@ -435,9 +441,8 @@ struct alv_solver {
       if (p) (void)hipFree(p);
     for (float* p : export_ii_vec)
       if (p) (void)hipFree(p);
-    if (bnd_dev) (void)hipFree(bnd_dev);
-    if (ctl_host) (void)hipHostFree(ctl_host);
-    if (bnd_host) (void)hipHostFree(bnd_host);
+    if (step_dev) (void)hipFree(step_dev);
+    if (step_host) (void)hipHostFree(step_host);
     if (body_host) (void)hipHostFree(body_host);
     if (par_host) (void)hipHostFree(par_host);
     for (auto& g : seg)
@@ -1519,9 +1524,7 @@ static int solver_alloc(alv_solver* S) {
   }
   ALV_ALLOC(A.partials, grid_for(cap) + 1);
   ALV_ALLOC(A.fpartials, grid_for(cap) + 1);
-  ALV_ALLOC(A.body_sums, std::max<size_t>(1, max_bnd) * 6);
   ALV_ALLOC(A.body_partials, std::max<size_t>(1, max_bnd) * 256 * 6);
-  ALV_ALLOC(A.ctl, NS);
   if (pcap) {
     A.pcap = pcap;
     const size_t pspan = size_t(grid_for(pcap)) * kBlock;
@@ -1565,14 +1568,22 @@ static int solver_alloc(alv_solver* S) {
   ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->par_host), NS * sizeof(SimParams),
                         hipHostMallocDefault));
   memset(S->par_host, 0, NS * sizeof(SimParams));
-  ALV_HIP(hipMalloc(&S->bnd_dev, S->bnd_capacity * sizeof(BoundaryDev)));
-  ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->bnd_host),
-                        S->bnd_capacity * sizeof(BoundaryDev), hipHostMallocDefault));
-  ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->ctl_host), NS * sizeof(StepCtl),
-                        hipHostMallocDefault));
+  // The control blocks and body sums come back through coherent pinned
+  // memory that the kernels store into (no copy command per readback).
+  static_assert(sizeof(StepCtl) % alignof(BoundaryDev) == 0, "boundaries follow the blocks");
+  const size_t step_bytes = NS * sizeof(StepCtl) + S->bnd_capacity * sizeof(BoundaryDev);
+  ALV_HIP(hipMalloc(&S->step_dev, step_bytes));
+  ALV_HIP(hipHostMalloc(&S->step_host, step_bytes, hipHostMallocCoherent));
+  memset(S->step_host, 0, step_bytes);
+  S->ctl_host = static_cast<StepCtl*>(S->step_host);
+  S->bnd_host = reinterpret_cast<BoundaryDev*>(S->ctl_host + NS);
+  A.ctl = static_cast<StepCtl*>(S->step_dev);
+  S->bnd_dev = reinterpret_cast<BoundaryDev*>(A.ctl + NS);
   ALV_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->body_host),
-                        S->bnd_capacity * 6 * sizeof(double), hipHostMallocDefault));
-  memset(S->ctl_host, 0, NS * sizeof(StepCtl));
+                        S->bnd_capacity * 6 * sizeof(double), hipHostMallocCoherent));
+  memset(S->body_host, 0, S->bnd_capacity * 6 * sizeof(double));
+  ALV_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&A.ctl_pub), S->ctl_host, 0));
+  ALV_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&A.body_sums), S->body_host, 0));
   A.bnd = S->bnd_dev;
   return ALV_OK;
 }
@@ -1874,9 +1885,6 @@ int upload_step_inputs(alv_solver* S) {
     }
     nb = std::max<size_t>(nb, sc.bnd_base + b);
   }
-  if (nb)
-    ALV_HIP(hipMemcpyAsync(S->bnd_dev, S->bnd_host, nb * sizeof(BoundaryDev),
-                           hipMemcpyHostToDevice, S->stream));
   for (size_t k = 0; k < S->scenes.size(); ++k) {
     StepCtl* c = S->ctl_host + k;
     memset(c, 0, sizeof(StepCtl));
@@ -1892,14 +1900,16 @@ int upload_step_inputs(alv_solver* S) {
     c->max_dens = S->set.max_density_solve;
     c->length_scale = S->cn.particle_radius * 2.0f;
   }
-  ALV_HIP(hipMemcpyAsync(S->A.ctl, S->ctl_host, S->scenes.size() * sizeof(StepCtl),
+  // the control blocks and the boundaries in one copy
+  ALV_HIP(hipMemcpyAsync(S->step_dev, S->step_host,
+                         S->scenes.size() * sizeof(StepCtl) + nb * sizeof(BoundaryDev),
                          hipMemcpyHostToDevice, S->stream));
   return ALV_OK;
 }
 
+// Every kernel that changes a control block publishes it to ctl_host
+// (publish_ctl), so the blocks are current once the stream has drained.
 int read_ctl(alv_solver* S) {
-  ALV_HIP(hipMemcpyAsync(S->ctl_host, S->A.ctl, S->scenes.size() * sizeof(StepCtl),
-                         hipMemcpyDeviceToHost, S->stream));
   ALV_HIP(hipStreamSynchronize(S->stream));
   return ALV_OK;
 }
@@ -1942,8 +1952,7 @@ int pellet_launch(alv_solver* S, const Launch& L, Launch* Lp) {
 // leaves it out, since only a step reads it, after its own neighbour phase.
 int neighbor_phase(alv_solver* S, const Launch& L, int with_factor, bool gather_state = true) {
   DevArrays& A = S->A;
-  launch_fill_u32(A.cell_count, 0u, L.cells, S->stream);
-  launch_fill_u32(A.anomaly, 0u, 3, S->stream);
+  launch_clear_grid(L, A, S->stream);
   if (L.active == 0) return ALV_OK;
   launch_cell_count(L, A, S->stream);
   launch_reserve_padding(L, A, S->stream);
@@ -1993,8 +2002,8 @@ void fill_report(alv_solver_report* r, const StepCtl& c) {
 }
 
 int finish_step(alv_solver* S, const Launch& L) {
-  // rigid force sums (solver_df.hpp:347-353) are enqueued with the control
-  // block copy, so one wait brings back both
+  // rigid force sums (solver_df.hpp:347-353): k_body_sums stores them in
+  // body_host, so the control blocks' wait brings back both
   alv_pile* pile = S->pile;
   bool any_dynamic = false;
   for (const auto& B : pile->bnd) any_dynamic |= B.dynamic != 0;
@@ -2002,8 +2011,6 @@ int finish_step(alv_solver* S, const Launch& L) {
   if (body_sums) {
     launch_body_sums(L, S->A, S->stream);
     ALV_HIP(hipGetLastError());
-    ALV_HIP(hipMemcpyAsync(S->body_host, S->A.body_sums, pile->bnd.size() * 6 * sizeof(double),
-                           hipMemcpyDeviceToHost, S->stream));
   }
   int rc = read_ctl(S);
   if (rc != ALV_OK) return rc;

@@ -105,8 +105,13 @@ struct DevArrays {
   double* partials;  // per-block FP64 partial sums
   float* fpartials;  // per-block float maxima
   double* body_partials;  // [B, 256, 6]
-  double* body_sums;  // [B, 6]
+  // [B, 6] in the host's pinned, coherent buffer (mapped): k_body_sums
+  // stores the sums there, so the host reads them without a copy command
+  double* body_sums;
   StepCtl* ctl;
+  // the host's pinned, coherent copy of the control blocks (mapped): every
+  // kernel that changes a block publishes it there (publish_ctl)
+  StepCtl* ctl_pub;
   const BoundaryDev* bnd;
 };
 
@@ -194,6 +199,8 @@ struct Launch {
 
 // neighbour search (counting sort)
 void launch_fill_u32(unsigned* dst, unsigned value, size_t count, hipStream_t s);
+// zeroes the cell counts (L.cells) and the grid anomaly flags
+void launch_clear_grid(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_cell_count(const Launch& L, const DevArrays& A, hipStream_t s);
 // reserves each scene's unused slots at the end of its outside-the-grid bin
 void launch_reserve_padding(const Launch& L, const DevArrays& A, hipStream_t s);

@@ -113,6 +113,14 @@ __device__ inline float block_max_f(float v, float* lds) {
     for (unsigned i = 0; i < blockDim.x / 64u; ++i) r = std_max(r, lds[i]);
   return r;
 }
+// Publishes a control block that this thread has just updated to the host's
+// copy (pinned, coherent, mapped): the host reads it after waiting for the
+// stream, with no copy command.  The system-scope fence orders the stores
+// before the kernel's completion.
+__device__ inline void publish_ctl(const DevArrays& A, const StepCtl* ctl, unsigned scene) {
+  A.ctl_pub[scene] = *ctl;
+  __threadfence_system();
+}
 // exclusive scan of one u32 per thread over a block of 256 threads
 __device__ inline unsigned block_excl_scan_u32(unsigned v, unsigned* lds, unsigned* total) {
   const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
@@ -1368,6 +1376,7 @@ __global__ __launch_bounds__(256) void k_div_finalize(SceneTab T, DevArrays A) {
     ctl->mean_density_change = mean;
     bool again = (mean > ctl->tol_div / ctl->dt || it < ctl->min_div) && it < ctl->max_div;
     ctl->div_done = again ? 0u : 1u;
+    publish_ctl(A, ctl, scene_);
   }
 }
 
@@ -1384,6 +1393,7 @@ __global__ void k_ctl_reset(DevArrays A) {
   ctl->dens_done = 0u;  // decided after update_dt (tolerance does not use dt)
   bool dens_go = (mx > ctl->tol_dens || 0u < ctl->min_dens) && 0u < ctl->max_dens;
   ctl->dens_done = dens_go ? 0u : 1u;
+  publish_ctl(A, ctl, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1447,6 +1457,7 @@ __global__ __launch_bounds__(256) void k_update_dt(SceneTab T, DevArrays A) {
     ctl->div_dt = ctl->dt;
     ctl->dt = dt;
     ctl->utilized_cfl = dt * max_v / ls;
+    publish_ctl(A, ctl, scene_);
   }
 }
 
@@ -1577,6 +1588,7 @@ __global__ __launch_bounds__(256) void k_p_finalize(SceneTab T, DevArrays A) {
     ctl->mean_density_error = mean;
     bool again = (mean > ctl->tol_dens || it < ctl->min_dens) && it < ctl->max_dens;
     ctl->dens_done = again ? 0u : 1u;
+    publish_ctl(A, ctl, scene_);
   }
 }
 
@@ -1846,6 +1858,7 @@ __global__ __launch_bounds__(256) void k_ii_finalize(SceneTab T, DevArrays A) {
     ctl->mean_density_error = mean;
     bool again = (mean > ctl->tol_dens || it < ctl->min_dens) && it < ctl->max_dens;
     ctl->dens_done = again ? 0u : 1u;
+    publish_ctl(A, ctl, scene_);
   }
 }
 // compute_pressure_accels (runner.hpp:2774-2817)
@@ -2132,9 +2145,10 @@ __global__ __launch_bounds__(256) void k_body_sums(SceneTab T, DevArrays A, unsi
     for (unsigned k = threadIdx.x; k < nblk; k += blockDim.x)
       acc += A.body_partials[(size_t(b) * nblk + k) * 6 + c];
     double t = block_sum_d(acc, lds);
-    if (threadIdx.x == 0) A.body_sums[b * 6 + c] = t;
+    if (threadIdx.x == 0) A.body_sums[b * 6 + c] = t;  // host memory (mapped)
     __syncthreads();
   }
+  if (threadIdx.x == 0) __threadfence_system();
 }
 
 // sorted -> user order for the API arrays that are synchronised lazily
@@ -2673,6 +2687,18 @@ __global__ __launch_bounds__(256) void k_fill_u32(unsigned* __restrict__ d, unsi
                                                   size_t n) {
   const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) d[i] = v;
+}
+// zeroes the cell counts and the three grid anomaly flags (one launch)
+__global__ __launch_bounds__(256) void k_clear_grid(unsigned* counts, unsigned cells,
+                                                    unsigned* anomaly) {
+  const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < cells) counts[i] = 0u;
+  if (i < 3u) anomaly[i] = 0u;
+}
+void launch_clear_grid(const Launch& L, const DevArrays& A, hipStream_t s) {
+  const unsigned nb = std::max(1u, (L.cells + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_clear_grid, dim3(nb), dim3(kBlock), 0, s, A.cell_count, L.cells,
+                     A.anomaly);
 }
 void launch_fill_u32(unsigned* dst, unsigned value, size_t count, hipStream_t s) {
   if (!count) return;
