@@ -1960,8 +1960,10 @@ int neighbor_phase(alv_solver* S, const Launch& L, int with_factor, bool gather_
   launch_scatter(L, A, S->stream);
   launch_cell_sort(L, A, S->stream);
   launch_tiles(L, A, S->stream);
-  const int kind = !gather_state ? ALV_KIND_NONE
-                   : S->kind == ALV_SOLVER_DF ? ALV_KIND_DF : ALV_KIND_II;
+  const int kind = !gather_state            ? ALV_KIND_NONE
+                   : S->kind != ALV_SOLVER_DF ? ALV_KIND_II
+                   : with_factor             ? ALV_KIND_DF
+                                             : ALV_KIND_DF_FACTOR;
   launch_gather_boundary(L, A, kind, S->stream);
   launch_neighbors(L, A, with_factor, S->stream);
   ALV_HIP(hipGetLastError());
@@ -1989,11 +1991,19 @@ int run_loop(alv_solver* S, bool divergence, unsigned max_iter, unsigned* last,
   return ALV_OK;
 }
 
-void fill_report(alv_solver_report* r, const StepCtl& c) {
-  r->num_divergence_solve = c.div_iter;
-  r->num_density_solve = c.dens_iter;
-  r->mean_density_change = c.mean_density_change;
-  r->mean_density_error = c.mean_density_error;
+// A DFSPH step with a solve switched off leaves that solve's count and mean
+// as they were (the reference sets them inside the solve's enable test,
+// solver_df.hpp:129-130, 294-295).
+void fill_report(const alv_solver* S, alv_solver_report* r, const StepCtl& c) {
+  const bool df = S->kind == ALV_SOLVER_DF;
+  if (!df || S->set.enable_divergence_solve) {
+    r->num_divergence_solve = c.div_iter;
+    r->mean_density_change = c.mean_density_change;
+  }
+  if (!df || S->set.enable_density_solve) {
+    r->num_density_solve = c.dens_iter;
+    r->mean_density_error = c.mean_density_error;
+  }
   r->particle_max_v2 = c.particle_max_v2;
   r->pile_max_v2 = c.pile_max_v2;
   r->max_v2 = c.max_v2;
@@ -2024,7 +2034,7 @@ int finish_step(alv_solver* S, const Launch& L) {
     for (size_t k = 0; k < S->scenes.size(); ++k) {
       auto& sc = S->scenes[k];
       const StepCtl& c = S->ctl_host[k];
-      fill_report(&sc.rep, c);
+      fill_report(S, &sc.rep, c);
       sc.rep.t = sc.t + c.dt;
       sc.rep.dt = c.dt;
       sc.t += c.dt;
@@ -2032,9 +2042,10 @@ int finish_step(alv_solver* S, const Launch& L) {
       mdiv = std::max(mdiv, c.div_iter);
       mdens = std::max(mdens, c.dens_iter);
     }
-    fill_report(&S->rep, S->ctl_host[0]);
-    S->rep.num_divergence_solve = mdiv;
-    S->rep.num_density_solve = mdens;
+    fill_report(S, &S->rep, S->ctl_host[0]);
+    const bool df = S->kind == ALV_SOLVER_DF;
+    if (!df || S->set.enable_divergence_solve) S->rep.num_divergence_solve = mdiv;
+    if (!df || S->set.enable_density_solve) S->rep.num_density_solve = mdens;
     if (mdiv) S->last_div = mdiv;
     if (mdens) S->last_dens = mdens;
     S->set.dt = S->scenes[0].dt;
@@ -2043,7 +2054,7 @@ int finish_step(alv_solver* S, const Launch& L) {
   }
   const StepCtl& c = *S->ctl_host;
   S->set.dt = c.dt;
-  fill_report(&S->rep, c);
+  fill_report(S, &S->rep, c);
   if (c.div_iter) S->last_div = c.div_iter;
   if (c.dens_iter) S->last_dens = c.dens_iter;
   if (body_sums) {
@@ -2265,6 +2276,16 @@ int alv_solver_step(alv_solver* S) {
     // the reference's warm start 0 rides on the neighbour search's walk over
     // the lists unless the learned warm start or feature logging replaces it
     const int fuse_ws = div && !S->net && !S->log_features;
+    // Without the divergence solve the reference does not compute the DFSPH
+    // factor (solver_df.hpp:81-91): the density solve reads the one each
+    // particle ended the last step with.  It goes to user order with the
+    // last step's permutation here and comes back in the new one
+    // (k_gather_boundary, ALV_KIND_DF_FACTOR).
+    if (!div && L.active && (S->dirty & ALV_F_ALPHA)) {
+      launch_unsort_fields(L, A, ALV_F_ALPHA, st);
+      ALV_HIP(hipGetLastError());
+      S->dirty &= ~ALV_F_ALPHA;
+    }
     rc = run_segment(S, 0, keyA, [&] {
       if (neighbor_phase(S, L, fuse_ws ? 2 : div) != ALV_OK) return;
       launch_ctl_reset(L, A, st);

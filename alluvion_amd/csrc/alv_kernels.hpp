@@ -365,7 +365,9 @@ enum : unsigned {
 
 // field codes written by launch_unsort_fields / gathered in gather_boundary
 // solver state gathered into sorted order by k_gather_boundary: DFSPH
-// stiffnesses, IISPH/ISPH pressure, or none (positions and velocities only)
-enum { ALV_KIND_DF = 0, ALV_KIND_II = 1, ALV_KIND_NONE = 2 };
+// stiffnesses (with DF_FACTOR also the DFSPH factor, for a step without the
+// divergence solve, which does not compute it), IISPH/ISPH pressure, or none
+// (positions and velocities only)
+enum { ALV_KIND_DF = 0, ALV_KIND_II = 1, ALV_KIND_NONE = 2, ALV_KIND_DF_FACTOR = 3 };
 
 }  // namespace alv

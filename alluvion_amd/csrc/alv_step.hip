@@ -330,9 +330,10 @@ __global__ __launch_bounds__(256) void k_gather_boundary(SceneTab T, DevArrays A
   vec3 v = ld3(A.uv, i);
   A.x[s] = f4(x, 0.0f);
   A.v[s] = f4(v, 0.0f);
-  if (kind == ALV_KIND_DF) {
+  if (kind == ALV_KIND_DF || kind == ALV_KIND_DF_FACTOR) {
     A.kappa[s] = A.ukappa[i];
     A.kappa_v[s] = A.ukappa_v[i];
+    if (kind == ALV_KIND_DF_FACTOR) A.alpha[s] = A.ualpha[i];
   } else if (kind == ALV_KIND_II) {
     A.pressure[s] = A.upressure[i];
   }
@@ -1276,13 +1277,16 @@ __global__ __launch_bounds__(256) void k_ws_div1(SceneTab T, DevArrays A) {
   add_v(A, s, dv);
 }
 // compute_velocity_of_density_change (runner.hpp:3297-3314); also stores the
-// first iteration's stiffness k = rho_adv * alpha.  Every density-change pass
-// of the divergence solve also computes the viscosity (compute_viscosity's
+// first iteration's stiffness k = rho_adv * alpha.  The density-change passes
+// of the divergence solve also compute the viscosity (compute_viscosity's
 // a = g + fluid and boundary terms, without the bodies' reactions) and the
-// per-block CFL maxima from the velocities it sees: the last pass that runs
-// sees the solve's final velocities, which k_viscosity would read.  Both
-// passes are bounded to 64 VGPRs (8 waves per SIMD); unbounded, the viscosity
-// terms take 66 (7 waves per SIMD), which measured 2 % slower.
+// per-block CFL maxima from the velocities they see, wherever the pass may be
+// the solve's last: the last pass that runs sees the final velocities, which
+// k_viscosity would read.  This pass is the last only if the control block
+// says no iteration follows; an iteration's k_div_err is not the last while
+// the iteration count stays below the minimum and the maximum.  Both passes
+// are bounded to 64 VGPRs (8 waves per SIMD); unbounded, the viscosity terms
+// take 66 (7 waves per SIMD), which measured 2 % slower.
 template <bool W, bool TL>
 __global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock, 8) void k_vel_density_change(SceneTab T, DevArrays A) {
   __shared__ float lds[kLPP * kBlock / 64];
@@ -1291,11 +1295,13 @@ __global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock, 8) void k_vel_density_
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   const float dt = A.ctl->dt;
+  const bool last = A.ctl->div_done != 0u;  // the same for the whole block
   float cfl = -FLT_MAX;
   if (s < p.first + p.n) {
     vec3 xi, vi, visc;
     const float dadv =
-        pass_density_change_visc<W, TL>(p, A, s, q, tiled_, tile_, xi, vi, visc);
+        last ? pass_density_change_visc<W, TL>(p, A, s, q, tiled_, tile_, xi, vi, visc)
+             : pass_density_change<W, TL>(p, A, s, q, tiled_, tile_);
     if (q == 0u) {
       A.rho_adv[s] = dadv;
       float inv_dt = 1.0f / dt;
@@ -1304,11 +1310,13 @@ __global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock, 8) void k_vel_density_
       const float k = dadv * alpha;
       A.kfac[s] = k;
       set_kw(A, s, k);
-      cfl = viscosity_finish(p, A, s, xi, vi, visc, dt);
+      if (last) cfl = viscosity_finish(p, A, s, xi, vi, visc, dt);
     }
   }
-  const float m = block_max_f(cfl, lds);
-  if (threadIdx.x == 0) A.fpartials[qblk_] = m;
+  if (last) {
+    const float m = block_max_f(cfl, lds);
+    if (threadIdx.x == 0) A.fpartials[qblk_] = m;
+  }
 }
 // divergence_solve_iteration (runner.hpp:3316-3363)
 template <bool W, bool TL>
@@ -1339,26 +1347,30 @@ __global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock, 8) void k_div_err(Scen
   bool tiled_ = false;
   if constexpr (TL) tiled_ = stage_xv(A, s / kBlock, tile_);
   const float dt = A.ctl->dt;
+  // k_div_finalize continues while the count it reaches is below both bounds
+  const unsigned it = A.ctl->div_iter + 1u;
+  const bool last = !(it < A.ctl->min_div && it < A.ctl->max_div);
   double part = 0.0;
   float cfl = -FLT_MAX;
   if (s < p.first + p.n) {
     vec3 xi, vi, visc;
     const float dadv =
-        pass_density_change_visc<W, TL>(p, A, s, q, tiled_, tile_, xi, vi, visc);
+        last ? pass_density_change_visc<W, TL>(p, A, s, q, tiled_, tile_, xi, vi, visc)
+             : pass_density_change<W, TL>(p, A, s, q, tiled_, tile_);
     if (q == 0u) {
       A.rho_adv[s] = dadv;
       const float k = dadv * A.alpha[s];
       A.kfac[s] = k;
       set_kw(A, s, k);
       part = double(dadv);
-      cfl = viscosity_finish(p, A, s, xi, vi, visc, dt);
+      if (last) cfl = viscosity_finish(p, A, s, xi, vi, visc, dt);
     }
   }
   const double t = block_sum_d(part, lds);
-  const float m = block_max_f(cfl, ldsm);
-  if (threadIdx.x == 0) {
-    A.partials[qblk_] = t;
-    A.fpartials[qblk_] = m;
+  if (threadIdx.x == 0) A.partials[qblk_] = t;
+  if (last) {
+    const float m = block_max_f(cfl, ldsm);
+    if (threadIdx.x == 0) A.fpartials[qblk_] = m;
   }
 }
 __global__ __launch_bounds__(256) void k_div_finalize(SceneTab T, DevArrays A) {

@@ -99,6 +99,41 @@ def test_config2_scaled_floating_bodies_per_step():
     assert min(forces) > 0.1
 
 
+def test_config2_scaled_viscosity_where_the_divergence_solve_ends():
+    """The viscosity (and the bodies' viscous reactions) of a DFSPH step
+    comes from whichever density-change pass of the divergence solve sees the
+    final velocities, or from the viscosity pass without that solve: checked
+    against the oracle with no iteration (the warm start's pass is the last),
+    exactly three (the first two passes skip it), the default bounds, and
+    neither solve.  (Without the divergence solve the reference's density
+    solve rescales the last step's factor by 1/dt^2 once more, runner.hpp:
+    3525-3541, and diverges within the step, so that case runs without the
+    density solve too.)"""
+    sc = config2_scaled()
+    dp, pile, solver, _, _ = scenes.build_solver(sc)
+    for _ in range(SETTLE):
+        solver.step()
+    orc = oracle_for(sc, dp, pile.volume_grids)
+    cases = [("no iteration", 1, 0, 0), ("three iterations", 1, 3, 3),
+             ("default bounds", 1, sc.min_divergence_solve, sc.max_divergence_solve),
+             ("neither solve", 0, sc.min_divergence_solve, sc.max_divergence_solve)]
+    for tag, enable, lo, hi in cases:
+        solver.enable_divergence_solve = bool(enable)
+        solver.enable_density_solve = bool(enable)
+        solver.min_divergence_solve = lo
+        solver.max_divergence_solve = hi
+        orc.update_settings(enable_divergence_solve=enable, enable_density_solve=enable,
+                            min_divergence_solve=lo, max_divergence_solve=hi)
+        sync(dp, solver, pile, orc)
+        before, rep, _ = step_pair(dp, solver, pile, orc)
+        if enable:
+            assert solver.num_divergence_solve == (hi if lo == hi else rep["num_divergence_solve"])
+        compare_fields(dp, solver, orc, tag=tag)
+        a = dp.coat(solver.particle_a).get()[:orc.n]
+        assert rel_err(a, orc.get("a")[:orc.n]) < REL, (tag, rel_err(a, orc.get("a")[:orc.n]))
+        assert check_bodies(pile, orc, before, float(solver.dt), tag) > 0.1
+
+
 def test_config2_full_size_one_step():
     sc = scenes.dam_break_rigid()
     assert sc.n == 1_048_576
