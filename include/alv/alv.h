@@ -294,7 +294,9 @@ typedef struct alv_solver_settings {
   int enable_density_solve;
 } alv_solver_settings;
 
-/* Report fields (solver.hpp:274-279, solver_df.hpp:381-385). */
+/* Report fields (solver.hpp:274-279, solver_df.hpp:381-385).  A DFSPH step
+ * with a solve disabled leaves that solve's count and mean as they were, as
+ * the reference sets them inside the solve (solver_df.hpp:129-130, 294-295). */
 typedef struct alv_solver_report {
   unsigned num_divergence_solve;
   unsigned num_density_solve;
