@@ -994,18 +994,25 @@ __device__ inline vec3 stiffness_dv(const SimParams& p, const DevArrays& A, unsi
     float4 xj[kNB];
 #pragma unroll
     for (int u = 0; u < kNB; ++u) xj[u] = src.X(j[u]);
+    // Every slot's term is computed and a skipped one (past the count, or
+    // |k_i + k_j| <= eps, where the reference adds nothing) is replaced by
+    // +0, which leaves the sum's bits unchanged; the gradient's division is
+    // the correctly rounded div_rn (the same bits as the IEEE division).
 #pragma unroll
     for (int u = 0; u < kNB; ++u) {
       const float k_sum = k_i + xj[u].w;
-      if (t0 + u < nn && fabsf(k_sum) > p.eps) {
-        vec3 d = wrapd<W>(xi - ld3(xj[u]), p);
-        if (kDivergenceForm) {
-          vec3 grad_p_j = (-p.vol) * (gradW(d, p));
-          dv -= (dt * k_sum) * grad_p_j;
-        } else {
-          dv += ((dt * k_sum) * p.vol) * (gradW(d, p));
-        }
-      }
+      const bool use = t0 + u < nn && fabsf(k_sum) > p.eps;
+      const vec3 d = wrapd<W>(xi - ld3(xj[u]), p);
+      vec3 term;
+      if (kDivergenceForm)
+        term = (dt * k_sum) * ((-p.vol) * gradW_rn(d, p));
+      else
+        term = ((dt * k_sum) * p.vol) * gradW_rn(d, p);
+      term = use ? term : mk(0.f, 0.f, 0.f);
+      if (kDivergenceForm)
+        dv -= term;
+      else
+        dv += term;
     }
   }
   if (fabsf(k_i) > p.eps) {
