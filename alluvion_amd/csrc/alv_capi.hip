@@ -1479,7 +1479,7 @@ static int solver_alloc(alv_solver* S) {
   const size_t CT = S->cells_total;
   ALV_ALLOC(A.cell_count, CT);
   ALV_ALLOC(A.cell_start, CT + 1);
-  ALV_ALLOC(A.scan_tmp, (CT + 4095) / 4096 + 1);
+  ALV_ALLOC(A.scan_tmp, (CT + kScanPerBlock - 1) / kScanPerBlock + 1);
   const size_t NS = S->scenes.size();
   ALV_ALLOC(S->par_dev, NS);
   ALV_ALLOC(S->blk_scene, size_t(grid_for(cap)) + 1);
@@ -1534,7 +1534,7 @@ static int solver_alloc(alv_solver* S) {
     ALV_ALLOC(A.psort_tmp, pspan);
     ALV_ALLOC(A.pcell_count, CT);
     ALV_ALLOC(A.pcell_start, CT + 1);
-    ALV_ALLOC(A.pscan_tmp, (CT + 4095) / 4096 + 1);
+    ALV_ALLOC(A.pscan_tmp, (CT + kScanPerBlock - 1) / kScanPerBlock + 1);
     ALV_ALLOC(A.px, pspan);
     ALV_ALLOC(A.pv, pspan);
     ALV_ALLOC(A.pbody, pspan);

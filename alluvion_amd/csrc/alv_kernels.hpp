@@ -204,6 +204,9 @@ void launch_clear_grid(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_cell_count(const Launch& L, const DevArrays& A, hipStream_t s);
 // reserves each scene's unused slots at the end of its outside-the-grid bin
 void launch_reserve_padding(const Launch& L, const DevArrays& A, hipStream_t s);
+// Cells per block of the cell-count scan (1024: four per thread, so a 1 M-cell
+// grid runs 1024 blocks, four per CU); `tmp` holds one sum per block.
+constexpr unsigned kScanPerBlock = 1024;
 void launch_scan(unsigned count, const unsigned* in, unsigned* out, unsigned* tmp,
                  hipStream_t s);
 void launch_scatter(const Launch& L, const DevArrays& A, hipStream_t s);

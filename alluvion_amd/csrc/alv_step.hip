@@ -163,7 +163,8 @@ __global__ __launch_bounds__(256) void k_cell_count(SceneTab T, DevArrays A) {
   A.slot[i] = atomicAdd(&A.cell_count[k], 1u);
 }
 
-constexpr unsigned kScanPerThread = 16, kScanPerBlock = kScanPerThread * kBlock;
+constexpr unsigned kScanPerThread = kScanPerBlock / kBlock;
+static_assert(kScanPerThread * kBlock == kScanPerBlock, "whole cells per thread");
 
 __global__ __launch_bounds__(256) void k_scan_reduce(unsigned count, const unsigned* in,
                                                      unsigned* block_sums) {
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(256) void k_scan_reduce(unsigned count, const unsig
   if (threadIdx.x == 0) block_sums[blockIdx.x] = unsigned(t);
 }
 
-// single block: exclusive scan of the block sums, in place (<= 4096 blocks)
+// single block: exclusive scan of the block sums, in place
 __global__ __launch_bounds__(256) void k_scan_top(unsigned nblocks, unsigned* sums) {
   __shared__ unsigned lds[4];
   const unsigned per = (nblocks + kBlock - 1) / kBlock;
