@@ -113,6 +113,10 @@ __device__ inline float block_max_f(float v, float* lds) {
     for (unsigned i = 0; i < blockDim.x / 64u; ++i) r = std_max(r, lds[i]);
   return r;
 }
+// Block size of the one-block-per-scene reductions over the per-block
+// partials (finalisers, time step): four partials per thread at 1 M
+// particles, so the loads of the block are in flight together.
+constexpr unsigned kFinBlock = 1024;
 // Publishes a control block that this thread has just updated to the host's
 // copy (pinned, coherent, mapped): the host reads it after waiting for the
 // stream, with no copy command.  The system-scope fence orders the stores
@@ -1381,8 +1385,8 @@ __global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock, 8) void k_div_err(Scen
     if (threadIdx.x == 0) A.fpartials[qblk_] = m;
   }
 }
-__global__ __launch_bounds__(256) void k_div_finalize(SceneTab T, DevArrays A) {
-  __shared__ double lds[4];
+__global__ __launch_bounds__(kFinBlock) void k_div_finalize(SceneTab T, DevArrays A) {
+  __shared__ double lds[kFinBlock / 64];
   ALV_SCENE_BLOCK(T, A);
   StepCtl* ctl = A.ctl;
   if (ctl->div_done) return;
@@ -1449,8 +1453,8 @@ __global__ __launch_bounds__(256) void k_viscosity(SceneTab T, DevArrays A) {
 }
 
 // Solver::update_dt (solver.hpp:153-169) on the device.
-__global__ __launch_bounds__(256) void k_update_dt(SceneTab T, DevArrays A) {
-  __shared__ float lds[4];
+__global__ __launch_bounds__(kFinBlock) void k_update_dt(SceneTab T, DevArrays A) {
+  __shared__ float lds[kFinBlock / 64];
   ALV_SCENE_BLOCK(T, A);
   float m = -FLT_MAX;
   for (unsigned b = blk_begin + threadIdx.x; b < blk_end; b += blockDim.x) m = std_max(m, A.fpartials[b]);
@@ -1593,8 +1597,8 @@ __global__ __launch_bounds__(TL ? kBlock : kLPP * kBlock) void k_p_err(SceneTab 
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) A.partials[qblk_] = t;
 }
-__global__ __launch_bounds__(256) void k_p_finalize(SceneTab T, DevArrays A) {
-  __shared__ double lds[4];
+__global__ __launch_bounds__(kFinBlock) void k_p_finalize(SceneTab T, DevArrays A) {
+  __shared__ double lds[kFinBlock / 64];
   ALV_SCENE_BLOCK(T, A);
   StepCtl* ctl = A.ctl;
   if (ctl->dens_done) return;
@@ -1863,8 +1867,8 @@ __global__ __launch_bounds__(256) void k_ii_iter1(SceneTab T, DevArrays A) {
   double t = block_sum_d(part, lds);
   if (threadIdx.x == 0) A.partials[xcd_block()] = t;
 }
-__global__ __launch_bounds__(256) void k_ii_finalize(SceneTab T, DevArrays A) {
-  __shared__ double lds[4];
+__global__ __launch_bounds__(kFinBlock) void k_ii_finalize(SceneTab T, DevArrays A) {
+  __shared__ double lds[kFinBlock / 64];
   ALV_SCENE_BLOCK(T, A);
   StepCtl* ctl = A.ctl;
   if (ctl->dens_done) return;
@@ -2699,9 +2703,14 @@ KernelProfiler* g_profiler = nullptr;
     if (g_profiler) g_profiler->end(#kernel, s);                                  \
   } while (0)
 // one block per scene
-#define ALV_LAUNCH_SCENES(kernel, L, s, ...) \
-  ALV_LAUNCH_GRID(kernel, dim3((L).scenes), s, (L).T, __VA_ARGS__)
 #define ALV_LAUNCH1(kernel, s, ...) ALV_LAUNCH_GRID(kernel, dim3(1), s, __VA_ARGS__)
+// one block of kFinBlock threads per scene
+#define ALV_LAUNCH_FIN(kernel, L, s, ...)                                          \
+  do {                                                                             \
+    if (g_profiler) g_profiler->begin(#kernel, s);                                 \
+    hipLaunchKernelGGL(kernel, dim3((L).scenes), dim3(kFinBlock), 0, s, (L).T, __VA_ARGS__); \
+    if (g_profiler) g_profiler->end(#kernel, s);                                   \
+  } while (0)
 
 __global__ __launch_bounds__(256) void k_fill_u32(unsigned* __restrict__ d, unsigned v,
                                                   size_t n) {
@@ -2784,13 +2793,13 @@ void launch_df_warm_start_div(const Launch& L, const DevArrays& A, hipStream_t s
 void launch_df_div_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_WT(k_div_iter, L, s, A);
   ALV_LAUNCH_WTP(k_div_err, L, s, A);
-  ALV_LAUNCH_SCENES(k_div_finalize, L, s, A);
+  ALV_LAUNCH_FIN(k_div_finalize, L, s, A);
 }
 void launch_df_viscosity(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_WT(k_viscosity, L, s, A);
 }
 void launch_update_dt(const Launch& L, const DevArrays& A, hipStream_t s) {
-  ALV_LAUNCH_SCENES(k_update_dt, L, s, A);
+  ALV_LAUNCH_FIN(k_update_dt, L, s, A);
 }
 void launch_df_integrate_np(const Launch& L, const DevArrays& A, int fin, hipStream_t s) {
   ALV_LAUNCH(k_integrate_np, L, s, A, fin);
@@ -2807,7 +2816,7 @@ void launch_df_warm_start_dens(const Launch& L, const DevArrays& A, hipStream_t 
 void launch_df_dens_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_WT(k_p_iter, L, s, A);
   ALV_LAUNCH_WTP(k_p_err, L, s, A);
-  ALV_LAUNCH_SCENES(k_p_finalize, L, s, A);
+  ALV_LAUNCH_FIN(k_p_finalize, L, s, A);
 }
 void launch_df_finish(const Launch& L, const DevArrays& A, int dens_finish, hipStream_t s) {
   ALV_LAUNCH_W(k_df_finish, L, s, A, dens_finish);
@@ -2832,7 +2841,7 @@ void launch_ii_predict(const Launch& L, const DevArrays& A, hipStream_t s) {
 void launch_ii_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_W(k_ii_iter0, L, s, A);
   ALV_LAUNCH_W(k_ii_iter1, L, s, A);
-  ALV_LAUNCH_SCENES(k_ii_finalize, L, s, A);
+  ALV_LAUNCH_FIN(k_ii_finalize, L, s, A);
 }
 void launch_is_prepare(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH(k_is_advect, L, s, A);
@@ -2840,7 +2849,7 @@ void launch_is_prepare(const Launch& L, const DevArrays& A, hipStream_t s) {
 }
 void launch_is_iteration(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH_W2(k_is_iter, false, L, s, A);
-  ALV_LAUNCH_SCENES(k_ii_finalize, L, s, A);
+  ALV_LAUNCH_FIN(k_ii_finalize, L, s, A);
 }
 void launch_is_settle(const Launch& L, const DevArrays& A, hipStream_t s) {
   ALV_LAUNCH(k_is_settle, L, s, A);
@@ -2964,7 +2973,7 @@ void launch_is_iteration_pellets(const Launch& L, const Launch& Lp, const DevArr
                                  hipStream_t s) {
   if (Lp.active) ALV_LAUNCH_W(k_extrapolate, Lp, s, A, 0);
   ALV_LAUNCH_W2(k_is_iter, true, L, s, A);
-  ALV_LAUNCH_SCENES(k_ii_finalize, L, s, A);
+  ALV_LAUNCH_FIN(k_ii_finalize, L, s, A);
 }
 void launch_is_finish_pellets(const Launch& L, const Launch& Lp, const DevArrays& A,
                               hipStream_t s) {
