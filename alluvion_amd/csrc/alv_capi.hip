@@ -2288,7 +2288,6 @@ int alv_solver_step(alv_solver* S) {
     }
     rc = run_segment(S, 0, keyA, [&] {
       if (neighbor_phase(S, L, fuse_ws ? 2 : div) != ALV_OK) return;
-      launch_ctl_reset(L, A, st);
       if (L.active && div) {
         launch_df_warm_start_div(L, A, st, S->log_features, S->net ? &mv : nullptr, S->net_div,
                                  fuse_ws);
@@ -2336,7 +2335,6 @@ int alv_solver_step(alv_solver* S) {
     rc = neighbor_phase(S, L, 0);
     if (rc != ALV_OK) return rc;
     launch_pellet_lists(L, Lp, A, st);
-    launch_ctl_reset(L, A, st);
     if (L.active) launch_pellet_viscosity(L, A, st);
     launch_update_dt(L, A, st);
     if (L.active) {
@@ -2351,7 +2349,6 @@ int alv_solver_step(alv_solver* S) {
     // SolverI::step (solver_i.hpp:79-330), volume-map boundaries
     rc = neighbor_phase(S, L, 0);
     if (rc != ALV_OK) return rc;
-    launch_ctl_reset(L, A, st);
     if (L.active) launch_ii_viscosity_predict(L, A, st);
     launch_update_dt(L, A, st);
     if (L.active) {
@@ -2366,7 +2363,6 @@ int alv_solver_step(alv_solver* S) {
     // SolverIi::step (solver_ii.hpp:76-243)
     rc = neighbor_phase(S, L, 0);
     if (rc != ALV_OK) return rc;
-    launch_ctl_reset(L, A, st);
     if (L.active) launch_ii_viscosity_predict(L, A, st);
     launch_update_dt(L, A, st);
     if (L.active) {

@@ -199,7 +199,8 @@ struct Launch {
 
 // neighbour search (counting sort)
 void launch_fill_u32(unsigned* dst, unsigned value, size_t count, hipStream_t s);
-// zeroes the cell counts (L.cells) and the grid anomaly flags
+// zeroes the cell counts (L.cells) and the grid anomaly flags and resets the
+// control blocks for the step's solves
 void launch_clear_grid(const Launch& L, const DevArrays& A, hipStream_t s);
 void launch_cell_count(const Launch& L, const DevArrays& A, hipStream_t s);
 // reserves each scene's unused slots at the end of its outside-the-grid bin
@@ -340,8 +341,6 @@ void launch_export_boundary(const Launch& L, const DevArrays& A, float4* rec, fl
 // density_err; vec = dii, dij_pj, pressure_accel (float3); nulls skipped
 void launch_export_iisph(const Launch& L, const DevArrays& A, float* const scalar[4],
                          float* const vec[3], hipStream_t s);
-// reset the control block for a fresh solve
-void launch_ctl_reset(const Launch& L, const DevArrays& A, hipStream_t s);
 
 // volume maps (setup)
 void launch_volume_field_analytic(const SimParams& p, const BoundaryDev& b, float* volume,

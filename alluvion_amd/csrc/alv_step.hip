@@ -1404,9 +1404,10 @@ __global__ __launch_bounds__(kFinBlock) void k_div_finalize(SceneTab T, DevArray
   }
 }
 
-// Control block reset before the solves of a step.
-__global__ void k_ctl_reset(DevArrays A) {
-  StepCtl* ctl = A.ctl + blockIdx.x;
+// Control block reset before the solves of a step (k_clear_grid, the step's
+// first kernel; the neighbour phase reads only dt from the block).
+__device__ inline void reset_ctl(const DevArrays& A, unsigned scene) {
+  StepCtl* ctl = A.ctl + scene;
   ctl->div_iter = 0u;
   ctl->dens_iter = 0u;
   const float mx = FLT_MAX;
@@ -1417,7 +1418,7 @@ __global__ void k_ctl_reset(DevArrays A) {
   ctl->dens_done = 0u;  // decided after update_dt (tolerance does not use dt)
   bool dens_go = (mx > ctl->tol_dens || 0u < ctl->min_dens) && 0u < ctl->max_dens;
   ctl->dens_done = dens_go ? 0u : 1u;
-  publish_ctl(A, ctl, blockIdx.x);
+  publish_ctl(A, ctl, scene);
 }
 
 // ---------------------------------------------------------------------------
@@ -2717,17 +2718,17 @@ __global__ __launch_bounds__(256) void k_fill_u32(unsigned* __restrict__ d, unsi
   const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i < n) d[i] = v;
 }
-// zeroes the cell counts and the three grid anomaly flags (one launch)
-__global__ __launch_bounds__(256) void k_clear_grid(unsigned* counts, unsigned cells,
-                                                    unsigned* anomaly) {
+// zeroes the cell counts and the three grid anomaly flags and resets every
+// scene's control block (one launch)
+__global__ __launch_bounds__(256) void k_clear_grid(DevArrays A, unsigned cells, unsigned scenes) {
   const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < cells) counts[i] = 0u;
-  if (i < 3u) anomaly[i] = 0u;
+  if (i < cells) A.cell_count[i] = 0u;
+  if (i < 3u) A.anomaly[i] = 0u;
+  if (i < scenes) reset_ctl(A, i);
 }
 void launch_clear_grid(const Launch& L, const DevArrays& A, hipStream_t s) {
-  const unsigned nb = std::max(1u, (L.cells + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_clear_grid, dim3(nb), dim3(kBlock), 0, s, A.cell_count, L.cells,
-                     A.anomaly);
+  const unsigned nb = std::max(1u, (std::max(L.cells, L.scenes) + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_clear_grid, dim3(nb), dim3(kBlock), 0, s, A, L.cells, L.scenes);
 }
 void launch_fill_u32(unsigned* dst, unsigned value, size_t count, hipStream_t s) {
   if (!count) return;
@@ -2884,9 +2885,6 @@ void launch_export_iisph(const Launch& L, const DevArrays& A, float* const scala
                          float* const vec[3], hipStream_t s) {
   ALV_LAUNCH(k_export_iisph, L, s, A, scalar[0], scalar[1], scalar[2], scalar[3], vec[0],
              vec[1], vec[2]);
-}
-void launch_ctl_reset(const Launch& L, const DevArrays& A, hipStream_t s) {
-  hipLaunchKernelGGL(k_ctl_reset, dim3(L.scenes), dim3(1), 0, s, A);
 }
 void launch_volume_field_analytic(const SimParams& p, const BoundaryDev& b, float* volume,
                                   const float* peirce, unsigned begin, unsigned end,
