@@ -231,6 +231,34 @@ def test_dfsph_step_parity_dam_break(warm):
     assert abs(solver.mean_density_error - rep["mean_density_error"]) <= 1e-6
 
 
+def test_dfsph_without_divergence_solve_carries_the_factor():
+    """Without the divergence solve the reference does not compute the DFSPH
+    factor: its density solve rescales the factor each particle ended the
+    last step with (solver_df.hpp:81-91, runner.hpp:3525-3541).  The HIP
+    path carries it across the re-sort, so after such a step the factor
+    equals the oracle's particle by particle.  (The step's velocities are not
+    compared: with the factor rescaled twice the reference's solve diverges.)
+    After 40 steps the column falls fast enough that particles cross cell
+    faces every step, so the sorted order changes under the factor."""
+    sc = scenes.dam_break_small()
+    dp, pile, solver = gpu_scene(sc)
+    for _ in range(40):
+        solver.step()
+    orc = oracle_like(sc, dp, pile)
+    sync_state(dp, solver, orc)
+    solver.step()
+    orc.step()
+    solver.enable_divergence_solve = False
+    orc.update_settings(enable_divergence_solve=0)
+    solver.step()
+    orc.step()
+    n = sc.n
+    g = dp.coat(solver.particle_dfsph_factor).get()[:n]
+    o = orc.get("dfsph_factor")[:n]
+    assert np.all(np.isfinite(g)) and np.abs(o).max() > 0
+    assert rel_err(g, o) < REL, rel_err(g, o)
+
+
 @pytest.mark.parametrize("kind", ["ii", "i"])
 def test_iisph_and_isph_step_parity_dam_break(kind):
     sc = scenes.dam_break_small()
